@@ -1,0 +1,693 @@
+// C ABI of libcpmgpu.so: contexts, operators, solves.  See include/cpm_gpu.h.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <map>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "../../include/cpm_gpu.h"
+#include "engine.h"
+#include "krylov.h"
+#include "ops.h"
+
+namespace cpmgpu {
+
+static thread_local std::string g_err;
+
+void hip_check(hipError_t e, const char* what) {
+    if (e != hipSuccess) throw HipError(std::string(what) + ": " + hipGetErrorString(e));
+}
+
+Buffer::Buffer(size_t n) : bytes(n) {
+    if (n == 0) return;
+    hipError_t e = hipMalloc(&p, n);
+    if (e != hipSuccess) {
+        p = nullptr;
+        (void)hipGetLastError();
+        throw NoMem("hipMalloc of " + std::to_string(n) + " bytes failed");
+    }
+}
+Buffer::~Buffer() {
+    if (p) (void)hipFree(p);
+}
+
+// ---------------------------------------------------------------- packing
+// Interleave up to NR caller vectors into a workspace vector [row][rhs] and back.
+struct PackEntry {
+    const double* src;   // caller vector (pack) or nullptr
+    double* dst;         // caller vector (unpack) or nullptr
+    int64_t D;
+    int64_t off;
+    int lane, nr;
+    int block0;
+};
+
+__global__ void k_pack(const PackEntry* ent, const int* bent, double* ws64, float* ws32, int to_ws) {
+    const PackEntry& e = ent[bent[blockIdx.x]];
+    const int64_t row = (int64_t)(blockIdx.x - e.block0) * 256 + threadIdx.x;
+    if (row >= e.D) return;
+    const size_t i = (size_t)(e.off + row * e.nr + e.lane);
+    if (to_ws) {
+        const double v = e.src ? e.src[row] : 0.0;
+        if (ws64) ws64[i] = v;
+        if (ws32) ws32[i] = (float)v;
+    } else {
+        e.dst[row] = ws64 ? ws64[i] : (double)ws32[i];
+    }
+}
+
+Ctx::Ctx(int dev) : device(dev) {
+    HIPCHK(hipSetDevice(dev));
+    HIPCHK(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
+    HIPCHK(hipHostMalloc(&host_states, sizeof(KState) * kHostStates));
+}
+Ctx::~Ctx() {
+    (void)hipStreamSynchronize(stream);
+    ws.clear();
+    (void)hipHostFree(host_states);
+    (void)hipStreamDestroy(stream);
+}
+
+double* Ctx::workspace(const char* name, size_t bytes) {
+    auto& b = ws[name];
+    if (!b || b->bytes < bytes) {
+        b.reset();
+        b.reset(new Buffer(std::max(bytes, (size_t)256)));
+    }
+    return b->as<double>();
+}
+
+// ---------------------------------------------------------------- operator
+Op::Op(Ctx* c, int64_t D_, int nD_) : ctx(c), D(D_), nD(nD_) {}
+
+void Op::upload_ell(const std::vector<int>& ec, const std::vector<double>& ev, int ew_,
+                    const std::vector<int>& lc, const std::vector<double>& lv, int lw_,
+                    const std::vector<double>& ld, const std::vector<uint8_t>& fl) {
+    ew = ew_;
+    lw = lw_;
+    auto up = [&](Buffer& b, const void* src, size_t bytes) {
+        b.~Buffer();
+        new (&b) Buffer(bytes);
+        if (bytes) HIPCHK(hipMemcpy(b.p, src, bytes, hipMemcpyHostToDevice));
+    };
+    std::vector<float> ev32(ev.begin(), ev.end()), lv32(lv.begin(), lv.end()), ld32(ld.begin(), ld.end());
+    up(ecol, ec.data(), ec.size() * sizeof(int));
+    up(eval64, ev.data(), ev.size() * sizeof(double));
+    up(eval32, ev32.data(), ev32.size() * sizeof(float));
+    up(lcol, lc.data(), lc.size() * sizeof(int));
+    up(lval64, lv.data(), lv.size() * sizeof(double));
+    up(lval32, lv32.data(), lv32.size() * sizeof(float));
+    up(ldiag64, ld.data(), ld.size() * sizeof(double));
+    up(ldiag32, ld32.data(), ld32.size() * sizeof(float));
+    up(flag, fl.data(), fl.size());
+}
+
+template <>
+OpView<double> Op::view<double>() const {
+    return OpView<double>{D, nD, ew, lw, ecol.as<int>(), eval64.as<double>(), lcol.as<int>(),
+                          lval64.as<double>(), ldiag64.as<double>(), flag.as<uint8_t>()};
+}
+template <>
+OpView<float> Op::view<float>() const {
+    return OpView<float>{D, nD, ew, lw, ecol.as<int>(), eval32.as<float>(), lcol.as<int>(),
+                         lval32.as<float>(), ldiag32.as<float>(), flag.as<uint8_t>()};
+}
+
+size_t Op::device_bytes() const {
+    return ecol.bytes + eval64.bytes + eval32.bytes + lcol.bytes + lval64.bytes + lval32.bytes + ldiag64.bytes +
+           ldiag32.bytes + flag.bytes;
+}
+
+Op* op_from_csr(Ctx* ctx, int64_t D, int nD, const int32_t* Lp, const int32_t* Lc, const double* Lv,
+                const int32_t* Ep, const int32_t* Ec, const double* Ev, const uint8_t* ident, double m, double n,
+                int precond, int precond_iters) {
+    if (D <= 0 || nD < 0 || nD > D) throw ArgError("invalid D / nD");
+    if (D > (int64_t)INT32_MAX) throw ArgError("D exceeds int32 column range");
+    std::vector<uint8_t> fl((size_t)D, 0);
+    for (int64_t r = 0; r < D; ++r) fl[(size_t)r] = ident[r] ? ROW_IDENTITY : 0;
+    // E -> ELL (identity rows are served by the flag)
+    int ew = 0;
+    for (int64_t r = 0; r < D; ++r)
+        if (!(fl[(size_t)r] & ROW_IDENTITY)) ew = std::max(ew, Ep[r + 1] - Ep[r]);
+    std::vector<int> ec((size_t)ew * D);
+    std::vector<double> ev((size_t)ew * D, 0.0);
+    for (int64_t r = 0; r < D; ++r) {
+        int s = 0;
+        if (!(fl[(size_t)r] & ROW_IDENTITY))
+            for (int32_t k = Ep[r]; k < Ep[r + 1]; ++k, ++s) {
+                if (Ec[k] < 0 || Ec[k] >= D) throw ArgError("E column out of range");
+                ec[(size_t)s * D + r] = Ec[k];
+                ev[(size_t)s * D + r] = Ev[k];
+            }
+        for (; s < ew; ++s) ec[(size_t)s * D + r] = (int)r;   // padding reads the row's own entry with weight 0
+    }
+    // L -> diagonal + off-diagonal ELL
+    int lw = 0;
+    for (int64_t r = 0; r < D; ++r) {
+        int c = 0;
+        for (int32_t k = Lp[r]; k < Lp[r + 1]; ++k) c += (Lc[k] != r);
+        lw = std::max(lw, c);
+    }
+    std::vector<int> lc((size_t)lw * D);
+    std::vector<double> lv((size_t)lw * D, 0.0), ld((size_t)D, 0.0);
+    for (int64_t r = 0; r < D; ++r) {
+        int s = 0;
+        bool has_diag = false;
+        for (int32_t k = Lp[r]; k < Lp[r + 1]; ++k) {
+            if (Lc[k] < 0 || Lc[k] >= D) throw ArgError("L column out of range");
+            if (Lc[k] == r) {
+                if (!has_diag) ld[(size_t)r] = Lv[k];   // first diagonal entry (GeneralizedBiCGStabSystem.h:321-326)
+                has_diag = true;
+                continue;
+            }
+            lc[(size_t)s * D + r] = Lc[k];
+            lv[(size_t)s * D + r] = Lv[k];
+            ++s;
+        }
+        for (; s < lw; ++s) lc[(size_t)s * D + r] = (int)r;
+        if (!has_diag) fl[(size_t)r] |= ROW_NODIAG;
+    }
+    std::unique_ptr<Op> op(new Op(ctx, D, nD));
+    op->upload_ell(ec, ev, ew, lc, lv, lw, ld, fl);
+    op->m = m;
+    op->n = n;
+    op->precond = precond;
+    op->precond_iters = precond_iters;
+    return op.release();
+}
+
+// ---------------------------------------------------------------- batch solve
+struct Group {
+    Op* op;
+    std::vector<int> inst;   // instance indices (<= NR)
+};
+
+template <typename T>
+static void fill_seg(Seg<T>& s, const Op* op, int64_t off, int block0, int nblocks, int nr, KState* st, double* part,
+                     unsigned* cnt) {
+    s.op = op->view<T>();
+    s.m = op->m;
+    s.n = op->n;
+    s.off = off;
+    s.block0 = block0;
+    s.nblocks = nblocks;
+    s.nr = nr;
+    s.precond = op->precond;
+    s.st = st;
+    s.part = part;
+    s.cnt = cnt;
+}
+
+template <int NR>
+static void run_batch(Ctx* ctx, const std::vector<Group>& groups, const std::vector<const double*>& rhs,
+                      const std::vector<double*>& xs, const cpm_solve_opts& o, std::vector<cpm_solve_info>& infos) {
+    const hipStream_t st = ctx->stream;
+    const int G = (int)groups.size();
+    const bool refine = o.precision == CPM_FP32_REFINE;
+    // layout
+    std::vector<int64_t> off(G);
+    std::vector<int> b0(G), nb(G);
+    int64_t total = 0;
+    int blocks = 0;
+    for (int g = 0; g < G; ++g) {
+        off[g] = total;
+        total += groups[g].op->D * NR;
+        total = (total + 63) & ~int64_t(63);
+        b0[g] = blocks;
+        nb[g] = (int)((groups[g].op->D + kBlock - 1) / kBlock);
+        blocks += nb[g];
+    }
+    // device tables: segments (x2 precisions), block map, states, partials, counters
+    const int nstate = G * kMaxNR;
+    const size_t part_stride = 2 * NR;
+    auto r256 = [](size_t n) { return (n + 255) & ~size_t(255); };
+    const size_t bytes_tables = r256(sizeof(Seg<double>) * G) + r256(sizeof(Seg<float>) * G) + r256(sizeof(int) * blocks) +
+                                2 * r256(sizeof(KState) * nstate) + 2 * r256(sizeof(double) * part_stride * blocks) +
+                                r256(sizeof(unsigned) * G * 2) + 256;
+    char* tb = reinterpret_cast<char*>(ctx->workspace("tables", bytes_tables));
+    char* const tb_end = tb + bytes_tables;
+    auto carve = [&](size_t n) {
+        char* p = tb;
+        tb += r256(n);
+        if (tb > tb_end) throw std::logic_error("table workspace overflow");
+        return p;
+    };
+    auto* segs64 = reinterpret_cast<Seg<double>*>(carve(sizeof(Seg<double>) * G));
+    auto* segs32 = reinterpret_cast<Seg<float>*>(carve(sizeof(Seg<float>) * G));
+    int* bseg = reinterpret_cast<int*>(carve(sizeof(int) * blocks));
+    KState* st64 = reinterpret_cast<KState*>(carve(sizeof(KState) * nstate));
+    KState* st32 = reinterpret_cast<KState*>(carve(sizeof(KState) * nstate));
+    double* part64 = reinterpret_cast<double*>(carve(sizeof(double) * part_stride * blocks));
+    double* part32 = reinterpret_cast<double*>(carve(sizeof(double) * part_stride * blocks));
+    unsigned* cnt = reinterpret_cast<unsigned*>(carve(sizeof(unsigned) * G * 2));
+
+    std::vector<Seg<double>> h64(G);
+    std::vector<Seg<float>> h32(G);
+    std::vector<int> hb(blocks);
+    for (int g = 0; g < G; ++g) {
+        const int nr = (int)groups[g].inst.size();
+        fill_seg(h64[g], groups[g].op, off[g], b0[g], nb[g], nr, st64 + g * kMaxNR, part64 + part_stride * b0[g],
+                 cnt + g);
+        fill_seg(h32[g], groups[g].op, off[g], b0[g], nb[g], nr, st32 + g * kMaxNR, part32 + part_stride * b0[g],
+                 cnt + G + g);
+        for (int b = 0; b < nb[g]; ++b) hb[b0[g] + b] = g;
+    }
+    HIPCHK(hipMemcpyAsync(segs64, h64.data(), sizeof(Seg<double>) * G, hipMemcpyHostToDevice, st));
+    HIPCHK(hipMemcpyAsync(segs32, h32.data(), sizeof(Seg<float>) * G, hipMemcpyHostToDevice, st));
+    HIPCHK(hipMemcpyAsync(bseg, hb.data(), sizeof(int) * blocks, hipMemcpyHostToDevice, st));
+    HIPCHK(hipMemsetAsync(cnt, 0, sizeof(unsigned) * G * 2, st));
+
+    // workspace vectors
+    const size_t n64 = refine ? 14 : 11, n32 = refine ? 11 : 0;
+    double* w64 = ctx->workspace("vec64", sizeof(double) * total * n64 + 1024);
+    float* w32 = refine ? reinterpret_cast<float*>(ctx->workspace("vec32", sizeof(float) * total * n32 + 1024)) : nullptr;
+    Vecs<double> V64;
+    {
+        double* q = w64;
+        double** f[] = {&V64.b, &V64.x, &V64.r, &V64.r0, &V64.p, &V64.v, &V64.s, &V64.t, &V64.y, &V64.z, &V64.tmp};
+        for (auto* pp : f) { *pp = q; q += total; }
+    }
+    Vecs<float> V32{};
+    if (refine) {
+        float* q = w32;
+        float** f[] = {&V32.b, &V32.x, &V32.r, &V32.r0, &V32.p, &V32.v, &V32.s, &V32.t, &V32.y, &V32.z, &V32.tmp};
+        for (auto* pp : f) { *pp = q; q += total; }
+    }
+
+    // pack right-hand sides
+    std::vector<PackEntry> pe;
+    std::vector<int> pb;
+    for (int g = 0; g < G; ++g)
+        for (int k = 0; k < NR; ++k) {
+            PackEntry e{};
+            e.src = k < (int)groups[g].inst.size() ? rhs[groups[g].inst[k]] : nullptr;
+            e.dst = k < (int)groups[g].inst.size() ? xs[groups[g].inst[k]] : nullptr;
+            e.D = groups[g].op->D;
+            e.off = off[g];
+            e.lane = k;
+            e.nr = NR;
+            e.block0 = (int)pb.size();
+            for (int b = 0; b < nb[g]; ++b) pb.push_back((int)pe.size());
+            pe.push_back(e);
+        }
+    PackEntry* dpe = reinterpret_cast<PackEntry*>(ctx->workspace("pack", sizeof(PackEntry) * pe.size() + sizeof(int) * pb.size() + 512));
+    int* dpb = reinterpret_cast<int*>(reinterpret_cast<char*>(dpe) + ((sizeof(PackEntry) * pe.size() + 255) & ~size_t(255)));
+    HIPCHK(hipMemcpyAsync(dpe, pe.data(), sizeof(PackEntry) * pe.size(), hipMemcpyHostToDevice, st));
+    HIPCHK(hipMemcpyAsync(dpb, pb.data(), sizeof(int) * pb.size(), hipMemcpyHostToDevice, st));
+    hipLaunchKernelGGL(k_pack, dim3((unsigned)pb.size()), dim3(256), 0, st, dpe, dpb, V64.b, (float*)nullptr, 1);
+
+    LaunchCfg<double> c64{segs64, bseg, blocks, st};
+    LaunchCfg<float> c32{segs32, bseg, blocks, st};
+    KState* hs = ctx->host_states;
+    if (nstate > Ctx::kHostStates) throw ArgError("batch too large");
+
+    // Krylov driver for one precision: runs until no instance is RUN.
+    auto krylov = [&](auto tag, KState* dst, auto& cfg, auto& V) {
+        using T = decltype(tag);
+        Launch<T, NR>::init(cfg, V.b, V.x, V.r, V.r0, V.p, V.v);
+        int chunk = 4;
+        for (;;) {
+            for (int c = 0; c < chunk; ++c) Launch<T, NR>::iteration(cfg, V);
+            HIPCHK(hipGetLastError());
+            HIPCHK(hipMemcpyAsync(hs, dst, sizeof(KState) * nstate, hipMemcpyDeviceToHost, st));
+            HIPCHK(hipStreamSynchronize(st));
+            bool run = false, rst = false;
+            for (int g = 0; g < G; ++g)
+                for (size_t k = 0; k < groups[g].inst.size(); ++k) {
+                    const int s = hs[g * kMaxNR + k].status;
+                    run |= s == ST_RUN;
+                    rst |= s == ST_RESTART;
+                }
+            if (rst) {
+                Launch<T, NR>::restart(cfg, V);
+                continue;
+            }
+            if (!run) break;
+            chunk = std::min(chunk * 2, 64);
+        }
+    };
+
+    if (!refine) {
+        launch_reset_states(st64, nstate, o.tol * o.tol, o.max_iters, st);
+        krylov(double(0), st64, c64, V64);
+        hipLaunchKernelGGL(k_pack, dim3((unsigned)pb.size()), dim3(256), 0, st, dpe, dpb, V64.x, (float*)nullptr, 0);
+        HIPCHK(hipMemcpyAsync(hs, st64, sizeof(KState) * nstate, hipMemcpyDeviceToHost, st));
+        HIPCHK(hipStreamSynchronize(st));
+        for (int g = 0; g < G; ++g)
+            for (size_t k = 0; k < groups[g].inst.size(); ++k) {
+                const KState& s = hs[g * kMaxNR + k];
+                cpm_solve_info& inf = infos[groups[g].inst[k]];
+                inf.status = s.status;
+                inf.iters = s.iters;
+                inf.restarts = s.restarts;
+                inf.refinements = 0;
+                inf.rel_residual = s.bsq > 0 ? std::sqrt(s.rr / s.bsq) : 0.0;
+            }
+        return;
+    }
+
+    // FP32 BiCGStab inside FP64 iterative refinement.  Outer state lives in
+    // st64 (max_it = refinement cap), inner state in st32.
+    double* x64 = V64.x;     // solution
+    double* r64 = V64.r;     // refinement residual
+    double* ax = V64.t;      // A64 x64
+    double* tmp = V64.tmp;
+    const double inner_tol = o.inner_tol > 0 ? o.inner_tol : 1e-4;
+    const int max_ref = o.max_refinements > 0 ? o.max_refinements : 40;
+    launch_reset_states(st64, nstate, o.tol * o.tol, max_ref, st);
+    RefLaunch<NR>::start(c64, V64.b, x64, r64);
+    for (;;) {
+        HIPCHK(hipMemcpyAsync(hs, st64, sizeof(KState) * nstate, hipMemcpyDeviceToHost, st));
+        HIPCHK(hipStreamSynchronize(st));
+        bool run = false;
+        for (int g = 0; g < G; ++g)
+            for (size_t k = 0; k < groups[g].inst.size(); ++k) run |= hs[g * kMaxNR + k].outer_status == ST_RUN;
+        if (!run) break;
+        RefLaunch<NR>::down(c64, r64, V32.b);
+        launch_reset_states(st32, nstate, inner_tol * inner_tol, o.max_iters, st);
+        krylov(float(0), st32, c32, V32);
+        RefLaunch<NR>::up(c64, V32.x, x64);
+        Launch<double, NR>::apply(c64, x64, tmp, ax);
+        RefLaunch<NR>::resid(c64, V64.b, ax, r64, st32);
+    }
+    hipLaunchKernelGGL(k_pack, dim3((unsigned)pb.size()), dim3(256), 0, st, dpe, dpb, x64, (float*)nullptr, 0);
+    HIPCHK(hipStreamSynchronize(st));
+    for (int g = 0; g < G; ++g)
+        for (size_t k = 0; k < groups[g].inst.size(); ++k) {
+            const KState& s = hs[g * kMaxNR + k];
+            cpm_solve_info& inf = infos[groups[g].inst[k]];
+            inf.status = s.outer_status;
+            inf.iters = s.total_iters;
+            inf.restarts = s.restarts;
+            inf.refinements = s.refinements;
+            inf.rel_residual = s.bsq > 0 ? std::sqrt(s.rr / s.bsq) : 0.0;
+        }
+}
+
+void solve_batch(Ctx* ctx, const std::vector<Op*>& ops, const std::vector<const double*>& rhs,
+                 const std::vector<double*>& xs, const cpm_solve_opts& o, std::vector<cpm_solve_info>& infos) {
+    const int n = (int)ops.size();
+    infos.assign(n, cpm_solve_info{});
+    if (n == 0) return;
+    if (o.precision != CPM_FP64 && o.precision != CPM_FP32_REFINE) throw ArgError("unknown precision");
+    for (Op* op : ops) {
+        if (!op || op->ctx != ctx) throw ArgError("operator belongs to another context");
+        if (op->precond == CPM_PRECOND_JACOBI) throw ArgError("Jacobi preconditioner is not supported by the GPU solver");
+    }
+    // group instances that share an operator (at most kMaxNR per group)
+    std::vector<Group> groups;
+    std::map<Op*, int> open;
+    for (int i = 0; i < n; ++i) {
+        auto it = open.find(ops[i]);
+        if (it == open.end() || groups[it->second].inst.size() == (size_t)kMaxNR) {
+            groups.push_back(Group{ops[i], {}});
+            open[ops[i]] = (int)groups.size() - 1;
+            it = open.find(ops[i]);
+        }
+        groups[it->second].inst.push_back(i);
+    }
+    size_t nr = 1;
+    for (auto& g : groups) nr = std::max(nr, g.inst.size());
+    if (nr == 1) run_batch<1>(ctx, groups, rhs, xs, o, infos);
+    else if (nr == 2) run_batch<2>(ctx, groups, rhs, xs, o, infos);
+    else if (nr <= 4) run_batch<4>(ctx, groups, rhs, xs, o, infos);
+    else run_batch<8>(ctx, groups, rhs, xs, o, infos);
+}
+
+void multiply(Op* op, const double* dx, double* dy, double m, double n) {
+    Ctx* ctx = op->ctx;
+    const hipStream_t st = ctx->stream;
+    char* tb = reinterpret_cast<char*>(ctx->workspace("mul_tables", 4096 + sizeof(int) * ((op->D + 255) / 256)));
+    Seg<double> h{};
+    const int nblocks = (int)((op->D + kBlock - 1) / kBlock);
+    fill_seg(h, op, 0, 0, nblocks, 1, nullptr, nullptr, nullptr);
+    h.m = m;
+    h.n = n;
+    std::vector<int> hb(nblocks, 0);
+    auto* dseg = reinterpret_cast<Seg<double>*>(tb);
+    int* dbseg = reinterpret_cast<int*>(tb + 4096);
+    HIPCHK(hipMemcpyAsync(dseg, &h, sizeof(h), hipMemcpyHostToDevice, st));
+    HIPCHK(hipMemcpyAsync(dbseg, hb.data(), sizeof(int) * nblocks, hipMemcpyHostToDevice, st));
+    double* tmp = ctx->workspace("mul_tmp", sizeof(double) * op->D);
+    LaunchCfg<double> c{dseg, dbseg, nblocks, st};
+    Launch<double, 1>::apply(c, dx, tmp, dy);
+    HIPCHK(hipGetLastError());
+}
+
+}  // namespace cpmgpu
+
+// ====================================================================== C ABI
+using namespace cpmgpu;
+
+// The opaque C handles are the engine objects themselves.
+static inline Ctx* C(cpm_ctx* c) { return reinterpret_cast<Ctx*>(c); }
+static inline const Ctx* C(const cpm_ctx* c) { return reinterpret_cast<const Ctx*>(c); }
+static inline Op* O(cpm_op* o) { return reinterpret_cast<Op*>(o); }
+static inline const Op* O(const cpm_op* o) { return reinterpret_cast<const Op*>(o); }
+
+template <typename F>
+static int guard(F&& f) {
+    try {
+        f();
+        return CPM_OK;
+    } catch (const ArgError& e) {
+        g_err = e.what();
+        return CPM_ERR_ARG;
+    } catch (const NoMem& e) {
+        g_err = e.what();
+        return CPM_ERR_NOMEM;
+    } catch (const GeomError& e) {
+        g_err = e.what();
+        return CPM_ERR_GEOMETRY;
+    } catch (const HipError& e) {
+        g_err = e.what();
+        return CPM_ERR_HIP;
+    } catch (const std::exception& e) {
+        g_err = e.what();
+        return CPM_ERR_STATE;
+    }
+}
+
+extern "C" {
+
+void cpm_solve_opts_default(cpm_solve_opts* o) {
+    if (!o) return;
+    o->tol = 1e-10;
+    o->max_iters = 100000;
+    o->precision = CPM_FP64;
+    o->inner_tol = 1e-4;
+    o->max_refinements = 40;
+}
+
+const char* cpm_last_error(void) { return g_err.c_str(); }
+
+int cpm_ctx_create(int device, cpm_ctx** out) {
+    return guard([&] {
+        if (!out) throw ArgError("out is null");
+        int n = 0;
+        HIPCHK(hipGetDeviceCount(&n));
+        if (device < 0 || device >= n) throw ArgError("no such device");
+        *out = reinterpret_cast<cpm_ctx*>(new Ctx(device));
+    });
+}
+
+int cpm_ctx_destroy(cpm_ctx* ctx) {
+    return guard([&] { delete C(ctx); });
+}
+
+int cpm_ctx_synchronize(cpm_ctx* ctx) {
+    return guard([&] {
+        if (!ctx) throw ArgError("null context");
+        HIPCHK(hipStreamSynchronize(C(ctx)->stream));
+    });
+}
+
+int64_t cpm_ctx_device_bytes(const cpm_ctx* ctx) {
+    if (!ctx) return -1;
+    int64_t b = 0;
+    for (auto& kv : C(ctx)->ws) b += kv.second ? (int64_t)kv.second->bytes : 0;
+    return b + C(ctx)->op_bytes;
+}
+
+int cpm_op_create_csr(cpm_ctx* ctx, int64_t D, int nD, const int32_t* Lp, const int32_t* Lc, const double* Lv,
+                      const int32_t* Ep, const int32_t* Ec, const double* Ev, const uint8_t* ident, double m,
+                      double n, int precond, int precond_iters, cpm_op** out) {
+    return guard([&] {
+        if (!ctx || !out || !Lp || !Lc || !Lv || !Ep || !Ec || !Ev || !ident) throw ArgError("null argument");
+        if (precond < 0 || precond > 2) throw ArgError("unknown preconditioner");
+        HIPCHK(hipSetDevice(C(ctx)->device));
+        Op* op = op_from_csr(C(ctx), D, nD, Lp, Lc, Lv, Ep, Ec, Ev, ident, m, n, precond, precond_iters);
+        C(ctx)->op_bytes += (int64_t)op->device_bytes();
+        *out = reinterpret_cast<cpm_op*>(op);
+    });
+}
+
+int cpm_op_destroy(cpm_op* op) {
+    return guard([&] {
+        if (!op) return;
+        O(op)->ctx->op_bytes -= (int64_t)O(op)->device_bytes();
+        delete O(op);
+    });
+}
+
+int64_t cpm_op_rows(const cpm_op* op) { return op ? O(op)->D : -1; }
+
+int cpm_op_set_mn(cpm_op* op, double m, double n) {
+    return guard([&] {
+        if (!op) throw ArgError("null operator");
+        O(op)->m = m;
+        O(op)->n = n;
+    });
+}
+
+int cpm_op_set_precond(cpm_op* op, int precond, int precond_iters) {
+    return guard([&] {
+        if (!op) throw ArgError("null operator");
+        if (precond < 0 || precond > 2) throw ArgError("unknown preconditioner");
+        O(op)->precond = precond;
+        O(op)->precond_iters = precond_iters;
+    });
+}
+
+int cpm_multiply_device(cpm_op* op, const double* dx, double* dy, double m, double n) {
+    return guard([&] {
+        if (!op || !dx || !dy) throw ArgError("null argument");
+        multiply(O(op), dx, dy, m, n);
+    });
+}
+
+int cpm_multiply(cpm_op* op, const double* x, double* y, double m, double n) {
+    return guard([&] {
+        if (!op || !x || !y) throw ArgError("null argument");
+        Op* o = O(op);
+        Ctx* ctx = o->ctx;
+        double* d = ctx->workspace("mul_io", sizeof(double) * o->D * 2);
+        HIPCHK(hipMemcpyAsync(d, x, sizeof(double) * o->D, hipMemcpyHostToDevice, ctx->stream));
+        multiply(o, d, d + o->D, m, n);
+        HIPCHK(hipMemcpyAsync(y, d + o->D, sizeof(double) * o->D, hipMemcpyDeviceToHost, ctx->stream));
+        HIPCHK(hipStreamSynchronize(ctx->stream));
+    });
+}
+
+static cpm_solve_opts resolve_opts(const cpm_solve_opts* o) {
+    cpm_solve_opts r;
+    cpm_solve_opts_default(&r);
+    if (o) r = *o;
+    if (!(r.tol >= 0)) throw ArgError("tol must be >= 0");
+    return r;
+}
+
+int cpm_solve_batch_device(cpm_ctx* ctx, int n_inst, cpm_op* const* ops, const double* const* d_rhs,
+                           double* const* d_x, const cpm_solve_opts* opts, cpm_solve_info* infos) {
+    return guard([&] {
+        if (!ctx || n_inst < 0 || (n_inst > 0 && (!ops || !d_rhs || !d_x))) throw ArgError("null argument");
+        cpm_solve_opts o = resolve_opts(opts);
+        std::vector<Op*> vo(n_inst);
+        for (int i = 0; i < n_inst; ++i) vo[i] = O(ops[i]);
+        std::vector<const double*> vr(d_rhs, d_rhs + n_inst);
+        std::vector<double*> vx(d_x, d_x + n_inst);
+        std::vector<cpm_solve_info> inf;
+        HIPCHK(hipSetDevice(C(ctx)->device));
+        solve_batch(C(ctx), vo, vr, vx, o, inf);
+        if (infos) std::copy(inf.begin(), inf.end(), infos);
+    });
+}
+
+int cpm_solve_batch(cpm_ctx* ctx, int n_inst, cpm_op* const* ops, const double* const* rhs, double* const* x,
+                    const cpm_solve_opts* opts, cpm_solve_info* infos) {
+    return guard([&] {
+        if (!ctx || n_inst < 0 || (n_inst > 0 && (!ops || !rhs || !x))) throw ArgError("null argument");
+        cpm_solve_opts o = resolve_opts(opts);
+        Ctx* cx = C(ctx);
+        HIPCHK(hipSetDevice(cx->device));
+        std::vector<Op*> vo(n_inst);
+        int64_t tot = 0;
+        for (int i = 0; i < n_inst; ++i) {
+            vo[i] = O(ops[i]);
+            if (!vo[i]) throw ArgError("null operator");
+            tot += vo[i]->D;
+        }
+        double* d = cx->workspace("batch_io", sizeof(double) * tot * 2 + 256);
+        std::vector<const double*> vr(n_inst);
+        std::vector<double*> vx(n_inst);
+        int64_t a = 0;
+        for (int i = 0; i < n_inst; ++i) {
+            vr[i] = d + a;
+            vx[i] = d + tot + a;
+            HIPCHK(hipMemcpyAsync(d + a, rhs[i], sizeof(double) * vo[i]->D, hipMemcpyHostToDevice, cx->stream));
+            a += vo[i]->D;
+        }
+        std::vector<cpm_solve_info> inf;
+        solve_batch(cx, vo, vr, vx, o, inf);
+        for (int i = 0; i < n_inst; ++i)
+            HIPCHK(hipMemcpyAsync(x[i], vx[i], sizeof(double) * vo[i]->D, hipMemcpyDeviceToHost, cx->stream));
+        HIPCHK(hipStreamSynchronize(cx->stream));
+        if (infos) std::copy(inf.begin(), inf.end(), infos);
+    });
+}
+
+int cpm_solve_device(cpm_op* op, const double* d_rhs, double* d_x, const cpm_solve_opts* opts, cpm_solve_info* info) {
+    if (!op) {
+        g_err = "null operator";
+        return CPM_ERR_ARG;
+    }
+    cpm_op* ops[1] = {op};
+    const double* r[1] = {d_rhs};
+    double* x[1] = {d_x};
+    return cpm_solve_batch_device(reinterpret_cast<cpm_ctx*>(O(op)->ctx), 1, ops, r, x, opts, info);
+}
+
+int cpm_solve(cpm_op* op, const double* rhs, double* x, const cpm_solve_opts* opts, cpm_solve_info* info) {
+    if (!op) {
+        g_err = "null operator";
+        return CPM_ERR_ARG;
+    }
+    cpm_op* ops[1] = {op};
+    const double* r[1] = {rhs};
+    double* xx[1] = {x};
+    return cpm_solve_batch(reinterpret_cast<cpm_ctx*>(O(op)->ctx), 1, ops, r, xx, opts, info);
+}
+
+int cpm_device_alloc(cpm_ctx* ctx, int64_t bytes, void** d_ptr) {
+    return guard([&] {
+        if (!ctx || !d_ptr || bytes < 0) throw ArgError("bad argument");
+        HIPCHK(hipSetDevice(C(ctx)->device));
+        void* p = nullptr;
+        if (hipMalloc(&p, (size_t)std::max<int64_t>(bytes, 1)) != hipSuccess) {
+            (void)hipGetLastError();
+            throw NoMem("hipMalloc failed");
+        }
+        C(ctx)->user_bytes[p] = bytes;
+        *d_ptr = p;
+    });
+}
+
+int cpm_device_free(cpm_ctx* ctx, void* p) {
+    return guard([&] {
+        if (!ctx) throw ArgError("null context");
+        C(ctx)->user_bytes.erase(p);
+        HIPCHK(hipFree(p));
+    });
+}
+
+int cpm_memcpy_h2d(cpm_ctx* ctx, void* dst, const void* src, int64_t bytes) {
+    return guard([&] {
+        if (!ctx) throw ArgError("null context");
+        HIPCHK(hipMemcpyAsync(dst, src, (size_t)bytes, hipMemcpyHostToDevice, C(ctx)->stream));
+        HIPCHK(hipStreamSynchronize(C(ctx)->stream));
+    });
+}
+
+int cpm_memcpy_d2h(cpm_ctx* ctx, void* dst, const void* src, int64_t bytes) {
+    return guard([&] {
+        if (!ctx) throw ArgError("null context");
+        HIPCHK(hipMemcpyAsync(dst, src, (size_t)bytes, hipMemcpyDeviceToHost, C(ctx)->stream));
+        HIPCHK(hipStreamSynchronize(C(ctx)->stream));
+    });
+}
+
+}  // extern "C"

@@ -1,0 +1,54 @@
+// Host-side handles of libcpmgpu: context and operator.
+#pragma once
+#include <map>
+#include <memory>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "engine.h"
+
+namespace cpmgpu {
+
+struct ArgError : std::runtime_error { using std::runtime_error::runtime_error; };
+struct NoMem : std::runtime_error { using std::runtime_error::runtime_error; };
+struct GeomError : std::runtime_error { using std::runtime_error::runtime_error; };
+struct HipError : std::runtime_error { using std::runtime_error::runtime_error; };
+
+struct Ctx {
+    static constexpr int kHostStates = 8192;
+    int device = 0;
+    hipStream_t stream = nullptr;
+    KState* host_states = nullptr;                            // pinned mirror of device states
+    std::map<std::string, std::unique_ptr<Buffer>> ws;        // grow-only named workspaces
+    std::map<void*, int64_t> user_bytes;
+    int64_t op_bytes = 0;
+    explicit Ctx(int dev);
+    ~Ctx();
+    double* workspace(const char* name, size_t bytes);
+};
+
+// Operator in device memory: E and L(off-diagonal) as slot-major ELL with
+// int32 columns shared by the FP64 and FP32 value arrays.
+struct Op {
+    Ctx* ctx = nullptr;
+    int64_t D = 0;
+    int nD = 0;
+    int ew = 0, lw = 0;
+    double m = 0, n = 0;
+    int precond = 1, precond_iters = 0;
+    Buffer ecol, eval64, eval32, lcol, lval64, lval32, ldiag64, ldiag32, flag;
+    Op() = default;
+    Op(Ctx* c, int64_t D_, int nD_);
+    void upload_ell(const std::vector<int>& ec, const std::vector<double>& ev, int ew,
+                    const std::vector<int>& lc, const std::vector<double>& lv, int lw,
+                    const std::vector<double>& ld, const std::vector<uint8_t>& fl);
+    template <typename T> OpView<T> view() const;
+    size_t device_bytes() const;
+};
+
+Op* op_from_csr(Ctx* ctx, int64_t D, int nD, const int32_t* Lp, const int32_t* Lc, const double* Lv,
+                const int32_t* Ep, const int32_t* Ec, const double* Ev, const uint8_t* ident, double m, double n,
+                int precond, int precond_iters);
+
+}  // namespace cpmgpu

@@ -1,0 +1,221 @@
+"""ctypes binding of libcpmgpu.so (the MI355X closest-point engine's C ABI, include/cpm_gpu.h).
+
+This is the Python side of the drop-in boundary used by the tests and the
+benchmark.  It never falls back to a CPU path: if the HIP library is missing
+or no GPU is visible, every call raises.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.normpath(os.path.join(_HERE, "..", "..", "libcpmgpu.so"))
+
+CPM_FP64 = 0
+CPM_FP32_REFINE = 1
+PRECOND_IDENTITY, PRECOND_DIAGONAL, PRECOND_JACOBI = 0, 1, 2
+STATUS = {1: "converged", 2: "maxit", 4: "zero_rhs", 5: "breakdown"}
+
+EXPORTS = [
+    "cpm_solve_opts_default", "cpm_last_error", "cpm_ctx_create", "cpm_ctx_destroy", "cpm_ctx_synchronize",
+    "cpm_ctx_device_bytes", "cpm_op_create_csr", "cpm_op_destroy", "cpm_op_rows", "cpm_op_set_mn",
+    "cpm_op_set_precond", "cpm_multiply", "cpm_multiply_device", "cpm_solve", "cpm_solve_device",
+    "cpm_solve_batch", "cpm_solve_batch_device", "cpm_device_alloc", "cpm_device_free", "cpm_memcpy_h2d",
+    "cpm_memcpy_d2h",
+]
+
+
+class SolveOpts(ctypes.Structure):
+    _fields_ = [("tol", ctypes.c_double), ("max_iters", ctypes.c_int), ("precision", ctypes.c_int),
+                ("inner_tol", ctypes.c_double), ("max_refinements", ctypes.c_int)]
+
+
+class SolveInfo(ctypes.Structure):
+    _fields_ = [("status", ctypes.c_int), ("iters", ctypes.c_int), ("restarts", ctypes.c_int),
+                ("refinements", ctypes.c_int), ("rel_residual", ctypes.c_double)]
+
+    def as_dict(self):
+        return dict(status=STATUS.get(self.status, self.status), iters=self.iters, restarts=self.restarts,
+                    refinements=self.refinements, rel_residual=self.rel_residual)
+
+
+class CpmError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"cpm error {code}: {msg}")
+        self.code = code
+
+
+_lib = None
+vp = ctypes.c_void_p
+
+
+def load(path=LIB_PATH):
+    """Load libcpmgpu.so (raises if it is not built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(path):
+        raise RuntimeError(f"libcpmgpu.so not built at {path}; run __graft_entry__.build()")
+    L = ctypes.CDLL(path)
+    L.cpm_last_error.restype = ctypes.c_char_p
+    L.cpm_ctx_create.argtypes = [ctypes.c_int, ctypes.POINTER(vp)]
+    L.cpm_ctx_destroy.argtypes = [vp]
+    L.cpm_ctx_synchronize.argtypes = [vp]
+    L.cpm_ctx_device_bytes.argtypes = [vp]
+    L.cpm_ctx_device_bytes.restype = ctypes.c_int64
+    L.cpm_op_create_csr.argtypes = [vp, ctypes.c_int64, ctypes.c_int, vp, vp, vp, vp, vp, vp, vp,
+                                    ctypes.c_double, ctypes.c_double, ctypes.c_int, ctypes.c_int,
+                                    ctypes.POINTER(vp)]
+    L.cpm_op_destroy.argtypes = [vp]
+    L.cpm_op_rows.argtypes = [vp]
+    L.cpm_op_rows.restype = ctypes.c_int64
+    L.cpm_op_set_mn.argtypes = [vp, ctypes.c_double, ctypes.c_double]
+    L.cpm_op_set_precond.argtypes = [vp, ctypes.c_int, ctypes.c_int]
+    L.cpm_multiply.argtypes = [vp, vp, vp, ctypes.c_double, ctypes.c_double]
+    L.cpm_multiply_device.argtypes = [vp, vp, vp, ctypes.c_double, ctypes.c_double]
+    L.cpm_solve.argtypes = [vp, vp, vp, ctypes.POINTER(SolveOpts), ctypes.POINTER(SolveInfo)]
+    L.cpm_solve_device.argtypes = [vp, vp, vp, ctypes.POINTER(SolveOpts), ctypes.POINTER(SolveInfo)]
+    L.cpm_solve_batch.argtypes = [vp, ctypes.c_int, vp, vp, vp, ctypes.POINTER(SolveOpts), ctypes.POINTER(SolveInfo)]
+    L.cpm_solve_batch_device.argtypes = [vp, ctypes.c_int, vp, vp, vp, ctypes.POINTER(SolveOpts),
+                                         ctypes.POINTER(SolveInfo)]
+    L.cpm_solve_opts_default.argtypes = [ctypes.POINTER(SolveOpts)]
+    L.cpm_device_alloc.argtypes = [vp, ctypes.c_int64, ctypes.POINTER(vp)]
+    L.cpm_device_free.argtypes = [vp, vp]
+    L.cpm_memcpy_h2d.argtypes = [vp, vp, vp, ctypes.c_int64]
+    L.cpm_memcpy_d2h.argtypes = [vp, vp, vp, ctypes.c_int64]
+    _lib = L
+    return L
+
+
+def _chk(rc):
+    if rc != 0:
+        raise CpmError(rc, load().cpm_last_error().decode())
+
+
+def _ptr(a):
+    return a.ctypes.data_as(vp)
+
+
+def opts(tol=1e-10, max_iters=100000, precision=CPM_FP64, inner_tol=1e-4, max_refinements=40):
+    o = SolveOpts()
+    load().cpm_solve_opts_default(ctypes.byref(o))
+    o.tol, o.max_iters, o.precision, o.inner_tol, o.max_refinements = tol, max_iters, precision, inner_tol, max_refinements
+    return o
+
+
+class Context:
+    def __init__(self, device=0):
+        self._h = vp()
+        _chk(load().cpm_ctx_create(device, ctypes.byref(self._h)))
+
+    def close(self):
+        if self._h:
+            load().cpm_ctx_destroy(self._h)
+            self._h = vp()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def synchronize(self):
+        _chk(load().cpm_ctx_synchronize(self._h))
+
+    def device_bytes(self):
+        return load().cpm_ctx_device_bytes(self._h)
+
+    def alloc(self, nbytes):
+        p = vp()
+        _chk(load().cpm_device_alloc(self._h, int(nbytes), ctypes.byref(p)))
+        return p
+
+    def free(self, p):
+        _chk(load().cpm_device_free(self._h, p))
+
+    def h2d(self, dptr, arr):
+        arr = np.ascontiguousarray(arr)
+        _chk(load().cpm_memcpy_h2d(self._h, dptr, _ptr(arr), arr.nbytes))
+
+    def d2h(self, arr, dptr):
+        _chk(load().cpm_memcpy_d2h(self._h, _ptr(arr), dptr, arr.nbytes))
+        return arr
+
+    def solve_batch(self, ops, rhss, o=None):
+        n = len(ops)
+        o = o or opts()
+        rhss = [np.ascontiguousarray(r, np.float64) for r in rhss]
+        xs = [np.zeros_like(r) for r in rhss]
+        OA = (vp * n)(*[op._h for op in ops])
+        RA = (vp * n)(*[r.ctypes.data for r in rhss])
+        XA = (vp * n)(*[x.ctypes.data for x in xs])
+        infos = (SolveInfo * n)()
+        _chk(load().cpm_solve_batch(self._h, n, OA, RA, XA, ctypes.byref(o), infos))
+        return xs, [infos[i].as_dict() for i in range(n)]
+
+    def solve_batch_device(self, ops, d_rhs, d_x, o=None):
+        n = len(ops)
+        o = o or opts()
+        OA = (vp * n)(*[op._h for op in ops])
+        RA = (vp * n)(*[p.value if isinstance(p, vp) else p for p in d_rhs])
+        XA = (vp * n)(*[p.value if isinstance(p, vp) else p for p in d_x])
+        infos = (SolveInfo * n)()
+        _chk(load().cpm_solve_batch_device(self._h, n, OA, RA, XA, ctypes.byref(o), infos))
+        return [infos[i].as_dict() for i in range(n)]
+
+
+class Operator:
+    """GPU operator A(m, n) = m I + n (Ldiag + Loff E) with identity rows (cpm::Solver::Init)."""
+
+    def __init__(self, ctx, L, E, ident, m, n, nD=0, precond=PRECOND_DIAGONAL, precond_iters=0):
+        self.ctx = ctx
+        D = len(L.ptr) - 1
+        self._keep = [np.ascontiguousarray(a) for a in (L.ptr, L.col, L.val, E.ptr, E.col, E.val)]
+        self._keep = [self._keep[0].astype(np.int32), self._keep[1].astype(np.int32), self._keep[2].astype(np.float64),
+                      self._keep[3].astype(np.int32), self._keep[4].astype(np.int32), self._keep[5].astype(np.float64)]
+        idn = np.ascontiguousarray(ident, np.uint8)
+        self._h = vp()
+        k = self._keep
+        _chk(load().cpm_op_create_csr(ctx._h, D, nD, _ptr(k[0]), _ptr(k[1]), _ptr(k[2]), _ptr(k[3]), _ptr(k[4]),
+                                      _ptr(k[5]), _ptr(idn), m, n, precond, precond_iters, ctypes.byref(self._h)))
+        self._keep = None
+        self.D = D
+
+    def close(self):
+        if self._h:
+            load().cpm_op_destroy(self._h)
+            self._h = vp()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_mn(self, m, n):
+        _chk(load().cpm_op_set_mn(self._h, m, n))
+
+    def set_precond(self, precond, iters=0):
+        _chk(load().cpm_op_set_precond(self._h, precond, iters))
+
+    def multiply(self, x, m, n):
+        x = np.ascontiguousarray(x, np.float64)
+        y = np.zeros_like(x)
+        _chk(load().cpm_multiply(self._h, _ptr(x), _ptr(y), m, n))
+        return y
+
+    def multiply_device(self, dx, dy, m, n):
+        _chk(load().cpm_multiply_device(self._h, dx, dy, m, n))
+
+    def solve(self, rhs, o=None):
+        rhs = np.ascontiguousarray(rhs, np.float64)
+        x = np.zeros_like(rhs)
+        info = SolveInfo()
+        _chk(load().cpm_solve(self._h, _ptr(rhs), _ptr(x), ctypes.byref(o or opts()), ctypes.byref(info)))
+        return x, info.as_dict()
+
+    def solve_device(self, d_rhs, d_x, o=None):
+        info = SolveInfo()
+        _chk(load().cpm_solve_device(self._h, d_rhs, d_x, ctypes.byref(o or opts()), ctypes.byref(info)))
+        return info.as_dict()

@@ -1,0 +1,140 @@
+/*
+ * cpm_gpu.h -- C ABI of the MI355X closest-point-method engine (libcpmgpu.so).
+ *
+ * The reference has no FFI: its boundary for the solve is the C++ class
+ * cpm::Solver (src/Solver.h:19-53) and, for the per-tube setup, the classes
+ * Tube (src/Tube.h:26-205), Interpolation (src/Interpolation.h:16-43) and
+ * FDMatrices (src/FDMatrices.h).  Each entry point below names the reference
+ * interface it replaces.  All functions are extern "C", take plain pointers
+ * and sizes, never throw, and return CPM_OK (0) or a negative status; the
+ * message of the last failure on the calling thread is cpm_last_error().
+ *
+ * Host buffers are owned by the caller and copied in/out (no retention).
+ * Device memory belongs to the handles.  One HIP stream per context; a
+ * context is used by one host thread at a time.
+ */
+#ifndef CPM_GPU_H
+#define CPM_GPU_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum cpm_status {
+    CPM_OK = 0,
+    CPM_ERR_ARG = -1,       /* invalid argument / inconsistent sizes        */
+    CPM_ERR_HIP = -2,       /* HIP runtime failure                           */
+    CPM_ERR_NOMEM = -3,     /* device allocation failed or budget exceeded   */
+    CPM_ERR_GEOMETRY = -4,  /* tube construction failed (stencil off tube)   */
+    CPM_ERR_STATE = -5      /* handle used in the wrong state                */
+};
+
+/* Preconditioner, src/Defines.h:13-18 */
+enum cpm_precond { CPM_PRECOND_IDENTITY = 0, CPM_PRECOND_DIAGONAL = 1, CPM_PRECOND_JACOBI = 2 };
+
+/* Precision of the Krylov iterations. */
+enum cpm_precision {
+    CPM_FP64 = 0,          /* reference arithmetic: FP64 BiCGStab               */
+    CPM_FP32_REFINE = 1    /* FP32 BiCGStab inside FP64 iterative refinement    */
+};
+
+/* Per-solve termination status (cpm_solve_info.status). */
+enum cpm_solve_status {
+    CPM_SOLVE_CONVERGED = 1,
+    CPM_SOLVE_MAXIT = 2,
+    CPM_SOLVE_ZERO_RHS = 4,
+    CPM_SOLVE_BREAKDOWN = 5   /* NaN/Inf in a recurrence scalar */
+};
+
+typedef struct cpm_ctx cpm_ctx;
+typedef struct cpm_op cpm_op;
+typedef struct cpm_tube cpm_tube;
+
+typedef struct cpm_solve_opts {
+    double tol;            /* relative residual ||b-Ax|| <= tol ||b||; reference 1e-10 (src/Solver.cpp:164) */
+    int max_iters;         /* reference 100000 (src/Solver.cpp:164)                                         */
+    int precision;         /* enum cpm_precision                                                            */
+    double inner_tol;      /* FP32_REFINE: relative tolerance of each inner FP32 solve (<=0: default 1e-4) */
+    int max_refinements;   /* FP32_REFINE: cap on outer refinement steps (<=0: default 40)                  */
+} cpm_solve_opts;
+
+typedef struct cpm_solve_info {
+    int status;            /* enum cpm_solve_status                              */
+    int iters;             /* BiCGStab iterations (summed over inner solves)     */
+    int restarts;          /* rho-breakdown restarts (src/BiCGStab.h:62-71)       */
+    int refinements;       /* outer FP64 refinement steps (FP32_REFINE)          */
+    double rel_residual;   /* ||b-Ax||/||b|| of the returned x (FP64 recurrence / refinement residual) */
+} cpm_solve_info;
+
+/* Defaults of the reference: tol 1e-10, max 100000, FP64. */
+void cpm_solve_opts_default(cpm_solve_opts* opts);
+
+const char* cpm_last_error(void);
+
+/* ---- context ---------------------------------------------------------- */
+int cpm_ctx_create(int device, cpm_ctx** out);
+int cpm_ctx_destroy(cpm_ctx* ctx);
+int cpm_ctx_synchronize(cpm_ctx* ctx);
+/* Bytes of device memory held by all handles of this context. */
+int64_t cpm_ctx_device_bytes(const cpm_ctx* ctx);
+
+/* ---- operator from host CSR -------------------------------------------
+ * Replaces cpm::Solver::Init(m, n, L, E, precond, identity_rows, nD)
+ * (src/Solver.h:22-23, src/Solver.cpp:8-68 -> GeneralizedBiCGStabSystem::
+ * SetComponents, src/GeneralizedBiCGStabSystem.h:37-53).  L and E are D x D
+ * row-major CSR with int32 indices -- exactly Eigen::SparseMatrix<double,
+ * RowMajor>'s outerIndexPtr()/innerIndexPtr()/valuePtr() -- and columns
+ * ascending inside each row.  identity_rows has D bytes (0/1).  The operator is
+ *   y = m x + n (Ldiag x + (L - Ldiag) E x),  identity rows y = x,
+ *   last nD rows y = (m - n) x
+ * (GeneralizedBiCGStabSystem::Multiply, src/GeneralizedBiCGStabSystem.h:191-211). */
+int cpm_op_create_csr(cpm_ctx* ctx, int64_t D, int nD,
+                      const int32_t* L_rowptr, const int32_t* L_col, const double* L_val,
+                      const int32_t* E_rowptr, const int32_t* E_col, const double* E_val,
+                      const uint8_t* identity_rows, double m, double n,
+                      int precond, int precond_iters, cpm_op** out);
+int cpm_op_destroy(cpm_op* op);
+int64_t cpm_op_rows(const cpm_op* op);
+/* Time stepping re-uses the operator with new (m, n) (problems/HeatWithIBC.cpp:166-167). */
+int cpm_op_set_mn(cpm_op* op, double m, double n);
+/* Replaces cpm::Solver::SetPrecond(precond_iters) (src/Solver.h:25). */
+int cpm_op_set_precond(cpm_op* op, int precond, int precond_iters);
+
+/* y = A(m, n) x with host buffers of D doubles.
+ * Replaces cpm::Solver::Multiply (src/Solver.cpp:176-192). */
+int cpm_multiply(cpm_op* op, const double* x, double* y, double m, double n);
+/* Same with device pointers (D doubles each), on the context stream. */
+int cpm_multiply_device(cpm_op* op, const double* d_x, double* d_y, double m, double n);
+
+/* Solve A x = rhs from x0 = 0 (host buffers).  Replaces cpm::Solver::Solve
+ * (src/Solver.cpp:69-114 -> BiCGStab::Solve, src/BiCGStab.h:30-108). */
+int cpm_solve(cpm_op* op, const double* rhs, double* x, const cpm_solve_opts* opts, cpm_solve_info* info);
+/* Same with device pointers; no host transfer of vectors. */
+int cpm_solve_device(cpm_op* op, const double* d_rhs, double* d_x, const cpm_solve_opts* opts,
+                     cpm_solve_info* info);
+
+/* Batched solve: n_inst independent systems, instance i solving
+ * ops[i] x_i = rhs_i, all advanced together (one launch per kernel covers every
+ * instance; converged instances are masked).  rhs/x are host pointer arrays,
+ * each of cpm_op_rows(ops[i]) doubles.  Instances that share an operator are
+ * grouped into multi-RHS sweeps that read the operator once per iteration.
+ * The reference solves these one after another (e.g.
+ * examples/DiffusionCurvesCodimZero/DiffusionCurvesCodimZero.cpp:49-52). */
+int cpm_solve_batch(cpm_ctx* ctx, int n_inst, cpm_op* const* ops, const double* const* rhs,
+                    double* const* x, const cpm_solve_opts* opts, cpm_solve_info* infos);
+int cpm_solve_batch_device(cpm_ctx* ctx, int n_inst, cpm_op* const* ops, const double* const* d_rhs,
+                           double* const* d_x, const cpm_solve_opts* opts, cpm_solve_info* infos);
+
+/* Device scratch for callers that keep vectors resident (bench, serving). */
+int cpm_device_alloc(cpm_ctx* ctx, int64_t bytes, void** d_ptr);
+int cpm_device_free(cpm_ctx* ctx, void* d_ptr);
+int cpm_memcpy_h2d(cpm_ctx* ctx, void* d_dst, const void* h_src, int64_t bytes);
+int cpm_memcpy_d2h(cpm_ctx* ctx, void* h_dst, const void* d_src, int64_t bytes);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* CPM_GPU_H */

@@ -1,0 +1,189 @@
+"""ctypes view of the CPU oracle (liborc.so).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and the
+CPU-baseline leg of bench.py as the checker.  The product library never
+loads it.
+"""
+import ctypes
+import os
+from dataclasses import dataclass
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB = None
+
+i32p = np.ctypeslib.ndpointer(np.int32, flags="C_CONTIGUOUS")
+i64p = np.ctypeslib.ndpointer(np.int64, flags="C_CONTIGUOUS")
+f64p = np.ctypeslib.ndpointer(np.float64, flags="C_CONTIGUOUS")
+u8p = np.ctypeslib.ndpointer(np.uint8, flags="C_CONTIGUOUS")
+
+
+def lib():
+    global _LIB
+    if _LIB is None:
+        path = os.path.join(_HERE, "liborc.so")
+        if not os.path.exists(path):
+            raise RuntimeError(f"oracle not built: {path} (run make -C oracle)")
+        L = ctypes.CDLL(path)
+        L.orc_last_error.restype = ctypes.c_char_p
+        L.orc_problem_poisson.restype = ctypes.c_void_p
+        L.orc_problem_poisson.argtypes = [ctypes.c_char_p, ctypes.c_int, ctypes.c_void_p, ctypes.c_int,
+                                          ctypes.c_double, ctypes.c_double, ctypes.c_int, ctypes.c_void_p,
+                                          ctypes.c_int, ctypes.c_double, ctypes.c_int, ctypes.c_void_p,
+                                          ctypes.c_int, ctypes.c_int]
+        L.orc_free.argtypes = [ctypes.c_void_p]
+        L.orc_info.argtypes = [ctypes.c_void_p, i64p, f64p]
+        L.orc_tube.argtypes = [ctypes.c_void_p, i64p, f64p, f64p, f64p, i64p, i64p, i64p]
+        L.orc_csr.argtypes = [ctypes.c_void_p, ctypes.c_int, i32p, i32p, f64p]
+        L.orc_ident.argtypes = [ctypes.c_void_p, u8p]
+        L.orc_build_explicit.argtypes = [ctypes.c_void_p, ctypes.c_double, ctypes.c_double, ctypes.c_int]
+        L.orc_build_plot_E.argtypes = [ctypes.c_void_p]
+        L.orc_multiply.argtypes = [ctypes.c_int64, ctypes.c_int, i32p, i32p, f64p, i32p, i32p, f64p, u8p,
+                                   ctypes.c_double, ctypes.c_double, f64p, f64p]
+        L.orc_solve.argtypes = [ctypes.c_int64, ctypes.c_int, i32p, i32p, f64p, i32p, i32p, f64p, u8p,
+                                ctypes.c_double, ctypes.c_double, ctypes.c_int, ctypes.c_int, f64p, f64p,
+                                ctypes.c_double, ctypes.c_int, ctypes.POINTER(ctypes.c_int),
+                                ctypes.POINTER(ctypes.c_double)]
+        L.orc_poisson_problem_error.restype = ctypes.c_double
+        L.orc_poisson_problem_error.argtypes = [ctypes.c_int, ctypes.c_double, ctypes.POINTER(ctypes.c_int)]
+        L.orc_weights_1d.argtypes = [f64p, ctypes.c_double, ctypes.c_int, f64p]
+        _LIB = L
+    return _LIB
+
+
+def _err():
+    return lib().orc_last_error().decode()
+
+
+@dataclass
+class CSR:
+    ptr: np.ndarray
+    col: np.ndarray
+    val: np.ndarray
+
+    @property
+    def rows(self):
+        return len(self.ptr) - 1
+
+    def to_dense(self, ncols=None):
+        n = ncols or self.rows
+        A = np.zeros((self.rows, n))
+        for r in range(self.rows):
+            s, e = self.ptr[r], self.ptr[r + 1]
+            A[r, self.col[s:e]] += self.val[s:e]
+        return A
+
+    def matvec(self, x):
+        rows = np.repeat(np.arange(self.rows), np.diff(self.ptr))
+        return np.bincount(rows, weights=self.val * x[self.col], minlength=self.rows)
+
+
+class Problem:
+    """Poisson problem (problems/Poisson.cpp) built by the oracle: tube, Lfull, Efull, identity rows."""
+
+    def __init__(self, surface="Sphere", dx=0.1, params=None, bbox=None, dim=3, verts=None,
+                 bdy_order=0, which_bdy=(), p=3):
+        L = lib()
+        use_def = params is None and verts is None
+        par = np.ascontiguousarray(params if params is not None else [0.0], dtype=np.float64)
+        bb = bbox if bbox is not None else (0.0, 0.0)
+        vv = np.ascontiguousarray(verts if verts is not None else np.zeros((1, dim)), dtype=np.float64)
+        wb = np.ascontiguousarray(list(which_bdy) or [0], dtype=np.int64)
+        self._h = L.orc_problem_poisson(surface.encode(), int(use_def), par.ctypes.data, len(par),
+                                        float(bb[0]), float(bb[1]), dim, vv.ctypes.data, len(vv), dx,
+                                        bdy_order, wb.ctypes.data, len(which_bdy), p)
+        if not self._h:
+            raise RuntimeError(_err())
+        self._info()
+
+    def _info(self):
+        ints = np.zeros(9, np.int64)
+        dbl = np.zeros(4)
+        lib().orc_info(self._h, ints, dbl)
+        (self.N, self.D, self.nnzL, self.nnzE, self.n, self.N1d, self.dim, self.p, self.nnz_extra) = map(int, ints)
+        self.xstart, self.r_t, self.rfac, self.dx = map(float, dbl)
+
+    def __del__(self):
+        if getattr(self, "_h", None):
+            lib().orc_free(self._h)
+            self._h = None
+
+    def tube(self):
+        N, d = self.N, self.dim
+        rec = np.zeros(N, np.int64)
+        x = np.zeros(N * d)
+        cp = np.zeros(N * d)
+        dist = np.zeros(N)
+        bdy = np.zeros(N, np.int64)
+        nbr = np.zeros(N * 6, np.int64)
+        bpt = np.zeros(N, np.int64)
+        lib().orc_tube(self._h, rec, x, cp, dist, bdy, nbr, bpt)
+        return dict(rec=rec, x=x.reshape(N, d), cp=cp.reshape(N, d), dist=dist, bdy=bdy,
+                    nbr=nbr.reshape(N, 6), bpt=bpt)
+
+    def _csr(self, which, rows, nnz):
+        ptr = np.zeros(rows + 1, np.int32)
+        col = np.zeros(max(nnz, 1), np.int32)
+        val = np.zeros(max(nnz, 1))
+        lib().orc_csr(self._h, which, ptr, col, val)
+        return CSR(ptr, col[:nnz], val[:nnz])
+
+    def L(self):
+        return self._csr(0, self.D, self.nnzL)
+
+    def E(self):
+        return self._csr(1, self.D, self.nnzE)
+
+    def ident(self):
+        out = np.zeros(self.D, np.uint8)
+        lib().orc_ident(self._h, out)
+        return out
+
+    def explicit_operator(self, m, n, nD=0):
+        nnz = lib().orc_build_explicit(self._h, m, n, nD)
+        if nnz < 0:
+            raise RuntimeError(_err())
+        self._info()
+        return self._csr(2, self.D, nnz)
+
+    def plot_E(self):
+        rows = lib().orc_build_plot_E(self._h)
+        if rows < 0:
+            raise RuntimeError(_err())
+        self._info()
+        return self._csr(2, rows, self.nnz_extra)
+
+
+def multiply(L, E, ident, x, m, n, nD=0):
+    y = np.zeros_like(x)
+    if lib().orc_multiply(len(x), nD, L.ptr, L.col, L.val, E.ptr, E.col, E.val,
+                          np.ascontiguousarray(ident, np.uint8), m, n, np.ascontiguousarray(x), y) != 0:
+        raise RuntimeError(_err())
+    return y
+
+
+def solve(L, E, ident, b, m, n, nD=0, precond=1, precond_iters=0, tol=1e-10, max_it=100000):
+    x = np.zeros_like(b)
+    it = ctypes.c_int(0)
+    rr = ctypes.c_double(0)
+    if lib().orc_solve(len(b), nD, L.ptr, L.col, L.val, E.ptr, E.col, E.val,
+                       np.ascontiguousarray(ident, np.uint8), m, n, precond, precond_iters,
+                       np.ascontiguousarray(b), x, tol, max_it, ctypes.byref(it), ctypes.byref(rr)) != 0:
+        raise RuntimeError(_err())
+    return x, it.value, rr.value
+
+
+def poisson_problem_error(problem, dx):
+    it = ctypes.c_int(0)
+    e = lib().orc_poisson_problem_error(problem, dx, ctypes.byref(it))
+    if e < 0:
+        raise RuntimeError(_err())
+    return e, it.value
+
+
+def weights_1d(xs, xq):
+    xs = np.ascontiguousarray(xs, np.float64)
+    w = np.zeros(len(xs))
+    lib().orc_weights_1d(xs, float(xq), len(xs), w)
+    return w

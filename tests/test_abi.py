@@ -1,0 +1,34 @@
+"""The C-ABI library loads and exports every symbol include/*.h declares (CPU only: no compute calls)."""
+import ctypes
+import glob
+import os
+import re
+
+import cpmgpu
+
+REPO = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def declared_symbols():
+    syms = set()
+    for h in glob.glob(os.path.join(REPO, "include", "*.h")):
+        txt = open(h).read()
+        txt = re.sub(r"/\*.*?\*/", "", txt, flags=re.S)
+        for m in re.finditer(r"\b(cpm_[a-z0-9_]+)\s*\(", txt):
+            syms.add(m.group(1))
+    return syms
+
+
+def test_library_exports_declared_symbols():
+    syms = declared_symbols()
+    assert len(syms) >= 20
+    lib = ctypes.CDLL(cpmgpu.LIB_PATH)
+    missing = [s for s in sorted(syms) if not hasattr(lib, s)]
+    assert not missing, missing
+    assert set(cpmgpu.EXPORTS) <= syms
+
+
+def test_default_opts_match_reference():
+    o = cpmgpu.opts()
+    # src/Solver.cpp:164: tolerance 1e-10, at most 100000 iterations
+    assert o.tol == 1e-10 and o.max_iters == 100000 and o.precision == cpmgpu.CPM_FP64

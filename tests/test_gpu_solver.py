@@ -1,0 +1,107 @@
+"""GPU parity: the HIP operator and BiCGStab against the CPU oracle, through the C ABI."""
+import numpy as np
+import pytest
+
+import cpmgpu
+import orc
+
+pytestmark = pytest.mark.gpu
+
+
+def _problem(surface="Sphere", dx=0.1, **kw):
+    P = orc.Problem(surface, dx, **kw)
+    return P, P.L(), P.E(), P.ident()
+
+
+@pytest.mark.parametrize("surface,dx,m,n", [("Sphere", 0.2, 1.0, -1.0), ("Sphere", 0.1, 0.0, 1.0),
+                                            ("Torus", 0.2, 1.0, -0.3)])
+def test_multiply_matches_oracle(gpu_ctx, surface, dx, m, n):
+    print(f"case multiply {surface} dx={dx} m={m} n={n}", flush=True)
+    P, L, E, I = _problem(surface, dx)
+    op = cpmgpu.Operator(gpu_ctx, L, E, I, m, n)
+    x = np.random.default_rng(3).standard_normal(P.D)
+    y_gpu = op.multiply(x, m, n)
+    y_ref = orc.multiply(L, E, I, x, m, n)
+    assert np.abs(y_gpu - y_ref).max() <= 1e-12 * np.abs(y_ref).max()
+
+
+def test_multiply_identity_rows(gpu_ctx):
+    print("case multiply hemisphere identity rows", flush=True)
+    P = orc.Problem("Hemisphere", 0.1, bdy_order=1, which_bdy=(1,))
+    L, E, I = P.L(), P.E(), P.ident()
+    assert I.sum() > 0
+    op = cpmgpu.Operator(gpu_ctx, L, E, I, 0.0, 1.0)
+    x = np.random.default_rng(4).standard_normal(P.D)
+    y = op.multiply(x, 0.0, 1.0)
+    assert np.array_equal(y[I == 1], x[I == 1])
+    assert np.abs(y - orc.multiply(L, E, I, x, 0.0, 1.0)).max() <= 1e-12 * np.abs(y).max()
+
+
+@pytest.mark.parametrize("surface,dx,m,n", [("Sphere", 0.1, 1.0, -1.0), ("Sphere", 0.2, 1.0, 1.0),
+                                            ("Torus", 0.2, 1.0, -1.0)])
+def test_fp64_solve_matches_oracle(gpu_ctx, surface, dx, m, n):
+    """FP64 mode: both sides solved to 1e-13 agree to 1e-10 relative; at the
+    reference stop (1e-10) the GPU residual obeys the same bound."""
+    print(f"case fp64 solve {surface} dx={dx}", flush=True)
+    P, L, E, I = _problem(surface, dx)
+    b = np.random.default_rng(5).standard_normal(P.D)
+    op = cpmgpu.Operator(gpu_ctx, L, E, I, m, n)
+    x_ref, it_ref, _ = orc.solve(L, E, I, b, m, n, tol=1e-13, max_it=20000)
+    x, info = op.solve(b, cpmgpu.opts(tol=1e-13, max_iters=20000))
+    assert info["status"] == "converged"
+    rel = np.linalg.norm(x - x_ref) / np.linalg.norm(x_ref)
+    assert rel <= 1e-10, rel
+    x10, info10 = op.solve(b)
+    A = P.explicit_operator(m, n)
+    assert np.linalg.norm(b - A.matvec(x10)) / np.linalg.norm(b) <= 2e-10
+    _, it10, _ = orc.solve(L, E, I, b, m, n, tol=1e-10)
+    assert abs(info10["iters"] - it10) <= max(3, 0.1 * it10)
+
+
+def test_fp32_refine_solve_accuracy(gpu_ctx):
+    print("case fp32+refine sphere dx=0.1", flush=True)
+    P, L, E, I = _problem("Sphere", 0.1)
+    b = np.random.default_rng(6).standard_normal(P.D)
+    op = cpmgpu.Operator(gpu_ctx, L, E, I, 1.0, -1.0)
+    x_ref, _, _ = orc.solve(L, E, I, b, 1.0, -1.0, tol=1e-14, max_it=20000)
+    x, info = op.solve(b, cpmgpu.opts(tol=1e-12, precision=cpmgpu.CPM_FP32_REFINE))
+    assert info["status"] == "converged" and info["refinements"] >= 1
+    rel = np.linalg.norm(x - x_ref) / np.linalg.norm(x_ref)
+    assert rel <= 1e-8, rel        # the FP32+refine accuracy bar vs the FP64 oracle
+
+
+def test_batch_equals_single(gpu_ctx):
+    """Multi-RHS + multi-operator batch: every instance equals its solo solve."""
+    print("case batch", flush=True)
+    P1, L1, E1, I1 = _problem("Sphere", 0.2)
+    P2, L2, E2, I2 = _problem("Torus", 0.25)
+    op1 = cpmgpu.Operator(gpu_ctx, L1, E1, I1, 1.0, -1.0)
+    op2 = cpmgpu.Operator(gpu_ctx, L2, E2, I2, 1.0, -0.5)
+    rng = np.random.default_rng(7)
+    ops = [op1, op2, op1, op1, op2]
+    rhs = [rng.standard_normal(o.D) for o in ops]
+    for prec in (cpmgpu.CPM_FP64, cpmgpu.CPM_FP32_REFINE):
+        o = cpmgpu.opts(tol=1e-11, precision=prec)
+        xs, infos = gpu_ctx.solve_batch(ops, rhs, o)
+        for op, b, x, inf in zip(ops, rhs, xs, infos):
+            xs1, inf1 = op.solve(b, o)
+            assert inf["status"] == "converged"
+            assert np.linalg.norm(x - xs1) <= 1e-9 * np.linalg.norm(xs1)
+
+
+def test_edge_cases(gpu_ctx):
+    print("case edge", flush=True)
+    P, L, E, I = _problem("Sphere", 0.2)
+    op = cpmgpu.Operator(gpu_ctx, L, E, I, 1.0, -1.0)
+    x, info = op.solve(np.zeros(P.D))
+    assert info["status"] == "zero_rhs" and not x.any()
+    b = np.random.default_rng(8).standard_normal(P.D)
+    x, info = op.solve(b, cpmgpu.opts(max_iters=1))
+    assert info["status"] == "maxit" and info["iters"] == 1
+    # one iteration from x0 = 0 matches the oracle's single iteration
+    x1, it1, _ = orc.solve(L, E, I, b, 1.0, -1.0, max_it=1)
+    assert np.allclose(x, x1, rtol=1e-12, atol=1e-12)
+    bn = b.copy()
+    bn[5] = np.nan
+    x, info = op.solve(bn)
+    assert info["status"] == "breakdown"

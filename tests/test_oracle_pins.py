@@ -1,0 +1,149 @@
+"""Pins of the CPU oracle against what fixes it independently of its own code:
+the reference's convergence-order known answers, closed forms, brute force and
+exact polynomial reproduction.  (CPU only.)"""
+import math
+
+import numpy as np
+import pytest
+
+import orc
+
+
+def test_tube_radius_and_grid_closed_forms():
+    # Tube.cpp:136 with m_order = 2: r_t = 1.0001 sqrt((dim-1)((p+1)/2)^2 + ((p+3)/2)^2) h
+    P = orc.Problem("Sphere", 0.05)
+    assert P.rfac == pytest.approx(1.0001 * math.sqrt(17.0), rel=0, abs=1e-15)
+    # Tube.cpp:148-157: bbox [-1,1], h=0.05 -> 40 spacings, n = ceil(log2(50)) = 6,
+    # N1d = 65, spacing_diff = ceil((64-40-10)/2) = 7, xstart = -1 - 12 h = -1.6
+    assert (P.n, P.N1d) == (6, 65)
+    assert P.xstart == pytest.approx(-1.6, abs=1e-15)
+    P2 = orc.Problem("Circle", 0.05, params=[1.0], bbox=(-1.0, 1.0), dim=2)
+    assert P2.rfac == pytest.approx(1.0001 * math.sqrt(13.0), abs=1e-15)
+
+
+@pytest.mark.parametrize("surface,dx", [("Sphere", 0.2), ("Sphere", 0.1), ("Torus", 0.25)])
+def test_tube_membership_brute_force(surface, dx):
+    """The BFS tube equals {grid nodes with |dist| <= r_t} (closed surfaces: one component)."""
+    P = orc.Problem(surface, dx)
+    T = P.tube()
+    N1d, x0 = P.N1d, P.xstart
+    g = x0 + dx * np.arange(N1d)
+    X, Y, Z = np.meshgrid(g, g, g, indexing="ij")
+    if surface == "Sphere":
+        dist = np.sqrt(X * X + Y * Y + Z * Z) - 1.0
+    else:
+        R, r = 2.0, 1.0
+        rho = np.sqrt(X * X + Y * Y)
+        dist = np.sqrt((rho - R) ** 2 + Z * Z) - r
+    inside = np.flatnonzero((np.abs(dist) <= P.r_t).ravel())
+    assert np.array_equal(np.sort(T["rec"]), inside)
+    # node 0 is the grid node nearest cp(xstart) (Tube.cpp:540-551)
+    # BFS order: graph distance from node 0 is non-decreasing along the order
+    nbr = T["nbr"]
+    lev = np.full(P.N, -1)
+    lev[0] = 0
+    frontier = [0]
+    while frontier:
+        nxt = []
+        for i in frontier:
+            for j in nbr[i]:
+                if j >= 0 and lev[j] < 0:
+                    lev[j] = lev[i] + 1
+                    nxt.append(j)
+        frontier = nxt
+    assert (np.diff(lev) >= 0).all()
+    # neighbour symmetry: +d of i is j  <=>  -d of j is i
+    for d in range(3):
+        i = np.flatnonzero(nbr[:, 2 * d] >= 0)
+        assert (nbr[nbr[i, 2 * d], 2 * d + 1] == i).all()
+
+
+def test_weights_textbook_lagrange():
+    xs = np.array([0.1, 0.2, 0.3, 0.4])
+    for xq in (0.1234, 0.25, 0.3999):
+        w = orc.weights_1d(xs, xq)
+        ref = [np.prod([(xq - xs[m]) / (xs[j] - xs[m]) for m in range(4) if m != j]) for j in range(4)]
+        assert np.allclose(w, ref, rtol=0, atol=1e-14)
+    w = orc.weights_1d(xs, 0.2)           # exact hit (Interpolation.cpp:169-184)
+    assert list(w) == [0.0, 1.0, 0.0, 0.0]
+
+
+@pytest.mark.parametrize("surface,dx", [("Sphere", 0.1), ("Torus", 0.2)])
+def test_E_reproduces_cubics_and_rows_sum_to_one(surface, dx):
+    P = orc.Problem(surface, dx)
+    T = P.tube()
+    E = P.E()
+    assert np.allclose(E.matvec(np.ones(P.N)), 1.0, atol=1e-13)
+    assert (np.diff(E.ptr) == 64).all()
+    x, cp = T["x"], T["cp"]
+    for q in (lambda v: v[:, 0] ** 3 - 2 * v[:, 1] * v[:, 2] ** 2 + 0.5 * v[:, 1],
+              lambda v: v[:, 0] * v[:, 1] * v[:, 2] + v[:, 2] ** 3):
+        assert np.allclose(E.matvec(q(x)), q(cp), atol=1e-12)
+
+
+def test_L_exact_on_quadratics():
+    P = orc.Problem("Sphere", 0.1)
+    T = P.tube()
+    L = P.L()
+    x = T["x"]
+    u = x[:, 0] ** 2 + 3 * x[:, 1] * x[:, 2] - x[:, 2] ** 2 + x[:, 0]
+    lap = 2.0 - 2.0   # d2/dx2 x^2 + d2/dz2 (-z^2)
+    full = (T["nbr"] >= 0).all(axis=1)
+    assert np.allclose(L.matvec(u)[full], lap, atol=1e-9)
+    assert (np.diff(L.ptr)[full] == 7).all()
+
+
+def test_multiply_equals_explicit_operator():
+    P = orc.Problem("Sphere", 0.2)
+    L, E, I = P.L(), P.E(), P.ident()
+    x = np.random.default_rng(1).standard_normal(P.D)
+    for m, n in ((1.0, -1.0), (0.0, 1.0), (1.0, -0.37)):
+        A = P.explicit_operator(m, n)
+        assert np.allclose(orc.multiply(L, E, I, x, m, n), A.matvec(x), rtol=0, atol=1e-9)
+
+
+def test_bicgstab_residual_independent():
+    P = orc.Problem("Sphere", 0.1)
+    L, E, I = P.L(), P.E(), P.ident()
+    b = np.random.default_rng(2).standard_normal(P.D)
+    x, it, rr = orc.solve(L, E, I, b, 1.0, -1.0, tol=1e-10)
+    A = P.explicit_operator(1.0, -1.0)
+    true_rel = np.linalg.norm(b - A.matvec(x)) / np.linalg.norm(b)
+    assert rr <= 1e-10 and true_rel < 1e-9 and it > 5
+
+
+@pytest.mark.parametrize("problem,dx0,levels,expected", [
+    (5, 0.2, 3, 2.18548),    # PoissonProblems.cpp:119-137,169 (1+Laplace) on the sphere
+    (6, 0.2, 3, 2.11767),    # :140-154,169 hemisphere, first-order Dirichlet rim
+    (4, 0.05, 5, 1.9876),    # :97-115,169 shifted Poisson on the circle (2-D)
+])
+def test_reference_convergence_orders(problem, dx0, levels, expected):
+    errs, dxs = [], []
+    for i in range(levels):
+        dx = dx0 * 0.5 ** i
+        e, _ = orc.poisson_problem_error(problem, dx)
+        errs.append(e)
+        dxs.append(dx)
+    orders = [math.log(errs[i - 1] / errs[i]) / math.log(dxs[i - 1] / dxs[i]) for i in range(1, levels)]
+    avg = sum(orders) / len(orders)
+    assert abs(avg - expected) < 1e-2      # the reference's own pass criterion (:233)
+    assert abs(avg - expected) < 1e-4      # and in fact to the printed digits
+
+
+def test_config1_closed_form_second_order():
+    """(1 - Laplace_S) u = f with u = z + 3xy, f = 3z + 21xy on the unit sphere (closed form)."""
+    errs = []
+    for dx in (0.2, 0.1):
+        P = orc.Problem("Sphere", dx)
+        T = P.tube()
+        cp = T["cp"]
+        b = 3 * cp[:, 2] + 21 * cp[:, 0] * cp[:, 1]
+        x, it, rr = orc.solve(P.L(), P.E(), P.ident(), b, 1.0, -1.0)
+        Ep = P.plot_E()
+        th = np.repeat(np.arange(200) * (2 * np.pi / 199), 200)
+        ph = np.tile(np.arange(200) * (np.pi / 199), 200)
+        xp = np.stack([np.sin(ph) * np.cos(th), np.sin(ph) * np.sin(th), np.cos(ph)], 1)
+        uex = xp[:, 2] + 3 * xp[:, 0] * xp[:, 1]
+        errs.append(np.abs(Ep.matvec(x) - uex).max())
+    order = math.log(errs[0] / errs[1]) / math.log(2)
+    assert errs[1] < 5e-3 and order > 1.7

@@ -1,0 +1,175 @@
+"""Benchmark: screened-Poisson solves/s on a closest-point tube, FP32 Krylov +
+FP64 refinement vs plain FP64, with p50/p99 latency, the E-gather roofline and
+the CPU oracle timed on the host.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload sphere|torus-knot]
+
+A step is one pass of the hot path over one batch: `--batch` right-hand sides
+on the workload's operator solved to a 1e-10 relative residual with FP32
+BiCGStab inside FP64 iterative refinement, inputs resident in HBM.  The tube,
+E and L are built by the GPU engine itself.  Prints one JSON line.
+"""
+import argparse
+import json
+import math
+import os
+import statistics
+import sys
+import time
+
+import numpy as np
+
+REPO = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(REPO, "cpm-ibc_amd", "python"))
+
+HBM_PEAK_GBS = 8000.0
+
+
+def sphere_workload(ctx, cpmgpu, batch, seed=0):
+    """Unit sphere, h = 0.05, p = 3, (1 - Laplace_S) u = f, f = 3z + 21xy at cp (+ seeded smooth RHS)."""
+    tube = cpmgpu.Tube(ctx, "Sphere", 0.05)
+    op = tube.operator(1.0, -1.0)
+    cp = tube.export()["cp"]
+    rng = np.random.default_rng(seed)
+    rhs = [3 * cp[:, 2] + 21 * cp[:, 0] * cp[:, 1]]
+    while len(rhs) < batch:
+        a = rng.standard_normal(10)
+        x, y, z = cp[:, 0], cp[:, 1], cp[:, 2]
+        rhs.append(a[0] + a[1] * x + a[2] * y + a[3] * z + a[4] * x * y + a[5] * y * z + a[6] * x * z
+                   + a[7] * (x * x - y * y) + a[8] * (3 * z * z - 1) + a[9] * x * y * z)
+    return dict(name="unit sphere h=0.05 p=3 screened Poisson (1-Lap)u=f, f=3z+21xy", tube=tube, op=op,
+                rhs=rhs, m=1.0, n=-1.0, surface=("Sphere", 0.05, None, None))
+
+
+def cpu_baseline(work, args):
+    """The oracle (a scalar C++ restatement of the reference solver) on one host core: one FP64
+    solve of the first right-hand side to the reference's 1e-10 stop; setup excluded."""
+    sys.path.insert(0, os.path.join(REPO, "oracle"))
+    import orc
+    surf, dx, params, bbox = work["surface"]
+    P = orc.Problem(surf, dx, params=params, bbox=bbox)
+    L, E, I = P.L(), P.E(), P.ident()
+    b = work["rhs"][0]
+    t0 = time.perf_counter()
+    x, it, rr = orc.solve(L, E, I, b, work["m"], work["n"], tol=1e-10)
+    t = time.perf_counter() - t0
+    x_tight, _, _ = orc.solve(L, E, I, b, work["m"], work["n"], tol=1e-14, max_it=50000)
+    return dict(value=1.0 / t, unit="solves/s", cores=1, kind="port",
+                sample=f"1 FP64 solve (oracle, 1 thread) of the same system, {P.D} rows, {it} iterations, "
+                       f"{t:.2f} s"), x_tight
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--batch", type=int, default=8)
+    ap.add_argument("--workload", default="sphere")
+    ap.add_argument("--latency-samples", type=int, default=30)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (independent solves shard "
+              f"across GPUs with no communication; see DESIGN.md)", file=sys.stderr)
+        sys.exit(2)
+
+    import cpmgpu
+    ctx = cpmgpu.Context(0)
+    t0 = time.perf_counter()
+    work = sphere_workload(ctx, cpmgpu, args.batch)
+    setup_s = time.perf_counter() - t0
+    op, rhs = work["op"], work["rhs"]
+    D = op.D
+    B = len(rhs)
+    tol = 1e-10
+
+    # resident inputs/outputs
+    d_rhs = [ctx.alloc(8 * D) for _ in range(B)]
+    d_x = [ctx.alloc(8 * D) for _ in range(B)]
+    for d, r in zip(d_rhs, rhs):
+        ctx.h2d(d, np.ascontiguousarray(r))
+    o32 = cpmgpu.opts(tol=tol, precision=cpmgpu.CPM_FP32_REFINE)
+    o64 = cpmgpu.opts(tol=tol, precision=cpmgpu.CPM_FP64)
+    ops = [op] * B
+
+    for _ in range(args.warmup):
+        infos = ctx.solve_batch_device(ops, d_rhs, d_x, o32)
+    ctx.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        infos = ctx.solve_batch_device(ops, d_rhs, d_x, o32)
+    ctx.synchronize()
+    t_batch = time.perf_counter() - t0
+    value = B * args.steps / t_batch
+    ms_per_step = 1000 * t_batch / args.steps
+    x32 = [ctx.d2h(np.zeros(D), d) for d in d_x]
+    info32 = infos[0]
+
+    # plain FP64 GPU path: one right-hand side per solve, reference algorithm
+    ctx.solve_batch_device([op], d_rhs[:1], d_x[:1], o64)
+    n64 = max(3, args.steps // 2)
+    t0 = time.perf_counter()
+    for k in range(n64):
+        inf64 = ctx.solve_batch_device([op], [d_rhs[k % B]], [d_x[k % B]], o64)[0]
+    ctx.synchronize()
+    fp64_rate = n64 / (time.perf_counter() - t0)
+    inf64 = ctx.solve_batch_device([op], d_rhs[:1], d_x[:1], o64)[0]
+    x64 = ctx.d2h(np.zeros(D), d_x[0])
+
+    # latency of a single request (one RHS, FP32+refine)
+    lat = []
+    for k in range(args.latency_samples):
+        t0 = time.perf_counter()
+        ctx.solve_batch_device([op], [d_rhs[k % B]], [d_x[k % B]], o32)
+        lat.append(1000 * (time.perf_counter() - t0))
+    lat.sort()
+    p50 = statistics.median(lat)
+    p99 = lat[min(len(lat) - 1, int(math.ceil(0.99 * len(lat))) - 1)]
+
+    # roofline of the dominant kernel: E-gather, FP32, 8 interleaved RHS (what each batched iteration runs)
+    nr = 8 if B >= 8 else (4 if B >= 4 else (2 if B >= 2 else 1))
+    eg_ms, eg_bytes = op.profile_egather(cpmgpu.CPM_FP32_REFINE, nr, reps=50)
+    achieved = eg_bytes / (eg_ms * 1e-3) / 1e9
+    roof = dict(bound="hbm", achieved=round(achieved, 1), peak=HBM_PEAK_GBS, unit="GB/s",
+                frac=round(achieved / HBM_PEAK_GBS, 4), traffic=None,
+                kernel=f"k_egather<float,{nr}>", avg_launch_ms=round(eg_ms, 5), bytes_per_launch=int(eg_bytes))
+    eg64_ms, eg64_bytes = op.profile_egather(cpmgpu.CPM_FP64, 1, reps=50)
+
+    out = {
+        "metric": "screened-Poisson solves/s + p50/p99 latency, FP32+refine vs FP64 at 1e-8",
+        "value": round(value, 3),
+        "unit": "solves/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_per_step, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32 Krylov + f64 refinement",
+        "data": "synthetic (GPU-built tube of an analytic surface; analytic + seeded random RHS)",
+        "config": {"workload": work["name"], "rows": D, "rhs_per_step": B, "tol": tol,
+                   "precond": "diagonal", "tube_build_ms": round(work["tube"].info["build_ms"], 2),
+                   "setup_s": round(setup_s, 3)},
+        "fp64_solves_per_s": round(fp64_rate, 3),
+        "fp32refine_vs_fp64": round(value / fp64_rate, 3),
+        "p50_ms": round(p50, 3),
+        "p99_ms": round(p99, 3),
+        "iters_fp32refine": info32["iters"], "refinements": info32["refinements"],
+        "iters_fp64": inf64["iters"],
+        "egather_fp64_GBs": round(eg64_bytes / (eg64_ms * 1e-3) / 1e9, 1),
+        "roofline": roof,
+    }
+    if not args.no_cpu:
+        cb, x_or = cpu_baseline(work, args)
+        out["cpu_baseline"] = cb
+        out["err_vs_oracle_fp32refine"] = float(np.linalg.norm(x32[0] - x_or) / np.linalg.norm(x_or))
+        out["err_vs_oracle_fp64"] = float(np.linalg.norm(x64 - x_or) / np.linalg.norm(x_or))
+        out["gpu_vs_cpu"] = round(value / cb["value"], 1)
+    print(json.dumps(out))
+    ctx.close()
+
+
+if __name__ == "__main__":
+    main()

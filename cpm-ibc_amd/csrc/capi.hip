@@ -55,7 +55,7 @@ __global__ void k_pack(const PackEntry* ent, const int* bent, double* ws64, floa
         const double v = e.src ? e.src[row] : 0.0;
         if (ws64) ws64[i] = v;
         if (ws32) ws32[i] = (float)v;
-    } else {
+    } else if (e.dst) {   // padding lanes of a group have no caller vector
         e.dst[row] = ws64 ? ws64[i] : (double)ws32[i];
     }
 }
@@ -438,6 +438,43 @@ void multiply(Op* op, const double* dx, double* dy, double m, double n) {
     HIPCHK(hipGetLastError());
 }
 
+template <typename T, int NR>
+static void profile_egather_t(Op* op, int reps, double* avg_ms, double* bytes) {
+    Ctx* ctx = op->ctx;
+    const hipStream_t st = ctx->stream;
+    const int nblocks = (int)((op->D + kBlock - 1) / kBlock);
+    Seg<T> h{};
+    fill_seg(h, op, 0, 0, nblocks, NR, nullptr, nullptr, nullptr);
+    char* tb = reinterpret_cast<char*>(ctx->workspace("prof_tables", 4096 + sizeof(int) * nblocks));
+    auto* dseg = reinterpret_cast<Seg<T>*>(tb);
+    int* dbseg = reinterpret_cast<int*>(tb + 4096);
+    std::vector<int> hb(nblocks, 0);
+    HIPCHK(hipMemcpyAsync(dseg, &h, sizeof(h), hipMemcpyHostToDevice, st));
+    HIPCHK(hipMemcpyAsync(dbseg, hb.data(), sizeof(int) * nblocks, hipMemcpyHostToDevice, st));
+    T* x = reinterpret_cast<T*>(ctx->workspace("prof_x", sizeof(T) * op->D * NR));
+    T* y = reinterpret_cast<T*>(ctx->workspace("prof_y", sizeof(T) * op->D * NR));
+    HIPCHK(hipMemsetAsync(x, 0, sizeof(T) * op->D * NR, st));
+    LaunchCfg<T> c{dseg, dbseg, nblocks, st};
+    launch_egather<T, NR>(c, x, y);   // warm-up
+    hipEvent_t e0, e1;
+    HIPCHK(hipEventCreate(&e0));
+    HIPCHK(hipEventCreate(&e1));
+    HIPCHK(hipEventRecord(e0, st));
+    for (int r = 0; r < reps; ++r) launch_egather<T, NR>(c, x, y);
+    HIPCHK(hipEventRecord(e1, st));
+    HIPCHK(hipEventSynchronize(e1));
+    float ms = 0;
+    HIPCHK(hipEventElapsedTime(&ms, e0, e1));
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    std::vector<uint8_t> fl((size_t)op->D);
+    HIPCHK(hipMemcpy(fl.data(), op->flag.p, fl.size(), hipMemcpyDeviceToHost));
+    int64_t gathered = 0;
+    for (uint8_t f : fl) gathered += !(f & ROW_IDENTITY);
+    *avg_ms = ms / reps;
+    *bytes = (double)gathered * op->ew * (4.0 + sizeof(T)) + 2.0 * op->D * NR * sizeof(T) + (double)op->D;
+}
+
 }  // namespace cpmgpu
 
 // ====================================================================== C ABI
@@ -449,11 +486,12 @@ static inline const Ctx* C(const cpm_ctx* c) { return reinterpret_cast<const Ctx
 static inline Op* O(cpm_op* o) { return reinterpret_cast<Op*>(o); }
 static inline const Op* O(const cpm_op* o) { return reinterpret_cast<const Op*>(o); }
 
-template <typename F>
-static int guard(F&& f) {
+namespace cpmgpu {
+void set_last_error(const std::string& s) { g_err = s; }
+// Map the exception in flight to a status code and record its message.
+int status_of_current_exception() {
     try {
-        f();
-        return CPM_OK;
+        throw;
     } catch (const ArgError& e) {
         g_err = e.what();
         return CPM_ERR_ARG;
@@ -469,6 +507,20 @@ static int guard(F&& f) {
     } catch (const std::exception& e) {
         g_err = e.what();
         return CPM_ERR_STATE;
+    } catch (...) {
+        g_err = "unknown error";
+        return CPM_ERR_STATE;
+    }
+}
+}  // namespace cpmgpu
+
+template <typename F>
+static int guard(F&& f) {
+    try {
+        f();
+        return CPM_OK;
+    } catch (...) {
+        return status_of_current_exception();
     }
 }
 
@@ -650,6 +702,22 @@ int cpm_solve(cpm_op* op, const double* rhs, double* x, const cpm_solve_opts* op
     const double* r[1] = {rhs};
     double* xx[1] = {x};
     return cpm_solve_batch(reinterpret_cast<cpm_ctx*>(O(op)->ctx), 1, ops, r, xx, opts, info);
+}
+
+int cpm_profile_egather(cpm_op* op, int precision, int nrhs, int reps, double* avg_ms, double* bytes) {
+    return guard([&] {
+        if (!op || !avg_ms || !bytes || reps <= 0) throw ArgError("bad argument");
+        Op* o = O(op);
+        HIPCHK(hipSetDevice(o->ctx->device));
+        const bool f32 = precision == CPM_FP32_REFINE;
+        switch (nrhs) {
+        case 1: f32 ? profile_egather_t<float, 1>(o, reps, avg_ms, bytes) : profile_egather_t<double, 1>(o, reps, avg_ms, bytes); break;
+        case 2: f32 ? profile_egather_t<float, 2>(o, reps, avg_ms, bytes) : profile_egather_t<double, 2>(o, reps, avg_ms, bytes); break;
+        case 4: f32 ? profile_egather_t<float, 4>(o, reps, avg_ms, bytes) : profile_egather_t<double, 4>(o, reps, avg_ms, bytes); break;
+        case 8: f32 ? profile_egather_t<float, 8>(o, reps, avg_ms, bytes) : profile_egather_t<double, 8>(o, reps, avg_ms, bytes); break;
+        default: throw ArgError("nrhs must be 1, 2, 4 or 8");
+        }
+    });
 }
 
 int cpm_device_alloc(cpm_ctx* ctx, int64_t bytes, void** d_ptr) {

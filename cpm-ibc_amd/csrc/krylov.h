@@ -35,6 +35,10 @@ struct RefLaunch {
     static void resid(const LaunchCfg<double>& c, const double* b, const double* ax, double* r, const KState* inner);
 };
 
+// E-gather alone (identity rows copy), for measurement.
+template <typename T, int NR>
+void launch_egather(const LaunchCfg<T>& c, const T* x, T* out);
+
 // State reset before a (re)started solve: status, tolerance^2, iteration cap.
 void launch_reset_states(KState* st, int n, double tol2, int max_it, hipStream_t s);
 

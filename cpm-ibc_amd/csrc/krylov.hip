@@ -634,7 +634,12 @@ void RefLaunch<NR>::resid(const LaunchCfg<double>& c, const double* b, const dou
     hipLaunchKernelGGL((k_ref_resid<NR>), dim3(c.nblocks), dim3(kBlock), 0, c.stream, c.segs, c.bseg, b, ax, r, inner);
 }
 
-#define CPM_INST(T, NR) template struct Launch<T, NR>;
+template <typename T, int NR>
+void launch_egather(const LaunchCfg<T>& c, const T* x, T* out) {
+    hipLaunchKernelGGL((k_egather<T, NR>), dim3(c.nblocks), dim3(kBlock), 0, c.stream, c.segs, c.bseg, x, out, 0);
+}
+
+#define CPM_INST(T, NR) template struct Launch<T, NR>; template void launch_egather<T, NR>(const LaunchCfg<T>&, const T*, T*);
 CPM_INST(double, 1) CPM_INST(double, 2) CPM_INST(double, 4) CPM_INST(double, 8)
 CPM_INST(float, 1) CPM_INST(float, 2) CPM_INST(float, 4) CPM_INST(float, 8)
 template struct RefLaunch<1>;

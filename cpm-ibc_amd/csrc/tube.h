@@ -1,0 +1,71 @@
+// Tube construction on the GPU: closest points, banded index set in the
+// reference's breadth-first order, neighbours, base points, E and L rows.
+#pragma once
+#include <string>
+#include <vector>
+
+#include "engine.h"
+#include "ops.h"
+
+namespace cpmgpu {
+
+enum SurfType : int {
+    SURF_SPHERE = 1, SURF_HEMISPHERE = 2, SURF_TORUS = 3, SURF_CIRCLE3D = 4, SURF_POINT = 5,
+    SURF_POLYLINE = 6, SURF_ELLIPSOID = 7
+};
+
+// Device description of a surface (src/Surface.cpp:61-190 dispatch, resolved
+// once on the host instead of per query by string compare).
+struct SurfDev {
+    int type;
+    int dim;
+    int is_open;
+    int nseg;                 // polyline segments
+    double P[16];             // parameters (SurfaceSpecifier order)
+    const double* verts;      // polyline vertices, 3 per vertex
+    double ep0[3], ep1[3];    // open-polyline end points (boundary tags)
+};
+
+struct SurfaceHost {
+    std::string type;
+    std::vector<double> params;
+    double bbox[2] = {0, 0};
+    int dim = 3;
+    std::vector<double> verts;   // polyline vertices (x,y,z)...
+    bool is_open = false;
+};
+
+SurfaceHost surface_from_desc(const char* type, const double* params, int nparams, double bb0, double bb1, int dim,
+                              const double* verts, int64_t nverts);
+
+struct Tube {
+    Ctx* ctx = nullptr;
+    SurfaceHost surf;
+    SurfDev sd{};
+    Buffer surf_verts;
+    int dim = 3, p = 3;
+    double dx = 0, xstart = 0, rfac = 0, r_t = 0;
+    int64_t n = 0, N1d = 0, N = 0;
+    Buffer gidx;        // int32 [N1d^3]: tube index of a grid node, or a sentinel
+    Buffer rec;         // int64 [N]  linear grid index of node i
+    Buffer cpx;         // double [N*3]
+    Buffer dist;        // double [N]
+    Buffer bdy;         // int32 [N]
+    Buffer nbr;         // int32 [N*6], -1 = none
+    Buffer bpt;         // int32 [N]
+    int64_t levels = 0;
+    double build_ms = 0;
+};
+
+Tube* tube_build(Ctx* ctx, const SurfaceHost& s, int p, double dx);
+
+// E rows at query points xq (device, 3 doubles each) as slot-major ELL of width
+// (p+1)^dim; columns are tube indices.  Throws GeomError when a stencil leaves
+// the tube (the reference throws in Tube::ComputeBasePoints).
+void build_E_ell(const Tube& t, const double* d_xq, int64_t nq, int* ecol, double* eval, float* eval32,
+                 int64_t ld, hipStream_t st);
+
+// Operator (m I + n (Ld + Loff E)) of the plain closest-point problem on a GPU tube.
+Op* op_from_tube(Tube* t, double m, double n, int precond, int precond_iters);
+
+}  // namespace cpmgpu

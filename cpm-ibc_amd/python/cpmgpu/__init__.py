@@ -22,7 +22,8 @@ EXPORTS = [
     "cpm_ctx_device_bytes", "cpm_op_create_csr", "cpm_op_destroy", "cpm_op_rows", "cpm_op_set_mn",
     "cpm_op_set_precond", "cpm_multiply", "cpm_multiply_device", "cpm_solve", "cpm_solve_device",
     "cpm_solve_batch", "cpm_solve_batch_device", "cpm_device_alloc", "cpm_device_free", "cpm_memcpy_h2d",
-    "cpm_memcpy_d2h",
+    "cpm_memcpy_d2h", "cpm_tube_create", "cpm_tube_destroy", "cpm_tube_get_info", "cpm_tube_export",
+    "cpm_tube_export_E", "cpm_tube_export_L", "cpm_tube_interp_matrix", "cpm_op_create_from_tube", "cpm_profile_egather",
 ]
 
 
@@ -38,6 +39,16 @@ class SolveInfo(ctypes.Structure):
     def as_dict(self):
         return dict(status=STATUS.get(self.status, self.status), iters=self.iters, restarts=self.restarts,
                     refinements=self.refinements, rel_residual=self.rel_residual)
+
+
+class TubeInfo(ctypes.Structure):
+    _fields_ = [("nodes", ctypes.c_int64), ("n", ctypes.c_int64), ("N1d", ctypes.c_int64), ("nnz_L", ctypes.c_int64),
+                ("levels", ctypes.c_int64), ("xstart", ctypes.c_double), ("dx", ctypes.c_double),
+                ("tube_radius", ctypes.c_double), ("tube_radius_factor", ctypes.c_double),
+                ("build_ms", ctypes.c_double), ("dim", ctypes.c_int), ("p", ctypes.c_int)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
 
 
 class CpmError(RuntimeError):
@@ -84,6 +95,18 @@ def load(path=LIB_PATH):
     L.cpm_device_free.argtypes = [vp, vp]
     L.cpm_memcpy_h2d.argtypes = [vp, vp, vp, ctypes.c_int64]
     L.cpm_memcpy_d2h.argtypes = [vp, vp, vp, ctypes.c_int64]
+    L.cpm_tube_create.argtypes = [vp, ctypes.c_char_p, vp, ctypes.c_int, ctypes.c_double, ctypes.c_double,
+                                  ctypes.c_int, vp, ctypes.c_int64, ctypes.c_int, ctypes.c_double, ctypes.POINTER(vp)]
+    L.cpm_tube_destroy.argtypes = [vp]
+    L.cpm_tube_get_info.argtypes = [vp, ctypes.POINTER(TubeInfo)]
+    L.cpm_tube_export.argtypes = [vp, vp, vp, vp, vp, vp, vp, vp]
+    L.cpm_tube_export_E.argtypes = [vp, vp, vp, vp]
+    L.cpm_tube_export_L.argtypes = [vp, vp, vp, vp]
+    L.cpm_tube_interp_matrix.argtypes = [vp, vp, ctypes.c_int64, vp, vp, vp]
+    L.cpm_op_create_from_tube.argtypes = [vp, ctypes.c_double, ctypes.c_double, ctypes.c_int, ctypes.c_int,
+                                          ctypes.POINTER(vp)]
+    L.cpm_profile_egather.argtypes = [vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.POINTER(ctypes.c_double),
+                                      ctypes.POINTER(ctypes.c_double)]
     _lib = L
     return L
 
@@ -168,8 +191,12 @@ class Context:
 class Operator:
     """GPU operator A(m, n) = m I + n (Ldiag + Loff E) with identity rows (cpm::Solver::Init)."""
 
-    def __init__(self, ctx, L, E, ident, m, n, nD=0, precond=PRECOND_DIAGONAL, precond_iters=0):
+    def __init__(self, ctx, L, E, ident, m, n, nD=0, precond=PRECOND_DIAGONAL, precond_iters=0, handle=None):
         self.ctx = ctx
+        if handle is not None:
+            self._h = handle
+            self.D = load().cpm_op_rows(handle)
+            return
         D = len(L.ptr) - 1
         self._keep = [np.ascontiguousarray(a) for a in (L.ptr, L.col, L.val, E.ptr, E.col, E.val)]
         self._keep = [self._keep[0].astype(np.int32), self._keep[1].astype(np.int32), self._keep[2].astype(np.float64),
@@ -215,7 +242,99 @@ class Operator:
         _chk(load().cpm_solve(self._h, _ptr(rhs), _ptr(x), ctypes.byref(o or opts()), ctypes.byref(info)))
         return x, info.as_dict()
 
+    def profile_egather(self, precision=CPM_FP64, nrhs=1, reps=50):
+        ms = ctypes.c_double()
+        by = ctypes.c_double()
+        _chk(load().cpm_profile_egather(self._h, precision, nrhs, reps, ctypes.byref(ms), ctypes.byref(by)))
+        return ms.value, by.value
+
     def solve_device(self, d_rhs, d_x, o=None):
         info = SolveInfo()
         _chk(load().cpm_solve_device(self._h, d_rhs, d_x, ctypes.byref(o or opts()), ctypes.byref(info)))
         return info.as_dict()
+
+
+class CSR:
+    def __init__(self, ptr, col, val):
+        self.ptr, self.col, self.val = ptr, col, val
+
+    @property
+    def rows(self):
+        return len(self.ptr) - 1
+
+    def matvec(self, x):
+        rows = np.repeat(np.arange(self.rows), np.diff(self.ptr))
+        return np.bincount(rows, weights=self.val * x[self.col], minlength=self.rows)
+
+
+class Tube:
+    """GPU-built closest-point tube (Tube(SurfaceSpecifier&, p, dx))."""
+
+    def __init__(self, ctx, surface="Sphere", dx=0.1, params=None, bbox=None, p=3, verts=None):
+        self.ctx = ctx
+        par = np.ascontiguousarray(params if params is not None else [0.0], np.float64)
+        npar = 0 if params is None else len(par)
+        bb = bbox if bbox is not None else (0.0, 0.0)
+        vv = np.ascontiguousarray(verts if verts is not None else np.zeros((1, 3)), np.float64)
+        nv = 0 if verts is None else len(vv)
+        self._h = vp()
+        _chk(load().cpm_tube_create(ctx._h, surface.encode(), _ptr(par), npar, float(bb[0]), float(bb[1]), 3,
+                                    _ptr(vv), nv, p, dx, ctypes.byref(self._h)))
+        inf = TubeInfo()
+        _chk(load().cpm_tube_get_info(self._h, ctypes.byref(inf)))
+        self.info = inf.as_dict()
+        self.N = self.info["nodes"]
+
+    def close(self):
+        if self._h:
+            load().cpm_tube_destroy(self._h)
+            self._h = vp()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def export(self):
+        N = self.N
+        rec = np.zeros(N, np.int64)
+        x = np.zeros((N, 3))
+        cp = np.zeros((N, 3))
+        dist = np.zeros(N)
+        bdy = np.zeros(N, np.int32)
+        nbr = np.zeros((N, 6), np.int32)
+        bpt = np.zeros(N, np.int32)
+        _chk(load().cpm_tube_export(self._h, _ptr(rec), _ptr(x), _ptr(cp), _ptr(dist), _ptr(bdy), _ptr(nbr), _ptr(bpt)))
+        return dict(rec=rec, x=x, cp=cp, dist=dist, bdy=bdy, nbr=nbr, bpt=bpt)
+
+    def E(self):
+        W = (self.info["p"] + 1) ** 3
+        ptr = np.zeros(self.N + 1, np.int32)
+        col = np.zeros(self.N * W, np.int32)
+        val = np.zeros(self.N * W)
+        _chk(load().cpm_tube_export_E(self._h, _ptr(ptr), _ptr(col), _ptr(val)))
+        return CSR(ptr, col, val)
+
+    def L(self):
+        nnz = self.info["nnz_L"]
+        ptr = np.zeros(self.N + 1, np.int32)
+        col = np.zeros(nnz, np.int32)
+        val = np.zeros(nnz)
+        _chk(load().cpm_tube_export_L(self._h, _ptr(ptr), _ptr(col), _ptr(val)))
+        return CSR(ptr, col, val)
+
+    def interp(self, xq):
+        xq = np.ascontiguousarray(xq, np.float64).reshape(-1, 3)
+        W = (self.info["p"] + 1) ** 3
+        nq = len(xq)
+        ptr = np.zeros(nq + 1, np.int32)
+        col = np.zeros(nq * W, np.int32)
+        val = np.zeros(nq * W)
+        _chk(load().cpm_tube_interp_matrix(self._h, _ptr(xq), nq, _ptr(ptr), _ptr(col), _ptr(val)))
+        return CSR(ptr, col, val)
+
+    def operator(self, m, n, precond=PRECOND_DIAGONAL, precond_iters=0):
+        h = vp()
+        _chk(load().cpm_op_create_from_tube(self._h, m, n, precond, precond_iters, ctypes.byref(h)))
+        return Operator(self.ctx, None, None, None, m, n, handle=h)

@@ -128,6 +128,66 @@ int cpm_solve_batch(cpm_ctx* ctx, int n_inst, cpm_op* const* ops, const double* 
 int cpm_solve_batch_device(cpm_ctx* ctx, int n_inst, cpm_op* const* ops, const double* const* d_rhs,
                            double* const* d_x, const cpm_solve_opts* opts, cpm_solve_info* infos);
 
+/* ---- tube (closest-point computational band) -------------------------
+ * Replaces Tube(SurfaceSpecifier&, interp_deg, dx) (src/Tube.h:45,
+ * src/Tube.cpp:88-461): closest points on the GPU, the banded index set in
+ * the reference's breadth-first order (node i here is node i there),
+ * neighbours (src/Tube.cpp:620-695) and interpolation base points
+ * (:464-531).  surface_type / params / bbox follow SurfaceSpecifier
+ * (src/SurfaceSpecifier.cpp:65-256); nparams = 0 selects the reference
+ * defaults.  "Polyline" takes nverts vertices (x,y,z) as consecutive segments.
+ * Supported on the GPU: Sphere (radius[, centre]), Hemisphere, Torus (R, r),
+ * Circle3D (r, centre, normal), Point, Polyline; dim must be 3. */
+typedef struct cpm_tube_info {
+    int64_t nodes;          /* nNodes()                                  */
+    int64_t n;              /* grid has 2^n + 1 nodes per axis           */
+    int64_t N1d;
+    int64_t nnz_L;          /* entries of L (diagonal + tube neighbours) */
+    int64_t levels;         /* breadth-first levels                      */
+    double xstart;          /* grid origin (same on every axis)          */
+    double dx;
+    double tube_radius;
+    double tube_radius_factor;
+    double build_ms;        /* wall time of cpm_tube_create              */
+    int dim;
+    int p;                  /* interpolation degree                      */
+} cpm_tube_info;
+
+int cpm_tube_create(cpm_ctx* ctx, const char* surface_type, const double* params, int nparams, double bbox_lo,
+                    double bbox_hi, int dim, const double* verts, int64_t nverts, int p, double dx,
+                    cpm_tube** out);
+int cpm_tube_destroy(cpm_tube* tube);
+int cpm_tube_get_info(const cpm_tube* tube, cpm_tube_info* info);
+/* Node data in tube order (any pointer may be NULL): linear grid index
+ * (Grid.cpp:387-408), coordinates x (3 per node), closest points, distance,
+ * boundary tag, 6 neighbours (+x,-x,+y,-y,+z,-z; -1 = none, src/Node.h:23),
+ * base point (tube index). */
+int cpm_tube_export(const cpm_tube* tube, int64_t* rec, double* x, double* cpx, double* dist, int32_t* bdy,
+                    int32_t* nbr, int32_t* bpt);
+/* E (nodes x nodes, (p+1)^3 entries per row, ascending columns) -- the matrix of
+ * Interpolation(t).BuildInterpolationMatrix (src/Interpolation.cpp:8-121). */
+int cpm_tube_export_E(const cpm_tube* tube, int32_t* rowptr, int32_t* col, double* val);
+/* L (nodes x nodes, nnz_L entries) -- FDMatrices::BuildLaplacianMatrix (src/FDMatrices.cpp:11-49). */
+int cpm_tube_export_L(const cpm_tube* tube, int32_t* rowptr, int32_t* col, double* val);
+/* Interpolation rows at arbitrary points inside the tube, nq x nodes with
+ * (p+1)^3 entries per row: Interpolation(t, xq) (src/Interpolation.cpp:23-35),
+ * e.g. the plotting matrix of Helpers::GetPlotInterpMatrix (src/Helpers.cpp:97-103). */
+int cpm_tube_interp_matrix(const cpm_tube* tube, const double* xq, int64_t nq, int32_t* rowptr, int32_t* col,
+                           double* val);
+/* Operator of the plain closest-point problem built on the GPU from the tube:
+ * L and E of Helpers::SetupMatrices (src/Helpers.cpp:86-93) and Poisson::
+ * Initialize without boundaries (problems/Poisson.cpp:233-237). */
+int cpm_op_create_from_tube(cpm_tube* tube, double m, double n, int precond, int precond_iters, cpm_op** out);
+
+/* ---- measurement ---------------------------------------------------------
+ * No counterpart in the reference.  Times the operator's E-gather kernel (the
+ * dominant kernel of every BiCGStab iteration) on the context stream with HIP
+ * events: `reps` back-to-back launches over `nrhs` interleaved right-hand
+ * sides (1, 2, 4 or 8) in the given precision.  Returns the average launch
+ * duration and the launch's algorithmic HBM bytes: E columns + values once,
+ * x read once, tmp written once, one flag byte per row. */
+int cpm_profile_egather(cpm_op* op, int precision, int nrhs, int reps, double* avg_ms, double* bytes_per_launch);
+
 /* Device scratch for callers that keep vectors resident (bench, serving). */
 int cpm_device_alloc(cpm_ctx* ctx, int64_t bytes, void** d_ptr);
 int cpm_device_free(cpm_ctx* ctx, void* d_ptr);

@@ -14,7 +14,7 @@ def declared_symbols():
     for h in glob.glob(os.path.join(REPO, "include", "*.h")):
         txt = open(h).read()
         txt = re.sub(r"/\*.*?\*/", "", txt, flags=re.S)
-        for m in re.finditer(r"\b(cpm_[a-z0-9_]+)\s*\(", txt):
+        for m in re.finditer(r"\b(cpm_[A-Za-z0-9_]+)\s*\(", txt):
             syms.add(m.group(1))
     return syms
 

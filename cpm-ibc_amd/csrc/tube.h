@@ -59,6 +59,17 @@ struct Tube {
 
 Tube* tube_build(Ctx* ctx, const SurfaceHost& s, int p, double dx);
 
+// Shared kernels (defined in tube.hip).
+constexpr int32_t kUnvisited = 0x7fffffff;
+constexpr int32_t kOutside = 0x7ffffffe;
+constexpr int32_t kCand = 0x40000000;   // candidate key base; indices are < kCand
+__global__ void k_fill_i32(int32_t* a, int64_t n, int32_t v);
+__global__ void k_count(const int32_t* slot, int64_t ns, int* bcount);
+__global__ void k_scan(int* bcount, int64_t nb, int* total);
+__device__ void cp_surface(const SurfDev& s, const double* x, double* c, double& dist, int& bdy);
+__device__ void weights_1d(const double* xs, double xq, int S, double* w);
+SurfDev make_surfdev(const SurfaceHost& s, const double* dverts);
+
 // E rows at query points xq (device, 3 doubles each) as slot-major ELL of width
 // (p+1)^dim; columns are tube indices.  Throws GeomError when a stencil leaves
 // the tube (the reference throws in Tube::ComputeBasePoints).

@@ -27,9 +27,6 @@
 
 namespace cpmgpu {
 
-constexpr int32_t kUnvisited = 0x7fffffff;
-constexpr int32_t kOutside = 0x7ffffffe;
-constexpr int32_t kCand = 0x40000000;      // candidate key base; tube indices are < kCand
 
 // ---------------------------------------------------------------- closest points
 __device__ __forceinline__ double dsq(double a) { return a * a; }
@@ -421,7 +418,7 @@ SurfaceHost surface_from_desc(const char* type, const double* params, int nparam
     return s;
 }
 
-static SurfDev make_surfdev(const SurfaceHost& s, const double* dverts) {
+SurfDev make_surfdev(const SurfaceHost& s, const double* dverts) {
     SurfDev d{};
     d.dim = 3;
     for (size_t i = 0; i < s.params.size() && i < 16; ++i) d.P[i] = s.params[i];

@@ -9,6 +9,7 @@
 #include "../../include/cpm_gpu.h"
 #include "ops.h"
 #include "tube.h"
+#include "ibc.h"
 
 using namespace cpmgpu;
 
@@ -21,6 +22,34 @@ namespace {
 inline Ctx* C(cpm_ctx* c) { return reinterpret_cast<Ctx*>(c); }
 inline Tube* T(cpm_tube* t) { return reinterpret_cast<Tube*>(t); }
 inline const Tube* T(const cpm_tube* t) { return reinterpret_cast<const Tube*>(t); }
+inline Problem* PR(cpm_problem* p) { return reinterpret_cast<Problem*>(p); }
+inline const Problem* PR(const cpm_problem* p) { return reinterpret_cast<const Problem*>(p); }
+
+// ELL (slot-major) -> CSR with ascending columns, duplicates summed, zeros dropped.
+void ell_to_csr(int64_t D, int w, const std::vector<int>& col, const std::vector<double>& val, const double* diag,
+                std::vector<int32_t>& ptr, std::vector<int32_t>& c, std::vector<double>& v) {
+    ptr.assign(D + 1, 0);
+    c.clear();
+    v.clear();
+    std::vector<std::pair<int32_t, double>> row;
+    for (int64_t r = 0; r < D; ++r) {
+        row.clear();
+        for (int s = 0; s < w; ++s) row.push_back({col[(size_t)s * D + r], val[(size_t)s * D + r]});
+        if (diag) row.push_back({(int32_t)r, diag[r]});
+        std::sort(row.begin(), row.end(), [](const auto& a, const auto& b) { return a.first < b.first; });
+        for (size_t k = 0; k < row.size();) {
+            size_t e = k;
+            double acc = row[k].second;
+            while (e + 1 < row.size() && row[e + 1].first == row[k].first) acc += row[++e].second;
+            if (acc != 0.0) {
+                c.push_back(row[k].first);
+                v.push_back(acc);
+            }
+            k = e + 1;
+        }
+        ptr[r + 1] = (int32_t)c.size();
+    }
+}
 
 template <typename F>
 int guard(F&& f) {
@@ -163,6 +192,122 @@ int cpm_tube_export_L(const cpm_tube* tp, int32_t* rowptr, int32_t* col, double*
             }
         }
         rowptr[N] = (int32_t)k;
+    });
+}
+
+int cpm_problem_create(cpm_tube* tp, int n_ibc, const cpm_ibc_desc* ibcs, cpm_problem** out) {
+    return guard([&] {
+        if (!tp || !out || n_ibc < 0 || (n_ibc > 0 && !ibcs)) throw ArgError("null argument");
+        Tube* t = T(tp);
+        HIPCHK(hipSetDevice(t->ctx->device));
+        std::vector<IbcSpec> specs((size_t)n_ibc);
+        for (int c = 0; c < n_ibc; ++c) {
+            const cpm_ibc_desc& d = ibcs[c];
+            if (!d.type) throw ArgError("IBC type is null");
+            specs[c].surf = surface_from_desc(d.type, d.params, d.nparams, 0.0, 0.0, 3, d.verts, d.nverts);
+            specs[c].order = d.boundary_order;
+            specs[c].btype[0] = d.boundary_type[0];
+            specs[c].btype[1] = d.boundary_type[1];
+            specs[c].tol = d.on_ibc_tolerance;
+            specs[c].dof_scale = d.dof_radius_scale > 0 ? d.dof_radius_scale : 1.0;
+        }
+        *out = reinterpret_cast<cpm_problem*>(problem_build(t, specs));
+    });
+}
+
+int cpm_problem_destroy(cpm_problem* p) {
+    return guard([&] { delete PR(p); });
+}
+
+int cpm_problem_get_info(const cpm_problem* pp, cpm_problem_info* info) {
+    return guard([&] {
+        if (!pp || !info) throw ArgError("null argument");
+        const Problem& P = *PR(pp);
+        info->rows = P.D;
+        info->tube_nodes = P.N;
+        info->ibc_dofs = P.G;
+        info->dirichlet_rows = P.K;
+        info->identity_rows_start = P.identity_rows_start;
+        info->tagging_size0 = P.ibc.empty() ? 0 : P.ibc[0].nt;
+        info->dof_size0 = P.ibc.empty() ? 0 : P.ibc[0].nd;
+        info->n_ibc = (int)P.ibc.size();
+        info->build_ms = P.build_ms;
+    });
+}
+
+int cpm_problem_rhs_points(const cpm_problem* pp, double* pts, int32_t* kind) {
+    return guard([&] {
+        if (!pp) throw ArgError("null problem");
+        const Problem& P = *PR(pp);
+        if (pts) HIPCHK(hipMemcpy(pts, P.rhs_pts.p, P.D * 24, hipMemcpyDeviceToHost));
+        if (kind) HIPCHK(hipMemcpy(kind, P.rhs_kind.p, P.D * 4, hipMemcpyDeviceToHost));
+    });
+}
+
+int cpm_problem_subset(const cpm_problem* pp, int c, int which, int32_t* set_of_sub, double* cpc, double* dirs) {
+    return guard([&] {
+        if (!pp) throw ArgError("null problem");
+        const Problem& P = *PR(pp);
+        if (c < 0 || c >= (int)P.ibc.size()) throw ArgError("no such IBC");
+        const IbcData& I = P.ibc[c];
+        const int64_t n = which == 0 ? I.nt : I.nd;
+        if (set_of_sub) HIPCHK(hipMemcpy(set_of_sub, (which == 0 ? I.t_set : I.d_set).p, n * 4, hipMemcpyDeviceToHost));
+        if (cpc) HIPCHK(hipMemcpy(cpc, (which == 0 ? I.t_cpc : I.d_cpc).p, n * 24, hipMemcpyDeviceToHost));
+        if (dirs) HIPCHK(hipMemcpy(dirs, (which == 0 ? I.t_dir : I.d_dir).p, n * 24, hipMemcpyDeviceToHost));
+    });
+}
+
+int cpm_problem_export(const cpm_problem* pp, int32_t* Lp, int32_t* Lc, double* Lv, int32_t* Ep, int32_t* Ec,
+                       double* Ev, uint8_t* ident, int64_t* nnzL, int64_t* nnzE) {
+    return guard([&] {
+        if (!pp) throw ArgError("null problem");
+        const Problem& P = *PR(pp);
+        const int64_t D = P.D;
+        std::vector<int> ec((size_t)P.ew * D), lc((size_t)6 * D);
+        std::vector<double> ev((size_t)P.ew * D), lv((size_t)6 * D), ld(D);
+        std::vector<uint8_t> fl(D);
+        HIPCHK(hipMemcpy(ec.data(), P.ecol.p, ec.size() * 4, hipMemcpyDeviceToHost));
+        HIPCHK(hipMemcpy(ev.data(), P.eval64.p, ev.size() * 8, hipMemcpyDeviceToHost));
+        HIPCHK(hipMemcpy(lc.data(), P.lcol.p, lc.size() * 4, hipMemcpyDeviceToHost));
+        HIPCHK(hipMemcpy(lv.data(), P.lval64.p, lv.size() * 8, hipMemcpyDeviceToHost));
+        HIPCHK(hipMemcpy(ld.data(), P.ldiag64.p, ld.size() * 8, hipMemcpyDeviceToHost));
+        HIPCHK(hipMemcpy(fl.data(), P.flag.p, fl.size(), hipMemcpyDeviceToHost));
+        // identity rows carry e_r in E (Helpers::SetIdentityRows semantics)
+        for (int64_t r = 0; r < D; ++r)
+            if (fl[r] & ROW_IDENTITY) {
+                for (int s = 0; s < P.ew; ++s) { ec[(size_t)s * D + r] = (int)r; ev[(size_t)s * D + r] = 0.0; }
+                ev[r] = 1.0;
+            }
+        std::vector<int32_t> p1, c1, p2, c2;
+        std::vector<double> v1, v2;
+        ell_to_csr(D, 6, lc, lv, ld.data(), p1, c1, v1);
+        ell_to_csr(D, P.ew, ec, ev, nullptr, p2, c2, v2);
+        if (nnzL) *nnzL = (int64_t)c1.size();
+        if (nnzE) *nnzE = (int64_t)c2.size();
+        if (Lp && Lc && Lv) {
+            std::copy(p1.begin(), p1.end(), Lp);
+            std::copy(c1.begin(), c1.end(), Lc);
+            std::copy(v1.begin(), v1.end(), Lv);
+        }
+        if (Ep && Ec && Ev) {
+            std::copy(p2.begin(), p2.end(), Ep);
+            std::copy(c2.begin(), c2.end(), Ec);
+            std::copy(v2.begin(), v2.end(), Ev);
+        }
+        if (ident)
+            for (int64_t r = 0; r < D; ++r) ident[r] = (fl[r] & ROW_IDENTITY) ? 1 : 0;
+    });
+}
+
+int cpm_op_create_from_problem(cpm_problem* pp, double m, double n, int precond, int precond_iters, cpm_op** out) {
+    return guard([&] {
+        if (!pp || !out) throw ArgError("null argument");
+        if (precond < 0 || precond > 2) throw ArgError("unknown preconditioner");
+        Problem* P = PR(pp);
+        HIPCHK(hipSetDevice(P->ctx->device));
+        Op* op = op_from_problem(P, m, n, precond, precond_iters);
+        P->ctx->op_bytes += (int64_t)op->device_bytes();
+        *out = reinterpret_cast<cpm_op*>(op);
     });
 }
 

@@ -23,7 +23,9 @@ EXPORTS = [
     "cpm_op_set_precond", "cpm_multiply", "cpm_multiply_device", "cpm_solve", "cpm_solve_device",
     "cpm_solve_batch", "cpm_solve_batch_device", "cpm_device_alloc", "cpm_device_free", "cpm_memcpy_h2d",
     "cpm_memcpy_d2h", "cpm_tube_create", "cpm_tube_destroy", "cpm_tube_get_info", "cpm_tube_export",
-    "cpm_tube_export_E", "cpm_tube_export_L", "cpm_tube_interp_matrix", "cpm_op_create_from_tube", "cpm_profile_egather",
+    "cpm_tube_export_E", "cpm_tube_export_L", "cpm_tube_interp_matrix", "cpm_op_create_from_tube", "cpm_profile_egather", "cpm_problem_create", "cpm_problem_destroy",
+    "cpm_problem_get_info", "cpm_problem_rhs_points", "cpm_problem_subset", "cpm_problem_export",
+    "cpm_op_create_from_problem",
 ]
 
 
@@ -46,6 +48,23 @@ class TubeInfo(ctypes.Structure):
                 ("levels", ctypes.c_int64), ("xstart", ctypes.c_double), ("dx", ctypes.c_double),
                 ("tube_radius", ctypes.c_double), ("tube_radius_factor", ctypes.c_double),
                 ("build_ms", ctypes.c_double), ("dim", ctypes.c_int), ("p", ctypes.c_int)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+class IbcDesc(ctypes.Structure):
+    _fields_ = [("type", ctypes.c_char_p), ("params", ctypes.c_void_p), ("nparams", ctypes.c_int),
+                ("verts", ctypes.c_void_p), ("nverts", ctypes.c_int64), ("boundary_order", ctypes.c_int),
+                ("boundary_type", ctypes.c_int * 2), ("on_ibc_tolerance", ctypes.c_double),
+                ("dof_radius_scale", ctypes.c_double)]
+
+
+class ProblemInfo(ctypes.Structure):
+    _fields_ = [("rows", ctypes.c_int64), ("tube_nodes", ctypes.c_int64), ("ibc_dofs", ctypes.c_int64),
+                ("dirichlet_rows", ctypes.c_int64), ("identity_rows_start", ctypes.c_int64),
+                ("tagging_size0", ctypes.c_int64), ("dof_size0", ctypes.c_int64), ("n_ibc", ctypes.c_int),
+                ("build_ms", ctypes.c_double)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
@@ -107,6 +126,15 @@ def load(path=LIB_PATH):
                                           ctypes.POINTER(vp)]
     L.cpm_profile_egather.argtypes = [vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.POINTER(ctypes.c_double),
                                       ctypes.POINTER(ctypes.c_double)]
+    L.cpm_problem_create.argtypes = [vp, ctypes.c_int, vp, ctypes.POINTER(vp)]
+    L.cpm_problem_destroy.argtypes = [vp]
+    L.cpm_problem_get_info.argtypes = [vp, ctypes.POINTER(ProblemInfo)]
+    L.cpm_problem_rhs_points.argtypes = [vp, vp, vp]
+    L.cpm_problem_subset.argtypes = [vp, ctypes.c_int, ctypes.c_int, vp, vp, vp]
+    L.cpm_problem_export.argtypes = [vp, vp, vp, vp, vp, vp, vp, vp, ctypes.POINTER(ctypes.c_int64),
+                                     ctypes.POINTER(ctypes.c_int64)]
+    L.cpm_op_create_from_problem.argtypes = [vp, ctypes.c_double, ctypes.c_double, ctypes.c_int, ctypes.c_int,
+                                             ctypes.POINTER(vp)]
     _lib = L
     return L
 
@@ -338,3 +366,87 @@ class Tube:
         h = vp()
         _chk(load().cpm_op_create_from_tube(self._h, m, n, precond, precond_iters, ctypes.byref(h)))
         return Operator(self.ctx, None, None, None, m, n, handle=h)
+
+
+class IBC:
+    """An interior boundary curve (SurfaceSpecifier + IBCMetaData)."""
+
+    def __init__(self, type, params=None, verts=None, boundary_order=2, boundary_type=(0, 0), on_ibc_tolerance=0.0,
+                 dof_radius_scale=1.0):
+        self.type, self.params, self.verts = type, params, verts
+        self.boundary_order, self.boundary_type = boundary_order, boundary_type
+        self.on_ibc_tolerance, self.dof_radius_scale = on_ibc_tolerance, dof_radius_scale
+
+
+class Problem:
+    """PoissonWithIBC on a GPU tube: subsets, directions and the IBC-modified operator built on the device."""
+
+    def __init__(self, tube, ibcs):
+        self.tube = tube
+        n = len(ibcs)
+        descs = (IbcDesc * n)()
+        self._keep = []
+        for d, i in zip(descs, ibcs):
+            par = np.ascontiguousarray(i.params if i.params is not None else [0.0], np.float64)
+            vv = np.ascontiguousarray(i.verts if i.verts is not None else np.zeros((1, 3)), np.float64)
+            self._keep += [par, vv]
+            d.type = i.type.encode()
+            d.params = par.ctypes.data
+            d.nparams = 0 if i.params is None else len(par)
+            d.verts = vv.ctypes.data
+            d.nverts = 0 if i.verts is None else len(vv)
+            d.boundary_order = i.boundary_order
+            d.boundary_type[0], d.boundary_type[1] = i.boundary_type
+            d.on_ibc_tolerance = i.on_ibc_tolerance
+            d.dof_radius_scale = i.dof_radius_scale
+        self._h = vp()
+        _chk(load().cpm_problem_create(tube._h, n, ctypes.cast(descs, vp), ctypes.byref(self._h)))
+        inf = ProblemInfo()
+        _chk(load().cpm_problem_get_info(self._h, ctypes.byref(inf)))
+        self.info = inf.as_dict()
+        self.D = self.info["rows"]
+
+    def close(self):
+        if self._h:
+            load().cpm_problem_destroy(self._h)
+            self._h = vp()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def rhs_points(self):
+        pts = np.zeros((self.D, 3))
+        kind = np.zeros(self.D, np.int32)
+        _chk(load().cpm_problem_rhs_points(self._h, _ptr(pts), _ptr(kind)))
+        return pts, kind
+
+    def subset(self, c=0, which=0):
+        n = self.info["tagging_size0" if which == 0 else "dof_size0"] if c == 0 else None
+        if n is None:
+            raise ValueError("subset sizes are reported for curve 0")
+        s = np.zeros(n, np.int32)
+        cp = np.zeros((n, 3))
+        d = np.zeros((n, 3))
+        _chk(load().cpm_problem_subset(self._h, c, which, _ptr(s), _ptr(cp), _ptr(d)))
+        return s, cp, d
+
+    def export(self):
+        nl = ctypes.c_int64()
+        ne = ctypes.c_int64()
+        _chk(load().cpm_problem_export(self._h, None, None, None, None, None, None, None, ctypes.byref(nl),
+                                       ctypes.byref(ne)))
+        D = self.D
+        Lp, Lc, Lv = np.zeros(D + 1, np.int32), np.zeros(nl.value, np.int32), np.zeros(nl.value)
+        Ep, Ec, Ev = np.zeros(D + 1, np.int32), np.zeros(ne.value, np.int32), np.zeros(ne.value)
+        idn = np.zeros(D, np.uint8)
+        _chk(load().cpm_problem_export(self._h, _ptr(Lp), _ptr(Lc), _ptr(Lv), _ptr(Ep), _ptr(Ec), _ptr(Ev), _ptr(idn),
+                                       ctypes.byref(nl), ctypes.byref(ne)))
+        return CSR(Lp, Lc, Lv), CSR(Ep, Ec, Ev), idn
+
+    def operator(self, m, n, precond=PRECOND_DIAGONAL, precond_iters=0):
+        h = vp()
+        _chk(load().cpm_op_create_from_problem(self._h, m, n, precond, precond_iters, ctypes.byref(h)))
+        return Operator(self.tube.ctx, None, None, None, m, n, handle=h)

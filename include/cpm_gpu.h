@@ -179,6 +179,58 @@ int cpm_tube_interp_matrix(const cpm_tube* tube, const double* xq, int64_t nq, i
  * Initialize without boundaries (problems/Poisson.cpp:233-237). */
 int cpm_op_create_from_tube(cpm_tube* tube, double m, double n, int precond, int precond_iters, cpm_op** out);
 
+/* ---- problems with interior boundary conditions -------------------------
+ * Replaces PoissonWithIBC(surface_specs, ibc_specs, dx, p)
+ * (problems/PoissonWithIBC.cpp:8-47,162-192): IBC subsets
+ * (src/IBCSubsets.cpp:11-116), stencil directions from the Jacobian of the
+ * closest point (src/IBCSubset.cpp:190-268, src/Geometry.cpp:27-98), and the
+ * IBC-modified Lfull / Efull with identity rows
+ * (src/IBCMatrixManipulations.cpp:12-71,118-251,423-463), all on the GPU.
+ * Rows: [tube N | ghost DOFs G | 2nd-order Dirichlet rows K]. */
+typedef struct cpm_ibc_desc {
+    const char* type;          /* "Polyline", "Circle3D" or "Point"                          */
+    const double* params;      /* SurfaceSpecifier parameters (Circle3D: r, centre, normal;  */
+    int nparams;               /*  Point: x, y, z)                                           */
+    const double* verts;       /* Polyline: nverts x 3 vertices, consecutive segments        */
+    int64_t nverts;
+    int boundary_order;        /* IBCMetaData::boundary_order (1 or 2)                       */
+    int boundary_type[2];      /* 0 Dirichlet, 1 Neumann (IBCMetaData::boundary_type)         */
+    double on_ibc_tolerance;   /* IBCMetaData::on_ibc_tolerance                              */
+    double dof_radius_scale;   /* IBCMetaData::ibc_DOFSubset_radius_scale (<= 0: 1.0)        */
+} cpm_ibc_desc;
+
+typedef struct cpm_problem cpm_problem;
+
+typedef struct cpm_problem_info {
+    int64_t rows;                 /* D = N + G + K                                  */
+    int64_t tube_nodes;           /* N                                              */
+    int64_t ibc_dofs;             /* G  (IBCSubsets::TotalIBCDOFs)                   */
+    int64_t dirichlet_rows;       /* K  (IBCSubsets::Num2ndOrderDirichletRows)       */
+    int64_t identity_rows_start;  /* first 2nd-order Dirichlet row (0 when none)    */
+    int64_t tagging_size0;        /* tagging subset size of curve 0                 */
+    int64_t dof_size0;            /* DOF subset size of curve 0                     */
+    int n_ibc;
+    double build_ms;
+} cpm_problem_info;
+
+int cpm_problem_create(cpm_tube* tube, int n_ibc, const cpm_ibc_desc* ibcs, cpm_problem** out);
+int cpm_problem_destroy(cpm_problem* prob);
+int cpm_problem_get_info(const cpm_problem* prob, cpm_problem_info* info);
+/* Evaluation point of every row's right-hand side (PoissonWithIBC::GetRHS,
+ * problems/PoissonWithIBC.cpp:50-90): kind 0 = f at cp(x), 1 = f at cp_c,
+ * 2 = boundary value at cp_c. */
+int cpm_problem_rhs_points(const cpm_problem* prob, double* pts, int32_t* kind);
+/* Tagging (which = 0) or DOF (which = 1) subset of curve c in subset order:
+ * tube index, closest point on the curve, stencil direction. */
+int cpm_problem_subset(const cpm_problem* prob, int c, int which, int32_t* set_of_sub, double* cpc, double* dirs);
+/* Lfull / Efull as CSR (ascending columns, explicit zeros dropped) and the
+ * identity rows.  Call with NULL arrays to get nnz_L / nnz_E. */
+int cpm_problem_export(const cpm_problem* prob, int32_t* L_rowptr, int32_t* L_col, double* L_val, int32_t* E_rowptr,
+                       int32_t* E_col, double* E_val, uint8_t* identity_rows, int64_t* nnz_L, int64_t* nnz_E);
+/* Operator m I + n (Ld + Loff E) of the problem (cpm::Solver::Init with the
+ * problem's Lfull, Efull, identity rows). */
+int cpm_op_create_from_problem(cpm_problem* prob, double m, double n, int precond, int precond_iters, cpm_op** out);
+
 /* ---- measurement ---------------------------------------------------------
  * No counterpart in the reference.  Times the operator's E-gather kernel (the
  * dominant kernel of every BiCGStab iteration) on the context stream with HIP

@@ -48,6 +48,17 @@ def lib():
         L.orc_poisson_problem_error.restype = ctypes.c_double
         L.orc_poisson_problem_error.argtypes = [ctypes.c_int, ctypes.c_double, ctypes.POINTER(ctypes.c_int)]
         L.orc_weights_1d.argtypes = [f64p, ctypes.c_double, ctypes.c_int, f64p]
+        L.orc_problem_poisson_ibc.restype = ctypes.c_void_p
+        L.orc_problem_poisson_ibc.argtypes = [ctypes.c_char_p, ctypes.c_int, ctypes.c_void_p, ctypes.c_int,
+                                              ctypes.c_double, ctypes.c_double, ctypes.c_int, ctypes.c_void_p,
+                                              ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
+                                              ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_double,
+                                              ctypes.c_int]
+        L.orc_ibc_info.argtypes = [ctypes.c_void_p, i64p]
+        L.orc_ibc_subset.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, i64p, f64p, f64p]
+        L.orc_rhs_points.argtypes = [ctypes.c_void_p, f64p, np.ctypeslib.ndpointer(np.int32, flags="C_CONTIGUOUS")]
+        L.orc_ibc_study_order.restype = ctypes.c_double
+        L.orc_ibc_study_order.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.POINTER(ctypes.c_int)]
         _LIB = L
     return _LIB
 
@@ -187,3 +198,62 @@ def weights_1d(xs, xq):
     w = np.zeros(len(xs))
     lib().orc_weights_1d(xs, float(xq), len(xs), w)
     return w
+
+
+class IBC:
+    """One interior boundary curve: SurfaceSpecifier + IBCMetaData (src/IBCMetaData.h:9-24)."""
+
+    def __init__(self, type, params=None, verts=None, boundary_order=2, boundary_type=(0, 0), on_ibc_tolerance=0.0):
+        self.type, self.params, self.verts = type, params, verts
+        self.boundary_order, self.boundary_type, self.on_ibc_tolerance = boundary_order, boundary_type, on_ibc_tolerance
+
+
+class IBCProblem(Problem):
+    """PoissonWithIBC (problems/PoissonWithIBC.cpp) built by the oracle."""
+
+    def __init__(self, surface="Torus", dx=0.1, params=None, bbox=None, ibcs=(), p=3):
+        L = lib()
+        use_def = params is None
+        par = np.ascontiguousarray(params if params is not None else [0.0], dtype=np.float64)
+        bb = bbox if bbox is not None else (0.0, 0.0)
+        types = (ctypes.c_char_p * len(ibcs))(*[i.type.encode() for i in ibcs])
+        ip = np.ascontiguousarray(np.concatenate([np.asarray(i.params or [], float) for i in ibcs] + [np.zeros(1)]))
+        nip = np.ascontiguousarray([len(i.params or []) for i in ibcs], np.int32)
+        vv = [np.asarray(i.verts, float).reshape(-1, 3) for i in ibcs if i.verts is not None]
+        iv = np.ascontiguousarray(np.concatenate(vv + [np.zeros((1, 3))]).ravel())
+        nv = np.ascontiguousarray([len(i.verts) if i.verts is not None else 0 for i in ibcs], np.int64)
+        bo = np.ascontiguousarray([i.boundary_order for i in ibcs], np.int32)
+        bt = np.ascontiguousarray([t for i in ibcs for t in i.boundary_type], np.int32)
+        tl = np.ascontiguousarray([i.on_ibc_tolerance for i in ibcs], np.float64)
+        self._h = L.orc_problem_poisson_ibc(surface.encode(), int(use_def), par.ctypes.data, len(par), float(bb[0]),
+                                            float(bb[1]), len(ibcs), types, ip.ctypes.data, nip.ctypes.data,
+                                            iv.ctypes.data, nv.ctypes.data, bo.ctypes.data, bt.ctypes.data,
+                                            tl.ctypes.data, dx, p)
+        if not self._h:
+            raise RuntimeError(_err())
+        self._info()
+        ii = np.zeros(6, np.int64)
+        L.orc_ibc_info(self._h, ii)
+        self.G, self.K, self.identity_rows_start, self.n_ibc, self.n_tag0, self.n_dof0 = map(int, ii)
+
+    def subset(self, c=0, which=0):
+        n = self.n_tag0 if which == 0 else self.n_dof0
+        s = np.zeros(n, np.int64)
+        cp = np.zeros(n * 3)
+        d = np.zeros(n * 3)
+        lib().orc_ibc_subset(self._h, c, which, s, cp, d)
+        return s, cp.reshape(-1, 3), d.reshape(-1, 3)
+
+    def rhs_points(self):
+        pts = np.zeros(self.D * 3)
+        kind = np.zeros(self.D, np.int32)
+        lib().orc_rhs_points(self._h, pts, kind)
+        return pts.reshape(-1, 3), kind
+
+
+def ibc_study_order(problem, g):
+    it = ctypes.c_int(0)
+    o = lib().orc_ibc_study_order(problem, g, ctypes.byref(it))
+    if o < 0:
+        raise RuntimeError(_err())
+    return o

@@ -7,9 +7,11 @@
 #include <string>
 
 #include "orc.h"
+#include "orc_ibc.h"
 
 namespace orc {
 double poisson_problem_error(int problem, double dx, int* iters);
+double ibc_study_order(int problem, int g, int* iters_last);
 }
 
 using namespace orc;
@@ -18,8 +20,12 @@ static thread_local std::string g_err;
 
 namespace {
 struct Handle {
-    PoissonProblem P;
+    PoissonProblem P;              // tube, Lfull, Efull, ident (IBC problems fill the same fields)
+    IBCData ibc;
+    bool has_ibc = false;
     CSR extra;   // explicit operator or other matrix results
+    Vec rhs_pts;                   // evaluation point of every row (3 per row)
+    std::vector<int32_t> rhs_kind; // 0 tube, 1 ghost (f at cp_c), 2 identity (g at cp_c)
 };
 
 SurfaceSpec make_spec(const char* type, int use_defaults, const double* params, int np, double bb0, double bb1,
@@ -73,6 +79,128 @@ void* orc_problem_poisson(const char* type, int use_defaults, const double* para
     } catch (const std::exception& e) {
         g_err = e.what();
         return nullptr;
+    }
+}
+
+// PoissonWithIBC(surface, ibc_specs, dx, p).  IBC i has type itypes[i],
+// niparams[i] parameters (concatenated in iparams) or, for "Polyline",
+// nverts[i] vertices (concatenated in iverts), boundary order/type and
+// on_ibc_tolerance.
+void* orc_problem_poisson_ibc(const char* type, int use_defaults, const double* params, int np, double bb0,
+                              double bb1, int nibc, const char* const* itypes, const double* iparams,
+                              const int* niparams, const double* iverts, const int64_t* nverts, const int* border,
+                              const int* btype, const double* tol, double dx, int p) {
+    try {
+        SurfaceSpec s = make_spec(type, use_defaults, params, np, bb0, bb1, 3, nullptr, 0);
+        std::vector<SurfaceSpec> specs;
+        size_t po = 0, vo = 0;
+        for (int c = 0; c < nibc; ++c) {
+            SurfaceSpec is;
+            std::string ty(itypes[c]);
+            if (ty == "Polyline") {
+                std::vector<Vec> v((size_t)nverts[c], Vec(3));
+                for (int64_t i = 0; i < nverts[c]; ++i)
+                    for (int d = 0; d < 3; ++d) v[(size_t)i][(size_t)d] = iverts[3 * (vo + i) + d];
+                vo += (size_t)nverts[c];
+                is = SurfaceSpec::polyline(v, {bb0, bb1});
+            } else {
+                is.type = ty;
+                is.params.assign(iparams + po, iparams + po + niparams[c]);
+                is.bbox = {bb0, bb1};
+                is.dim = 3;
+            }
+            po += (size_t)niparams[c];
+            is.meta.boundary_order = border[c];
+            is.meta.boundary_type = {btype[2 * c], btype[2 * c + 1]};
+            is.meta.on_ibc_tolerance = tol[c];
+            specs.push_back(is);
+        }
+        auto h = std::make_unique<Handle>();
+        PoissonIBCProblem Q = make_poisson_ibc(s, specs, dx, p);
+        h->P.tube = std::move(Q.tube);
+        h->P.L = std::move(Q.L);
+        h->P.E = std::move(Q.E);
+        h->P.Lfull = std::move(Q.Lfull);
+        h->P.Efull = std::move(Q.Efull);
+        h->P.ident = std::move(Q.ident);
+        h->ibc = std::move(Q.ibc);
+        h->has_ibc = true;
+        // evaluation points of the right-hand side, PoissonWithIBC::GetRHS (problems/PoissonWithIBC.cpp:50-90)
+        const i64 D = h->P.Lfull.rows, N = h->P.tube.N();
+        h->rhs_pts.assign((size_t)D * 3, 0.0);
+        h->rhs_kind.assign((size_t)D, -1);
+        for (i64 i = 0; i < N; ++i) {
+            for (int d = 0; d < 3; ++d) h->rhs_pts[(size_t)(3 * i + d)] = h->P.tube.cpx[(size_t)(3 * i + d)];
+            h->rhs_kind[(size_t)i] = 0;
+        }
+        const IBCData& I = h->ibc;
+        for (size_t c = 0; c < I.dof.size(); ++c)
+            for (size_t i = 0; i < I.dof[c].set_of_sub.size(); ++i) {
+                const int ty = I.meta[c].boundary_type[0];
+                if (ty == 1 || (ty == 0 && I.meta[c].boundary_order == 2)) {
+                    const i64 r = (i64)i + I.start[c];
+                    for (int d = 0; d < 3; ++d) h->rhs_pts[(size_t)(3 * r + d)] = I.dof[c].cpc[i][d];
+                    h->rhs_kind[(size_t)r] = 1;
+                }
+                if (ty == 0) {
+                    const i64 r = I.meta[c].boundary_order == 2
+                                      ? I.identity_rows_start + I.dof[c].dir_sub_of_set[i] + I.dir_start[c]
+                                      : I.identity_rows_start + (i64)i + I.start[c];
+                    for (int d = 0; d < 3; ++d) h->rhs_pts[(size_t)(3 * r + d)] = I.dof[c].cpc[i][d];
+                    h->rhs_kind[(size_t)r] = 2;
+                }
+            }
+        return h.release();
+    } catch (const std::exception& e) {
+        g_err = e.what();
+        return nullptr;
+    }
+}
+
+// ints: total IBC DOFs G, 2nd-order rows K, identity_rows_start, #IBCs, tagging size of IBC 0, DOF size of IBC 0
+void orc_ibc_info(void* hp, int64_t* ints) {
+    Handle* h = static_cast<Handle*>(hp);
+    const IBCData& I = h->ibc;
+    ints[0] = I.total;
+    ints[1] = I.num2;
+    ints[2] = I.identity_rows_start;
+    ints[3] = (int64_t)I.dof.size();
+    ints[4] = I.tag.empty() ? 0 : (int64_t)I.tag[0].set_of_sub.size();
+    ints[5] = I.dof.empty() ? 0 : (int64_t)I.dof[0].set_of_sub.size();
+}
+
+// Tagging / DOF subsets of IBC c: tube index of each member (in order), cp_c,
+// directions (3 each).
+void orc_ibc_subset(void* hp, int c, int which, int64_t* set_of_sub, double* cpc, double* dirs) {
+    Handle* h = static_cast<Handle*>(hp);
+    const IBCData& I = h->ibc;
+    if (which == 0) {
+        const TagSubset& T = I.tag[(size_t)c];
+        for (size_t i = 0; i < T.set_of_sub.size(); ++i) {
+            set_of_sub[i] = T.set_of_sub[i];
+            for (int d = 0; d < 3; ++d) { cpc[3 * i + d] = T.cpc[i][d]; dirs[3 * i + d] = T.directions[i][d]; }
+        }
+    } else {
+        const DofSubset& T = I.dof[(size_t)c];
+        for (size_t i = 0; i < T.set_of_sub.size(); ++i) {
+            set_of_sub[i] = T.set_of_sub[i];
+            for (int d = 0; d < 3; ++d) { cpc[3 * i + d] = T.cpc[i][d]; dirs[3 * i + d] = T.directions[i][d]; }
+        }
+    }
+}
+
+void orc_rhs_points(void* hp, double* pts, int32_t* kind) {
+    Handle* h = static_cast<Handle*>(hp);
+    std::memcpy(pts, h->rhs_pts.data(), h->rhs_pts.size() * 8);
+    std::memcpy(kind, h->rhs_kind.data(), h->rhs_kind.size() * 4);
+}
+
+double orc_ibc_study_order(int problem, int g, int* iters) {
+    try {
+        return ibc_study_order(problem, g, iters);
+    } catch (const std::exception& e) {
+        g_err = e.what();
+        return -1.0;
     }
 }
 

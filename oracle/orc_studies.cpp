@@ -83,3 +83,87 @@ double poisson_problem_error(int problem, double dx, int* iters) {
 }
 
 }  // namespace orc
+
+// ---------------------------------------------------------------- IBC studies
+// convergence_studies/PoissonProblemsWithIBC/PoissonProblemsWithIBC.cpp:
+// problem 2 PoissonSphereCircleIBC (:230-256), problem 5 PoissonSpherePointIBC
+// (:274-301), boundary order 2, ten seeded geometries (:398-424).
+#include "orc_ibc.h"
+
+namespace orc {
+
+static const double kRand1[10] = {0.0, 0.170, 0.297, 0.331, 0.481, 0.511, 0.631, 0.735, 0.813, 0.973};
+static const double kRand2[10] = {0.25, 0.874, 0.473, 0.973, 0.812, 0.693, 0.873, 0.912, 0.945, 1.0};
+static const double kRand3[10] = {0.55, 0.123, -0.743, 0.547, -0.239, -0.693, 0.731, -0.175, 0.453, -0.364};
+
+// ComputeCircle3DParams (:117-138); normalises `normal` in place like the reference.
+static Vec circle3d_params(Vec& normal) {
+    double nn = std::sqrt(normal[0] * normal[0] + normal[1] * normal[1] + normal[2] * normal[2]);
+    if (nn > 0.0) for (auto& c : normal) c /= nn;
+    const double r = 0.2, theta = 0.1, phi = 0.2;
+    const double pp[3] = {r * std::cos(phi) * std::sin(theta), r * std::sin(phi) * std::sin(theta), r * std::cos(theta)};
+    const double rho = -(pp[0] * normal[0] + pp[1] * normal[1] + pp[2] * normal[2]);
+    const double cen[3] = {rho * normal[0], rho * normal[1], rho * normal[2]};
+    return {std::sqrt(1 - rho * rho), cen[0], cen[1], cen[2], normal[0], normal[1], normal[2]};
+}
+
+static double sphere_harmonic(const double* x) {
+    const double th = std::atan2(x[1], x[0]), ph = std::acos(x[2] / 1.0);
+    return std::cos(3 * th) * std::pow(std::sin(ph), 3) * (9 * std::pow(std::cos(ph), 2) - 1);
+}
+
+SurfaceSpec ibc_study_spec(int problem, int g, Vec& normal_state) {
+    SurfaceSpec s;
+    if (problem == 2) {
+        s.type = "Circle3D";
+        s.params = circle3d_params(normal_state);
+    } else if (problem == 5) {
+        const double th = 2.0 * M_PI * kRand1[g] - M_PI, ph = 2.0 * M_PI * kRand2[g] - M_PI;
+        s.type = "Point";
+        s.params = {std::cos(ph) * std::cos(th), std::cos(ph) * std::sin(th), std::sin(ph)};
+    } else {
+        throw std::runtime_error("unsupported IBC study problem");
+    }
+    s.bbox = {-1.0, 1.0};
+    s.dim = 3;
+    return s;
+}
+
+// One level of the study: error of the 2nd-order Dirichlet IBC solve at dx.
+double ibc_problem_error(int problem, int g, double dx, Vec& normal_state, int* iters) {
+    SurfaceSpec ibc = ibc_study_spec(problem, g, normal_state);
+    ibc.meta.on_ibc_tolerance = 0.1 * dx * dx;
+    ibc.meta.boundary_order = 2;
+    PoissonIBCProblem P = make_poisson_ibc(SurfaceSpec::defaults("Sphere"), {ibc}, dx);
+    auto f = [](const double* x) { return 30 * sphere_harmonic(x); };
+    Vec rhs = poisson_ibc_rhs(P, f, sphere_harmonic);
+    System S;
+    S.m = 0.0;
+    S.n = -1.0;
+    S.L = &P.Lfull;
+    S.E = &P.Efull;
+    S.ident = P.ident;
+    S.precond = PRECOND_DIAGONAL;
+    Vec u(rhs.size());
+    SolveInfo info = bicgstab(S, rhs.data(), u.data(), 1e-10, 100000);
+    if (iters) *iters = info.iters;
+    CSR Pe = ibc_plot_E(P);
+    Vec up((size_t)Pe.rows);
+    Pe.matvec(u.data(), up.data());
+    double e = 0.0;
+    for (size_t i = 0; i < up.size(); ++i) e = std::max(e, std::fabs(up[i] - sphere_harmonic(P.tube.surf.xp[i].data())));
+    return e;
+}
+
+double ibc_study_order(int problem, int g, int* iters_last) {
+    Vec normal = {kRand1[g], kRand2[g], kRand3[g]};
+    Vec err, dxs;
+    for (int lv = 0; lv < 3; ++lv) {
+        const double dx = std::pow(0.5, lv) * 0.2;
+        err.push_back(ibc_problem_error(problem, g, dx, normal, iters_last));
+        dxs.push_back(dx);
+    }
+    return average_order(err, dxs);
+}
+
+}  // namespace orc

@@ -1,0 +1,122 @@
+"""GPU assembly of interior-boundary-condition problems vs the oracle
+(PoissonWithIBC): subsets in identical order, directions, Lfull / Efull,
+identity rows, and solve parity; plus the config-2 torus-knot system."""
+import numpy as np
+import pytest
+
+import cpmgpu
+import orc
+from cpmgpu.workloads import polyline_parameter, torus_knot_polyline
+
+pytestmark = pytest.mark.gpu
+
+
+def circle3d_params(g):
+    r1 = [0.0, 0.170, 0.297, 0.331, 0.481, 0.511, 0.631, 0.735, 0.813, 0.973]
+    r2 = [0.25, 0.874, 0.473, 0.973, 0.812, 0.693, 0.873, 0.912, 0.945, 1.0]
+    r3 = [0.55, 0.123, -0.743, 0.547, -0.239, -0.693, 0.731, -0.175, 0.453, -0.364]
+    n = np.array([r1[g], r2[g], r3[g]])
+    n /= np.linalg.norm(n)
+    pp = 0.2 * np.array([np.cos(0.2) * np.sin(0.1), np.sin(0.2) * np.sin(0.1), np.cos(0.1)])
+    rho = -pp @ n
+    return [np.sqrt(1 - rho * rho), *(rho * n), *n]
+
+
+def _csr_equal(A, B, tol):
+    assert np.array_equal(A.ptr, B.ptr)
+    assert np.array_equal(A.col, B.col)
+    assert np.abs(A.val - B.val).max() <= tol * max(1.0, np.abs(B.val).max())
+
+
+def _compare(gctx, surface, dx, params, bbox, ibc_kw, check_solve=True, m=0.0, n=-1.0):
+    oi = orc.IBC(**ibc_kw)
+    P = orc.IBCProblem(surface, dx, params=params, bbox=bbox, ibcs=[oi])
+    tg = cpmgpu.Tube(gctx, surface, dx, params=params, bbox=bbox)
+    Q = cpmgpu.Problem(tg, [cpmgpu.IBC(**ibc_kw)])
+    info = Q.info
+    print(f"D={P.D} G={P.G} K={P.K} tagging={P.n_tag0} dof={P.n_dof0} build_ms={info['build_ms']:.1f}", flush=True)
+    assert (info["rows"], info["ibc_dofs"], info["dirichlet_rows"]) == (P.D, P.G, P.K)
+    for which in (0, 1):
+        so, co, do = P.subset(0, which)
+        sg, cg, dg = Q.subset(0, which)
+        assert np.array_equal(sg, so)                 # same nodes in the same breadth-first order
+        assert np.abs(cg - co).max() <= 1e-14
+        scale = max(1e-300, np.abs(do).max())
+        assert np.abs(dg - do).max() <= 1e-9 * scale  # directions: Jcp eigenvectors, closed form vs Jacobi
+    Lg, Eg, Ig = Q.export()
+    Lo, Eo, Io = P.L(), P.E(), P.ident()
+    assert np.array_equal(Ig, Io)
+    # the oracle keeps explicit zeros (setFromTriplets); the export drops them
+    def prune(A):
+        keep = A.val != 0
+        rows = np.repeat(np.arange(A.rows), np.diff(A.ptr))[keep]
+        ptr = np.concatenate([[0], np.cumsum(np.bincount(rows, minlength=A.rows))]).astype(np.int32)
+        return orc.CSR(ptr, A.col[keep], A.val[keep])
+    _csr_equal(Lg, prune(Lo), 1e-15)
+    _csr_equal(Eg, prune(Eo), 1e-13)
+    pts_g, kind_g = Q.rhs_points()
+    pts_o, kind_o = P.rhs_points()
+    assert np.array_equal(kind_g, kind_o)
+    assert np.abs(pts_g - pts_o).max() <= 1e-14
+    if check_solve:
+        b = np.random.default_rng(11).standard_normal(P.D)
+        x_ref, _, _ = orc.solve(Lo, Eo, Io, b, m, n, tol=1e-13, max_it=20000)
+        op = Q.operator(m, n)
+        x, inf = op.solve(b, cpmgpu.opts(tol=1e-13, max_iters=20000))
+        rel = np.linalg.norm(x - x_ref) / np.linalg.norm(x_ref)
+        print(f"fp64 solve iters={inf['iters']} rel_diff={rel:.2e}", flush=True)
+        assert inf["status"] == "converged" and rel <= 1e-10
+    return P, Q
+
+
+@pytest.mark.parametrize("g", [0, 3])
+def test_sphere_circle_ibc(gpu_ctx, g):
+    """Problem 2 of the reference's IBC study (sphere + Circle3D, 2nd-order Dirichlet)."""
+    print(f"case sphere+circle g={g}", flush=True)
+    dx = 0.1
+    _compare(gpu_ctx, "Sphere", dx, None, None,
+             dict(type="Circle3D", params=circle3d_params(g), boundary_order=2, on_ibc_tolerance=0.1 * dx * dx))
+
+
+def test_sphere_point_ibc(gpu_ctx):
+    """Problem 5 of the reference's IBC study (sphere + point, 2nd-order Dirichlet)."""
+    print("case sphere+point", flush=True)
+    dx = 0.1
+    th, ph = 2 * np.pi * 0.170 - np.pi, 2 * np.pi * 0.874 - np.pi
+    _compare(gpu_ctx, "Sphere", dx, None, None,
+             dict(type="Point", params=[np.cos(ph) * np.cos(th), np.cos(ph) * np.sin(th), np.sin(ph)],
+                  boundary_order=2, on_ibc_tolerance=0.1 * dx * dx))
+
+
+def test_first_order_and_neumann(gpu_ctx):
+    print("case first-order dirichlet / neumann", flush=True)
+    dx = 0.1
+    for kw in (dict(boundary_order=1, boundary_type=(0, 0)), dict(boundary_order=2, boundary_type=(1, 1))):
+        _compare(gpu_ctx, "Sphere", dx, None, None,
+                 dict(type="Circle3D", params=circle3d_params(1), on_ibc_tolerance=0.1 * dx * dx, **kw),
+                 m=1.0, n=-1.0)
+
+
+def test_torus_knot_config2(gpu_ctx):
+    """Config 2: Laplace on the torus R=1, r=0.4, h=0.02 with a 2000-segment (2,3) torus-knot Dirichlet curve."""
+    print("case config-2 torus knot", flush=True)
+    dx = 0.02
+    V = torus_knot_polyline()
+    kw = dict(type="Polyline", verts=V, boundary_order=2, on_ibc_tolerance=0.1 * dx * dx)
+    P, Q = _compare(gpu_ctx, "Torus", dx, [1.0, 0.4], (-1.4, 1.4), kw, check_solve=False)
+    pts, kind = Q.rhs_points()
+    b = np.zeros(Q.D)
+    idr = np.flatnonzero(kind == 2)
+    b[idr] = np.sin(3 * polyline_parameter(V, pts[idr]))
+    op = Q.operator(0.0, -1.0)
+    x, inf = op.solve(b, cpmgpu.opts(tol=1e-12, max_iters=20000))
+    assert inf["status"] == "converged"
+    # residual of the GPU solution measured with the oracle's operator
+    r = b - orc.multiply(P.L(), P.E(), P.ident(), x, 0.0, -1.0)
+    rel_res = np.linalg.norm(r) / np.linalg.norm(b)
+    x32, i32 = op.solve(b, cpmgpu.opts(tol=1e-12, precision=cpmgpu.CPM_FP32_REFINE))
+    rel32 = np.linalg.norm(x32 - x) / np.linalg.norm(x)
+    print(f"cfg2 fp64 iters={inf['iters']} oracle-residual={rel_res:.2e}; fp32+refine iters={i32['iters']} "
+          f"refinements={i32['refinements']} rel_diff={rel32:.2e}", flush=True)
+    assert rel_res <= 5e-12
+    assert rel32 <= 1e-8

@@ -208,7 +208,7 @@ __global__ void k_tbfs_emit(const int32_t* slot, int64_t ns, const int* boff, co
     for (int k = 0; k < 3; ++k) t_cpc[3 * t + k] = ccp[3 * ci + k];
     t_dist[t] = cdist[ci];
     t_bdy[t] = cbdy[ci];
-    next[before] = (int32_t)t;
+    next[before] = nb;   // the frontier holds tube indices
 }
 
 // ---------------------------------------------------------------- ordered selection

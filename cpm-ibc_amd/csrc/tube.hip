@@ -361,7 +361,7 @@ __global__ void k_erows(int64_t nq, const double* xq, const int32_t* gidx, GridD
         for (int j = 0; j < S; ++j)
             for (int k = 0; k < S; ++k, ++s) {
                 const int32_t gv = gidx[((b[0] + i) * g.N1d + (b[1] + j)) * g.N1d + (b[2] + k)];
-                int col = (int)q;
+                int col = 0;
                 if (gv >= 0 && gv < kCand) col = gv;
                 else atomicOr(err, 8);   // stencil leaves the tube
                 const double v = w[2][k] * w[1][j] * w[0][i];

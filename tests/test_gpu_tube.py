@@ -39,7 +39,7 @@ def test_tube_matches_oracle(gpu_ctx, surface, dx, params, bbox):
     diff = np.flatnonzero(Tg["bpt"] != To["bpt"])
     print(f"nodes={P.N} base-point ties={tie.sum()} differing={len(diff)}")
     assert tie[diff].all()
-    assert len(diff) <= max(8, 1e-3 * P.N)
+    assert len(diff) <= max(8, 0.1 * tie.sum())
     if P.N < 400000:
         Lg, Lo = tg.L(), P.L()
         assert np.array_equal(Lg.ptr, Lo.ptr) and np.array_equal(Lg.col, Lo.col) and np.array_equal(Lg.val, Lo.val)
@@ -51,7 +51,7 @@ def test_tube_matches_oracle(gpu_ctx, surface, dx, params, bbox):
         cg, co = Eg.col.reshape(-1, W), Eo.col.reshape(-1, W)
         vg, vo = Eg.val.reshape(-1, W), Eo.val.reshape(-1, W)
         assert np.array_equal(cg[same], co[same])
-        assert np.abs(vg[same] - vo[same]).max() <= 1e-14
+        assert np.abs(vg[same] - vo[same]).max() <= 2e-13   # weights from cp that differ by ulps
         # at tie nodes both stencils interpolate smooth data identically
         x = To["x"]
         q = x[:, 0] ** 3 - 2 * x[:, 1] * x[:, 2] ** 2 + 0.5 * x[:, 1] + x[:, 2]

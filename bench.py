@@ -100,7 +100,7 @@ def main():
     ap.add_argument("--batch", type=int, default=8)
     ap.add_argument("--workload", default="torus-knot", choices=["torus-knot", "sphere"])
     ap.add_argument("--cpu-iters", type=int, default=100)
-    ap.add_argument("--inner-tol", type=float, default=1e-4)
+    ap.add_argument("--update-delta", type=float, default=1e-3)
     ap.add_argument("--latency-samples", type=int, default=30)
     ap.add_argument("--no-cpu", action="store_true")
     args = ap.parse_args()
@@ -127,7 +127,7 @@ def main():
     d_x = [ctx.alloc(8 * D) for _ in range(B)]
     for d, r in zip(d_rhs, rhs):
         ctx.h2d(d, np.ascontiguousarray(r))
-    o32 = cpmgpu.opts(tol=tol, precision=cpmgpu.CPM_FP32_REFINE, inner_tol=args.inner_tol)
+    o32 = cpmgpu.opts(tol=tol, precision=cpmgpu.CPM_FP32_REFINE, update_delta=args.update_delta)
     o64 = cpmgpu.opts(tol=tol, precision=cpmgpu.CPM_FP64)
     ops = [op] * B
 
@@ -194,7 +194,7 @@ def main():
         "fp32refine_vs_fp64": round(value / fp64_rate, 3),
         "p50_ms": round(p50, 3),
         "p99_ms": round(p99, 3),
-        "iters_fp32refine": info32["iters"], "refinements": info32["refinements"],
+        "iters_fp32refine": info32["iters"], "updates": info32["updates"],
         "iters_fp64": inf64["iters"],
         "egather_fp64_GBs": round(eg64_bytes / (eg64_ms * 1e-3) / 1e9, 1),
         "roofline": roof,

@@ -187,8 +187,7 @@ struct Group {
 };
 
 template <typename T>
-static void fill_seg(Seg<T>& s, const Op* op, int64_t off, int block0, int nblocks, int nr, KState* st, double* part,
-                     unsigned* cnt) {
+static void fill_seg(Seg<T>& s, const Op* op, int64_t off, int block0, int nblocks, int nr, KState* st, double* part) {
     s.op = op->view<T>();
     s.m = op->m;
     s.n = op->n;
@@ -199,7 +198,6 @@ static void fill_seg(Seg<T>& s, const Op* op, int64_t off, int block0, int nbloc
     s.precond = op->precond;
     s.st = st;
     s.part = part;
-    s.cnt = cnt;
 }
 
 template <int NR>
@@ -221,13 +219,14 @@ static void run_batch(Ctx* ctx, const std::vector<Group>& groups, const std::vec
         nb[g] = (int)((groups[g].op->D + kBlock - 1) / kBlock);
         blocks += nb[g];
     }
-    // device tables: segments (x2 precisions), block map, states, partials, counters
+    // device tables: segments (one per precision), block map, states, partials.
+    // The FP64 and FP32 segments of a group share one state record per rhs.
     const int nstate = G * kMaxNR;
+    if (nstate > Ctx::kHostStates) throw ArgError("batch too large");
     const size_t part_stride = 2 * NR;
     auto r256 = [](size_t n) { return (n + 255) & ~size_t(255); };
     const size_t bytes_tables = r256(sizeof(Seg<double>) * G) + r256(sizeof(Seg<float>) * G) + r256(sizeof(int) * blocks) +
-                                2 * r256(sizeof(KState) * nstate) + 2 * r256(sizeof(double) * part_stride * blocks) +
-                                r256(sizeof(unsigned) * G * 2) + 256;
+                                r256(sizeof(KState) * nstate) + r256(sizeof(double) * part_stride * blocks) + 256;
     char* tb = reinterpret_cast<char*>(ctx->workspace("tables", bytes_tables));
     char* const tb_end = tb + bytes_tables;
     auto carve = [&](size_t n) {
@@ -239,43 +238,37 @@ static void run_batch(Ctx* ctx, const std::vector<Group>& groups, const std::vec
     auto* segs64 = reinterpret_cast<Seg<double>*>(carve(sizeof(Seg<double>) * G));
     auto* segs32 = reinterpret_cast<Seg<float>*>(carve(sizeof(Seg<float>) * G));
     int* bseg = reinterpret_cast<int*>(carve(sizeof(int) * blocks));
-    KState* st64 = reinterpret_cast<KState*>(carve(sizeof(KState) * nstate));
-    KState* st32 = reinterpret_cast<KState*>(carve(sizeof(KState) * nstate));
-    double* part64 = reinterpret_cast<double*>(carve(sizeof(double) * part_stride * blocks));
-    double* part32 = reinterpret_cast<double*>(carve(sizeof(double) * part_stride * blocks));
-    unsigned* cnt = reinterpret_cast<unsigned*>(carve(sizeof(unsigned) * G * 2));
+    KState* sts = reinterpret_cast<KState*>(carve(sizeof(KState) * nstate));
+    double* part = reinterpret_cast<double*>(carve(sizeof(double) * part_stride * blocks));
 
     std::vector<Seg<double>> h64(G);
     std::vector<Seg<float>> h32(G);
     std::vector<int> hb(blocks);
     for (int g = 0; g < G; ++g) {
         const int nr = (int)groups[g].inst.size();
-        fill_seg(h64[g], groups[g].op, off[g], b0[g], nb[g], nr, st64 + g * kMaxNR, part64 + part_stride * b0[g],
-                 cnt + g);
-        fill_seg(h32[g], groups[g].op, off[g], b0[g], nb[g], nr, st32 + g * kMaxNR, part32 + part_stride * b0[g],
-                 cnt + G + g);
+        fill_seg(h64[g], groups[g].op, off[g], b0[g], nb[g], nr, sts + g * kMaxNR, part + part_stride * b0[g]);
+        fill_seg(h32[g], groups[g].op, off[g], b0[g], nb[g], nr, sts + g * kMaxNR, part + part_stride * b0[g]);
         for (int b = 0; b < nb[g]; ++b) hb[b0[g] + b] = g;
     }
     HIPCHK(hipMemcpyAsync(segs64, h64.data(), sizeof(Seg<double>) * G, hipMemcpyHostToDevice, st));
-    HIPCHK(hipMemcpyAsync(segs32, h32.data(), sizeof(Seg<float>) * G, hipMemcpyHostToDevice, st));
+    if (refine) HIPCHK(hipMemcpyAsync(segs32, h32.data(), sizeof(Seg<float>) * G, hipMemcpyHostToDevice, st));
     HIPCHK(hipMemcpyAsync(bseg, hb.data(), sizeof(int) * blocks, hipMemcpyHostToDevice, st));
-    HIPCHK(hipMemsetAsync(cnt, 0, sizeof(unsigned) * G * 2, st));
 
-    // workspace vectors
-    const size_t n64 = refine ? 14 : 11, n32 = refine ? 11 : 0;
-    double* w64 = ctx->workspace("vec64", sizeof(double) * total * n64 + 1024);
-    float* w32 = refine ? reinterpret_cast<float*>(ctx->workspace("vec32", sizeof(float) * total * n32 + 1024)) : nullptr;
-    Vecs<double> V64;
-    {
-        double* q = w64;
-        double** f[] = {&V64.b, &V64.x, &V64.r, &V64.r0, &V64.p, &V64.v, &V64.s, &V64.t, &V64.y, &V64.z, &V64.tmp};
-        for (auto* pp : f) { *pp = q; q += total; }
-    }
+    // workspace vectors, each `total` entries: FP64 b x r r0 p v s t y z tmp;
+    // mixed: FP64 b x r t tmp and FP32 x r r0 p v s t y z tmp.
+    Vecs<double> V64{};
     Vecs<float> V32{};
-    if (refine) {
-        float* q = w32;
-        float** f[] = {&V32.b, &V32.x, &V32.r, &V32.r0, &V32.p, &V32.v, &V32.s, &V32.t, &V32.y, &V32.z, &V32.tmp};
-        for (auto* pp : f) { *pp = q; q += total; }
+    {
+        std::vector<double**> f64;
+        if (refine) f64 = {&V64.b, &V64.x, &V64.r, &V64.t, &V64.tmp};
+        else f64 = {&V64.b, &V64.x, &V64.r, &V64.r0, &V64.p, &V64.v, &V64.s, &V64.t, &V64.y, &V64.z, &V64.tmp};
+        double* q = ctx->workspace("vec64", sizeof(double) * total * f64.size() + 1024);
+        for (auto* pp : f64) { *pp = q; q += total; }
+        if (refine) {
+            float** f32[] = {&V32.x, &V32.r, &V32.r0, &V32.p, &V32.v, &V32.s, &V32.t, &V32.y, &V32.z, &V32.tmp};
+            float* q32 = reinterpret_cast<float*>(ctx->workspace("vec32", sizeof(float) * total * 10 + 1024));
+            for (auto* pp : f32) { *pp = q32; q32 += total; }
+        }
     }
 
     // pack right-hand sides
@@ -300,91 +293,68 @@ static void run_batch(Ctx* ctx, const std::vector<Group>& groups, const std::vec
     HIPCHK(hipMemcpyAsync(dpb, pb.data(), sizeof(int) * pb.size(), hipMemcpyHostToDevice, st));
     hipLaunchKernelGGL(k_pack, dim3((unsigned)pb.size()), dim3(256), 0, st, dpe, dpb, V64.b, (float*)nullptr, 1);
 
-    LaunchCfg<double> c64{segs64, bseg, blocks, st};
-    LaunchCfg<float> c32{segs32, bseg, blocks, st};
+    const LaunchCfg<double> c64{segs64, bseg, blocks, G, st};
+    const LaunchCfg<float> c32{segs32, bseg, blocks, G, st};
     KState* hs = ctx->host_states;
-    if (nstate > Ctx::kHostStates) throw ArgError("batch too large");
+    const double delta = o.update_delta > 0 ? std::min(o.update_delta, 0.999) : 1e-3;
+    launch_reset_states(sts, nstate, o.tol * o.tol, o.max_iters, delta * delta, st);
 
-    // Krylov driver for one precision: runs until no instance is RUN.
-    auto krylov = [&](auto tag, KState* dst, auto& cfg, auto& V) {
-        using T = decltype(tag);
-        Launch<T, NR>::init(cfg, V.b, V.x, V.r, V.r0, V.p, V.v);
-        int chunk = 4;
+    // Iterations are queued in growing chunks; every kernel skips instances
+    // that have stopped, so the host only looks at the states between chunks.
+    auto poll = [&](bool& run, bool& restart) {
+        HIPCHK(hipGetLastError());
+        HIPCHK(hipMemcpyAsync(hs, sts, sizeof(KState) * nstate, hipMemcpyDeviceToHost, st));
+        HIPCHK(hipStreamSynchronize(st));
+        run = restart = false;
+        for (int g = 0; g < G; ++g)
+            for (size_t k = 0; k < groups[g].inst.size(); ++k) {
+                const int s = hs[g * kMaxNR + k].status;
+                run |= s == ST_RUN || (s >= ST_UPDATE && s <= ST_RESCALE);
+                restart |= s == ST_RESTART;
+            }
+    };
+    int chunk = 4;
+    bool run = false, restart = false;
+    if (!refine) {
+        Launch<double, NR>::init(c64, V64.b, V64.x, V64.r, V64.r0, V64.p, V64.v);
         for (;;) {
-            for (int c = 0; c < chunk; ++c) Launch<T, NR>::iteration(cfg, V);
-            HIPCHK(hipGetLastError());
-            HIPCHK(hipMemcpyAsync(hs, dst, sizeof(KState) * nstate, hipMemcpyDeviceToHost, st));
-            HIPCHK(hipStreamSynchronize(st));
-            bool run = false, rst = false;
-            for (int g = 0; g < G; ++g)
-                for (size_t k = 0; k < groups[g].inst.size(); ++k) {
-                    const int s = hs[g * kMaxNR + k].status;
-                    run |= s == ST_RUN;
-                    rst |= s == ST_RESTART;
-                }
-            if (rst) {
-                Launch<T, NR>::restart(cfg, V);
+            for (int c = 0; c < chunk; ++c) Launch<double, NR>::iteration(c64, V64, false);
+            poll(run, restart);
+            if (restart) {   // rare: handled between chunks, the other instances wait
+                Launch<double, NR>::restart(c64, V64);
                 continue;
             }
             if (!run) break;
             chunk = std::min(chunk * 2, 64);
         }
-    };
-
-    if (!refine) {
-        launch_reset_states(st64, nstate, o.tol * o.tol, o.max_iters, st);
-        krylov(double(0), st64, c64, V64);
-        hipLaunchKernelGGL(k_pack, dim3((unsigned)pb.size()), dim3(256), 0, st, dpe, dpb, V64.x, (float*)nullptr, 0);
-        HIPCHK(hipMemcpyAsync(hs, st64, sizeof(KState) * nstate, hipMemcpyDeviceToHost, st));
-        HIPCHK(hipStreamSynchronize(st));
-        for (int g = 0; g < G; ++g)
-            for (size_t k = 0; k < groups[g].inst.size(); ++k) {
-                const KState& s = hs[g * kMaxNR + k];
-                cpm_solve_info& inf = infos[groups[g].inst[k]];
-                inf.status = s.status;
-                inf.iters = s.iters;
-                inf.restarts = s.restarts;
-                inf.refinements = 0;
-                inf.rel_residual = s.bsq > 0 ? std::sqrt(s.rr / s.bsq) : 0.0;
+    } else {
+        double* x64 = V64.x;
+        double* r64 = V64.r;
+        MixedLaunch<NR>::start(c64, V64.b, x64, r64);
+        MixedLaunch<NR>::down(c64, r64, V32);
+        for (;;) {
+            for (int c = 0; c < chunk; ++c) {
+                Launch<float, NR>::iteration(c32, V32, true);
+                MixedLaunch<NR>::update(c64, V64.b, x64, r64, V64.t, V64.tmp, V32);   // skips lanes that did not ask
             }
-        return;
+            poll(run, restart);
+            if (!run) break;
+            chunk = std::min(chunk * 2, 64);
+        }
     }
-
-    // FP32 BiCGStab inside FP64 iterative refinement.  Outer state lives in
-    // st64 (max_it = refinement cap), inner state in st32.
-    double* x64 = V64.x;     // solution
-    double* r64 = V64.r;     // refinement residual
-    double* ax = V64.t;      // A64 x64
-    double* tmp = V64.tmp;
-    const double inner_tol = o.inner_tol > 0 ? o.inner_tol : 1e-4;
-    const int max_ref = o.max_refinements > 0 ? o.max_refinements : 40;
-    launch_reset_states(st64, nstate, o.tol * o.tol, max_ref, st);
-    RefLaunch<NR>::start(c64, V64.b, x64, r64);
-    for (;;) {
-        HIPCHK(hipMemcpyAsync(hs, st64, sizeof(KState) * nstate, hipMemcpyDeviceToHost, st));
-        HIPCHK(hipStreamSynchronize(st));
-        bool run = false;
-        for (int g = 0; g < G; ++g)
-            for (size_t k = 0; k < groups[g].inst.size(); ++k) run |= hs[g * kMaxNR + k].outer_status == ST_RUN;
-        if (!run) break;
-        RefLaunch<NR>::down(c64, r64, V32.b);
-        launch_reset_states(st32, nstate, inner_tol * inner_tol, o.max_iters, st);
-        krylov(float(0), st32, c32, V32);
-        RefLaunch<NR>::up(c64, V32.x, x64);
-        Launch<double, NR>::apply(c64, x64, tmp, ax);
-        RefLaunch<NR>::resid(c64, V64.b, ax, r64, st32);
-    }
-    hipLaunchKernelGGL(k_pack, dim3((unsigned)pb.size()), dim3(256), 0, st, dpe, dpb, x64, (float*)nullptr, 0);
+    hipLaunchKernelGGL(k_pack, dim3((unsigned)pb.size()), dim3(256), 0, st, dpe, dpb, V64.x, (float*)nullptr, 0);
+    HIPCHK(hipGetLastError());
     HIPCHK(hipStreamSynchronize(st));
     for (int g = 0; g < G; ++g)
         for (size_t k = 0; k < groups[g].inst.size(); ++k) {
             const KState& s = hs[g * kMaxNR + k];
             cpm_solve_info& inf = infos[groups[g].inst[k]];
-            inf.status = s.outer_status;
-            inf.iters = s.total_iters;
+            inf.status = s.status;
+            inf.iters = s.iters;
             inf.restarts = s.restarts;
-            inf.refinements = s.refinements;
-            inf.rel_residual = s.bsq > 0 ? std::sqrt(s.rr / s.bsq) : 0.0;
+            inf.updates = refine ? s.updates : 0;
+            const double rr = refine ? s.rr_true : s.rr;
+            inf.rel_residual = s.bsq > 0 ? std::sqrt(rr / s.bsq) : 0.0;
         }
 }
 
@@ -424,7 +394,7 @@ void multiply(Op* op, const double* dx, double* dy, double m, double n) {
     char* tb = reinterpret_cast<char*>(ctx->workspace("mul_tables", 4096 + sizeof(int) * ((op->D + 255) / 256)));
     Seg<double> h{};
     const int nblocks = (int)((op->D + kBlock - 1) / kBlock);
-    fill_seg(h, op, 0, 0, nblocks, 1, nullptr, nullptr, nullptr);
+    fill_seg(h, op, 0, 0, nblocks, 1, nullptr, nullptr);
     h.m = m;
     h.n = n;
     std::vector<int> hb(nblocks, 0);
@@ -433,7 +403,7 @@ void multiply(Op* op, const double* dx, double* dy, double m, double n) {
     HIPCHK(hipMemcpyAsync(dseg, &h, sizeof(h), hipMemcpyHostToDevice, st));
     HIPCHK(hipMemcpyAsync(dbseg, hb.data(), sizeof(int) * nblocks, hipMemcpyHostToDevice, st));
     double* tmp = ctx->workspace("mul_tmp", sizeof(double) * op->D);
-    LaunchCfg<double> c{dseg, dbseg, nblocks, st};
+    LaunchCfg<double> c{dseg, dbseg, nblocks, 1, st};
     Launch<double, 1>::apply(c, dx, tmp, dy);
     HIPCHK(hipGetLastError());
 }
@@ -444,7 +414,7 @@ static void profile_egather_t(Op* op, int reps, double* avg_ms, double* bytes) {
     const hipStream_t st = ctx->stream;
     const int nblocks = (int)((op->D + kBlock - 1) / kBlock);
     Seg<T> h{};
-    fill_seg(h, op, 0, 0, nblocks, NR, nullptr, nullptr, nullptr);
+    fill_seg(h, op, 0, 0, nblocks, NR, nullptr, nullptr);
     char* tb = reinterpret_cast<char*>(ctx->workspace("prof_tables", 4096 + sizeof(int) * nblocks));
     auto* dseg = reinterpret_cast<Seg<T>*>(tb);
     int* dbseg = reinterpret_cast<int*>(tb + 4096);
@@ -454,7 +424,7 @@ static void profile_egather_t(Op* op, int reps, double* avg_ms, double* bytes) {
     T* x = reinterpret_cast<T*>(ctx->workspace("prof_x", sizeof(T) * op->D * NR));
     T* y = reinterpret_cast<T*>(ctx->workspace("prof_y", sizeof(T) * op->D * NR));
     HIPCHK(hipMemsetAsync(x, 0, sizeof(T) * op->D * NR, st));
-    LaunchCfg<T> c{dseg, dbseg, nblocks, st};
+    LaunchCfg<T> c{dseg, dbseg, nblocks, 1, st};
     launch_egather<T, NR>(c, x, y);   // warm-up
     hipEvent_t e0, e1;
     HIPCHK(hipEventCreate(&e0));
@@ -531,8 +501,8 @@ void cpm_solve_opts_default(cpm_solve_opts* o) {
     o->tol = 1e-10;
     o->max_iters = 100000;
     o->precision = CPM_FP64;
-    o->inner_tol = 1e-4;
-    o->max_refinements = 40;
+    o->update_delta = 1e-3;
+    o->reserved = 0;
 }
 
 const char* cpm_last_error(void) { return g_err.c_str(); }

@@ -16,16 +16,24 @@ constexpr int kMaxNR = 8;            // right-hand sides per multi-RHS sweep
 // Row flags (one byte per row).
 enum : uint8_t { ROW_IDENTITY = 1, ROW_NODIAG = 2 };
 
-// Per (instance, rhs) Krylov state, written only by the finaliser block of a
-// reducing kernel and read (uniformly) by every block of the next kernels.
-enum : int { ST_RUN = 0, ST_CONVERGED = 1, ST_MAXIT = 2, ST_RESTART = 3, ST_ZERO_RHS = 4, ST_BREAKDOWN = 5 };
+// Per (instance, rhs) Krylov state.  Written only by the one-block finalise
+// kernels (krylov.hip) and read uniformly by every block of the next kernels.
+enum : int {
+    ST_RUN = 0, ST_CONVERGED = 1, ST_MAXIT = 2, ST_RESTART = 3, ST_ZERO_RHS = 4, ST_BREAKDOWN = 5,
+    // mixed precision: the FP32 recurrence asks for an FP64 refinement step,
+    // after which the FP32 vectors are rebuilt from the rescaled true residual
+    ST_UPDATE = 6, ST_UPDATE_MAXIT = 7, ST_RESCALE = 8
+};
 
 struct KState {
     double rho, rho_old, alpha, omega, beta;
     double bsq, r0sq, tol2, rr;
-    double scale;        // refinement: |r64| of the current correction solve
+    double rr_true;      // mixed: ||b - A x64||^2 at the last refinement step
+    double scale;        // mixed: 1 / ||b - A x64|| (FP32 vectors are scaled by it)
+    double maxrr;        // mixed: largest scaled recurrence residual^2 since the last step
+    double delta2;       // mixed: refine when rr < delta2 * maxrr
     int iters, restarts, status, max_it;
-    int outer_status, refinements, total_iters, pad;
+    int updates, pad0, pad1, pad2;
 };
 
 // Device view of an operator in one precision.  E and Loff are ELL matrices
@@ -55,8 +63,7 @@ struct Seg {
     int nr;              // rhs in this group (<= template NR; extra lanes idle)
     int precond;         // CPM_PRECOND_*
     KState* st;          // nr states
-    double* part;        // nblocks * NR * 2 partial sums
-    unsigned* cnt;       // arrival counter for the finaliser
+    double* part;        // nblocks * 2 * NR partial sums of the last reducing kernel
 };
 
 void hip_check(hipError_t e, const char* what);

@@ -2,23 +2,41 @@
 //
 // Restates the reference's right-preconditioned BiCGStab (src/BiCGStab.h:30-108)
 // and its matrix-free operator (src/GeneralizedBiCGStabSystem.h:191-261) as
-// seven bandwidth-bound kernels per iteration:
+// bandwidth-bound kernels; one iteration is
 //
 //   pupdate   p = (p - w v) beta + r ; y = P^-1 p                 (:268-271)
 //   egather   tmp = E y          (identity rows copy)              (:235-261)
 //   lfinish   v = m y + n (Ld y + Loff tmp) ; partial (r0, v)       (:93-115,:166-189)
+//   fin       alpha = rho / (r0, v)                                (:273)
 //   supdate   s = r - alpha v ; z = P^-1 s                         (:275-277)
 //   egather   tmp = E z
 //   lfinish   t = m z + n (Ld z + Loff tmp) ; partials (t,t),(t,s)  (:278-288)
-//   xrupdate  x += alpha y ; x += w z ; r = s - w t ; (r,r),(r0,r)  (:290-292, loop test :253, rho :255)
+//   fin       omega = (t, s) / (t, t)
+//   xrupdate  x += alpha y ; x += w z ; r = s - w t ; (r,r),(r0,r)  (:290-292)
+//   fin       loop test (:253), rho (:255), restart test (:256), beta (:267)
 //
-// Every scalar of the recurrence (rho, alpha, omega, beta, the loop test and
-// the rho-restart test) is computed on the device by the last workgroup of the
-// reducing kernel ("finaliser"), so iterations run without host round trips.
+// Reductions: every workgroup writes its partial sums; a one-workgroup-per-
+// instance "fin" kernel adds them in a fixed order and runs the scalar part
+// of the recurrence on the device, so iterations need no host round trip and
+// results are reproducible run to run.  (No device-scope fences: on the
+// multi-XCD MI355X a release/acquire at agent scope writes back / invalidates
+// L2 and costs more than the whole kernel.)
+//
 // One launch covers every instance of a batch: workgroups map to (instance,
-// row-block) through a segment table; converged instances are skipped.
+// row block) through a segment table; finished instances are skipped.
 // Right-hand sides of one operator are interleaved [row][rhs] so one read of
-// E's ELL slot serves all of them.
+// an E slot serves all of them.
+//
+// Precision CPM_FP32_REFINE: iterative refinement with FP32 BiCGStab inner
+// solves, run as one continuous device-side recurrence.  The solution is
+// accumulated in FP64; when the FP32 recurrence residual has dropped by
+// delta (default 1e-3) since the last step, or meets the tolerance, or
+// hits the rho-restart rule, x64 += x32 and the FP32 state is rebuilt from
+// the true FP64 residual b - A64 x64 rescaled to unit norm (r0 = r, p = v =
+// 0).  Convergence is declared on the true FP64 residual.  (Keeping p across
+// steps, as in reliable-update schemes for one operator, diverges here once
+// the residual approaches the FP32 operator's own rounding: the FP32 and
+// FP64 operators differ in the last bits.)
 #include <hip/hip_runtime.h>
 
 #include <cmath>
@@ -53,20 +71,16 @@ __device__ __forceinline__ double wave_sum(double v) {
     return v;
 }
 
-// Reduce NV*NR per-thread doubles over the workgroup, publish the block's
-// partials and let the last-arriving workgroup of the segment sum all of them
-// in a fixed order.  Returns true in every thread of the finaliser block, with
-// the totals in tot[] (valid in all threads).
-template <typename T, int NR, int NV>
-__device__ bool reduce_arrive(double (&v)[NV][NR], const Seg<T>& sg, int lb, double (&tot)[NV][NR]) {
+// Workgroup sums of NV*NR per-thread values, written to this block's partials.
+template <int NV, int NR>
+__device__ void block_partials(const double (&v)[NV][NR], double* part_block) {
     __shared__ double red[kBlock / 64][NV * NR];
-    __shared__ int is_last;
     const int lanei = threadIdx.x & 63, wave = threadIdx.x >> 6;
 #pragma unroll
     for (int a = 0; a < NV; ++a)
 #pragma unroll
         for (int k = 0; k < NR; ++k) {
-            double w = wave_sum(v[a][k]);
+            const double w = wave_sum(v[a][k]);
             if (lanei == 0) red[wave][a * NR + k] = w;
         }
     __syncthreads();
@@ -74,50 +88,8 @@ __device__ bool reduce_arrive(double (&v)[NV][NR], const Seg<T>& sg, int lb, dou
         double s = 0.0;
 #pragma unroll
         for (int w = 0; w < kBlock / 64; ++w) s += red[w][threadIdx.x];
-        sg.part[(size_t)lb * NV * NR + threadIdx.x] = s;
+        part_block[threadIdx.x] = s;
     }
-    __threadfence();
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        unsigned prev = atomicAdd(sg.cnt, 1u);
-        is_last = (prev == (unsigned)sg.nblocks - 1u);
-    }
-    __syncthreads();
-    if (!is_last) return false;
-    __threadfence();
-    // fixed-order sum of the partials: thread j takes blocks j, j+256, ...
-    double acc[NV][NR];
-#pragma unroll
-    for (int a = 0; a < NV; ++a)
-#pragma unroll
-        for (int k = 0; k < NR; ++k) acc[a][k] = 0.0;
-    for (int b = threadIdx.x; b < sg.nblocks; b += kBlock)
-#pragma unroll
-        for (int a = 0; a < NV; ++a)
-#pragma unroll
-            for (int k = 0; k < NR; ++k)
-                acc[a][k] += __hip_atomic_load(sg.part + (size_t)b * NV * NR + a * NR + k, __ATOMIC_RELAXED,
-                                               __HIP_MEMORY_SCOPE_AGENT);
-    __syncthreads();
-#pragma unroll
-    for (int a = 0; a < NV; ++a)
-#pragma unroll
-        for (int k = 0; k < NR; ++k) {
-            double w = wave_sum(acc[a][k]);
-            if (lanei == 0) red[wave][a * NR + k] = w;
-        }
-    __syncthreads();
-#pragma unroll
-    for (int a = 0; a < NV; ++a)
-#pragma unroll
-        for (int k = 0; k < NR; ++k) {
-            double s = 0.0;
-#pragma unroll
-            for (int w = 0; w < kBlock / 64; ++w) s += red[w][a * NR + k];
-            tot[a][k] = s;
-        }
-    if (threadIdx.x == 0) atomicExch(sg.cnt, 0u);
-    return true;
 }
 
 template <typename T>
@@ -128,11 +100,27 @@ __device__ __forceinline__ const Seg<T>& seg_of(const Seg<T>* segs, const int* b
     return sg;
 }
 
-template <typename T, int NR>
-__device__ __forceinline__ bool any_running(const Seg<T>& sg) {
-    bool run = false;
-    for (int k = 0; k < sg.nr; ++k) run |= (sg.st[k].status == ST_RUN);
-    return run;
+// Which instances a reducing kernel / its finaliser act on.
+enum Act : int { ACT_ALL = 0, ACT_RUN = 1, ACT_RESTART = 2, ACT_UPDATE = 3, ACT_RESCALE = 4 };
+
+template <typename T>
+__device__ __forceinline__ bool lane_active(const Seg<T>& sg, int k, int act) {
+    if (k >= sg.nr) return false;
+    const int s = sg.st[k].status;
+    switch (act) {
+    case ACT_RUN: return s == ST_RUN;
+    case ACT_RESTART: return s == ST_RESTART;
+    case ACT_UPDATE: return s == ST_UPDATE || s == ST_UPDATE_MAXIT;
+    case ACT_RESCALE: return s == ST_RESCALE;
+    default: return true;
+    }
+}
+
+template <typename T>
+__device__ __forceinline__ bool seg_active(const Seg<T>& sg, int act) {
+    bool a = false;
+    for (int k = 0; k < sg.nr; ++k) a |= lane_active(sg, k, act);
+    return a;
 }
 
 // Diagonal preconditioner, GeneralizedBiCGStabSystem.h:303-338 (the branch the
@@ -147,7 +135,7 @@ __device__ __forceinline__ T precond_apply(const Seg<T>& sg, int64_t row, uint8_
     return val / ((T)sg.m + sg.op.ldiag[row] * (T)sg.n);
 }
 
-// ---------------------------------------------------------------- kernels
+// ---------------------------------------------------------------- vector kernels
 template <typename T, int NR>
 __global__ __launch_bounds__(kBlock) void k_init(const Seg<T>* segs, const int* bseg, const T* b, T* x, T* r,
                                                  T* r0, T* p, T* v) {
@@ -169,29 +157,7 @@ __global__ __launch_bounds__(kBlock) void k_init(const Seg<T>* segs, const int* 
 #pragma unroll
         for (int k = 0; k < NR; ++k) pv[0][k] = (double)lane(bv, k) * (double)lane(bv, k);
     }
-    double tot[1][NR];
-    if (reduce_arrive<T, NR, 1>(pv, sg, lb, tot) && threadIdx.x == 0) {
-        for (int k = 0; k < sg.nr; ++k) {
-            KState& s = sg.st[k];
-            const double bsq = tot[0][k];
-            s.bsq = bsq;
-            s.r0sq = bsq;
-            s.rr = bsq;
-            s.rho_old = 1.0;
-            s.alpha = 1.0;
-            s.omega = 1.0;
-            s.iters = 0;
-            s.restarts = 0;
-            if (std::isnan(bsq) || std::isinf(bsq)) { s.status = ST_BREAKDOWN; continue; }
-            if (bsq == 0.0) { s.status = ST_ZERO_RHS; continue; }
-            if (!(bsq > s.tol2 * bsq)) { s.status = ST_CONVERGED; continue; }  // tol >= 1
-            if (s.max_it <= 0) { s.status = ST_MAXIT; continue; }
-            s.tol2 = s.tol2 * bsq;   // tol2 held tol^2 until now
-            s.rho = bsq;             // (r0, r) with r0 = r = b
-            s.beta = s.rho;          // (rho / 1) * (1 / 1)
-            s.status = ST_RUN;
-        }
-    }
+    block_partials<1, NR>(pv, sg.part + (size_t)lb * NR);
 }
 
 template <typename T, int NR>
@@ -200,7 +166,7 @@ __global__ __launch_bounds__(kBlock) void k_pupdate(const Seg<T>* segs, const in
     int lb;
     int64_t row;
     const Seg<T>& sg = seg_of(segs, bseg, lb, row);
-    if (row >= sg.op.D || !any_running<T, NR>(sg)) return;
+    if (row >= sg.op.D || !seg_active(sg, ACT_RUN)) return;
     const size_t i = (size_t)(sg.off + row * NR);
     const uint8_t f = sg.op.flag[row];
     auto pv = vload<T, NR>(p + i);
@@ -209,8 +175,10 @@ __global__ __launch_bounds__(kBlock) void k_pupdate(const Seg<T>* segs, const in
     typename VT<T, NR>::type yv;
 #pragma unroll
     for (int k = 0; k < NR; ++k) {
-        const T beta = (T)sg.st[k < sg.nr ? k : 0].beta, w = (T)sg.st[k < sg.nr ? k : 0].omega;
+        const KState& st = sg.st[k < sg.nr ? k : 0];
+        const T beta = (T)st.beta, w = (T)st.omega;
         T pk = (-w * lane(vv, k) + lane(pv, k)) * beta + lane(rv, k);   // p.Copy(-w,v,p); p*=beta; p+=r
+        if (k >= sg.nr || st.status != ST_RUN) pk = lane(pv, k);
         setlane(pv, k, pk);
         setlane(yv, k, precond_apply(sg, row, f, pk));
     }
@@ -219,14 +187,15 @@ __global__ __launch_bounds__(kBlock) void k_pupdate(const Seg<T>* segs, const in
 }
 
 // tmp = E x.  Rows are independent; a thread owns one row and all NR rhs.
+// act >= 0: skip instances with no lane in that state (uniform per segment).
 template <typename T, int NR>
 __global__ __launch_bounds__(kBlock) void k_egather(const Seg<T>* segs, const int* bseg, const T* x, T* out,
-                                                    int skip_done) {
+                                                    int act) {
     int lb;
     int64_t row;
     const Seg<T>& sg = seg_of(segs, bseg, lb, row);
     if (row >= sg.op.D) return;
-    if (skip_done && !any_running<T, NR>(sg)) return;
+    if (act >= 0 && !seg_active(sg, act)) return;
     const size_t i = (size_t)(sg.off + row * NR);
     typename VT<T, NR>::type acc = (typename VT<T, NR>::type)(T)0;
     if (sg.op.flag[row] & ROW_IDENTITY) {
@@ -244,10 +213,7 @@ __global__ __launch_bounds__(kBlock) void k_egather(const Seg<T>* segs, const in
 #pragma unroll
             for (int u = 0; u < 8; ++u) { c[u] = col[(int64_t)(s + u) * D]; w[u] = val[(int64_t)(s + u) * D]; }
 #pragma unroll
-            for (int u = 0; u < 8; ++u) {
-                auto xv = vload<T, NR>(xb + (size_t)c[u] * NR);
-                acc += w[u] * xv;
-            }
+            for (int u = 0; u < 8; ++u) acc += w[u] * vload<T, NR>(xb + (size_t)c[u] * NR);
         }
         for (; s < ew; ++s) {
             const int c = col[(int64_t)s * D];
@@ -258,15 +224,16 @@ __global__ __launch_bounds__(kBlock) void k_egather(const Seg<T>* segs, const in
 }
 
 // out = m x + n (Ld x + Loff tmp); identity rows out = x; trailing nD rows
-// (m - n) x.  RED = 0: no reduction; 1: (r0, out) -> alpha; 2: (out,out),
-// (out, s) -> omega.  Returns early for finished instances only when RED > 0.
+// (m - n) x.  RED = 0: no reduction; 1: partial (other, out); 2: partials
+// (out,out), (out, other).  Reducing variants skip finished instances.
 template <typename T, int NR, int RED>
 __global__ __launch_bounds__(kBlock) void k_lfinish(const Seg<T>* segs, const int* bseg, const T* x,
-                                                    const T* tmp, T* out, const T* other) {
+                                                    const T* tmp, T* out, const T* other, int act) {
     int lb;
     int64_t row;
     const Seg<T>& sg = seg_of(segs, bseg, lb, row);
-    if (RED > 0 && !any_running<T, NR>(sg)) return;   // uniform per segment: every block of it leaves
+    if (RED > 0) act = ACT_RUN;
+    if (act >= 0 && !seg_active(sg, act)) return;   // uniform per segment
     constexpr int NV = RED == 2 ? 2 : 1;
     double pv[NV][NR];
 #pragma unroll
@@ -297,30 +264,17 @@ __global__ __launch_bounds__(kBlock) void k_lfinish(const Seg<T>* segs, const in
             auto ov = vload<T, NR>(other + i);
 #pragma unroll
             for (int k = 0; k < NR; ++k) {
+                const double rk = (double)lane(res, k);
                 if (RED == 1) {
-                    pv[0][k] = (double)(lane(ov, k) * lane(res, k));
+                    pv[0][k] = (double)lane(ov, k) * rk;
                 } else {
-                    pv[0][k] = (double)(lane(res, k) * lane(res, k));
-                    if (NV == 2) pv[NV - 1][k] = (double)(lane(res, k) * lane(ov, k));
+                    pv[0][k] = rk * rk;
+                    pv[NV - 1][k] = rk * (double)lane(ov, k);
                 }
             }
         }
     }
-    if (RED == 0) return;
-    double tot[NV][NR];
-    if (reduce_arrive<T, NR, NV>(pv, sg, lb, tot) && threadIdx.x == 0) {
-        for (int k = 0; k < sg.nr; ++k) {
-            KState& s = sg.st[k];
-            if (s.status != ST_RUN) continue;
-            if (RED == 1) {
-                s.alpha = s.rho / tot[0][k];                       // BiCGStab.h:273
-            } else {
-                const double val = tot[0][k];
-                s.omega = val > 0.0 ? tot[NV - 1][k] / val : 0.0;  // :280-288
-            }
-            if (!std::isfinite(RED == 1 ? s.alpha : s.omega)) s.status = ST_BREAKDOWN;
-        }
-    }
+    if (RED > 0) block_partials<NV, NR>(pv, sg.part + (size_t)lb * NV * NR);
 }
 
 template <typename T, int NR>
@@ -329,7 +283,7 @@ __global__ __launch_bounds__(kBlock) void k_supdate(const Seg<T>* segs, const in
     int lb;
     int64_t row;
     const Seg<T>& sg = seg_of(segs, bseg, lb, row);
-    if (row >= sg.op.D || !any_running<T, NR>(sg)) return;
+    if (row >= sg.op.D || !seg_active(sg, ACT_RUN)) return;
     const size_t i = (size_t)(sg.off + row * NR);
     const uint8_t f = sg.op.flag[row];
     auto rv = vload<T, NR>(r + i);
@@ -352,7 +306,7 @@ __global__ __launch_bounds__(kBlock) void k_xrupdate(const Seg<T>* segs, const i
     int lb;
     int64_t row;
     const Seg<T>& sg = seg_of(segs, bseg, lb, row);
-    if (!any_running<T, NR>(sg)) return;
+    if (!seg_active(sg, ACT_RUN)) return;
     double pv[2][NR];
 #pragma unroll
     for (int k = 0; k < NR; ++k) { pv[0][k] = 0.0; pv[1][k] = 0.0; }
@@ -364,58 +318,34 @@ __global__ __launch_bounds__(kBlock) void k_xrupdate(const Seg<T>* segs, const i
         auto sv = vload<T, NR>(s + i);
         auto tv = vload<T, NR>(t + i);
         auto r0v = vload<T, NR>(r0 + i);
-        typename VT<T, NR>::type rv;
+        auto rv = vload<T, NR>(r + i);
 #pragma unroll
         for (int k = 0; k < NR; ++k) {
-            const KState& st = sg.st[k < sg.nr ? k : 0];
+            if (!lane_active(sg, k, ACT_RUN)) continue;
+            const KState& st = sg.st[k];
             const T a = (T)st.alpha, w = (T)st.omega;
             T xk = a * lane(yv, k) + lane(xv, k);   // x.Copy(alpha, y, x)
             xk = w * lane(zv, k) + xk;               // x.Copy(w, z, x)
             T rk = -w * lane(tv, k) + lane(sv, k);   // r.Copy(-w, t, s)
-            if (st.status != ST_RUN) { xk = lane(xv, k); rk = (T)0; }
             setlane(xv, k, xk);
             setlane(rv, k, rk);
-            pv[0][k] = (double)(rk * rk);
-            pv[1][k] = (double)(lane(r0v, k) * rk);
+            pv[0][k] = (double)rk * (double)rk;
+            pv[1][k] = (double)lane(r0v, k) * (double)rk;
         }
         vstore<T, NR>(x + i, xv);
         vstore<T, NR>(r + i, rv);
     }
-    double tot[2][NR];
-    if (reduce_arrive<T, NR, 2>(pv, sg, lb, tot) && threadIdx.x == 0) {
-        for (int k = 0; k < sg.nr; ++k) {
-            KState& st = sg.st[k];
-            if (st.status != ST_RUN) continue;
-            ++st.iters;
-            const double rr = tot[0][k];
-            st.rr = rr;
-            // loop test of the next iteration, BiCGStab.h:253
-            if (!(rr > st.tol2)) { st.status = std::isnan(rr) ? ST_BREAKDOWN : ST_CONVERGED; continue; }
-            if (st.iters >= st.max_it) { st.status = ST_MAXIT; continue; }
-            st.rho_old = st.rho;
-            st.rho = tot[1][k];                                  // :255
-            const double eps = 2.220446049250313e-16;
-            if (std::fabs(st.rho) < eps * eps * st.r0sq) {       // :256
-                st.status = ST_RESTART;
-                continue;
-            }
-            st.beta = (st.rho / st.rho_old) * (st.alpha / st.omega);   // :267
-            if (!std::isfinite(st.beta)) st.status = ST_BREAKDOWN;
-        }
-    }
+    block_partials<2, NR>(pv, sg.part + (size_t)lb * 2 * NR);
 }
 
-// Restart, BiCGStab.h:257-264: r = b - A x (A x already in ax), r0 = r,
-// rho = r0sq = (r, r).
+// FP64 restart, BiCGStab.h:257-264: r = b - A x (A x already in ax), r0 = r; partial (r, r).
 template <typename T, int NR>
 __global__ __launch_bounds__(kBlock) void k_restart(const Seg<T>* segs, const int* bseg, const T* b,
                                                     const T* ax, T* r, T* r0) {
     int lb;
     int64_t row;
     const Seg<T>& sg = seg_of(segs, bseg, lb, row);
-    bool any = false;
-    for (int k = 0; k < sg.nr; ++k) any |= (sg.st[k].status == ST_RESTART);
-    if (!any) return;
+    if (!seg_active(sg, ACT_RESTART)) return;
     double pv[1][NR];
 #pragma unroll
     for (int k = 0; k < NR; ++k) pv[0][k] = 0.0;
@@ -427,37 +357,23 @@ __global__ __launch_bounds__(kBlock) void k_restart(const Seg<T>* segs, const in
         auto r0v = vload<T, NR>(r0 + i);
 #pragma unroll
         for (int k = 0; k < NR; ++k) {
-            if (k < sg.nr && sg.st[k].status == ST_RESTART) {
-                T rk = lane(bv, k) - lane(av, k);
-                setlane(rv, k, rk);
-                setlane(r0v, k, rk);
-                pv[0][k] = (double)(rk * rk);
-            }
+            if (!lane_active(sg, k, ACT_RESTART)) continue;
+            T rk = lane(bv, k) - lane(av, k);
+            setlane(rv, k, rk);
+            setlane(r0v, k, rk);
+            pv[0][k] = (double)rk * (double)rk;
         }
         vstore<T, NR>(r + i, rv);
         vstore<T, NR>(r0 + i, r0v);
     }
-    double tot[1][NR];
-    if (reduce_arrive<T, NR, 1>(pv, sg, lb, tot) && threadIdx.x == 0) {
-        for (int k = 0; k < sg.nr; ++k) {
-            KState& st = sg.st[k];
-            if (st.status != ST_RESTART) continue;
-            st.rho = st.r0sq = tot[0][k];
-            if (st.restarts++ == 0) st.iters = 0;
-            st.beta = (st.rho / st.rho_old) * (st.alpha / st.omega);
-            st.status = std::isfinite(st.beta) ? ST_RUN : ST_BREAKDOWN;
-        }
-    }
+    block_partials<1, NR>(pv, sg.part + (size_t)lb * NR);
 }
 
-// ---------------------------------------------------------------- refinement
-// FP64 iterative refinement around an FP32 BiCGStab (rows interleave NR rhs).
-//   start:   x64 = 0, r64 = b64, (b,b)
-//   step:    r32 = r64 / |r64|  ->  FP32 solve A d = r32  ->  x64 += |r64| d
-//            r64 = b64 - A64 x64 ; (r64, r64)
+// ---------------------------------------------------------------- mixed precision
+// x64 = 0, r64 = b ; partial (b, b)
 template <int NR>
-__global__ __launch_bounds__(kBlock) void k_ref_start(const Seg<double>* segs, const int* bseg, const double* b,
-                                                      double* x, double* r) {
+__global__ __launch_bounds__(kBlock) void k_mx_start(const Seg<double>* segs, const int* bseg, const double* b,
+                                                     double* x, double* r) {
     int lb;
     int64_t row;
     const Seg<double>& sg = seg_of(segs, bseg, lb, row);
@@ -472,72 +388,74 @@ __global__ __launch_bounds__(kBlock) void k_ref_start(const Seg<double>* segs, c
 #pragma unroll
         for (int k = 0; k < NR; ++k) pv[0][k] = lane(bv, k) * lane(bv, k);
     }
-    double tot[1][NR];
-    if (reduce_arrive<double, NR, 1>(pv, sg, lb, tot) && threadIdx.x == 0) {
-        for (int k = 0; k < sg.nr; ++k) {
-            KState& st = sg.st[k];
-            st.bsq = tot[0][k];
-            st.rr = st.bsq;
-            st.refinements = 0;
-            st.total_iters = 0;
-            st.scale = std::sqrt(st.bsq);
-            if (!std::isfinite(st.bsq)) st.outer_status = ST_BREAKDOWN;
-            else if (st.bsq == 0.0) st.outer_status = ST_ZERO_RHS;
-            else if (!(st.bsq > st.tol2 * st.bsq)) st.outer_status = ST_CONVERGED;
-            else { st.tol2 = st.tol2 * st.bsq; st.outer_status = ST_RUN; }
+    block_partials<1, NR>(pv, sg.part + (size_t)lb * NR);
+}
+
+// FP32 (re)start of the lanes in ST_RESCALE: r32 = r64 * scale, r0 = r32,
+// p = v = x32 = 0 ; partial (r32, r32).
+template <int NR>
+__global__ __launch_bounds__(kBlock) void k_mx_down(const Seg<double>* segs, const int* bseg, const double* r64,
+                                                    float* r, float* r0, float* p, float* v, float* x) {
+    int lb;
+    int64_t row;
+    const Seg<double>& sg = seg_of(segs, bseg, lb, row);
+    if (!seg_active(sg, ACT_RESCALE)) return;
+    double pv[1][NR];
+#pragma unroll
+    for (int k = 0; k < NR; ++k) pv[0][k] = 0.0;
+    if (row < sg.op.D) {
+        const size_t i = (size_t)(sg.off + row * NR);
+        auto rv = vload<double, NR>(r64 + i);
+        auto ro = vload<float, NR>(r + i);
+        auto r0o = vload<float, NR>(r0 + i);
+        auto po = vload<float, NR>(p + i);
+        auto vo = vload<float, NR>(v + i);
+        auto xo = vload<float, NR>(x + i);
+#pragma unroll
+        for (int k = 0; k < NR; ++k) {
+            if (!lane_active(sg, k, ACT_RESCALE)) continue;
+            const float rk = (float)(lane(rv, k) * sg.st[k].scale);
+            setlane(ro, k, rk);
+            setlane(r0o, k, rk);
+            setlane(po, k, 0.0f);
+            setlane(vo, k, 0.0f);
+            setlane(xo, k, 0.0f);
+            pv[0][k] = (double)rk * (double)rk;
         }
+        vstore<float, NR>(r + i, ro);
+        vstore<float, NR>(r0 + i, r0o);
+        vstore<float, NR>(p + i, po);
+        vstore<float, NR>(v + i, vo);
+        vstore<float, NR>(x + i, xo);
     }
+    block_partials<1, NR>(pv, sg.part + (size_t)lb * NR);
 }
 
-// b32 = r64 / scale for running instances (zero otherwise so the inner solve
-// of a finished instance sees a zero rhs and stops at once).
+// Refinement step, part 1: x64 += x32 / scale.
 template <int NR>
-__global__ __launch_bounds__(kBlock) void k_ref_down(const Seg<double>* segs, const int* bseg, const double* r,
-                                                     float* b32) {
+__global__ __launch_bounds__(kBlock) void k_mx_upx(const Seg<double>* segs, const int* bseg, double* x64,
+                                                   const float* x32) {
     int lb;
     int64_t row;
     const Seg<double>& sg = seg_of(segs, bseg, lb, row);
-    if (row >= sg.op.D) return;
+    if (row >= sg.op.D || !seg_active(sg, ACT_UPDATE)) return;
     const size_t i = (size_t)(sg.off + row * NR);
-    auto rv = vload<double, NR>(r + i);
-    typename VT<float, NR>::type o;
+    auto xv = vload<double, NR>(x64 + i);
+    auto dv = vload<float, NR>(x32 + i);
 #pragma unroll
-    for (int k = 0; k < NR; ++k) {
-        const KState& st = sg.st[k < sg.nr ? k : 0];
-        const bool run = k < sg.nr && st.outer_status == ST_RUN;
-        setlane(o, k, run ? (float)(lane(rv, k) / st.scale) : 0.0f);
-    }
-    vstore<float, NR>(b32 + i, o);
+    for (int k = 0; k < NR; ++k)
+        if (lane_active(sg, k, ACT_UPDATE)) setlane(xv, k, lane(xv, k) + (double)lane(dv, k) / sg.st[k].scale);
+    vstore<double, NR>(x64 + i, xv);
 }
 
+// Refinement step, part 2: r64 = b - A64 x64 (A64 x64 in ax) ; partial (r64, r64).
 template <int NR>
-__global__ __launch_bounds__(kBlock) void k_ref_up(const Seg<double>* segs, const int* bseg, const float* d32,
-                                                   double* x) {
+__global__ __launch_bounds__(kBlock) void k_mx_upr(const Seg<double>* segs, const int* bseg, const double* b,
+                                                   const double* ax, double* r64) {
     int lb;
     int64_t row;
     const Seg<double>& sg = seg_of(segs, bseg, lb, row);
-    if (row >= sg.op.D) return;
-    const size_t i = (size_t)(sg.off + row * NR);
-    auto dv = vload<float, NR>(d32 + i);
-    auto xv = vload<double, NR>(x + i);
-#pragma unroll
-    for (int k = 0; k < NR; ++k) {
-        const KState& st = sg.st[k < sg.nr ? k : 0];
-        if (k < sg.nr && st.outer_status == ST_RUN) setlane(xv, k, lane(xv, k) + st.scale * (double)lane(dv, k));
-    }
-    vstore<double, NR>(x + i, xv);
-}
-
-// r64 = b - ax ; (r64, r64) ; outer loop test.
-template <int NR>
-__global__ __launch_bounds__(kBlock) void k_ref_resid(const Seg<double>* segs, const int* bseg, const double* b,
-                                                      const double* ax, double* r, const KState* inner) {
-    int lb;
-    int64_t row;
-    const Seg<double>& sg = seg_of(segs, bseg, lb, row);
-    bool any = false;
-    for (int k = 0; k < sg.nr; ++k) any |= (sg.st[k].outer_status == ST_RUN);
-    if (!any) return;
+    if (!seg_active(sg, ACT_UPDATE)) return;
     double pv[1][NR];
 #pragma unroll
     for (int k = 0; k < NR; ++k) pv[0][k] = 0.0;
@@ -545,106 +463,241 @@ __global__ __launch_bounds__(kBlock) void k_ref_resid(const Seg<double>* segs, c
         const size_t i = (size_t)(sg.off + row * NR);
         auto bv = vload<double, NR>(b + i);
         auto av = vload<double, NR>(ax + i);
-        auto rv = vload<double, NR>(r + i);
+        auto rv = vload<double, NR>(r64 + i);
 #pragma unroll
-        for (int k = 0; k < NR; ++k)
-            if (k < sg.nr && sg.st[k].outer_status == ST_RUN) {
-                const double rk = lane(bv, k) - lane(av, k);
-                setlane(rv, k, rk);
-                pv[0][k] = rk * rk;
-            }
-        vstore<double, NR>(r + i, rv);
-    }
-    double tot[1][NR];
-    if (reduce_arrive<double, NR, 1>(pv, sg, lb, tot) && threadIdx.x == 0) {
-        const int seg_index = bseg[blockIdx.x];
-        for (int k = 0; k < sg.nr; ++k) {
-            KState& st = sg.st[k];
-            if (st.outer_status != ST_RUN) continue;
-            const KState& in = inner[seg_index * kMaxNR + k];
-            st.total_iters += in.iters;
-            st.restarts += in.restarts;
-            ++st.refinements;
-            st.rr = tot[0][k];
-            st.scale = std::sqrt(st.rr);
-            if (!(st.rr > st.tol2)) st.outer_status = std::isnan(st.rr) ? ST_BREAKDOWN : ST_CONVERGED;
-            else if (in.status == ST_BREAKDOWN) st.outer_status = ST_BREAKDOWN;
-            else if (st.refinements >= st.max_it) st.outer_status = ST_MAXIT;
+        for (int k = 0; k < NR; ++k) {
+            if (!lane_active(sg, k, ACT_UPDATE)) continue;
+            const double rk = lane(bv, k) - lane(av, k);
+            setlane(rv, k, rk);
+            pv[0][k] = rk * rk;
         }
+        vstore<double, NR>(r64 + i, rv);
+    }
+    block_partials<1, NR>(pv, sg.part + (size_t)lb * NR);
+}
+
+// ---------------------------------------------------------------- finalisers
+enum FinKind : int {
+    FIN_INIT = 0, FIN_ALPHA, FIN_OMEGA, FIN_ITER64, FIN_RESTART64,
+    FIN_MX_START, FIN_MX_INIT, FIN_ITER_MX, FIN_MX_UPDATE
+};
+
+template <int KIND> struct FinTraits;
+template <> struct FinTraits<FIN_INIT> { static constexpr int NV = 1, ACT = ACT_ALL; };
+template <> struct FinTraits<FIN_ALPHA> { static constexpr int NV = 1, ACT = ACT_RUN; };
+template <> struct FinTraits<FIN_OMEGA> { static constexpr int NV = 2, ACT = ACT_RUN; };
+template <> struct FinTraits<FIN_ITER64> { static constexpr int NV = 2, ACT = ACT_RUN; };
+template <> struct FinTraits<FIN_RESTART64> { static constexpr int NV = 1, ACT = ACT_RESTART; };
+template <> struct FinTraits<FIN_MX_START> { static constexpr int NV = 1, ACT = ACT_ALL; };
+template <> struct FinTraits<FIN_MX_INIT> { static constexpr int NV = 1, ACT = ACT_RESCALE; };
+template <> struct FinTraits<FIN_ITER_MX> { static constexpr int NV = 2, ACT = ACT_RUN; };
+template <> struct FinTraits<FIN_MX_UPDATE> { static constexpr int NV = 1, ACT = ACT_UPDATE; };
+
+constexpr double kEps = 2.220446049250313e-16;   // NumTraits<double>::epsilon (BiCGStab.h:250)
+
+// The scalar part of the recurrence for one (instance, rhs).
+template <int KIND>
+__device__ void fin_scalar(KState& s, const double* tot) {
+    if (KIND == FIN_INIT) {   // BiCGStab.h:235-252 with x0 = 0: r = r0 = b
+        const double bsq = tot[0];
+        s.bsq = s.r0sq = s.rr = bsq;
+        s.rho_old = s.alpha = s.omega = 1.0;
+        s.iters = s.restarts = 0;
+        if (!std::isfinite(bsq)) { s.status = ST_BREAKDOWN; return; }
+        if (bsq == 0.0) { s.status = ST_ZERO_RHS; return; }
+        if (!(bsq > s.tol2 * bsq)) { s.status = ST_CONVERGED; return; }   // tol >= 1
+        if (s.max_it <= 0) { s.status = ST_MAXIT; return; }
+        s.tol2 *= bsq;                 // tol2 held tol^2
+        s.rho = bsq;                   // (r0, r) with r0 = r = b
+        s.beta = s.rho;                // (rho / 1) * (1 / 1)
+        s.status = ST_RUN;
+    } else if (KIND == FIN_ALPHA) {
+        s.alpha = s.rho / tot[0];                                      // :273
+        if (!std::isfinite(s.alpha)) s.status = ST_BREAKDOWN;
+    } else if (KIND == FIN_OMEGA) {
+        s.omega = tot[0] > 0.0 ? tot[1] / tot[0] : 0.0;                // :280-288
+        if (!std::isfinite(s.omega)) s.status = ST_BREAKDOWN;
+    } else if (KIND == FIN_ITER64) {
+        ++s.iters;
+        s.rr = tot[0];
+        if (!(s.rr > s.tol2)) { s.status = std::isnan(s.rr) ? ST_BREAKDOWN : ST_CONVERGED; return; }   // :253
+        if (s.iters >= s.max_it) { s.status = ST_MAXIT; return; }
+        s.rho_old = s.rho;
+        s.rho = tot[1];                                                // :255
+        if (fabs(s.rho) < kEps * kEps * s.r0sq) { s.status = ST_RESTART; return; }   // :256
+        s.beta = (s.rho / s.rho_old) * (s.alpha / s.omega);           // :267
+        if (!std::isfinite(s.beta)) s.status = ST_BREAKDOWN;
+    } else if (KIND == FIN_RESTART64) {                                // :257-264
+        s.rho = s.r0sq = tot[0];
+        if (s.restarts++ == 0) s.iters = 0;
+        s.beta = (s.rho / s.rho_old) * (s.alpha / s.omega);
+        s.status = std::isfinite(s.beta) ? ST_RUN : ST_BREAKDOWN;
+    } else if (KIND == FIN_MX_START) {
+        const double bsq = tot[0];
+        s.bsq = s.rr_true = bsq;
+        s.iters = s.restarts = s.updates = 0;
+        if (!std::isfinite(bsq)) { s.status = ST_BREAKDOWN; return; }
+        if (bsq == 0.0) { s.status = ST_ZERO_RHS; return; }
+        if (!(bsq > s.tol2 * bsq)) { s.status = ST_CONVERGED; return; }
+        if (s.max_it <= 0) { s.status = ST_MAXIT; return; }
+        s.tol2 *= bsq;
+        s.scale = 1.0 / sqrt(bsq);
+        s.status = ST_RESCALE;
+    } else if (KIND == FIN_MX_INIT) {   // FP32 recurrence (re)started from the scaled residual
+        const double rr = tot[0];
+        s.rr = s.rho = s.r0sq = s.maxrr = rr;
+        s.rho_old = s.alpha = s.omega = 1.0;
+        s.beta = s.rho;
+        s.status = (rr > 0.0 && std::isfinite(rr)) ? ST_RUN : ST_BREAKDOWN;
+    } else if (KIND == FIN_ITER_MX) {
+        ++s.iters;
+        const double rr = tot[0];
+        s.rr = rr;
+        if (std::isnan(rr)) { s.status = ST_BREAKDOWN; return; }
+        // scaled tolerance: ||r|| <= tol ||b|| with r scaled by `scale`
+        if (!(rr > s.tol2 * s.scale * s.scale)) { s.status = ST_UPDATE; return; }
+        if (s.iters >= s.max_it) { s.status = ST_UPDATE_MAXIT; return; }
+        if (rr < s.delta2 * s.maxrr) { s.status = ST_UPDATE; return; }
+        s.rho_old = s.rho;
+        s.rho = tot[1];
+        if (fabs(s.rho) < kEps * kEps * s.r0sq) { ++s.restarts; s.status = ST_UPDATE; return; }   // BiCGStab.h:256
+        s.beta = (s.rho / s.rho_old) * (s.alpha / s.omega);
+        if (!std::isfinite(s.beta)) { s.status = ST_BREAKDOWN; return; }
+        if (rr > s.maxrr) s.maxrr = rr;
+    } else if (KIND == FIN_MX_UPDATE) {
+        ++s.updates;
+        const double rt = tot[0];
+        s.rr_true = rt;
+        if (std::isnan(rt)) { s.status = ST_BREAKDOWN; return; }
+        if (!(rt > s.tol2)) { s.status = ST_CONVERGED; return; }
+        if (s.status == ST_UPDATE_MAXIT) { s.status = ST_MAXIT; return; }
+        s.scale = 1.0 / sqrt(rt);
+        s.status = ST_RESCALE;
     }
 }
 
-__global__ void k_reset_states(KState* st, int n, double tol2, int max_it) {
+// One workgroup per segment: fixed-order sum of the producing kernel's
+// block partials, then the scalar recurrence for each active lane.
+template <typename T, int NR, int KIND>
+__global__ __launch_bounds__(kBlock) void k_fin(const Seg<T>* segs, int nseg) {
+    constexpr int NV = FinTraits<KIND>::NV;
+    const Seg<T>& sg = segs[blockIdx.x];
+    if (!seg_active(sg, FinTraits<KIND>::ACT)) return;   // same test the producer used
+    double acc[NV][NR];
+#pragma unroll
+    for (int a = 0; a < NV; ++a)
+#pragma unroll
+        for (int k = 0; k < NR; ++k) acc[a][k] = 0.0;
+    for (int b = threadIdx.x; b < sg.nblocks; b += kBlock)
+#pragma unroll
+        for (int a = 0; a < NV; ++a)
+#pragma unroll
+            for (int k = 0; k < NR; ++k) acc[a][k] += sg.part[(size_t)b * NV * NR + a * NR + k];
+    __shared__ double red[kBlock / 64][NV * NR];
+    const int lanei = threadIdx.x & 63, wave = threadIdx.x >> 6;
+#pragma unroll
+    for (int a = 0; a < NV; ++a)
+#pragma unroll
+        for (int k = 0; k < NR; ++k) {
+            const double w = wave_sum(acc[a][k]);
+            if (lanei == 0) red[wave][a * NR + k] = w;
+        }
+    __syncthreads();
+    if (threadIdx.x != 0) return;
+    for (int k = 0; k < sg.nr; ++k) {
+        if (!lane_active(sg, k, FinTraits<KIND>::ACT)) continue;
+        double tot[NV];
+        for (int a = 0; a < NV; ++a) {
+            double s = 0.0;
+            for (int w = 0; w < kBlock / 64; ++w) s += red[w][a * NR + k];
+            tot[a] = s;
+        }
+        fin_scalar<KIND>(sg.st[k], tot);
+    }
+}
+
+__global__ void k_reset_states(KState* st, int n, double tol2, int max_it, double delta2) {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     KState z = {};
     z.tol2 = tol2;
     z.max_it = max_it;
+    z.delta2 = delta2;
     z.status = ST_RUN;
-    z.outer_status = ST_RUN;
     st[i] = z;
 }
 
-void launch_reset_states(KState* st, int n, double tol2, int max_it, hipStream_t s) {
-    hipLaunchKernelGGL(k_reset_states, dim3((n + 255) / 256), dim3(256), 0, s, st, n, tol2, max_it);
+void launch_reset_states(KState* st, int n, double tol2, int max_it, double delta2, hipStream_t s) {
+    hipLaunchKernelGGL(k_reset_states, dim3((n + 255) / 256), dim3(256), 0, s, st, n, tol2, max_it, delta2);
 }
 
 // ---------------------------------------------------------------- launchers
+#define LAUNCH(kern, ...) hipLaunchKernelGGL(kern, dim3(c.nblocks), dim3(kBlock), 0, c.stream, c.segs, c.bseg, __VA_ARGS__)
+#define FIN(T, NR, KIND) hipLaunchKernelGGL((k_fin<T, NR, KIND>), dim3(c.nseg), dim3(kBlock), 0, c.stream, c.segs, c.nseg)
+
 template <typename T, int NR>
 void Launch<T, NR>::init(const LaunchCfg<T>& c, const T* b, T* x, T* r, T* r0, T* p, T* v) {
-    hipLaunchKernelGGL((k_init<T, NR>), dim3(c.nblocks), dim3(kBlock), 0, c.stream, c.segs, c.bseg, b, x, r, r0, p, v);
+    LAUNCH((k_init<T, NR>), b, x, r, r0, p, v);
+    FIN(T, NR, FIN_INIT);
 }
 template <typename T, int NR>
-void Launch<T, NR>::iteration(const LaunchCfg<T>& c, Vecs<T>& w) {
-    hipLaunchKernelGGL((k_pupdate<T, NR>), dim3(c.nblocks), dim3(kBlock), 0, c.stream, c.segs, c.bseg, w.p, w.v, w.r, w.y);
-    hipLaunchKernelGGL((k_egather<T, NR>), dim3(c.nblocks), dim3(kBlock), 0, c.stream, c.segs, c.bseg, w.y, w.tmp, 1);
-    hipLaunchKernelGGL((k_lfinish<T, NR, 1>), dim3(c.nblocks), dim3(kBlock), 0, c.stream, c.segs, c.bseg, w.y, w.tmp, w.v, w.r0);
-    hipLaunchKernelGGL((k_supdate<T, NR>), dim3(c.nblocks), dim3(kBlock), 0, c.stream, c.segs, c.bseg, w.r, w.v, w.s, w.z);
-    hipLaunchKernelGGL((k_egather<T, NR>), dim3(c.nblocks), dim3(kBlock), 0, c.stream, c.segs, c.bseg, w.z, w.tmp, 1);
-    hipLaunchKernelGGL((k_lfinish<T, NR, 2>), dim3(c.nblocks), dim3(kBlock), 0, c.stream, c.segs, c.bseg, w.z, w.tmp, w.t, w.s);
-    hipLaunchKernelGGL((k_xrupdate<T, NR>), dim3(c.nblocks), dim3(kBlock), 0, c.stream, c.segs, c.bseg, w.x, w.y, w.z, w.r, w.s, w.t, w.r0);
+void Launch<T, NR>::iteration(const LaunchCfg<T>& c, Vecs<T>& w, bool mixed) {
+    LAUNCH((k_pupdate<T, NR>), w.p, w.v, w.r, w.y);
+    LAUNCH((k_egather<T, NR>), w.y, w.tmp, (int)ACT_RUN);
+    LAUNCH((k_lfinish<T, NR, 1>), w.y, w.tmp, w.v, w.r0, (int)ACT_RUN);
+    FIN(T, NR, FIN_ALPHA);
+    LAUNCH((k_supdate<T, NR>), w.r, w.v, w.s, w.z);
+    LAUNCH((k_egather<T, NR>), w.z, w.tmp, (int)ACT_RUN);
+    LAUNCH((k_lfinish<T, NR, 2>), w.z, w.tmp, w.t, w.s, (int)ACT_RUN);
+    FIN(T, NR, FIN_OMEGA);
+    LAUNCH((k_xrupdate<T, NR>), w.x, w.y, w.z, w.r, w.s, w.t, w.r0);
+    if (mixed) FIN(T, NR, FIN_ITER_MX);
+    else FIN(T, NR, FIN_ITER64);
 }
 template <typename T, int NR>
 void Launch<T, NR>::restart(const LaunchCfg<T>& c, Vecs<T>& w) {
-    hipLaunchKernelGGL((k_egather<T, NR>), dim3(c.nblocks), dim3(kBlock), 0, c.stream, c.segs, c.bseg, w.x, w.tmp, 0);
-    hipLaunchKernelGGL((k_lfinish<T, NR, 0>), dim3(c.nblocks), dim3(kBlock), 0, c.stream, c.segs, c.bseg, w.x, w.tmp, w.t, (const T*)nullptr);
-    hipLaunchKernelGGL((k_restart<T, NR>), dim3(c.nblocks), dim3(kBlock), 0, c.stream, c.segs, c.bseg, w.b, w.t, w.r, w.r0);
+    LAUNCH((k_egather<T, NR>), w.x, w.tmp, (int)ACT_RESTART);
+    LAUNCH((k_lfinish<T, NR, 0>), w.x, w.tmp, w.t, (const T*)nullptr, (int)ACT_RESTART);
+    LAUNCH((k_restart<T, NR>), w.b, w.t, w.r, w.r0);
+    FIN(T, NR, FIN_RESTART64);
 }
 template <typename T, int NR>
-void Launch<T, NR>::apply(const LaunchCfg<T>& c, const T* x, T* tmp, T* y) {
-    hipLaunchKernelGGL((k_egather<T, NR>), dim3(c.nblocks), dim3(kBlock), 0, c.stream, c.segs, c.bseg, x, tmp, 0);
-    hipLaunchKernelGGL((k_lfinish<T, NR, 0>), dim3(c.nblocks), dim3(kBlock), 0, c.stream, c.segs, c.bseg, x, tmp, y, (const T*)nullptr);
-}
-
-template <int NR>
-void RefLaunch<NR>::start(const LaunchCfg<double>& c, const double* b, double* x, double* r) {
-    hipLaunchKernelGGL((k_ref_start<NR>), dim3(c.nblocks), dim3(kBlock), 0, c.stream, c.segs, c.bseg, b, x, r);
-}
-template <int NR>
-void RefLaunch<NR>::down(const LaunchCfg<double>& c, const double* r, float* b32) {
-    hipLaunchKernelGGL((k_ref_down<NR>), dim3(c.nblocks), dim3(kBlock), 0, c.stream, c.segs, c.bseg, r, b32);
-}
-template <int NR>
-void RefLaunch<NR>::up(const LaunchCfg<double>& c, const float* d32, double* x) {
-    hipLaunchKernelGGL((k_ref_up<NR>), dim3(c.nblocks), dim3(kBlock), 0, c.stream, c.segs, c.bseg, d32, x);
-}
-template <int NR>
-void RefLaunch<NR>::resid(const LaunchCfg<double>& c, const double* b, const double* ax, double* r,
-                          const KState* inner) {
-    hipLaunchKernelGGL((k_ref_resid<NR>), dim3(c.nblocks), dim3(kBlock), 0, c.stream, c.segs, c.bseg, b, ax, r, inner);
+void Launch<T, NR>::apply(const LaunchCfg<T>& c, const T* x, T* tmp, T* y, int act) {
+    LAUNCH((k_egather<T, NR>), x, tmp, act);
+    LAUNCH((k_lfinish<T, NR, 0>), x, tmp, y, (const T*)nullptr, act);
 }
 
 template <typename T, int NR>
 void launch_egather(const LaunchCfg<T>& c, const T* x, T* out) {
-    hipLaunchKernelGGL((k_egather<T, NR>), dim3(c.nblocks), dim3(kBlock), 0, c.stream, c.segs, c.bseg, x, out, 0);
+    LAUNCH((k_egather<T, NR>), x, out, -1);
+}
+
+template <int NR>
+void MixedLaunch<NR>::start(const LaunchCfg<double>& c, const double* b, double* x64, double* r64) {
+    LAUNCH((k_mx_start<NR>), b, x64, r64);
+    FIN(double, NR, FIN_MX_START);
+}
+template <int NR>
+void MixedLaunch<NR>::down(const LaunchCfg<double>& c, const double* r64, Vecs<float>& w) {
+    LAUNCH((k_mx_down<NR>), r64, w.r, w.r0, w.p, w.v, w.x);
+    FIN(double, NR, FIN_MX_INIT);
+}
+template <int NR>
+void MixedLaunch<NR>::update(const LaunchCfg<double>& c, const double* b, double* x64, double* r64, double* ax,
+                             double* tmp, Vecs<float>& w) {
+    LAUNCH((k_mx_upx<NR>), x64, (const float*)w.x);
+    Launch<double, NR>::apply(c, x64, tmp, ax, (int)ACT_UPDATE);
+    LAUNCH((k_mx_upr<NR>), b, ax, r64);
+    FIN(double, NR, FIN_MX_UPDATE);
+    down(c, r64, w);
 }
 
 #define CPM_INST(T, NR) template struct Launch<T, NR>; template void launch_egather<T, NR>(const LaunchCfg<T>&, const T*, T*);
 CPM_INST(double, 1) CPM_INST(double, 2) CPM_INST(double, 4) CPM_INST(double, 8)
 CPM_INST(float, 1) CPM_INST(float, 2) CPM_INST(float, 4) CPM_INST(float, 8)
-template struct RefLaunch<1>;
-template struct RefLaunch<2>;
-template struct RefLaunch<4>;
-template struct RefLaunch<8>;
+template struct MixedLaunch<1>;
+template struct MixedLaunch<2>;
+template struct MixedLaunch<4>;
+template struct MixedLaunch<8>;
 
 }  // namespace cpmgpu

@@ -31,16 +31,16 @@ EXPORTS = [
 
 class SolveOpts(ctypes.Structure):
     _fields_ = [("tol", ctypes.c_double), ("max_iters", ctypes.c_int), ("precision", ctypes.c_int),
-                ("inner_tol", ctypes.c_double), ("max_refinements", ctypes.c_int)]
+                ("update_delta", ctypes.c_double), ("reserved", ctypes.c_int)]
 
 
 class SolveInfo(ctypes.Structure):
     _fields_ = [("status", ctypes.c_int), ("iters", ctypes.c_int), ("restarts", ctypes.c_int),
-                ("refinements", ctypes.c_int), ("rel_residual", ctypes.c_double)]
+                ("updates", ctypes.c_int), ("rel_residual", ctypes.c_double)]
 
     def as_dict(self):
         return dict(status=STATUS.get(self.status, self.status), iters=self.iters, restarts=self.restarts,
-                    refinements=self.refinements, rel_residual=self.rel_residual)
+                    updates=self.updates, rel_residual=self.rel_residual)
 
 
 class TubeInfo(ctypes.Structure):
@@ -148,10 +148,10 @@ def _ptr(a):
     return a.ctypes.data_as(vp)
 
 
-def opts(tol=1e-10, max_iters=100000, precision=CPM_FP64, inner_tol=1e-4, max_refinements=40):
+def opts(tol=1e-10, max_iters=100000, precision=CPM_FP64, update_delta=1e-3):
     o = SolveOpts()
     load().cpm_solve_opts_default(ctypes.byref(o))
-    o.tol, o.max_iters, o.precision, o.inner_tol, o.max_refinements = tol, max_iters, precision, inner_tol, max_refinements
+    o.tol, o.max_iters, o.precision, o.update_delta, o.reserved = tol, max_iters, precision, update_delta, 0
     return o
 
 

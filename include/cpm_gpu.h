@@ -38,7 +38,7 @@ enum cpm_precond { CPM_PRECOND_IDENTITY = 0, CPM_PRECOND_DIAGONAL = 1, CPM_PRECO
 /* Precision of the Krylov iterations. */
 enum cpm_precision {
     CPM_FP64 = 0,          /* reference arithmetic: FP64 BiCGStab               */
-    CPM_FP32_REFINE = 1    /* FP32 BiCGStab inside FP64 iterative refinement    */
+    CPM_FP32_REFINE = 1    /* FP64 iterative refinement around a continuous FP32 BiCGStab      */
 };
 
 /* Per-solve termination status (cpm_solve_info.status). */
@@ -57,16 +57,16 @@ typedef struct cpm_solve_opts {
     double tol;            /* relative residual ||b-Ax|| <= tol ||b||; reference 1e-10 (src/Solver.cpp:164) */
     int max_iters;         /* reference 100000 (src/Solver.cpp:164)                                         */
     int precision;         /* enum cpm_precision                                                            */
-    double inner_tol;      /* FP32_REFINE: relative tolerance of each inner FP32 solve (<=0: default 1e-4) */
-    int max_refinements;   /* FP32_REFINE: cap on outer refinement steps (<=0: default 40)                  */
+    double update_delta;   /* FP32_REFINE: FP32 residual drop that triggers an FP64 refinement step (<=0: 1e-3) */
+    int reserved;          /* must be 0                                                                     */
 } cpm_solve_opts;
 
 typedef struct cpm_solve_info {
     int status;            /* enum cpm_solve_status                              */
-    int iters;             /* BiCGStab iterations (summed over inner solves)     */
+    int iters;             /* BiCGStab iterations                                */
     int restarts;          /* rho-breakdown restarts (src/BiCGStab.h:62-71)       */
-    int refinements;       /* outer FP64 refinement steps (FP32_REFINE)          */
-    double rel_residual;   /* ||b-Ax||/||b|| of the returned x (FP64 recurrence / refinement residual) */
+    int updates;           /* FP64 refinement steps (FP32_REFINE)                 */
+    double rel_residual;   /* ||b-Ax||/||b||: FP64 recurrence residual (FP64), true FP64 residual (FP32_REFINE) */
 } cpm_solve_info;
 
 /* Defaults of the reference: tol 1e-10, max 100000, FP64. */

@@ -135,7 +135,7 @@ def test_torus_knot_config2(gpu_ctx):
     x32, i32 = op.solve(b, cpmgpu.opts(tol=1e-11, precision=cpmgpu.CPM_FP32_REFINE))
     rel32 = np.linalg.norm(x32 - x) / np.linalg.norm(x)
     print(f"cfg2 fp64 iters={inf['iters']} oracle-residual={rel_res:.2e}; fp32+refine iters={i32['iters']} "
-          f"refinements={i32['refinements']} rel_diff={rel32:.2e}", flush=True)
+          f"updates={i32['updates']} rel_diff={rel32:.2e}", flush=True)
     # the recurrence residual (1e-12) and the true FP64 residual part near the FP64
     # floor of this system (||A|| ~ 1e4): the true residual stays well under the
     # reference's own 1e-10 stop

@@ -37,9 +37,12 @@ Buffer::~Buffer() {
 
 // ---------------------------------------------------------------- packing
 // Interleave up to NR caller vectors into a workspace vector [row][rhs] and back.
+// The operator's internal row r holds the caller's row perm[r] (tube-built
+// operators run in lexicographic order).
 struct PackEntry {
     const double* src;   // caller vector (pack) or nullptr
     double* dst;         // caller vector (unpack) or nullptr
+    const int* perm;     // internal row -> caller row, nullptr = identity
     int64_t D;
     int64_t off;
     int lane, nr;
@@ -51,12 +54,13 @@ __global__ void k_pack(const PackEntry* ent, const int* bent, double* ws64, floa
     const int64_t row = (int64_t)(blockIdx.x - e.block0) * 256 + threadIdx.x;
     if (row >= e.D) return;
     const size_t i = (size_t)(e.off + row * e.nr + e.lane);
+    const int64_t cr = e.perm ? e.perm[row] : row;
     if (to_ws) {
-        const double v = e.src ? e.src[row] : 0.0;
+        const double v = e.src ? e.src[cr] : 0.0;
         if (ws64) ws64[i] = v;
         if (ws32) ws32[i] = (float)v;
     } else if (e.dst) {   // padding lanes of a group have no caller vector
-        e.dst[row] = ws64 ? ws64[i] : (double)ws32[i];
+        e.dst[cr] = ws64 ? ws64[i] : (double)ws32[i];
     }
 }
 
@@ -106,20 +110,42 @@ void Op::upload_ell(const std::vector<int>& ec, const std::vector<double>& ev, i
     up(flag, fl.data(), fl.size());
 }
 
-template <>
-OpView<double> Op::view<double>() const {
-    return OpView<double>{D, nD, ew, lw, ecol.as<int>(), eval64.as<double>(), lcol.as<int>(),
-                          lval64.as<double>(), ldiag64.as<double>(), flag.as<uint8_t>()};
+template <typename T>
+static OpView<T> make_view(const Op& o, const Buffer& ev, const Buffer& lv, const Buffer& ld, const Buffer& tw,
+                           const Buffer& xv) {
+    OpView<T> v{};
+    v.D = o.D;
+    v.nD = o.nD;
+    v.ew = o.ew;
+    v.lw = o.lw;
+    v.ecol = o.ecol.as<int>();
+    v.eval = ev.as<T>();
+    v.lcol = o.lcol.as<int>();
+    v.lval = lv.as<T>();
+    v.ldiag = ld.as<T>();
+    v.flag = o.flag.as<uint8_t>();
+    v.fmt = o.fmt;
+    v.tpc = o.tpc.as<int>();
+    v.tpw = tw.as<T>();
+    v.exc = o.exc.as<int>();
+    v.emask = o.emask.as<unsigned long long>();
+    v.eoff = o.eoff.as<int>();
+    v.ecolx = o.ecolx.as<int>();
+    v.xcol = o.xcol.as<int>();
+    v.xval = xv.as<T>();
+    v.loff = (T)o.loff;
+    v.ldu = (T)o.ldu;
+    return v;
 }
 template <>
-OpView<float> Op::view<float>() const {
-    return OpView<float>{D, nD, ew, lw, ecol.as<int>(), eval32.as<float>(), lcol.as<int>(),
-                         lval32.as<float>(), ldiag32.as<float>(), flag.as<uint8_t>()};
-}
+OpView<double> Op::view<double>() const { return make_view<double>(*this, eval64, lval64, ldiag64, tpw64, xval64); }
+template <>
+OpView<float> Op::view<float>() const { return make_view<float>(*this, eval32, lval32, ldiag32, tpw32, xval32); }
 
 size_t Op::device_bytes() const {
     return ecol.bytes + eval64.bytes + eval32.bytes + lcol.bytes + lval64.bytes + lval32.bytes + ldiag64.bytes +
-           ldiag32.bytes + flag.bytes;
+           ldiag32.bytes + flag.bytes + perm.bytes + tpc.bytes + tpw64.bytes + tpw32.bytes + exc.bytes + emask.bytes +
+           eoff.bytes + ecolx.bytes + xcol.bytes + xval64.bytes + xval32.bytes;
 }
 
 Op* op_from_csr(Ctx* ctx, int64_t D, int nD, const int32_t* Lp, const int32_t* Lc, const double* Lv,
@@ -189,6 +215,7 @@ struct Group {
 template <typename T>
 static void fill_seg(Seg<T>& s, const Op* op, int64_t off, int block0, int nblocks, int nr, KState* st, double* part) {
     s.op = op->view<T>();
+    s.op64 = op->view<double>();
     s.m = op->m;
     s.n = op->n;
     s.off = off;
@@ -279,6 +306,7 @@ static void run_batch(Ctx* ctx, const std::vector<Group>& groups, const std::vec
             PackEntry e{};
             e.src = k < (int)groups[g].inst.size() ? rhs[groups[g].inst[k]] : nullptr;
             e.dst = k < (int)groups[g].inst.size() ? xs[groups[g].inst[k]] : nullptr;
+            e.perm = groups[g].op->perm.as<int>();
             e.D = groups[g].op->D;
             e.off = off[g];
             e.lane = k;
@@ -309,7 +337,7 @@ static void run_batch(Ctx* ctx, const std::vector<Group>& groups, const std::vec
         for (int g = 0; g < G; ++g)
             for (size_t k = 0; k < groups[g].inst.size(); ++k) {
                 const int s = hs[g * kMaxNR + k].status;
-                run |= s == ST_RUN || (s >= ST_UPDATE && s <= ST_RESCALE);
+                run |= s == ST_RUN || (s >= ST_UPDATE && s <= ST_READY);
                 restart |= s == ST_RESTART;
             }
     };
@@ -318,7 +346,7 @@ static void run_batch(Ctx* ctx, const std::vector<Group>& groups, const std::vec
     if (!refine) {
         Launch<double, NR>::init(c64, V64.b, V64.x, V64.r, V64.r0, V64.p, V64.v);
         for (;;) {
-            for (int c = 0; c < chunk; ++c) Launch<double, NR>::iteration(c64, V64, false);
+            for (int c = 0; c < chunk; ++c) Launch<double, NR>::iteration(c64, V64, false, MixVec{});
             poll(run, restart);
             if (restart) {   // rare: handled between chunks, the other instances wait
                 Launch<double, NR>::restart(c64, V64);
@@ -332,11 +360,9 @@ static void run_batch(Ctx* ctx, const std::vector<Group>& groups, const std::vec
         double* r64 = V64.r;
         MixedLaunch<NR>::start(c64, V64.b, x64, r64);
         MixedLaunch<NR>::down(c64, r64, V32);
+        const MixVec mv{x64, r64, V64.b, V64.tmp};
         for (;;) {
-            for (int c = 0; c < chunk; ++c) {
-                Launch<float, NR>::iteration(c32, V32, true);
-                MixedLaunch<NR>::update(c64, V64.b, x64, r64, V64.t, V64.tmp, V32);   // skips lanes that did not ask
-            }
+            for (int c = 0; c < chunk; ++c) Launch<float, NR>::iteration(c32, V32, true, mv);
             poll(run, restart);
             if (!run) break;
             chunk = std::min(chunk * 2, 64);
@@ -391,20 +417,37 @@ void solve_batch(Ctx* ctx, const std::vector<Op*>& ops, const std::vector<const 
 void multiply(Op* op, const double* dx, double* dy, double m, double n) {
     Ctx* ctx = op->ctx;
     const hipStream_t st = ctx->stream;
-    char* tb = reinterpret_cast<char*>(ctx->workspace("mul_tables", 4096 + sizeof(int) * ((op->D + 255) / 256)));
-    Seg<double> h{};
     const int nblocks = (int)((op->D + kBlock - 1) / kBlock);
+    char* tb = reinterpret_cast<char*>(ctx->workspace("mul_tables", 8192 + sizeof(int) * nblocks));
+    Seg<double> h{};
     fill_seg(h, op, 0, 0, nblocks, 1, nullptr, nullptr);
     h.m = m;
     h.n = n;
     std::vector<int> hb(nblocks, 0);
+    PackEntry pe[2]{};
+    pe[0].src = dx;
+    pe[1].dst = dy;
+    for (auto& e : pe) {
+        e.perm = op->perm.as<int>();
+        e.D = op->D;
+        e.nr = 1;
+    }
     auto* dseg = reinterpret_cast<Seg<double>*>(tb);
-    int* dbseg = reinterpret_cast<int*>(tb + 4096);
+    auto* dpe = reinterpret_cast<PackEntry*>(tb + 4096);
+    int* dbseg = reinterpret_cast<int*>(tb + 8192);
     HIPCHK(hipMemcpyAsync(dseg, &h, sizeof(h), hipMemcpyHostToDevice, st));
+    HIPCHK(hipMemcpyAsync(dpe, pe, sizeof(pe), hipMemcpyHostToDevice, st));
     HIPCHK(hipMemcpyAsync(dbseg, hb.data(), sizeof(int) * nblocks, hipMemcpyHostToDevice, st));
-    double* tmp = ctx->workspace("mul_tmp", sizeof(double) * op->D);
+    double* w = ctx->workspace("mul_tmp", sizeof(double) * op->D * 3);
+    double *xi = w, *yi = w + op->D, *tmp = w + 2 * op->D;
     LaunchCfg<double> c{dseg, dbseg, nblocks, 1, st};
-    Launch<double, 1>::apply(c, dx, tmp, dy);
+    if (op->perm.p) {   // caller order -> internal order and back
+        hipLaunchKernelGGL(k_pack, dim3(nblocks), dim3(256), 0, st, dpe, dbseg, xi, (float*)nullptr, 1);
+        Launch<double, 1>::apply(c, xi, tmp, yi);
+        hipLaunchKernelGGL(k_pack, dim3(nblocks), dim3(256), 0, st, dpe + 1, dbseg, yi, (float*)nullptr, 0);
+    } else {
+        Launch<double, 1>::apply(c, dx, tmp, dy);
+    }
     HIPCHK(hipGetLastError());
 }
 
@@ -442,7 +485,14 @@ static void profile_egather_t(Op* op, int reps, double* avg_ms, double* bytes) {
     int64_t gathered = 0;
     for (uint8_t f : fl) gathered += !(f & ROW_IDENTITY);
     *avg_ms = ms / reps;
-    *bytes = (double)gathered * op->ew * (4.0 + sizeof(T)) + 2.0 * op->D * NR * sizeof(T) + (double)op->D;
+    // algorithmic bytes: operator data of the gathered rows, x read once, out written once, flags
+    double opb;
+    if (op->fmt == OP_TP)
+        opb = (double)gathered * (16 * 4.0 + 12.0 * sizeof(T) + 4.0) + (double)op->nexc * (8 + 4 + 4 + sizeof(T)) +
+              4.0 * op->nmoved;
+    else
+        opb = (double)gathered * op->ew * (4.0 + sizeof(T));
+    *bytes = opb + 2.0 * op->D * NR * sizeof(T) + (double)op->D;
 }
 
 }  // namespace cpmgpu
@@ -557,6 +607,8 @@ int cpm_op_destroy(cpm_op* op) {
 }
 
 int64_t cpm_op_rows(const cpm_op* op) { return op ? O(op)->D : -1; }
+
+int cpm_op_format(const cpm_op* op) { return op ? O(op)->fmt : -1; }
 
 int cpm_op_set_mn(cpm_op* op, double m, double n) {
     return guard([&] {

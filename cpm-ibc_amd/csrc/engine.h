@@ -20,9 +20,10 @@ enum : uint8_t { ROW_IDENTITY = 1, ROW_NODIAG = 2 };
 // kernels (krylov.hip) and read uniformly by every block of the next kernels.
 enum : int {
     ST_RUN = 0, ST_CONVERGED = 1, ST_MAXIT = 2, ST_RESTART = 3, ST_ZERO_RHS = 4, ST_BREAKDOWN = 5,
-    // mixed precision: the FP32 recurrence asks for an FP64 refinement step,
-    // after which the FP32 vectors are rebuilt from the rescaled true residual
-    ST_UPDATE = 6, ST_UPDATE_MAXIT = 7, ST_RESCALE = 8
+    // mixed precision: the FP32 recurrence asks for an FP64 refinement step
+    // (UPDATE*), the true residual is rescaled into the FP32 vectors
+    // (RESCALE), and the lane rejoins at the next iteration (READY)
+    ST_UPDATE = 6, ST_UPDATE_MAXIT = 7, ST_RESCALE = 8, ST_READY = 9
 };
 
 struct KState {
@@ -36,9 +37,20 @@ struct KState {
     int updates, pad0, pad1, pad2;
 };
 
-// Device view of an operator in one precision.  E and Loff are ELL matrices
-// stored slot-major (entry s of row r at [s * D + r]) so that a wavefront's 64
-// rows read 64 consecutive words per slot.
+enum : int { OP_ELL = 0, OP_TP = 1 };
+
+// Device view of an operator in one precision.  Arrays are slot-major (entry
+// s of row r at [s * D + r]) so that a wavefront's 64 rows read 64
+// consecutive words per slot.
+//
+// OP_ELL: E and Loff are ELL matrices of widths ew and lw.
+// OP_TP (operators built on a tube, p = 3): rows are in the tube's
+// lexicographic order, so the four z-neighbours b + (i, j, 0..3) of a stencil
+// line are consecutive columns; E row = sum over (i, j, k) of
+// (wz[k] wy[j]) wx[i] x[tpc(i, j) + k] (the reference's slot order and
+// product order), except for the slots flagged in the row's exception record,
+// whose columns were moved by an interior boundary (and one optional extra
+// entry).  L off-diagonals all equal loff (lcol -1 = absent), L_ii = ldu.
 template <typename T>
 struct OpView {
     int64_t D;
@@ -50,6 +62,16 @@ struct OpView {
     const T* lval;
     const T* ldiag;      // L_ii (0 where L has no diagonal)
     const uint8_t* flag;
+    int fmt;             // OP_ELL / OP_TP
+    const int* tpc;      // [16][D] first column of stencil line (i, j)
+    const T* tpw;        // [12][D] wx[4], wy[4], wz[4]; a row's sign is folded into wx
+    const int* exc;      // [D] exception record of the row, -1 = none
+    const unsigned long long* emask;   // [nexc] bit s: slot s has a moved column
+    const int* eoff;     // [nexc] first moved column in ecolx
+    const int* ecolx;    // moved columns, in slot order
+    const int* xcol;     // [nexc] extra column after the stencil, -1 = none
+    const T* xval;       // [nexc]
+    T loff, ldu;
 };
 
 // One instance group of a batched launch: an operator, NR right-hand sides
@@ -57,13 +79,14 @@ struct OpView {
 template <typename T>
 struct Seg {
     OpView<T> op;
+    OpView<double> op64; // the same operator in FP64 (refinement steps of FP32 segments)
     double m, n;
     int64_t off;         // element offset of row 0 in every workspace vector (rows interleave NR rhs)
     int block0, nblocks;
     int nr;              // rhs in this group (<= template NR; extra lanes idle)
     int precond;         // CPM_PRECOND_*
     KState* st;          // nr states
-    double* part;        // nblocks * 2 * NR partial sums of the last reducing kernel
+    double* part;        // [2 * NR][nblocks] partial sums of the last reducing kernel
 };
 
 void hip_check(hipError_t e, const char* what);

@@ -51,6 +51,7 @@ struct Problem {
     Buffer normals;             // whole-tube unoriented normals [N*3]
     Buffer ecol, eval64, eval32, lcol, lval64, lval32, ldiag64, ldiag32, flag;
     Buffer rhs_pts, rhs_kind;
+    Buffer qpts;                // [3 * D] query point of each E row (NaN: identity rows)
     double build_ms = 0;
 };
 

@@ -114,6 +114,11 @@ __device__ __forceinline__ double grad_entry(int32_t cp_, int32_t cm, double vp,
 
 // ---------------------------------------------------------------- whole-tube normals
 // Gradients of the closest point coordinates: G[(r*3+c)][i] = (Dc_c cp_r)_i.
+__global__ void k_fill_f64(double* a, int64_t n, double v) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) a[i] = v;
+}
+
 __global__ void k_grad_tube(int64_t N, const int32_t* nbr, const double* cpx, double dx, double* G) {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= N) return;
@@ -835,6 +840,9 @@ Problem* problem_build(Tube* t, const std::vector<IbcSpec>& specs) {
                        P->flag.as<uint8_t>(), err);
     realloc<double>(P->rhs_pts, D * 3);
     realloc<int32_t>(P->rhs_kind, D);
+    realloc<double>(P->qpts, D * 3);
+    hipLaunchKernelGGL(k_fill_f64, dim3(nblk(3 * D)), dim3(256), 0, st, P->qpts.as<double>(), 3 * D, (double)NAN);
+    HIPCHK(hipMemcpyAsync(P->qpts.p, t->cpx.p, (size_t)N * 24, hipMemcpyDeviceToDevice, st));
     hipLaunchKernelGGL(k_rhs_tube, dim3(nblk(N)), dim3(256), 0, st, N, t->cpx.as<double>(), P->rhs_pts.as<double>(),
                        P->rhs_kind.as<int32_t>());
     for (int c = 0; c < a.nibc; ++c) {
@@ -845,6 +853,7 @@ Problem* problem_build(Tube* t, const std::vector<IbcSpec>& specs) {
         int* ecg = reinterpret_cast<int*>(ctx->workspace("prob_ecg", (size_t)W * I.nd * 4));
         double* evg = ctx->workspace("prob_evg", (size_t)W * I.nd * 8);
         build_E_ell(*t, q, I.nd, ecg, evg, nullptr, I.nd, st);
+        HIPCHK(hipMemcpyAsync(P->qpts.as<double>() + 3 * I.start, q, (size_t)I.nd * 24, hipMemcpyDeviceToDevice, st));
         const int64_t ident0 = ident_start + I.dir_start;
         hipLaunchKernelGGL(k_asm_ghost, dim3(nblk(I.nd)), dim3(256), 0, st, a, c, I.nd, I.spec.order, I.spec.btype[0],
                            ident0, ecg, evg, W, I.d_set.as<int32_t>(), t->nbr.as<int32_t>(), P->ecol.as<int>(),
@@ -867,6 +876,23 @@ Op* op_from_problem(Problem* P, double m, double n, int precond, int precond_ite
     const hipStream_t st = P->ctx->stream;
     const int64_t D = P->D;
     std::unique_ptr<Op> op(new Op(P->ctx, D, 0));
+    op->m = m;
+    op->n = n;
+    op->precond = precond;
+    op->precond_iters = precond_iters;
+    TpSource src;
+    src.D = D;
+    src.N = P->N;
+    src.ecol = P->ecol.as<int>();
+    src.eval = P->eval64.as<double>();
+    src.ew = P->ew;
+    src.lcol = P->lcol.as<int>();
+    src.lval = P->lval64.as<double>();
+    src.lw = 6;
+    src.ldiag = P->ldiag64.as<double>();
+    src.flag = P->flag.as<uint8_t>();
+    src.qpts = P->qpts.as<double>();
+    if (!tp_disabled() && op_build_tp(*op, *P->tube, src)) return op.release();
     op->ew = P->ew;
     op->lw = 6;
     auto cp = [&](Buffer& dst, const Buffer& src) {

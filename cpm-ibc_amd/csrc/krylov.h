@@ -20,24 +20,33 @@ struct Vecs {
     T *b, *x, *r, *r0, *p, *v, *s, *t, *y, *z, *tmp;
 };
 
+// FP64 vectors of a mixed-precision solve (same element layout as the FP32
+// workspace), used by the FP32 iteration kernels for refinement steps.
+struct MixVec {
+    double* x64;          // solution
+    double* r64;          // true residual b - A64 x64
+    const double* b64;    // right-hand sides
+    double* tmp64;        // E64 x64
+};
+
 template <typename T, int NR>
 struct Launch {
     static void init(const LaunchCfg<T>& c, const T* b, T* x, T* r, T* r0, T* p, T* v);
-    // mixed = true: the loop test asks for reliable updates instead of stopping
-    static void iteration(const LaunchCfg<T>& c, Vecs<T>& w, bool mixed);
+    // mixed: FP32 recurrence whose lanes take FP64 refinement steps in place
+    // (mv holds the FP64 vectors); otherwise the reference's FP64 loop
+    static void iteration(const LaunchCfg<T>& c, Vecs<T>& w, bool mixed, const MixVec& mv);
     static void restart(const LaunchCfg<T>& c, Vecs<T>& w);
-    // y = A x; act >= 0 restricts the work to instances with a lane in that state
-    static void apply(const LaunchCfg<T>& c, const T* x, T* tmp, T* y, int act = -1);
+    // y = A x; lanes = status mask restricting the work to instances with such a lane (0: all)
+    static void apply(const LaunchCfg<T>& c, const T* x, T* tmp, T* y, unsigned lanes = 0);
 };
 
-// FP32 BiCGStab with reliable FP64 updates (the segments of both precisions
-// share one state array).
+// Start of a mixed-precision solve: x64 = 0, r64 = b, then the FP32 vectors
+// from the scaled residual (the segments of both precisions share one state
+// array).
 template <int NR>
 struct MixedLaunch {
     static void start(const LaunchCfg<double>& c, const double* b, double* x64, double* r64);
     static void down(const LaunchCfg<double>& c, const double* r64, Vecs<float>& w);
-    static void update(const LaunchCfg<double>& c, const double* b, double* x64, double* r64, double* ax, double* tmp,
-                       Vecs<float>& w);
 };
 
 // E-gather alone (identity rows copy), for measurement.

@@ -25,21 +25,31 @@
 // One launch covers every instance of a batch: workgroups map to (instance,
 // row block) through a segment table; finished instances are skipped.
 // Right-hand sides of one operator are interleaved [row][rhs] so one read of
-// an E slot serves all of them.
+// an E slot serves all of them.  Each rhs lane has its own state; a kernel
+// acts on the lanes whose status is in its mask.
 //
 // Precision CPM_FP32_REFINE: iterative refinement with FP32 BiCGStab inner
 // solves, run as one continuous device-side recurrence.  The solution is
 // accumulated in FP64; when the FP32 recurrence residual has dropped by
-// delta (default 1e-3) since the last step, or meets the tolerance, or
-// hits the rho-restart rule, x64 += x32 and the FP32 state is rebuilt from
-// the true FP64 residual b - A64 x64 rescaled to unit norm (r0 = r, p = v =
-// 0).  Convergence is declared on the true FP64 residual.  (Keeping p across
-// steps, as in reliable-update schemes for one operator, diverges here once
-// the residual approaches the FP32 operator's own rounding: the FP32 and
-// FP64 operators differ in the last bits.)
+// delta (default 1e-3) since the last step, or meets the tolerance, or hits
+// the rho-restart rule, the lane takes a refinement step inside the next
+// iteration's kernels (no extra launches):
+//
+//   pupdate   x64 += x32 / scale ; x32 = 0
+//   egather   tmp64 = E64 x64
+//   lfinish   r64 = b - A64 x64 ; partial (r64, r64)
+//   fin       converged on the true residual?  else scale = 1 / ||r64||
+//   supdate   r32 = r64 scale ; r0 = r32 ; p = v = 0
+//   lfinish   partial (r32, r32)
+//   fin       rho = r0sq = (r32, r32) ; the lane rejoins the next iteration
+//
+// (Keeping p across steps, as in reliable-update schemes for one operator,
+// diverges here once the residual approaches the FP32 operator's own
+// rounding: the FP32 and FP64 operators differ in the last bits.)
 #include <hip/hip_runtime.h>
 
 #include <cmath>
+#include <type_traits>
 
 #include "engine.h"
 #include "krylov.h"
@@ -71,9 +81,11 @@ __device__ __forceinline__ double wave_sum(double v) {
     return v;
 }
 
-// Workgroup sums of NV*NR per-thread values, written to this block's partials.
+// Workgroup sums of NV*NR per-thread values.  Partials are stored
+// component-major, part[c * nblocks + lb], so that the finaliser reads each
+// component with coalesced loads.
 template <int NV, int NR>
-__device__ void block_partials(const double (&v)[NV][NR], double* part_block) {
+__device__ void block_partials(const double (&v)[NV][NR], double* part, int lb, int nblocks) {
     __shared__ double red[kBlock / 64][NV * NR];
     const int lanei = threadIdx.x & 63, wave = threadIdx.x >> 6;
 #pragma unroll
@@ -88,7 +100,7 @@ __device__ void block_partials(const double (&v)[NV][NR], double* part_block) {
         double s = 0.0;
 #pragma unroll
         for (int w = 0; w < kBlock / 64; ++w) s += red[w][threadIdx.x];
-        part_block[threadIdx.x] = s;
+        part[(size_t)threadIdx.x * nblocks + lb] = s;
     }
 }
 
@@ -100,26 +112,27 @@ __device__ __forceinline__ const Seg<T>& seg_of(const Seg<T>* segs, const int* b
     return sg;
 }
 
-// Which instances a reducing kernel / its finaliser act on.
-enum Act : int { ACT_ALL = 0, ACT_RUN = 1, ACT_RESTART = 2, ACT_UPDATE = 3, ACT_RESCALE = 4 };
+// Lane masks: bit s set = act on lanes whose status is s.
+constexpr unsigned M_RUN = 1u << ST_RUN;
+constexpr unsigned M_RESTART = 1u << ST_RESTART;
+constexpr unsigned M_UPDATE = (1u << ST_UPDATE) | (1u << ST_UPDATE_MAXIT);
+constexpr unsigned M_RESCALE = 1u << ST_RESCALE;
+constexpr unsigned M_READY = 1u << ST_READY;
+constexpr unsigned M_ALL = 0xffffffffu;
+// The iteration's stages (the FP64 loop only ever has RUN lanes among these).
+constexpr unsigned M_STAGE1 = M_RUN | M_UPDATE;    // pupdate .. fin alpha
+constexpr unsigned M_STAGE2 = M_RUN | M_RESCALE;   // supdate, lfinish t .. fin omega
+constexpr unsigned M_STAGE3 = M_RUN | M_READY;     // fin of the loop test
 
 template <typename T>
-__device__ __forceinline__ bool lane_active(const Seg<T>& sg, int k, int act) {
-    if (k >= sg.nr) return false;
-    const int s = sg.st[k].status;
-    switch (act) {
-    case ACT_RUN: return s == ST_RUN;
-    case ACT_RESTART: return s == ST_RESTART;
-    case ACT_UPDATE: return s == ST_UPDATE || s == ST_UPDATE_MAXIT;
-    case ACT_RESCALE: return s == ST_RESCALE;
-    default: return true;
-    }
+__device__ __forceinline__ bool lane_in(const Seg<T>& sg, int k, unsigned mask) {
+    return k < sg.nr && ((mask >> sg.st[k].status) & 1u);
 }
 
 template <typename T>
-__device__ __forceinline__ bool seg_active(const Seg<T>& sg, int act) {
+__device__ __forceinline__ bool seg_any(const Seg<T>& sg, unsigned mask) {
     bool a = false;
-    for (int k = 0; k < sg.nr; ++k) a |= lane_active(sg, k, act);
+    for (int k = 0; k < sg.nr; ++k) a |= ((mask >> sg.st[k].status) & 1u) != 0;
     return a;
 }
 
@@ -132,7 +145,111 @@ __device__ __forceinline__ T precond_apply(const Seg<T>& sg, int64_t row, uint8_
     if (row >= dof) return val / (T)(sg.m - sg.n);   // :335-338 overwrite the trailing rows last
     if (f & ROW_IDENTITY) return val;
     if (f & ROW_NODIAG) return val;
-    return val / ((T)sg.m + sg.op.ldiag[row] * (T)sg.n);
+    const T ld = sg.op.fmt == OP_TP ? sg.op.ldu : sg.op.ldiag[row];
+    return val / ((T)sg.m + ld * (T)sg.n);
+}
+
+// E row of the tensor-product format: slot (i, j, k) in the reference's
+// order, value (wz[k] wy[j]) wx[i] formed exactly as when E was built.  One
+// stencil line (4 consecutive x rows) per step, two steps in flight: fully
+// unrolled, the 64 gathers of NR-wide rows would take every register.
+// `stride` = elements between consecutive rows of xb.
+template <typename T, typename V, int W>
+__device__ __forceinline__ void egather_tp(const OpView<T>& op, int64_t row, const T* xb, int stride, V& acc) {
+    const int64_t D = op.D;
+    T wz[4];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) wz[q] = op.tpw[(8 + q) * D + row];
+    const int e = op.exc[row];
+    if (e < 0) {
+#pragma unroll 2
+        for (int l = 0; l < 16; ++l) {
+            const T wx = op.tpw[(l >> 2) * D + row];
+            const T wy = op.tpw[(4 + (l & 3)) * D + row];
+            const T* xp = xb + (size_t)op.tpc[l * D + row] * stride;
+            const V x0 = vload<T, W>(xp), x1 = vload<T, W>(xp + stride), x2 = vload<T, W>(xp + 2 * stride),
+                    x3 = vload<T, W>(xp + 3 * stride);
+            acc += ((wz[0] * wy) * wx) * x0;
+            acc += ((wz[1] * wy) * wx) * x1;
+            acc += ((wz[2] * wy) * wx) * x2;
+            acc += ((wz[3] * wy) * wx) * x3;
+        }
+    } else {   // columns moved by an interior boundary, and the extra entry
+        const unsigned long long mask = op.emask[e];
+        const int* mc = op.ecolx + op.eoff[e];
+#pragma unroll 1
+        for (int l = 0; l < 16; ++l) {
+            const T wx = op.tpw[(l >> 2) * D + row];
+            const T wy = op.tpw[(4 + (l & 3)) * D + row];
+            const int c0 = op.tpc[l * D + row];
+#pragma unroll
+            for (int k = 0; k < 4; ++k) {
+                int c = c0 + k;
+                if ((mask >> (4 * l + k)) & 1ull) c = *mc++;
+                acc += ((wz[k] * wy) * wx) * vload<T, W>(xb + (size_t)c * stride);
+            }
+        }
+        const int xc = op.xcol[e];
+        if (xc >= 0) acc += op.xval[e] * vload<T, W>(xb + (size_t)xc * stride);
+    }
+}
+
+// E row of the ELL format (eight slots in flight).
+template <typename T, typename V, int W>
+__device__ __forceinline__ void egather_ell(const OpView<T>& op, int64_t row, const T* xb, int stride, V& acc) {
+    const int64_t D = op.D;
+    const int* col = op.ecol + row;
+    const T* val = op.eval + row;
+    const int ew = op.ew;
+    int s = 0;
+    for (; s + 8 <= ew; s += 8) {
+        int c[8];
+        T w[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) { c[u] = col[(int64_t)(s + u) * D]; w[u] = val[(int64_t)(s + u) * D]; }
+#pragma unroll
+        for (int u = 0; u < 8; ++u) acc += w[u] * vload<T, W>(xb + (size_t)c[u] * stride);
+    }
+    for (; s < ew; ++s) acc += val[(int64_t)s * D] * vload<T, W>(xb + (size_t)col[(int64_t)s * D] * stride);
+}
+
+// (E x)_row for W consecutive values per row of xb.
+template <typename T, int W>
+__device__ __forceinline__ typename VT<T, W>::type erow(const OpView<T>& op, int64_t row, const T* xb, int stride) {
+    typedef typename VT<T, W>::type V;
+    V acc = (V)(T)0;
+    if (op.flag[row] & ROW_IDENTITY) return vload<T, W>(xb + (size_t)row * stride);
+    if (op.fmt == OP_TP) egather_tp<T, V, W>(op, row, xb, stride, acc);
+    else egather_ell<T, V, W>(op, row, xb, stride, acc);
+    return acc;
+}
+
+// (A x)_row = m x + n (Ld x + Loff tmp), identity rows x, trailing nD rows
+// (m - n) x; xv = x_row, tb = tmp of the segment, W values per row.
+template <typename T, int W>
+__device__ __forceinline__ typename VT<T, W>::type arow(const OpView<T>& op, double m, double n, int64_t row,
+                                                        const typename VT<T, W>::type& xv, const T* tb, int stride) {
+    typedef typename VT<T, W>::type V;
+    const int64_t D = op.D;
+    if (row >= D - op.nD) return (T)(m - n) * xv;
+    if (op.flag[row] & ROW_IDENTITY) return xv;
+    V lo = (V)(T)0;
+    T ld;
+    if (op.fmt == OP_TP) {
+#pragma unroll
+        for (int s = 0; s < 6; ++s) {
+            const int c = op.lcol[(int64_t)s * D + row];
+            if (c >= 0) lo += op.loff * vload<T, W>(tb + (size_t)c * stride);
+        }
+        ld = op.ldu;
+    } else {
+        for (int s = 0; s < op.lw; ++s) {
+            const int c = op.lcol[(int64_t)s * D + row];
+            lo += op.lval[(int64_t)s * D + row] * vload<T, W>(tb + (size_t)c * stride);
+        }
+        ld = op.ldiag[row];
+    }
+    return (T)m * xv + (T)n * (ld * xv + lo);
 }
 
 // ---------------------------------------------------------------- vector kernels
@@ -157,83 +274,81 @@ __global__ __launch_bounds__(kBlock) void k_init(const Seg<T>* segs, const int* 
 #pragma unroll
         for (int k = 0; k < NR; ++k) pv[0][k] = (double)lane(bv, k) * (double)lane(bv, k);
     }
-    block_partials<1, NR>(pv, sg.part + (size_t)lb * NR);
+    block_partials<1, NR>(pv, sg.part, lb, sg.nblocks);
 }
 
+// RUN lanes: p = (p - w v) beta + r ; y = P^-1 p.
+// UPDATE lanes (mixed): x64 += x32 / scale ; x32 = 0.
 template <typename T, int NR>
 __global__ __launch_bounds__(kBlock) void k_pupdate(const Seg<T>* segs, const int* bseg, T* p, const T* v,
-                                                    const T* r, T* y) {
+                                                    const T* r, T* y, T* x, MixVec mv) {
     int lb;
     int64_t row;
     const Seg<T>& sg = seg_of(segs, bseg, lb, row);
-    if (row >= sg.op.D || !seg_active(sg, ACT_RUN)) return;
+    if (row >= sg.op.D || !seg_any(sg, M_STAGE1)) return;
     const size_t i = (size_t)(sg.off + row * NR);
-    const uint8_t f = sg.op.flag[row];
-    auto pv = vload<T, NR>(p + i);
-    auto vv = vload<T, NR>(v + i);
-    auto rv = vload<T, NR>(r + i);
-    typename VT<T, NR>::type yv;
+    if (seg_any(sg, M_RUN)) {
+        const uint8_t f = sg.op.flag[row];
+        auto pv = vload<T, NR>(p + i);
+        auto vv = vload<T, NR>(v + i);
+        auto rv = vload<T, NR>(r + i);
+        typename VT<T, NR>::type yv;
 #pragma unroll
-    for (int k = 0; k < NR; ++k) {
-        const KState& st = sg.st[k < sg.nr ? k : 0];
-        const T beta = (T)st.beta, w = (T)st.omega;
-        T pk = (-w * lane(vv, k) + lane(pv, k)) * beta + lane(rv, k);   // p.Copy(-w,v,p); p*=beta; p+=r
-        if (k >= sg.nr || st.status != ST_RUN) pk = lane(pv, k);
-        setlane(pv, k, pk);
-        setlane(yv, k, precond_apply(sg, row, f, pk));
+        for (int k = 0; k < NR; ++k) {
+            const KState& st = sg.st[k < sg.nr ? k : 0];
+            const T beta = (T)st.beta, w = (T)st.omega;
+            T pk = (-w * lane(vv, k) + lane(pv, k)) * beta + lane(rv, k);   // p.Copy(-w,v,p); p*=beta; p+=r
+            if (!lane_in(sg, k, M_RUN)) pk = lane(pv, k);
+            setlane(pv, k, pk);
+            setlane(yv, k, precond_apply(sg, row, f, pk));
+        }
+        vstore<T, NR>(p + i, pv);
+        vstore<T, NR>(y + i, yv);
     }
-    vstore<T, NR>(p + i, pv);
-    vstore<T, NR>(y + i, yv);
+    if (std::is_same<T, float>::value && seg_any(sg, M_UPDATE)) {
+        auto xv = vload<T, NR>(x + i);
+        auto xd = vload<double, NR>(mv.x64 + i);
+#pragma unroll
+        for (int k = 0; k < NR; ++k)
+            if (lane_in(sg, k, M_UPDATE)) {
+                setlane(xd, k, lane(xd, k) + (double)lane(xv, k) / sg.st[k].scale);
+                setlane(xv, k, (T)0);
+            }
+        vstore<double, NR>(mv.x64 + i, xd);
+        vstore<T, NR>(x + i, xv);
+    }
 }
 
 // tmp = E x.  Rows are independent; a thread owns one row and all NR rhs.
-// act >= 0: skip instances with no lane in that state (uniform per segment).
+// UPDATE lanes (mixed, first gather of the iteration): tmp64 = E64 x64.
 template <typename T, int NR>
 __global__ __launch_bounds__(kBlock) void k_egather(const Seg<T>* segs, const int* bseg, const T* x, T* out,
-                                                    int act) {
+                                                    unsigned lanes, MixVec mv) {
     int lb;
     int64_t row;
     const Seg<T>& sg = seg_of(segs, bseg, lb, row);
     if (row >= sg.op.D) return;
-    if (act >= 0 && !seg_active(sg, act)) return;
+    if (lanes && !seg_any(sg, lanes)) return;
     const size_t i = (size_t)(sg.off + row * NR);
-    typename VT<T, NR>::type acc = (typename VT<T, NR>::type)(T)0;
-    if (sg.op.flag[row] & ROW_IDENTITY) {
-        acc = vload<T, NR>(x + i);
-    } else {
-        const int64_t D = sg.op.D;
-        const int* col = sg.op.ecol + row;
-        const T* val = sg.op.eval + row;
-        const T* xb = x + sg.off;
-        const int ew = sg.op.ew;
-        int s = 0;
-        for (; s + 8 <= ew; s += 8) {
-            int c[8];
-            T w[8];
-#pragma unroll
-            for (int u = 0; u < 8; ++u) { c[u] = col[(int64_t)(s + u) * D]; w[u] = val[(int64_t)(s + u) * D]; }
-#pragma unroll
-            for (int u = 0; u < 8; ++u) acc += w[u] * vload<T, NR>(xb + (size_t)c[u] * NR);
-        }
-        for (; s < ew; ++s) {
-            const int c = col[(int64_t)s * D];
-            acc += val[(int64_t)s * D] * vload<T, NR>(xb + (size_t)c * NR);
-        }
+    if (!lanes || seg_any(sg, lanes & ~M_UPDATE)) vstore<T, NR>(out + i, erow<T, NR>(sg.op, row, x + sg.off, NR));
+    if (std::is_same<T, float>::value && (lanes & M_UPDATE) && seg_any(sg, M_UPDATE)) {
+        for (int k = 0; k < sg.nr; ++k)
+            if (lane_in(sg, k, M_UPDATE)) mv.tmp64[i + k] = erow<double, 1>(sg.op64, row, mv.x64 + sg.off + k, NR);
     }
-    vstore<T, NR>(out + i, acc);
 }
 
-// out = m x + n (Ld x + Loff tmp); identity rows out = x; trailing nD rows
-// (m - n) x.  RED = 0: no reduction; 1: partial (other, out); 2: partials
-// (out,out), (out, other).  Reducing variants skip finished instances.
+// out = A x from tmp = E x.  RED = 0: no reduction; 1: partial (other, out);
+// 2: partials (out, out), (out, other).
+// Mixed, RED = 1: UPDATE lanes r64 = b - A64 x64, partial (r64, r64).
+// Mixed, RED = 2: RESCALE lanes partial (r32, r32) (rv32 = r32).
 template <typename T, int NR, int RED>
 __global__ __launch_bounds__(kBlock) void k_lfinish(const Seg<T>* segs, const int* bseg, const T* x,
-                                                    const T* tmp, T* out, const T* other, int act) {
+                                                    const T* tmp, T* out, const T* other, const T* rv32,
+                                                    unsigned lanes, MixVec mv) {
     int lb;
     int64_t row;
     const Seg<T>& sg = seg_of(segs, bseg, lb, row);
-    if (RED > 0) act = ACT_RUN;
-    if (act >= 0 && !seg_active(sg, act)) return;   // uniform per segment
+    if (lanes && !seg_any(sg, lanes)) return;   // uniform per segment
     constexpr int NV = RED == 2 ? 2 : 1;
     double pv[NV][NR];
 #pragma unroll
@@ -242,62 +357,90 @@ __global__ __launch_bounds__(kBlock) void k_lfinish(const Seg<T>* segs, const in
         for (int k = 0; k < NR; ++k) pv[a][k] = 0.0;
     if (row < sg.op.D) {
         const size_t i = (size_t)(sg.off + row * NR);
-        const int64_t D = sg.op.D;
-        const uint8_t f = sg.op.flag[row];
-        auto xv = vload<T, NR>(x + i);
-        typename VT<T, NR>::type res;
-        if (row >= D - sg.op.nD) {
-            res = (T)(sg.m - sg.n) * xv;
-        } else if (f & ROW_IDENTITY) {
-            res = xv;
-        } else {
-            typename VT<T, NR>::type lo = (typename VT<T, NR>::type)(T)0;
-            const T* tb = tmp + sg.off;
-            for (int s = 0; s < sg.op.lw; ++s) {
-                const int c = sg.op.lcol[(int64_t)s * D + row];
-                lo += sg.op.lval[(int64_t)s * D + row] * vload<T, NR>(tb + (size_t)c * NR);
-            }
-            res = (T)sg.m * xv + (T)sg.n * (sg.op.ldiag[row] * xv + lo);
-        }
-        vstore<T, NR>(out + i, res);
-        if (RED > 0) {
-            auto ov = vload<T, NR>(other + i);
+        if (!lanes || seg_any(sg, lanes & ~(M_UPDATE | M_RESCALE))) {
+            const auto res = arow<T, NR>(sg.op, sg.m, sg.n, row, vload<T, NR>(x + i), tmp + sg.off, NR);
+            vstore<T, NR>(out + i, res);
+            if (RED > 0) {
+                auto ov = vload<T, NR>(other + i);
 #pragma unroll
-            for (int k = 0; k < NR; ++k) {
-                const double rk = (double)lane(res, k);
-                if (RED == 1) {
-                    pv[0][k] = (double)lane(ov, k) * rk;
-                } else {
-                    pv[0][k] = rk * rk;
-                    pv[NV - 1][k] = rk * (double)lane(ov, k);
+                for (int k = 0; k < NR; ++k) {
+                    const double rk = (double)lane(res, k);
+                    if (RED == 1) {
+                        pv[0][k] = (double)lane(ov, k) * rk;
+                    } else {
+                        pv[0][k] = rk * rk;
+                        pv[NV - 1][k] = rk * (double)lane(ov, k);
+                    }
                 }
             }
         }
+        if (std::is_same<T, float>::value && RED == 1 && seg_any(sg, M_UPDATE)) {
+            for (int k = 0; k < sg.nr; ++k)
+                if (lane_in(sg, k, M_UPDATE)) {
+                    const double ax =
+                        arow<double, 1>(sg.op64, sg.m, sg.n, row, mv.x64[i + k], mv.tmp64 + sg.off + k, NR);
+                    const double rk = mv.b64[i + k] - ax;
+                    mv.r64[i + k] = rk;
+                    pv[0][k] = rk * rk;
+                }
+        }
+        if (std::is_same<T, float>::value && RED == 2 && seg_any(sg, M_RESCALE)) {
+            auto rr = vload<T, NR>(rv32 + i);
+#pragma unroll
+            for (int k = 0; k < NR; ++k)
+                if (lane_in(sg, k, M_RESCALE)) {
+                    const double rk = (double)lane(rr, k);
+                    pv[0][k] = rk * rk;
+                    pv[NV - 1][k] = 0.0;
+                }
+        }
     }
-    if (RED > 0) block_partials<NV, NR>(pv, sg.part + (size_t)lb * NV * NR);
+    if (RED > 0) block_partials<NV, NR>(pv, sg.part, lb, sg.nblocks);
 }
 
+// RUN lanes: s = r - alpha v ; z = P^-1 s.
+// RESCALE lanes (mixed): r32 = r64 scale ; r0 = r32 ; p = v = 0.
 template <typename T, int NR>
-__global__ __launch_bounds__(kBlock) void k_supdate(const Seg<T>* segs, const int* bseg, const T* r,
-                                                    const T* v, T* s, T* z) {
+__global__ __launch_bounds__(kBlock) void k_supdate(const Seg<T>* segs, const int* bseg, T* r, T* v, T* s, T* z,
+                                                    T* r0, T* p, MixVec mv) {
     int lb;
     int64_t row;
     const Seg<T>& sg = seg_of(segs, bseg, lb, row);
-    if (row >= sg.op.D || !seg_active(sg, ACT_RUN)) return;
+    if (row >= sg.op.D || !seg_any(sg, M_STAGE2)) return;
     const size_t i = (size_t)(sg.off + row * NR);
     const uint8_t f = sg.op.flag[row];
     auto rv = vload<T, NR>(r + i);
     auto vv = vload<T, NR>(v + i);
-    typename VT<T, NR>::type sv, zv;
+    if (seg_any(sg, M_RUN)) {
+        typename VT<T, NR>::type sv, zv;
 #pragma unroll
-    for (int k = 0; k < NR; ++k) {
-        const T a = (T)sg.st[k < sg.nr ? k : 0].alpha;
-        T sk = -a * lane(vv, k) + lane(rv, k);   // s.Copy(-alpha, v, r)
-        setlane(sv, k, sk);
-        setlane(zv, k, precond_apply(sg, row, f, sk));
+        for (int k = 0; k < NR; ++k) {
+            const T a = (T)sg.st[k < sg.nr ? k : 0].alpha;
+            T sk = -a * lane(vv, k) + lane(rv, k);   // s.Copy(-alpha, v, r)
+            setlane(sv, k, sk);
+            setlane(zv, k, precond_apply(sg, row, f, sk));
+        }
+        vstore<T, NR>(s + i, sv);
+        vstore<T, NR>(z + i, zv);
     }
-    vstore<T, NR>(s + i, sv);
-    vstore<T, NR>(z + i, zv);
+    if (std::is_same<T, float>::value && seg_any(sg, M_RESCALE)) {
+        auto r0v = vload<T, NR>(r0 + i);
+        auto pv = vload<T, NR>(p + i);
+        auto rd = vload<double, NR>(mv.r64 + i);
+#pragma unroll
+        for (int k = 0; k < NR; ++k)
+            if (lane_in(sg, k, M_RESCALE)) {
+                const T rk = (T)(lane(rd, k) * sg.st[k].scale);
+                setlane(rv, k, rk);
+                setlane(r0v, k, rk);
+                setlane(pv, k, (T)0);
+                setlane(vv, k, (T)0);
+            }
+        vstore<T, NR>(r + i, rv);
+        vstore<T, NR>(r0 + i, r0v);
+        vstore<T, NR>(p + i, pv);
+        vstore<T, NR>(v + i, vv);
+    }
 }
 
 template <typename T, int NR>
@@ -306,7 +449,7 @@ __global__ __launch_bounds__(kBlock) void k_xrupdate(const Seg<T>* segs, const i
     int lb;
     int64_t row;
     const Seg<T>& sg = seg_of(segs, bseg, lb, row);
-    if (!seg_active(sg, ACT_RUN)) return;
+    if (!seg_any(sg, M_RUN)) return;
     double pv[2][NR];
 #pragma unroll
     for (int k = 0; k < NR; ++k) { pv[0][k] = 0.0; pv[1][k] = 0.0; }
@@ -321,7 +464,7 @@ __global__ __launch_bounds__(kBlock) void k_xrupdate(const Seg<T>* segs, const i
         auto rv = vload<T, NR>(r + i);
 #pragma unroll
         for (int k = 0; k < NR; ++k) {
-            if (!lane_active(sg, k, ACT_RUN)) continue;
+            if (!lane_in(sg, k, M_RUN)) continue;
             const KState& st = sg.st[k];
             const T a = (T)st.alpha, w = (T)st.omega;
             T xk = a * lane(yv, k) + lane(xv, k);   // x.Copy(alpha, y, x)
@@ -335,7 +478,7 @@ __global__ __launch_bounds__(kBlock) void k_xrupdate(const Seg<T>* segs, const i
         vstore<T, NR>(x + i, xv);
         vstore<T, NR>(r + i, rv);
     }
-    block_partials<2, NR>(pv, sg.part + (size_t)lb * 2 * NR);
+    block_partials<2, NR>(pv, sg.part, lb, sg.nblocks);
 }
 
 // FP64 restart, BiCGStab.h:257-264: r = b - A x (A x already in ax), r0 = r; partial (r, r).
@@ -345,7 +488,7 @@ __global__ __launch_bounds__(kBlock) void k_restart(const Seg<T>* segs, const in
     int lb;
     int64_t row;
     const Seg<T>& sg = seg_of(segs, bseg, lb, row);
-    if (!seg_active(sg, ACT_RESTART)) return;
+    if (!seg_any(sg, M_RESTART)) return;
     double pv[1][NR];
 #pragma unroll
     for (int k = 0; k < NR; ++k) pv[0][k] = 0.0;
@@ -357,7 +500,7 @@ __global__ __launch_bounds__(kBlock) void k_restart(const Seg<T>* segs, const in
         auto r0v = vload<T, NR>(r0 + i);
 #pragma unroll
         for (int k = 0; k < NR; ++k) {
-            if (!lane_active(sg, k, ACT_RESTART)) continue;
+            if (!lane_in(sg, k, M_RESTART)) continue;
             T rk = lane(bv, k) - lane(av, k);
             setlane(rv, k, rk);
             setlane(r0v, k, rk);
@@ -366,10 +509,10 @@ __global__ __launch_bounds__(kBlock) void k_restart(const Seg<T>* segs, const in
         vstore<T, NR>(r + i, rv);
         vstore<T, NR>(r0 + i, r0v);
     }
-    block_partials<1, NR>(pv, sg.part + (size_t)lb * NR);
+    block_partials<1, NR>(pv, sg.part, lb, sg.nblocks);
 }
 
-// ---------------------------------------------------------------- mixed precision
+// ---------------------------------------------------------------- mixed precision start
 // x64 = 0, r64 = b ; partial (b, b)
 template <int NR>
 __global__ __launch_bounds__(kBlock) void k_mx_start(const Seg<double>* segs, const int* bseg, const double* b,
@@ -388,10 +531,10 @@ __global__ __launch_bounds__(kBlock) void k_mx_start(const Seg<double>* segs, co
 #pragma unroll
         for (int k = 0; k < NR; ++k) pv[0][k] = lane(bv, k) * lane(bv, k);
     }
-    block_partials<1, NR>(pv, sg.part + (size_t)lb * NR);
+    block_partials<1, NR>(pv, sg.part, lb, sg.nblocks);
 }
 
-// FP32 (re)start of the lanes in ST_RESCALE: r32 = r64 * scale, r0 = r32,
+// FP32 start of the lanes in ST_RESCALE: r32 = r64 * scale, r0 = r32,
 // p = v = x32 = 0 ; partial (r32, r32).
 template <int NR>
 __global__ __launch_bounds__(kBlock) void k_mx_down(const Seg<double>* segs, const int* bseg, const double* r64,
@@ -399,7 +542,7 @@ __global__ __launch_bounds__(kBlock) void k_mx_down(const Seg<double>* segs, con
     int lb;
     int64_t row;
     const Seg<double>& sg = seg_of(segs, bseg, lb, row);
-    if (!seg_active(sg, ACT_RESCALE)) return;
+    if (!seg_any(sg, M_RESCALE)) return;
     double pv[1][NR];
 #pragma unroll
     for (int k = 0; k < NR; ++k) pv[0][k] = 0.0;
@@ -413,7 +556,7 @@ __global__ __launch_bounds__(kBlock) void k_mx_down(const Seg<double>* segs, con
         auto xo = vload<float, NR>(x + i);
 #pragma unroll
         for (int k = 0; k < NR; ++k) {
-            if (!lane_active(sg, k, ACT_RESCALE)) continue;
+            if (!lane_in(sg, k, M_RESCALE)) continue;
             const float rk = (float)(lane(rv, k) * sg.st[k].scale);
             setlane(ro, k, rk);
             setlane(r0o, k, rk);
@@ -428,72 +571,46 @@ __global__ __launch_bounds__(kBlock) void k_mx_down(const Seg<double>* segs, con
         vstore<float, NR>(v + i, vo);
         vstore<float, NR>(x + i, xo);
     }
-    block_partials<1, NR>(pv, sg.part + (size_t)lb * NR);
-}
-
-// Refinement step, part 1: x64 += x32 / scale.
-template <int NR>
-__global__ __launch_bounds__(kBlock) void k_mx_upx(const Seg<double>* segs, const int* bseg, double* x64,
-                                                   const float* x32) {
-    int lb;
-    int64_t row;
-    const Seg<double>& sg = seg_of(segs, bseg, lb, row);
-    if (row >= sg.op.D || !seg_active(sg, ACT_UPDATE)) return;
-    const size_t i = (size_t)(sg.off + row * NR);
-    auto xv = vload<double, NR>(x64 + i);
-    auto dv = vload<float, NR>(x32 + i);
-#pragma unroll
-    for (int k = 0; k < NR; ++k)
-        if (lane_active(sg, k, ACT_UPDATE)) setlane(xv, k, lane(xv, k) + (double)lane(dv, k) / sg.st[k].scale);
-    vstore<double, NR>(x64 + i, xv);
-}
-
-// Refinement step, part 2: r64 = b - A64 x64 (A64 x64 in ax) ; partial (r64, r64).
-template <int NR>
-__global__ __launch_bounds__(kBlock) void k_mx_upr(const Seg<double>* segs, const int* bseg, const double* b,
-                                                   const double* ax, double* r64) {
-    int lb;
-    int64_t row;
-    const Seg<double>& sg = seg_of(segs, bseg, lb, row);
-    if (!seg_active(sg, ACT_UPDATE)) return;
-    double pv[1][NR];
-#pragma unroll
-    for (int k = 0; k < NR; ++k) pv[0][k] = 0.0;
-    if (row < sg.op.D) {
-        const size_t i = (size_t)(sg.off + row * NR);
-        auto bv = vload<double, NR>(b + i);
-        auto av = vload<double, NR>(ax + i);
-        auto rv = vload<double, NR>(r64 + i);
-#pragma unroll
-        for (int k = 0; k < NR; ++k) {
-            if (!lane_active(sg, k, ACT_UPDATE)) continue;
-            const double rk = lane(bv, k) - lane(av, k);
-            setlane(rv, k, rk);
-            pv[0][k] = rk * rk;
-        }
-        vstore<double, NR>(r64 + i, rv);
-    }
-    block_partials<1, NR>(pv, sg.part + (size_t)lb * NR);
+    block_partials<1, NR>(pv, sg.part, lb, sg.nblocks);
 }
 
 // ---------------------------------------------------------------- finalisers
 enum FinKind : int {
-    FIN_INIT = 0, FIN_ALPHA, FIN_OMEGA, FIN_ITER64, FIN_RESTART64,
-    FIN_MX_START, FIN_MX_INIT, FIN_ITER_MX, FIN_MX_UPDATE
+    FIN_INIT = 0, FIN_ALPHA, FIN_OMEGA, FIN_ITER64, FIN_RESTART64, FIN_MX_START, FIN_MX_INIT, FIN_ITER_MX
 };
 
+// Lanes each finaliser acts on (the producing kernel used the same mask) and
+// the number of partial vectors it reads.
 template <int KIND> struct FinTraits;
-template <> struct FinTraits<FIN_INIT> { static constexpr int NV = 1, ACT = ACT_ALL; };
-template <> struct FinTraits<FIN_ALPHA> { static constexpr int NV = 1, ACT = ACT_RUN; };
-template <> struct FinTraits<FIN_OMEGA> { static constexpr int NV = 2, ACT = ACT_RUN; };
-template <> struct FinTraits<FIN_ITER64> { static constexpr int NV = 2, ACT = ACT_RUN; };
-template <> struct FinTraits<FIN_RESTART64> { static constexpr int NV = 1, ACT = ACT_RESTART; };
-template <> struct FinTraits<FIN_MX_START> { static constexpr int NV = 1, ACT = ACT_ALL; };
-template <> struct FinTraits<FIN_MX_INIT> { static constexpr int NV = 1, ACT = ACT_RESCALE; };
-template <> struct FinTraits<FIN_ITER_MX> { static constexpr int NV = 2, ACT = ACT_RUN; };
-template <> struct FinTraits<FIN_MX_UPDATE> { static constexpr int NV = 1, ACT = ACT_UPDATE; };
+template <> struct FinTraits<FIN_INIT> { static constexpr int NV = 1; static constexpr unsigned M = M_ALL; };
+template <> struct FinTraits<FIN_ALPHA> { static constexpr int NV = 1; static constexpr unsigned M = M_STAGE1; };
+template <> struct FinTraits<FIN_OMEGA> { static constexpr int NV = 2; static constexpr unsigned M = M_STAGE2; };
+template <> struct FinTraits<FIN_ITER64> { static constexpr int NV = 2; static constexpr unsigned M = M_STAGE3; };
+template <> struct FinTraits<FIN_RESTART64> { static constexpr int NV = 1; static constexpr unsigned M = M_RESTART; };
+template <> struct FinTraits<FIN_MX_START> { static constexpr int NV = 1; static constexpr unsigned M = M_ALL; };
+template <> struct FinTraits<FIN_MX_INIT> { static constexpr int NV = 1; static constexpr unsigned M = M_RESCALE; };
+template <> struct FinTraits<FIN_ITER_MX> { static constexpr int NV = 2; static constexpr unsigned M = M_STAGE3; };
 
 constexpr double kEps = 2.220446049250313e-16;   // NumTraits<double>::epsilon (BiCGStab.h:250)
+
+// FP32 recurrence (re)started from the scaled true residual.
+__device__ __forceinline__ void mx_init(KState& s, double rr, int next) {
+    s.rr = s.rho = s.r0sq = s.maxrr = rr;
+    s.rho_old = s.alpha = s.omega = 1.0;
+    s.beta = s.rho;
+    s.status = (rr > 0.0 && std::isfinite(rr)) ? next : ST_BREAKDOWN;
+}
+
+// Refinement step: the true residual ||b - A64 x64||^2 is rt.
+__device__ __forceinline__ void mx_update(KState& s, double rt) {
+    ++s.updates;
+    s.rr_true = rt;
+    if (std::isnan(rt)) { s.status = ST_BREAKDOWN; return; }
+    if (!(rt > s.tol2)) { s.status = ST_CONVERGED; return; }
+    if (s.status == ST_UPDATE_MAXIT) { s.status = ST_MAXIT; return; }
+    s.scale = 1.0 / sqrt(rt);
+    s.status = ST_RESCALE;
+}
 
 // The scalar part of the recurrence for one (instance, rhs).
 template <int KIND>
@@ -512,9 +629,11 @@ __device__ void fin_scalar(KState& s, const double* tot) {
         s.beta = s.rho;                // (rho / 1) * (1 / 1)
         s.status = ST_RUN;
     } else if (KIND == FIN_ALPHA) {
+        if (s.status != ST_RUN) { mx_update(s, tot[0]); return; }
         s.alpha = s.rho / tot[0];                                      // :273
         if (!std::isfinite(s.alpha)) s.status = ST_BREAKDOWN;
     } else if (KIND == FIN_OMEGA) {
+        if (s.status != ST_RUN) { mx_init(s, tot[0], ST_READY); return; }
         s.omega = tot[0] > 0.0 ? tot[1] / tot[0] : 0.0;                // :280-288
         if (!std::isfinite(s.omega)) s.status = ST_BREAKDOWN;
     } else if (KIND == FIN_ITER64) {
@@ -543,13 +662,10 @@ __device__ void fin_scalar(KState& s, const double* tot) {
         s.tol2 *= bsq;
         s.scale = 1.0 / sqrt(bsq);
         s.status = ST_RESCALE;
-    } else if (KIND == FIN_MX_INIT) {   // FP32 recurrence (re)started from the scaled residual
-        const double rr = tot[0];
-        s.rr = s.rho = s.r0sq = s.maxrr = rr;
-        s.rho_old = s.alpha = s.omega = 1.0;
-        s.beta = s.rho;
-        s.status = (rr > 0.0 && std::isfinite(rr)) ? ST_RUN : ST_BREAKDOWN;
+    } else if (KIND == FIN_MX_INIT) {
+        mx_init(s, tot[0], ST_RUN);
     } else if (KIND == FIN_ITER_MX) {
+        if (s.status == ST_READY) { s.status = ST_RUN; return; }
         ++s.iters;
         const double rr = tot[0];
         s.rr = rr;
@@ -564,15 +680,6 @@ __device__ void fin_scalar(KState& s, const double* tot) {
         s.beta = (s.rho / s.rho_old) * (s.alpha / s.omega);
         if (!std::isfinite(s.beta)) { s.status = ST_BREAKDOWN; return; }
         if (rr > s.maxrr) s.maxrr = rr;
-    } else if (KIND == FIN_MX_UPDATE) {
-        ++s.updates;
-        const double rt = tot[0];
-        s.rr_true = rt;
-        if (std::isnan(rt)) { s.status = ST_BREAKDOWN; return; }
-        if (!(rt > s.tol2)) { s.status = ST_CONVERGED; return; }
-        if (s.status == ST_UPDATE_MAXIT) { s.status = ST_MAXIT; return; }
-        s.scale = 1.0 / sqrt(rt);
-        s.status = ST_RESCALE;
     }
 }
 
@@ -581,37 +688,32 @@ __device__ void fin_scalar(KState& s, const double* tot) {
 template <typename T, int NR, int KIND>
 __global__ __launch_bounds__(kBlock) void k_fin(const Seg<T>* segs, int nseg) {
     constexpr int NV = FinTraits<KIND>::NV;
+    constexpr unsigned M = FinTraits<KIND>::M;
     const Seg<T>& sg = segs[blockIdx.x];
-    if (!seg_active(sg, FinTraits<KIND>::ACT)) return;   // same test the producer used
-    double acc[NV][NR];
-#pragma unroll
-    for (int a = 0; a < NV; ++a)
-#pragma unroll
-        for (int k = 0; k < NR; ++k) acc[a][k] = 0.0;
-    for (int b = threadIdx.x; b < sg.nblocks; b += kBlock)
-#pragma unroll
-        for (int a = 0; a < NV; ++a)
-#pragma unroll
-            for (int k = 0; k < NR; ++k) acc[a][k] += sg.part[(size_t)b * NV * NR + a * NR + k];
-    __shared__ double red[kBlock / 64][NV * NR];
+    if (!seg_any(sg, M)) return;   // same test the producer used
+    // wave w sums components w, w + 4, ...; lanes stride over the blocks
+    __shared__ double red[NV * NR];
     const int lanei = threadIdx.x & 63, wave = threadIdx.x >> 6;
-#pragma unroll
-    for (int a = 0; a < NV; ++a)
-#pragma unroll
-        for (int k = 0; k < NR; ++k) {
-            const double w = wave_sum(acc[a][k]);
-            if (lanei == 0) red[wave][a * NR + k] = w;
+    for (int c = wave; c < NV * NR; c += kBlock / 64) {
+        const double* pc = sg.part + (size_t)c * sg.nblocks;
+        double a0 = 0.0, a1 = 0.0, a2 = 0.0, a3 = 0.0;
+        int b = lanei;
+        for (; b + 192 < sg.nblocks; b += 256) {
+            a0 += pc[b];
+            a1 += pc[b + 64];
+            a2 += pc[b + 128];
+            a3 += pc[b + 192];
         }
+        for (; b < sg.nblocks; b += 64) a0 += pc[b];
+        const double t = wave_sum((a0 + a1) + (a2 + a3));
+        if (lanei == 0) red[c] = t;
+    }
     __syncthreads();
     if (threadIdx.x != 0) return;
     for (int k = 0; k < sg.nr; ++k) {
-        if (!lane_active(sg, k, FinTraits<KIND>::ACT)) continue;
+        if (!lane_in(sg, k, M)) continue;
         double tot[NV];
-        for (int a = 0; a < NV; ++a) {
-            double s = 0.0;
-            for (int w = 0; w < kBlock / 64; ++w) s += red[w][a * NR + k];
-            tot[a] = s;
-        }
+        for (int a = 0; a < NV; ++a) tot[a] = red[a * NR + k];
         fin_scalar<KIND>(sg.st[k], tot);
     }
 }
@@ -641,14 +743,14 @@ void Launch<T, NR>::init(const LaunchCfg<T>& c, const T* b, T* x, T* r, T* r0, T
     FIN(T, NR, FIN_INIT);
 }
 template <typename T, int NR>
-void Launch<T, NR>::iteration(const LaunchCfg<T>& c, Vecs<T>& w, bool mixed) {
-    LAUNCH((k_pupdate<T, NR>), w.p, w.v, w.r, w.y);
-    LAUNCH((k_egather<T, NR>), w.y, w.tmp, (int)ACT_RUN);
-    LAUNCH((k_lfinish<T, NR, 1>), w.y, w.tmp, w.v, w.r0, (int)ACT_RUN);
+void Launch<T, NR>::iteration(const LaunchCfg<T>& c, Vecs<T>& w, bool mixed, const MixVec& mv) {
+    LAUNCH((k_pupdate<T, NR>), w.p, w.v, w.r, w.y, w.x, mv);
+    LAUNCH((k_egather<T, NR>), w.y, w.tmp, M_STAGE1, mv);
+    LAUNCH((k_lfinish<T, NR, 1>), w.y, w.tmp, w.v, w.r0, (const T*)nullptr, M_STAGE1, mv);
     FIN(T, NR, FIN_ALPHA);
-    LAUNCH((k_supdate<T, NR>), w.r, w.v, w.s, w.z);
-    LAUNCH((k_egather<T, NR>), w.z, w.tmp, (int)ACT_RUN);
-    LAUNCH((k_lfinish<T, NR, 2>), w.z, w.tmp, w.t, w.s, (int)ACT_RUN);
+    LAUNCH((k_supdate<T, NR>), w.r, w.v, w.s, w.z, w.r0, w.p, mv);
+    LAUNCH((k_egather<T, NR>), w.z, w.tmp, M_RUN, mv);
+    LAUNCH((k_lfinish<T, NR, 2>), w.z, w.tmp, w.t, w.s, (const T*)w.r, M_STAGE2, mv);
     FIN(T, NR, FIN_OMEGA);
     LAUNCH((k_xrupdate<T, NR>), w.x, w.y, w.z, w.r, w.s, w.t, w.r0);
     if (mixed) FIN(T, NR, FIN_ITER_MX);
@@ -656,20 +758,23 @@ void Launch<T, NR>::iteration(const LaunchCfg<T>& c, Vecs<T>& w, bool mixed) {
 }
 template <typename T, int NR>
 void Launch<T, NR>::restart(const LaunchCfg<T>& c, Vecs<T>& w) {
-    LAUNCH((k_egather<T, NR>), w.x, w.tmp, (int)ACT_RESTART);
-    LAUNCH((k_lfinish<T, NR, 0>), w.x, w.tmp, w.t, (const T*)nullptr, (int)ACT_RESTART);
+    const MixVec none{};
+    LAUNCH((k_egather<T, NR>), w.x, w.tmp, M_RESTART, none);
+    LAUNCH((k_lfinish<T, NR, 0>), w.x, w.tmp, w.t, (const T*)nullptr, (const T*)nullptr, M_RESTART, none);
     LAUNCH((k_restart<T, NR>), w.b, w.t, w.r, w.r0);
     FIN(T, NR, FIN_RESTART64);
 }
 template <typename T, int NR>
-void Launch<T, NR>::apply(const LaunchCfg<T>& c, const T* x, T* tmp, T* y, int act) {
-    LAUNCH((k_egather<T, NR>), x, tmp, act);
-    LAUNCH((k_lfinish<T, NR, 0>), x, tmp, y, (const T*)nullptr, act);
+void Launch<T, NR>::apply(const LaunchCfg<T>& c, const T* x, T* tmp, T* y, unsigned lanes) {
+    const MixVec none{};
+    LAUNCH((k_egather<T, NR>), x, tmp, lanes, none);
+    LAUNCH((k_lfinish<T, NR, 0>), x, tmp, y, (const T*)nullptr, (const T*)nullptr, lanes, none);
 }
 
 template <typename T, int NR>
 void launch_egather(const LaunchCfg<T>& c, const T* x, T* out) {
-    LAUNCH((k_egather<T, NR>), x, out, -1);
+    const MixVec none{};
+    LAUNCH((k_egather<T, NR>), x, out, 0u, none);
 }
 
 template <int NR>
@@ -681,15 +786,6 @@ template <int NR>
 void MixedLaunch<NR>::down(const LaunchCfg<double>& c, const double* r64, Vecs<float>& w) {
     LAUNCH((k_mx_down<NR>), r64, w.r, w.r0, w.p, w.v, w.x);
     FIN(double, NR, FIN_MX_INIT);
-}
-template <int NR>
-void MixedLaunch<NR>::update(const LaunchCfg<double>& c, const double* b, double* x64, double* r64, double* ax,
-                             double* tmp, Vecs<float>& w) {
-    LAUNCH((k_mx_upx<NR>), x64, (const float*)w.x);
-    Launch<double, NR>::apply(c, x64, tmp, ax, (int)ACT_UPDATE);
-    LAUNCH((k_mx_upr<NR>), b, ax, r64);
-    FIN(double, NR, FIN_MX_UPDATE);
-    down(c, r64, w);
 }
 
 #define CPM_INST(T, NR) template struct Launch<T, NR>; template void launch_egather<T, NR>(const LaunchCfg<T>&, const T*, T*);

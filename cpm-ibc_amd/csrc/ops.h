@@ -28,8 +28,9 @@ struct Ctx {
     double* workspace(const char* name, size_t bytes);
 };
 
-// Operator in device memory: E and L(off-diagonal) as slot-major ELL with
-// int32 columns shared by the FP64 and FP32 value arrays.
+// Operator in device memory, in one of two execution formats (see OpView):
+// slot-major ELL with int32 columns shared by the FP64 and FP32 value arrays,
+// or the tensor-product format of tube-built operators.
 struct Op {
     Ctx* ctx = nullptr;
     int64_t D = 0;
@@ -38,6 +39,12 @@ struct Op {
     double m = 0, n = 0;
     int precond = 1, precond_iters = 0;
     Buffer ecol, eval64, eval32, lcol, lval64, lval32, ldiag64, ldiag32, flag;
+    // OP_TP
+    int fmt = OP_ELL;
+    Buffer perm;              // int32 [D]: internal row -> caller row (empty = identity)
+    Buffer tpc, tpw64, tpw32, exc, emask, eoff, ecolx, xcol, xval64, xval32;
+    int64_t nexc = 0, nmoved = 0;
+    double loff = 0, ldu = 0;
     Op() = default;
     Op(Ctx* c, int64_t D_, int nD_);
     void upload_ell(const std::vector<int>& ec, const std::vector<double>& ev, int ew,

@@ -1,0 +1,320 @@
+// Tensor-product execution format of operators built on a tube.
+//
+// The reference stores E as a general sparse matrix: 64 (column, value)
+// pairs per row for p = 3 (Interpolation.cpp:204-225), i.e. 768 bytes of
+// FP64 per row, read twice per BiCGStab iteration.  Every E row is in fact
+// the tensor product of three 1-D barycentric weight vectors over the 4x4x4
+// grid nodes b + (i, j, k).  Once the tube's rows are renumbered in
+// lexicographic grid order (z fastest), the nodes b + (i, j, 0..3) of a
+// stencil line are four consecutive rows, so a row is described by 16 line
+// starts and 12 weights (160 bytes in FP64, 112 in FP32), plus an exception
+// record where an interior boundary moved some of its columns (PoissonWithIBC
+// MoveToIBCColumns) or appended the Dirichlet identity column.
+//
+// The builder derives the format from the reference-order ELL rows and the
+// rows' query points, and accepts an operator only if every row is
+// reproduced exactly: the same columns and bit-identical FP64 values
+// (w_z w_y) w_x, the order of Interpolation.cpp:95-107.  Otherwise the caller
+// keeps the ELL format.
+#include <hip/hip_runtime.h>
+
+#include "tube.h"
+
+namespace cpmgpu {
+
+namespace {
+
+constexpr int kS = 4;   // p = 3
+
+__global__ void k_lex_count(const int32_t* gidx, int64_t G, int* bcount) {
+    const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    const int32_t v = i < G ? gidx[i] : -1;
+    const int occ = (v >= 0 && v < kCand) ? 1 : 0;
+    const unsigned long long m = __ballot(occ);
+    __shared__ int ws[4];
+    if ((threadIdx.x & 63) == 0) ws[threadIdx.x >> 6] = __popcll(m);
+    __syncthreads();
+    if (threadIdx.x == 0) bcount[blockIdx.x] = ws[0] + ws[1] + ws[2] + ws[3];
+}
+
+// Tube nodes in linear grid order: lex[pos] = tube index, rlex[tube index] = pos.
+__global__ void k_lex_emit(const int32_t* gidx, int64_t G, const int* boff, int64_t N, int32_t* lex, int32_t* rlex,
+                           int* err) {
+    const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    const int32_t v = i < G ? gidx[i] : -1;
+    const int occ = (v >= 0 && v < kCand) ? 1 : 0;
+    const unsigned long long m = __ballot(occ);
+    __shared__ int ws[4];
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    if (lane == 0) ws[w] = __popcll(m);
+    __syncthreads();
+    if (!occ) return;
+    int pos = boff[blockIdx.x];
+    for (int k = 0; k < w; ++k) pos += ws[k];
+    pos += __popcll(m & ((1ull << lane) - 1ull));
+    if (pos >= N || v >= N) { atomicOr(err, 1); return; }
+    lex[pos] = v;
+    rlex[v] = pos;
+}
+
+struct TpArgs {
+    int64_t D, N;
+    const int* ecol;
+    const double* eval;
+    int ew;
+    const int* lcol;
+    const double* lval;
+    int lw;
+    const double* ldiag;
+    const uint8_t* flag;
+    const double* qpts;
+    const int32_t* gidx;
+    GridDev g;
+    const int32_t* lex;
+    const int32_t* rlex;
+    double loff, ldu;
+};
+
+__device__ __forceinline__ int64_t to_internal(const TpArgs& a, int64_t c) { return c < a.N ? a.rlex[c] : c; }
+__device__ __forceinline__ int64_t to_reference(const TpArgs& a, int64_t r) { return r < a.N ? a.lex[r] : r; }
+
+// Error bits of the builder (any bit: the operator keeps the ELL format).
+enum : int {
+    TP_STENCIL = 2, TP_OUTSIDE = 4, TP_GAP = 8, TP_VALUE = 16, TP_TAIL = 32, TP_LVAL = 64, TP_LDIAG = 128
+};
+
+// Pass 1, one thread per internal row: line starts, weights, L columns, the
+// row's moved-slot mask and its exception counts.
+__global__ void k_tp_rows(TpArgs a, int* tpc, double* tpw64, float* tpw32, int* lcol_out, uint8_t* flag_out,
+                          unsigned long long* mask_out, int* n_exc, int* n_mov, int* err) {
+    const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= a.D) return;
+    const int64_t D = a.D;
+    const int64_t R = to_reference(a, r);
+    const uint8_t f = a.flag[R];
+    flag_out[r] = f;
+    mask_out[r] = 0;
+    n_exc[r] = 0;
+    n_mov[r] = 0;
+    if (f & ROW_IDENTITY) {
+        for (int l = 0; l < kS * kS; ++l) tpc[l * D + r] = 0;
+        for (int q = 0; q < 3 * kS; ++q) { tpw64[q * D + r] = 0.0; tpw32[q * D + r] = 0.0f; }
+        for (int s = 0; s < 6; ++s) lcol_out[s * D + r] = -1;
+        return;
+    }
+    if (f & ROW_NODIAG) { atomicOr(err, TP_LDIAG); return; }
+    int64_t b[3];
+    double w[3][5];
+    if (!stencil_weights(a.g, kS - 1, a.qpts + 3 * R, b, w)) { atomicOr(err, TP_STENCIL); return; }
+    const int64_t N1 = a.g.N1d;
+    // sign of the row (ghost rows of Dirichlet curves are negated)
+    double sign = 0.0;
+    for (int s = 0; s < kS * kS * kS && sign == 0.0; ++s) {
+        const int i = s / 16, j = (s / 4) % 4, k = s % 4;
+        const double v = w[2][k] * w[1][j] * w[0][i];
+        if (v == 0.0) continue;
+        const double e = a.eval[s * D + R];
+        sign = e == v ? 1.0 : (e == -v ? -1.0 : 2.0);
+    }
+    if (sign != 1.0 && sign != -1.0) { atomicOr(err, TP_VALUE); return; }
+    unsigned long long mask = 0;
+    int moved = 0;
+    for (int i = 0; i < kS; ++i)
+        for (int j = 0; j < kS; ++j) {
+            int64_t c0 = -1;
+            for (int k = 0; k < kS; ++k) {
+                const int s = (i * kS + j) * kS + k;
+                const int32_t gv = a.gidx[((b[0] + i) * N1 + (b[1] + j)) * N1 + (b[2] + k)];
+                if (!(gv >= 0 && gv < a.N)) { atomicOr(err, TP_OUTSIDE); return; }
+                const int64_t ci = a.rlex[gv];
+                if (k == 0) c0 = ci;
+                else if (ci != c0 + k) { atomicOr(err, TP_GAP); return; }
+                const double v = w[2][k] * w[1][j] * (sign * w[0][i]);
+                if (!(a.eval[s * D + R] == v)) { atomicOr(err, TP_VALUE); return; }
+                if (a.ecol[s * D + R] != gv) { mask |= 1ull << s; ++moved; }
+            }
+            tpc[(i * kS + j) * D + r] = (int)c0;
+        }
+    for (int q = 0; q < kS; ++q) {
+        const double wx = sign * w[0][q];
+        tpw64[q * D + r] = wx;
+        tpw64[(kS + q) * D + r] = w[1][q];
+        tpw64[(2 * kS + q) * D + r] = w[2][q];
+        tpw32[q * D + r] = (float)wx;
+        tpw32[(kS + q) * D + r] = (float)w[1][q];
+        tpw32[(2 * kS + q) * D + r] = (float)w[2][q];
+    }
+    bool extra = false;
+    for (int s = kS * kS * kS; s < a.ew; ++s) {
+        const double v = a.eval[s * D + R];
+        const int c = a.ecol[s * D + R];
+        if (v == 0.0 && c == R) continue;                 // padding
+        if (s == kS * kS * kS) { extra = true; continue; }
+        atomicOr(err, TP_TAIL);
+        return;
+    }
+    for (int s = 0; s < 6; ++s) {
+        int out = -1;
+        if (s < a.lw) {
+            const double v = a.lval[s * D + R];
+            const int c = a.lcol[s * D + R];
+            if (!(v == 0.0 && c == R)) {
+                if (!(v == a.loff)) { atomicOr(err, TP_LVAL); return; }
+                out = (int)to_internal(a, c);
+            }
+        }
+        lcol_out[s * D + r] = out;
+    }
+    if (!(a.ldiag[R] == a.ldu)) { atomicOr(err, TP_LDIAG); return; }
+    mask_out[r] = mask;
+    n_exc[r] = (moved > 0 || extra) ? 1 : 0;
+    n_mov[r] = moved;
+}
+
+// Pass 2: exception records (n_exc / n_mov now hold exclusive offsets).
+__global__ void k_tp_exc(TpArgs a, const unsigned long long* mask_in, const int* exc_off, const int* mov_off,
+                         int64_t nexc_total, int* exc, unsigned long long* emask, int* eoff, int* ecolx, int* xcol,
+                         double* xval64, float* xval32) {
+    const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= a.D) return;
+    const int64_t D = a.D;
+    const int64_t R = to_reference(a, r);
+    const int e = exc_off[r];
+    const int e_next = r + 1 < a.D ? exc_off[r + 1] : (int)nexc_total;
+    if (e_next == e || (a.flag[R] & ROW_IDENTITY)) { exc[r] = -1; return; }
+    exc[r] = e;
+    const unsigned long long mask = mask_in[r];
+    emask[e] = mask;
+    eoff[e] = mov_off[r];
+    int m = mov_off[r];
+    for (int s = 0; s < kS * kS * kS; ++s)
+        if ((mask >> s) & 1ull) ecolx[m++] = (int)to_internal(a, a.ecol[s * D + R]);
+    xcol[e] = -1;
+    xval64[e] = 0.0;
+    xval32[e] = 0.0f;
+    if (a.ew > kS * kS * kS) {
+        const int s = kS * kS * kS;
+        const double v = a.eval[s * D + R];
+        const int c = a.ecol[s * D + R];
+        if (!(v == 0.0 && c == R)) {
+            xcol[e] = (int)to_internal(a, c);
+            xval64[e] = v;
+            xval32[e] = (float)v;
+        }
+    }
+}
+
+__global__ void k_perm_full(const int32_t* lex, int64_t N, int64_t D, int32_t* perm) {
+    const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (r < D) perm[r] = r < N ? lex[r] : (int32_t)r;
+}
+
+inline unsigned nblk(int64_t n) { return (unsigned)((n + 255) / 256); }
+
+}  // namespace
+
+static void ensure_lex(Tube& t) {
+    if (t.lex.p) return;
+    const hipStream_t st = t.ctx->stream;
+    const int64_t G = t.N1d * t.N1d * t.N1d;
+    const int64_t nb = (G + 255) / 256;
+    Buffer bcount((size_t)(nb + 1) * 4), flags(64);
+    Buffer lex((size_t)t.N * 4), rlex((size_t)t.N * 4);
+    int* total = bcount.as<int>() + nb;
+    HIPCHK(hipMemsetAsync(flags.p, 0, 64, st));
+    hipLaunchKernelGGL(k_lex_count, dim3((unsigned)nb), dim3(256), 0, st, t.gidx.as<int32_t>(), G, bcount.as<int>());
+    hipLaunchKernelGGL(k_scan, dim3(1), dim3(256), 0, st, bcount.as<int>(), nb, total);
+    hipLaunchKernelGGL(k_lex_emit, dim3((unsigned)nb), dim3(256), 0, st, t.gidx.as<int32_t>(), G, bcount.as<int>(),
+                       t.N, lex.as<int32_t>(), rlex.as<int32_t>(), flags.as<int>());
+    HIPCHK(hipGetLastError());
+    int h[2] = {0, 0};
+    HIPCHK(hipMemcpyAsync(&h[0], total, 4, hipMemcpyDeviceToHost, st));
+    HIPCHK(hipMemcpyAsync(&h[1], flags.p, 4, hipMemcpyDeviceToHost, st));
+    HIPCHK(hipStreamSynchronize(st));
+    if (h[0] != t.N || h[1]) throw std::logic_error("tube index grid is inconsistent with the node list");
+    t.lex = std::move(lex);
+    t.rlex = std::move(rlex);
+}
+
+bool op_build_tp(Op& op, Tube& t, const TpSource& src) {
+    if (t.p != 3 || t.dim != 3 || src.ew < kS * kS * kS || src.lw > 6 || op.nD != 0) return false;
+    if (src.D > (int64_t)INT32_MAX / 2) return false;
+    ensure_lex(t);
+    const hipStream_t st = t.ctx->stream;
+    const int64_t D = src.D;
+    TpArgs a{};
+    a.D = D;
+    a.N = src.N;
+    a.ecol = src.ecol;
+    a.eval = src.eval;
+    a.ew = src.ew;
+    a.lcol = src.lcol;
+    a.lval = src.lval;
+    a.lw = src.lw;
+    a.ldiag = src.ldiag;
+    a.flag = src.flag;
+    a.qpts = src.qpts;
+    a.gidx = t.gidx.as<int32_t>();
+    a.g = GridDev{t.N1d, t.xstart, t.dx, t.r_t};
+    a.lex = t.lex.as<int32_t>();
+    a.rlex = t.rlex.as<int32_t>();
+    a.loff = 1.0 / (t.dx * t.dx);          // FDMatrices.cpp:25-49: 1 / h^2 off the diagonal
+    a.ldu = -6.0 / (t.dx * t.dx);          // -2 dim / h^2 on it
+
+    Buffer tpc((size_t)16 * D * 4), tpw64((size_t)12 * D * 8), tpw32((size_t)12 * D * 4), lcol((size_t)6 * D * 4),
+        flag((size_t)D), mask((size_t)D * 8), nexc((size_t)(D + 1) * 4), nmov((size_t)(D + 1) * 4), flags(64),
+        perm((size_t)D * 4), exc((size_t)D * 4);
+    HIPCHK(hipMemsetAsync(flags.p, 0, 64, st));
+    int* tot = flags.as<int>() + 4;
+    hipLaunchKernelGGL(k_tp_rows, dim3(nblk(D)), dim3(256), 0, st, a, tpc.as<int>(), tpw64.as<double>(),
+                       tpw32.as<float>(), lcol.as<int>(), flag.as<uint8_t>(), mask.as<unsigned long long>(),
+                       nexc.as<int>(), nmov.as<int>(), flags.as<int>());
+    hipLaunchKernelGGL(k_scan, dim3(1), dim3(256), 0, st, nexc.as<int>(), D, tot);
+    hipLaunchKernelGGL(k_scan, dim3(1), dim3(256), 0, st, nmov.as<int>(), D, tot + 1);
+    HIPCHK(hipGetLastError());
+    int h[6] = {0};
+    HIPCHK(hipMemcpyAsync(h, flags.p, 24, hipMemcpyDeviceToHost, st));
+    HIPCHK(hipStreamSynchronize(st));
+    if (h[0]) return false;
+    const int64_t ne = h[4], nm = h[5];
+    Buffer emask((size_t)std::max<int64_t>(ne, 1) * 8), eoff((size_t)std::max<int64_t>(ne, 1) * 4),
+        ecolx((size_t)std::max<int64_t>(nm, 1) * 4), xcol((size_t)std::max<int64_t>(ne, 1) * 4),
+        xval64((size_t)std::max<int64_t>(ne, 1) * 8), xval32((size_t)std::max<int64_t>(ne, 1) * 4);
+    hipLaunchKernelGGL(k_tp_exc, dim3(nblk(D)), dim3(256), 0, st, a, mask.as<unsigned long long>(), nexc.as<int>(),
+                       nmov.as<int>(), ne, exc.as<int>(), emask.as<unsigned long long>(), eoff.as<int>(),
+                       ecolx.as<int>(), xcol.as<int>(), xval64.as<double>(), xval32.as<float>());
+    hipLaunchKernelGGL(k_perm_full, dim3(nblk(D)), dim3(256), 0, st, t.lex.as<int32_t>(), src.N, D, perm.as<int32_t>());
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipStreamSynchronize(st));
+
+    op.fmt = OP_TP;
+    op.ew = kS * kS * kS;
+    op.lw = 6;
+    op.loff = a.loff;
+    op.ldu = a.ldu;
+    op.nexc = ne;
+    op.nmoved = nm;
+    op.tpc = std::move(tpc);
+    op.tpw64 = std::move(tpw64);
+    op.tpw32 = std::move(tpw32);
+    op.lcol = std::move(lcol);
+    op.flag = std::move(flag);
+    op.perm = std::move(perm);
+    op.exc = std::move(exc);
+    op.emask = std::move(emask);
+    op.eoff = std::move(eoff);
+    op.ecolx = std::move(ecolx);
+    op.xcol = std::move(xcol);
+    op.xval64 = std::move(xval64);
+    op.xval32 = std::move(xval32);
+    op.ecol = Buffer();
+    op.eval64 = Buffer();
+    op.eval32 = Buffer();
+    op.lval64 = Buffer();
+    op.lval32 = Buffer();
+    op.ldiag64 = Buffer();
+    op.ldiag32 = Buffer();
+    return true;
+}
+
+}  // namespace cpmgpu

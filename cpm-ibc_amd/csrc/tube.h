@@ -38,6 +38,11 @@ struct SurfaceHost {
 SurfaceHost surface_from_desc(const char* type, const double* params, int nparams, double bb0, double bb1, int dim,
                               const double* verts, int64_t nverts);
 
+struct GridDev {
+    int64_t N1d;
+    double xstart, dx, r_t;
+};
+
 struct Tube {
     Ctx* ctx = nullptr;
     SurfaceHost surf;
@@ -55,6 +60,7 @@ struct Tube {
     Buffer bpt;         // int32 [N]
     int64_t levels = 0;
     double build_ms = 0;
+    Buffer lex, rlex;   // int32 [N]: lexicographic position -> tube index and back (built on demand)
 };
 
 Tube* tube_build(Ctx* ctx, const SurfaceHost& s, int p, double dx);
@@ -68,6 +74,9 @@ __global__ void k_count(const int32_t* slot, int64_t ns, int* bcount);
 __global__ void k_scan(int* bcount, int64_t nb, int* total);
 __device__ void cp_surface(const SurfDev& s, const double* x, double* c, double& dist, int& bdy);
 __device__ void weights_1d(const double* xs, double xq, int S, double* w);
+// Base point b (grid coordinates) and 1-D barycentric weights of the degree-p
+// stencil at xq (Interpolation.cpp:52-121); false when it leaves the grid.
+__device__ bool stencil_weights(const GridDev& g, int p, const double* xq, int64_t* b, double (*w)[5]);
 SurfDev make_surfdev(const SurfaceHost& s, const double* dverts);
 
 // E rows at query points xq (device, 3 doubles each) as slot-major ELL of width
@@ -78,5 +87,26 @@ void build_E_ell(const Tube& t, const double* d_xq, int64_t nq, int* ecol, doubl
 
 // Operator (m I + n (Ld + Loff E)) of the plain closest-point problem on a GPU tube.
 Op* op_from_tube(Tube* t, double m, double n, int precond, int precond_iters);
+
+// Reference-order operator rows handed to the tensor-product format builder.
+struct TpSource {
+    int64_t D = 0, N = 0;                     // rows; the first N are tube nodes
+    const int* ecol = nullptr;                // E as slot-major ELL, ld = D, tube-order columns
+    const double* eval = nullptr;
+    int ew = 0;
+    const int* lcol = nullptr;                // L off-diagonal ELL, ld = D
+    const double* lval = nullptr;
+    int lw = 0;
+    const double* ldiag = nullptr;
+    const uint8_t* flag = nullptr;
+    const double* qpts = nullptr;             // [3 * D] query point of each non-identity E row
+};
+
+// Give op the tensor-product execution format (rows reordered lexicographically
+// on the tube).  Returns false, leaving op untouched, when some row is not a
+// tensor-product stencil with uniform L values (the caller keeps ELL).
+bool op_build_tp(Op& op, Tube& t, const TpSource& src);
+// CPMGPU_OP_FORMAT=ell keeps tube-built operators in the ELL format.
+bool tp_disabled();
 
 }  // namespace cpmgpu

@@ -19,6 +19,7 @@
 
 #include <chrono>
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 
 #include "tube.h"
@@ -144,11 +145,6 @@ __device__ void cp_surface(const SurfDev& s, const double* x, double* c, double&
 }
 
 // ---------------------------------------------------------------- grid helpers
-struct GridDev {
-    int64_t N1d;
-    double xstart, dx, r_t;
-};
-
 __device__ __forceinline__ void trip_of(int64_t l, int64_t N1d, int64_t* t) {
     t[2] = l % N1d;
     t[1] = (l / N1d) % N1d;
@@ -340,22 +336,28 @@ __device__ void weights_1d(const double* xs, double xq, int S, double* w) {
 // E rows at query points (Interpolation.cpp:52-121, 204-225): base point,
 // 1-D weights per axis, value (w_z w_y) w_x, column = tube index of
 // (b + (i, j, k)); x outer, z inner.
+__device__ bool stencil_weights(const GridDev& g, int p, const double* xq, int64_t* b, double (*w)[5]) {
+    const int S = p + 1;
+    const int64_t shift = (p % 2 == 0) ? p / 2 : (p - 1) / 2;
+    for (int d = 0; d < 3; ++d) {
+        const double s = (xq[d] - g.xstart) / g.dx;
+        b[d] = (int64_t)((p % 2 == 0) ? rint(s) : floor(s)) - shift;
+        if (!(b[d] >= 0 && b[d] + S <= g.N1d)) return false;
+        double xs[5];
+        for (int i = 0; i < S; ++i) xs[i] = g.xstart + (b[d] + i) * g.dx;
+        weights_1d(xs, xq[d], S, w[d]);
+    }
+    return true;
+}
+
 __global__ void k_erows(int64_t nq, const double* xq, const int32_t* gidx, GridDev g, int p, int* ecol, double* eval,
                         float* eval32, int64_t ld, int* err) {
     const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (q >= nq) return;
     const int S = p + 1;
-    const int64_t shift = (p % 2 == 0) ? p / 2 : (p - 1) / 2;
     int64_t b[3];
     double w[3][5];
-    for (int d = 0; d < 3; ++d) {
-        const double s = (xq[3 * q + d] - g.xstart) / g.dx;
-        b[d] = (int64_t)((p % 2 == 0) ? rint(s) : floor(s)) - shift;
-        if (b[d] < 0 || b[d] + S > g.N1d) { atomicOr(err, 8); return; }
-        double xs[5];
-        for (int i = 0; i < S; ++i) xs[i] = g.xstart + (b[d] + i) * g.dx;
-        weights_1d(xs, xq[3 * q + d], S, w[d]);
-    }
+    if (!stencil_weights(g, p, xq + 3 * q, b, w)) { atomicOr(err, 8); return; }
     int s = 0;
     for (int i = 0; i < S; ++i)
         for (int j = 0; j < S; ++j)
@@ -591,6 +593,11 @@ void build_E_ell(const Tube& t, const double* d_xq, int64_t nq, int* ecol, doubl
     if (h) throw GeomError("interpolation stencil leaves the tube (base point not within the tube)");
 }
 
+bool tp_disabled() {
+    const char* e = std::getenv("CPMGPU_OP_FORMAT");
+    return e && std::string(e) == "ell";
+}
+
 Op* op_from_tube(Tube* t, double m, double n, int precond, int precond_iters) {
     const hipStream_t st = t->ctx->stream;
     const int64_t N = t->N;
@@ -618,6 +625,18 @@ Op* op_from_tube(Tube* t, double m, double n, int precond, int precond_iters) {
     op->n = n;
     op->precond = precond;
     op->precond_iters = precond_iters;
+    TpSource src;
+    src.D = src.N = N;
+    src.ecol = op->ecol.as<int>();
+    src.eval = op->eval64.as<double>();
+    src.ew = op->ew;
+    src.lcol = op->lcol.as<int>();
+    src.lval = op->lval64.as<double>();
+    src.lw = op->lw;
+    src.ldiag = op->ldiag64.as<double>();
+    src.flag = op->flag.as<uint8_t>();
+    src.qpts = t->cpx.as<double>();
+    if (!tp_disabled()) op_build_tp(*op, *t, src);   // ELL stays when a row does not fit
     return op.release();
 }
 

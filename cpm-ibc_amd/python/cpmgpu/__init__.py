@@ -19,7 +19,7 @@ STATUS = {1: "converged", 2: "maxit", 4: "zero_rhs", 5: "breakdown"}
 
 EXPORTS = [
     "cpm_solve_opts_default", "cpm_last_error", "cpm_ctx_create", "cpm_ctx_destroy", "cpm_ctx_synchronize",
-    "cpm_ctx_device_bytes", "cpm_op_create_csr", "cpm_op_destroy", "cpm_op_rows", "cpm_op_set_mn",
+    "cpm_ctx_device_bytes", "cpm_op_create_csr", "cpm_op_destroy", "cpm_op_rows", "cpm_op_format", "cpm_op_set_mn",
     "cpm_op_set_precond", "cpm_multiply", "cpm_multiply_device", "cpm_solve", "cpm_solve_device",
     "cpm_solve_batch", "cpm_solve_batch_device", "cpm_device_alloc", "cpm_device_free", "cpm_memcpy_h2d",
     "cpm_memcpy_d2h", "cpm_tube_create", "cpm_tube_destroy", "cpm_tube_get_info", "cpm_tube_export",
@@ -100,6 +100,8 @@ def load(path=LIB_PATH):
     L.cpm_op_destroy.argtypes = [vp]
     L.cpm_op_rows.argtypes = [vp]
     L.cpm_op_rows.restype = ctypes.c_int64
+    L.cpm_op_format.argtypes = [vp]
+    L.cpm_op_format.restype = ctypes.c_int
     L.cpm_op_set_mn.argtypes = [vp, ctypes.c_double, ctypes.c_double]
     L.cpm_op_set_precond.argtypes = [vp, ctypes.c_int, ctypes.c_int]
     L.cpm_multiply.argtypes = [vp, vp, vp, ctypes.c_double, ctypes.c_double]
@@ -247,6 +249,11 @@ class Operator:
             self.close()
         except Exception:
             pass
+
+    @property
+    def format(self):
+        """'ell' (general sparse rows) or 'tp' (tensor-product stencils of a tube-built operator)."""
+        return {0: "ell", 1: "tp"}[load().cpm_op_format(self._h)]
 
     def set_mn(self, m, n):
         _chk(load().cpm_op_set_mn(self._h, m, n))

@@ -98,6 +98,11 @@ int cpm_op_create_csr(cpm_ctx* ctx, int64_t D, int nD,
                       int precond, int precond_iters, cpm_op** out);
 int cpm_op_destroy(cpm_op* op);
 int64_t cpm_op_rows(const cpm_op* op);
+/* Execution format of an operator: 0 general ELL, 1 tensor-product stencils
+ * (tube-built operators; an internal row order, invisible to callers).  No
+ * counterpart in the reference, which stores E as a general sparse matrix
+ * (src/Interpolation.cpp:204-225). */
+int cpm_op_format(const cpm_op* op);
 /* Time stepping re-uses the operator with new (m, n) (problems/HeatWithIBC.cpp:166-167). */
 int cpm_op_set_mn(cpm_op* op, double m, double n);
 /* Replaces cpm::Solver::SetPrecond(precond_iters) (src/Solver.h:25). */

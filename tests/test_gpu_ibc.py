@@ -141,3 +141,37 @@ def test_torus_knot_config2(gpu_ctx):
     # reference's own 1e-10 stop
     assert rel_res <= 1e-10
     assert rel32 <= 1e-8
+
+
+@pytest.mark.parametrize("kw", [dict(boundary_order=2, boundary_type=(0, 0)),
+                                dict(boundary_order=1, boundary_type=(0, 0)),
+                                dict(boundary_order=2, boundary_type=(1, 1))])
+def test_tensor_product_format(gpu_ctx, monkeypatch, kw):
+    """The tensor-product execution format (lexicographic rows, moved IBC columns
+    as exceptions, negated ghost rows, the Dirichlet identity column) applies the
+    same operator as the ELL rows it was derived from, and as the oracle."""
+    print(f"case tp format {kw}", flush=True)
+    dx = 0.1
+    ibc = dict(type="Circle3D", params=circle3d_params(0), on_ibc_tolerance=0.1 * dx * dx, **kw)
+    tg = cpmgpu.Tube(gpu_ctx, "Sphere", dx)
+    Q = cpmgpu.Problem(tg, [cpmgpu.IBC(**ibc)])
+    op_tp = Q.operator(1.0, -1.0)
+    monkeypatch.setenv("CPMGPU_OP_FORMAT", "ell")
+    op_ell = Q.operator(1.0, -1.0)
+    assert (op_tp.format, op_ell.format) == ("tp", "ell")
+    P = orc.IBCProblem("Sphere", dx, ibcs=[orc.IBC(**ibc)])
+    x = np.random.default_rng(12).standard_normal(Q.D)
+    for m, n in ((1.0, -1.0), (0.0, 1.0)):
+        y_tp, y_ell = op_tp.multiply(x, m, n), op_ell.multiply(x, m, n)
+        scale = np.abs(y_ell).max()
+        # same slot order and the same FP64 products: equal up to the sign of zero
+        assert np.abs(y_tp - y_ell).max() <= 1e-15 * scale
+        y_o = orc.multiply(P.L(), P.E(), P.ident(), x, m, n)
+        assert np.abs(y_tp - y_o).max() <= 1e-12 * scale
+    b = np.random.default_rng(13).standard_normal(Q.D)
+    for prec in (cpmgpu.CPM_FP64, cpmgpu.CPM_FP32_REFINE):
+        o = cpmgpu.opts(tol=1e-11, precision=prec)
+        x1, i1 = op_tp.solve(b, o)
+        x2, i2 = op_ell.solve(b, o)
+        assert i1["status"] == i2["status"] == "converged"
+        assert np.linalg.norm(x1 - x2) <= 1e-8 * np.linalg.norm(x2)

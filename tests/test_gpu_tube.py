@@ -97,3 +97,26 @@ def test_solve_on_gpu_tube_matches_oracle(gpu_ctx, prec):
     rel = np.linalg.norm(x - x_ref) / np.linalg.norm(x_ref)
     assert info["status"] == "converged"
     assert rel <= (1e-10 if prec == cpmgpu.CPM_FP64 else 1e-8), rel
+
+
+@pytest.mark.parametrize("surface,dx,params,bbox", [("Torus", 0.1, None, None), ("Sphere", 0.05, None, None)])
+def test_tensor_product_format(gpu_ctx, monkeypatch, surface, dx, params, bbox):
+    """Tube operators run in the tensor-product format; its result equals the ELL
+    rows it was derived from and the oracle's operator."""
+    print(f"case tp format {surface} dx={dx}", flush=True)
+    tg = cpmgpu.Tube(gpu_ctx, surface, dx, params=params, bbox=bbox)
+    op_tp = tg.operator(1.0, -1.0)
+    monkeypatch.setenv("CPMGPU_OP_FORMAT", "ell")
+    op_ell = tg.operator(1.0, -1.0)
+    assert (op_tp.format, op_ell.format) == ("tp", "ell")
+    P = orc.Problem(surface, dx, params=params, bbox=bbox)
+    x = np.random.default_rng(14).standard_normal(op_tp.D)
+    y_tp, y_ell = op_tp.multiply(x, 1.0, -1.0), op_ell.multiply(x, 1.0, -1.0)
+    scale = np.abs(y_ell).max()
+    assert np.abs(y_tp - y_ell).max() <= 1e-15 * scale
+    assert np.abs(y_tp - orc.multiply(P.L(), P.E(), P.ident(), x, 1.0, -1.0)).max() <= 1e-12 * scale
+    b = np.random.default_rng(15).standard_normal(op_tp.D)
+    xs, infos = gpu_ctx.solve_batch([op_tp, op_ell, op_tp], [b, b, 2 * b], cpmgpu.opts(tol=1e-11))
+    assert all(i["status"] == "converged" for i in infos)
+    assert np.linalg.norm(xs[0] - xs[1]) <= 1e-9 * np.linalg.norm(xs[1])
+    assert np.linalg.norm(xs[2] - 2 * xs[1]) <= 1e-9 * np.linalg.norm(xs[2])

@@ -128,9 +128,7 @@ static OpView<T> make_view(const Op& o, const Buffer& ev, const Buffer& lv, cons
     v.tpc = o.tpc.as<int>();
     v.tpw = tw.as<T>();
     v.exc = o.exc.as<int>();
-    v.emask = o.emask.as<unsigned long long>();
-    v.eoff = o.eoff.as<int>();
-    v.ecolx = o.ecolx.as<int>();
+    v.ecx = o.ecx.as<int>();
     v.xcol = o.xcol.as<int>();
     v.xval = xv.as<T>();
     v.loff = (T)o.loff;
@@ -144,8 +142,8 @@ OpView<float> Op::view<float>() const { return make_view<float>(*this, eval32, l
 
 size_t Op::device_bytes() const {
     return ecol.bytes + eval64.bytes + eval32.bytes + lcol.bytes + lval64.bytes + lval32.bytes + ldiag64.bytes +
-           ldiag32.bytes + flag.bytes + perm.bytes + tpc.bytes + tpw64.bytes + tpw32.bytes + exc.bytes + emask.bytes +
-           eoff.bytes + ecolx.bytes + xcol.bytes + xval64.bytes + xval32.bytes;
+           ldiag32.bytes + flag.bytes + perm.bytes + tpc.bytes + tpw64.bytes + tpw32.bytes + exc.bytes + ecx.bytes +
+           xcol.bytes + xval64.bytes + xval32.bytes;
 }
 
 Op* op_from_csr(Ctx* ctx, int64_t D, int nD, const int32_t* Lp, const int32_t* Lc, const double* Lv,
@@ -488,8 +486,7 @@ static void profile_egather_t(Op* op, int reps, double* avg_ms, double* bytes) {
     // algorithmic bytes: operator data of the gathered rows, x read once, out written once, flags
     double opb;
     if (op->fmt == OP_TP)
-        opb = (double)gathered * (16 * 4.0 + 12.0 * sizeof(T) + 4.0) + (double)op->nexc * (8 + 4 + 4 + sizeof(T)) +
-              4.0 * op->nmoved;
+        opb = (double)gathered * (16 * 4.0 + 12.0 * sizeof(T) + 4.0) + (double)op->nexc * (64 * 4.0 + 4 + sizeof(T));
     else
         opb = (double)gathered * op->ew * (4.0 + sizeof(T));
     *bytes = opb + 2.0 * op->D * NR * sizeof(T) + (double)op->D;

@@ -50,7 +50,8 @@ enum : int { OP_ELL = 0, OP_TP = 1 };
 // (wz[k] wy[j]) wx[i] x[tpc(i, j) + k] (the reference's slot order and
 // product order), except for the slots flagged in the row's exception record,
 // whose columns were moved by an interior boundary (and one optional extra
-// entry).  L off-diagonals all equal loff (lcol -1 = absent), L_ii = ldu.
+// entry): such a row lists the columns of all 64 slots in its record.
+// L off-diagonals all equal loff (lcol -1 = absent), L_ii = ldu.
 template <typename T>
 struct OpView {
     int64_t D;
@@ -66,9 +67,7 @@ struct OpView {
     const int* tpc;      // [16][D] first column of stencil line (i, j)
     const T* tpw;        // [12][D] wx[4], wy[4], wz[4]; a row's sign is folded into wx
     const int* exc;      // [D] exception record of the row, -1 = none
-    const unsigned long long* emask;   // [nexc] bit s: slot s has a moved column
-    const int* eoff;     // [nexc] first moved column in ecolx
-    const int* ecolx;    // moved columns, in slot order
+    const int* ecx;      // [nexc][64] column of every slot of an exception row
     const int* xcol;     // [nexc] extra column after the stencil, -1 = none
     const T* xval;       // [nexc]
     T loff, ldu;

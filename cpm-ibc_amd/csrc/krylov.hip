@@ -60,13 +60,25 @@ namespace cpmgpu {
 template <typename T, int NR> struct VT { typedef T __attribute__((ext_vector_type(NR))) type; };
 template <typename T> struct VT<T, 1> { typedef T type; };
 
+// Every array lives in global memory, but pointers taken from the segment
+// table reach the compiler as generic pointers and would be accessed with
+// flat instructions; these helpers address them as global memory.
+#define CPM_GLOBAL __attribute__((address_space(1)))
+template <typename T>
+__device__ __forceinline__ T ldg(const T* p) {
+    return *(const CPM_GLOBAL T*)p;
+}
+template <typename T>
+__device__ __forceinline__ void stg(T* p, const T& v) {
+    *(CPM_GLOBAL T*)p = v;
+}
 template <typename T, int NR>
 __device__ __forceinline__ typename VT<T, NR>::type vload(const T* p) {
-    return *reinterpret_cast<const typename VT<T, NR>::type*>(p);
+    return ldg(reinterpret_cast<const typename VT<T, NR>::type*>(p));
 }
 template <typename T, int NR>
 __device__ __forceinline__ void vstore(T* p, const typename VT<T, NR>::type& v) {
-    *reinterpret_cast<typename VT<T, NR>::type*>(p) = v;
+    stg(reinterpret_cast<typename VT<T, NR>::type*>(p), v);
 }
 template <typename V> __device__ __forceinline__ auto lane(const V& v, int k) { return v[k]; }
 __device__ __forceinline__ double lane(double v, int) { return v; }
@@ -100,12 +112,12 @@ __device__ void block_partials(const double (&v)[NV][NR], double* part, int lb, 
         double s = 0.0;
 #pragma unroll
         for (int w = 0; w < kBlock / 64; ++w) s += red[w][threadIdx.x];
-        part[(size_t)threadIdx.x * nblocks + lb] = s;
+        stg(&part[(size_t)threadIdx.x * nblocks + lb], s);
     }
 }
 
 template <typename T>
-__device__ __forceinline__ const Seg<T>& seg_of(const Seg<T>* segs, const int* bseg, int& lb, int64_t& row) {
+__device__ __forceinline__ const Seg<T>& seg_of(const Seg<T>* __restrict__ segs, const int* __restrict__ bseg, int& lb, int64_t& row) {
     const Seg<T>& sg = segs[bseg[blockIdx.x]];
     lb = (int)blockIdx.x - sg.block0;
     row = (int64_t)lb * kBlock + threadIdx.x;
@@ -126,13 +138,13 @@ constexpr unsigned M_STAGE3 = M_RUN | M_READY;     // fin of the loop test
 
 template <typename T>
 __device__ __forceinline__ bool lane_in(const Seg<T>& sg, int k, unsigned mask) {
-    return k < sg.nr && ((mask >> sg.st[k].status) & 1u);
+    return k < sg.nr && ((mask >> ldg(&sg.st[k].status)) & 1u);
 }
 
 template <typename T>
 __device__ __forceinline__ bool seg_any(const Seg<T>& sg, unsigned mask) {
     bool a = false;
-    for (int k = 0; k < sg.nr; ++k) a |= ((mask >> sg.st[k].status) & 1u) != 0;
+    for (int k = 0; k < sg.nr; ++k) a |= ((mask >> ldg(&sg.st[k].status)) & 1u) != 0;
     return a;
 }
 
@@ -145,52 +157,56 @@ __device__ __forceinline__ T precond_apply(const Seg<T>& sg, int64_t row, uint8_
     if (row >= dof) return val / (T)(sg.m - sg.n);   // :335-338 overwrite the trailing rows last
     if (f & ROW_IDENTITY) return val;
     if (f & ROW_NODIAG) return val;
-    const T ld = sg.op.fmt == OP_TP ? sg.op.ldu : sg.op.ldiag[row];
+    const T ld = sg.op.fmt == OP_TP ? sg.op.ldu : ldg(&sg.op.ldiag[row]);
     return val / ((T)sg.m + ld * (T)sg.n);
 }
 
 // E row of the tensor-product format: slot (i, j, k) in the reference's
 // order, value (wz[k] wy[j]) wx[i] formed exactly as when E was built.  One
-// stencil line (4 consecutive x rows) per step, two steps in flight: fully
-// unrolled, the 64 gathers of NR-wide rows would take every register.
-// `stride` = elements between consecutive rows of xb.
-template <typename T, typename V, int W>
-__device__ __forceinline__ void egather_tp(const OpView<T>& op, int64_t row, const T* xb, int stride, V& acc) {
+// stencil line (4 consecutive x rows) per step, two steps in flight (fully
+// unrolled, the 64 NR-wide gathers would take every register).  A row whose
+// columns an interior boundary moved runs the same loop with the line's four
+// columns read from its exception record.  Both loops are kept free of
+// branches: a conditional load inside the loop costs twice its time even
+// when never taken.  `stride` = elements between consecutive rows of xb.
+template <typename T, typename V, int W, bool EXC>
+__device__ __forceinline__ void egather_tp_lines(const OpView<T>& op, int64_t row, const T* xb, int stride, int e,
+                                                 V& acc) {
+    typedef int I4 __attribute__((ext_vector_type(4)));
     const int64_t D = op.D;
     T wz[4];
 #pragma unroll
-    for (int q = 0; q < 4; ++q) wz[q] = op.tpw[(8 + q) * D + row];
-    const int e = op.exc[row];
-    if (e < 0) {
+    for (int q = 0; q < 4; ++q) wz[q] = ldg(&op.tpw[(8 + q) * D + row]);
+    const I4* ex = reinterpret_cast<const I4*>(op.ecx) + (size_t)e * 16;
 #pragma unroll 2
-        for (int l = 0; l < 16; ++l) {
-            const T wx = op.tpw[(l >> 2) * D + row];
-            const T wy = op.tpw[(4 + (l & 3)) * D + row];
-            const T* xp = xb + (size_t)op.tpc[l * D + row] * stride;
-            const V x0 = vload<T, W>(xp), x1 = vload<T, W>(xp + stride), x2 = vload<T, W>(xp + 2 * stride),
-                    x3 = vload<T, W>(xp + 3 * stride);
-            acc += ((wz[0] * wy) * wx) * x0;
-            acc += ((wz[1] * wy) * wx) * x1;
-            acc += ((wz[2] * wy) * wx) * x2;
-            acc += ((wz[3] * wy) * wx) * x3;
+    for (int l = 0; l < 16; ++l) {
+        const T wx = ldg(&op.tpw[(l >> 2) * D + row]);
+        const T wy = ldg(&op.tpw[(4 + (l & 3)) * D + row]);
+        I4 c;
+        if (EXC) {
+            c = ldg(ex + l);
+        } else {
+            const int c0 = ldg(&op.tpc[l * D + row]);
+            c = I4{c0, c0 + 1, c0 + 2, c0 + 3};
         }
-    } else {   // columns moved by an interior boundary, and the extra entry
-        const unsigned long long mask = op.emask[e];
-        const int* mc = op.ecolx + op.eoff[e];
-#pragma unroll 1
-        for (int l = 0; l < 16; ++l) {
-            const T wx = op.tpw[(l >> 2) * D + row];
-            const T wy = op.tpw[(4 + (l & 3)) * D + row];
-            const int c0 = op.tpc[l * D + row];
-#pragma unroll
-            for (int k = 0; k < 4; ++k) {
-                int c = c0 + k;
-                if ((mask >> (4 * l + k)) & 1ull) c = *mc++;
-                acc += ((wz[k] * wy) * wx) * vload<T, W>(xb + (size_t)c * stride);
-            }
-        }
-        const int xc = op.xcol[e];
-        if (xc >= 0) acc += op.xval[e] * vload<T, W>(xb + (size_t)xc * stride);
+        const V x0 = vload<T, W>(xb + (size_t)c.x * stride), x1 = vload<T, W>(xb + (size_t)c.y * stride),
+                x2 = vload<T, W>(xb + (size_t)c.z * stride), x3 = vload<T, W>(xb + (size_t)c.w * stride);
+        acc += ((wz[0] * wy) * wx) * x0;
+        acc += ((wz[1] * wy) * wx) * x1;
+        acc += ((wz[2] * wy) * wx) * x2;
+        acc += ((wz[3] * wy) * wx) * x3;
+    }
+}
+
+template <typename T, typename V, int W>
+__device__ __forceinline__ void egather_tp(const OpView<T>& op, int64_t row, const T* xb, int stride, V& acc) {
+    const int e = ldg(&op.exc[row]);
+    if (e < 0) {
+        egather_tp_lines<T, V, W, false>(op, row, xb, stride, 0, acc);
+    } else {
+        egather_tp_lines<T, V, W, true>(op, row, xb, stride, e, acc);
+        const int xc = ldg(&op.xcol[e]);   // the extra entry after the stencil
+        if (xc >= 0) acc += ldg(&op.xval[e]) * vload<T, W>(xb + (size_t)xc * stride);
     }
 }
 
@@ -206,11 +222,11 @@ __device__ __forceinline__ void egather_ell(const OpView<T>& op, int64_t row, co
         int c[8];
         T w[8];
 #pragma unroll
-        for (int u = 0; u < 8; ++u) { c[u] = col[(int64_t)(s + u) * D]; w[u] = val[(int64_t)(s + u) * D]; }
+        for (int u = 0; u < 8; ++u) { c[u] = ldg(&col[(int64_t)(s + u) * D]); w[u] = ldg(&val[(int64_t)(s + u) * D]); }
 #pragma unroll
         for (int u = 0; u < 8; ++u) acc += w[u] * vload<T, W>(xb + (size_t)c[u] * stride);
     }
-    for (; s < ew; ++s) acc += val[(int64_t)s * D] * vload<T, W>(xb + (size_t)col[(int64_t)s * D] * stride);
+    for (; s < ew; ++s) acc += ldg(&val[(int64_t)s * D]) * vload<T, W>(xb + (size_t)ldg(&col[(int64_t)s * D]) * stride);
 }
 
 // (E x)_row for W consecutive values per row of xb.
@@ -218,7 +234,7 @@ template <typename T, int W>
 __device__ __forceinline__ typename VT<T, W>::type erow(const OpView<T>& op, int64_t row, const T* xb, int stride) {
     typedef typename VT<T, W>::type V;
     V acc = (V)(T)0;
-    if (op.flag[row] & ROW_IDENTITY) return vload<T, W>(xb + (size_t)row * stride);
+    if (ldg(&op.flag[row]) & ROW_IDENTITY) return vload<T, W>(xb + (size_t)row * stride);
     if (op.fmt == OP_TP) egather_tp<T, V, W>(op, row, xb, stride, acc);
     else egather_ell<T, V, W>(op, row, xb, stride, acc);
     return acc;
@@ -232,29 +248,29 @@ __device__ __forceinline__ typename VT<T, W>::type arow(const OpView<T>& op, dou
     typedef typename VT<T, W>::type V;
     const int64_t D = op.D;
     if (row >= D - op.nD) return (T)(m - n) * xv;
-    if (op.flag[row] & ROW_IDENTITY) return xv;
+    if (ldg(&op.flag[row]) & ROW_IDENTITY) return xv;
     V lo = (V)(T)0;
     T ld;
     if (op.fmt == OP_TP) {
 #pragma unroll
         for (int s = 0; s < 6; ++s) {
-            const int c = op.lcol[(int64_t)s * D + row];
+            const int c = ldg(&op.lcol[(int64_t)s * D + row]);
             if (c >= 0) lo += op.loff * vload<T, W>(tb + (size_t)c * stride);
         }
         ld = op.ldu;
     } else {
         for (int s = 0; s < op.lw; ++s) {
-            const int c = op.lcol[(int64_t)s * D + row];
-            lo += op.lval[(int64_t)s * D + row] * vload<T, W>(tb + (size_t)c * stride);
+            const int c = ldg(&op.lcol[(int64_t)s * D + row]);
+            lo += ldg(&op.lval[(int64_t)s * D + row]) * vload<T, W>(tb + (size_t)c * stride);
         }
-        ld = op.ldiag[row];
+        ld = ldg(&op.ldiag[row]);
     }
     return (T)m * xv + (T)n * (ld * xv + lo);
 }
 
 // ---------------------------------------------------------------- vector kernels
 template <typename T, int NR>
-__global__ __launch_bounds__(kBlock) void k_init(const Seg<T>* segs, const int* bseg, const T* b, T* x, T* r,
+__global__ __launch_bounds__(kBlock) void k_init(const Seg<T>* __restrict__ segs, const int* __restrict__ bseg, const T* b, T* x, T* r,
                                                  T* r0, T* p, T* v) {
     int lb;
     int64_t row;
@@ -280,7 +296,7 @@ __global__ __launch_bounds__(kBlock) void k_init(const Seg<T>* segs, const int* 
 // RUN lanes: p = (p - w v) beta + r ; y = P^-1 p.
 // UPDATE lanes (mixed): x64 += x32 / scale ; x32 = 0.
 template <typename T, int NR>
-__global__ __launch_bounds__(kBlock) void k_pupdate(const Seg<T>* segs, const int* bseg, T* p, const T* v,
+__global__ __launch_bounds__(kBlock) void k_pupdate(const Seg<T>* __restrict__ segs, const int* __restrict__ bseg, T* p, const T* v,
                                                     const T* r, T* y, T* x, MixVec mv) {
     int lb;
     int64_t row;
@@ -288,15 +304,15 @@ __global__ __launch_bounds__(kBlock) void k_pupdate(const Seg<T>* segs, const in
     if (row >= sg.op.D || !seg_any(sg, M_STAGE1)) return;
     const size_t i = (size_t)(sg.off + row * NR);
     if (seg_any(sg, M_RUN)) {
-        const uint8_t f = sg.op.flag[row];
+        const uint8_t f = ldg(&sg.op.flag[row]);
         auto pv = vload<T, NR>(p + i);
         auto vv = vload<T, NR>(v + i);
         auto rv = vload<T, NR>(r + i);
         typename VT<T, NR>::type yv;
 #pragma unroll
         for (int k = 0; k < NR; ++k) {
-            const KState& st = sg.st[k < sg.nr ? k : 0];
-            const T beta = (T)st.beta, w = (T)st.omega;
+            const KState* st = &sg.st[k < sg.nr ? k : 0];
+            const T beta = (T)ldg(&st->beta), w = (T)ldg(&st->omega);
             T pk = (-w * lane(vv, k) + lane(pv, k)) * beta + lane(rv, k);   // p.Copy(-w,v,p); p*=beta; p+=r
             if (!lane_in(sg, k, M_RUN)) pk = lane(pv, k);
             setlane(pv, k, pk);
@@ -311,7 +327,7 @@ __global__ __launch_bounds__(kBlock) void k_pupdate(const Seg<T>* segs, const in
 #pragma unroll
         for (int k = 0; k < NR; ++k)
             if (lane_in(sg, k, M_UPDATE)) {
-                setlane(xd, k, lane(xd, k) + (double)lane(xv, k) / sg.st[k].scale);
+                setlane(xd, k, lane(xd, k) + (double)lane(xv, k) / ldg(&sg.st[k].scale));
                 setlane(xv, k, (T)0);
             }
         vstore<double, NR>(mv.x64 + i, xd);
@@ -322,7 +338,7 @@ __global__ __launch_bounds__(kBlock) void k_pupdate(const Seg<T>* segs, const in
 // tmp = E x.  Rows are independent; a thread owns one row and all NR rhs.
 // UPDATE lanes (mixed, first gather of the iteration): tmp64 = E64 x64.
 template <typename T, int NR>
-__global__ __launch_bounds__(kBlock) void k_egather(const Seg<T>* segs, const int* bseg, const T* x, T* out,
+__global__ __launch_bounds__(kBlock) void k_egather(const Seg<T>* __restrict__ segs, const int* __restrict__ bseg, const T* x, T* out,
                                                     unsigned lanes, MixVec mv) {
     int lb;
     int64_t row;
@@ -333,7 +349,7 @@ __global__ __launch_bounds__(kBlock) void k_egather(const Seg<T>* segs, const in
     if (!lanes || seg_any(sg, lanes & ~M_UPDATE)) vstore<T, NR>(out + i, erow<T, NR>(sg.op, row, x + sg.off, NR));
     if (std::is_same<T, float>::value && (lanes & M_UPDATE) && seg_any(sg, M_UPDATE)) {
         for (int k = 0; k < sg.nr; ++k)
-            if (lane_in(sg, k, M_UPDATE)) mv.tmp64[i + k] = erow<double, 1>(sg.op64, row, mv.x64 + sg.off + k, NR);
+            if (lane_in(sg, k, M_UPDATE)) stg(&mv.tmp64[i + k], erow<double, 1>(sg.op64, row, mv.x64 + sg.off + k, NR));
     }
 }
 
@@ -342,7 +358,7 @@ __global__ __launch_bounds__(kBlock) void k_egather(const Seg<T>* segs, const in
 // Mixed, RED = 1: UPDATE lanes r64 = b - A64 x64, partial (r64, r64).
 // Mixed, RED = 2: RESCALE lanes partial (r32, r32) (rv32 = r32).
 template <typename T, int NR, int RED>
-__global__ __launch_bounds__(kBlock) void k_lfinish(const Seg<T>* segs, const int* bseg, const T* x,
+__global__ __launch_bounds__(kBlock) void k_lfinish(const Seg<T>* __restrict__ segs, const int* __restrict__ bseg, const T* x,
                                                     const T* tmp, T* out, const T* other, const T* rv32,
                                                     unsigned lanes, MixVec mv) {
     int lb;
@@ -378,9 +394,9 @@ __global__ __launch_bounds__(kBlock) void k_lfinish(const Seg<T>* segs, const in
             for (int k = 0; k < sg.nr; ++k)
                 if (lane_in(sg, k, M_UPDATE)) {
                     const double ax =
-                        arow<double, 1>(sg.op64, sg.m, sg.n, row, mv.x64[i + k], mv.tmp64 + sg.off + k, NR);
-                    const double rk = mv.b64[i + k] - ax;
-                    mv.r64[i + k] = rk;
+                        arow<double, 1>(sg.op64, sg.m, sg.n, row, ldg(&mv.x64[i + k]), mv.tmp64 + sg.off + k, NR);
+                    const double rk = ldg(&mv.b64[i + k]) - ax;
+                    stg(&mv.r64[i + k], rk);
                     pv[0][k] = rk * rk;
                 }
         }
@@ -401,21 +417,21 @@ __global__ __launch_bounds__(kBlock) void k_lfinish(const Seg<T>* segs, const in
 // RUN lanes: s = r - alpha v ; z = P^-1 s.
 // RESCALE lanes (mixed): r32 = r64 scale ; r0 = r32 ; p = v = 0.
 template <typename T, int NR>
-__global__ __launch_bounds__(kBlock) void k_supdate(const Seg<T>* segs, const int* bseg, T* r, T* v, T* s, T* z,
+__global__ __launch_bounds__(kBlock) void k_supdate(const Seg<T>* __restrict__ segs, const int* __restrict__ bseg, T* r, T* v, T* s, T* z,
                                                     T* r0, T* p, MixVec mv) {
     int lb;
     int64_t row;
     const Seg<T>& sg = seg_of(segs, bseg, lb, row);
     if (row >= sg.op.D || !seg_any(sg, M_STAGE2)) return;
     const size_t i = (size_t)(sg.off + row * NR);
-    const uint8_t f = sg.op.flag[row];
+    const uint8_t f = ldg(&sg.op.flag[row]);
     auto rv = vload<T, NR>(r + i);
     auto vv = vload<T, NR>(v + i);
     if (seg_any(sg, M_RUN)) {
         typename VT<T, NR>::type sv, zv;
 #pragma unroll
         for (int k = 0; k < NR; ++k) {
-            const T a = (T)sg.st[k < sg.nr ? k : 0].alpha;
+            const T a = (T)ldg(&sg.st[k < sg.nr ? k : 0].alpha);
             T sk = -a * lane(vv, k) + lane(rv, k);   // s.Copy(-alpha, v, r)
             setlane(sv, k, sk);
             setlane(zv, k, precond_apply(sg, row, f, sk));
@@ -430,7 +446,7 @@ __global__ __launch_bounds__(kBlock) void k_supdate(const Seg<T>* segs, const in
 #pragma unroll
         for (int k = 0; k < NR; ++k)
             if (lane_in(sg, k, M_RESCALE)) {
-                const T rk = (T)(lane(rd, k) * sg.st[k].scale);
+                const T rk = (T)(lane(rd, k) * ldg(&sg.st[k].scale));
                 setlane(rv, k, rk);
                 setlane(r0v, k, rk);
                 setlane(pv, k, (T)0);
@@ -444,7 +460,7 @@ __global__ __launch_bounds__(kBlock) void k_supdate(const Seg<T>* segs, const in
 }
 
 template <typename T, int NR>
-__global__ __launch_bounds__(kBlock) void k_xrupdate(const Seg<T>* segs, const int* bseg, T* x, const T* y,
+__global__ __launch_bounds__(kBlock) void k_xrupdate(const Seg<T>* __restrict__ segs, const int* __restrict__ bseg, T* x, const T* y,
                                                      const T* z, T* r, const T* s, const T* t, const T* r0) {
     int lb;
     int64_t row;
@@ -465,8 +481,7 @@ __global__ __launch_bounds__(kBlock) void k_xrupdate(const Seg<T>* segs, const i
 #pragma unroll
         for (int k = 0; k < NR; ++k) {
             if (!lane_in(sg, k, M_RUN)) continue;
-            const KState& st = sg.st[k];
-            const T a = (T)st.alpha, w = (T)st.omega;
+            const T a = (T)ldg(&sg.st[k].alpha), w = (T)ldg(&sg.st[k].omega);
             T xk = a * lane(yv, k) + lane(xv, k);   // x.Copy(alpha, y, x)
             xk = w * lane(zv, k) + xk;               // x.Copy(w, z, x)
             T rk = -w * lane(tv, k) + lane(sv, k);   // r.Copy(-w, t, s)
@@ -483,7 +498,7 @@ __global__ __launch_bounds__(kBlock) void k_xrupdate(const Seg<T>* segs, const i
 
 // FP64 restart, BiCGStab.h:257-264: r = b - A x (A x already in ax), r0 = r; partial (r, r).
 template <typename T, int NR>
-__global__ __launch_bounds__(kBlock) void k_restart(const Seg<T>* segs, const int* bseg, const T* b,
+__global__ __launch_bounds__(kBlock) void k_restart(const Seg<T>* __restrict__ segs, const int* __restrict__ bseg, const T* b,
                                                     const T* ax, T* r, T* r0) {
     int lb;
     int64_t row;
@@ -515,7 +530,7 @@ __global__ __launch_bounds__(kBlock) void k_restart(const Seg<T>* segs, const in
 // ---------------------------------------------------------------- mixed precision start
 // x64 = 0, r64 = b ; partial (b, b)
 template <int NR>
-__global__ __launch_bounds__(kBlock) void k_mx_start(const Seg<double>* segs, const int* bseg, const double* b,
+__global__ __launch_bounds__(kBlock) void k_mx_start(const Seg<double>* __restrict__ segs, const int* __restrict__ bseg, const double* b,
                                                      double* x, double* r) {
     int lb;
     int64_t row;
@@ -537,7 +552,7 @@ __global__ __launch_bounds__(kBlock) void k_mx_start(const Seg<double>* segs, co
 // FP32 start of the lanes in ST_RESCALE: r32 = r64 * scale, r0 = r32,
 // p = v = x32 = 0 ; partial (r32, r32).
 template <int NR>
-__global__ __launch_bounds__(kBlock) void k_mx_down(const Seg<double>* segs, const int* bseg, const double* r64,
+__global__ __launch_bounds__(kBlock) void k_mx_down(const Seg<double>* __restrict__ segs, const int* __restrict__ bseg, const double* r64,
                                                     float* r, float* r0, float* p, float* v, float* x) {
     int lb;
     int64_t row;
@@ -557,7 +572,7 @@ __global__ __launch_bounds__(kBlock) void k_mx_down(const Seg<double>* segs, con
 #pragma unroll
         for (int k = 0; k < NR; ++k) {
             if (!lane_in(sg, k, M_RESCALE)) continue;
-            const float rk = (float)(lane(rv, k) * sg.st[k].scale);
+            const float rk = (float)(lane(rv, k) * ldg(&sg.st[k].scale));
             setlane(ro, k, rk);
             setlane(r0o, k, rk);
             setlane(po, k, 0.0f);
@@ -686,36 +701,47 @@ __device__ void fin_scalar(KState& s, const double* tot) {
 // One workgroup per segment: fixed-order sum of the producing kernel's
 // block partials, then the scalar recurrence for each active lane.
 template <typename T, int NR, int KIND>
-__global__ __launch_bounds__(kBlock) void k_fin(const Seg<T>* segs, int nseg) {
+__global__ __launch_bounds__(kBlock) void k_fin(const Seg<T>* __restrict__ segs, int nseg) {
     constexpr int NV = FinTraits<KIND>::NV;
     constexpr unsigned M = FinTraits<KIND>::M;
     const Seg<T>& sg = segs[blockIdx.x];
     if (!seg_any(sg, M)) return;   // same test the producer used
-    // wave w sums components w, w + 4, ...; lanes stride over the blocks
-    __shared__ double red[NV * NR];
+    // Wave w sums components w, w + 4, ...; lanes stride over the blocks with
+    // all of a wave's components and 16 blocks per lane in flight at once
+    // (the partials were just written by another kernel, so every round
+    // costs a full memory latency).
+    constexpr int NC = NV * NR, CPW = (NC + 3) / 4, U = 16;
+    __shared__ double red[NC];
     const int lanei = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    for (int c = wave; c < NV * NR; c += kBlock / 64) {
-        const double* pc = sg.part + (size_t)c * sg.nblocks;
-        double a0 = 0.0, a1 = 0.0, a2 = 0.0, a3 = 0.0;
-        int b = lanei;
-        for (; b + 192 < sg.nblocks; b += 256) {
-            a0 += pc[b];
-            a1 += pc[b + 64];
-            a2 += pc[b + 128];
-            a3 += pc[b + 192];
-        }
-        for (; b < sg.nblocks; b += 64) a0 += pc[b];
-        const double t = wave_sum((a0 + a1) + (a2 + a3));
-        if (lanei == 0) red[c] = t;
+    double acc[CPW];
+#pragma unroll
+    for (int cc = 0; cc < CPW; ++cc) acc[cc] = 0.0;
+    for (int b0 = lanei; b0 < sg.nblocks; b0 += 64 * U) {
+        double v[CPW][U];
+#pragma unroll
+        for (int cc = 0; cc < CPW; ++cc)
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                const int c = wave + 4 * cc, b = b0 + 64 * u;
+                v[cc][u] = (c < NC && b < sg.nblocks) ? ldg(&sg.part[(size_t)c * sg.nblocks + b]) : 0.0;
+            }
+#pragma unroll
+        for (int cc = 0; cc < CPW; ++cc)
+#pragma unroll
+            for (int u = 0; u < U; ++u) acc[cc] += v[cc][u];
+    }
+#pragma unroll
+    for (int cc = 0; cc < CPW; ++cc) {
+        const int c = wave + 4 * cc;
+        const double t = wave_sum(acc[cc]);
+        if (lanei == 0 && c < NC) red[c] = t;
     }
     __syncthreads();
-    if (threadIdx.x != 0) return;
-    for (int k = 0; k < sg.nr; ++k) {
-        if (!lane_in(sg, k, M)) continue;
-        double tot[NV];
-        for (int a = 0; a < NV; ++a) tot[a] = red[a * NR + k];
-        fin_scalar<KIND>(sg.st[k], tot);
-    }
+    const int k = threadIdx.x;   // one thread per rhs lane
+    if (k >= sg.nr || !lane_in(sg, k, M)) return;
+    double tot[NV];
+    for (int a = 0; a < NV; ++a) tot[a] = red[a * NR + k];
+    fin_scalar<KIND>(sg.st[k], tot);
 }
 
 __global__ void k_reset_states(KState* st, int n, double tol2, int max_it, double delta2) {

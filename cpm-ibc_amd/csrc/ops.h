@@ -42,7 +42,7 @@ struct Op {
     // OP_TP
     int fmt = OP_ELL;
     Buffer perm;              // int32 [D]: internal row -> caller row (empty = identity)
-    Buffer tpc, tpw64, tpw32, exc, emask, eoff, ecolx, xcol, xval64, xval32;
+    Buffer tpc, tpw64, tpw32, exc, ecx, xcol, xval64, xval32;
     int64_t nexc = 0, nmoved = 0;
     double loff = 0, ldu = 0;
     Op() = default;

@@ -7,9 +7,10 @@
 // grid nodes b + (i, j, k).  Once the tube's rows are renumbered in
 // lexicographic grid order (z fastest), the nodes b + (i, j, 0..3) of a
 // stencil line are four consecutive rows, so a row is described by 16 line
-// starts and 12 weights (160 bytes in FP64, 112 in FP32), plus an exception
-// record where an interior boundary moved some of its columns (PoissonWithIBC
-// MoveToIBCColumns) or appended the Dirichlet identity column.
+// starts and 12 weights (160 bytes in FP64, 112 in FP32).  Where an interior
+// boundary moved some of a row's columns (PoissonWithIBC MoveToIBCColumns) or
+// appended the Dirichlet identity column, an exception record lists the
+// columns of all 64 slots and the extra entry.
 //
 // The builder derives the format from the reference-order ELL rows and the
 // rows' query points, and accepts an operator only if every row is
@@ -171,9 +172,9 @@ __global__ void k_tp_rows(TpArgs a, int* tpc, double* tpw64, float* tpw32, int* 
     n_mov[r] = moved;
 }
 
-// Pass 2: exception records (n_exc / n_mov now hold exclusive offsets).
-__global__ void k_tp_exc(TpArgs a, const unsigned long long* mask_in, const int* exc_off, const int* mov_off,
-                         int64_t nexc_total, int* exc, unsigned long long* emask, int* eoff, int* ecolx, int* xcol,
+// Pass 2: exception records (n_exc now holds exclusive offsets): the
+// internal column of every slot, and the extra entry.
+__global__ void k_tp_exc(TpArgs a, const int* exc_off, int64_t nexc_total, int* exc, int* ecx, int* xcol,
                          double* xval64, float* xval32) {
     const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (r >= a.D) return;
@@ -183,12 +184,7 @@ __global__ void k_tp_exc(TpArgs a, const unsigned long long* mask_in, const int*
     const int e_next = r + 1 < a.D ? exc_off[r + 1] : (int)nexc_total;
     if (e_next == e || (a.flag[R] & ROW_IDENTITY)) { exc[r] = -1; return; }
     exc[r] = e;
-    const unsigned long long mask = mask_in[r];
-    emask[e] = mask;
-    eoff[e] = mov_off[r];
-    int m = mov_off[r];
-    for (int s = 0; s < kS * kS * kS; ++s)
-        if ((mask >> s) & 1ull) ecolx[m++] = (int)to_internal(a, a.ecol[s * D + R]);
+    for (int s = 0; s < kS * kS * kS; ++s) ecx[(int64_t)e * 64 + s] = (int)to_internal(a, a.ecol[s * D + R]);
     xcol[e] = -1;
     xval64[e] = 0.0;
     xval32[e] = 0.0f;
@@ -277,12 +273,10 @@ bool op_build_tp(Op& op, Tube& t, const TpSource& src) {
     HIPCHK(hipStreamSynchronize(st));
     if (h[0]) return false;
     const int64_t ne = h[4], nm = h[5];
-    Buffer emask((size_t)std::max<int64_t>(ne, 1) * 8), eoff((size_t)std::max<int64_t>(ne, 1) * 4),
-        ecolx((size_t)std::max<int64_t>(nm, 1) * 4), xcol((size_t)std::max<int64_t>(ne, 1) * 4),
+    Buffer ecx((size_t)std::max<int64_t>(ne, 1) * 64 * 4), xcol((size_t)std::max<int64_t>(ne, 1) * 4),
         xval64((size_t)std::max<int64_t>(ne, 1) * 8), xval32((size_t)std::max<int64_t>(ne, 1) * 4);
-    hipLaunchKernelGGL(k_tp_exc, dim3(nblk(D)), dim3(256), 0, st, a, mask.as<unsigned long long>(), nexc.as<int>(),
-                       nmov.as<int>(), ne, exc.as<int>(), emask.as<unsigned long long>(), eoff.as<int>(),
-                       ecolx.as<int>(), xcol.as<int>(), xval64.as<double>(), xval32.as<float>());
+    hipLaunchKernelGGL(k_tp_exc, dim3(nblk(D)), dim3(256), 0, st, a, nexc.as<int>(), ne, exc.as<int>(), ecx.as<int>(),
+                       xcol.as<int>(), xval64.as<double>(), xval32.as<float>());
     hipLaunchKernelGGL(k_perm_full, dim3(nblk(D)), dim3(256), 0, st, t.lex.as<int32_t>(), src.N, D, perm.as<int32_t>());
     HIPCHK(hipGetLastError());
     HIPCHK(hipStreamSynchronize(st));
@@ -301,9 +295,7 @@ bool op_build_tp(Op& op, Tube& t, const TpSource& src) {
     op.flag = std::move(flag);
     op.perm = std::move(perm);
     op.exc = std::move(exc);
-    op.emask = std::move(emask);
-    op.eoff = std::move(eoff);
-    op.ecolx = std::move(ecolx);
+    op.ecx = std::move(ecx);
     op.xcol = std::move(xcol);
     op.xval64 = std::move(xval64);
     op.xval32 = std::move(xval32);

@@ -492,6 +492,33 @@ static void profile_egather_t(Op* op, int reps, double* avg_ms, double* bytes) {
     *bytes = opb + 2.0 * op->D * NR * sizeof(T) + (double)op->D;
 }
 
+__global__ void k_copy16(const uint4* __restrict__ src, uint4* __restrict__ dst, int64_t n) {
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) dst[i] = src[i];
+}
+
+static void profile_copy(Ctx* ctx, int64_t bytes, int reps, double* avg_ms) {
+    const hipStream_t st = ctx->stream;
+    const int64_t n = bytes / 16;
+    char* b = reinterpret_cast<char*>(ctx->workspace("copy_bufs", (size_t)bytes * 2));
+    const uint4* src = reinterpret_cast<const uint4*>(b);
+    uint4* dst = reinterpret_cast<uint4*>(b + bytes);
+    const unsigned grid = (unsigned)std::min<int64_t>((n + 255) / 256, 8192);
+    hipLaunchKernelGGL(k_copy16, dim3(grid), dim3(256), 0, st, src, dst, n);   // warm-up
+    hipEvent_t e0, e1;
+    HIPCHK(hipEventCreate(&e0));
+    HIPCHK(hipEventCreate(&e1));
+    HIPCHK(hipEventRecord(e0, st));
+    for (int r = 0; r < reps; ++r) hipLaunchKernelGGL(k_copy16, dim3(grid), dim3(256), 0, st, src, dst, n);
+    HIPCHK(hipEventRecord(e1, st));
+    HIPCHK(hipEventSynchronize(e1));
+    float ms = 0;
+    HIPCHK(hipEventElapsedTime(&ms, e0, e1));
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    *avg_ms = ms / reps;
+}
+
 }  // namespace cpmgpu
 
 // ====================================================================== C ABI
@@ -736,6 +763,14 @@ int cpm_profile_egather(cpm_op* op, int precision, int nrhs, int reps, double* a
         case 8: f32 ? profile_egather_t<float, 8>(o, reps, avg_ms, bytes) : profile_egather_t<double, 8>(o, reps, avg_ms, bytes); break;
         default: throw ArgError("nrhs must be 1, 2, 4 or 8");
         }
+    });
+}
+
+int cpm_profile_copy(cpm_ctx* ctx, int64_t bytes, int reps, double* avg_ms) {
+    return guard([&] {
+        if (!ctx || !avg_ms || reps <= 0 || bytes <= 0 || bytes % 16) throw ArgError("bad argument");
+        HIPCHK(hipSetDevice(C(ctx)->device));
+        profile_copy(C(ctx), bytes, reps, avg_ms);
     });
 }
 

@@ -116,10 +116,26 @@ __device__ void block_partials(const double (&v)[NV][NR], double* part, int lb, 
     }
 }
 
+constexpr int kXcdRun = 64;
+
 template <typename T>
 __device__ __forceinline__ const Seg<T>& seg_of(const Seg<T>* __restrict__ segs, const int* __restrict__ bseg, int& lb, int64_t& row) {
-    const Seg<T>& sg = segs[bseg[blockIdx.x]];
-    lb = (int)blockIdx.x - sg.block0;
+    // Workgroups are dispatched to the 8 XCDs round-robin (workgroup b to XCD
+    // b mod 8).  Renumber them so that each XCD takes runs of kXcdRun
+    // consecutive row blocks: the x rows a run's stencils gather then come
+    // from that XCD's own L2 instead of every XCD fetching all of them, and
+    // costly rows (interior-boundary rows sit together at the end) still
+    // spread over all XCDs.  The tail that does not fill whole runs keeps
+    // the plain order.
+    const int b = (int)blockIdx.x, n = (int)gridDim.x;
+    const int nfull = n / (8 * kXcdRun) * (8 * kXcdRun);
+    int vb = b;
+    if (b < nfull) {
+        const int xcd = b % 8, k = b / 8;
+        vb = ((k / kXcdRun) * 8 + xcd) * kXcdRun + k % kXcdRun;
+    }
+    const Seg<T>& sg = segs[bseg[vb]];
+    lb = vb - sg.block0;
     row = (int64_t)lb * kBlock + threadIdx.x;
     return sg;
 }

@@ -23,7 +23,7 @@ EXPORTS = [
     "cpm_op_set_precond", "cpm_multiply", "cpm_multiply_device", "cpm_solve", "cpm_solve_device",
     "cpm_solve_batch", "cpm_solve_batch_device", "cpm_device_alloc", "cpm_device_free", "cpm_memcpy_h2d",
     "cpm_memcpy_d2h", "cpm_tube_create", "cpm_tube_destroy", "cpm_tube_get_info", "cpm_tube_export",
-    "cpm_tube_export_E", "cpm_tube_export_L", "cpm_tube_interp_matrix", "cpm_op_create_from_tube", "cpm_profile_egather", "cpm_problem_create", "cpm_problem_destroy",
+    "cpm_tube_export_E", "cpm_tube_export_L", "cpm_tube_interp_matrix", "cpm_op_create_from_tube", "cpm_profile_egather", "cpm_profile_copy", "cpm_problem_create", "cpm_problem_destroy",
     "cpm_problem_get_info", "cpm_problem_rhs_points", "cpm_problem_subset", "cpm_problem_export",
     "cpm_op_create_from_problem",
 ]
@@ -126,6 +126,7 @@ def load(path=LIB_PATH):
     L.cpm_tube_interp_matrix.argtypes = [vp, vp, ctypes.c_int64, vp, vp, vp]
     L.cpm_op_create_from_tube.argtypes = [vp, ctypes.c_double, ctypes.c_double, ctypes.c_int, ctypes.c_int,
                                           ctypes.POINTER(vp)]
+    L.cpm_profile_copy.argtypes = [vp, ctypes.c_int64, ctypes.c_int, ctypes.POINTER(ctypes.c_double)]
     L.cpm_profile_egather.argtypes = [vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.POINTER(ctypes.c_double),
                                       ctypes.POINTER(ctypes.c_double)]
     L.cpm_problem_create.argtypes = [vp, ctypes.c_int, vp, ctypes.POINTER(vp)]
@@ -194,6 +195,12 @@ class Context:
     def d2h(self, arr, dptr):
         _chk(load().cpm_memcpy_d2h(self._h, _ptr(arr), dptr, arr.nbytes))
         return arr
+
+    def profile_copy(self, nbytes=1 << 28, reps=20):
+        """Average ms of a plain device copy of nbytes (kernel k_copy16)."""
+        ms = ctypes.c_double()
+        _chk(load().cpm_profile_copy(self._h, int(nbytes), reps, ctypes.byref(ms)))
+        return ms.value
 
     def solve_batch(self, ops, rhss, o=None):
         n = len(ops)

@@ -245,6 +245,11 @@ int cpm_op_create_from_problem(cpm_problem* prob, double m, double n, int precon
  * x read once, tmp written once, one flag byte per row. */
 int cpm_profile_egather(cpm_op* op, int precision, int nrhs, int reps, double* avg_ms, double* bytes_per_launch);
 
+/* Plain device-to-device copy of `bytes` (multiple of 16), `reps` launches:
+ * the reference point for HBM bandwidth and for scaling hardware traffic
+ * counters (kernel `k_copy16`).  No counterpart in the reference. */
+int cpm_profile_copy(cpm_ctx* ctx, int64_t bytes, int reps, double* avg_ms);
+
 /* Device scratch for callers that keep vectors resident (bench, serving). */
 int cpm_device_alloc(cpm_ctx* ctx, int64_t bytes, void** d_ptr);
 int cpm_device_free(cpm_ctx* ctx, void* d_ptr);

@@ -671,17 +671,22 @@ std::vector<CSR> build_grad_subset(const Tube& t, const std::vector<i64>& set_of
 CSR plot_E(const Tube& t) { return build_E(t, t.surf.xp); }
 
 // ---------------------------------------------------------------- operator
+Parallel g_parallel;
+
 void System::multiply(const double* x, double* y, double mm, double nn) const {
     // GeneralizedBiCGStabSystem.h:213-233 with Interpolation :235-261,
     // LnodiagTimesX :93-115, LdiagTimesX :65-91, FinalizeMultiplication :166-189
     const i64 D = L->rows, dof = dofs();
+    const bool par = g_parallel.apply;
     Vec tmp((size_t)D, 0.0);
+#pragma omp parallel for schedule(static) if (par)
     for (i64 k = 0; k < E->rows; ++k) {
         if (ident[(size_t)k]) { tmp[(size_t)k] = x[k]; continue; }
         double s = 0.0;
         for (i64 e = E->ptr[(size_t)k]; e < E->ptr[(size_t)k + 1]; ++e) s += E->val[(size_t)e] * x[E->col[(size_t)e]];
         tmp[(size_t)k] = s;
     }
+#pragma omp parallel for schedule(static) if (par)
     for (i64 k = 0; k < D; ++k) {      // result = Loff * tmp
         double s = 0.0;
         for (i64 e = L->ptr[(size_t)k]; e < L->ptr[(size_t)k + 1]; ++e)
@@ -689,10 +694,12 @@ void System::multiply(const double* x, double* y, double mm, double nn) const {
         y[k] = s;
     }
     Vec ld((size_t)D, 0.0);              // tmp = Ldiag * x
+#pragma omp parallel for schedule(static) if (par)
     for (i64 k = 0; k < dof; ++k)
         for (i64 e = L->ptr[(size_t)k]; e < L->ptr[(size_t)k + 1]; ++e)
             if (L->col[(size_t)e] == k) ld[(size_t)k] += L->val[(size_t)e] * x[k];
     for (int i = 0; i < nD; ++i) ld[(size_t)(dof + i)] = (mm - nn) * x[dof + i];
+#pragma omp parallel for schedule(static) if (par)
     for (i64 q = 0; q < dof; ++q) {
         if (ident[(size_t)q]) y[q] = x[q];
         else y[q] = mm * x[q] + nn * (ld[(size_t)q] + y[q]);
@@ -702,6 +709,7 @@ void System::multiply(const double* x, double* y, double mm, double nn) const {
 
 void System::diag_inverse(const double* in, double* out) const {   // :281-340 (computed == false)
     const i64 D = L->rows, dof = dofs();
+#pragma omp parallel for schedule(static) if (g_parallel.apply)
     for (i64 k = 0; k < D; ++k) {
         if (ident[(size_t)k]) { out[k] = in[k]; continue; }
         bool found = false;
@@ -773,19 +781,27 @@ SolveInfo bicgstab(const System& sys, const double* bp, double* xp, double tol, 
             if (restarts++ == 0) iters = 0;
         }
         double beta = (rho / rho_old) * (alpha / w);
+        const bool pv = g_parallel.vec;
+#pragma omp parallel for schedule(static) if (pv)
         for (size_t i = 0; i < D; ++i) p[i] = -w * v[i] + p[i];
+#pragma omp parallel for schedule(static) if (pv)
         for (size_t i = 0; i < D; ++i) p[i] *= beta;
+#pragma omp parallel for schedule(static) if (pv)
         for (size_t i = 0; i < D; ++i) p[i] += r[i];
         sys.precondition(p.data(), y.data());
         sys.multiply(y.data(), v.data());
         alpha = rho / vdot(r0, v);
+#pragma omp parallel for schedule(static) if (pv)
         for (size_t i = 0; i < D; ++i) s[i] = -alpha * v[i] + r[i];
         sys.precondition(s.data(), z.data());
         sys.multiply(z.data(), t.data());
         double val = vdot(t, t);
         w = val > 0.0 ? vdot(t, s) / val : 0.0;
+#pragma omp parallel for schedule(static) if (pv)
         for (size_t i = 0; i < D; ++i) x[i] = alpha * y[i] + x[i];
+#pragma omp parallel for schedule(static) if (pv)
         for (size_t i = 0; i < D; ++i) x[i] = w * z[i] + x[i];
+#pragma omp parallel for schedule(static) if (pv)
         for (size_t i = 0; i < D; ++i) r[i] = -w * t[i] + s[i];
         ++iters;
     }

@@ -147,6 +147,15 @@ struct System {
 };
 
 struct SolveInfo { int iters = 0; int restarts = 0; double rel_res = 0; };
+
+// Host-thread parallelism of the restated solver, for timing the CPU baseline
+// only (results do not depend on it: every row and element is computed by one
+// thread in the same order; inner products stay serial, as Eigen's dot is).
+//   vec:   BiCGStab vector updates in OpenMP loops (BiCGStabVector.h:29-71, as shipped)
+//   apply: operator apply and preconditioner in OpenMP loops (what -DOPENMP
+//          enables in GeneralizedBiCGStabSystem.h:71-72, off as shipped)
+struct Parallel { bool vec = false, apply = false; };
+extern Parallel g_parallel;
 // BiCGStab.h:30-108 with Solver.cpp:150-164 (x0 = 0, min 1, max_it).
 SolveInfo bicgstab(const System& sys, const double* b, double* x, double tol, int max_it);
 

@@ -174,6 +174,11 @@ def multiply(L, E, ident, x, m, n, nD=0):
     return y
 
 
+def set_parallel(vec=False, apply=False, threads=0):
+    """Host-thread parallelism of the solver (timing only; results are unchanged)."""
+    lib().orc_set_parallel(int(vec), int(apply), int(threads))
+
+
 def solve(L, E, ident, b, m, n, nD=0, precond=1, precond_iters=0, tol=1e-10, max_it=100000):
     x = np.zeros_like(b)
     it = ctypes.c_int(0)

@@ -2,6 +2,7 @@
 // TEST INFRASTRUCTURE ONLY (see orc.h).  CSR arrays cross this boundary as
 // int32 row pointers / columns, the layout of Eigen::SparseMatrix<double,
 // RowMajor> and of the GPU engine's cpm_op_create_csr.
+#include <omp.h>
 #include <cstring>
 #include <memory>
 #include <string>
@@ -314,6 +315,12 @@ int orc_solve(int64_t D, int nD, const int32_t* Lp, const int32_t* Lc, const dou
         g_err = e.what();
         return -1;
     }
+}
+
+void orc_set_parallel(int vec, int apply, int threads) {
+    g_parallel.vec = vec != 0;
+    g_parallel.apply = apply != 0;
+    if (threads > 0) omp_set_num_threads(threads);
 }
 
 double orc_poisson_problem_error(int problem, double dx, int* iters) {

@@ -4,6 +4,8 @@ import glob
 import os
 import re
 
+import pytest
+
 import cpmgpu
 
 REPO = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
@@ -32,3 +34,16 @@ def test_default_opts_match_reference():
     o = cpmgpu.opts()
     # src/Solver.cpp:164: tolerance 1e-10, at most 100000 iterations
     assert o.tol == 1e-10 and o.max_iters == 100000 and o.precision == cpmgpu.CPM_FP64
+
+
+def test_cpp_facade_compiles(tmp_path):
+    """The C++ cpm::Solver facade and the drop-in test program compile against the C ABI header."""
+    import shutil
+    import subprocess
+    if shutil.which("g++") is None:
+        pytest.skip("g++ not available")
+    src = os.path.join(REPO, "tests", "cpp", "solver_dropin.cpp")
+    out = subprocess.run(["g++", "-std=c++17", "-fsyntax-only", "-Wall", "-Werror",
+                          "-I", os.path.join(REPO, "include"), "-I", os.path.join(REPO, "cpm-ibc_amd", "host"), src],
+                         capture_output=True, text=True, timeout=120)
+    assert out.returncode == 0, out.stderr

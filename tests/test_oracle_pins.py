@@ -147,3 +147,27 @@ def test_config1_closed_form_second_order():
         errs.append(np.abs(Ep.matvec(x) - uex).max())
     order = math.log(errs[0] / errs[1]) / math.log(2)
     assert errs[1] < 5e-3 and order > 1.7
+
+
+# PoissonProblemsWithIBC.cpp:376-377,387-396: average convergence orders of the
+# reference's IBC studies (2nd-order Dirichlet), per seeded geometry g.
+IBC_ORDERS = {
+    2: [2.03624, 2.20571, 2.09173, 2.0828, 2.1572, 2.10696, 1.97858, 1.97003, 2.17008, 1.9772],   # sphere + circle
+    5: [2.16283, 2.13198, 1.92074, 2.22093, 2.27176, 2.36021, 1.89006, 2.12619, 2.22663, 2.70325],  # sphere + point
+}
+
+
+@pytest.mark.parametrize("problem,g", [(2, 0), (2, 3), (5, 0)])
+def test_reference_ibc_convergence_orders(problem, g):
+    """The oracle's PoissonWithIBC reproduces the reference's IBC known answers
+    (the reference's own pass criterion is 1e-2, PoissonProblemsWithIBC.cpp:493)."""
+    order = orc.ibc_study_order(problem, g)
+    assert abs(order - IBC_ORDERS[problem][g]) < 1e-2, order
+
+
+@pytest.mark.slow
+@pytest.mark.parametrize("problem,g", [(p, g) for p in IBC_ORDERS for g in range(len(IBC_ORDERS[p]))
+                                       if (p, g) not in ((2, 0), (2, 3), (5, 0))])
+def test_reference_ibc_convergence_orders_all(problem, g):
+    order = orc.ibc_study_order(problem, g)
+    assert abs(order - IBC_ORDERS[problem][g]) < 1e-2, order

@@ -221,6 +221,7 @@ static void fill_seg(Seg<T>& s, const Op* op, int64_t off, int block0, int nbloc
     s.nblocks = nblocks;
     s.nr = nr;
     s.precond = op->precond;
+    s.precond_iters = op->precond_iters;
     s.st = st;
     s.part = part;
 }
@@ -319,8 +320,11 @@ static void run_batch(Ctx* ctx, const std::vector<Group>& groups, const std::vec
     HIPCHK(hipMemcpyAsync(dpb, pb.data(), sizeof(int) * pb.size(), hipMemcpyHostToDevice, st));
     hipLaunchKernelGGL(k_pack, dim3((unsigned)pb.size()), dim3(256), 0, st, dpe, dpb, V64.b, (float*)nullptr, 1);
 
-    const LaunchCfg<double> c64{segs64, bseg, blocks, G, st};
-    const LaunchCfg<float> c32{segs32, bseg, blocks, G, st};
+    int sweeps = 0;
+    for (auto& g : groups)
+        if (g.op->precond == CPM_PRECOND_JACOBI) sweeps = std::max(sweeps, g.op->precond_iters);
+    const LaunchCfg<double> c64{segs64, bseg, blocks, G, st, sweeps};
+    const LaunchCfg<float> c32{segs32, bseg, blocks, G, st, sweeps};
     KState* hs = ctx->host_states;
     const double delta = o.update_delta > 0 ? std::min(o.update_delta, 0.999) : 1e-3;
     launch_reset_states(sts, nstate, o.tol * o.tol, o.max_iters, delta * delta, st);
@@ -390,7 +394,7 @@ void solve_batch(Ctx* ctx, const std::vector<Op*>& ops, const std::vector<const 
     if (o.precision != CPM_FP64 && o.precision != CPM_FP32_REFINE) throw ArgError("unknown precision");
     for (Op* op : ops) {
         if (!op || op->ctx != ctx) throw ArgError("operator belongs to another context");
-        if (op->precond == CPM_PRECOND_JACOBI) throw ArgError("Jacobi preconditioner is not supported by the GPU solver");
+        if (op->precond == CPM_PRECOND_JACOBI && op->precond_iters > 100000) throw ArgError("too many Jacobi sweeps");
     }
     // group instances that share an operator (at most kMaxNR per group)
     std::vector<Group> groups;
@@ -438,7 +442,7 @@ void multiply(Op* op, const double* dx, double* dy, double m, double n) {
     HIPCHK(hipMemcpyAsync(dbseg, hb.data(), sizeof(int) * nblocks, hipMemcpyHostToDevice, st));
     double* w = ctx->workspace("mul_tmp", sizeof(double) * op->D * 3);
     double *xi = w, *yi = w + op->D, *tmp = w + 2 * op->D;
-    LaunchCfg<double> c{dseg, dbseg, nblocks, 1, st};
+    LaunchCfg<double> c{dseg, dbseg, nblocks, 1, st, 0};
     if (op->perm.p) {   // caller order -> internal order and back
         hipLaunchKernelGGL(k_pack, dim3(nblocks), dim3(256), 0, st, dpe, dbseg, xi, (float*)nullptr, 1);
         Launch<double, 1>::apply(c, xi, tmp, yi);
@@ -465,7 +469,7 @@ static void profile_egather_t(Op* op, int reps, double* avg_ms, double* bytes) {
     T* x = reinterpret_cast<T*>(ctx->workspace("prof_x", sizeof(T) * op->D * NR));
     T* y = reinterpret_cast<T*>(ctx->workspace("prof_y", sizeof(T) * op->D * NR));
     HIPCHK(hipMemsetAsync(x, 0, sizeof(T) * op->D * NR, st));
-    LaunchCfg<T> c{dseg, dbseg, nblocks, 1, st};
+    LaunchCfg<T> c{dseg, dbseg, nblocks, 1, st, 0};
     launch_egather<T, NR>(c, x, y);   // warm-up
     hipEvent_t e0, e1;
     HIPCHK(hipEventCreate(&e0));

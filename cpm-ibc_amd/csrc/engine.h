@@ -84,6 +84,7 @@ struct Seg {
     int block0, nblocks;
     int nr;              // rhs in this group (<= template NR; extra lanes idle)
     int precond;         // CPM_PRECOND_*
+    int precond_iters;   // Jacobi sweeps
     KState* st;          // nr states
     double* part;        // [2 * NR][nblocks] partial sums of the last reducing kernel
 };

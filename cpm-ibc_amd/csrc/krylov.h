@@ -11,6 +11,7 @@ struct LaunchCfg {
     int nblocks;
     int nseg;
     hipStream_t stream;
+    int jacobi_sweeps;     // largest precond_iters over Jacobi-preconditioned segments (0: none)
 };
 
 // Workspace vectors of one precision; every vector spans all segments

@@ -164,17 +164,32 @@ __device__ __forceinline__ bool seg_any(const Seg<T>& sg, unsigned mask) {
     return a;
 }
 
-// Diagonal preconditioner, GeneralizedBiCGStabSystem.h:303-338 (the branch the
+// Diagonal_Inverse, GeneralizedBiCGStabSystem.h:303-338 (the branch the
 // reference runs: Compute() is never called).
 template <typename T>
-__device__ __forceinline__ T precond_apply(const Seg<T>& sg, int64_t row, uint8_t f, T val) {
-    if (sg.precond != 1 /*diagonal*/) return val;
+__device__ __forceinline__ T diag_inverse(const Seg<T>& sg, int64_t row, uint8_t f, T val) {
     const int64_t dof = sg.op.D - sg.op.nD;
     if (row >= dof) return val / (T)(sg.m - sg.n);   // :335-338 overwrite the trailing rows last
     if (f & ROW_IDENTITY) return val;
     if (f & ROW_NODIAG) return val;
     const T ld = sg.op.fmt == OP_TP ? sg.op.ldu : ldg(&sg.op.ldiag[row]);
     return val / ((T)sg.m + ld * (T)sg.n);
+}
+
+constexpr double kJacobiOmega = 2.0 / 3.0;   // GeneralizedBiCGStabSystem.h:364
+
+// Precondition, GeneralizedBiCGStabSystem.h:350-385.  Jacobi: the first of the
+// precond_iters damped sweeps from result = 0 (A 0 = 0), result = omega
+// D^-1 x on the rows below dofs and 0 on the trailing nD rows; the further
+// sweeps (k_jacobi) need an operator apply each.
+template <typename T>
+__device__ __forceinline__ T precond_apply(const Seg<T>& sg, int64_t row, uint8_t f, T val) {
+    if (sg.precond == 1 /*diagonal*/) return diag_inverse(sg, row, f, val);
+    if (sg.precond == 2 /*jacobi*/) {
+        if (sg.precond_iters <= 0 || row >= sg.op.D - sg.op.nD) return (T)0;
+        return (T)kJacobiOmega * diag_inverse(sg, row, f, val);
+    }
+    return val;
 }
 
 // E row of the tensor-product format: slot (i, j, k) in the reference's
@@ -473,6 +488,28 @@ __global__ __launch_bounds__(kBlock) void k_supdate(const Seg<T>* __restrict__ s
         vstore<T, NR>(p + i, pv);
         vstore<T, NR>(v + i, vv);
     }
+}
+
+// Jacobi sweep `sweep` (>= 1) of RUN lanes: y += omega D^-1 (x - A y) on the
+// rows below dofs, with A y already in ay (GeneralizedBiCGStabSystem.h:366-378).
+template <typename T, int NR>
+__global__ __launch_bounds__(kBlock) void k_jacobi(const Seg<T>* __restrict__ segs, const int* __restrict__ bseg,
+                                                   const T* x, const T* ay, T* y, int sweep) {
+    int lb;
+    int64_t row;
+    const Seg<T>& sg = seg_of(segs, bseg, lb, row);
+    if (sg.precond != 2 || sweep >= sg.precond_iters) return;
+    if (row >= sg.op.D - sg.op.nD || !seg_any(sg, M_RUN)) return;
+    const size_t i = (size_t)(sg.off + row * NR);
+    const uint8_t f = ldg(&sg.op.flag[row]);
+    auto xv = vload<T, NR>(x + i);
+    auto av = vload<T, NR>(ay + i);
+    auto yv = vload<T, NR>(y + i);
+#pragma unroll
+    for (int k = 0; k < NR; ++k)
+        if (lane_in(sg, k, M_RUN))
+            setlane(yv, k, lane(yv, k) + (T)kJacobiOmega * diag_inverse(sg, row, f, (T)(lane(xv, k) - lane(av, k))));
+    vstore<T, NR>(y + i, yv);
 }
 
 template <typename T, int NR>
@@ -787,10 +824,18 @@ void Launch<T, NR>::init(const LaunchCfg<T>& c, const T* b, T* x, T* r, T* r0, T
 template <typename T, int NR>
 void Launch<T, NR>::iteration(const LaunchCfg<T>& c, Vecs<T>& w, bool mixed, const MixVec& mv) {
     LAUNCH((k_pupdate<T, NR>), w.p, w.v, w.r, w.y, w.x, mv);
+    for (int sw = 1; sw < c.jacobi_sweeps; ++sw) {   // y = P^-1 p, Jacobi sweeps after the first
+        apply(c, w.y, w.tmp, w.t, M_RUN);
+        LAUNCH((k_jacobi<T, NR>), w.p, w.t, w.y, sw);
+    }
     LAUNCH((k_egather<T, NR>), w.y, w.tmp, M_STAGE1, mv);
     LAUNCH((k_lfinish<T, NR, 1>), w.y, w.tmp, w.v, w.r0, (const T*)nullptr, M_STAGE1, mv);
     FIN(T, NR, FIN_ALPHA);
     LAUNCH((k_supdate<T, NR>), w.r, w.v, w.s, w.z, w.r0, w.p, mv);
+    for (int sw = 1; sw < c.jacobi_sweeps; ++sw) {   // z = P^-1 s
+        apply(c, w.z, w.tmp, w.t, M_RUN);
+        LAUNCH((k_jacobi<T, NR>), w.s, w.t, w.z, sw);
+    }
     LAUNCH((k_egather<T, NR>), w.z, w.tmp, M_RUN, mv);
     LAUNCH((k_lfinish<T, NR, 2>), w.z, w.tmp, w.t, w.s, (const T*)w.r, M_STAGE2, mv);
     FIN(T, NR, FIN_OMEGA);

@@ -105,3 +105,23 @@ def test_edge_cases(gpu_ctx):
     bn[5] = np.nan
     x, info = op.solve(bn)
     assert info["status"] == "breakdown"
+
+
+@pytest.mark.parametrize("sweeps", [1, 3])
+def test_jacobi_preconditioner_matches_oracle(gpu_ctx, sweeps):
+    """Damped Jacobi preconditioning (GeneralizedBiCGStabSystem.h:360-380), as the
+    geodesic pipeline uses it (problems/GeodesicDistance.cpp:118-119)."""
+    print(f"case jacobi sweeps={sweeps}", flush=True)
+    P, L, E, I = _problem("Sphere", 0.1)
+    b = np.random.default_rng(9).standard_normal(P.D)
+    op = cpmgpu.Operator(gpu_ctx, L, E, I, 1.0, -1.0, precond=cpmgpu.PRECOND_JACOBI, precond_iters=sweeps)
+    x_ref, it_ref, _ = orc.solve(L, E, I, b, 1.0, -1.0, precond=2, precond_iters=sweeps, tol=1e-13, max_it=20000)
+    x, info = op.solve(b, cpmgpu.opts(tol=1e-13, max_iters=20000))
+    assert info["status"] == "converged"
+    assert np.linalg.norm(x - x_ref) / np.linalg.norm(x_ref) <= 1e-10
+    _, it10, _ = orc.solve(L, E, I, b, 1.0, -1.0, precond=2, precond_iters=sweeps, tol=1e-10)
+    x10, info10 = op.solve(b)
+    assert abs(info10["iters"] - it10) <= max(3, 0.1 * it10)
+    x32, i32 = op.solve(b, cpmgpu.opts(tol=1e-11, precision=cpmgpu.CPM_FP32_REFINE))
+    assert i32["status"] == "converged"
+    assert np.linalg.norm(x32 - x_ref) / np.linalg.norm(x_ref) <= 1e-8

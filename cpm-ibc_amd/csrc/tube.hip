@@ -32,6 +32,142 @@ namespace cpmgpu {
 // ---------------------------------------------------------------- closest points
 __device__ __forceinline__ double dsq(double a) { return a * a; }
 
+// Ellipsoid closest point: Eberly, "Distance from a Point to an Ellipse, an
+// Ellipsoid, or a Hyperellipsoid" (Geometric Tools, 2013).  In the first
+// octant with semi-axes e0 >= e1 >= e2 the closest point is
+// x_i = r_i y_i / (s + r_i), r_i = (e_i / e_last)^2, where s is the root of
+// sum_i (r_i z_i / (s + r_i))^2 = 1 (z_i = y_i / e_i), found by bisection until
+// the bracket stops shrinking.  Only + - * / and sqrt: the host oracle gets
+// the same bits.  (The reference runs LBFGS from the nearest of 10000
+// parameter points, Surface.cpp:1363-1400,1459-1530, to tolerance 1e-6.)
+__device__ double ell_robust_length(const double* v, int n) {
+    double m = 0.0;
+    for (int i = 0; i < n; ++i) m = fmax(m, fabs(v[i]));
+    if (m == 0.0) return 0.0;
+    double s = 0.0;
+    for (int i = 0; i < n; ++i) s += (v[i] / m) * (v[i] / m);
+    return m * sqrt(s);
+}
+
+__device__ double ell_root(int n, const double* r, const double* z, double g) {
+    double num[3];
+    for (int i = 0; i < n - 1; ++i) num[i] = r[i] * z[i];
+    num[n - 1] = z[n - 1];
+    double s0 = z[n - 1] - 1.0, s1 = g < 0.0 ? 0.0 : ell_robust_length(num, n) - 1.0, s = 0.0;
+    for (int it = 0; it < 1100; ++it) {
+        s = 0.5 * (s0 + s1);
+        if (s == s0 || s == s1) break;
+        double gs = -1.0;
+        for (int i = 0; i < n; ++i) {
+            const double q = num[i] / (s + r[i]);
+            gs += q * q;
+        }
+        if (gs > 0.0) s0 = s;
+        else if (gs < 0.0) s1 = s;
+        else break;
+    }
+    return s;
+}
+
+__device__ void ell_point2(double e0, double e1, double y0, double y1, double& x0, double& x1) {
+    if (y1 > 0.0) {
+        if (y0 > 0.0) {
+            const double z[2] = {y0 / e0, y1 / e1};
+            const double g = z[0] * z[0] + z[1] * z[1] - 1.0;
+            if (g != 0.0) {
+                const double r[2] = {(e0 / e1) * (e0 / e1), 1.0};
+                const double s = ell_root(2, r, z, g);
+                x0 = r[0] * y0 / (s + r[0]);
+                x1 = y1 / (s + 1.0);
+            } else {
+                x0 = y0;
+                x1 = y1;
+            }
+        } else {
+            x0 = 0.0;
+            x1 = e1;
+        }
+    } else {
+        const double num0 = e0 * y0, den0 = e0 * e0 - e1 * e1;
+        if (num0 < den0) {
+            const double xd0 = num0 / den0;
+            x0 = e0 * xd0;
+            x1 = e1 * sqrt(1.0 - xd0 * xd0);
+        } else {
+            x0 = e0;
+            x1 = 0.0;
+        }
+    }
+}
+
+__device__ void ell_point3(const double* e, const double* y, double* x) {
+    if (y[2] > 0.0) {
+        if (y[1] > 0.0) {
+            if (y[0] > 0.0) {
+                const double z[3] = {y[0] / e[0], y[1] / e[1], y[2] / e[2]};
+                const double g = z[0] * z[0] + z[1] * z[1] + z[2] * z[2] - 1.0;
+                if (g != 0.0) {
+                    const double r[3] = {(e[0] / e[2]) * (e[0] / e[2]), (e[1] / e[2]) * (e[1] / e[2]), 1.0};
+                    const double s = ell_root(3, r, z, g);
+                    x[0] = r[0] * y[0] / (s + r[0]);
+                    x[1] = r[1] * y[1] / (s + r[1]);
+                    x[2] = y[2] / (s + 1.0);
+                } else {
+                    x[0] = y[0];
+                    x[1] = y[1];
+                    x[2] = y[2];
+                }
+            } else {
+                x[0] = 0.0;
+                ell_point2(e[1], e[2], y[1], y[2], x[1], x[2]);
+            }
+        } else if (y[0] > 0.0) {
+            x[1] = 0.0;
+            ell_point2(e[0], e[2], y[0], y[2], x[0], x[2]);
+        } else {
+            x[0] = 0.0;
+            x[1] = 0.0;
+            x[2] = e[2];
+        }
+    } else {
+        const double den0 = e[0] * e[0] - e[2] * e[2], den1 = e[1] * e[1] - e[2] * e[2];
+        const double num0 = e[0] * y[0], num1 = e[1] * y[1];
+        bool done = false;
+        if (num0 < den0 && num1 < den1) {
+            const double xd0 = num0 / den0, xd1 = num1 / den1;
+            const double disc = 1.0 - xd0 * xd0 - xd1 * xd1;
+            if (disc > 0.0) {
+                x[0] = e[0] * xd0;
+                x[1] = e[1] * xd1;
+                x[2] = e[2] * sqrt(disc);
+                done = true;
+            }
+        }
+        if (!done) {
+            x[2] = 0.0;
+            ell_point2(e[0], e[1], y[0], y[1], x[0], x[1]);
+        }
+    }
+}
+
+__device__ void cp_ellipsoid(const double* abc, const double* x, double* cp) {
+    int perm[3] = {0, 1, 2};   // axes by decreasing semi-axis
+    for (int i = 0; i < 3; ++i)
+        for (int j = i + 1; j < 3; ++j)
+            if (abc[perm[j]] > abc[perm[i]]) {
+                const int t = perm[i];
+                perm[i] = perm[j];
+                perm[j] = t;
+            }
+    double e[3], y[3], q[3];
+    for (int k = 0; k < 3; ++k) {
+        e[k] = abc[perm[k]];
+        y[k] = fabs(x[perm[k]]);
+    }
+    ell_point3(e, y, q);
+    for (int k = 0; k < 3; ++k) cp[perm[k]] = copysign(q[k], x[perm[k]]);
+}
+
 __device__ void cp_surface(const SurfDev& s, const double* x, double* c, double& dist, int& bdy) {
     const double* P = s.P;
     bdy = 0;
@@ -114,6 +250,11 @@ __device__ void cp_surface(const SurfDev& s, const double* x, double* c, double&
             for (int d = 0; d < 3; ++d) c[d] = cen[d] + R * t[d] / nt;
             dist = R;
         }
+        break;
+    }
+    case SURF_ELLIPSOID: {
+        cp_ellipsoid(s.P, x, c);
+        dist = sqrt((x[0] - c[0]) * (x[0] - c[0]) + (x[1] - c[1]) * (x[1] - c[1]) + (x[2] - c[2]) * (x[2] - c[2]));
         break;
     }
     case SURF_POLYLINE: {   // exact FP64 segment projection, first minimum wins
@@ -411,6 +552,7 @@ SurfaceHost surface_from_desc(const char* type, const double* params, int nparam
         if (s.type == "Sphere" || s.type == "Hemisphere") { s.params = {1.0}; bb0 = -1; bb1 = 1; }
         else if (s.type == "Torus") { s.params = {2.0, 1.0}; bb0 = -3; bb1 = 3; }
         else if (s.type == "Circle3D") { s.params = {1, 0, 0, 0, 0, 0, 1}; bb0 = -1; bb1 = 1; }
+        else if (s.type == "Ellipsoid") { s.params = {1.5, 1.0, 0.75}; bb0 = -1.5; bb1 = 1.5; }
         else throw ArgError("no default parameters for surface '" + s.type + "'");
     } else {
         s.params.assign(params, params + nparams);
@@ -431,6 +573,11 @@ SurfDev make_surfdev(const SurfaceHost& s, const double* dverts) {
     else if (s.type == "Torus") d.type = SURF_TORUS;
     else if (s.type == "Circle3D") d.type = SURF_CIRCLE3D;
     else if (s.type == "Point") d.type = SURF_POINT;
+    else if (s.type == "Ellipsoid") {
+        if (s.params.size() < 3 || !(s.params[0] > 0 && s.params[1] > 0 && s.params[2] > 0))
+            throw ArgError("Ellipsoid needs three positive semi-axes");
+        d.type = SURF_ELLIPSOID;
+    }
     else if (s.type == "Polyline") {
         d.type = SURF_POLYLINE;
         d.verts = dverts;

@@ -182,6 +182,22 @@ void Surface::parameterise() {
             xp[i][1] = (R + r * std::cos(thetap[i])) * std::sin(phip[i]);
             xp[i][2] = r * std::sin(thetap[i]);
         }
+    } else if (ty == "Ellipsoid") {               // :341-349, 2480-2511, Np = 10000
+        size_t Np = 10000, sq_n = 100;
+        double a = P[0], b = P[1], c = P[2], du = 2 * M_PI / (sq_n - 1), dv = M_PI / (sq_n - 1);
+        xp.assign(Np, Vec(3));
+        thetap.assign(Np, 0);
+        phip.assign(Np, 0);
+        for (size_t i = 0; i < sq_n; ++i)
+            for (size_t j = 0; j < sq_n; ++j) {
+                phip[i * sq_n + j] = i * du;
+                thetap[i * sq_n + j] = j * dv;
+            }
+        for (size_t i = 0; i < Np; ++i) {
+            xp[i][0] = a * std::sin(thetap[i]) * std::cos(phip[i]);
+            xp[i][1] = b * std::sin(thetap[i]) * std::sin(phip[i]);
+            xp[i][2] = c * std::cos(thetap[i]);
+        }
     } else if (ty == "Point") {
         xp.assign(1, Vec(P.begin(), P.begin() + s_.dim));
     } else if (ty == "Circle3D") {                // :1883-1904, Np = 100
@@ -203,6 +219,138 @@ void Surface::parameterise() {
         }
     }
     // other surfaces: no parameterisation needed by the oracle
+}
+
+// ---------------------------------------------------------------- ellipsoid closest point
+// Eberly, "Distance from a Point to an Ellipse, an Ellipsoid, or a
+// Hyperellipsoid" (Geometric Tools, 2013): reduce to the first octant with
+// semi-axes e0 >= e1 >= e2, then find the root s of
+//   sum_i (r_i z_i / (s + r_i))^2 = 1,  r_i = (e_i / e_last)^2, z_i = y_i / e_i,
+// by bisection until the interval stops shrinking; x_i = r_i y_i / (s + r_i).
+static double robust_length(const double* v, int n) {
+    double m = 0.0;
+    for (int i = 0; i < n; ++i) m = std::max(m, std::fabs(v[i]));
+    if (m == 0.0) return 0.0;
+    double s = 0.0;
+    for (int i = 0; i < n; ++i) s += (v[i] / m) * (v[i] / m);
+    return m * std::sqrt(s);
+}
+
+static double ellipsoid_root(int n, const double* r, const double* z, double g) {
+    // r[n-1] == 1
+    double num[3];
+    for (int i = 0; i < n - 1; ++i) num[i] = r[i] * z[i];
+    num[n - 1] = z[n - 1];
+    double s0 = z[n - 1] - 1.0, s1 = g < 0.0 ? 0.0 : robust_length(num, n) - 1.0, s = 0.0;
+    for (int it = 0; it < 1100; ++it) {
+        s = 0.5 * (s0 + s1);
+        if (s == s0 || s == s1) break;
+        double gs = -1.0;
+        for (int i = 0; i < n; ++i) {
+            const double q = num[i] / (s + r[i]);
+            gs += q * q;
+        }
+        if (gs > 0.0) s0 = s;
+        else if (gs < 0.0) s1 = s;
+        else break;
+    }
+    return s;
+}
+
+static void ellipse_point(double e0, double e1, double y0, double y1, double& x0, double& x1) {
+    if (y1 > 0.0) {
+        if (y0 > 0.0) {
+            const double z[2] = {y0 / e0, y1 / e1};
+            const double g = z[0] * z[0] + z[1] * z[1] - 1.0;
+            if (g != 0.0) {
+                const double r[2] = {(e0 / e1) * (e0 / e1), 1.0};
+                const double s = ellipsoid_root(2, r, z, g);
+                x0 = r[0] * y0 / (s + r[0]);
+                x1 = y1 / (s + 1.0);
+            } else {
+                x0 = y0;
+                x1 = y1;
+            }
+        } else {
+            x0 = 0.0;
+            x1 = e1;
+        }
+    } else {
+        const double num0 = e0 * y0, den0 = e0 * e0 - e1 * e1;
+        if (num0 < den0) {
+            const double xd0 = num0 / den0;
+            x0 = e0 * xd0;
+            x1 = e1 * std::sqrt(1.0 - xd0 * xd0);
+        } else {
+            x0 = e0;
+            x1 = 0.0;
+        }
+    }
+}
+
+static void ellipsoid_point(const double* e, const double* y, double* x) {
+    if (y[2] > 0.0) {
+        if (y[1] > 0.0) {
+            if (y[0] > 0.0) {
+                const double z[3] = {y[0] / e[0], y[1] / e[1], y[2] / e[2]};
+                const double g = z[0] * z[0] + z[1] * z[1] + z[2] * z[2] - 1.0;
+                if (g != 0.0) {
+                    const double r[3] = {(e[0] / e[2]) * (e[0] / e[2]), (e[1] / e[2]) * (e[1] / e[2]), 1.0};
+                    const double s = ellipsoid_root(3, r, z, g);
+                    x[0] = r[0] * y[0] / (s + r[0]);
+                    x[1] = r[1] * y[1] / (s + r[1]);
+                    x[2] = y[2] / (s + 1.0);
+                } else {
+                    x[0] = y[0];
+                    x[1] = y[1];
+                    x[2] = y[2];
+                }
+            } else {
+                x[0] = 0.0;
+                ellipse_point(e[1], e[2], y[1], y[2], x[1], x[2]);
+            }
+        } else if (y[0] > 0.0) {
+            x[1] = 0.0;
+            ellipse_point(e[0], e[2], y[0], y[2], x[0], x[2]);
+        } else {
+            x[0] = 0.0;
+            x[1] = 0.0;
+            x[2] = e[2];
+        }
+    } else {
+        const double den0 = e[0] * e[0] - e[2] * e[2], den1 = e[1] * e[1] - e[2] * e[2];
+        const double num0 = e[0] * y[0], num1 = e[1] * y[1];
+        bool done = false;
+        if (num0 < den0 && num1 < den1) {
+            const double xd0 = num0 / den0, xd1 = num1 / den1;
+            const double disc = 1.0 - xd0 * xd0 - xd1 * xd1;
+            if (disc > 0.0) {
+                x[0] = e[0] * xd0;
+                x[1] = e[1] * xd1;
+                x[2] = e[2] * std::sqrt(disc);
+                done = true;
+            }
+        }
+        if (!done) {
+            x[2] = 0.0;
+            ellipse_point(e[0], e[1], y[0], y[1], x[0], x[1]);
+        }
+    }
+}
+
+// Ellipsoid x^2/a^2 + y^2/b^2 + z^2/c^2 = 1 centred at the origin.
+void cp_ellipsoid(const double* abc, const double* x, double* cp) {
+    int perm[3] = {0, 1, 2};   // axes by decreasing semi-axis (stable)
+    for (int i = 0; i < 3; ++i)
+        for (int j = i + 1; j < 3; ++j)
+            if (abc[perm[j]] > abc[perm[i]]) std::swap(perm[i], perm[j]);
+    double e[3], y[3], q[3];
+    for (int k = 0; k < 3; ++k) {
+        e[k] = abc[perm[k]];
+        y[k] = std::fabs(x[perm[k]]);
+    }
+    ellipsoid_point(e, y, q);
+    for (int k = 0; k < 3; ++k) cp[perm[k]] = std::copysign(q[k], x[perm[k]]);
 }
 
 void Surface::cp(const double* x, double* cpx, double& dist, i64& bdy) const {
@@ -287,6 +435,14 @@ void Surface::cp(const double* x, double* cpx, double& dist, i64& bdy) const {
             for (int d = 0; d < 3; ++d) cpx[d] = c[d] + R * t[d] / nt;
             dist = R;
         }
+    } else if (ty == "Ellipsoid") {
+        // The reference minimises |r(u,v) - x|^2 with LBFGS from the nearest of
+        // 10000 parameter points (Surface.cpp:1363-1400,1459-1530; LBFGSpp,
+        // epsilon 1e-6).  Here: the exact closest point by Eberly's robust
+        // bisection ("Distance from a Point to an Ellipse, an Ellipsoid, or a
+        // Hyperellipsoid"), which the LBFGS iteration approximates.
+        cp_ellipsoid(P.data(), x, cpx);
+        dist = distance(x, cpx, 3);
     } else if (ty == "Polyline") {
         cp_polyline(x, cpx, dist, bdy);
     } else if (ty == "Triangulation") {

@@ -73,6 +73,9 @@ struct SurfaceSpec {
 
 // Surface, src/Surface.cpp.  Closest point + parameterisation points used for
 // plotting / error measurement.
+// Exact closest point on the ellipsoid x^2/a^2 + y^2/b^2 + z^2/c^2 = 1 (abc = a, b, c).
+void cp_ellipsoid(const double* abc, const double* x, double* cp);
+
 class Surface {
 public:
     Surface() = default;

@@ -11,7 +11,8 @@ pytestmark = pytest.mark.gpu
 CASES = [("Sphere", 0.2, None, None), ("Sphere", 0.1, None, None), ("Sphere", 0.05, None, None),
          ("Torus", 0.2, None, None), ("Hemisphere", 0.1, None, None),
          ("Torus", 0.02, [1.0, 0.4], (-1.4, 1.4)),            # the torus of the IBC workload, h = 0.02
-         ("Sphere", 0.1, [0.7, 0.1, -0.05, 0.02], (-0.7, 0.85))]
+         ("Sphere", 0.1, [0.7, 0.1, -0.05, 0.02], (-0.7, 0.85)),
+         ("Ellipsoid", 0.1, None, None), ("Ellipsoid", 0.05, [0.6, 1.3, 0.9], (-1.3, 1.3))]
 
 
 @pytest.mark.parametrize("surface,dx,params,bbox", CASES)

@@ -143,7 +143,7 @@ OpView<float> Op::view<float>() const { return make_view<float>(*this, eval32, l
 size_t Op::device_bytes() const {
     return ecol.bytes + eval64.bytes + eval32.bytes + lcol.bytes + lval64.bytes + lval32.bytes + ldiag64.bytes +
            ldiag32.bytes + flag.bytes + perm.bytes + tpc.bytes + tpw64.bytes + tpw32.bytes + exc.bytes + ecx.bytes +
-           xcol.bytes + xval64.bytes + xval32.bytes;
+           xcol.bytes + xval64.bytes + xval32.bytes + hrow.bytes + hcol.bytes + hval.bytes;
 }
 
 Op* op_from_csr(Ctx* ctx, int64_t D, int nD, const int32_t* Lp, const int32_t* Lc, const double* Lv,
@@ -197,6 +197,38 @@ Op* op_from_csr(Ctx* ctx, int64_t D, int nD, const int32_t* Lp, const int32_t* L
     }
     std::unique_ptr<Op> op(new Op(ctx, D, nD));
     op->upload_ell(ec, ev, ew, lc, lv, lw, ld, fl);
+    // full L rows (diagonal included) of the identity rows
+    std::vector<int> hr;
+    int hw = 1;
+    for (int64_t r = 0; r < D; ++r)
+        if (fl[(size_t)r] & ROW_IDENTITY) {
+            hr.push_back((int)r);
+            hw = std::max(hw, Lp[r + 1] - Lp[r]);
+        }
+    if (!hr.empty()) {
+        const size_t nh = hr.size();
+        std::vector<int> hc((size_t)hw * nh);
+        std::vector<double> hv((size_t)hw * nh, 0.0);
+        for (size_t k = 0; k < nh; ++k) {
+            const int r = hr[k];
+            int s = 0;
+            for (int32_t q = Lp[r]; q < Lp[r + 1]; ++q, ++s) {
+                hc[(size_t)s * nh + k] = Lc[q];
+                hv[(size_t)s * nh + k] = Lv[q];
+            }
+            for (; s < hw; ++s) hc[(size_t)s * nh + k] = r;
+        }
+        auto up = [](Buffer& b, const void* src, size_t bytes) {
+            b.~Buffer();
+            new (&b) Buffer(bytes);
+            HIPCHK(hipMemcpy(b.p, src, bytes, hipMemcpyHostToDevice));
+        };
+        up(op->hrow, hr.data(), nh * sizeof(int));
+        up(op->hcol, hc.data(), hc.size() * sizeof(int));
+        up(op->hval, hv.data(), hv.size() * sizeof(double));
+        op->nh = (int64_t)nh;
+        op->hw = hw;
+    }
     op->m = m;
     op->n = n;
     op->precond = precond;
@@ -496,6 +528,85 @@ static void profile_egather_t(Op* op, int reps, double* avg_ms, double* bytes) {
     *bytes = opb + 2.0 * op->D * NR * sizeof(T) + (double)op->D;
 }
 
+// Identity rows of a caller-order vector y take the values of g.
+__global__ void k_set_identity_rows(const uint8_t* flag, const int* perm, int64_t D, const double* g, double* y) {
+    const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= D || !(flag[r] & ROW_IDENTITY)) return;
+    const int64_t cr = perm ? perm[r] : r;
+    y[cr] = g[cr];
+}
+
+// z = x + dt (Lfull x) on the identity rows, from the operator's held Lfull rows.
+__global__ void k_held_apply(int64_t nh, int hw, const int* hrow, const int* hcol, const double* hval,
+                             const uint8_t* flag, double dt, const double* x, double* z) {
+    const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= nh) return;
+    const int r = hrow[k];
+    if (!(flag[r] & ROW_IDENTITY)) return;
+    double acc = 0.0;
+    for (int s = 0; s < hw; ++s) acc += hval[(int64_t)s * nh + k] * x[hcol[(int64_t)s * nh + k]];
+    z[r] = x[r] + dt * acc;
+}
+
+// HeatWithIBC::TimeStep (problems/HeatWithIBC.cpp:80-103).
+static void heat_step(Op* op, int method, double dt, const double* d_g, double* d_u, const cpm_solve_opts& o,
+                      cpm_solve_info* info) {
+    Ctx* ctx = op->ctx;
+    const hipStream_t st = ctx->stream;
+    const int64_t D = op->D;
+    const unsigned nb = (unsigned)((D + 255) / 256);
+    if (method == 0) {
+        // internal order: x = P u ; z = x + dt (L_ii x + L_off x) ; y = E z ; u = P^T y
+        const int nblocks = (int)((D + kBlock - 1) / kBlock);
+        char* tb = reinterpret_cast<char*>(ctx->workspace("heat_tables", 8192 + sizeof(int) * nblocks));
+        Seg<double> h{};
+        fill_seg(h, op, 0, 0, nblocks, 1, nullptr, nullptr);
+        h.m = 1.0;
+        h.n = dt;
+        std::vector<int> hb(nblocks, 0);
+        PackEntry pe[2]{};
+        pe[0].src = d_u;
+        pe[1].dst = d_u;
+        for (auto& e : pe) {
+            e.perm = op->perm.as<int>();
+            e.D = D;
+            e.nr = 1;
+        }
+        auto* dseg = reinterpret_cast<Seg<double>*>(tb);
+        auto* dpe = reinterpret_cast<PackEntry*>(tb + 4096);
+        int* dbseg = reinterpret_cast<int*>(tb + 8192);
+        HIPCHK(hipMemcpyAsync(dseg, &h, sizeof(h), hipMemcpyHostToDevice, st));
+        HIPCHK(hipMemcpyAsync(dpe, pe, sizeof(pe), hipMemcpyHostToDevice, st));
+        HIPCHK(hipMemcpyAsync(dbseg, hb.data(), sizeof(int) * nblocks, hipMemcpyHostToDevice, st));
+        double* w = ctx->workspace("heat_vecs", sizeof(double) * D * 3);
+        double *xi = w, *zi = w + D, *yi = w + 2 * D;
+        LaunchCfg<double> c{dseg, dbseg, nblocks, 1, st, 0};
+        hipLaunchKernelGGL(k_pack, dim3(nblocks), dim3(256), 0, st, dpe, dbseg, xi, (float*)nullptr, 1);
+        launch_lfinish_plain<double>(c, xi, xi, zi);   // L applied to x itself: tmp = x
+        if (op->nh > 0)
+            hipLaunchKernelGGL(k_held_apply, dim3((unsigned)((op->nh + 255) / 256)), dim3(256), 0, st, op->nh, op->hw,
+                               op->hrow.as<int>(), op->hcol.as<int>(), op->hval.as<double>(), op->flag.as<uint8_t>(),
+                               dt, xi, zi);
+        launch_egather<double, 1>(c, zi, yi);
+        hipLaunchKernelGGL(k_pack, dim3(nblocks), dim3(256), 0, st, dpe + 1, dbseg, yi, (float*)nullptr, 0);
+        hipLaunchKernelGGL(k_set_identity_rows, dim3(nb), dim3(256), 0, st, op->flag.as<uint8_t>(), op->perm.as<int>(),
+                           D, d_g, d_u);
+        HIPCHK(hipGetLastError());
+        HIPCHK(hipStreamSynchronize(st));
+        if (info) *info = cpm_solve_info{CPM_SOLVE_CONVERGED, 0, 0, 0, 0.0};
+        return;
+    }
+    double* rhs = ctx->workspace("heat_rhs", sizeof(double) * D);
+    if (method == 2) multiply(op, d_u, rhs, 1.0, 0.5 * dt);
+    else HIPCHK(hipMemcpyAsync(rhs, d_u, sizeof(double) * D, hipMemcpyDeviceToDevice, st));
+    hipLaunchKernelGGL(k_set_identity_rows, dim3(nb), dim3(256), 0, st, op->flag.as<uint8_t>(), op->perm.as<int>(), D,
+                       d_g, rhs);
+    HIPCHK(hipGetLastError());
+    std::vector<cpm_solve_info> infos;
+    solve_batch(ctx, {op}, {rhs}, {d_u}, o, infos);
+    if (info) *info = infos[0];
+}
+
 __global__ void k_copy16(const uint4* __restrict__ src, uint4* __restrict__ dst, int64_t n) {
     const int64_t stride = (int64_t)gridDim.x * blockDim.x;
     for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) dst[i] = src[i];
@@ -767,6 +878,18 @@ int cpm_profile_egather(cpm_op* op, int precision, int nrhs, int reps, double* a
         case 8: f32 ? profile_egather_t<float, 8>(o, reps, avg_ms, bytes) : profile_egather_t<double, 8>(o, reps, avg_ms, bytes); break;
         default: throw ArgError("nrhs must be 1, 2, 4 or 8");
         }
+    });
+}
+
+int cpm_heat_step_device(cpm_op* op, int method, double dt, const double* d_g, double* d_u,
+                         const cpm_solve_opts* opts, cpm_solve_info* info) {
+    return guard([&] {
+        if (!op || !d_g || !d_u) throw ArgError("null argument");
+        if (method < 0 || method > 2) throw ArgError("method must be 0 (explicit), 1 (backward Euler) or 2 (Crank-Nicolson)");
+        if (!(dt > 0)) throw ArgError("dt must be positive");
+        if (O(op)->nD != 0) throw ArgError("time stepping needs an operator without trailing nD rows");
+        HIPCHK(hipSetDevice(O(op)->ctx->device));
+        heat_step(O(op), method, dt, d_g, d_u, resolve_opts(opts), info);
     });
 }
 

@@ -872,6 +872,42 @@ Problem* problem_build(Tube* t, const std::vector<IbcSpec>& specs) {
     return P.release();
 }
 
+// Lfull rows N..D-1 (ghost and identity rows, whose internal index equals the
+// reference one) for explicit stepping: slot 0 the diagonal, then the six
+// off-diagonal slots; columns of tube nodes renumbered by rlex when the
+// operator runs in the lexicographic order.
+__global__ void k_held_rows(int64_t N, int64_t D, const int* lcol, const double* lval, const double* ldiag,
+                            const int32_t* rlex, int* hrow, int* hcol, double* hval) {
+    const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t nh = D - N;
+    if (k >= nh) return;
+    const int64_t r = N + k;
+    hrow[k] = (int)r;
+    hcol[k] = (int)r;
+    hval[k] = ldiag[r];
+    for (int s = 0; s < 6; ++s) {
+        const int c = lcol[(int64_t)s * D + r];
+        hcol[(s + 1) * nh + k] = (rlex && c < N) ? rlex[c] : c;
+        hval[(s + 1) * nh + k] = lval[(int64_t)s * D + r];
+    }
+}
+
+static void attach_held_rows(Op& op, Problem* P, const int32_t* rlex) {
+    const int64_t nh = P->D - P->N;
+    if (nh <= 0) return;
+    const hipStream_t st = P->ctx->stream;
+    op.hrow = Buffer((size_t)nh * 4);
+    op.hcol = Buffer((size_t)nh * 7 * 4);
+    op.hval = Buffer((size_t)nh * 7 * 8);
+    hipLaunchKernelGGL(k_held_rows, dim3((unsigned)((nh + 255) / 256)), dim3(256), 0, st, P->N, P->D,
+                       P->lcol.as<int>(), P->lval64.as<double>(), P->ldiag64.as<double>(), rlex,
+                       op.hrow.as<int>(), op.hcol.as<int>(), op.hval.as<double>());
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipStreamSynchronize(st));
+    op.nh = nh;
+    op.hw = 7;
+}
+
 Op* op_from_problem(Problem* P, double m, double n, int precond, int precond_iters) {
     const hipStream_t st = P->ctx->stream;
     const int64_t D = P->D;
@@ -892,7 +928,10 @@ Op* op_from_problem(Problem* P, double m, double n, int precond, int precond_ite
     src.ldiag = P->ldiag64.as<double>();
     src.flag = P->flag.as<uint8_t>();
     src.qpts = P->qpts.as<double>();
-    if (!tp_disabled() && op_build_tp(*op, *P->tube, src)) return op.release();
+    if (!tp_disabled() && op_build_tp(*op, *P->tube, src)) {
+        attach_held_rows(*op, P, P->tube->rlex.as<int32_t>());
+        return op.release();
+    }
     op->ew = P->ew;
     op->lw = 6;
     auto cp = [&](Buffer& dst, const Buffer& src) {
@@ -910,6 +949,7 @@ Op* op_from_problem(Problem* P, double m, double n, int precond, int precond_ite
     cp(op->ldiag32, P->ldiag32);
     cp(op->flag, P->flag);
     HIPCHK(hipStreamSynchronize(st));
+    attach_held_rows(*op, P, nullptr);
     op->m = m;
     op->n = n;
     op->precond = precond;

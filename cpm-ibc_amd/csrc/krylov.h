@@ -50,6 +50,10 @@ struct MixedLaunch {
     static void down(const LaunchCfg<double>& c, const double* r64, Vecs<float>& w);
 };
 
+// out = m x + n (L_ii x + L_off tmp) alone (the tail of an apply).
+template <typename T>
+void launch_lfinish_plain(const LaunchCfg<T>& c, const T* x, const T* tmp, T* out);
+
 // E-gather alone (identity rows copy), for measurement.
 template <typename T, int NR>
 void launch_egather(const LaunchCfg<T>& c, const T* x, T* out);

@@ -858,6 +858,13 @@ void Launch<T, NR>::apply(const LaunchCfg<T>& c, const T* x, T* tmp, T* y, unsig
     LAUNCH((k_lfinish<T, NR, 0>), x, tmp, y, (const T*)nullptr, (const T*)nullptr, lanes, none);
 }
 
+template <typename T>
+void launch_lfinish_plain(const LaunchCfg<T>& c, const T* x, const T* tmp, T* out) {
+    const MixVec none{};
+    LAUNCH((k_lfinish<T, 1, 0>), x, tmp, out, (const T*)nullptr, (const T*)nullptr, 0u, none);
+}
+template void launch_lfinish_plain<double>(const LaunchCfg<double>&, const double*, const double*, double*);
+
 template <typename T, int NR>
 void launch_egather(const LaunchCfg<T>& c, const T* x, T* out) {
     const MixVec none{};

@@ -45,6 +45,13 @@ struct Op {
     Buffer tpc, tpw64, tpw32, exc, ecx, xcol, xval64, xval32;
     int64_t nexc = 0, nmoved = 0;
     double loff = 0, ldu = 0;
+    // Lfull rows of the rows that solves treat as identity rows (internal
+    // order, hw slots each, padded with weight 0): explicit time stepping
+    // applies Lfull to every row (problems/HeatWithIBC.cpp:84-88).  Entries
+    // whose row is not an identity row are skipped.
+    Buffer hrow, hcol, hval;  // int32 [nh], int32 [hw][nh], double [hw][nh]
+    int64_t nh = 0;
+    int hw = 0;
     Op() = default;
     Op(Ctx* c, int64_t D_, int nD_);
     void upload_ell(const std::vector<int>& ec, const std::vector<double>& ev, int ew,

@@ -23,7 +23,7 @@ EXPORTS = [
     "cpm_op_set_precond", "cpm_multiply", "cpm_multiply_device", "cpm_solve", "cpm_solve_device",
     "cpm_solve_batch", "cpm_solve_batch_device", "cpm_device_alloc", "cpm_device_free", "cpm_memcpy_h2d",
     "cpm_memcpy_d2h", "cpm_tube_create", "cpm_tube_destroy", "cpm_tube_get_info", "cpm_tube_export",
-    "cpm_tube_export_E", "cpm_tube_export_L", "cpm_tube_interp_matrix", "cpm_op_create_from_tube", "cpm_profile_egather", "cpm_profile_copy", "cpm_problem_create", "cpm_problem_destroy",
+    "cpm_tube_export_E", "cpm_tube_export_L", "cpm_tube_interp_matrix", "cpm_op_create_from_tube", "cpm_profile_egather", "cpm_profile_copy", "cpm_heat_step_device", "cpm_problem_create", "cpm_problem_destroy",
     "cpm_problem_get_info", "cpm_problem_rhs_points", "cpm_problem_subset", "cpm_problem_export",
     "cpm_op_create_from_problem",
 ]
@@ -126,6 +126,8 @@ def load(path=LIB_PATH):
     L.cpm_tube_interp_matrix.argtypes = [vp, vp, ctypes.c_int64, vp, vp, vp]
     L.cpm_op_create_from_tube.argtypes = [vp, ctypes.c_double, ctypes.c_double, ctypes.c_int, ctypes.c_int,
                                           ctypes.POINTER(vp)]
+    L.cpm_heat_step_device.argtypes = [vp, ctypes.c_int, ctypes.c_double, vp, vp, ctypes.POINTER(SolveOpts),
+                                       ctypes.POINTER(SolveInfo)]
     L.cpm_profile_copy.argtypes = [vp, ctypes.c_int64, ctypes.c_int, ctypes.POINTER(ctypes.c_double)]
     L.cpm_profile_egather.argtypes = [vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.POINTER(ctypes.c_double),
                                       ctypes.POINTER(ctypes.c_double)]
@@ -289,6 +291,12 @@ class Operator:
         by = ctypes.c_double()
         _chk(load().cpm_profile_egather(self._h, precision, nrhs, reps, ctypes.byref(ms), ctypes.byref(by)))
         return ms.value, by.value
+
+    def heat_step_device(self, method, dt, d_g, d_u, o=None):
+        """One HeatWithIBC time step on device vectors (0 explicit, 1 backward Euler, 2 Crank-Nicolson)."""
+        info = SolveInfo()
+        _chk(load().cpm_heat_step_device(self._h, method, dt, d_g, d_u, ctypes.byref(o or opts()), ctypes.byref(info)))
+        return info.as_dict()
 
     def solve_device(self, d_rhs, d_x, o=None):
         info = SolveInfo()

@@ -236,6 +236,20 @@ int cpm_problem_export(const cpm_problem* prob, int32_t* L_rowptr, int32_t* L_co
  * problem's Lfull, Efull, identity rows). */
 int cpm_op_create_from_problem(cpm_problem* prob, double m, double n, int precond, int precond_iters, cpm_op** out);
 
+/* ---- time stepping ------------------------------------------------------
+ * One step of HeatWithIBC::TimeStep (problems/HeatWithIBC.cpp:80-103) on
+ * device vectors of the operator's D rows (the problem's row order):
+ *   method 0, explicit:         u <- E (u + dt L u), then identity rows <- g
+ *   method 1, backward Euler:   rhs = u,             identity rows <- g, u <- A^-1 rhs
+ *   method 2, Crank-Nicolson:   rhs = A(1, dt/2) u,  identity rows <- g, u <- A^-1 rhs
+ * For methods 1 and 2 `op` is the problem operator with (m, n) = (1, -dt) or
+ * (1, -dt/2), as HeatWithIBC::Initialize builds it (:160-166); for method 0 its
+ * L and E are applied and (m, n) are ignored.  d_g holds this step's Dirichlet
+ * values on the identity rows (SetDirichletValue, :50-78); other entries are
+ * not read.  info may be NULL (method 0 solves nothing). */
+int cpm_heat_step_device(cpm_op* op, int method, double dt, const double* d_g, double* d_u,
+                         const cpm_solve_opts* opts, cpm_solve_info* info);
+
 /* ---- measurement ---------------------------------------------------------
  * No counterpart in the reference.  Times the operator's E-gather kernel (the
  * dominant kernel of every BiCGStab iteration) on the context stream with HIP

@@ -57,6 +57,13 @@ def lib():
         L.orc_ibc_info.argtypes = [ctypes.c_void_p, i64p]
         L.orc_ibc_subset.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, i64p, f64p, f64p]
         L.orc_rhs_points.argtypes = [ctypes.c_void_p, f64p, np.ctypeslib.ndpointer(np.int32, flags="C_CONTIGUOUS")]
+        L.orc_heat_run.restype = ctypes.c_int64
+        L.orc_heat_run.argtypes = [ctypes.c_int, ctypes.c_double, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                   ctypes.c_void_p, ctypes.POINTER(ctypes.c_double)]
+        L.orc_heat_study_order.restype = ctypes.c_double
+        L.orc_heat_study_order.argtypes = [ctypes.c_int]
+        L.orc_heat_problem_error.restype = ctypes.c_double
+        L.orc_heat_problem_error.argtypes = [ctypes.c_int, ctypes.c_double, ctypes.c_int, ctypes.c_int]
         L.orc_ibc_study_order.restype = ctypes.c_double
         L.orc_ibc_study_order.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.POINTER(ctypes.c_int)]
         _LIB = L
@@ -254,6 +261,34 @@ class IBCProblem(Problem):
         kind = np.zeros(self.D, np.int32)
         lib().orc_rhs_points(self._h, pts, kind)
         return pts.reshape(-1, 3), kind
+
+
+def heat_study_order(method):
+    """HeatProblemsWithIBC problems 3-5 (method 0 explicit, 1 backward Euler, 2 Crank-Nicolson)."""
+    o = lib().orc_heat_study_order(method)
+    if o < 0:
+        raise RuntimeError(_err())
+    return o
+
+
+def heat_run(method, dx, steps, boundary_order=2, boundary_type=0):
+    """Final state after `steps` steps of the heat study's system at dx (and the step size)."""
+    dt = ctypes.c_double(0)
+    D = lib().orc_heat_run(method, dx, 0, boundary_order, boundary_type, None, ctypes.byref(dt))
+    if D < 0:
+        raise RuntimeError(_err())
+    u = np.zeros(D)
+    if lib().orc_heat_run(method, dx, steps, boundary_order, boundary_type, u.ctypes.data, ctypes.byref(dt)) < 0:
+        raise RuntimeError(_err())
+    return u, dt.value
+
+
+def heat_problem_error(method, dx, boundary_order=2, boundary_type=0):
+    """Max error at the plot points against exp(-2 t) cos(phi) at t = 0.1."""
+    e = lib().orc_heat_problem_error(method, dx, boundary_order, boundary_type)
+    if e < 0:
+        raise RuntimeError(_err())
+    return e
 
 
 def ibc_study_order(problem, g):

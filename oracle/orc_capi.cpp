@@ -13,6 +13,9 @@
 namespace orc {
 double poisson_problem_error(int problem, double dx, int* iters);
 double ibc_study_order(int problem, int g, int* iters_last);
+double heat_study_order(int method);
+double heat_problem_error(int method, double dx, int order, int btype);
+int64_t heat_run_study(int method, double dx, int steps, int order, int btype, double* u_out, double* dt_out);
 }
 
 using namespace orc;
@@ -202,6 +205,33 @@ double orc_ibc_study_order(int problem, int g, int* iters) {
     } catch (const std::exception& e) {
         g_err = e.what();
         return -1.0;
+    }
+}
+
+double orc_heat_study_order(int method) {
+    try {
+        return heat_study_order(method);
+    } catch (const std::exception& e) {
+        g_err = e.what();
+        return -1.0;
+    }
+}
+
+double orc_heat_problem_error(int method, double dx, int order, int btype) {
+    try {
+        return heat_problem_error(method, dx, order, btype);
+    } catch (const std::exception& e) {
+        g_err = e.what();
+        return -1.0;
+    }
+}
+
+int64_t orc_heat_run(int method, double dx, int steps, int order, int btype, double* u_out, double* dt_out) {
+    try {
+        return heat_run_study(method, dx, steps, order, btype, u_out, dt_out);
+    } catch (const std::exception& e) {
+        g_err = e.what();
+        return -1;
     }
 }
 

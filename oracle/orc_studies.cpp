@@ -167,3 +167,163 @@ double ibc_study_order(int problem, int g, int* iters_last) {
 }
 
 }  // namespace orc
+
+// ---------------------------------------------------------------- heat with an IBC
+// HeatWithIBC (problems/HeatWithIBC.cpp:6-170) and the study
+// convergence_studies/HeatProblemsWithIBC/HeatProblemsWithIBC.cpp:
+// SphereCircleIBC (:90-193), 2nd-order Dirichlet, methods 0 explicit,
+// 1 backward Euler, 2 Crank-Nicolson (problems 3-5), final time 0.1.
+namespace orc {
+
+// ComputeCircle3DParams (:71-88) with the study's plane point; normalises `normal`.
+static Vec circle3d_params_pp(Vec normal, const double* pp) {
+    double nn = std::sqrt(normal[0] * normal[0] + normal[1] * normal[1] + normal[2] * normal[2]);
+    if (nn > 0.0) for (auto& c : normal) c /= nn;
+    const double rho = -(pp[0] * normal[0] + pp[1] * normal[1] + pp[2] * normal[2]);
+    return {std::sqrt(1 - rho * rho), rho * normal[0], rho * normal[1], rho * normal[2], normal[0], normal[1], normal[2]};
+}
+
+SurfaceSpec heat_study_ibc(double dx, int order = 2, int btype = 0) {
+    const double r = 0.27, theta = 0.132, phi = 0.221;    // :96-100
+    const double pp[3] = {r * std::cos(phi) * std::sin(theta), r * std::sin(phi) * std::sin(theta), r * std::cos(theta)};
+    SurfaceSpec s;
+    s.type = "Circle3D";
+    s.params = circle3d_params_pp({1.1, 1.234, 1.321}, pp);
+    s.bbox = {-1.0, 1.0};
+    s.dim = 3;
+    s.meta.on_ibc_tolerance = 0.1 * dx * dx;
+    s.meta.boundary_order = order;     // g_boundary_order, g_boundary_type (:113-115)
+    s.meta.boundary_type[0] = btype;
+    return s;
+}
+
+// f = cos(phi), phi = acos(z / R) (:133-134), R = 1
+static double heat_f(const double* x) { return std::cos(std::acos(x[2] / 1.0)); }
+
+// Time-step size of the study (:118-127).
+void heat_study_dt(int method, double dx, double final_time, double* dt, int* steps) {
+    double d = method > 0 ? 0.1 * dx : 0.1 * dx * dx;
+    const int n = (int)std::ceil(final_time / d);
+    *dt = final_time / n;
+    *steps = n;
+}
+
+// Row holding the Dirichlet value of DOF i of curve c (unoriented curves):
+// the 2nd-order identity row, or the ghost row itself at 1st order; -1 for
+// Neumann curves (HeatWithIBC.cpp:63-72).
+static i64 heat_dirichlet_row(const IBCData& I, size_t c, size_t i) {
+    if (I.meta[c].boundary_type[0] != 0) return -1;
+    return I.meta[c].boundary_order == 2 ? I.identity_rows_start + I.dof[c].dir_sub_of_set[i] + I.dir_start[c]
+                                         : I.identity_rows_start + (i64)i + I.start[c];
+}
+
+// The study's initial condition (:139-181): f at cp on tube rows, f at the
+// curve's closest point on the ghost rows of Neumann and 2nd-order Dirichlet
+// curves, the Dirichlet value uexact(0) on the Dirichlet rows.
+Vec heat_initial(const PoissonIBCProblem& P) {
+    const Tube& t = P.tube;
+    const i64 N = t.N();
+    Vec u((size_t)P.Lfull.rows, 0.0);
+    for (i64 i = 0; i < N; ++i) u[(size_t)i] = heat_f(&t.cpx[(size_t)(i * 3)]);
+    const IBCData& I = P.ibc;
+    for (size_t c = 0; c < I.dof.size(); ++c) {
+        const int ty = I.meta[c].boundary_type[0];
+        for (size_t i = 0; i < I.dof[c].set_of_sub.size(); ++i) {
+            const double v = heat_f(I.dof[c].cpc[i].data());
+            if (ty == 1 || (ty == 0 && I.meta[c].boundary_order == 2)) u[(size_t)I.start[c] + i] = v;
+            const i64 r = heat_dirichlet_row(I, c, i);
+            if (r >= 0) u[(size_t)r] = v;
+        }
+    }
+    return u;
+}
+
+// SetDirichletValue (problems/HeatWithIBC.cpp:50-78): u on the Dirichlet rows.
+static void heat_set_dirichlet(const PoissonIBCProblem& P, Vec& u, double t) {
+    const IBCData& I = P.ibc;
+    for (size_t c = 0; c < I.dof.size(); ++c)
+        for (size_t i = 0; i < I.dof[c].set_of_sub.size(); ++i) {
+            const i64 r = heat_dirichlet_row(I, c, i);
+            if (r >= 0) u[(size_t)r] = std::exp(-2 * t) * heat_f(I.dof[c].cpc[i].data());
+        }
+}
+
+// TimeStep (problems/HeatWithIBC.cpp:80-103), `steps` steps from u.
+int heat_run(const PoissonIBCProblem& P, int method, double dt, int steps, Vec& u) {
+    const double r = method == 1 ? 1.0 : 0.5;
+    System S;                                     // Initialize :160-166: Solver(1, -r dt)
+    S.m = 1.0;
+    S.n = -r * dt;
+    S.L = &P.Lfull;
+    S.E = &P.Efull;
+    S.ident = P.ident;
+    S.precond = PRECOND_DIAGONAL;
+    const size_t D = u.size();
+    Vec tmp(D), res(D);
+    int iters = 0;
+    for (int k = 0; k < steps; ++k) {
+        const double t = (k + 1) * dt;
+        if (method > 0) {
+            if (method == 2) {
+                S.multiply(u.data(), tmp.data(), 1.0, 0.5 * dt);
+                u = tmp;
+            }
+            heat_set_dirichlet(P, u, t);
+            SolveInfo info = bicgstab(S, u.data(), res.data(), 1e-10, 100000);
+            iters += info.iters;
+            u = res;
+        } else {
+            P.Lfull.matvec(u.data(), tmp.data());
+            for (size_t i = 0; i < D; ++i) tmp[i] = u[i] + dt * tmp[i];
+            P.Efull.matvec(tmp.data(), u.data());   // closest-point extension
+            heat_set_dirichlet(P, u, t);
+        }
+    }
+    return iters;
+}
+
+double heat_problem_error(int method, double dx, int order, int btype) {
+    PoissonIBCProblem P = make_poisson_ibc(SurfaceSpec::defaults("Sphere"), {heat_study_ibc(dx, order, btype)}, dx);
+    double dt;
+    int steps;
+    heat_study_dt(method, dx, 0.1, &dt, &steps);
+    Vec u = heat_initial(P);
+    heat_run(P, method, dt, steps, u);
+    CSR Pe = ibc_plot_E(P);
+    Vec up((size_t)Pe.rows);
+    Pe.matvec(u.data(), up.data());
+    double e = 0.0;
+    for (size_t i = 0; i < up.size(); ++i)
+        e = std::max(e, std::fabs(up[i] - std::exp(-2 * 0.1) * heat_f(P.tube.surf.xp[i].data())));
+    return e;
+}
+
+double heat_study_order(int method) {   // ConvergenceStudy(0.1, 3, Heat) (:363-366)
+    Vec err, dxs;
+    for (int lv = 0; lv < 3; ++lv) {
+        const double dx = std::pow(0.5, lv) * 0.1;
+        err.push_back(heat_problem_error(method, dx, 2, 0));
+        dxs.push_back(dx);
+    }
+    return average_order(err, dxs);
+}
+
+}  // namespace orc
+
+namespace orc {
+// The heat study's system at dx with `steps` steps of `method` from the
+// study's initial condition; returns D and (when u != null) the final state.
+int64_t heat_run_study(int method, double dx, int steps, int order, int btype, double* u_out, double* dt_out) {
+    PoissonIBCProblem P = make_poisson_ibc(SurfaceSpec::defaults("Sphere"), {heat_study_ibc(dx, order, btype)}, dx);
+    double dt;
+    int n;
+    heat_study_dt(method, dx, 0.1, &dt, &n);
+    if (dt_out) *dt_out = dt;
+    Vec u = heat_initial(P);
+    if (u_out) {
+        heat_run(P, method, dt, steps, u);
+        std::copy(u.begin(), u.end(), u_out);
+    }
+    return (int64_t)u.size();
+}
+}  // namespace orc

@@ -2,11 +2,20 @@
 the reference's convergence-order known answers, closed forms, brute force and
 exact polynomial reproduction.  (CPU only.)"""
 import math
+import os
 
 import numpy as np
 import pytest
 
 import orc
+
+
+@pytest.fixture
+def parallel_oracle():
+    """Host threads for the long studies (results do not depend on them)."""
+    orc.set_parallel(True, True, min(8, os.cpu_count() or 1))
+    yield
+    orc.set_parallel(False, False, 0)
 
 
 def test_tube_radius_and_grid_closed_forms():
@@ -168,7 +177,7 @@ def test_reference_ibc_convergence_orders(problem, g):
 @pytest.mark.slow
 @pytest.mark.parametrize("problem,g", [(p, g) for p in IBC_ORDERS for g in range(len(IBC_ORDERS[p]))
                                        if (p, g) not in ((2, 0), (2, 3), (5, 0))])
-def test_reference_ibc_convergence_orders_all(problem, g):
+def test_reference_ibc_convergence_orders_all(parallel_oracle, problem, g):
     order = orc.ibc_study_order(problem, g)
     assert abs(order - IBC_ORDERS[problem][g]) < 1e-2, order
 
@@ -197,3 +206,22 @@ def test_ellipsoid_closest_point(params, bbox):
     for i in idx:
         dmin = np.sqrt(((S - x[i]) ** 2).sum(1).min())
         assert dist[i] <= dmin + 1e-12
+
+
+def test_heat_first_order_dirichlet_converges():
+    """HeatWithIBC with a 1st-order Dirichlet curve (g_boundary_order = 1,
+    HeatProblemsWithIBC.cpp:113-115): backward Euler to t = 0.1 against the
+    closed form exp(-2 t) cos(phi); the Dirichlet value sits on the ghost rows
+    themselves, and placing it elsewhere stops the convergence."""
+    e = [orc.heat_problem_error(1, dx, 1, 0) for dx in (0.2, 0.1)]
+    rate = np.log2(e[0] / e[1])
+    assert e[1] < 0.03 and 0.8 < rate < 1.5, (e, rate)
+
+
+# HeatProblemsWithIBC.cpp:304-307 (boundary type 0, order 2), problems 3-5:
+# SphereCircleIBC with explicit, backward-Euler and Crank-Nicolson steps.
+@pytest.mark.slow
+@pytest.mark.parametrize("method,expected", [(0, 1.98885), (1, 1.15183), (2, 2.01012)])
+def test_reference_heat_ibc_convergence_orders(parallel_oracle, method, expected):
+    order = orc.heat_study_order(method)
+    assert abs(order - expected) < 1e-2, order      # the reference's criterion (:372)

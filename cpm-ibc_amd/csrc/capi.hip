@@ -5,6 +5,7 @@
 #include <cmath>
 #include <cstring>
 #include <map>
+#include <tuple>
 #include <mutex>
 #include <string>
 #include <vector>
@@ -239,6 +240,7 @@ Op* op_from_csr(Ctx* ctx, int64_t D, int nD, const int32_t* Lp, const int32_t* L
 // ---------------------------------------------------------------- batch solve
 struct Group {
     Op* op;
+    double m, n;             // operator coefficients of this group's solves
     std::vector<int> inst;   // instance indices (<= NR)
 };
 
@@ -306,6 +308,8 @@ static void run_batch(Ctx* ctx, const std::vector<Group>& groups, const std::vec
         const int nr = (int)groups[g].inst.size();
         fill_seg(h64[g], groups[g].op, off[g], b0[g], nb[g], nr, sts + g * kMaxNR, part + part_stride * b0[g]);
         fill_seg(h32[g], groups[g].op, off[g], b0[g], nb[g], nr, sts + g * kMaxNR, part + part_stride * b0[g]);
+        h64[g].m = h32[g].m = groups[g].m;
+        h64[g].n = h32[g].n = groups[g].n;
         for (int b = 0; b < nb[g]; ++b) hb[b0[g] + b] = g;
     }
     HIPCHK(hipMemcpyAsync(segs64, h64.data(), sizeof(Seg<double>) * G, hipMemcpyHostToDevice, st));
@@ -419,7 +423,8 @@ static void run_batch(Ctx* ctx, const std::vector<Group>& groups, const std::vec
 }
 
 void solve_batch(Ctx* ctx, const std::vector<Op*>& ops, const std::vector<const double*>& rhs,
-                 const std::vector<double*>& xs, const cpm_solve_opts& o, std::vector<cpm_solve_info>& infos) {
+                 const std::vector<double*>& xs, const cpm_solve_opts& o, std::vector<cpm_solve_info>& infos,
+                 const std::vector<std::pair<double, double>>* mn = nullptr) {
     const int n = (int)ops.size();
     infos.assign(n, cpm_solve_info{});
     if (n == 0) return;
@@ -428,15 +433,18 @@ void solve_batch(Ctx* ctx, const std::vector<Op*>& ops, const std::vector<const 
         if (!op || op->ctx != ctx) throw ArgError("operator belongs to another context");
         if (op->precond == CPM_PRECOND_JACOBI && op->precond_iters > 100000) throw ArgError("too many Jacobi sweeps");
     }
-    // group instances that share an operator (at most kMaxNR per group)
+    // group instances that share an operator and its (m, n) (at most kMaxNR per group)
     std::vector<Group> groups;
-    std::map<Op*, int> open;
+    std::map<std::tuple<Op*, double, double>, int> open;
     for (int i = 0; i < n; ++i) {
-        auto it = open.find(ops[i]);
+        const double m = mn ? (*mn)[(size_t)i].first : ops[i]->m;
+        const double nn = mn ? (*mn)[(size_t)i].second : ops[i]->n;
+        const auto key = std::make_tuple(ops[i], m, nn);
+        auto it = open.find(key);
         if (it == open.end() || groups[it->second].inst.size() == (size_t)kMaxNR) {
-            groups.push_back(Group{ops[i], {}});
-            open[ops[i]] = (int)groups.size() - 1;
-            it = open.find(ops[i]);
+            groups.push_back(Group{ops[i], m, nn, {}});
+            open[key] = (int)groups.size() - 1;
+            it = open.find(key);
         }
         groups[it->second].inst.push_back(i);
     }
@@ -548,49 +556,87 @@ __global__ void k_held_apply(int64_t nh, int hw, const int* hrow, const int* hco
     z[r] = x[r] + dt * acc;
 }
 
+// Explicit stepping in the operator's internal order: one segment per
+// diffusion coefficient c (apply with m = 1, n = c) and pack entries moving
+// caller vectors in and out, uploaded together.
+struct StepTables {
+    Seg<double>* seg = nullptr;
+    PackEntry* pe = nullptr;
+    int* bseg = nullptr;
+    int nblocks = 0;
+};
+
+static StepTables step_tables(Op* op, const char* name, const std::vector<double>& coef,
+                              std::vector<PackEntry> pes) {
+    Ctx* ctx = op->ctx;
+    const hipStream_t st = ctx->stream;
+    StepTables t;
+    t.nblocks = (int)((op->D + kBlock - 1) / kBlock);
+    const size_t seg_bytes = (sizeof(Seg<double>) * coef.size() + 255) / 256 * 256;
+    const size_t pe_bytes = (sizeof(PackEntry) * pes.size() + 255) / 256 * 256;
+    char* tb = reinterpret_cast<char*>(ctx->workspace(name, seg_bytes + pe_bytes + sizeof(int) * t.nblocks));
+    std::vector<Seg<double>> h(coef.size());
+    for (size_t k = 0; k < coef.size(); ++k) {
+        fill_seg(h[k], op, 0, 0, t.nblocks, 1, nullptr, nullptr);
+        h[k].m = 1.0;
+        h[k].n = coef[k];
+    }
+    for (auto& e : pes) {
+        e.perm = op->perm.as<int>();
+        e.D = op->D;
+        e.nr = 1;
+    }
+    std::vector<int> hb(t.nblocks, 0);
+    t.seg = reinterpret_cast<Seg<double>*>(tb);
+    t.pe = reinterpret_cast<PackEntry*>(tb + seg_bytes);
+    t.bseg = reinterpret_cast<int*>(tb + seg_bytes + pe_bytes);
+    HIPCHK(hipMemcpyAsync(t.seg, h.data(), sizeof(Seg<double>) * h.size(), hipMemcpyHostToDevice, st));
+    HIPCHK(hipMemcpyAsync(t.pe, pes.data(), sizeof(PackEntry) * pes.size(), hipMemcpyHostToDevice, st));
+    HIPCHK(hipMemcpyAsync(t.bseg, hb.data(), sizeof(int) * t.nblocks, hipMemcpyHostToDevice, st));
+    return t;
+}
+
+static PackEntry pack_in(const double* src) { PackEntry e{}; e.src = src; return e; }
+static PackEntry pack_out(double* dst) { PackEntry e{}; e.dst = dst; return e; }
+
+// z = x + c (Lfull x), internal order, on every row: the tube rows through the
+// apply kernel with tmp = x, the identity rows from the held Lfull rows.
+static void explicit_diffusion(Op* op, const StepTables& t, int k, double c, const double* x, double* z) {
+    const hipStream_t st = op->ctx->stream;
+    LaunchCfg<double> cfg{t.seg + k, t.bseg, t.nblocks, 1, st, 0};
+    launch_lfinish_plain<double>(cfg, x, x, z);
+    if (op->nh > 0)
+        hipLaunchKernelGGL(k_held_apply, dim3((unsigned)((op->nh + 255) / 256)), dim3(256), 0, st, op->nh, op->hw,
+                           op->hrow.as<int>(), op->hcol.as<int>(), op->hval.as<double>(), op->flag.as<uint8_t>(), c,
+                           x, z);
+}
+
+static void extension(Op* op, const StepTables& t, const double* z, double* y) {
+    LaunchCfg<double> cfg{t.seg, t.bseg, t.nblocks, 1, op->ctx->stream, 0};
+    launch_egather<double, 1>(cfg, z, y);
+}
+
+static void set_dirichlet(Op* op, const double* g, double* u) {
+    hipLaunchKernelGGL(k_set_identity_rows, dim3((unsigned)((op->D + 255) / 256)), dim3(256), 0, op->ctx->stream,
+                       op->flag.as<uint8_t>(), op->perm.as<int>(), op->D, g, u);
+}
+
 // HeatWithIBC::TimeStep (problems/HeatWithIBC.cpp:80-103).
 static void heat_step(Op* op, int method, double dt, const double* d_g, double* d_u, const cpm_solve_opts& o,
                       cpm_solve_info* info) {
     Ctx* ctx = op->ctx;
     const hipStream_t st = ctx->stream;
     const int64_t D = op->D;
-    const unsigned nb = (unsigned)((D + 255) / 256);
     if (method == 0) {
-        // internal order: x = P u ; z = x + dt (L_ii x + L_off x) ; y = E z ; u = P^T y
-        const int nblocks = (int)((D + kBlock - 1) / kBlock);
-        char* tb = reinterpret_cast<char*>(ctx->workspace("heat_tables", 8192 + sizeof(int) * nblocks));
-        Seg<double> h{};
-        fill_seg(h, op, 0, 0, nblocks, 1, nullptr, nullptr);
-        h.m = 1.0;
-        h.n = dt;
-        std::vector<int> hb(nblocks, 0);
-        PackEntry pe[2]{};
-        pe[0].src = d_u;
-        pe[1].dst = d_u;
-        for (auto& e : pe) {
-            e.perm = op->perm.as<int>();
-            e.D = D;
-            e.nr = 1;
-        }
-        auto* dseg = reinterpret_cast<Seg<double>*>(tb);
-        auto* dpe = reinterpret_cast<PackEntry*>(tb + 4096);
-        int* dbseg = reinterpret_cast<int*>(tb + 8192);
-        HIPCHK(hipMemcpyAsync(dseg, &h, sizeof(h), hipMemcpyHostToDevice, st));
-        HIPCHK(hipMemcpyAsync(dpe, pe, sizeof(pe), hipMemcpyHostToDevice, st));
-        HIPCHK(hipMemcpyAsync(dbseg, hb.data(), sizeof(int) * nblocks, hipMemcpyHostToDevice, st));
+        // internal order: x = P u ; z = x + dt Lfull x ; y = E z ; u = P^T y ; Dirichlet rows
+        const StepTables t = step_tables(op, "heat_tables", {dt}, {pack_in(d_u), pack_out(d_u)});
         double* w = ctx->workspace("heat_vecs", sizeof(double) * D * 3);
         double *xi = w, *zi = w + D, *yi = w + 2 * D;
-        LaunchCfg<double> c{dseg, dbseg, nblocks, 1, st, 0};
-        hipLaunchKernelGGL(k_pack, dim3(nblocks), dim3(256), 0, st, dpe, dbseg, xi, (float*)nullptr, 1);
-        launch_lfinish_plain<double>(c, xi, xi, zi);   // L applied to x itself: tmp = x
-        if (op->nh > 0)
-            hipLaunchKernelGGL(k_held_apply, dim3((unsigned)((op->nh + 255) / 256)), dim3(256), 0, st, op->nh, op->hw,
-                               op->hrow.as<int>(), op->hcol.as<int>(), op->hval.as<double>(), op->flag.as<uint8_t>(),
-                               dt, xi, zi);
-        launch_egather<double, 1>(c, zi, yi);
-        hipLaunchKernelGGL(k_pack, dim3(nblocks), dim3(256), 0, st, dpe + 1, dbseg, yi, (float*)nullptr, 0);
-        hipLaunchKernelGGL(k_set_identity_rows, dim3(nb), dim3(256), 0, st, op->flag.as<uint8_t>(), op->perm.as<int>(),
-                           D, d_g, d_u);
+        hipLaunchKernelGGL(k_pack, dim3(t.nblocks), dim3(256), 0, st, t.pe, t.bseg, xi, (float*)nullptr, 1);
+        explicit_diffusion(op, t, 0, dt, xi, zi);
+        extension(op, t, zi, yi);
+        hipLaunchKernelGGL(k_pack, dim3(t.nblocks), dim3(256), 0, st, t.pe + 1, t.bseg, yi, (float*)nullptr, 0);
+        set_dirichlet(op, d_g, d_u);
         HIPCHK(hipGetLastError());
         HIPCHK(hipStreamSynchronize(st));
         if (info) *info = cpm_solve_info{CPM_SOLVE_CONVERGED, 0, 0, 0, 0.0};
@@ -599,12 +645,87 @@ static void heat_step(Op* op, int method, double dt, const double* d_g, double* 
     double* rhs = ctx->workspace("heat_rhs", sizeof(double) * D);
     if (method == 2) multiply(op, d_u, rhs, 1.0, 0.5 * dt);
     else HIPCHK(hipMemcpyAsync(rhs, d_u, sizeof(double) * D, hipMemcpyDeviceToDevice, st));
-    hipLaunchKernelGGL(k_set_identity_rows, dim3(nb), dim3(256), 0, st, op->flag.as<uint8_t>(), op->perm.as<int>(), D,
-                       d_g, rhs);
+    set_dirichlet(op, d_g, rhs);
     HIPCHK(hipGetLastError());
     std::vector<cpm_solve_info> infos;
     solve_batch(ctx, {op}, {rhs}, {d_u}, o, infos);
     if (info) *info = infos[0];
+}
+
+// Gray-Scott reaction added to the diffused fields (ReactionDiffusionWithIBC.cpp:89-93):
+// zu += dt (-u v^2 + F (1 - u)), zv += dt (u v^2 - (F + k) v), from the old u, v.
+__global__ void k_gray_scott(int64_t D, double F, double k, double dt, const double* u, const double* v, double* zu,
+                             double* zv) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= D) return;
+    const double a = u[i], b = v[i];
+    zu[i] += dt * (-a * b * b + F * (1.0 - a));
+    zv[i] += dt * (a * b * b - (F + k) * b);
+}
+
+// IMEX right-hand sides: ru = u + dt Ru(u, v), rv = v + dt Rv(u, v).
+__global__ void k_gray_scott_rhs(int64_t D, double F, double k, double dt, const double* u, const double* v,
+                                 double* ru, double* rv) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= D) return;
+    const double a = u[i], b = v[i];
+    ru[i] = a + dt * (-a * b * b + F * (1.0 - a));
+    rv[i] = b + dt * (a * b * b - (F + k) * b);
+}
+
+// ReactionDiffusionWithIBC::TimeStep (problems/ReactionDiffusionWithIBC.cpp:79-104)
+// for nf fields (u, v pairs); method 1 treats diffusion implicitly instead:
+// (1 - dt nu L) u' = u + dt Ru(u, v), one batched solve for all 2 nf fields.
+static void rd_step(Op* op, const cpm_rd_params& p, int nf, double* const* d_u, double* const* d_v,
+                    const double* d_gu, const double* d_gv, const cpm_solve_opts& o, cpm_solve_info* infos) {
+    Ctx* ctx = op->ctx;
+    const hipStream_t st = ctx->stream;
+    const int64_t D = op->D;
+    const unsigned nb = (unsigned)((D + 255) / 256);
+    if (p.method == 0) {
+        double* w = ctx->workspace("rd_vecs", sizeof(double) * D * 4);
+        double *xu = w, *xv = w + D, *zu = w + 2 * D, *zv = w + 3 * D;
+        for (int f = 0; f < nf; ++f) {
+            const StepTables t = step_tables(op, "rd_tables", {p.nu_u * p.dt, p.nu_v * p.dt},
+                                             {pack_in(d_u[f]), pack_in(d_v[f]), pack_out(d_u[f]), pack_out(d_v[f])});
+            hipLaunchKernelGGL(k_pack, dim3(t.nblocks), dim3(256), 0, st, t.pe, t.bseg, xu, (float*)nullptr, 1);
+            hipLaunchKernelGGL(k_pack, dim3(t.nblocks), dim3(256), 0, st, t.pe + 1, t.bseg, xv, (float*)nullptr, 1);
+            explicit_diffusion(op, t, 0, p.nu_u * p.dt, xu, zu);
+            explicit_diffusion(op, t, 1, p.nu_v * p.dt, xv, zv);
+            hipLaunchKernelGGL(k_gray_scott, dim3(nb), dim3(256), 0, st, D, p.F, p.k, p.dt, xu, xv, zu, zv);
+            extension(op, t, zu, xu);   // the old fields are no longer needed
+            extension(op, t, zv, xv);
+            hipLaunchKernelGGL(k_pack, dim3(t.nblocks), dim3(256), 0, st, t.pe + 2, t.bseg, xu, (float*)nullptr, 0);
+            hipLaunchKernelGGL(k_pack, dim3(t.nblocks), dim3(256), 0, st, t.pe + 3, t.bseg, xv, (float*)nullptr, 0);
+            set_dirichlet(op, d_gu, d_u[f]);
+            set_dirichlet(op, d_gv, d_v[f]);
+            HIPCHK(hipGetLastError());
+            HIPCHK(hipStreamSynchronize(st));   // the tables are rewritten for the next field
+        }
+        if (infos)
+            for (int i = 0; i < 2 * nf; ++i) infos[i] = cpm_solve_info{CPM_SOLVE_CONVERGED, 0, 0, 0, 0.0};
+        return;
+    }
+    double* w = ctx->workspace("rd_rhs", sizeof(double) * D * 2 * nf);
+    std::vector<Op*> ops;
+    std::vector<const double*> rhs;
+    std::vector<double*> xs;
+    std::vector<std::pair<double, double>> mn;
+    for (int f = 0; f < nf; ++f) {
+        double *ru = w + 2 * f * D, *rv = ru + D;
+        hipLaunchKernelGGL(k_gray_scott_rhs, dim3(nb), dim3(256), 0, st, D, p.F, p.k, p.dt, d_u[f], d_v[f], ru, rv);
+        set_dirichlet(op, d_gu, ru);
+        set_dirichlet(op, d_gv, rv);
+        ops.insert(ops.end(), {op, op});
+        rhs.insert(rhs.end(), {ru, rv});
+        xs.insert(xs.end(), {d_u[f], d_v[f]});
+        mn.push_back({1.0, -p.dt * p.nu_u});
+        mn.push_back({1.0, -p.dt * p.nu_v});
+    }
+    HIPCHK(hipGetLastError());
+    std::vector<cpm_solve_info> inf;
+    solve_batch(ctx, ops, rhs, xs, o, inf, &mn);
+    if (infos) std::copy(inf.begin(), inf.end(), infos);
 }
 
 __global__ void k_copy16(const uint4* __restrict__ src, uint4* __restrict__ dst, int64_t n) {
@@ -890,6 +1011,32 @@ int cpm_heat_step_device(cpm_op* op, int method, double dt, const double* d_g, d
         if (O(op)->nD != 0) throw ArgError("time stepping needs an operator without trailing nD rows");
         HIPCHK(hipSetDevice(O(op)->ctx->device));
         heat_step(O(op), method, dt, d_g, d_u, resolve_opts(opts), info);
+    });
+}
+
+void cpm_rd_params_default(double dx, cpm_rd_params* p) {
+    if (!p) return;
+    p->F = 0.054;                          // ReactionDiffusionWithIBC.cpp:83-84
+    p->k = 0.063;
+    p->nu_u = 1.0 / ((3.0 / dx) * (3.0 / dx));   // :81-82
+    p->nu_v = p->nu_u / 3.0;
+    p->dt = 0.1 * (1.0 / std::max(p->nu_u, p->nu_v)) * dx * dx;   // examples/ReactionDiffusionFish/ReactionDiffusionFish.cpp:61-63
+    p->method = 0;
+    p->reserved = 0;
+}
+
+int cpm_rd_step_device(cpm_op* op, const cpm_rd_params* p, int n_fields, double* const* d_u, double* const* d_v,
+                       const double* d_g_u, const double* d_g_v, const cpm_solve_opts* opts, cpm_solve_info* infos) {
+    return guard([&] {
+        if (!op || !p || !d_u || !d_v || !d_g_u || !d_g_v) throw ArgError("null argument");
+        if (n_fields <= 0 || n_fields > 4096) throw ArgError("n_fields must be in 1..4096");
+        for (int f = 0; f < n_fields; ++f)
+            if (!d_u[f] || !d_v[f]) throw ArgError("null field pointer");
+        if (p->method != 0 && p->method != 1) throw ArgError("method must be 0 (explicit) or 1 (implicit diffusion)");
+        if (!(p->dt > 0) || !(p->nu_u >= 0) || !(p->nu_v >= 0)) throw ArgError("dt must be positive, nu non-negative");
+        if (O(op)->nD != 0) throw ArgError("time stepping needs an operator without trailing nD rows");
+        HIPCHK(hipSetDevice(O(op)->ctx->device));
+        rd_step(O(op), *p, n_fields, d_u, d_v, d_g_u, d_g_v, resolve_opts(opts), infos);
     });
 }
 

@@ -23,7 +23,7 @@ EXPORTS = [
     "cpm_op_set_precond", "cpm_multiply", "cpm_multiply_device", "cpm_solve", "cpm_solve_device",
     "cpm_solve_batch", "cpm_solve_batch_device", "cpm_device_alloc", "cpm_device_free", "cpm_memcpy_h2d",
     "cpm_memcpy_d2h", "cpm_tube_create", "cpm_tube_destroy", "cpm_tube_get_info", "cpm_tube_export",
-    "cpm_tube_export_E", "cpm_tube_export_L", "cpm_tube_interp_matrix", "cpm_op_create_from_tube", "cpm_profile_egather", "cpm_profile_copy", "cpm_heat_step_device", "cpm_problem_create", "cpm_problem_destroy",
+    "cpm_tube_export_E", "cpm_tube_export_L", "cpm_tube_interp_matrix", "cpm_op_create_from_tube", "cpm_profile_egather", "cpm_profile_copy", "cpm_heat_step_device", "cpm_rd_params_default", "cpm_rd_step_device", "cpm_problem_create", "cpm_problem_destroy",
     "cpm_problem_get_info", "cpm_problem_rhs_points", "cpm_problem_subset", "cpm_problem_export",
     "cpm_op_create_from_problem",
 ]
@@ -41,6 +41,11 @@ class SolveInfo(ctypes.Structure):
     def as_dict(self):
         return dict(status=STATUS.get(self.status, self.status), iters=self.iters, restarts=self.restarts,
                     updates=self.updates, rel_residual=self.rel_residual)
+
+
+class RdParams(ctypes.Structure):
+    _fields_ = [("F", ctypes.c_double), ("k", ctypes.c_double), ("nu_u", ctypes.c_double), ("nu_v", ctypes.c_double),
+                ("dt", ctypes.c_double), ("method", ctypes.c_int), ("reserved", ctypes.c_int)]
 
 
 class TubeInfo(ctypes.Structure):
@@ -128,6 +133,10 @@ def load(path=LIB_PATH):
                                           ctypes.POINTER(vp)]
     L.cpm_heat_step_device.argtypes = [vp, ctypes.c_int, ctypes.c_double, vp, vp, ctypes.POINTER(SolveOpts),
                                        ctypes.POINTER(SolveInfo)]
+    L.cpm_rd_params_default.restype = None
+    L.cpm_rd_params_default.argtypes = [ctypes.c_double, ctypes.POINTER(RdParams)]
+    L.cpm_rd_step_device.argtypes = [vp, ctypes.POINTER(RdParams), ctypes.c_int, ctypes.POINTER(vp),
+                                     ctypes.POINTER(vp), vp, vp, ctypes.POINTER(SolveOpts), ctypes.POINTER(SolveInfo)]
     L.cpm_profile_copy.argtypes = [vp, ctypes.c_int64, ctypes.c_int, ctypes.POINTER(ctypes.c_double)]
     L.cpm_profile_egather.argtypes = [vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.POINTER(ctypes.c_double),
                                       ctypes.POINTER(ctypes.c_double)]
@@ -151,6 +160,16 @@ def _chk(rc):
 
 def _ptr(a):
     return a.ctypes.data_as(vp)
+
+
+def rd_params(dx, method=0, **kw):
+    """Gray-Scott parameters with the reference's defaults at grid spacing dx; keyword overrides."""
+    p = RdParams()
+    load().cpm_rd_params_default(dx, ctypes.byref(p))
+    p.method = method
+    for k, v in kw.items():
+        setattr(p, k, v)
+    return p
 
 
 def opts(tol=1e-10, max_iters=100000, precision=CPM_FP64, update_delta=1e-3):
@@ -297,6 +316,16 @@ class Operator:
         info = SolveInfo()
         _chk(load().cpm_heat_step_device(self._h, method, dt, d_g, d_u, ctypes.byref(o or opts()), ctypes.byref(info)))
         return info.as_dict()
+
+    def rd_step_device(self, params, d_us, d_vs, d_gu, d_gv, o=None):
+        """One Gray-Scott reaction-diffusion step of every (u, v) field pair on the device
+        (params: RdParams, method 0 explicit as the reference, 1 implicit diffusion)."""
+        n = len(d_us)
+        us, vs = (ctypes.c_void_p * n)(*d_us), (ctypes.c_void_p * n)(*d_vs)
+        infos = (SolveInfo * (2 * n))()
+        _chk(load().cpm_rd_step_device(self._h, ctypes.byref(params), n, us, vs, d_gu, d_gv, ctypes.byref(o or opts()),
+                                       infos))
+        return [i.as_dict() for i in infos]
 
     def solve_device(self, d_rhs, d_x, o=None):
         info = SolveInfo()

@@ -250,6 +250,32 @@ int cpm_op_create_from_problem(cpm_problem* prob, double m, double n, int precon
 int cpm_heat_step_device(cpm_op* op, int method, double dt, const double* d_g, double* d_u,
                          const cpm_solve_opts* opts, cpm_solve_info* info);
 
+/* Gray-Scott reaction-diffusion, ReactionDiffusionWithIBC::TimeStep
+ * (problems/ReactionDiffusionWithIBC.cpp:79-104), with its constants as
+ * defaults (cpm_rd_params_default: F 0.054, k 0.063, nu_u (dx/3)^2,
+ * nu_v nu_u/3 (:81-84), dt 0.1 dx^2 / max(nu) as
+ * examples/ReactionDiffusionFish/ReactionDiffusionFish.cpp:61-63).
+ *   method 0, explicit (the reference): u <- E (u + dt nu_u L u + dt Ru(u, v)),
+ *             v likewise, then identity rows <- g
+ *   method 1, implicit diffusion (no counterpart in the reference):
+ *             (1 - dt nu_u L) u' = u + dt Ru(u, v) with the identity rows of
+ *             the right-hand side set to g, solved as the problem's operator
+ *             with (m, n) = (1, -dt nu); the 2 n_fields solves run as one batch.
+ * Ru = -u v^2 + F (1 - u), Rv = u v^2 - (F + k) v.  op's (m, n) are not used.
+ * d_u[f], d_v[f] (f < n_fields): device fields of D rows, updated in place;
+ * d_g_u / d_g_v: Dirichlet values read on the identity rows.  infos (may be
+ * NULL) receives 2 n_fields entries, u then v of each field. */
+typedef struct {
+    double F, k;        /* feed and kill rates */
+    double nu_u, nu_v;  /* diffusion coefficients */
+    double dt;
+    int method;         /* 0 explicit, 1 implicit diffusion */
+    int reserved;
+} cpm_rd_params;
+void cpm_rd_params_default(double dx, cpm_rd_params* p);
+int cpm_rd_step_device(cpm_op* op, const cpm_rd_params* p, int n_fields, double* const* d_u, double* const* d_v,
+                       const double* d_g_u, const double* d_g_v, const cpm_solve_opts* opts, cpm_solve_info* infos);
+
 /* ---- measurement ---------------------------------------------------------
  * No counterpart in the reference.  Times the operator's E-gather kernel (the
  * dominant kernel of every BiCGStab iteration) on the context stream with HIP

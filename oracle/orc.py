@@ -60,6 +60,9 @@ def lib():
         L.orc_heat_run.restype = ctypes.c_int64
         L.orc_heat_run.argtypes = [ctypes.c_int, ctypes.c_double, ctypes.c_int, ctypes.c_int, ctypes.c_int,
                                    ctypes.c_void_p, ctypes.POINTER(ctypes.c_double)]
+        L.orc_rd_run.restype = ctypes.c_int64
+        L.orc_rd_run.argtypes = [ctypes.c_int, ctypes.c_double, ctypes.c_int, ctypes.c_int, f64p, ctypes.c_int,
+                                 ctypes.c_void_p, ctypes.c_void_p]
         L.orc_heat_study_order.restype = ctypes.c_double
         L.orc_heat_study_order.argtypes = [ctypes.c_int]
         L.orc_heat_problem_error.restype = ctypes.c_double
@@ -297,3 +300,17 @@ def ibc_study_order(problem, g):
     if o < 0:
         raise RuntimeError(_err())
     return o
+
+
+def rd_run(method, dx, u0, v0, F, k, nu_u, nu_v, dt, steps, boundary_order=2, boundary_type=0):
+    """Gray-Scott reaction-diffusion steps on the heat study's system (sphere and
+    Circle3D curve) from (u0, v0); method 0 explicit, 1 implicit diffusion."""
+    prm = np.array([F, k, nu_u, nu_v, dt], np.float64)
+    D = lib().orc_rd_run(method, dx, boundary_order, boundary_type, prm, 0, None, None)
+    if D < 0:
+        raise RuntimeError(_err())
+    assert len(u0) == D and len(v0) == D
+    u, v = np.array(u0, np.float64), np.array(v0, np.float64)
+    if lib().orc_rd_run(method, dx, boundary_order, boundary_type, prm, steps, u.ctypes.data, v.ctypes.data) < 0:
+        raise RuntimeError(_err())
+    return u, v

@@ -16,6 +16,8 @@ double ibc_study_order(int problem, int g, int* iters_last);
 double heat_study_order(int method);
 double heat_problem_error(int method, double dx, int order, int btype);
 int64_t heat_run_study(int method, double dx, int steps, int order, int btype, double* u_out, double* dt_out);
+int64_t rd_run_study(int method, double dx, int order, int btype, const double* prm, int steps, double* u_io,
+                     double* v_io);
 }
 
 using namespace orc;
@@ -229,6 +231,16 @@ double orc_heat_problem_error(int method, double dx, int order, int btype) {
 int64_t orc_heat_run(int method, double dx, int steps, int order, int btype, double* u_out, double* dt_out) {
     try {
         return heat_run_study(method, dx, steps, order, btype, u_out, dt_out);
+    } catch (const std::exception& e) {
+        g_err = e.what();
+        return -1;
+    }
+}
+
+int64_t orc_rd_run(int method, double dx, int order, int btype, const double* prm, int steps, double* u_io,
+                   double* v_io) {
+    try {
+        return rd_run_study(method, dx, order, btype, prm, steps, u_io, v_io);
     } catch (const std::exception& e) {
         g_err = e.what();
         return -1;

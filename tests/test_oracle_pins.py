@@ -225,3 +225,38 @@ def test_heat_first_order_dirichlet_converges():
 def test_reference_heat_ibc_convergence_orders(parallel_oracle, method, expected):
     order = orc.heat_study_order(method)
     assert abs(order - expected) < 1e-2, order      # the reference's criterion (:372)
+
+
+def _heat_study_circle():
+    r, th, ph = 0.27, 0.132, 0.221
+    pp = np.array([r * np.cos(ph) * np.sin(th), r * np.sin(ph) * np.sin(th), r * np.cos(th)])
+    n = np.array([1.1, 1.234, 1.321])
+    n /= np.linalg.norm(n)
+    rho = -pp @ n
+    return [np.sqrt(1 - rho * rho), *(rho * n), *n]
+
+
+@pytest.mark.parametrize("method", [0, 1])
+def test_rd_uniform_state_follows_reaction_ode(method):
+    """ReactionDiffusionWithIBC::TimeStep (ReactionDiffusionWithIBC.cpp:79-104) with a
+    Neumann curve from a state uniform on every row.  Lfull rows with all six
+    neighbours sum to 0 and Efull rows to 1, and E reads only such rows, so on
+    them each step is the forward-Euler map of the scalar Gray-Scott ODE (the
+    implicit variant solves (1 - dt nu L) u' = rhs, i.e. u' = rhs there).  Rows
+    whose stencil is cut by the tube edge keep other values, unread."""
+    dx, F, k, nu = 0.2, 0.054, 0.063, (0.2 / 3) ** 2
+    dt = 0.9 if method == 0 else 5.0
+    P = orc.IBCProblem("Sphere", dx, ibcs=[orc.IBC("Circle3D", params=_heat_study_circle(), boundary_order=2,
+                                                   boundary_type=(1, 1), on_ibc_tolerance=0.1 * dx * dx)])
+    L, E = P.L(), P.E()
+    rowsum = np.bincount(np.repeat(np.arange(P.D), np.diff(L.ptr)), weights=L.val, minlength=P.D)
+    complete = np.abs(rowsum) < 1e-8
+    assert complete[E.col[E.val != 0]].all()       # E never reads a cut row
+    u, v = orc.rd_run(method, dx, np.full(P.D, 0.7), np.full(P.D, 0.2), F, k, nu, nu / 3, dt, 4, 2, 1)
+    a, b = 0.7, 0.2
+    for _ in range(4):
+        a, b = a + dt * (-a * b * b + F * (1 - a)), b + dt * (a * b * b - (F + k) * b)
+    du, dv = np.abs(u - a)[complete].max(), np.abs(v - b)[complete].max()
+    assert du < 1e-12 and dv < 1e-12, (du, dv)
+    if method == 0:                                 # the final extension rewrites every row
+        assert np.abs(u - a).max() < 1e-12

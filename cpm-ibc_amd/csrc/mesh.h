@@ -1,0 +1,47 @@
+// Triangle-mesh closest points on the GPU: a bounding-volume hierarchy built
+// on the host (binned surface-area heuristic) and a per-query FP64 traversal.
+// The reference queries fcpw's SAH BVH (src/Surface.cpp:1751-1779,2537-2592,
+// FP32 inside fcpw); here every distance is FP64 and the result is the exact
+// minimiser of the computed distance, lowest triangle index on ties, i.e. the
+// brute-force answer.
+#pragma once
+#include <vector>
+
+#include "engine.h"
+
+namespace cpmgpu {
+
+// Binary BVH node, 64 bytes.  count > 0: leaf holding prim[first .. first+count);
+// count == 0: internal node whose children are nodes first and first + 1.
+struct BvhNode {
+    double lo[3], hi[3];
+    int first;
+    int count;
+    int pad[2];
+};
+
+struct MeshDev {
+    const double* v = nullptr;      // vertices, 3 per vertex
+    const int* f = nullptr;         // triangles, 3 vertex indices each
+    const BvhNode* nodes = nullptr;
+    const int* prim = nullptr;      // triangle indices in leaf order
+    int ntri = 0;
+    double slack = 0;               // distance slack of the box test (rounding of the triangle distances)
+};
+
+constexpr int kBvhStack = 64;       // traversal stack; the builder keeps the depth below it
+
+struct MeshBvh {
+    Buffer v, f, nodes, prim;
+    int ntri = 0, nnodes = 0, depth = 0;
+    double slack = 0;
+    MeshDev dev() const;
+};
+
+// Validates the mesh, builds the hierarchy and uploads everything.
+void mesh_build(MeshBvh& m, const std::vector<double>& verts, const std::vector<int>& faces);
+
+// Closest point of x on the mesh and its distance.
+__device__ void cp_mesh(const MeshDev& m, const double* x, double* c, double& dist);
+
+}  // namespace cpmgpu

@@ -1,0 +1,298 @@
+// Triangle-mesh closest points: host BVH builder and device traversal.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <numeric>
+#include <string>
+
+#include "mesh.h"
+#include "ops.h"
+
+namespace cpmgpu {
+
+MeshDev MeshBvh::dev() const {
+    MeshDev d;
+    d.v = v.as<double>();
+    d.f = f.as<int>();
+    d.nodes = nodes.as<BvhNode>();
+    d.prim = prim.as<int>();
+    d.ntri = ntri;
+    d.slack = slack;
+    return d;
+}
+
+// ---------------------------------------------------------------- builder
+namespace {
+
+constexpr int kLeaf = 4;          // triangles per leaf
+constexpr int kBins = 16;         // SAH bins per split
+constexpr int kSahDepth = 32;     // deeper nodes split at the median, which bounds the depth
+
+struct Box {
+    double lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
+    void grow(const double* a, const double* b) {
+        for (int d = 0; d < 3; ++d) {
+            lo[d] = std::min(lo[d], a[d]);
+            hi[d] = std::max(hi[d], b[d]);
+        }
+    }
+    void grow(const Box& o) { grow(o.lo, o.hi); }
+    double area() const {
+        const double e0 = hi[0] - lo[0], e1 = hi[1] - lo[1], e2 = hi[2] - lo[2];
+        return e0 < 0 ? 0.0 : 2.0 * (e0 * e1 + e1 * e2 + e2 * e0);
+    }
+};
+
+}  // namespace
+
+void mesh_build(MeshBvh& m, const std::vector<double>& verts, const std::vector<int>& faces) {
+    if (verts.size() % 3 || faces.size() % 3) throw ArgError("mesh arrays must hold 3 values per vertex / face");
+    const int64_t nv = (int64_t)verts.size() / 3, nf = (int64_t)faces.size() / 3;
+    if (nv < 3 || nf < 1) throw ArgError("mesh needs at least one triangle");
+    if (nf > (1 << 28)) throw ArgError("too many triangles");
+    for (int x : faces)
+        if (x < 0 || x >= nv) throw ArgError("face index out of range");
+    double scale = 0;
+    for (double x : verts) {
+        if (!std::isfinite(x)) throw ArgError("non-finite vertex coordinate");
+        scale = std::max(scale, std::fabs(x));
+    }
+    const int F = (int)nf;
+    std::vector<Box> tb((size_t)F);
+    std::vector<double> cen((size_t)F * 3);
+    for (int t = 0; t < F; ++t) {
+        for (int k = 0; k < 3; ++k) {
+            const double* p = &verts[(size_t)faces[(size_t)3 * t + k] * 3];
+            tb[(size_t)t].grow(p, p);
+        }
+        for (int d = 0; d < 3; ++d) cen[(size_t)3 * t + d] = 0.5 * (tb[(size_t)t].lo[d] + tb[(size_t)t].hi[d]);
+    }
+    std::vector<int> prim((size_t)F);
+    std::iota(prim.begin(), prim.end(), 0);
+    std::vector<BvhNode> nodes;
+    nodes.reserve((size_t)2 * F);
+    nodes.push_back(BvhNode{});
+    struct Item { int node, start, end, depth; };
+    std::vector<Item> work{{0, 0, F, 0}};
+    int depth = 0;
+    while (!work.empty()) {
+        const Item it = work.back();
+        work.pop_back();
+        depth = std::max(depth, it.depth);
+        Box nb, cb;
+        for (int k = it.start; k < it.end; ++k) {
+            const int t = prim[(size_t)k];
+            nb.grow(tb[(size_t)t]);
+            cb.grow(&cen[(size_t)3 * t], &cen[(size_t)3 * t]);
+        }
+        BvhNode& nd = nodes[(size_t)it.node];
+        for (int d = 0; d < 3; ++d) {
+            nd.lo[d] = nb.lo[d];
+            nd.hi[d] = nb.hi[d];
+        }
+        const int n = it.end - it.start;
+        if (n <= kLeaf) {
+            nd.first = it.start;
+            nd.count = n;
+            continue;
+        }
+        int axis = 0;
+        for (int d = 1; d < 3; ++d)
+            if (cb.hi[d] - cb.lo[d] > cb.hi[axis] - cb.lo[axis]) axis = d;
+        const double lo = cb.lo[axis], ext = cb.hi[axis] - cb.lo[axis];
+        int mid = -1;
+        if (ext > 0 && it.depth < kSahDepth) {
+            auto bin_of = [&](int t) {
+                const int b = (int)(kBins * ((cen[(size_t)3 * t + axis] - lo) / ext));
+                return std::min(kBins - 1, std::max(0, b));
+            };
+            int cnt[kBins] = {};
+            Box bb[kBins];
+            for (int k = it.start; k < it.end; ++k) {
+                const int t = prim[(size_t)k];
+                const int b = bin_of(t);
+                ++cnt[b];
+                bb[b].grow(tb[(size_t)t]);
+            }
+            double ra[kBins];
+            int rn[kBins];
+            Box acc;
+            int an = 0;
+            for (int b = kBins - 1; b > 0; --b) {   // ra[b], rn[b]: bins b..end
+                acc.grow(bb[b]);
+                an += cnt[b];
+                ra[b] = acc.area();
+                rn[b] = an;
+            }
+            Box lacc;
+            int ln = 0, best_b = -1;
+            double best_cost = INFINITY;
+            for (int b = 0; b < kBins - 1; ++b) {   // split between bins b and b + 1
+                lacc.grow(bb[b]);
+                ln += cnt[b];
+                if (ln == 0 || rn[b + 1] == 0) continue;
+                const double cost = lacc.area() * ln + ra[b + 1] * rn[b + 1];
+                if (cost < best_cost) { best_cost = cost; best_b = b; }
+            }
+            if (best_b >= 0) {
+                auto pm = std::partition(prim.begin() + it.start, prim.begin() + it.end,
+                                         [&](int t) { return bin_of(t) <= best_b; });
+                mid = (int)(pm - prim.begin());
+                if (mid == it.start || mid == it.end) mid = -1;
+            }
+        }
+        if (mid < 0) {   // median split on the widest centroid axis
+            mid = it.start + n / 2;
+            std::nth_element(prim.begin() + it.start, prim.begin() + mid, prim.begin() + it.end, [&](int a, int b) {
+                const double ca = cen[(size_t)3 * a + axis], cb2 = cen[(size_t)3 * b + axis];
+                return ca < cb2 || (ca == cb2 && a < b);
+            });
+        }
+        const int left = (int)nodes.size();
+        nodes.push_back(BvhNode{});
+        nodes.push_back(BvhNode{});
+        nodes[(size_t)it.node].first = left;
+        nodes[(size_t)it.node].count = 0;
+        work.push_back({left + 1, mid, it.end, it.depth + 1});
+        work.push_back({left, it.start, mid, it.depth + 1});
+    }
+    if (depth >= kBvhStack - 2) throw GeomError("mesh hierarchy deeper than the traversal stack");
+    auto up = [](Buffer& b, const void* src, size_t bytes) {
+        b = Buffer(bytes);
+        HIPCHK(hipMemcpy(b.p, src, bytes, hipMemcpyHostToDevice));
+    };
+    up(m.v, verts.data(), verts.size() * sizeof(double));
+    up(m.f, faces.data(), faces.size() * sizeof(int));
+    up(m.nodes, nodes.data(), nodes.size() * sizeof(BvhNode));
+    up(m.prim, prim.data(), prim.size() * sizeof(int));
+    m.ntri = F;
+    m.nnodes = (int)nodes.size();
+    m.depth = depth;
+    // A triangle's computed distance is within a few ulps of coordinate size
+    // of the true one, which is at least the box distance.
+    m.slack = 1e-12 * (1.0 + scale);
+}
+
+// ---------------------------------------------------------------- queries
+#pragma clang fp contract(off)
+
+__device__ __forceinline__ double dot3(const double* a, const double* b) {
+    double s = 0.0;
+    s += a[0] * b[0];
+    s += a[1] * b[1];
+    s += a[2] * b[2];
+    return s;
+}
+
+// Closest point on triangle abc to p: Ericson, "Real-Time Collision
+// Detection", section 5.1.5 (Voronoi regions of the vertices, edges, face).
+__device__ void cp_triangle(const double* p, const double* a, const double* b, const double* c, double* out) {
+    double ab[3], ac[3], ap[3];
+    for (int d = 0; d < 3; ++d) { ab[d] = b[d] - a[d]; ac[d] = c[d] - a[d]; ap[d] = p[d] - a[d]; }
+    const double d1 = dot3(ab, ap), d2 = dot3(ac, ap);
+    if (d1 <= 0 && d2 <= 0) { for (int d = 0; d < 3; ++d) out[d] = a[d]; return; }
+    double bp[3];
+    for (int d = 0; d < 3; ++d) bp[d] = p[d] - b[d];
+    const double d3 = dot3(ab, bp), d4 = dot3(ac, bp);
+    if (d3 >= 0 && d4 <= d3) { for (int d = 0; d < 3; ++d) out[d] = b[d]; return; }
+    const double vc = d1 * d4 - d3 * d2;
+    if (vc <= 0 && d1 >= 0 && d3 <= 0) {
+        const double v = d1 / (d1 - d3);
+        for (int d = 0; d < 3; ++d) out[d] = a[d] + v * ab[d];
+        return;
+    }
+    double cpv[3];
+    for (int d = 0; d < 3; ++d) cpv[d] = p[d] - c[d];
+    const double d5 = dot3(ab, cpv), d6 = dot3(ac, cpv);
+    if (d6 >= 0 && d5 <= d6) { for (int d = 0; d < 3; ++d) out[d] = c[d]; return; }
+    const double vb = d5 * d2 - d1 * d6;
+    if (vb <= 0 && d2 >= 0 && d6 <= 0) {
+        const double w = d2 / (d2 - d6);
+        for (int d = 0; d < 3; ++d) out[d] = a[d] + w * ac[d];
+        return;
+    }
+    const double va = d3 * d6 - d5 * d4;
+    if (va <= 0 && (d4 - d3) >= 0 && (d5 - d6) >= 0) {
+        const double w = (d4 - d3) / ((d4 - d3) + (d5 - d6));
+        for (int d = 0; d < 3; ++d) out[d] = b[d] + w * (c[d] - b[d]);
+        return;
+    }
+    const double den = 1.0 / (va + vb + vc);
+    const double v = vb * den, w = vc * den;
+    for (int d = 0; d < 3; ++d) out[d] = a[d] + ab[d] * v + ac[d] * w;
+}
+
+__device__ __forceinline__ double dist3(const double* a, const double* b) {
+    double s = 0.0;
+    for (int d = 0; d < 3; ++d) s += (a[d] - b[d]) * (a[d] - b[d]);
+    return sqrt(s);
+}
+
+#pragma clang fp contract(on)
+
+// Squared distance from x to the node's box.
+__device__ __forceinline__ double box_d2(const BvhNode* nd, const double* x) {
+    double s = 0.0;
+    for (int d = 0; d < 3; ++d) {
+        const double e = fmax(fmax(nd->lo[d] - x[d], x[d] - nd->hi[d]), 0.0);
+        s += e * e;
+    }
+    return s;
+}
+
+// Depth-first traversal, nearer child first.  A box is skipped only when it is
+// farther than the best distance plus the slack, so every triangle whose
+// computed distance could tie or beat the best is visited; ties go to the
+// lower triangle index.
+__device__ void cp_mesh(const MeshDev& m, const double* x, double* c, double& dist) {
+    int stack[kBvhStack];
+    int sp = 0;
+    double best = INFINITY;
+    int bt = 0x7fffffff;
+    c[0] = c[1] = c[2] = NAN;
+    int node = 0;
+    for (;;) {
+        const BvhNode* nd = m.nodes + node;
+        if (nd->count > 0) {
+            for (int k = 0; k < nd->count; ++k) {
+                const int t = m.prim[nd->first + k];
+                const int* fv = m.f + 3 * (int64_t)t;
+                double q[3];
+                cp_triangle(x, m.v + 3 * (int64_t)fv[0], m.v + 3 * (int64_t)fv[1], m.v + 3 * (int64_t)fv[2], q);
+                const double dd = dist3(x, q);
+                if (dd < best || (dd == best && t < bt)) {
+                    best = dd;
+                    bt = t;
+                    c[0] = q[0]; c[1] = q[1]; c[2] = q[2];
+                }
+            }
+        } else {
+            const double lim = (best + m.slack) * (best + m.slack);
+            const int l = nd->first;
+            const double dl = box_d2(m.nodes + l, x), dr = box_d2(m.nodes + l + 1, x);
+            const bool vl = dl <= lim, vr = dr <= lim;
+            if (vl && vr) {
+                if (sp == kBvhStack) { dist = NAN; return; }   // unreachable: the builder bounds the depth
+                const bool lfirst = dl <= dr;
+                stack[sp++] = lfirst ? l + 1 : l;
+                node = lfirst ? l : l + 1;
+                continue;
+            }
+            if (vl || vr) {
+                node = vl ? l : l + 1;
+                continue;
+            }
+        }
+        const double lim = (best + m.slack) * (best + m.slack);
+        bool more = false;
+        while (sp > 0) {
+            const int cand = stack[--sp];
+            if (box_d2(m.nodes + cand, x) <= lim) { node = cand; more = true; break; }
+        }
+        if (!more) break;
+    }
+    dist = best;
+}
+
+}  // namespace cpmgpu

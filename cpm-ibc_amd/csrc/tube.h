@@ -5,13 +5,14 @@
 #include <vector>
 
 #include "engine.h"
+#include "mesh.h"
 #include "ops.h"
 
 namespace cpmgpu {
 
 enum SurfType : int {
     SURF_SPHERE = 1, SURF_HEMISPHERE = 2, SURF_TORUS = 3, SURF_CIRCLE3D = 4, SURF_POINT = 5,
-    SURF_POLYLINE = 6, SURF_ELLIPSOID = 7
+    SURF_POLYLINE = 6, SURF_ELLIPSOID = 7, SURF_TRIANGULATION = 8
 };
 
 // Device description of a surface (src/Surface.cpp:61-190 dispatch, resolved
@@ -24,6 +25,7 @@ struct SurfDev {
     double P[16];             // parameters (SurfaceSpecifier order)
     const double* verts;      // polyline vertices, 3 per vertex
     double ep0[3], ep1[3];    // open-polyline end points (boundary tags)
+    MeshDev mesh;             // triangulations
 };
 
 struct SurfaceHost {
@@ -31,12 +33,17 @@ struct SurfaceHost {
     std::vector<double> params;
     double bbox[2] = {0, 0};
     int dim = 3;
-    std::vector<double> verts;   // polyline vertices (x,y,z)...
+    std::vector<double> verts;   // polyline / triangulation vertices (x,y,z)...
+    std::vector<int> faces;      // triangulation: 3 vertex indices per triangle
     bool is_open = false;
 };
 
 SurfaceHost surface_from_desc(const char* type, const double* params, int nparams, double bb0, double bb1, int dim,
                               const double* verts, int64_t nverts);
+// "Triangulation" (SurfaceSpecifier.cpp:298-363): bbox_lo >= bbox_hi selects
+// the reference's box [min(0, coordinates), max(0, coordinates)].
+SurfaceHost surface_mesh(const double* verts, int64_t nverts, const int32_t* faces, int64_t nfaces, double bb0,
+                         double bb1);
 
 struct GridDev {
     int64_t N1d;
@@ -48,6 +55,7 @@ struct Tube {
     SurfaceHost surf;
     SurfDev sd{};
     Buffer surf_verts;
+    MeshBvh mesh;       // triangulations: hierarchy over the triangles
     int dim = 3, p = 3;
     double dx = 0, xstart = 0, rfac = 0, r_t = 0;
     int64_t n = 0, N1d = 0, N = 0;
@@ -77,7 +85,7 @@ __device__ void weights_1d(const double* xs, double xq, int S, double* w);
 // Base point b (grid coordinates) and 1-D barycentric weights of the degree-p
 // stencil at xq (Interpolation.cpp:52-121); false when it leaves the grid.
 __device__ bool stencil_weights(const GridDev& g, int p, const double* xq, int64_t* b, double (*w)[5]);
-SurfDev make_surfdev(const SurfaceHost& s, const double* dverts);
+SurfDev make_surfdev(const SurfaceHost& s, const double* dverts, const MeshDev* mesh = nullptr);
 
 // E rows at query points xq (device, 3 doubles each) as slot-major ELL of width
 // (p+1)^dim; columns are tube indices.  Throws GeomError when a stencil leaves

@@ -257,6 +257,9 @@ __device__ void cp_surface(const SurfDev& s, const double* x, double* c, double&
         dist = sqrt((x[0] - c[0]) * (x[0] - c[0]) + (x[1] - c[1]) * (x[1] - c[1]) + (x[2] - c[2]) * (x[2] - c[2]));
         break;
     }
+    case SURF_TRIANGULATION:   // Surface.cpp:1751-1779 (|signed distance|, boundary tag 0)
+        cp_mesh(s.mesh, x, c, dist);
+        break;
     case SURF_POLYLINE: {   // exact FP64 segment projection, first minimum wins
         double best = INFINITY;
         for (int k = 0; k < s.nseg; ++k) {
@@ -562,7 +565,29 @@ SurfaceHost surface_from_desc(const char* type, const double* params, int nparam
     return s;
 }
 
-SurfDev make_surfdev(const SurfaceHost& s, const double* dverts) {
+SurfaceHost surface_mesh(const double* verts, int64_t nverts, const int32_t* faces, int64_t nfaces, double bb0,
+                         double bb1) {
+    if (!verts || !faces || nverts < 3 || nfaces < 1) throw ArgError("mesh needs vertices and triangles");
+    SurfaceHost s;
+    s.type = "Triangulation";
+    s.dim = 3;
+    s.verts.assign(verts, verts + 3 * nverts);
+    s.faces.assign(faces, faces + 3 * nfaces);
+    s.is_open = false;   // SurfaceSpecifier.cpp:319
+    if (!(bb0 < bb1)) {  // SurfaceSpecifier::SetSurface() (:346-363)
+        bb0 = 0.0;
+        bb1 = 0.0;
+        for (double x : s.verts) {
+            bb0 = std::min(bb0, x);
+            bb1 = std::max(bb1, x);
+        }
+    }
+    s.bbox[0] = bb0;
+    s.bbox[1] = bb1;
+    return s;
+}
+
+SurfDev make_surfdev(const SurfaceHost& s, const double* dverts, const MeshDev* mesh) {
     SurfDev d{};
     d.dim = 3;
     for (size_t i = 0; i < s.params.size() && i < 16; ++i) d.P[i] = s.params[i];
@@ -578,7 +603,11 @@ SurfDev make_surfdev(const SurfaceHost& s, const double* dverts) {
             throw ArgError("Ellipsoid needs three positive semi-axes");
         d.type = SURF_ELLIPSOID;
     }
-    else if (s.type == "Polyline") {
+    else if (s.type == "Triangulation") {
+        if (!mesh) throw ArgError("triangulation without its hierarchy");
+        d.type = SURF_TRIANGULATION;
+        d.mesh = *mesh;
+    } else if (s.type == "Polyline") {
         d.type = SURF_POLYLINE;
         d.verts = dverts;
         d.nseg = (int)(s.verts.size() / 3) - 1;
@@ -602,12 +631,18 @@ Tube* tube_build(Ctx* ctx, const SurfaceHost& s, int p, double dx) {
     T->dim = 3;
     T->p = p;
     T->dx = dx;
-    if (!s.verts.empty()) {
-        T->surf_verts.~Buffer();
-        new (&T->surf_verts) Buffer(s.verts.size() * sizeof(double));
-        HIPCHK(hipMemcpy(T->surf_verts.p, s.verts.data(), s.verts.size() * sizeof(double), hipMemcpyHostToDevice));
+    if (s.type == "Triangulation") {
+        mesh_build(T->mesh, s.verts, s.faces);
+        const MeshDev md = T->mesh.dev();
+        T->sd = make_surfdev(s, nullptr, &md);
+    } else {
+        if (!s.verts.empty()) {
+            T->surf_verts.~Buffer();
+            new (&T->surf_verts) Buffer(s.verts.size() * sizeof(double));
+            HIPCHK(hipMemcpy(T->surf_verts.p, s.verts.data(), s.verts.size() * sizeof(double), hipMemcpyHostToDevice));
+        }
+        T->sd = make_surfdev(s, T->surf_verts.as<double>());
     }
-    T->sd = make_surfdev(s, T->surf_verts.as<double>());
     // Tube.cpp:131-157 (m_order = 2)
     const int order = 2;
     T->rfac = 1.0001 * std::sqrt((T->dim - 1) * std::pow(0.5 * (p + 1), 2) + std::pow(0.5 * (order + p + 1), 2));

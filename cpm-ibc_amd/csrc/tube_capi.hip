@@ -100,6 +100,16 @@ int cpm_tube_create(cpm_ctx* ctx, const char* surface_type, const double* params
     });
 }
 
+int cpm_tube_create_mesh(cpm_ctx* ctx, const double* verts, int64_t nverts, const int32_t* faces, int64_t nfaces,
+                         double bbox_lo, double bbox_hi, int p, double dx, cpm_tube** out) {
+    return guard([&] {
+        if (!ctx || !out) throw ArgError("null argument");
+        HIPCHK(hipSetDevice(C(ctx)->device));
+        SurfaceHost s = surface_mesh(verts, nverts, faces, nfaces, bbox_lo, bbox_hi);
+        *out = reinterpret_cast<cpm_tube*>(tube_build(C(ctx), s, p, dx));
+    });
+}
+
 int cpm_tube_destroy(cpm_tube* t) {
     return guard([&] { delete T(t); });
 }

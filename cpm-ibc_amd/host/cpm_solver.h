@@ -237,6 +237,17 @@ class Tube {
               "Tube");
         check(cpm_tube_get_info(t_, &info_), "Tube");
     }
+    // Triangle mesh ("Triangulation", SurfaceSpecifier(".obj")): vertices 3 per
+    // vertex, faces 3 indices per triangle; bbox_lo >= bbox_hi takes the
+    // reference's box from the vertices.
+    Tube(const std::vector<double>& verts, const std::vector<int32_t>& faces, Scalar dx, double bbox_lo = 0,
+         double bbox_hi = 0, int p = 3, std::shared_ptr<Context> ctx = Context::shared())
+        : ctx_(std::move(ctx)) {
+        check(cpm_tube_create_mesh(ctx_->get(), verts.data(), (int64_t)verts.size() / 3, faces.data(),
+                                   (int64_t)faces.size() / 3, bbox_lo, bbox_hi, p, dx, &t_),
+              "Tube");
+        check(cpm_tube_get_info(t_, &info_), "Tube");
+    }
     ~Tube() { cpm_tube_destroy(t_); }
     Tube(const Tube&) = delete;
     Tube& operator=(const Tube&) = delete;

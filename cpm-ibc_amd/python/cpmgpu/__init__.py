@@ -22,7 +22,7 @@ EXPORTS = [
     "cpm_ctx_device_bytes", "cpm_op_create_csr", "cpm_op_destroy", "cpm_op_rows", "cpm_op_format", "cpm_op_set_mn",
     "cpm_op_set_precond", "cpm_multiply", "cpm_multiply_device", "cpm_solve", "cpm_solve_device",
     "cpm_solve_batch", "cpm_solve_batch_device", "cpm_device_alloc", "cpm_device_free", "cpm_memcpy_h2d",
-    "cpm_memcpy_d2h", "cpm_tube_create", "cpm_tube_destroy", "cpm_tube_get_info", "cpm_tube_export",
+    "cpm_memcpy_d2h", "cpm_tube_create", "cpm_tube_create_mesh", "cpm_tube_destroy", "cpm_tube_get_info", "cpm_tube_export",
     "cpm_tube_export_E", "cpm_tube_export_L", "cpm_tube_interp_matrix", "cpm_op_create_from_tube", "cpm_profile_egather", "cpm_profile_copy", "cpm_heat_step_device", "cpm_rd_params_default", "cpm_rd_step_device", "cpm_problem_create", "cpm_problem_destroy",
     "cpm_problem_get_info", "cpm_problem_rhs_points", "cpm_problem_subset", "cpm_problem_export",
     "cpm_op_create_from_problem",
@@ -123,6 +123,8 @@ def load(path=LIB_PATH):
     L.cpm_memcpy_d2h.argtypes = [vp, vp, vp, ctypes.c_int64]
     L.cpm_tube_create.argtypes = [vp, ctypes.c_char_p, vp, ctypes.c_int, ctypes.c_double, ctypes.c_double,
                                   ctypes.c_int, vp, ctypes.c_int64, ctypes.c_int, ctypes.c_double, ctypes.POINTER(vp)]
+    L.cpm_tube_create_mesh.argtypes = [vp, vp, ctypes.c_int64, vp, ctypes.c_int64, ctypes.c_double, ctypes.c_double,
+                                       ctypes.c_int, ctypes.c_double, ctypes.POINTER(vp)]
     L.cpm_tube_destroy.argtypes = [vp]
     L.cpm_tube_get_info.argtypes = [vp, ctypes.POINTER(TubeInfo)]
     L.cpm_tube_export.argtypes = [vp, vp, vp, vp, vp, vp, vp, vp]
@@ -349,16 +351,25 @@ class CSR:
 class Tube:
     """GPU-built closest-point tube (Tube(SurfaceSpecifier&, p, dx))."""
 
-    def __init__(self, ctx, surface="Sphere", dx=0.1, params=None, bbox=None, p=3, verts=None):
+    def __init__(self, ctx, surface="Sphere", dx=0.1, params=None, bbox=None, p=3, verts=None, faces=None):
+        """surface "Triangulation" takes verts (nv x 3) and faces (nf x 3); bbox None then
+        gives the reference's box [min(0, coordinates), max(0, coordinates)]."""
         self.ctx = ctx
-        par = np.ascontiguousarray(params if params is not None else [0.0], np.float64)
-        npar = 0 if params is None else len(par)
-        bb = bbox if bbox is not None else (0.0, 0.0)
-        vv = np.ascontiguousarray(verts if verts is not None else np.zeros((1, 3)), np.float64)
-        nv = 0 if verts is None else len(vv)
         self._h = vp()
-        _chk(load().cpm_tube_create(ctx._h, surface.encode(), _ptr(par), npar, float(bb[0]), float(bb[1]), 3,
-                                    _ptr(vv), nv, p, dx, ctypes.byref(self._h)))
+        if surface == "Triangulation":
+            vv = np.ascontiguousarray(verts, np.float64).reshape(-1, 3)
+            ff = np.ascontiguousarray(faces, np.int32).reshape(-1, 3)
+            bb = bbox if bbox is not None else (0.0, 0.0)
+            _chk(load().cpm_tube_create_mesh(ctx._h, _ptr(vv), len(vv), _ptr(ff), len(ff), float(bb[0]), float(bb[1]),
+                                             p, dx, ctypes.byref(self._h)))
+        else:
+            par = np.ascontiguousarray(params if params is not None else [0.0], np.float64)
+            npar = 0 if params is None else len(par)
+            bb = bbox if bbox is not None else (0.0, 0.0)
+            vv = np.ascontiguousarray(verts if verts is not None else np.zeros((1, 3)), np.float64)
+            nv = 0 if verts is None else len(vv)
+            _chk(load().cpm_tube_create(ctx._h, surface.encode(), _ptr(par), npar, float(bb[0]), float(bb[1]), 3,
+                                        _ptr(vv), nv, p, dx, ctypes.byref(self._h)))
         inf = TubeInfo()
         _chk(load().cpm_tube_get_info(self._h, ctypes.byref(inf)))
         self.info = inf.as_dict()

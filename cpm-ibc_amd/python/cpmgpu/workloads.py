@@ -28,3 +28,79 @@ def polyline_parameter(V, pts, t0=0.0, t1=2 * np.pi, chunk=512):
         i = np.argmin(d, axis=1)
         out[s:s + chunk] = t0 + (t1 - t0) * (i + tt[np.arange(len(P)), i]) / nseg
     return out
+
+
+def icosphere(subdivisions=0, radius=1.0):
+    """Unit icosahedron refined `subdivisions` times by edge midpoints projected to the sphere:
+    20 * 4**subdivisions triangles, outward (counter-clockwise) orientation."""
+    g = (1 + 5 ** 0.5) / 2
+    V = [(-1, g, 0), (1, g, 0), (-1, -g, 0), (1, -g, 0), (0, -1, g), (0, 1, g), (0, -1, -g), (0, 1, -g),
+         (g, 0, -1), (g, 0, 1), (-g, 0, -1), (-g, 0, 1)]
+    V = [np.array(v, float) / np.linalg.norm(v) for v in V]
+    F = [(0, 11, 5), (0, 5, 1), (0, 1, 7), (0, 7, 10), (0, 10, 11), (1, 5, 9), (5, 11, 4), (11, 10, 2), (10, 7, 6),
+         (7, 1, 8), (3, 9, 4), (3, 4, 2), (3, 2, 6), (3, 6, 8), (3, 8, 9), (4, 9, 5), (2, 4, 11), (6, 2, 10),
+         (8, 6, 7), (9, 8, 1)]
+    for _ in range(subdivisions):
+        mid = {}
+
+        def m(a, b):
+            k = (a, b) if a < b else (b, a)
+            if k not in mid:
+                p = V[a] + V[b]
+                V.append(p / np.linalg.norm(p))
+                mid[k] = len(V) - 1
+            return mid[k]
+
+        F = [f for a, b, c in F for f in ((a, m(a, b), m(c, a)), (b, m(b, c), m(a, b)), (c, m(c, a), m(b, c)),
+                                          (m(a, b), m(b, c), m(c, a)))]
+    return radius * np.array(V), np.array(F, np.int32)
+
+
+def real_sph_harm(lmax, pts):
+    """Real spherical harmonics Y_lm (l <= lmax, |m| <= l) at the directions of pts, orthonormal on the
+    unit sphere, from the associated-Legendre recurrence; columns ordered (l, m = -l..l)."""
+    P = np.asarray(pts, float)
+    r = np.linalg.norm(P, axis=1)
+    ct = P[:, 2] / r
+    st = np.sqrt(np.maximum(0.0, 1 - ct * ct))
+    ph = np.arctan2(P[:, 1], P[:, 0])
+    # normalised associated Legendre functions bar P_l^m(ct) (no Condon-Shortley phase)
+    Pl = {(0, 0): np.full_like(ct, np.sqrt(1 / (4 * np.pi)))}
+    for m in range(1, lmax + 1):
+        Pl[(m, m)] = np.sqrt((2 * m + 1) / (2 * m)) * st * Pl[(m - 1, m - 1)]
+    for m in range(0, lmax):
+        Pl[(m + 1, m)] = np.sqrt(2 * m + 3) * ct * Pl[(m, m)]
+    for m in range(0, lmax + 1):
+        for l in range(m + 2, lmax + 1):
+            a = np.sqrt((4 * l * l - 1) / (l * l - m * m))
+            b = np.sqrt(((l - 1) ** 2 - m * m) / (4 * (l - 1) ** 2 - 1))
+            Pl[(l, m)] = a * (ct * Pl[(l - 1, m)] - b * Pl[(l - 2, m)])
+    cols = []
+    for l in range(lmax + 1):
+        for m in range(-l, l + 1):
+            if m < 0:
+                cols.append(np.sqrt(2) * Pl[(l, -m)] * np.sin(-m * ph))
+            elif m == 0:
+                cols.append(Pl[(l, 0)])
+            else:
+                cols.append(np.sqrt(2) * Pl[(l, m)] * np.cos(m * ph))
+    return np.stack(cols, 1)
+
+
+def displaced_icosphere(subdivisions=6, amplitude=0.1, lmax=6, seed=4):
+    """Icosphere with radial displacement 1 + amplitude g / max|g|, g = sum_{l <= lmax} b_lm Y_lm,
+    b_lm ~ N(0, 1) from numpy's default_rng(seed): the mesh surface of the serving workload
+    (subdivisions 6: 81,920 triangles)."""
+    V, F = icosphere(subdivisions)
+    Y = real_sph_harm(lmax, V)
+    b = np.random.default_rng(seed).standard_normal(Y.shape[1])
+    g = Y @ b
+    return V * (1 + amplitude * g / np.abs(g).max())[:, None], F
+
+
+def box_mesh(lo=-1.0, hi=1.0):
+    """Axis-aligned cube [lo, hi]^3 as 12 outward triangles."""
+    c = np.array([[x, y, z] for x in (lo, hi) for y in (lo, hi) for z in (lo, hi)], float)
+    F = np.array([[0, 1, 3], [0, 3, 2], [4, 6, 7], [4, 7, 5], [0, 4, 5], [0, 5, 1], [2, 3, 7], [2, 7, 6],
+                  [0, 2, 6], [0, 6, 4], [1, 5, 7], [1, 7, 3]], np.int32)
+    return c, F

@@ -142,7 +142,8 @@ int cpm_solve_batch_device(cpm_ctx* ctx, int n_inst, cpm_op* const* ops, const d
  * (src/SurfaceSpecifier.cpp:65-256); nparams = 0 selects the reference
  * defaults.  "Polyline" takes nverts vertices (x,y,z) as consecutive segments.
  * Supported on the GPU: Sphere (radius[, centre]), Hemisphere, Torus (R, r),
- * Circle3D (r, centre, normal), Point, Polyline; dim must be 3. */
+ * Ellipsoid (a, b, c), Circle3D (r, centre, normal), Point, Polyline, and
+ * triangle meshes through cpm_tube_create_mesh; dim must be 3. */
 typedef struct cpm_tube_info {
     int64_t nodes;          /* nNodes()                                  */
     int64_t n;              /* grid has 2^n + 1 nodes per axis           */
@@ -161,6 +162,15 @@ typedef struct cpm_tube_info {
 int cpm_tube_create(cpm_ctx* ctx, const char* surface_type, const double* params, int nparams, double bbox_lo,
                     double bbox_hi, int dim, const double* verts, int64_t nverts, int p, double dx,
                     cpm_tube** out);
+/* Tube around a closed triangle mesh: SurfaceSpecifier(".obj") with surface
+ * type "Triangulation" (src/SurfaceSpecifier.cpp:298-363) and the closest
+ * point of Surface::cpTri (src/Surface.cpp:1751-1779), which the reference
+ * answers with fcpw's BVH (:2537-2592).  verts: nverts x 3, faces: nfaces x 3
+ * vertex indices.  bbox_lo >= bbox_hi takes the reference's box
+ * [min(0, coordinates), max(0, coordinates)].  Closest points are the exact
+ * FP64 minimisers (lowest triangle index on ties), from a BVH built here. */
+int cpm_tube_create_mesh(cpm_ctx* ctx, const double* verts, int64_t nverts, const int32_t* faces, int64_t nfaces,
+                         double bbox_lo, double bbox_hi, int p, double dx, cpm_tube** out);
 int cpm_tube_destroy(cpm_tube* tube);
 int cpm_tube_get_info(const cpm_tube* tube, cpm_tube_info* info);
 /* Node data in tube order (any pointer may be NULL): linear grid index

@@ -27,6 +27,10 @@ def lib():
             raise RuntimeError(f"oracle not built: {path} (run make -C oracle)")
         L = ctypes.CDLL(path)
         L.orc_last_error.restype = ctypes.c_char_p
+        L.orc_set_mesh.restype = None
+        L.orc_set_mesh.argtypes = [f64p, ctypes.c_int64, np.ctypeslib.ndpointer(np.int32, flags="C_CONTIGUOUS"),
+                                   ctypes.c_int64]
+        L.orc_mesh_cp.argtypes = [f64p, ctypes.c_int64, f64p, f64p]
         L.orc_problem_poisson.restype = ctypes.c_void_p
         L.orc_problem_poisson.argtypes = [ctypes.c_char_p, ctypes.c_int, ctypes.c_void_p, ctypes.c_int,
                                           ctypes.c_double, ctypes.c_double, ctypes.c_int, ctypes.c_void_p,
@@ -174,6 +178,32 @@ class Problem:
             raise RuntimeError(_err())
         self._info()
         return self._csr(2, rows, self.nnz_extra)
+
+
+def set_mesh(verts, faces):
+    """Triangle mesh (nv x 3 vertices, nf x 3 indices) for the next "Triangulation" surface on this thread."""
+    v = np.ascontiguousarray(verts, np.float64).reshape(-1, 3)
+    f = np.ascontiguousarray(faces, np.int32).reshape(-1, 3)
+    lib().orc_set_mesh(v.ravel(), len(v), f.ravel(), len(f))
+
+
+def mesh_cp(verts, faces, x):
+    """Closest points and distances on a triangle mesh (brute force over triangles, Ericson 5.1.5)."""
+    set_mesh(verts, faces)
+    x = np.ascontiguousarray(x, np.float64).reshape(-1, 3)
+    cp = np.zeros_like(x)
+    d = np.zeros(len(x))
+    if lib().orc_mesh_cp(x.ravel(), len(x), cp.ravel(), d) < 0:
+        raise RuntimeError(_err())
+    return cp, d
+
+
+def mesh_problem(verts, faces, dx, bbox=None, p=3):
+    """Poisson problem on a triangle mesh (SurfaceSpecifier(".obj"), closest points by brute force)."""
+    set_mesh(verts, faces)
+    if bbox is None:
+        return Problem("Triangulation", dx, p=p)
+    return Problem("Triangulation", dx, params=[0.0], bbox=bbox, p=p)
 
 
 def multiply(L, E, ident, x, m, n, nD=0):

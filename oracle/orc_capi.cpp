@@ -34,16 +34,39 @@ struct Handle {
     std::vector<int32_t> rhs_kind; // 0 tube, 1 ghost (f at cp_c), 2 identity (g at cp_c)
 };
 
+// Triangle mesh handed in by orc_set_mesh for the next "Triangulation" surface.
+thread_local std::vector<Vec> g_mesh_v;
+thread_local std::vector<std::vector<i64>> g_mesh_f;
+
 SurfaceSpec make_spec(const char* type, int use_defaults, const double* params, int np, double bb0, double bb1,
                       int dim, const double* verts, int nv) {
     std::string ty(type);
+    if (ty == "Triangulation") {   // SurfaceSpecifier.cpp:298-363
+        if (g_mesh_f.empty()) throw std::runtime_error("orc_set_mesh first");
+        SurfaceSpec s;
+        s.type = ty;
+        s.dim = 3;
+        s.verts = g_mesh_v;
+        s.faces = g_mesh_f;
+        s.is_open = false;
+        if (use_defaults) {        // SetSurface(): [min(0, coordinates), max(0, coordinates)]
+            bb0 = 0.0;
+            bb1 = 0.0;
+            for (const auto& v : s.verts)
+                for (double x : v) {
+                    bb0 = std::min(bb0, x);
+                    bb1 = std::max(bb1, x);
+                }
+        }
+        s.bbox = {bb0, bb1};
+        return s;
+    }
     if (ty == "Polyline") {
         std::vector<Vec> v((size_t)nv, Vec((size_t)dim));
         for (int i = 0; i < nv; ++i)
             for (int d = 0; d < dim; ++d) v[(size_t)i][(size_t)d] = verts[i * dim + d];
         return SurfaceSpec::polyline(v, {bb0, bb1});
     }
-    if (ty == "Triangulation") throw std::runtime_error("use orc_tube_mesh for triangulations");
     SurfaceSpec s = SurfaceSpec::defaults(ty);
     if (!use_defaults) {
         s.params.assign(params, params + np);
@@ -72,6 +95,32 @@ CSR import_csr(int64_t rows, const int32_t* ptr, const int32_t* col, const doubl
 extern "C" {
 
 const char* orc_last_error() { return g_err.c_str(); }
+
+// Closest points on the mesh of orc_set_mesh at n query points (Surface::cpTri,
+// Surface.cpp:1751-1779: |signed distance|).
+int orc_mesh_cp(const double* x, int64_t n, double* cp, double* dist) {
+    try {
+        SurfaceSpec s = make_spec("Triangulation", 1, nullptr, 0, 0, 0, 3, nullptr, 0);
+        Surface S(s);
+        for (int64_t i = 0; i < n; ++i) {
+            i64 b;
+            S.cp(x + 3 * i, cp + 3 * i, dist[i], b);
+        }
+        return 0;
+    } catch (const std::exception& e) {
+        g_err = e.what();
+        return -1;
+    }
+}
+
+void orc_set_mesh(const double* v, int64_t nv, const int32_t* f, int64_t nf) {
+    g_mesh_v.assign((size_t)nv, Vec(3));
+    for (int64_t i = 0; i < nv; ++i)
+        for (int d = 0; d < 3; ++d) g_mesh_v[(size_t)i][(size_t)d] = v[3 * i + d];
+    g_mesh_f.assign((size_t)nf, std::vector<i64>(3));
+    for (int64_t i = 0; i < nf; ++i)
+        for (int k = 0; k < 3; ++k) g_mesh_f[(size_t)i][(size_t)k] = f[3 * i + k];
+}
 
 void* orc_problem_poisson(const char* type, int use_defaults, const double* params, int np, double bb0, double bb1,
                           int dim, const double* verts, int nv, double dx, int bdy_order, const int64_t* which_bdy,

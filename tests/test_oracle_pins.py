@@ -260,3 +260,72 @@ def test_rd_uniform_state_follows_reaction_ode(method):
     assert du < 1e-12 and dv < 1e-12, (du, dv)
     if method == 0:                                 # the final extension rewrites every row
         assert np.abs(u - a).max() < 1e-12
+
+
+def _cp_mesh_by_projection(V, F, x):
+    """Closest point on a triangle mesh by an independent route: the nearest of
+    the plane projection (when its barycentric coordinates are all >= 0) and the
+    clamped projections onto the three edges, over all triangles.  Returns the
+    closest point, the distance and the margin to the next distinct candidate."""
+    A, B, C = V[F[:, 0]], V[F[:, 1]], V[F[:, 2]]
+    N = np.cross(B - A, C - A)
+    cps, ds, margins = [], [], []
+    for p in x:
+        cand = []
+        t = ((p - A) * N).sum(1) / (N * N).sum(1)
+        q = p - t[:, None] * N
+        # barycentric coordinates of q by sub-triangle areas along N
+        l0 = (np.cross(B - q, C - q) * N).sum(1)
+        l1 = (np.cross(C - q, A - q) * N).sum(1)
+        l2 = (np.cross(A - q, B - q) * N).sum(1)
+        inside = (l0 >= 0) & (l1 >= 0) & (l2 >= 0)
+        cand.append(np.where(inside[:, None], q, np.nan))
+        for P, Q in ((A, B), (B, C), (C, A)):
+            s = np.clip(((p - P) * (Q - P)).sum(1) / ((Q - P) ** 2).sum(1), 0, 1)
+            cand.append(P + s[:, None] * (Q - P))
+        Cq = np.concatenate(cand)
+        d = np.linalg.norm(Cq - p, axis=1)
+        d[np.isnan(d)] = np.inf
+        i = np.argmin(d)
+        far = np.linalg.norm(Cq - Cq[i], axis=1) > 1e-9
+        cps.append(Cq[i])
+        ds.append(d[i])
+        margins.append(d[far].min() - d[i] if far.any() else np.inf)
+    return np.array(cps), np.array(ds), np.array(margins)
+
+
+@pytest.mark.parametrize("mesh", ["icosphere", "displaced"])
+def test_mesh_closest_point_independent_route(mesh):
+    """Surface::cpTri (Surface.cpp:1751-1779; fcpw in the reference, parity with
+    fcpw unpinned): the oracle's Ericson region walk equals the projection
+    route on random points around the mesh."""
+    from cpmgpu import workloads
+    V, F = workloads.icosphere(2) if mesh == "icosphere" else workloads.displaced_icosphere(2, seed=7)
+    x = np.random.default_rng(5).uniform(-1.4, 1.4, (400, 3))
+    cp, d = orc.mesh_cp(V, F, x)
+    cq, dq, margin = _cp_mesh_by_projection(V, F, x)
+    assert np.abs(d - dq).max() < 1e-13
+    assert np.abs(d - np.linalg.norm(cp - x, axis=1)).max() < 1e-14
+    uniq = margin > 1e-9
+    assert uniq.mean() > 0.95 and np.abs(cp - cq)[uniq].max() < 1e-12
+
+
+def test_mesh_closest_point_box_closed_form():
+    """On the cube [-1, 1]^3 made of 12 triangles: outside points project to
+    clip(x, -1, 1); inside points to the nearest face (unique nearest face only)."""
+    from cpmgpu import workloads
+    V, F = workloads.box_mesh()
+    x = np.random.default_rng(6).uniform(-2, 2, (3000, 3))
+    cp, d = orc.mesh_cp(V, F, x)
+    out = (np.abs(x) > 1).any(1)
+    c = np.clip(x, -1, 1)
+    assert np.abs(cp[out] - c[out]).max() < 1e-15
+    assert np.abs(d[out] - np.linalg.norm(x[out] - c[out], axis=1)).max() < 1e-15
+    gap = 1 - np.abs(x[~out])
+    assert np.abs(d[~out] - gap.min(1)).max() < 1e-15
+    srt = np.sort(gap, 1)
+    uniq = srt[:, 1] - srt[:, 0] > 1e-12
+    k = np.argmin(gap, 1)
+    exp = x[~out].copy()
+    exp[np.arange(len(exp)), k] = np.sign(exp[np.arange(len(exp)), k])
+    assert np.abs(cp[~out][uniq] - exp[uniq]).max() < 1e-15

@@ -424,7 +424,7 @@ static void run_batch(Ctx* ctx, const std::vector<Group>& groups, const std::vec
 
 void solve_batch(Ctx* ctx, const std::vector<Op*>& ops, const std::vector<const double*>& rhs,
                  const std::vector<double*>& xs, const cpm_solve_opts& o, std::vector<cpm_solve_info>& infos,
-                 const std::vector<std::pair<double, double>>* mn = nullptr) {
+                 const std::vector<std::pair<double, double>>* mn) {
     const int n = (int)ops.size();
     infos.assign(n, cpm_solve_info{});
     if (n == 0) return;
@@ -616,6 +616,17 @@ static void extension(Op* op, const StepTables& t, const double* z, double* y) {
     launch_egather<double, 1>(cfg, z, y);
 }
 
+void apply_E(Op* op, const double* x, double* y) {
+    const hipStream_t st = op->ctx->stream;
+    const StepTables t = step_tables(op, "applyE_tables", {0.0}, {pack_in(x), pack_out(y)});
+    double* w = op->ctx->workspace("applyE_vecs", sizeof(double) * op->D * 2);
+    hipLaunchKernelGGL(k_pack, dim3(t.nblocks), dim3(256), 0, st, t.pe, t.bseg, w, (float*)nullptr, 1);
+    extension(op, t, w, w + op->D);
+    hipLaunchKernelGGL(k_pack, dim3(t.nblocks), dim3(256), 0, st, t.pe + 1, t.bseg, w + op->D, (float*)nullptr, 0);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipStreamSynchronize(st));   // the tables are rewritten by the next call
+}
+
 static void set_dirichlet(Op* op, const double* g, double* u) {
     hipLaunchKernelGGL(k_set_identity_rows, dim3((unsigned)((op->D + 255) / 256)), dim3(256), 0, op->ctx->stream,
                        op->flag.as<uint8_t>(), op->perm.as<int>(), op->D, g, u);
@@ -792,6 +803,14 @@ int status_of_current_exception() {
         return CPM_ERR_STATE;
     }
 }
+
+cpm_solve_opts resolve_opts(const cpm_solve_opts* o) {
+    cpm_solve_opts r;
+    cpm_solve_opts_default(&r);
+    if (o) r = *o;
+    if (!(r.tol >= 0)) throw ArgError("tol must be >= 0");
+    return r;
+}
 }  // namespace cpmgpu
 
 template <typename F>
@@ -905,14 +924,6 @@ int cpm_multiply(cpm_op* op, const double* x, double* y, double m, double n) {
         HIPCHK(hipMemcpyAsync(y, d + o->D, sizeof(double) * o->D, hipMemcpyDeviceToHost, ctx->stream));
         HIPCHK(hipStreamSynchronize(ctx->stream));
     });
-}
-
-static cpm_solve_opts resolve_opts(const cpm_solve_opts* o) {
-    cpm_solve_opts r;
-    cpm_solve_opts_default(&r);
-    if (o) r = *o;
-    if (!(r.tol >= 0)) throw ArgError("tol must be >= 0");
-    return r;
 }
 
 int cpm_solve_batch_device(cpm_ctx* ctx, int n_inst, cpm_op* const* ops, const double* const* d_rhs,

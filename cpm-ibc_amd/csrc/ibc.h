@@ -57,5 +57,9 @@ struct Problem {
 
 Problem* problem_build(Tube* tube, const std::vector<IbcSpec>& specs);
 Op* op_from_problem(Problem* P, double m, double n, int precond, int precond_iters);
+// GeodesicDistance::ComputeDistance up to its plotting interpolation: phi on
+// the D rows into d_phi; infos[0] the heat solve, infos[1..3] the Poisson solves.
+void geodesic_distance(Problem* P, double dt, int sweeps, const cpm_solve_opts& o, double* d_phi,
+                       cpm_solve_info* infos);
 
 }  // namespace cpmgpu

@@ -6,6 +6,7 @@
 #include <string>
 #include <vector>
 
+#include "../../include/cpm_gpu.h"
 #include "engine.h"
 
 namespace cpmgpu {
@@ -60,6 +61,16 @@ struct Op {
     template <typename T> OpView<T> view() const;
     size_t device_bytes() const;
 };
+
+// Batched BiCGStab over instances (ops[i], rhs[i] -> xs[i], caller-order
+// device vectors); mn, when given, overrides each instance's (m, n).
+void solve_batch(Ctx* ctx, const std::vector<Op*>& ops, const std::vector<const double*>& rhs,
+                 const std::vector<double*>& xs, const cpm_solve_opts& o, std::vector<cpm_solve_info>& infos,
+                 const std::vector<std::pair<double, double>>* mn = nullptr);
+// y = E x for caller-order device vectors of the operator's D rows (identity
+// rows copy x); x and y may not alias.
+void apply_E(Op* op, const double* x, double* y);
+cpm_solve_opts resolve_opts(const cpm_solve_opts* o);
 
 Op* op_from_csr(Ctx* ctx, int64_t D, int nD, const int32_t* Lp, const int32_t* Lc, const double* Lv,
                 const int32_t* Ep, const int32_t* Ec, const double* Ev, const uint8_t* ident, double m, double n,

@@ -321,6 +321,18 @@ int cpm_op_create_from_problem(cpm_problem* pp, double m, double n, int precond,
     });
 }
 
+int cpm_geodesic_distance(cpm_problem* prob, double dt, int jacobi_sweeps, const cpm_solve_opts* opts, double* d_phi,
+                          cpm_solve_info* infos) {
+    return guard([&] {
+        if (!prob || !d_phi) throw ArgError("null argument");
+        if (!(dt > 0)) throw ArgError("dt must be positive");
+        if (jacobi_sweeps < 1 || jacobi_sweeps > 100000) throw ArgError("jacobi_sweeps must be in 1..100000");
+        Problem* P = reinterpret_cast<Problem*>(prob);
+        HIPCHK(hipSetDevice(P->ctx->device));
+        geodesic_distance(P, dt, jacobi_sweeps, resolve_opts(opts), d_phi, infos);
+    });
+}
+
 int cpm_op_create_from_tube(cpm_tube* tp, double m, double n, int precond, int precond_iters, cpm_op** out) {
     return guard([&] {
         if (!tp || !out) throw ArgError("null argument");

@@ -246,6 +246,20 @@ int cpm_problem_export(const cpm_problem* prob, int32_t* L_rowptr, int32_t* L_co
  * problem's Lfull, Efull, identity rows). */
 int cpm_op_create_from_problem(cpm_problem* prob, double m, double n, int precond, int precond_iters, cpm_op** out);
 
+/* Geodesic distance from the problem's curves by the heat method,
+ * GeodesicDistance::ComputeDistance (problems/GeodesicDistance.cpp:11-67):
+ * heat flow from the curves, Solver(1, -dt) with jacobi_sweeps Jacobi sweeps
+ * (the reference uses dt = dx^2 and 2000 sweeps, :112-115); three rounds of
+ * normalised tangential gradient (centred differences, FDMatrices.cpp:64-92,
+ * extended by the tube's E), divergence and Poisson solve Solver(0, 1),
+ * shifted to minimum 0 (:36-60, :167-313).  d_phi receives phi on the
+ * problem's D rows (device memory; the reference then applies its plotting
+ * interpolation, :62-66).  Rows other than tube rows of the Poisson
+ * right-hand sides are 0.  infos (may be NULL): 4 entries, the heat solve and
+ * the three Poisson solves. */
+int cpm_geodesic_distance(cpm_problem* prob, double dt, int jacobi_sweeps, const cpm_solve_opts* opts, double* d_phi,
+                          cpm_solve_info* infos);
+
 /* ---- time stepping ------------------------------------------------------
  * One step of HeatWithIBC::TimeStep (problems/HeatWithIBC.cpp:80-103) on
  * device vectors of the operator's D rows (the problem's row order):

@@ -31,6 +31,8 @@ def lib():
         L.orc_set_mesh.argtypes = [f64p, ctypes.c_int64, np.ctypeslib.ndpointer(np.int32, flags="C_CONTIGUOUS"),
                                    ctypes.c_int64]
         L.orc_mesh_cp.argtypes = [f64p, ctypes.c_int64, f64p, f64p]
+        L.orc_geodesic.argtypes = [ctypes.c_void_p, ctypes.c_double, ctypes.c_int, f64p,
+                                   np.ctypeslib.ndpointer(np.int32, flags="C_CONTIGUOUS")]
         L.orc_problem_poisson.restype = ctypes.c_void_p
         L.orc_problem_poisson.argtypes = [ctypes.c_char_p, ctypes.c_int, ctypes.c_void_p, ctypes.c_int,
                                           ctypes.c_double, ctypes.c_double, ctypes.c_int, ctypes.c_void_p,
@@ -288,6 +290,15 @@ class IBCProblem(Problem):
         d = np.zeros(n * 3)
         lib().orc_ibc_subset(self._h, c, which, s, cp, d)
         return s, cp.reshape(-1, 3), d.reshape(-1, 3)
+
+    def geodesic(self, dt, sweeps=2000):
+        """GeodesicDistance::ComputeDistance before its plotting interpolation: phi on the D rows
+        and the iteration counts of the heat solve and the three Poisson solves."""
+        phi = np.zeros(self.D)
+        it = np.zeros(4, np.int32)
+        if lib().orc_geodesic(self._h, dt, sweeps, phi, it) < 0:
+            raise RuntimeError(_err())
+        return phi, it
 
     def rhs_points(self):
         pts = np.zeros(self.D * 3)

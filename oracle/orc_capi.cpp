@@ -16,6 +16,8 @@ double ibc_study_order(int problem, int g, int* iters_last);
 double heat_study_order(int method);
 double heat_problem_error(int method, double dx, int order, int btype);
 int64_t heat_run_study(int method, double dx, int steps, int order, int btype, double* u_out, double* dt_out);
+std::array<int, 4> geodesic_run(const Tube& t, const IBCData& I, const CSR& E, const CSR& Lfull, const CSR& Efull,
+                                const std::vector<uint8_t>& ident, double dt, int sweeps, Vec& phi);
 int64_t rd_run_study(int method, double dx, int order, int btype, const double* prm, int steps, double* u_io,
                      double* v_io);
 }
@@ -290,6 +292,22 @@ int64_t orc_rd_run(int method, double dx, int order, int btype, const double* pr
                    double* v_io) {
     try {
         return rd_run_study(method, dx, order, btype, prm, steps, u_io, v_io);
+    } catch (const std::exception& e) {
+        g_err = e.what();
+        return -1;
+    }
+}
+
+// GeodesicDistance::ComputeDistance on an IBC problem: phi (D rows), iterations of the four solves.
+int orc_geodesic(void* hp, double dt, int sweeps, double* phi, int32_t* iters) {
+    try {
+        auto* h = static_cast<Handle*>(hp);
+        if (!h->has_ibc) throw std::runtime_error("the geodesic distance needs an IBC problem");
+        Vec p;
+        const auto it = geodesic_run(h->P.tube, h->ibc, h->P.E, h->P.Lfull, h->P.Efull, h->P.ident, dt, sweeps, p);
+        std::copy(p.begin(), p.end(), phi);
+        for (int k = 0; k < 4; ++k) iters[k] = it[(size_t)k];
+        return 0;
     } catch (const std::exception& e) {
         g_err = e.what();
         return -1;

@@ -395,3 +395,105 @@ int64_t rd_run_study(int method, double dx, int order, int btype, const double* 
     return (int64_t)D;
 }
 }  // namespace orc
+
+// ---------------------------------------------------------------- geodesic distance
+// GeodesicDistance::ComputeDistance (problems/GeodesicDistance.cpp:11-67) up
+// to the plotting interpolation: phi on the D rows.  Rows other than tube
+// rows of the Poisson right-hand sides are 0 (the reference sizes that vector
+// N + G and fills only its Dirichlet rows, :268-290).
+namespace orc {
+
+static double smooth_h(double scale, double t) { return 0.5 * std::tanh(-scale * t) + 0.5; }   // :73-78
+
+// Dc[d] x on the tube rows, FDMatrices::BuildGradientMatrices (FDMatrices.cpp:64-92):
+// +0.5/dx at the +d neighbour, -0.5/dx at the -d neighbour.
+static double dc_row(const Tube& t, i64 i, int d, double c, const double* x) {
+    double s = 0.0;
+    const i64 p = t.nbr[(size_t)(6 * i + 2 * d)], m = t.nbr[(size_t)(6 * i + 2 * d + 1)];
+    if (p >= 0) s += c * x[(size_t)p];
+    if (m >= 0) s += (-c) * x[(size_t)m];
+    return s;
+}
+
+// Step2 + ComputeDivergence (:171-262): div = sum_d Dc[d] P(E Dc[d] u) / |.|
+static void geodesic_div(const Tube& t, const CSR& E, const std::vector<std::array<double, 3>>& nrm, const Vec& u,
+                         Vec& div) {
+    const i64 N = t.N();
+    const double c = 0.5 / t.dx;
+    std::vector<Vec> g(3, Vec((size_t)N)), eg(3, Vec((size_t)N));
+    for (int d = 0; d < 3; ++d) {
+        for (i64 i = 0; i < N; ++i) g[(size_t)d][(size_t)i] = dc_row(t, i, d, c, u.data());
+        E.matvec(g[(size_t)d].data(), eg[(size_t)d].data());
+    }
+    for (i64 i = 0; i < N; ++i) {
+        double v[3] = {eg[0][(size_t)i], eg[1][(size_t)i], eg[2][(size_t)i]};
+        const double* n = nrm[(size_t)i].data();
+        double s = 0.0;
+        for (int d = 0; d < 3; ++d) s += v[d] * n[d];
+        for (int d = 0; d < 3; ++d) v[d] = v[d] - s * n[d];
+        double q = 0.0;
+        for (int d = 0; d < 3; ++d) q += v[d] * v[d];
+        q = std::sqrt(q);
+        if (q != 0.0)
+            for (int d = 0; d < 3; ++d) v[d] /= q;
+        for (int d = 0; d < 3; ++d) eg[(size_t)d][(size_t)i] = v[d];
+    }
+    std::fill(div.begin(), div.end(), 0.0);
+    for (i64 i = 0; i < N; ++i) {
+        double s = 0.0;
+        for (int d = 0; d < 3; ++d) s += dc_row(t, i, d, c, eg[(size_t)d].data());
+        div[(size_t)i] = s;
+    }
+}
+
+std::array<int, 4> geodesic_run(const Tube& t, const IBCData& I, const CSR& E, const CSR& Lfull, const CSR& Efull,
+                                const std::vector<uint8_t>& ident, double dt, int sweeps, Vec& phi) {
+    const i64 N = t.N(), D = Lfull.rows;
+    std::array<int, 4> iters{};
+    const double scale = std::atanh(1.0 - t.dx) / t.r_t;       // :70-72
+    Vec rhs((size_t)D, 0.0), uf((size_t)D), u((size_t)N), u2((size_t)N), div((size_t)D);
+    auto cpd = [&](size_t c, size_t i) {
+        const auto& v = I.dof[c].cp_diff[i];
+        double s = 0.0;
+        for (int d = 0; d < 3; ++d) s += v[d] * v[d];
+        return std::sqrt(s);
+    };
+    for (size_t c = 0; c < I.dof.size(); ++c)
+        for (size_t i = 0; i < I.dof[c].set_of_sub.size(); ++i) {
+            const i64 r = heat_dirichlet_row(I, c, i);
+            if (r >= 0) rhs[(size_t)r] = smooth_h(scale, -cpd(c, i));
+        }
+    for (size_t c = 0; c < I.dof.size(); ++c)
+        for (size_t i = 0; i < I.dof[c].set_of_sub.size(); ++i)
+            rhs[(size_t)I.dof[c].set_of_sub[i]] = smooth_h(scale, cpd(c, i));
+    System S1, S3;
+    for (System* S : {&S1, &S3}) {
+        S->L = &Lfull;
+        S->E = &Efull;
+        S->ident = ident;
+    }
+    S1.m = 1.0;
+    S1.n = -dt;
+    S1.precond = PRECOND_JACOBI;
+    S1.precond_iters = sweeps;
+    S3.m = 0.0;
+    S3.n = 1.0;
+    S3.precond = PRECOND_DIAGONAL;
+    iters[0] = bicgstab(S1, rhs.data(), uf.data(), 1e-10, 100000).iters;
+    for (i64 i = 0; i < N; ++i) u[(size_t)i] = -uf[(size_t)i];
+    const auto nrm = tube_jcp_normals(t);
+    phi.assign((size_t)D, 0.0);
+    for (int pass = 0; pass < 3; ++pass) {
+        if (pass > 0) {
+            for (i64 i = 0; i < N; ++i) u2[(size_t)i] = phi[(size_t)i];
+            E.matvec(u2.data(), u.data());
+        }
+        geodesic_div(t, E, nrm, u, div);
+        iters[1 + pass] = bicgstab(S3, div.data(), phi.data(), 1e-10, 100000).iters;
+        double mn = std::numeric_limits<double>::max();
+        for (double v : phi) mn = std::min(mn, v);
+        for (double& v : phi) v -= mn;
+    }
+    return iters;
+}
+}  // namespace orc

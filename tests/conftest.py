@@ -21,3 +21,12 @@ def gpu_ctx():
     ctx = cpmgpu.Context(0)
     yield ctx
     ctx.close()
+
+
+@pytest.fixture
+def parallel_oracle():
+    """Host threads for the long oracle runs (results do not depend on them)."""
+    import orc
+    orc.set_parallel(True, True, min(8, os.cpu_count() or 1))
+    yield
+    orc.set_parallel(False, False, 0)

@@ -10,14 +10,6 @@ import pytest
 import orc
 
 
-@pytest.fixture
-def parallel_oracle():
-    """Host threads for the long studies (results do not depend on them)."""
-    orc.set_parallel(True, True, min(8, os.cpu_count() or 1))
-    yield
-    orc.set_parallel(False, False, 0)
-
-
 def test_tube_radius_and_grid_closed_forms():
     # Tube.cpp:136 with m_order = 2: r_t = 1.0001 sqrt((dim-1)((p+1)/2)^2 + ((p+3)/2)^2) h
     P = orc.Problem("Sphere", 0.05)
@@ -329,3 +321,29 @@ def test_mesh_closest_point_box_closed_form():
     exp = x[~out].copy()
     exp[np.arange(len(exp)), k] = np.sign(exp[np.arange(len(exp)), k])
     assert np.abs(cp[~out][uniq] - exp[uniq]).max() < 1e-15
+
+
+def _geodesic_sphere_point_error(dx):
+    P = orc.IBCProblem("Sphere", dx, ibcs=[orc.IBC("Point", params=[0.0, 0.0, 1.0], boundary_order=1,
+                                                   on_ibc_tolerance=0.1 * dx * dx)])
+    phi, it = P.geodesic(dx * dx, 2000)
+    up = P.plot_E().matvec(phi)
+    th = np.repeat(np.arange(200) * (2 * np.pi / 199), 200)
+    ph = np.tile(np.arange(200) * (np.pi / 199), 200)
+    exact = np.arccos(np.clip(np.cos(ph), -1, 1))          # distance from the north pole on the unit sphere
+    e = np.abs(up - exact)
+    return phi, it, e.max(), e.mean()
+
+
+def test_geodesic_distance_sphere_point(parallel_oracle):
+    """GeodesicDistance::ComputeDistance (GeodesicDistance.cpp:11-67, dt = dx^2, 2000 Jacobi
+    sweeps as in examples/GeodesicDistanceDzuikPoint) from a point on the unit sphere against
+    the exact distance acos(z) at the plot points: phi >= 0 with minimum 0, errors of the
+    heat method's size that shrink with dx (a wrong sign, a missing normal projection or
+    extension gives O(1) errors)."""
+    phi2, it2, emax2, emean2 = _geodesic_sphere_point_error(0.2)
+    phi1, it1, emax1, emean1 = _geodesic_sphere_point_error(0.1)
+    assert phi1.min() == 0.0 and phi2.min() == 0.0
+    assert it1[0] >= 1 and all(k > 0 for k in it1[1:])
+    assert emax1 < 0.1 and emean1 < 0.035, (emax1, emean1)
+    assert emean1 < 0.85 * emean2 and emax1 < 0.8 * emax2, (emean1, emean2, emax1, emax2)

@@ -25,6 +25,7 @@ struct IbcData {
     Buffer t_sub;               // int32 [N]: tagging index of a tube node or -1
     Buffer t_set, t_bdy;        // int32 [nt]
     Buffer t_cpc, t_cpdiff, t_dir, t_dist;   // double [nt*3] (dist [nt])
+    Buffer t_bn;                // double [nt*3]: binormal t x n (0 for point curves)
     Buffer d_sub;               // int32 [N]: DOF index of a tube node or -1
     Buffer d_set;               // int32 [nd]
     Buffer d_cpc, d_dir;        // double [nd*3]
@@ -57,6 +58,9 @@ struct Problem {
 
 Problem* problem_build(Tube* tube, const std::vector<IbcSpec>& specs);
 Op* op_from_problem(Problem* P, double m, double n, int precond, int precond_iters);
+// Plotting rows at nq host points: tube interpolation modified by the curves
+// (AddIBCPlotInterpolation), nq x D CSR with (p+1)^3 entries per row.
+void problem_plot_matrix(Problem* P, const double* xq, int64_t nq, int32_t* rowptr, int32_t* col, double* val);
 // GeodesicDistance::ComputeDistance up to its plotting interpolation: phi on
 // the D rows into d_phi; infos[0] the heat solve, infos[1..3] the Poisson solves.
 void geodesic_distance(Problem* P, double dt, int sweeps, const cpm_solve_opts& o, double* d_phi,

@@ -292,7 +292,7 @@ __global__ void k_onibc_fix(const int32_t* list, int64_t nl, const int32_t* t_se
 // (IBCSubset.cpp:359-383).  ecol/eval: E rows at cp_c (tube columns).
 __global__ void k_frame(int64_t nt, const int* ecol, const double* eval, int ew, int64_t ld, const double* tube_normals,
                         const int32_t* t_set, const int32_t* t_sub, const int32_t* nbr, const double* t_cpc,
-                        const double* cpdiff, double dx, double* gsub, int pass, double* dir) {
+                        const double* cpdiff, double dx, double* gsub, int pass, double* dir, double* bn) {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= nt) return;
     if (pass == 0) {   // subset-restricted gradients of cp_C
@@ -340,12 +340,18 @@ __global__ void k_frame(int64_t nt, const int* ecol, const double* eval, int ew,
     const double* d = cpdiff + 3 * i;
     if (fabs(lam) < 1e-8) {
         const double sdn = dot3(n, d);
-        for (int k = 0; k < 3; ++k) dir[3 * i + k] = d[k] - sdn * n[k];
+        for (int k = 0; k < 3; ++k) {
+            dir[3 * i + k] = d[k] - sdn * n[k];
+            bn[3 * i + k] = 0.0;
+        }
     } else {
         double b[3];
         cross3(t, n, b);
         const double sb = dot3(b, d);
-        for (int k = 0; k < 3; ++k) dir[3 * i + k] = sb * b[k];
+        for (int k = 0; k < 3; ++k) {
+            dir[3 * i + k] = sb * b[k];
+            bn[3 * i + k] = b[k];
+        }
     }
 }
 
@@ -724,12 +730,13 @@ static void tagging_subset(Tube* t, IbcData& I, const double* tube_normals, cons
     build_E_ell(*t, I.t_cpc.as<double>(), nt, ec, ev, nullptr, nt, st);
     double* gsub = ctx->workspace("ibc_gsub", (size_t)9 * nt * 8);
     realloc<double>(I.t_dir, nt * 3);
+    realloc<double>(I.t_bn, nt * 3);
     hipLaunchKernelGGL(k_frame, dim3(nblk(nt)), dim3(256), 0, st, nt, ec, ev, W, nt, tube_normals, I.t_set.as<int32_t>(),
                        I.t_sub.as<int32_t>(), t->nbr.as<int32_t>(), I.t_cpc.as<double>(), I.t_cpdiff.as<double>(), t->dx,
-                       gsub, 0, I.t_dir.as<double>());
+                       gsub, 0, I.t_dir.as<double>(), I.t_bn.as<double>());
     hipLaunchKernelGGL(k_frame, dim3(nblk(nt)), dim3(256), 0, st, nt, ec, ev, W, nt, tube_normals, I.t_set.as<int32_t>(),
                        I.t_sub.as<int32_t>(), t->nbr.as<int32_t>(), I.t_cpc.as<double>(), I.t_cpdiff.as<double>(), t->dx,
-                       gsub, 1, I.t_dir.as<double>());
+                       gsub, 1, I.t_dir.as<double>(), I.t_bn.as<double>());
     HIPCHK(hipStreamSynchronize(st));
 }
 
@@ -955,6 +962,157 @@ Op* op_from_problem(Problem* P, double m, double n, int precond, int precond_ite
     op->precond = precond;
     op->precond_iters = precond_iters;
     return op.release();
+}
+
+// ---------------------------------------------------------------- plotting rows
+// Per-curve tables of the plotting interpolation (device array).
+struct PlotIbc {
+    SurfDev sd;
+    const int32_t* t_sub;
+    const double* t_cpdiff;
+    const double* t_bn;
+    double tol;
+};
+
+// AddIBCPlotInterpolation, not oriented (IBCMatrixManipulations.cpp:74-106
+// with the plot points' directions of PlotPointsCPDiff :254-416): for every
+// curve whose distance from the plot point is below 2 r_t, cp_diff = x - cp_c
+// (the nearest tube node's tagging cp_diff when x lies on the curve), the
+// binormal interpolated without orientation from the tagging subset at cp_c
+// (sign-aligned to the entry of smallest subset index, normalised), and the
+// direction (b . cp_diff) b; stencil entries then move to ghost columns as
+// for the tube rows.  Columns ascending in the output row.
+__global__ void k_plot_rows(const PlotIbc* pc, const IbcDev* ib, int nibc, GridDev g, int p, const int32_t* gidx,
+                            int64_t nq, const double* xq, const int* ecol, const double* eval, int W, int* col_out,
+                            double* val_out, int* err) {
+    const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= nq) return;
+    const double* x = xq + 3 * q;
+    double dirs[kMaxIbc][3];
+    const double* rowdir[kMaxIbc];
+    const int S = p + 1;
+    for (int c = 0; c < nibc; ++c) {
+        rowdir[c] = nullptr;
+        double cc[3], dist;
+        int bd;
+        cp_surface(pc[c].sd, x, cc, dist, bd);
+        double diff[3] = {x[0] - cc[0], x[1] - cc[1], x[2] - cc[2]};
+        const double nd = sqrt(dot3(diff, diff));
+        if (!(nd < 2.0 * g.r_t)) continue;
+        if (nd < pc[c].tol) {   // Tube.h:105-110 nearest node
+            int64_t t[3];
+            bool in = true;
+            for (int d = 0; d < 3; ++d) {
+                t[d] = (int64_t)rint((x[d] - g.xstart) / g.dx);
+                in = in && t[d] >= 0 && t[d] < g.N1d;
+            }
+            if (in) {
+                const int32_t ni = gidx[(t[0] * g.N1d + t[1]) * g.N1d + t[2]];
+                const int32_t ts = (ni >= 0 && ni < kCand) ? pc[c].t_sub[ni] : -1;
+                if (ts >= 0)
+                    for (int k = 0; k < 3; ++k) diff[k] = pc[c].t_cpdiff[3 * (int64_t)ts + k];
+            }
+        }
+        double bn[3] = {0.0, 0.0, 0.0};
+        int64_t b[3];
+        double w[3][5];
+        if (stencil_weights(g, p, cc, b, w)) {
+            int32_t first = 0x7fffffff;
+            for (int i = 0; i < S; ++i)
+                for (int j = 0; j < S; ++j)
+                    for (int k = 0; k < S; ++k) {
+                        const int32_t gv = gidx[((b[0] + i) * g.N1d + (b[1] + j)) * g.N1d + (b[2] + k)];
+                        if (gv < 0 || gv >= kCand) continue;
+                        const int32_t ts = pc[c].t_sub[gv];
+                        if (ts >= 0 && ts < first) first = ts;
+                    }
+            if (first != 0x7fffffff) {
+                const double* f0 = pc[c].t_bn + 3 * (int64_t)first;
+                for (int i = 0; i < S; ++i)
+                    for (int j = 0; j < S; ++j)
+                        for (int k = 0; k < S; ++k) {
+                            const int32_t gv = gidx[((b[0] + i) * g.N1d + (b[1] + j)) * g.N1d + (b[2] + k)];
+                            if (gv < 0 || gv >= kCand) continue;
+                            const int32_t ts = pc[c].t_sub[gv];
+                            if (ts < 0) continue;
+                            const double* v = pc[c].t_bn + 3 * (int64_t)ts;
+                            const double wv = w[2][k] * w[1][j] * w[0][i];
+                            const double sg = dot3(f0, v) > 0.0 ? 1.0 : -1.0;
+                            for (int d = 0; d < 3; ++d) bn[d] += sg * (wv * v[d]);
+                        }
+                const double nb = sqrt(dot3(bn, bn));
+                if (nb > 0.0)
+                    for (int d = 0; d < 3; ++d) bn[d] /= nb;
+            }
+        }
+        const double sb = dot3(bn, diff);
+        for (int d = 0; d < 3; ++d) dirs[c][d] = sb * bn[d];
+        rowdir[c] = dirs[c];
+    }
+    int cols[64];
+    double vals[64];
+    for (int s = 0; s < W; ++s) {
+        const int64_t col = ecol[(int64_t)s * nq + q];
+        int nm;
+        const int64_t nc = moved_col(ib, nibc, rowdir, col, 1.0, &nm);
+        if (nm > 1) atomicOr(err, 1);
+        const double v = eval[(int64_t)s * nq + q];
+        int k = s;   // insertion by column
+        while (k > 0 && cols[k - 1] > (int)nc) {
+            cols[k] = cols[k - 1];
+            vals[k] = vals[k - 1];
+            --k;
+        }
+        cols[k] = (int)nc;
+        vals[k] = v;
+    }
+    for (int s = 0; s < W; ++s) {
+        col_out[q * W + s] = cols[s];
+        val_out[q * W + s] = vals[s];
+    }
+}
+
+void problem_plot_matrix(Problem* P, const double* xq, int64_t nq, int32_t* rowptr, int32_t* col, double* val) {
+    Tube* t = P->tube;
+    Ctx* ctx = P->ctx;
+    const hipStream_t st = ctx->stream;
+    const int S = t->p + 1, W = S * S * S;
+    if (W > 64) throw ArgError("plotting rows support interpolation degree <= 3");
+    const int nibc = (int)P->ibc.size();
+    std::vector<PlotIbc> hp((size_t)nibc);
+    std::vector<IbcDev> hi((size_t)nibc);
+    for (int c = 0; c < nibc; ++c) {
+        IbcData& I = P->ibc[(size_t)c];
+        hp[(size_t)c] = PlotIbc{I.sd, I.t_sub.as<int32_t>(), I.t_cpdiff.as<double>(), I.t_bn.as<double>(), I.spec.tol};
+        hi[(size_t)c] = IbcDev{I.t_sub.as<int32_t>(), I.t_bdy.as<int32_t>(), I.t_dir.as<double>(),
+                               I.d_sub.as<int32_t>(), I.d_dir.as<double>(), I.start};
+    }
+    const size_t tb = (sizeof(PlotIbc) * nibc + 255) / 256 * 256, ib_bytes = (sizeof(IbcDev) * nibc + 255) / 256 * 256;
+    char* w = reinterpret_cast<char*>(ctx->workspace("plot_tables", tb + ib_bytes + 256));
+    auto* dpc = reinterpret_cast<PlotIbc*>(w);
+    auto* dib = reinterpret_cast<IbcDev*>(w + tb);
+    int* err = reinterpret_cast<int*>(w + tb + ib_bytes);
+    HIPCHK(hipMemcpyAsync(dpc, hp.data(), sizeof(PlotIbc) * nibc, hipMemcpyHostToDevice, st));
+    HIPCHK(hipMemcpyAsync(dib, hi.data(), sizeof(IbcDev) * nibc, hipMemcpyHostToDevice, st));
+    HIPCHK(hipMemsetAsync(err, 0, 4, st));
+    double* dxq = ctx->workspace("plot_xq", (size_t)nq * 24);
+    HIPCHK(hipMemcpyAsync(dxq, xq, (size_t)nq * 24, hipMemcpyHostToDevice, st));
+    int* ec = reinterpret_cast<int*>(ctx->workspace("plot_ec", (size_t)W * nq * 4));
+    double* ev = ctx->workspace("plot_ev", (size_t)W * nq * 8);
+    build_E_ell(*t, dxq, nq, ec, ev, nullptr, nq, st);
+    int* oc = reinterpret_cast<int*>(ctx->workspace("plot_oc", (size_t)W * nq * 4));
+    double* ov = ctx->workspace("plot_ov", (size_t)W * nq * 8);
+    GridDev g{t->N1d, t->xstart, t->dx, t->r_t};
+    hipLaunchKernelGGL(k_plot_rows, dim3(nblk(nq)), dim3(256), 0, st, dpc, dib, nibc, g, t->p, t->gidx.as<int32_t>(),
+                       nq, dxq, ec, ev, W, oc, ov, err);
+    HIPCHK(hipGetLastError());
+    int h = 0;
+    HIPCHK(hipMemcpyAsync(&h, err, 4, hipMemcpyDeviceToHost, st));
+    HIPCHK(hipMemcpyAsync(col, oc, (size_t)W * nq * 4, hipMemcpyDeviceToHost, st));
+    HIPCHK(hipMemcpyAsync(val, ov, (size_t)W * nq * 8, hipMemcpyDeviceToHost, st));
+    HIPCHK(hipStreamSynchronize(st));
+    if (h & 1) throw GeomError("a plotting stencil entry crosses more than one IBC (curves too close together)");
+    for (int64_t r = 0; r <= nq; ++r) rowptr[r] = (int32_t)(r * W);
 }
 
 }  // namespace cpmgpu

@@ -321,6 +321,21 @@ int cpm_op_create_from_problem(cpm_problem* pp, double m, double n, int precond,
     });
 }
 
+int cpm_problem_plot_matrix(cpm_problem* prob, const double* xq, int64_t nq, int32_t* rowptr, int32_t* col,
+                            double* val) {
+    return guard([&] {
+        if (!prob || (nq > 0 && (!xq || !rowptr || !col || !val))) throw ArgError("null argument");
+        if (nq < 0) throw ArgError("nq must be >= 0");
+        Problem* P = reinterpret_cast<Problem*>(prob);
+        HIPCHK(hipSetDevice(P->ctx->device));
+        if (nq == 0) {
+            if (rowptr) rowptr[0] = 0;
+            return;
+        }
+        problem_plot_matrix(P, xq, nq, rowptr, col, val);
+    });
+}
+
 int cpm_geodesic_distance(cpm_problem* prob, double dt, int jacobi_sweeps, const cpm_solve_opts* opts, double* d_phi,
                           cpm_solve_info* infos) {
     return guard([&] {

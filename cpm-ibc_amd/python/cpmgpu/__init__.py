@@ -25,7 +25,7 @@ EXPORTS = [
     "cpm_memcpy_d2h", "cpm_tube_create", "cpm_tube_create_mesh", "cpm_tube_destroy", "cpm_tube_get_info", "cpm_tube_export",
     "cpm_tube_export_E", "cpm_tube_export_L", "cpm_tube_interp_matrix", "cpm_op_create_from_tube", "cpm_profile_egather", "cpm_profile_copy", "cpm_heat_step_device", "cpm_rd_params_default", "cpm_rd_step_device", "cpm_problem_create", "cpm_problem_destroy",
     "cpm_problem_get_info", "cpm_problem_rhs_points", "cpm_problem_subset", "cpm_problem_export",
-    "cpm_op_create_from_problem", "cpm_geodesic_distance",
+    "cpm_op_create_from_problem", "cpm_geodesic_distance", "cpm_problem_plot_matrix",
 ]
 
 
@@ -139,6 +139,7 @@ def load(path=LIB_PATH):
     L.cpm_rd_params_default.argtypes = [ctypes.c_double, ctypes.POINTER(RdParams)]
     L.cpm_rd_step_device.argtypes = [vp, ctypes.POINTER(RdParams), ctypes.c_int, ctypes.POINTER(vp),
                                      ctypes.POINTER(vp), vp, vp, ctypes.POINTER(SolveOpts), ctypes.POINTER(SolveInfo)]
+    L.cpm_problem_plot_matrix.argtypes = [vp, vp, ctypes.c_int64, vp, vp, vp]
     L.cpm_geodesic_distance.argtypes = [vp, ctypes.c_double, ctypes.c_int, ctypes.POINTER(SolveOpts), vp,
                                         ctypes.POINTER(SolveInfo)]
     L.cpm_profile_copy.argtypes = [vp, ctypes.c_int64, ctypes.c_int, ctypes.POINTER(ctypes.c_double)]
@@ -514,6 +515,16 @@ class Problem:
         h = vp()
         _chk(load().cpm_op_create_from_problem(self._h, m, n, precond, precond_iters, ctypes.byref(h)))
         return Operator(self.tube.ctx, None, None, None, m, n, handle=h)
+
+    def plot_matrix(self, xq):
+        """Plotting rows at points xq (n x 3) with the curves' ghost columns (AddIBCPlotInterpolation)."""
+        xq = np.ascontiguousarray(xq, np.float64).reshape(-1, 3)
+        W = (self.tube.info["p"] + 1) ** 3
+        ptr = np.zeros(len(xq) + 1, np.int32)
+        col = np.zeros(len(xq) * W, np.int32)
+        val = np.zeros(len(xq) * W)
+        _chk(load().cpm_problem_plot_matrix(self._h, _ptr(xq), len(xq), _ptr(ptr), _ptr(col), _ptr(val)))
+        return CSR(ptr, col, val)
 
     def geodesic(self, dt, sweeps=2000, o=None):
         """GeodesicDistance::ComputeDistance before its plotting interpolation: phi on the D rows

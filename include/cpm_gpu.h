@@ -246,6 +246,15 @@ int cpm_problem_export(const cpm_problem* prob, int32_t* L_rowptr, int32_t* L_co
  * problem's Lfull, Efull, identity rows). */
 int cpm_op_create_from_problem(cpm_problem* prob, double m, double n, int precond, int precond_iters, cpm_op** out);
 
+/* Plotting rows of a problem: Interpolation(tube, xq) (as
+ * cpm_tube_interp_matrix) with the stencil entries across a curve moved to
+ * its ghost columns, IBCMatrixManipulations::AddIBCPlotInterpolation
+ * (src/IBCMatrixManipulations.cpp:74-106) with the plot points' directions of
+ * PlotPointsCPDiff (:254-416), curves not oriented.  nq x D CSR with (p+1)^3
+ * entries per row (p <= 3), ascending columns; rowptr has nq + 1 entries. */
+int cpm_problem_plot_matrix(cpm_problem* prob, const double* xq, int64_t nq, int32_t* rowptr, int32_t* col,
+                            double* val);
+
 /* Geodesic distance from the problem's curves by the heat method,
  * GeodesicDistance::ComputeDistance (problems/GeodesicDistance.cpp:11-67):
  * heat flow from the curves, Solver(1, -dt) with jacobi_sweeps Jacobi sweeps

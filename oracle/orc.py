@@ -31,6 +31,7 @@ def lib():
         L.orc_set_mesh.argtypes = [f64p, ctypes.c_int64, np.ctypeslib.ndpointer(np.int32, flags="C_CONTIGUOUS"),
                                    ctypes.c_int64]
         L.orc_mesh_cp.argtypes = [f64p, ctypes.c_int64, f64p, f64p]
+        L.orc_build_ibc_plot_E.argtypes = [ctypes.c_void_p, f64p, ctypes.c_int64]
         L.orc_geodesic.argtypes = [ctypes.c_void_p, ctypes.c_double, ctypes.c_int, f64p,
                                    np.ctypeslib.ndpointer(np.int32, flags="C_CONTIGUOUS")]
         L.orc_problem_poisson.restype = ctypes.c_void_p
@@ -290,6 +291,15 @@ class IBCProblem(Problem):
         d = np.zeros(n * 3)
         lib().orc_ibc_subset(self._h, c, which, s, cp, d)
         return s, cp.reshape(-1, 3), d.reshape(-1, 3)
+
+    def plot_matrix(self, xq):
+        """Plotting rows at xq (n x 3) with the curves' ghost columns (AddIBCPlotInterpolation)."""
+        xq = np.ascontiguousarray(xq, np.float64).reshape(-1, 3)
+        rows = lib().orc_build_ibc_plot_E(self._h, xq.ravel(), len(xq))
+        if rows < 0:
+            raise RuntimeError(_err())
+        self._info()
+        return self._csr(2, rows, self.nnz_extra)
 
     def geodesic(self, dt, sweeps=2000):
         """GeodesicDistance::ComputeDistance before its plotting interpolation: phi on the D rows

@@ -371,6 +371,22 @@ int orc_build_explicit(void* hp, double m, double n, int nD) {
 }
 
 // Plot interpolation matrix at the surface parameterisation points into "extra".
+// Plotting rows at nq points with the curves' ghost columns (AddIBCPlotInterpolation).
+int orc_build_ibc_plot_E(void* hp, const double* xq, int64_t nq) {
+    try {
+        Handle* h = static_cast<Handle*>(hp);
+        if (!h->has_ibc) throw std::runtime_error("not an IBC problem");
+        std::vector<Vec> xp((size_t)nq, Vec(3));
+        for (int64_t i = 0; i < nq; ++i)
+            for (int d = 0; d < 3; ++d) xp[(size_t)i][(size_t)d] = xq[3 * i + d];
+        h->extra = ibc_plot_E_at(h->P.tube, h->ibc, xp);
+        return (int)h->extra.rows;
+    } catch (const std::exception& e) {
+        g_err = e.what();
+        return -1;
+    }
+}
+
 int orc_build_plot_E(void* hp) {
     try {
         Handle* h = static_cast<Handle*>(hp);

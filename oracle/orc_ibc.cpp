@@ -484,11 +484,11 @@ Vec poisson_ibc_rhs(const PoissonIBCProblem& P, const std::function<double(const
 
 // GetPlotInterpMatrix with AddIBCPlotInterpolation (not oriented),
 // IBCMatrixManipulations.cpp:74-106 + PlotPointsCPDiff :254-416.
-CSR ibc_plot_E(const PoissonIBCProblem& P) {
-    const Tube& t = P.tube;
-    const IBCData& I = P.ibc;
-    CSR A = plot_E(t);
-    const auto& xp = t.surf.xp;
+CSR ibc_plot_E(const PoissonIBCProblem& P) { return ibc_plot_E_at(P.tube, P.ibc, P.tube.surf.xp); }
+
+// The same rows at arbitrary points xp (GetInterpMatrix(xp), problems/HeatWithIBC.cpp:117-128).
+CSR ibc_plot_E_at(const Tube& t, const IBCData& I, const std::vector<Vec>& xp) {
+    CSR A = build_E(t, xp);
     const size_t C = I.dof.size();
     std::vector<std::vector<uint8_t>> tags(C);
     for (size_t c = 0; c < C; ++c) {

@@ -51,5 +51,6 @@ PoissonIBCProblem make_poisson_ibc(const SurfaceSpec& s, const std::vector<Surfa
 Vec poisson_ibc_rhs(const PoissonIBCProblem& P, const std::function<double(const double*)>& f,
                     const std::function<double(const double*)>& exact);
 CSR ibc_plot_E(const PoissonIBCProblem& P);
+CSR ibc_plot_E_at(const Tube& t, const IBCData& I, const std::vector<Vec>& xp);
 
 }  // namespace orc

@@ -175,3 +175,33 @@ def test_tensor_product_format(gpu_ctx, monkeypatch, kw):
         x2, i2 = op_ell.solve(b, o)
         assert i1["status"] == i2["status"] == "converged"
         assert np.linalg.norm(x1 - x2) <= 1e-8 * np.linalg.norm(x2)
+
+
+@pytest.mark.parametrize("kw", [
+    dict(type="Circle3D", params=circle3d_params(3), boundary_order=2, on_ibc_tolerance=0.1 * 0.1 ** 2),
+    dict(type="Circle3D", params=circle3d_params(5), boundary_order=1, boundary_type=(1, 1),
+         on_ibc_tolerance=0.1 * 0.1 ** 2),
+    dict(type="Point", params=[0.0, 0.6, 0.8], boundary_order=1, on_ibc_tolerance=0.1 * 0.1 ** 2),
+])
+def test_plot_matrix_matches_oracle(gpu_ctx, kw):
+    """Plotting rows with the curves' ghost columns (AddIBCPlotInterpolation,
+    IBCMatrixManipulations.cpp:74-106,254-416) at the sphere's 200 x 200
+    parameter points (Surface.cpp:1973-1999), GPU vs oracle."""
+    print(f"case plot matrix {kw['type']} order={kw['boundary_order']}", flush=True)
+    dx = 0.1
+    P = orc.IBCProblem("Sphere", dx, ibcs=[orc.IBC(**kw)])
+    tg = cpmgpu.Tube(gpu_ctx, "Sphere", dx)
+    Q = cpmgpu.Problem(tg, [cpmgpu.IBC(**kw)])
+    th = np.repeat(np.arange(200) * (2 * np.pi / 199), 200)
+    ph = np.tile(np.arange(200) * (np.pi / 199), 200)
+    xp = np.stack([np.sin(ph) * np.cos(th), np.sin(ph) * np.sin(th), np.cos(ph)], 1)
+    Ao = P.plot_matrix(xp)
+    Ag = Q.plot_matrix(xp)
+    assert np.array_equal(Ag.ptr, Ao.ptr)
+    moved_o = (Ao.col >= P.N).reshape(len(xp), -1).any(1)
+    moved_g = (Ag.col >= P.N).reshape(len(xp), -1).any(1)
+    print(f"rows with ghost columns: oracle={moved_o.sum()} gpu={moved_g.sum()}", flush=True)
+    assert moved_o.sum() > 0 or kw["type"] == "Point"
+    D = P.D
+    x = np.concatenate([P.tube()["cp"] @ np.array([1.0, -2.0, 0.5]), np.linspace(-1, 1, D - P.N)])
+    _operator_parity(Ag, Ao, x, what="plot rows")

@@ -327,9 +327,10 @@ def _geodesic_sphere_point_error(dx):
     P = orc.IBCProblem("Sphere", dx, ibcs=[orc.IBC("Point", params=[0.0, 0.0, 1.0], boundary_order=1,
                                                    on_ibc_tolerance=0.1 * dx * dx)])
     phi, it = P.geodesic(dx * dx, 2000)
-    up = P.plot_E().matvec(phi)
     th = np.repeat(np.arange(200) * (2 * np.pi / 199), 200)
     ph = np.tile(np.arange(200) * (np.pi / 199), 200)
+    xp = np.stack([np.sin(ph) * np.cos(th), np.sin(ph) * np.sin(th), np.cos(ph)], 1)
+    up = P.plot_matrix(xp).matvec(phi)                      # plotEfull * phi (GeodesicDistance.cpp:62-66)
     exact = np.arccos(np.clip(np.cos(ph), -1, 1))          # distance from the north pole on the unit sphere
     e = np.abs(up - exact)
     return phi, it, e.max(), e.mean()

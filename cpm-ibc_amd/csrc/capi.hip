@@ -134,6 +134,11 @@ static OpView<T> make_view(const Op& o, const Buffer& ev, const Buffer& lv, cons
     v.xval = xv.as<T>();
     v.loff = (T)o.loff;
     v.ldu = (T)o.ldu;
+    v.tln = o.tln.as<int>();
+    v.tlo = o.tlo.as<int>();
+    v.tlr = o.tlr.as<int>();
+    v.tpl = o.tpl.as<int16_t>();
+    v.tlx = o.tlx.as<int16_t>();
     return v;
 }
 template <>
@@ -144,7 +149,8 @@ OpView<float> Op::view<float>() const { return make_view<float>(*this, eval32, l
 size_t Op::device_bytes() const {
     return ecol.bytes + eval64.bytes + eval32.bytes + lcol.bytes + lval64.bytes + lval32.bytes + ldiag64.bytes +
            ldiag32.bytes + flag.bytes + perm.bytes + tpc.bytes + tpw64.bytes + tpw32.bytes + exc.bytes + ecx.bytes +
-           xcol.bytes + xval64.bytes + xval32.bytes + hrow.bytes + hcol.bytes + hval.bytes;
+           xcol.bytes + xval64.bytes + xval32.bytes + hrow.bytes + hcol.bytes + hval.bytes + tln.bytes + tlo.bytes +
+           tlr.bytes + tpl.bytes + tlx.bytes;
 }
 
 Op* op_from_csr(Ctx* ctx, int64_t D, int nD, const int32_t* Lp, const int32_t* Lc, const double* Lv,

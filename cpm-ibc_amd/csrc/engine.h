@@ -71,7 +71,18 @@ struct OpView {
     const int* xcol;     // [nexc] extra column after the stencil, -1 = none
     const T* xval;       // [nexc]
     T loff, ldu;
+    // LDS tiles of the tensor-product format (kBlock rows each): the sorted
+    // distinct x rows the tile's stencil lines read, gathered into LDS once.
+    const int* tln;      // [ntiles] rows in the tile's list, -1 = the tile reads x directly
+    const int* tlo;      // [ntiles] offset of the tile's list in tlr
+    const int* tlr;      // lists of x rows
+    const int16_t* tpl;  // [16][D] position of line (i, j)'s first row in the tile's list
+    const int16_t* tlx;  // [nexc][kExcPos] positions of an exception row's 64 columns and extra entry
 };
+
+constexpr int kLdsRows = 2560;   // longest LDS tile list
+constexpr int kLdsBytes = 81920; // LDS of one E-gather workgroup (gfx950: 160 KB per CU, two workgroups)
+constexpr int kExcPos = 72;      // positions per exception record (64 slots, extra entry, padding)
 
 // One instance group of a batched launch: an operator, NR right-hand sides
 // and the contiguous block range that covers its D rows.

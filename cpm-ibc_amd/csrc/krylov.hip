@@ -241,6 +241,90 @@ __device__ __forceinline__ void egather_tp(const OpView<T>& op, int64_t row, con
     }
 }
 
+// LDS rows per tile for T x NR values per row (0: the list would not fit in
+// kLdsBytes of LDS and every tile reads x directly).
+template <typename T, int NR>
+struct LdsTile {
+    static constexpr int kRows = (kLdsRows * NR * (int)sizeof(T) <= kLdsBytes) ? kLdsRows : 0;
+};
+
+// A row's tensor-product metadata, loaded before the tile's x is in LDS so
+// that its latency overlaps the fill.
+template <typename T>
+struct TpRow {
+    T w[12];      // wx[4], wy[4], wz[4]
+    int p[16];    // position of each line's first x row in the tile's list
+};
+
+template <typename T>
+__device__ __forceinline__ void tp_row_load(const OpView<T>& op, int64_t row, TpRow<T>& m) {
+    const int64_t D = op.D;
+#pragma unroll
+    for (int q = 0; q < 12; ++q) m.w[q] = ldg(&op.tpw[q * D + row]);
+#pragma unroll
+    for (int l = 0; l < 16; ++l) m.p[l] = ldg(&op.tpl[l * D + row]);
+}
+
+// E row of the tensor-product format from the tile's LDS copy of x: the same
+// products in the same order as egather_tp_lines, so both paths agree to the
+// bit.  lx[p] holds the x row at position p of the tile's list.
+template <typename T, typename V, int W>
+__device__ __forceinline__ void egather_tp_lds(const TpRow<T>& m, const V* lx, V& acc) {
+#pragma unroll
+    for (int l = 0; l < 16; ++l) {
+        const T wx = m.w[l >> 2], wy = m.w[4 + (l & 3)];
+        const V* q = lx + m.p[l];
+        const V x0 = q[0], x1 = q[1], x2 = q[2], x3 = q[3];
+        acc += ((m.w[8] * wy) * wx) * x0;
+        acc += ((m.w[9] * wy) * wx) * x1;
+        acc += ((m.w[10] * wy) * wx) * x2;
+        acc += ((m.w[11] * wy) * wx) * x3;
+    }
+}
+
+// An exception row's LDS positions (kExcPos int16 packed in pairs).
+struct ExcRow {
+    int pk[kExcPos / 2];
+    __device__ __forceinline__ int pos(int s) const { return (int)(int16_t)(pk[s >> 1] >> (16 * (s & 1))); }
+};
+
+template <typename T>
+__device__ __forceinline__ void exc_row_load(const OpView<T>& op, int e, ExcRow& m) {
+    typedef int I4 __attribute__((ext_vector_type(4)));
+    const I4* src = reinterpret_cast<const I4*>(op.tlx + (size_t)e * kExcPos);
+#pragma unroll
+    for (int q = 0; q < kExcPos / 8; ++q) {
+        const I4 v = ldg(src + q);
+        m.pk[4 * q] = v.x;
+        m.pk[4 * q + 1] = v.y;
+        m.pk[4 * q + 2] = v.z;
+        m.pk[4 * q + 3] = v.w;
+    }
+}
+
+// Exception row from LDS: the slots and the extra entry of egather_tp's
+// exception path, in its order.
+template <typename T, typename V, int W>
+__device__ __forceinline__ void egather_tp_lds_exc(const OpView<T>& op, int64_t row, int e, const ExcRow& m,
+                                                   const V* lx, V& acc) {
+    const int64_t D = op.D;
+    T wz[4];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) wz[q] = ldg(&op.tpw[(8 + q) * D + row]);
+#pragma unroll
+    for (int l = 0; l < 16; ++l) {
+        const T wx = ldg(&op.tpw[(l >> 2) * D + row]);
+        const T wy = ldg(&op.tpw[(4 + (l & 3)) * D + row]);
+        const V x0 = lx[m.pos(4 * l)], x1 = lx[m.pos(4 * l + 1)], x2 = lx[m.pos(4 * l + 2)], x3 = lx[m.pos(4 * l + 3)];
+        acc += ((wz[0] * wy) * wx) * x0;
+        acc += ((wz[1] * wy) * wx) * x1;
+        acc += ((wz[2] * wy) * wx) * x2;
+        acc += ((wz[3] * wy) * wx) * x3;
+    }
+    const int xp = m.pos(64);
+    if (xp >= 0) acc += ldg(&op.xval[e]) * lx[xp];
+}
+
 // E row of the ELL format (eight slots in flight).
 template <typename T, typename V, int W>
 __device__ __forceinline__ void egather_ell(const OpView<T>& op, int64_t row, const T* xb, int stride, V& acc) {
@@ -368,16 +452,63 @@ __global__ __launch_bounds__(kBlock) void k_pupdate(const Seg<T>* __restrict__ s
 
 // tmp = E x.  Rows are independent; a thread owns one row and all NR rhs.
 // UPDATE lanes (mixed, first gather of the iteration): tmp64 = E64 x64.
+// E x for the instance segments.  Tiles with an LDS list copy their x rows
+// into LDS once and gather every stencil line from there; the other tiles
+// (lists longer than the LDS, or operators without tiles) read x directly.
+// The FP64 gathers of refinement lanes (FP32 segments) always read directly.
 template <typename T, int NR>
 __global__ __launch_bounds__(kBlock) void k_egather(const Seg<T>* __restrict__ segs, const int* __restrict__ bseg, const T* x, T* out,
                                                     unsigned lanes, MixVec mv) {
+    typedef typename VT<T, NR>::type V;
+    constexpr int LR = LdsTile<T, NR>::kRows;
+    __shared__ V lx[LR > 0 ? LR : 1];
     int lb;
     int64_t row;
     const Seg<T>& sg = seg_of(segs, bseg, lb, row);
-    if (row >= sg.op.D) return;
-    if (lanes && !seg_any(sg, lanes)) return;
+    if (lanes && !seg_any(sg, lanes)) return;               // uniform per workgroup
+    const bool main = !lanes || seg_any(sg, lanes & ~M_UPDATE);
     const size_t i = (size_t)(sg.off + row * NR);
-    if (!lanes || seg_any(sg, lanes & ~M_UPDATE)) vstore<T, NR>(out + i, erow<T, NR>(sg.op, row, x + sg.off, NR));
+    const T* xb = x + sg.off;
+    const int nu = (LR > 0 && main && sg.op.fmt == OP_TP && sg.op.tln) ? ldg(&sg.op.tln[lb]) : -1;
+    if (nu >= 0 && nu <= LR) {
+        // the row's metadata, then the tile's x rows into LDS (all loads of
+        // the fill in flight together), then every stencil line from LDS
+        const bool act = row < sg.op.D;
+        const int e = act ? ldg(&sg.op.exc[row]) : -1;
+        const int kind = act ? ((ldg(&sg.op.flag[row]) & ROW_IDENTITY) ? 2 : (e < 0 ? 0 : 1)) : 3;
+        TpRow<T> m;
+        ExcRow mx;
+        if (kind == 0) tp_row_load(sg.op, row, m);
+        else if (kind == 1) exc_row_load(sg.op, e, mx);
+        const int* lst = sg.op.tlr + ldg(&sg.op.tlo[lb]);
+        constexpr int KF = LR > 0 ? (LR + kBlock - 1) / kBlock : 1;
+        int src[KF];
+#pragma unroll
+        for (int k = 0; k < KF; ++k) {
+            const int u = (int)threadIdx.x + k * kBlock;
+            src[k] = u < nu ? ldg(&lst[u]) : 0;
+        }
+        V buf[KF];
+#pragma unroll
+        for (int k = 0; k < KF; ++k)
+            if ((int)threadIdx.x + k * kBlock < nu) buf[k] = vload<T, NR>(xb + (size_t)src[k] * NR);
+#pragma unroll
+        for (int k = 0; k < KF; ++k) {
+            const int u = (int)threadIdx.x + k * kBlock;
+            if (u < nu) lx[u] = buf[k];
+        }
+        __syncthreads();
+        if (act) {
+            V acc = (V)(T)0;
+            if (kind == 2) acc = vload<T, NR>(xb + (size_t)row * NR);
+            else if (kind == 0) egather_tp_lds<T, V, NR>(m, lx, acc);
+            else egather_tp_lds_exc<T, V, NR>(sg.op, row, e, mx, lx, acc);   // rows an interior boundary moved
+            vstore<T, NR>(out + i, acc);
+        }
+    } else if (main && row < sg.op.D) {
+        vstore<T, NR>(out + i, erow<T, NR>(sg.op, row, xb, NR));
+    }
+    if (row >= sg.op.D) return;
     if (std::is_same<T, float>::value && (lanes & M_UPDATE) && seg_any(sg, M_UPDATE)) {
         for (int k = 0; k < sg.nr; ++k)
             if (lane_in(sg, k, M_UPDATE)) stg(&mv.tmp64[i + k], erow<double, 1>(sg.op64, row, mv.x64 + sg.off + k, NR));
@@ -814,6 +945,7 @@ void launch_reset_states(KState* st, int n, double tol2, int max_it, double delt
 
 // ---------------------------------------------------------------- launchers
 #define LAUNCH(kern, ...) hipLaunchKernelGGL(kern, dim3(c.nblocks), dim3(kBlock), 0, c.stream, c.segs, c.bseg, __VA_ARGS__)
+#define EGATHER(...) LAUNCH((k_egather<T, NR>), __VA_ARGS__)
 #define FIN(T, NR, KIND) hipLaunchKernelGGL((k_fin<T, NR, KIND>), dim3(c.nseg), dim3(kBlock), 0, c.stream, c.segs, c.nseg)
 
 template <typename T, int NR>
@@ -828,7 +960,7 @@ void Launch<T, NR>::iteration(const LaunchCfg<T>& c, Vecs<T>& w, bool mixed, con
         apply(c, w.y, w.tmp, w.t, M_RUN);
         LAUNCH((k_jacobi<T, NR>), w.p, w.t, w.y, sw);
     }
-    LAUNCH((k_egather<T, NR>), w.y, w.tmp, M_STAGE1, mv);
+    EGATHER(w.y, w.tmp, M_STAGE1, mv);
     LAUNCH((k_lfinish<T, NR, 1>), w.y, w.tmp, w.v, w.r0, (const T*)nullptr, M_STAGE1, mv);
     FIN(T, NR, FIN_ALPHA);
     LAUNCH((k_supdate<T, NR>), w.r, w.v, w.s, w.z, w.r0, w.p, mv);
@@ -836,7 +968,7 @@ void Launch<T, NR>::iteration(const LaunchCfg<T>& c, Vecs<T>& w, bool mixed, con
         apply(c, w.z, w.tmp, w.t, M_RUN);
         LAUNCH((k_jacobi<T, NR>), w.s, w.t, w.z, sw);
     }
-    LAUNCH((k_egather<T, NR>), w.z, w.tmp, M_RUN, mv);
+    EGATHER(w.z, w.tmp, M_RUN, mv);
     LAUNCH((k_lfinish<T, NR, 2>), w.z, w.tmp, w.t, w.s, (const T*)w.r, M_STAGE2, mv);
     FIN(T, NR, FIN_OMEGA);
     LAUNCH((k_xrupdate<T, NR>), w.x, w.y, w.z, w.r, w.s, w.t, w.r0);
@@ -846,7 +978,7 @@ void Launch<T, NR>::iteration(const LaunchCfg<T>& c, Vecs<T>& w, bool mixed, con
 template <typename T, int NR>
 void Launch<T, NR>::restart(const LaunchCfg<T>& c, Vecs<T>& w) {
     const MixVec none{};
-    LAUNCH((k_egather<T, NR>), w.x, w.tmp, M_RESTART, none);
+    EGATHER(w.x, w.tmp, M_RESTART, none);
     LAUNCH((k_lfinish<T, NR, 0>), w.x, w.tmp, w.t, (const T*)nullptr, (const T*)nullptr, M_RESTART, none);
     LAUNCH((k_restart<T, NR>), w.b, w.t, w.r, w.r0);
     FIN(T, NR, FIN_RESTART64);
@@ -854,7 +986,7 @@ void Launch<T, NR>::restart(const LaunchCfg<T>& c, Vecs<T>& w) {
 template <typename T, int NR>
 void Launch<T, NR>::apply(const LaunchCfg<T>& c, const T* x, T* tmp, T* y, unsigned lanes) {
     const MixVec none{};
-    LAUNCH((k_egather<T, NR>), x, tmp, lanes, none);
+    EGATHER(x, tmp, lanes, none);
     LAUNCH((k_lfinish<T, NR, 0>), x, tmp, y, (const T*)nullptr, (const T*)nullptr, lanes, none);
 }
 
@@ -868,7 +1000,7 @@ template void launch_lfinish_plain<double>(const LaunchCfg<double>&, const doubl
 template <typename T, int NR>
 void launch_egather(const LaunchCfg<T>& c, const T* x, T* out) {
     const MixVec none{};
-    LAUNCH((k_egather<T, NR>), x, out, 0u, none);
+    EGATHER(x, out, 0u, none);
 }
 
 template <int NR>

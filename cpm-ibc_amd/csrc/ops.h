@@ -46,6 +46,7 @@ struct Op {
     Buffer tpc, tpw64, tpw32, exc, ecx, xcol, xval64, xval32;
     int64_t nexc = 0, nmoved = 0;
     double loff = 0, ldu = 0;
+    Buffer tln, tlo, tlr, tpl, tlx;   // LDS tiles (OP_TP)
     // Lfull rows of the rows that solves treat as identity rows (internal
     // order, hw slots each, padded with weight 0): explicit time stepping
     // applies Lfull to every row (problems/HeatWithIBC.cpp:84-88).  Entries

@@ -19,6 +19,11 @@
 // keeps the ELL format.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+#include <cstdlib>
+#include <string>
+#include <vector>
+
 #include "tube.h"
 
 namespace cpmgpu {
@@ -232,6 +237,80 @@ static void ensure_lex(Tube& t) {
     t.rlex = std::move(rlex);
 }
 
+// CPMGPU_LDS_TILES=0 builds tensor-product operators without LDS tiles.
+static bool lds_tiles_disabled() {
+    const char* e = std::getenv("CPMGPU_LDS_TILES");
+    return e && std::string(e) == "0";
+}
+
+// LDS tiles: for every block of kBlock internal rows, the sorted distinct x
+// rows its E rows read, and for each stencil line the position of its first
+// row in that list (the line's four rows are consecutive integers, hence
+// consecutive in the list); rows with an exception record get the positions
+// of their 64 columns and of the extra entry instead (tlx, kExcPos per
+// record, -1 = no extra entry).  Tiles whose list would exceed kLdsRows keep
+// reading x directly (tln = -1).
+static void build_lds_tiles(Op& op, int64_t D, int64_t ne, const Buffer& tpc, const Buffer& exc, const Buffer& ecx,
+                            const Buffer& xcol) {
+    std::vector<int> hc((size_t)kS * kS * D), he((size_t)D), hx((size_t)std::max<int64_t>(ne, 1) * 64),
+        hxc((size_t)std::max<int64_t>(ne, 1));
+    HIPCHK(hipMemcpy(hc.data(), tpc.p, hc.size() * 4, hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(he.data(), exc.p, he.size() * 4, hipMemcpyDeviceToHost));
+    if (ne > 0) {
+        HIPCHK(hipMemcpy(hx.data(), ecx.p, (size_t)ne * 64 * 4, hipMemcpyDeviceToHost));
+        HIPCHK(hipMemcpy(hxc.data(), xcol.p, (size_t)ne * 4, hipMemcpyDeviceToHost));
+    }
+    const int64_t ntiles = (D + kBlock - 1) / kBlock;
+    std::vector<int> tln((size_t)ntiles), tlo((size_t)ntiles), tlr;
+    std::vector<int16_t> tpl((size_t)kS * kS * D, 0), tlx((size_t)std::max<int64_t>(ne, 1) * kExcPos, -1);
+    std::vector<int> u;
+    for (int64_t tile = 0; tile < ntiles; ++tile) {
+        const int64_t r0 = tile * kBlock, r1 = std::min<int64_t>(D, r0 + kBlock);
+        u.clear();
+        for (int64_t r = r0; r < r1; ++r) {
+            const int e = he[(size_t)r];
+            if (e >= 0) {
+                for (int s = 0; s < 64; ++s) u.push_back(hx[(size_t)e * 64 + s]);
+                if (hxc[(size_t)e] >= 0) u.push_back(hxc[(size_t)e]);
+                continue;
+            }
+            for (int l = 0; l < kS * kS; ++l) {
+                const int c0 = hc[(size_t)l * D + r];
+                for (int k = 0; k < kS; ++k) u.push_back(c0 + k);
+            }
+        }
+        std::sort(u.begin(), u.end());
+        u.erase(std::unique(u.begin(), u.end()), u.end());
+        tlo[(size_t)tile] = (int)tlr.size();
+        if ((int64_t)u.size() > kLdsRows) {
+            tln[(size_t)tile] = -1;
+            continue;
+        }
+        tln[(size_t)tile] = (int)u.size();
+        tlr.insert(tlr.end(), u.begin(), u.end());
+        auto pos = [&](int c) { return (int16_t)(std::lower_bound(u.begin(), u.end(), c) - u.begin()); };
+        for (int64_t r = r0; r < r1; ++r) {
+            const int e = he[(size_t)r];
+            if (e >= 0) {
+                for (int s = 0; s < 64; ++s) tlx[(size_t)e * kExcPos + s] = pos(hx[(size_t)e * 64 + s]);
+                if (hxc[(size_t)e] >= 0) tlx[(size_t)e * kExcPos + 64] = pos(hxc[(size_t)e]);
+                continue;
+            }
+            for (int l = 0; l < kS * kS; ++l) tpl[(size_t)l * D + r] = pos(hc[(size_t)l * D + r]);
+        }
+    }
+    if (tlr.empty()) tlr.push_back(0);
+    auto up = [](Buffer& b, const void* p, size_t bytes) {
+        b = Buffer(bytes);
+        HIPCHK(hipMemcpy(b.p, p, bytes, hipMemcpyHostToDevice));
+    };
+    up(op.tln, tln.data(), tln.size() * 4);
+    up(op.tlo, tlo.data(), tlo.size() * 4);
+    up(op.tlr, tlr.data(), tlr.size() * 4);
+    up(op.tpl, tpl.data(), tpl.size() * 2);
+    up(op.tlx, tlx.data(), tlx.size() * 2);
+}
+
 bool op_build_tp(Op& op, Tube& t, const TpSource& src) {
     if (t.p != 3 || t.dim != 3 || src.ew < kS * kS * kS || src.lw > 6 || op.nD != 0) return false;
     if (src.D > (int64_t)INT32_MAX / 2) return false;
@@ -281,6 +360,7 @@ bool op_build_tp(Op& op, Tube& t, const TpSource& src) {
     HIPCHK(hipGetLastError());
     HIPCHK(hipStreamSynchronize(st));
 
+    if (!lds_tiles_disabled()) build_lds_tiles(op, D, ne, tpc, exc, ecx, xcol);
     op.fmt = OP_TP;
     op.ew = kS * kS * kS;
     op.lw = 6;

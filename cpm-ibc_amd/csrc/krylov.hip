@@ -888,43 +888,46 @@ template <typename T, int NR, int KIND>
 __global__ __launch_bounds__(kBlock) void k_fin(const Seg<T>* __restrict__ segs, int nseg) {
     constexpr int NV = FinTraits<KIND>::NV;
     constexpr unsigned M = FinTraits<KIND>::M;
-    const Seg<T>& sg = segs[blockIdx.x];
-    if (!seg_any(sg, M)) return;   // same test the producer used
-    // Wave w sums components w, w + 4, ...; lanes stride over the blocks with
-    // all of a wave's components and 16 blocks per lane in flight at once
-    // (the partials were just written by another kernel, so every round
-    // costs a full memory latency).
-    constexpr int NC = NV * NR, CPW = (NC + 3) / 4, U = 16;
-    __shared__ double red[NC];
+    // One workgroup per (segment, rhs lane): the lane's NV components over all
+    // block partials, U blocks per thread in flight at once (the partials were
+    // just written by another kernel, so every round costs a full memory
+    // latency), then a workgroup sum and the lane's scalar update.
+    const Seg<T>& sg = segs[blockIdx.x / NR];
+    const int k = (int)(blockIdx.x % NR);
+    if (k >= sg.nr || !lane_in(sg, k, M)) return;   // same test the producer used
+    constexpr int U = 8;
+    __shared__ double red[NV][kBlock / 64];
     const int lanei = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    double acc[CPW];
+    double acc[NV];
 #pragma unroll
-    for (int cc = 0; cc < CPW; ++cc) acc[cc] = 0.0;
-    for (int b0 = lanei; b0 < sg.nblocks; b0 += 64 * U) {
-        double v[CPW][U];
+    for (int a = 0; a < NV; ++a) acc[a] = 0.0;
+    for (int b0 = threadIdx.x; b0 < sg.nblocks; b0 += kBlock * U) {
+        double v[NV][U];
 #pragma unroll
-        for (int cc = 0; cc < CPW; ++cc)
+        for (int a = 0; a < NV; ++a)
 #pragma unroll
             for (int u = 0; u < U; ++u) {
-                const int c = wave + 4 * cc, b = b0 + 64 * u;
-                v[cc][u] = (c < NC && b < sg.nblocks) ? ldg(&sg.part[(size_t)c * sg.nblocks + b]) : 0.0;
+                const int b = b0 + kBlock * u;
+                v[a][u] = b < sg.nblocks ? ldg(&sg.part[(size_t)(a * NR + k) * sg.nblocks + b]) : 0.0;
             }
 #pragma unroll
-        for (int cc = 0; cc < CPW; ++cc)
+        for (int a = 0; a < NV; ++a)
 #pragma unroll
-            for (int u = 0; u < U; ++u) acc[cc] += v[cc][u];
+            for (int u = 0; u < U; ++u) acc[a] += v[a][u];
     }
 #pragma unroll
-    for (int cc = 0; cc < CPW; ++cc) {
-        const int c = wave + 4 * cc;
-        const double t = wave_sum(acc[cc]);
-        if (lanei == 0 && c < NC) red[c] = t;
+    for (int a = 0; a < NV; ++a) {
+        const double t = wave_sum(acc[a]);
+        if (lanei == 0) red[a][wave] = t;
     }
     __syncthreads();
-    const int k = threadIdx.x;   // one thread per rhs lane
-    if (k >= sg.nr || !lane_in(sg, k, M)) return;
+    if (threadIdx.x != 0) return;
     double tot[NV];
-    for (int a = 0; a < NV; ++a) tot[a] = red[a * NR + k];
+    for (int a = 0; a < NV; ++a) {
+        double t = 0.0;
+        for (int w = 0; w < kBlock / 64; ++w) t += red[a][w];
+        tot[a] = t;
+    }
     fin_scalar<KIND>(sg.st[k], tot);
 }
 
@@ -946,7 +949,7 @@ void launch_reset_states(KState* st, int n, double tol2, int max_it, double delt
 // ---------------------------------------------------------------- launchers
 #define LAUNCH(kern, ...) hipLaunchKernelGGL(kern, dim3(c.nblocks), dim3(kBlock), 0, c.stream, c.segs, c.bseg, __VA_ARGS__)
 #define EGATHER(...) LAUNCH((k_egather<T, NR>), __VA_ARGS__)
-#define FIN(T, NR, KIND) hipLaunchKernelGGL((k_fin<T, NR, KIND>), dim3(c.nseg), dim3(kBlock), 0, c.stream, c.segs, c.nseg)
+#define FIN(T, NR, KIND) hipLaunchKernelGGL((k_fin<T, NR, KIND>), dim3(c.nseg * NR), dim3(kBlock), 0, c.stream, c.segs, c.nseg)
 
 template <typename T, int NR>
 void Launch<T, NR>::init(const LaunchCfg<T>& c, const T* b, T* x, T* r, T* r0, T* p, T* v) {

@@ -96,17 +96,50 @@ __device__ __forceinline__ double wave_sum(double v) {
 // Workgroup sums of NV*NR per-thread values.  Partials are stored
 // component-major, part[c * nblocks + lb], so that the finaliser reads each
 // component with coalesced loads.
+// Wave sums of C = 2^L values per lane by a butterfly reduce-scatter: at step
+// s (lane offset 32 >> s) a lane keeps the half of its values that its bit of
+// the offset selects, summed with its partner's, so after L steps lane l holds
+// component (l >> (6 - L)) & (C - 1) summed over 2^L lanes; the last 6 - L
+// steps sum those groups.  2 (C - 1) + 6 - L exchanges instead of 6 C.
+// Returns the lane's component total.
+template <int C>
+__device__ __forceinline__ double wave_reduce_scatter(double (&v)[C], int lanei, int& comp) {
+    constexpr int L = C == 1 ? 0 : C == 2 ? 1 : C == 4 ? 2 : C == 8 ? 3 : 4;
+    static_assert((1 << L) == C, "component count must be 1, 2, 4, 8 or 16");
+#pragma unroll
+    for (int st = 0; st < L; ++st) {
+        const int o = 32 >> st, m = C >> st, h = m >> 1;
+        const bool up = (lanei & o) != 0;
+#pragma unroll
+        for (int j = 0; j < h; ++j) {
+            // both halves cross over and the lane selects its sum: a select
+            // between two array elements would be folded into a dynamically
+            // indexed load, and the array moved to LDS
+            const double lo = v[j] + __shfl_xor(v[j], o, 64);
+            const double hi = v[j + h] + __shfl_xor(v[j + h], o, 64);
+            v[j] = up ? hi : lo;
+        }
+    }
+    double w = v[0];
+#pragma unroll
+    for (int o = 32 >> L; o > 0; o >>= 1) w += __shfl_xor(w, o, 64);
+    comp = (lanei >> (6 - L)) & (C - 1);
+    return w;
+}
+
 template <int NV, int NR>
 __device__ void block_partials(const double (&v)[NV][NR], double* part, int lb, int nblocks) {
-    __shared__ double red[kBlock / 64][NV * NR];
+    constexpr int C = NV * NR;
+    __shared__ double red[kBlock / 64][C];
     const int lanei = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    double flat[C];
 #pragma unroll
     for (int a = 0; a < NV; ++a)
 #pragma unroll
-        for (int k = 0; k < NR; ++k) {
-            const double w = wave_sum(v[a][k]);
-            if (lanei == 0) red[wave][a * NR + k] = w;
-        }
+        for (int k = 0; k < NR; ++k) flat[a * NR + k] = v[a][k];
+    int comp;
+    const double w = wave_reduce_scatter<C>(flat, lanei, comp);
+    if ((lanei & ((64 / C) - 1)) == 0) red[wave][comp] = w;
     __syncthreads();
     if (threadIdx.x < NV * NR) {
         double s = 0.0;
@@ -367,10 +400,13 @@ __device__ __forceinline__ typename VT<T, W>::type arow(const OpView<T>& op, dou
     V lo = (V)(T)0;
     T ld;
     if (op.fmt == OP_TP) {
+        // a missing neighbour (-1) reads the row's own entry with weight 0:
+        // no conditional load in the loop
 #pragma unroll
         for (int s = 0; s < 6; ++s) {
             const int c = ldg(&op.lcol[(int64_t)s * D + row]);
-            if (c >= 0) lo += op.loff * vload<T, W>(tb + (size_t)c * stride);
+            const T wl = c >= 0 ? op.loff : (T)0;
+            lo += wl * vload<T, W>(tb + (size_t)(c >= 0 ? c : row) * stride);
         }
         ld = op.ldu;
     } else {
@@ -553,8 +589,9 @@ __global__ __launch_bounds__(kBlock) void k_lfinish(const Seg<T>* __restrict__ s
             }
         }
         if (std::is_same<T, float>::value && RED == 1 && seg_any(sg, M_UPDATE)) {
-            for (int k = 0; k < sg.nr; ++k)
-                if (lane_in(sg, k, M_UPDATE)) {
+#pragma unroll
+            for (int k = 0; k < NR; ++k)   // constant indices keep pv in registers
+                if (k < sg.nr && lane_in(sg, k, M_UPDATE)) {
                     const double ax =
                         arow<double, 1>(sg.op64, sg.m, sg.n, row, ldg(&mv.x64[i + k]), mv.tmp64 + sg.off + k, NR);
                     const double rk = ldg(&mv.b64[i + k]) - ax;

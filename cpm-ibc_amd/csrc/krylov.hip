@@ -281,6 +281,42 @@ struct LdsTile {
     static constexpr int kRows = (kLdsRows * NR * (int)sizeof(T) <= kLdsBytes) ? kLdsRows : 0;
 };
 
+// LDS copy of a tile's x rows.  Rows of 32 bytes are kept as two planes of
+// 16-byte halves (half h of row p at plane h, slot p): a ds_read_b128 of the
+// lanes' rows then meets fewer bank conflicts than with 32-byte rows (halved
+// on the torus-knot system).
+template <typename V>
+struct LdsRows {
+    typedef int I4 __attribute__((ext_vector_type(4)));
+    static constexpr bool kSplit = sizeof(V) == 32;
+    V* base;
+    int cap;
+    __device__ __forceinline__ V load(int p) const {
+        if constexpr (kSplit) {
+            const I4* h = reinterpret_cast<const I4*>(base);
+            const I4 a = h[p], b = h[cap + p];
+            V v;
+            __builtin_memcpy(reinterpret_cast<char*>(&v), &a, 16);
+            __builtin_memcpy(reinterpret_cast<char*>(&v) + 16, &b, 16);
+            return v;
+        } else {
+            return base[p];
+        }
+    }
+    __device__ __forceinline__ void store(int p, const V& v) const {
+        if constexpr (kSplit) {
+            I4* h = reinterpret_cast<I4*>(base);
+            I4 a, b;
+            __builtin_memcpy(&a, reinterpret_cast<const char*>(&v), 16);
+            __builtin_memcpy(&b, reinterpret_cast<const char*>(&v) + 16, 16);
+            h[p] = a;
+            h[cap + p] = b;
+        } else {
+            base[p] = v;
+        }
+    }
+};
+
 // A row's tensor-product metadata, loaded before the tile's x is in LDS so
 // that its latency overlaps the fill.
 template <typename T>
@@ -300,14 +336,14 @@ __device__ __forceinline__ void tp_row_load(const OpView<T>& op, int64_t row, Tp
 
 // E row of the tensor-product format from the tile's LDS copy of x: the same
 // products in the same order as egather_tp_lines, so both paths agree to the
-// bit.  lx[p] holds the x row at position p of the tile's list.
+// bit.  lx.load(p) is the x row at position p of the tile's list.
 template <typename T, typename V, int W>
-__device__ __forceinline__ void egather_tp_lds(const TpRow<T>& m, const V* lx, V& acc) {
+__device__ __forceinline__ void egather_tp_lds(const TpRow<T>& m, const LdsRows<V>& lx, V& acc) {
 #pragma unroll
     for (int l = 0; l < 16; ++l) {
         const T wx = m.w[l >> 2], wy = m.w[4 + (l & 3)];
-        const V* q = lx + m.p[l];
-        const V x0 = q[0], x1 = q[1], x2 = q[2], x3 = q[3];
+        const int q = m.p[l];
+        const V x0 = lx.load(q), x1 = lx.load(q + 1), x2 = lx.load(q + 2), x3 = lx.load(q + 3);
         acc += ((m.w[8] * wy) * wx) * x0;
         acc += ((m.w[9] * wy) * wx) * x1;
         acc += ((m.w[10] * wy) * wx) * x2;
@@ -339,7 +375,7 @@ __device__ __forceinline__ void exc_row_load(const OpView<T>& op, int e, ExcRow&
 // exception path, in its order.
 template <typename T, typename V, int W>
 __device__ __forceinline__ void egather_tp_lds_exc(const OpView<T>& op, int64_t row, int e, const ExcRow& m,
-                                                   const V* lx, V& acc) {
+                                                   const LdsRows<V>& lx, V& acc) {
     const int64_t D = op.D;
     T wz[4];
 #pragma unroll
@@ -348,14 +384,15 @@ __device__ __forceinline__ void egather_tp_lds_exc(const OpView<T>& op, int64_t 
     for (int l = 0; l < 16; ++l) {
         const T wx = ldg(&op.tpw[(l >> 2) * D + row]);
         const T wy = ldg(&op.tpw[(4 + (l & 3)) * D + row]);
-        const V x0 = lx[m.pos(4 * l)], x1 = lx[m.pos(4 * l + 1)], x2 = lx[m.pos(4 * l + 2)], x3 = lx[m.pos(4 * l + 3)];
+        const V x0 = lx.load(m.pos(4 * l)), x1 = lx.load(m.pos(4 * l + 1)), x2 = lx.load(m.pos(4 * l + 2)),
+                x3 = lx.load(m.pos(4 * l + 3));
         acc += ((wz[0] * wy) * wx) * x0;
         acc += ((wz[1] * wy) * wx) * x1;
         acc += ((wz[2] * wy) * wx) * x2;
         acc += ((wz[3] * wy) * wx) * x3;
     }
     const int xp = m.pos(64);
-    if (xp >= 0) acc += ldg(&op.xval[e]) * lx[xp];
+    if (xp >= 0) acc += ldg(&op.xval[e]) * lx.load(xp);
 }
 
 // E row of the ELL format (eight slots in flight).
@@ -498,6 +535,7 @@ __global__ __launch_bounds__(kBlock) void k_egather(const Seg<T>* __restrict__ s
     typedef typename VT<T, NR>::type V;
     constexpr int LR = LdsTile<T, NR>::kRows;
     __shared__ V lx[LR > 0 ? LR : 1];
+    const LdsRows<V> lrows{lx, LR > 0 ? LR : 1};
     int lb;
     int64_t row;
     const Seg<T>& sg = seg_of(segs, bseg, lb, row);
@@ -531,14 +569,14 @@ __global__ __launch_bounds__(kBlock) void k_egather(const Seg<T>* __restrict__ s
 #pragma unroll
         for (int k = 0; k < KF; ++k) {
             const int u = (int)threadIdx.x + k * kBlock;
-            if (u < nu) lx[u] = buf[k];
+            if (u < nu) lrows.store(u, buf[k]);
         }
         __syncthreads();
         if (act) {
             V acc = (V)(T)0;
             if (kind == 2) acc = vload<T, NR>(xb + (size_t)row * NR);
-            else if (kind == 0) egather_tp_lds<T, V, NR>(m, lx, acc);
-            else egather_tp_lds_exc<T, V, NR>(sg.op, row, e, mx, lx, acc);   // rows an interior boundary moved
+            else if (kind == 0) egather_tp_lds<T, V, NR>(m, lrows, acc);
+            else egather_tp_lds_exc<T, V, NR>(sg.op, row, e, mx, lrows, acc);   // rows an interior boundary moved
             vstore<T, NR>(out + i, acc);
         }
     } else if (main && row < sg.op.D) {

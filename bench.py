@@ -132,10 +132,11 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--batch", type=int, default=8)
+    ap.add_argument("--batch", type=int, default=8, help="interleaved lanes (1, 2, 4 or 8)")
+    ap.add_argument("--stream", type=int, default=32, help="right-hand sides streamed through the lanes per step")
     ap.add_argument("--workload", default="torus-knot", choices=["torus-knot", "sphere"])
     ap.add_argument("--cpu-iters", type=int, default=100)
-    ap.add_argument("--update-delta", type=float, default=1e-3)
+    ap.add_argument("--update-delta", type=float, default=1e-2)
     ap.add_argument("--latency-samples", type=int, default=30)
     ap.add_argument("--no-cpu", action="store_true")
     args = ap.parse_args()
@@ -147,37 +148,51 @@ def main():
     import cpmgpu
     ctx = cpmgpu.Context(0)
     t0 = time.perf_counter()
+    S = max(args.stream, args.batch)
     if args.workload == "sphere":
-        work = sphere_workload(ctx, cpmgpu, args.batch)
+        work = sphere_workload(ctx, cpmgpu, S)
     else:
-        work = torus_knot_workload(ctx, cpmgpu, args.batch)
+        work = torus_knot_workload(ctx, cpmgpu, S)
     setup_s = time.perf_counter() - t0
     op, rhs = work["op"], work["rhs"]
     D = op.D
-    B = len(rhs)
+    B = args.batch
     tol = 1e-10
 
     # resident inputs/outputs
-    d_rhs = [ctx.alloc(8 * D) for _ in range(B)]
-    d_x = [ctx.alloc(8 * D) for _ in range(B)]
+    d_rhs = [ctx.alloc(8 * D) for _ in range(S)]
+    d_x = [ctx.alloc(8 * D) for _ in range(S)]
     for d, r in zip(d_rhs, rhs):
         ctx.h2d(d, np.ascontiguousarray(r))
     o32 = cpmgpu.opts(tol=tol, precision=cpmgpu.CPM_FP32_REFINE, update_delta=args.update_delta)
     o64 = cpmgpu.opts(tol=tol, precision=cpmgpu.CPM_FP64)
     ops = [op] * B
 
+    # the step: S right-hand sides streamed through B interleaved lanes (FP32 + FP64 refinement)
     for _ in range(args.warmup):
-        infos = ctx.solve_batch_device(ops, d_rhs, d_x, o32)
+        infos = ctx.solve_stream_device(op, d_rhs, d_x, B, o32)
     ctx.synchronize()
     t0 = time.perf_counter()
     for _ in range(args.steps):
-        infos = ctx.solve_batch_device(ops, d_rhs, d_x, o32)
+        infos = ctx.solve_stream_device(op, d_rhs, d_x, B, o32)
     ctx.synchronize()
-    t_batch = time.perf_counter() - t0
-    value = B * args.steps / t_batch
-    ms_per_step = 1000 * t_batch / args.steps
-    x32 = [ctx.d2h(np.zeros(D), d) for d in d_x]
+    t_run = time.perf_counter() - t0
+    value = S * args.steps / t_run
+    ms_per_step = 1000 * t_run / args.steps
+    x32 = [ctx.d2h(np.zeros(D), d) for d in d_x[:1]]
     info32 = infos[0]
+    assert all(i["status"] == "converged" for i in infos), infos
+    iters_all = [i["iters"] for i in infos]
+
+    # one fixed batch of B lanes (the batch waits for its slowest system)
+    nb_steps = max(2, args.steps // 4)
+    ctx.solve_batch_device(ops, d_rhs[:B], d_x[:B], o32)
+    ctx.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(nb_steps):
+        ctx.solve_batch_device(ops, d_rhs[:B], d_x[:B], o32)
+    ctx.synchronize()
+    batch_rate = B * nb_steps / (time.perf_counter() - t0)
 
     # plain FP64 GPU path: one right-hand side per solve, reference algorithm
     ctx.solve_batch_device([op], d_rhs[:1], d_x[:1], o64)
@@ -211,9 +226,9 @@ def main():
     copy_bytes = 1 << 29
     copy_ms = ctx.profile_copy(copy_bytes, 20)
 
-    # the same batch with host buffers (PCIe transfers of rhs and x included)
+    # one batch with host buffers (PCIe transfers of rhs and x included)
     t0 = time.perf_counter()
-    ctx.solve_batch(ops, rhs, o32)
+    ctx.solve_batch(ops, rhs[:B], o32)
     pcie_rate = B / (time.perf_counter() - t0)
     eg64_ms, eg64_bytes = op.profile_egather(cpmgpu.CPM_FP64, 1, reps=50)
 
@@ -230,14 +245,16 @@ def main():
         "vs_baseline": None,
         "dtype": "f32 Krylov + f64 refinement",
         "data": "synthetic (GPU-built tube of an analytic surface; analytic + seeded random RHS)",
-        "config": {"workload": work["name"], "rows": D, "rhs_per_step": B, "tol": tol,
-                   "precond": "diagonal", "tube_build_ms": round(work["tube"].info["build_ms"], 2),
-                   "setup_s": round(setup_s, 3)},
+        "config": {"workload": work["name"], "rows": D, "rhs_per_step": S, "lanes": B, "tol": tol,
+                   "precond": "diagonal", "update_delta": args.update_delta,
+                   "tube_build_ms": round(work["tube"].info["build_ms"], 2), "setup_s": round(setup_s, 3)},
         "fp64_solves_per_s": round(fp64_rate, 3),
         "fp32refine_vs_fp64": round(value / fp64_rate, 3),
         "p50_ms": round(p50, 3),
         "p99_ms": round(p99, 3),
+        "batched_solves_per_s": round(batch_rate, 3),
         "iters_fp32refine": info32["iters"], "updates": info32["updates"],
+        "iters_fp32refine_mean": round(float(np.mean(iters_all)), 1), "iters_fp32refine_max": int(max(iters_all)),
         "iters_fp64": inf64["iters"],
         "egather_fp64_GBs": round(eg64_bytes / (eg64_ms * 1e-3) / 1e9, 1),
         "copy_GBs": round(2 * copy_bytes / (copy_ms * 1e-3) / 1e9, 1),

@@ -368,7 +368,7 @@ static void run_batch(Ctx* ctx, const std::vector<Group>& groups, const std::vec
     const LaunchCfg<double> c64{segs64, bseg, blocks, G, st, sweeps};
     const LaunchCfg<float> c32{segs32, bseg, blocks, G, st, sweeps};
     KState* hs = ctx->host_states;
-    const double delta = o.update_delta > 0 ? std::min(o.update_delta, 0.999) : 1e-3;
+    const double delta = o.update_delta > 0 ? std::min(o.update_delta, 0.999) : kDefaultUpdateDelta;
     launch_reset_states(sts, nstate, o.tol * o.tol, o.max_iters, delta * delta, st);
 
     // Iterations are queued in growing chunks; every kernel skips instances
@@ -428,6 +428,130 @@ static void run_batch(Ctx* ctx, const std::vector<Group>& groups, const std::vec
         }
 }
 
+// Continuous batching (FP32 + refinement): n systems of one operator through
+// NR interleaved lanes.  At every poll a lane whose system stopped hands its
+// solution out and takes the next right-hand side: b64 <- new b, x64 = x32 = 0
+// and a fresh state in the refinement status, whose step then computes
+// ||b - A 0||^2 = ||b||^2 and restarts the lane's Krylov space
+// (mx_update).  Lanes never wait for the slowest system of a batch.
+template <int NR>
+static void run_stream(Ctx* ctx, Op* op, const std::vector<const double*>& rhs, const std::vector<double*>& xs,
+                       const cpm_solve_opts& o, std::vector<cpm_solve_info>& infos) {
+    const hipStream_t st = ctx->stream;
+    const int n = (int)rhs.size();
+    const int64_t D = op->D;
+    const int nb = (int)((D + kBlock - 1) / kBlock);
+    const int64_t total = (D * NR + 63) & ~int64_t(63);
+    auto r256 = [](size_t k) { return (k + 255) & ~size_t(255); };
+    const size_t part_stride = 2 * NR;
+    const size_t bytes = r256(sizeof(Seg<double>)) + r256(sizeof(Seg<float>)) + r256(sizeof(int) * nb) +
+                         r256(sizeof(KState) * kMaxNR) + r256(sizeof(double) * part_stride * nb) +
+                         r256(sizeof(PackEntry) * 4) + 256;
+    char* tb = reinterpret_cast<char*>(ctx->workspace("stream_tables", bytes));
+    auto carve = [&](size_t k) { char* p = tb; tb += r256(k); return p; };
+    auto* seg64 = reinterpret_cast<Seg<double>*>(carve(sizeof(Seg<double>)));
+    auto* seg32 = reinterpret_cast<Seg<float>*>(carve(sizeof(Seg<float>)));
+    int* bseg = reinterpret_cast<int*>(carve(sizeof(int) * nb));   // all zero: one segment
+    KState* sts = reinterpret_cast<KState*>(carve(sizeof(KState) * kMaxNR));
+    double* part = reinterpret_cast<double*>(carve(sizeof(double) * part_stride * nb));
+    PackEntry* dpe = reinterpret_cast<PackEntry*>(carve(sizeof(PackEntry) * 4));
+    const int nr0 = std::min(n, NR);
+    Seg<double> h64{};
+    Seg<float> h32{};
+    fill_seg(h64, op, 0, 0, nb, nr0, sts, part);
+    fill_seg(h32, op, 0, 0, nb, nr0, sts, part);
+    std::vector<int> hb(nb, 0);
+    HIPCHK(hipMemcpyAsync(seg64, &h64, sizeof(h64), hipMemcpyHostToDevice, st));
+    HIPCHK(hipMemcpyAsync(seg32, &h32, sizeof(h32), hipMemcpyHostToDevice, st));
+    HIPCHK(hipMemcpyAsync(bseg, hb.data(), sizeof(int) * nb, hipMemcpyHostToDevice, st));
+    Vecs<double> V64{};
+    Vecs<float> V32{};
+    {
+        double* q = ctx->workspace("vec64", sizeof(double) * total * 5 + 1024);
+        for (double** pp : {&V64.b, &V64.x, &V64.r, &V64.t, &V64.tmp}) { *pp = q; q += total; }
+        float* q32 = reinterpret_cast<float*>(ctx->workspace("vec32", sizeof(float) * total * 10 + 1024));
+        for (float** pp : {&V32.x, &V32.r, &V32.r0, &V32.p, &V32.v, &V32.s, &V32.t, &V32.y, &V32.z, &V32.tmp}) {
+            *pp = q32;
+            q32 += total;
+        }
+    }
+    // one lane of one vector in or out: a single pack entry with block0 = 0
+    auto lane_pack = [&](int k, const double* src, double* dst, double* ws64, float* ws32, int to_ws) {
+        PackEntry e{};
+        e.src = src;
+        e.dst = dst;
+        e.perm = op->perm.as<int>();
+        e.D = D;
+        e.off = 0;
+        e.lane = k;
+        e.nr = NR;
+        e.block0 = 0;
+        HIPCHK(hipMemcpyAsync(dpe, &e, sizeof(e), hipMemcpyHostToDevice, st));
+        hipLaunchKernelGGL(k_pack, dim3((unsigned)nb), dim3(256), 0, st, dpe, bseg, ws64, ws32, to_ws);
+        HIPCHK(hipStreamSynchronize(st));   // the entry is rewritten by the next call
+    };
+    // all lanes start from zero vectors (padding lanes stay zero and idle)
+    HIPCHK(hipMemsetAsync(V64.b, 0, sizeof(double) * total * 5, st));
+    HIPCHK(hipMemsetAsync(V32.x, 0, sizeof(float) * total * 10, st));
+    const double delta = o.update_delta > 0 ? std::min(o.update_delta, 0.999) : kDefaultUpdateDelta;
+    KState fresh{};
+    fresh.tol2 = o.tol * o.tol;
+    fresh.max_it = o.max_iters;
+    fresh.delta2 = delta * delta;
+    fresh.status = ST_UPDATE;
+    fresh.bsq = -1.0;       // learnt by the first refinement step
+    fresh.updates = -1;     // that step is not a refinement
+    fresh.scale = 1.0;
+    KState idle{};
+    idle.status = ST_CONVERGED;
+    std::vector<KState> hst(kMaxNR, idle);
+    std::vector<int> cur(NR, -1);
+    int next = 0;
+    for (int k = 0; k < nr0; ++k) {
+        lane_pack(k, rhs[next], nullptr, V64.b, nullptr, 1);
+        hst[k] = fresh;
+        cur[k] = next++;
+    }
+    HIPCHK(hipMemcpyAsync(sts, hst.data(), sizeof(KState) * kMaxNR, hipMemcpyHostToDevice, st));
+    int sweeps = op->precond == CPM_PRECOND_JACOBI ? op->precond_iters : 0;
+    const LaunchCfg<float> c32{seg32, bseg, nb, 1, st, sweeps};
+    const MixVec mv{V64.x, V64.r, V64.b, V64.tmp};
+    KState* hs = ctx->host_states;
+    int done = 0;
+    while (done < n) {
+        for (int c = 0; c < 16; ++c) Launch<float, NR>::iteration(c32, V32, true, mv);
+        HIPCHK(hipGetLastError());
+        HIPCHK(hipMemcpyAsync(hs, sts, sizeof(KState) * kMaxNR, hipMemcpyDeviceToHost, st));
+        HIPCHK(hipStreamSynchronize(st));
+        bool changed = false;
+        for (int k = 0; k < nr0; ++k) {
+            if (cur[k] < 0) continue;
+            const KState& s = hs[k];
+            const bool running = s.status == ST_RUN || (s.status >= ST_UPDATE && s.status <= ST_READY);
+            if (running) continue;
+            cpm_solve_info& inf = infos[(size_t)cur[k]];
+            inf.status = s.status;
+            inf.iters = s.iters;
+            inf.restarts = s.restarts;
+            inf.updates = s.updates;
+            inf.rel_residual = s.bsq > 0 ? std::sqrt(s.rr_true / s.bsq) : 0.0;
+            lane_pack(k, nullptr, xs[(size_t)cur[k]], V64.x, nullptr, 0);
+            ++done;
+            cur[k] = -1;
+            KState ns = idle;
+            if (next < n) {
+                lane_pack(k, rhs[(size_t)next], nullptr, V64.b, nullptr, 1);
+                lane_pack(k, nullptr, nullptr, V64.x, V32.x, 1);   // x64 = x32 = 0
+                ns = fresh;
+                cur[k] = next++;
+            }
+            HIPCHK(hipMemcpyAsync(sts + k, &ns, sizeof(KState), hipMemcpyHostToDevice, st));
+            changed = true;
+        }
+        if (changed) HIPCHK(hipStreamSynchronize(st));
+    }
+}
+
 void solve_batch(Ctx* ctx, const std::vector<Op*>& ops, const std::vector<const double*>& rhs,
                  const std::vector<double*>& xs, const cpm_solve_opts& o, std::vector<cpm_solve_info>& infos,
                  const std::vector<std::pair<double, double>>* mn) {
@@ -460,6 +584,20 @@ void solve_batch(Ctx* ctx, const std::vector<Op*>& ops, const std::vector<const 
     else if (nr == 2) run_batch<2>(ctx, groups, rhs, xs, o, infos);
     else if (nr <= 4) run_batch<4>(ctx, groups, rhs, xs, o, infos);
     else run_batch<8>(ctx, groups, rhs, xs, o, infos);
+}
+
+static void solve_stream(Op* op, const std::vector<const double*>& rhs, const std::vector<double*>& xs, int lanes,
+                         const cpm_solve_opts& o, std::vector<cpm_solve_info>& infos) {
+    infos.assign(rhs.size(), cpm_solve_info{});
+    if (rhs.empty()) return;
+    if (o.precision != CPM_FP32_REFINE) throw ArgError("streamed solves run in FP32_REFINE precision");
+    if (op->nD != 0) throw ArgError("streamed solves need an operator without trailing nD rows");
+    if (op->precond == CPM_PRECOND_JACOBI && op->precond_iters > 100000) throw ArgError("too many Jacobi sweeps");
+    if (lanes == 1) run_stream<1>(op->ctx, op, rhs, xs, o, infos);
+    else if (lanes == 2) run_stream<2>(op->ctx, op, rhs, xs, o, infos);
+    else if (lanes == 4) run_stream<4>(op->ctx, op, rhs, xs, o, infos);
+    else if (lanes == 8) run_stream<8>(op->ctx, op, rhs, xs, o, infos);
+    else throw ArgError("lanes must be 1, 2, 4 or 8");
 }
 
 void multiply(Op* op, const double* dx, double* dy, double m, double n) {
@@ -534,11 +672,22 @@ static void profile_egather_t(Op* op, int reps, double* avg_ms, double* bytes) {
     for (uint8_t f : fl) gathered += !(f & ROW_IDENTITY);
     *avg_ms = ms / reps;
     // algorithmic bytes: operator data of the gathered rows, x read once, out written once, flags
+    // (tensor-product rows: 12 weights, the exception index, and either 16
+    // int16 line positions in the tile's list (LDS tiles) or 16 int32 line
+    // starts; exception rows 72 int16 positions or 64 int32 columns, plus the
+    // extra entry; LDS tiles also read their lists, 4 bytes per row)
     double opb;
-    if (op->fmt == OP_TP)
+    const double sT = sizeof(T);
+    if (op->fmt == OP_TP && op->tln.p) {
+        const int64_t* n = op->tile_rows;
+        opb = (double)n[0] * (16 * 2.0 + 12 * sT + 4) + (double)n[1] * (72 * 2.0 + 12 * sT + 4 + sT) +
+              (double)n[2] * (16 * 4.0 + 12 * sT + 4) + (double)n[3] * (64 * 4.0 + 12 * sT + 8 + sT) +
+              4.0 * (double)op->tile_list + 8.0 * (double)((op->D + kBlock - 1) / kBlock);
+    } else if (op->fmt == OP_TP) {
         opb = (double)gathered * (16 * 4.0 + 12.0 * sizeof(T) + 4.0) + (double)op->nexc * (64 * 4.0 + 4 + sizeof(T));
-    else
+    } else {
         opb = (double)gathered * op->ew * (4.0 + sizeof(T));
+    }
     *bytes = opb + 2.0 * op->D * NR * sizeof(T) + (double)op->D;
 }
 
@@ -836,7 +985,7 @@ void cpm_solve_opts_default(cpm_solve_opts* o) {
     o->tol = 1e-10;
     o->max_iters = 100000;
     o->precision = CPM_FP64;
-    o->update_delta = 1e-3;
+    o->update_delta = kDefaultUpdateDelta;
     o->reserved = 0;
 }
 
@@ -929,6 +1078,21 @@ int cpm_multiply(cpm_op* op, const double* x, double* y, double m, double n) {
         multiply(o, d, d + o->D, m, n);
         HIPCHK(hipMemcpyAsync(y, d + o->D, sizeof(double) * o->D, hipMemcpyDeviceToHost, ctx->stream));
         HIPCHK(hipStreamSynchronize(ctx->stream));
+    });
+}
+
+int cpm_solve_stream_device(cpm_op* op, int n_rhs, const double* const* d_rhs, double* const* d_x, int lanes,
+                            const cpm_solve_opts* opts, cpm_solve_info* infos) {
+    return guard([&] {
+        if (!op || n_rhs < 0 || (n_rhs > 0 && (!d_rhs || !d_x || !infos))) throw ArgError("null argument");
+        for (int i = 0; i < n_rhs; ++i)
+            if (!d_rhs[i] || !d_x[i]) throw ArgError("null vector");
+        HIPCHK(hipSetDevice(O(op)->ctx->device));
+        std::vector<const double*> r(d_rhs, d_rhs + n_rhs);
+        std::vector<double*> x(d_x, d_x + n_rhs);
+        std::vector<cpm_solve_info> inf;
+        solve_stream(O(op), r, x, lanes, resolve_opts(opts), inf);
+        std::copy(inf.begin(), inf.end(), infos);
     });
 }
 

@@ -80,6 +80,7 @@ struct OpView {
     const int16_t* tlx;  // [nexc][kExcPos] positions of an exception row's 64 columns and extra entry
 };
 
+constexpr double kDefaultUpdateDelta = 1e-2;   // FP32+refine: refine when the FP32 residual fell 100x
 constexpr int kLdsRows = 2560;   // longest LDS tile list
 constexpr int kLdsBytes = 81920; // LDS of one E-gather workgroup (gfx950: 160 KB per CU, two workgroups)
 constexpr int kExcPos = 72;      // positions per exception record (64 slots, extra entry, padding)

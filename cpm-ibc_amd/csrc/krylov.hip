@@ -875,10 +875,20 @@ __device__ __forceinline__ void mx_init(KState& s, double rr, int next) {
     s.status = (rr > 0.0 && std::isfinite(rr)) ? next : ST_BREAKDOWN;
 }
 
-// Refinement step: the true residual ||b - A64 x64||^2 is rt.
+// Refinement step: the true residual ||b - A64 x64||^2 is rt.  A lane that
+// took a new right-hand side (bsq < 0, x64 = 0) learns ||b||^2 = rt here, as
+// FIN_MX_START does for the first ones.
 __device__ __forceinline__ void mx_update(KState& s, double rt) {
     ++s.updates;
     s.rr_true = rt;
+    if (s.bsq < 0.0) {
+        s.bsq = rt;
+        if (!std::isfinite(rt)) { s.status = ST_BREAKDOWN; return; }
+        if (rt == 0.0) { s.status = ST_ZERO_RHS; return; }
+        if (!(rt > s.tol2 * rt)) { s.status = ST_CONVERGED; return; }
+        if (s.max_it <= 0) { s.status = ST_MAXIT; return; }
+        s.tol2 *= rt;
+    }
     if (std::isnan(rt)) { s.status = ST_BREAKDOWN; return; }
     if (!(rt > s.tol2)) { s.status = ST_CONVERGED; return; }
     if (s.status == ST_UPDATE_MAXIT) { s.status = ST_MAXIT; return; }

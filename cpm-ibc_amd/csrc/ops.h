@@ -47,6 +47,10 @@ struct Op {
     int64_t nexc = 0, nmoved = 0;
     double loff = 0, ldu = 0;
     Buffer tln, tlo, tlr, tpl, tlx;   // LDS tiles (OP_TP)
+    // gathered (non-identity) rows: [0] plain and [1] exception rows of LDS
+    // tiles, [2] plain and [3] exception rows read directly; rows in all lists
+    int64_t tile_rows[4] = {0, 0, 0, 0};
+    int64_t tile_list = 0;
     // Lfull rows of the rows that solves treat as identity rows (internal
     // order, hw slots each, padded with weight 0): explicit time stepping
     // applies Lfull to every row (problems/HeatWithIBC.cpp:84-88).  Entries

@@ -251,7 +251,9 @@ static bool lds_tiles_disabled() {
 // record, -1 = no extra entry).  Tiles whose list would exceed kLdsRows keep
 // reading x directly (tln = -1).
 static void build_lds_tiles(Op& op, int64_t D, int64_t ne, const Buffer& tpc, const Buffer& exc, const Buffer& ecx,
-                            const Buffer& xcol) {
+                            const Buffer& xcol, const Buffer& flag) {
+    std::vector<uint8_t> hf((size_t)D);
+    HIPCHK(hipMemcpy(hf.data(), flag.p, hf.size(), hipMemcpyDeviceToHost));
     std::vector<int> hc((size_t)kS * kS * D), he((size_t)D), hx((size_t)std::max<int64_t>(ne, 1) * 64),
         hxc((size_t)std::max<int64_t>(ne, 1));
     HIPCHK(hipMemcpy(hc.data(), tpc.p, hc.size() * 4, hipMemcpyDeviceToHost));
@@ -282,11 +284,15 @@ static void build_lds_tiles(Op& op, int64_t D, int64_t ne, const Buffer& tpc, co
         std::sort(u.begin(), u.end());
         u.erase(std::unique(u.begin(), u.end()), u.end());
         tlo[(size_t)tile] = (int)tlr.size();
-        if ((int64_t)u.size() > kLdsRows) {
+        const bool fits = (int64_t)u.size() <= kLdsRows;
+        for (int64_t r = r0; r < r1; ++r)
+            if (!(hf[(size_t)r] & ROW_IDENTITY)) ++op.tile_rows[(fits ? 0 : 2) + (he[(size_t)r] >= 0 ? 1 : 0)];
+        if (!fits) {
             tln[(size_t)tile] = -1;
             continue;
         }
         tln[(size_t)tile] = (int)u.size();
+        op.tile_list += (int64_t)u.size();
         tlr.insert(tlr.end(), u.begin(), u.end());
         auto pos = [&](int c) { return (int16_t)(std::lower_bound(u.begin(), u.end(), c) - u.begin()); };
         for (int64_t r = r0; r < r1; ++r) {
@@ -360,7 +366,7 @@ bool op_build_tp(Op& op, Tube& t, const TpSource& src) {
     HIPCHK(hipGetLastError());
     HIPCHK(hipStreamSynchronize(st));
 
-    if (!lds_tiles_disabled()) build_lds_tiles(op, D, ne, tpc, exc, ecx, xcol);
+    if (!lds_tiles_disabled()) build_lds_tiles(op, D, ne, tpc, exc, ecx, xcol, flag);
     op.fmt = OP_TP;
     op.ew = kS * kS * kS;
     op.lw = 6;

@@ -21,7 +21,7 @@ EXPORTS = [
     "cpm_solve_opts_default", "cpm_last_error", "cpm_ctx_create", "cpm_ctx_destroy", "cpm_ctx_synchronize",
     "cpm_ctx_device_bytes", "cpm_op_create_csr", "cpm_op_destroy", "cpm_op_rows", "cpm_op_format", "cpm_op_set_mn",
     "cpm_op_set_precond", "cpm_multiply", "cpm_multiply_device", "cpm_solve", "cpm_solve_device",
-    "cpm_solve_batch", "cpm_solve_batch_device", "cpm_device_alloc", "cpm_device_free", "cpm_memcpy_h2d",
+    "cpm_solve_batch", "cpm_solve_batch_device", "cpm_solve_stream_device", "cpm_device_alloc", "cpm_device_free", "cpm_memcpy_h2d",
     "cpm_memcpy_d2h", "cpm_tube_create", "cpm_tube_create_mesh", "cpm_tube_destroy", "cpm_tube_get_info", "cpm_tube_export",
     "cpm_tube_export_E", "cpm_tube_export_L", "cpm_tube_interp_matrix", "cpm_op_create_from_tube", "cpm_profile_egather", "cpm_profile_copy", "cpm_heat_step_device", "cpm_rd_params_default", "cpm_rd_step_device", "cpm_problem_create", "cpm_problem_destroy",
     "cpm_problem_get_info", "cpm_problem_rhs_points", "cpm_problem_subset", "cpm_problem_export",
@@ -142,6 +142,8 @@ def load(path=LIB_PATH):
     L.cpm_problem_plot_matrix.argtypes = [vp, vp, ctypes.c_int64, vp, vp, vp]
     L.cpm_geodesic_distance.argtypes = [vp, ctypes.c_double, ctypes.c_int, ctypes.POINTER(SolveOpts), vp,
                                         ctypes.POINTER(SolveInfo)]
+    L.cpm_solve_stream_device.argtypes = [vp, ctypes.c_int, ctypes.POINTER(vp), ctypes.POINTER(vp), ctypes.c_int,
+                                          ctypes.POINTER(SolveOpts), ctypes.POINTER(SolveInfo)]
     L.cpm_profile_copy.argtypes = [vp, ctypes.c_int64, ctypes.c_int, ctypes.POINTER(ctypes.c_double)]
     L.cpm_profile_egather.argtypes = [vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.POINTER(ctypes.c_double),
                                       ctypes.POINTER(ctypes.c_double)]
@@ -177,7 +179,7 @@ def rd_params(dx, method=0, **kw):
     return p
 
 
-def opts(tol=1e-10, max_iters=100000, precision=CPM_FP64, update_delta=1e-3):
+def opts(tol=1e-10, max_iters=100000, precision=CPM_FP64, update_delta=1e-2):
     o = SolveOpts()
     load().cpm_solve_opts_default(ctypes.byref(o))
     o.tol, o.max_iters, o.precision, o.update_delta, o.reserved = tol, max_iters, precision, update_delta, 0
@@ -239,6 +241,16 @@ class Context:
         infos = (SolveInfo * n)()
         _chk(load().cpm_solve_batch(self._h, n, OA, RA, XA, ctypes.byref(o), infos))
         return xs, [infos[i].as_dict() for i in range(n)]
+
+    def solve_stream_device(self, op, d_rhs, d_x, lanes=8, o=None):
+        """Continuous batching of many right-hand sides of one operator (FP32+refine)."""
+        n = len(d_rhs)
+        o = o or opts(precision=CPM_FP32_REFINE)
+        RA = (vp * n)(*d_rhs)
+        XA = (vp * n)(*d_x)
+        infos = (SolveInfo * n)()
+        _chk(load().cpm_solve_stream_device(op._h, n, RA, XA, lanes, ctypes.byref(o), infos))
+        return [infos[i].as_dict() for i in range(n)]
 
     def solve_batch_device(self, ops, d_rhs, d_x, o=None):
         n = len(ops)

@@ -57,7 +57,7 @@ typedef struct cpm_solve_opts {
     double tol;            /* relative residual ||b-Ax|| <= tol ||b||; reference 1e-10 (src/Solver.cpp:164) */
     int max_iters;         /* reference 100000 (src/Solver.cpp:164)                                         */
     int precision;         /* enum cpm_precision                                                            */
-    double update_delta;   /* FP32_REFINE: FP32 residual drop that triggers an FP64 refinement step (<=0: 1e-3) */
+    double update_delta;   /* FP32_REFINE: FP32 residual drop that triggers an FP64 refinement step (<=0: 1e-2) */
     int reserved;          /* must be 0                                                                     */
 } cpm_solve_opts;
 
@@ -132,6 +132,14 @@ int cpm_solve_batch(cpm_ctx* ctx, int n_inst, cpm_op* const* ops, const double* 
                     double* const* x, const cpm_solve_opts* opts, cpm_solve_info* infos);
 int cpm_solve_batch_device(cpm_ctx* ctx, int n_inst, cpm_op* const* ops, const double* const* d_rhs,
                            double* const* d_x, const cpm_solve_opts* opts, cpm_solve_info* infos);
+/* Streamed solves of one operator (continuous batching, FP32_REFINE only): the
+ * n_rhs systems op x_i = rhs_i flow through `lanes` (1, 2, 4 or 8)
+ * interleaved lanes; a lane whose system has stopped takes the next
+ * right-hand side at the next check (every 16 iterations), so a slow system
+ * does not hold the others' lanes.  Device pointers, D doubles each.  No
+ * counterpart in the reference, which solves one system per Solver::Solve. */
+int cpm_solve_stream_device(cpm_op* op, int n_rhs, const double* const* d_rhs, double* const* d_x, int lanes,
+                            const cpm_solve_opts* opts, cpm_solve_info* infos);
 
 /* ---- tube (closest-point computational band) -------------------------
  * Replaces Tube(SurfaceSpecifier&, interp_deg, dx) (src/Tube.h:45,

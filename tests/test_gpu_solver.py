@@ -125,3 +125,35 @@ def test_jacobi_preconditioner_matches_oracle(gpu_ctx, sweeps):
     x32, i32 = op.solve(b, cpmgpu.opts(tol=1e-11, precision=cpmgpu.CPM_FP32_REFINE))
     assert i32["status"] == "converged"
     assert np.linalg.norm(x32 - x_ref) / np.linalg.norm(x_ref) <= 1e-8
+
+
+@pytest.mark.parametrize("n,lanes", [(11, 8), (3, 8), (5, 1)])
+def test_stream_solves(gpu_ctx, n, lanes):
+    """Continuous batching (cpm_solve_stream_device): every system of the stream,
+    whichever lane and refill it ran in, matches its own FP64 solve (both at
+    tol 1e-10) to 1e-8; a zero right-hand side reports zero_rhs and x = 0."""
+    print(f"case stream n={n} lanes={lanes}", flush=True)
+    tube = cpmgpu.Tube(gpu_ctx, "Sphere", 0.05)
+    op = tube.operator(1.0, -1.0)
+    D = op.D
+    rng = np.random.default_rng(7)
+    B = [rng.standard_normal(D) * (1 + 10 * (k % 3)) for k in range(n)]
+    if n > 3:
+        B[2] = np.zeros(D)
+    d_b = [gpu_ctx.alloc(8 * D) for _ in B]
+    d_x = [gpu_ctx.alloc(8 * D) for _ in B]
+    for d, b in zip(d_b, B):
+        gpu_ctx.h2d(d, b)
+    infos = gpu_ctx.solve_stream_device(op, d_b, d_x, lanes, cpmgpu.opts(tol=1e-10, precision=cpmgpu.CPM_FP32_REFINE))
+    o64 = cpmgpu.opts(tol=1e-10)
+    for k, (b, d, inf) in enumerate(zip(B, d_x, infos)):
+        x = gpu_ctx.d2h(np.zeros(D), d)
+        if not b.any():
+            assert inf["status"] == "zero_rhs" and not x.any()
+            continue
+        assert inf["status"] == "converged", (k, inf)
+        x64, i64 = op.solve(b, o64)
+        rel = np.linalg.norm(x - x64) / np.linalg.norm(x64)
+        assert rel < 1e-8, (k, rel, inf)
+    for d in d_b + d_x:
+        gpu_ctx.free(d)

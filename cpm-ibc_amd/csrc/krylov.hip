@@ -283,8 +283,8 @@ struct LdsTile {
 
 // LDS copy of a tile's x rows.  Rows of 32 bytes are kept as two planes of
 // 16-byte halves (half h of row p at plane h, slot p): a ds_read_b128 of the
-// lanes' rows then meets fewer bank conflicts than with 32-byte rows (halved
-// on the torus-knot system).
+// lanes' rows then meets half the bank conflicts of 32-byte rows (measured on
+// the torus-knot system; 8-byte planes were slower again).
 template <typename V>
 struct LdsRows {
     typedef int I4 __attribute__((ext_vector_type(4)));

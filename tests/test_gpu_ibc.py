@@ -1,6 +1,6 @@
 """GPU assembly of interior-boundary-condition problems vs the oracle
 (PoissonWithIBC): subsets in identical order, directions, Lfull / Efull,
-identity rows, and solve parity; plus the config-2 torus-knot system."""
+identity rows, and solve parity; plus the torus-knot system (torus h = 0.02, 2000-segment knot)."""
 import numpy as np
 import pytest
 
@@ -115,9 +115,9 @@ def test_first_order_and_neumann(gpu_ctx):
                  m=1.0, n=-1.0)
 
 
-def test_torus_knot_config2(gpu_ctx):
+def test_torus_knot_system(gpu_ctx):
     """Config 2: Laplace on the torus R=1, r=0.4, h=0.02 with a 2000-segment (2,3) torus-knot Dirichlet curve."""
-    print("case config-2 torus knot", flush=True)
+    print("case torus knot h=0.02", flush=True)
     dx = 0.02
     V = torus_knot_polyline()
     kw = dict(type="Polyline", verts=V, boundary_order=2, on_ibc_tolerance=0.1 * dx * dx)
@@ -134,7 +134,7 @@ def test_torus_knot_config2(gpu_ctx):
     rel_res = np.linalg.norm(r) / np.linalg.norm(b)
     x32, i32 = op.solve(b, cpmgpu.opts(tol=1e-11, precision=cpmgpu.CPM_FP32_REFINE))
     rel32 = np.linalg.norm(x32 - x) / np.linalg.norm(x)
-    print(f"cfg2 fp64 iters={inf['iters']} oracle-residual={rel_res:.2e}; fp32+refine iters={i32['iters']} "
+    print(f"torus knot fp64 iters={inf['iters']} oracle-residual={rel_res:.2e}; fp32+refine iters={i32['iters']} "
           f"updates={i32['updates']} rel_diff={rel32:.2e}", flush=True)
     # the recurrence residual (1e-12) and the true FP64 residual part near the FP64
     # floor of this system (||A|| ~ 1e4): the true residual stays well under the

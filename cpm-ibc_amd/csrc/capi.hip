@@ -37,7 +37,8 @@ Buffer::~Buffer() {
 }
 
 // ---------------------------------------------------------------- packing
-// Interleave up to NR caller vectors into a workspace vector [row][rhs] and back.
+// Move up to NR caller vectors into a workspace vector and back: interleaved
+// [row][rhs] (nr = NR, ls = 1) or lane-major [rhs][row] (nr = 1, ls = D).
 // The operator's internal row r holds the caller's row perm[r] (tube-built
 // operators run in lexicographic order).
 struct PackEntry {
@@ -46,15 +47,16 @@ struct PackEntry {
     const int* perm;     // internal row -> caller row, nullptr = identity
     int64_t D;
     int64_t off;
-    int lane, nr;
+    int lane, nr;        // lane, row stride
     int block0;
+    int64_t ls = 1;      // lane stride
 };
 
 __global__ void k_pack(const PackEntry* ent, const int* bent, double* ws64, float* ws32, int to_ws) {
     const PackEntry& e = ent[bent[blockIdx.x]];
     const int64_t row = (int64_t)(blockIdx.x - e.block0) * 256 + threadIdx.x;
     if (row >= e.D) return;
-    const size_t i = (size_t)(e.off + row * e.nr + e.lane);
+    const size_t i = (size_t)(e.off + row * e.nr + e.lane * e.ls);
     const int64_t cr = e.perm ? e.perm[row] : row;
     if (to_ws) {
         const double v = e.src ? e.src[cr] : 0.0;
@@ -351,7 +353,8 @@ static void run_batch(Ctx* ctx, const std::vector<Group>& groups, const std::vec
             e.D = groups[g].op->D;
             e.off = off[g];
             e.lane = k;
-            e.nr = NR;
+            e.nr = refine ? 1 : NR;   // mixed: FP64 vectors lane-major
+            e.ls = refine ? e.D : 1;
             e.block0 = (int)pb.size();
             for (int b = 0; b < nb[g]; ++b) pb.push_back((int)pe.size());
             pe.push_back(e);
@@ -475,7 +478,8 @@ static void run_stream(Ctx* ctx, Op* op, const std::vector<const double*>& rhs, 
             q32 += total;
         }
     }
-    // one lane of one vector in or out: a single pack entry with block0 = 0
+    // one lane of one vector in or out: a single pack entry with block0 = 0;
+    // FP64 vectors are lane-major, FP32 ones interleaved
     auto lane_pack = [&](int k, const double* src, double* dst, double* ws64, float* ws32, int to_ws) {
         PackEntry e{};
         e.src = src;
@@ -484,7 +488,8 @@ static void run_stream(Ctx* ctx, Op* op, const std::vector<const double*>& rhs, 
         e.D = D;
         e.off = 0;
         e.lane = k;
-        e.nr = NR;
+        e.nr = ws64 ? 1 : NR;
+        e.ls = ws64 ? D : 1;
         e.block0 = 0;
         HIPCHK(hipMemcpyAsync(dpe, &e, sizeof(e), hipMemcpyHostToDevice, st));
         hipLaunchKernelGGL(k_pack, dim3((unsigned)nb), dim3(256), 0, st, dpe, bseg, ws64, ws32, to_ws);
@@ -541,7 +546,8 @@ static void run_stream(Ctx* ctx, Op* op, const std::vector<const double*>& rhs, 
             KState ns = idle;
             if (next < n) {
                 lane_pack(k, rhs[(size_t)next], nullptr, V64.b, nullptr, 1);
-                lane_pack(k, nullptr, nullptr, V64.x, V32.x, 1);   // x64 = x32 = 0
+                lane_pack(k, nullptr, nullptr, V64.x, nullptr, 1);   // x64 = 0
+                lane_pack(k, nullptr, nullptr, nullptr, V32.x, 1);   // x32 = 0
                 ns = fresh;
                 cur[k] = next++;
             }

@@ -31,7 +31,7 @@
 // Precision CPM_FP32_REFINE: iterative refinement with FP32 BiCGStab inner
 // solves, run as one continuous device-side recurrence.  The solution is
 // accumulated in FP64; when the FP32 recurrence residual has dropped by
-// delta (default 1e-3) since the last step, or meets the tolerance, or hits
+// delta (default 1e-2) since the last step, or meets the tolerance, or hits
 // the rho-restart rule, the lane takes a refinement step inside the next
 // iteration's kernels (no extra launches):
 //
@@ -42,6 +42,9 @@
 //   supdate   r32 = r64 scale ; r0 = r32 ; p = v = 0
 //   lfinish   partial (r32, r32)
 //   fin       rho = r0sq = (r32, r32) ; the lane rejoins the next iteration
+//
+// The FP64 vectors (x64, r64, b, tmp64) are lane-major within a segment,
+// the FP32 ones interleave the lanes per row.
 //
 // (Keeping p across steps, as in reliable-update schemes for one operator,
 // diverges here once the residual approaches the FP32 operator's own
@@ -188,6 +191,14 @@ constexpr unsigned M_STAGE3 = M_RUN | M_READY;     // fin of the loop test
 template <typename T>
 __device__ __forceinline__ bool lane_in(const Seg<T>& sg, int k, unsigned mask) {
     return k < sg.nr && ((mask >> ldg(&sg.st[k].status)) & 1u);
+}
+
+// The FP64 vectors of mixed-precision segments are lane-major: lane k is a
+// run of D values at off + k D, so a refinement step moves only the data of
+// the lanes that refine and its gathers read with stride 1.
+template <typename T>
+__device__ __forceinline__ size_t lm_at(const Seg<T>& sg, int k, int64_t row) {
+    return (size_t)(sg.off + (int64_t)k * sg.op.D + row);
 }
 
 template <typename T>
@@ -511,14 +522,13 @@ __global__ __launch_bounds__(kBlock) void k_pupdate(const Seg<T>* __restrict__ s
     }
     if (std::is_same<T, float>::value && seg_any(sg, M_UPDATE)) {
         auto xv = vload<T, NR>(x + i);
-        auto xd = vload<double, NR>(mv.x64 + i);
 #pragma unroll
         for (int k = 0; k < NR; ++k)
             if (lane_in(sg, k, M_UPDATE)) {
-                setlane(xd, k, lane(xd, k) + (double)lane(xv, k) / ldg(&sg.st[k].scale));
+                double* xd = mv.x64 + lm_at(sg, k, row);
+                stg(xd, ldg(xd) + (double)lane(xv, k) / ldg(&sg.st[k].scale));
                 setlane(xv, k, (T)0);
             }
-        vstore<double, NR>(mv.x64 + i, xd);
         vstore<T, NR>(x + i, xv);
     }
 }
@@ -585,7 +595,8 @@ __global__ __launch_bounds__(kBlock) void k_egather(const Seg<T>* __restrict__ s
     if (row >= sg.op.D) return;
     if (std::is_same<T, float>::value && (lanes & M_UPDATE) && seg_any(sg, M_UPDATE)) {
         for (int k = 0; k < sg.nr; ++k)
-            if (lane_in(sg, k, M_UPDATE)) stg(&mv.tmp64[i + k], erow<double, 1>(sg.op64, row, mv.x64 + sg.off + k, NR));
+            if (lane_in(sg, k, M_UPDATE))
+                stg(&mv.tmp64[lm_at(sg, k, row)], erow<double, 1>(sg.op64, row, mv.x64 + lm_at(sg, k, 0), 1));
     }
 }
 
@@ -630,10 +641,11 @@ __global__ __launch_bounds__(kBlock) void k_lfinish(const Seg<T>* __restrict__ s
 #pragma unroll
             for (int k = 0; k < NR; ++k)   // constant indices keep pv in registers
                 if (k < sg.nr && lane_in(sg, k, M_UPDATE)) {
+                    const size_t j = lm_at(sg, k, row);
                     const double ax =
-                        arow<double, 1>(sg.op64, sg.m, sg.n, row, ldg(&mv.x64[i + k]), mv.tmp64 + sg.off + k, NR);
-                    const double rk = ldg(&mv.b64[i + k]) - ax;
-                    stg(&mv.r64[i + k], rk);
+                        arow<double, 1>(sg.op64, sg.m, sg.n, row, ldg(&mv.x64[j]), mv.tmp64 + lm_at(sg, k, 0), 1);
+                    const double rk = ldg(&mv.b64[j]) - ax;
+                    stg(&mv.r64[j], rk);
                     pv[0][k] = rk * rk;
                 }
         }
@@ -679,11 +691,10 @@ __global__ __launch_bounds__(kBlock) void k_supdate(const Seg<T>* __restrict__ s
     if (std::is_same<T, float>::value && seg_any(sg, M_RESCALE)) {
         auto r0v = vload<T, NR>(r0 + i);
         auto pv = vload<T, NR>(p + i);
-        auto rd = vload<double, NR>(mv.r64 + i);
 #pragma unroll
         for (int k = 0; k < NR; ++k)
             if (lane_in(sg, k, M_RESCALE)) {
-                const T rk = (T)(lane(rd, k) * ldg(&sg.st[k].scale));
+                const T rk = (T)(ldg(&mv.r64[lm_at(sg, k, row)]) * ldg(&sg.st[k].scale));
                 setlane(rv, k, rk);
                 setlane(r0v, k, rk);
                 setlane(pv, k, (T)0);
@@ -798,12 +809,14 @@ __global__ __launch_bounds__(kBlock) void k_mx_start(const Seg<double>* __restri
 #pragma unroll
     for (int k = 0; k < NR; ++k) pv[0][k] = 0.0;
     if (row < sg.op.D) {
-        const size_t i = (size_t)(sg.off + row * NR);
-        auto bv = vload<double, NR>(b + i);
-        vstore<double, NR>(r + i, bv);
-        vstore<double, NR>(x + i, (typename VT<double, NR>::type)0.0);
 #pragma unroll
-        for (int k = 0; k < NR; ++k) pv[0][k] = lane(bv, k) * lane(bv, k);
+        for (int k = 0; k < NR; ++k) {
+            const size_t j = lm_at(sg, k, row);
+            const double bk = b[j];
+            r[j] = bk;
+            x[j] = 0.0;
+            pv[0][k] = bk * bk;
+        }
     }
     block_partials<1, NR>(pv, sg.part, lb, sg.nblocks);
 }
@@ -822,7 +835,6 @@ __global__ __launch_bounds__(kBlock) void k_mx_down(const Seg<double>* __restric
     for (int k = 0; k < NR; ++k) pv[0][k] = 0.0;
     if (row < sg.op.D) {
         const size_t i = (size_t)(sg.off + row * NR);
-        auto rv = vload<double, NR>(r64 + i);
         auto ro = vload<float, NR>(r + i);
         auto r0o = vload<float, NR>(r0 + i);
         auto po = vload<float, NR>(p + i);
@@ -831,7 +843,7 @@ __global__ __launch_bounds__(kBlock) void k_mx_down(const Seg<double>* __restric
 #pragma unroll
         for (int k = 0; k < NR; ++k) {
             if (!lane_in(sg, k, M_RESCALE)) continue;
-            const float rk = (float)(lane(rv, k) * ldg(&sg.st[k].scale));
+            const float rk = (float)(r64[lm_at(sg, k, row)] * ldg(&sg.st[k].scale));
             setlane(ro, k, rk);
             setlane(r0o, k, rk);
             setlane(po, k, 0.0f);

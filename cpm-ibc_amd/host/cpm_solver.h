@@ -224,6 +224,40 @@ class Solver {
     std::vector<cpm_solve_info> batch_info_;
 };
 
+// Mesh or curve of a Wavefront OBJ file (SimplePolygonMesh::readMeshFromObjFile,
+// src/SimplePolygonMesh.cpp:139-213): face_size 3 = triangles, 2 = polyline edges.
+struct ObjMesh {
+    std::vector<double> verts;    // 3 per vertex
+    std::vector<int32_t> faces;   // face_size per face, 0-based
+    int face_size = 0;
+};
+
+inline ObjMesh ReadObj(const std::string& path) {
+    int64_t nv = 0, nf = 0;
+    ObjMesh m;
+    check(cpm_obj_read(path.c_str(), nullptr, &nv, nullptr, &nf, &m.face_size), "ReadObj");
+    m.verts.resize((size_t)nv * 3);
+    m.faces.resize((size_t)nf * m.face_size);
+    check(cpm_obj_read(path.c_str(), m.verts.data(), &nv, m.faces.data(), &nf, &m.face_size), "ReadObj");
+    return m;
+}
+
+// Consecutive polyline edges as the vertex chain of a "Polyline" surface or
+// IBC (closed curves repeat their first vertex, src/SurfaceSpecifier.cpp:334-341).
+inline std::vector<double> PolylineChain(const ObjMesh& m) {
+    if (m.face_size != 2) throw Error(CPM_ERR_ARG, "PolylineChain: the mesh holds no polyline edges");
+    int64_t n = 0;
+    const int64_t nv = (int64_t)m.verts.size() / 3, ne = (int64_t)m.faces.size() / 2;
+    check(cpm_polyline_chain(m.verts.data(), nv, m.faces.data(), ne, nullptr, &n), "PolylineChain");
+    std::vector<double> c((size_t)n * 3);
+    check(cpm_polyline_chain(m.verts.data(), nv, m.faces.data(), ne, c.data(), &n), "PolylineChain");
+    return c;
+}
+
+struct ObjFile {
+    std::string path;
+};
+
 // GPU tube of an analytic surface (Tube(SurfaceSpecifier&, p, dx), src/Tube.cpp:88-461).
 class Tube {
   public:
@@ -246,6 +280,14 @@ class Tube {
         check(cpm_tube_create_mesh(ctx_->get(), verts.data(), (int64_t)verts.size() / 3, faces.data(),
                                    (int64_t)faces.size() / 3, bbox_lo, bbox_hi, p, dx, &t_),
               "Tube");
+        check(cpm_tube_get_info(t_, &info_), "Tube");
+    }
+    // The surface of an OBJ file, SurfaceSpecifier(file) (src/SurfaceSpecifier.cpp:298-363):
+    // triangles or a polyline.
+    Tube(const ObjFile& file, Scalar dx, double bbox_lo = 0, double bbox_hi = 0, int p = 3,
+         std::shared_ptr<Context> ctx = Context::shared())
+        : ctx_(std::move(ctx)) {
+        check(cpm_tube_create_obj(ctx_->get(), file.path.c_str(), bbox_lo, bbox_hi, p, dx, &t_), "Tube");
         check(cpm_tube_get_info(t_, &info_), "Tube");
     }
     ~Tube() { cpm_tube_destroy(t_); }

@@ -25,7 +25,8 @@ EXPORTS = [
     "cpm_memcpy_d2h", "cpm_tube_create", "cpm_tube_create_mesh", "cpm_tube_destroy", "cpm_tube_get_info", "cpm_tube_export",
     "cpm_tube_export_E", "cpm_tube_export_L", "cpm_tube_interp_matrix", "cpm_op_create_from_tube", "cpm_profile_egather", "cpm_profile_copy", "cpm_heat_step_device", "cpm_rd_params_default", "cpm_rd_step_device", "cpm_problem_create", "cpm_problem_destroy",
     "cpm_problem_get_info", "cpm_problem_rhs_points", "cpm_problem_subset", "cpm_problem_export",
-    "cpm_op_create_from_problem", "cpm_geodesic_distance", "cpm_problem_plot_matrix",
+    "cpm_op_create_from_problem", "cpm_geodesic_distance", "cpm_problem_plot_matrix", "cpm_obj_read",
+    "cpm_polyline_chain", "cpm_tube_create_obj",
 ]
 
 
@@ -125,6 +126,11 @@ def load(path=LIB_PATH):
                                   ctypes.c_int, vp, ctypes.c_int64, ctypes.c_int, ctypes.c_double, ctypes.POINTER(vp)]
     L.cpm_tube_create_mesh.argtypes = [vp, vp, ctypes.c_int64, vp, ctypes.c_int64, ctypes.c_double, ctypes.c_double,
                                        ctypes.c_int, ctypes.c_double, ctypes.POINTER(vp)]
+    L.cpm_tube_create_obj.argtypes = [vp, ctypes.c_char_p, ctypes.c_double, ctypes.c_double, ctypes.c_int,
+                                      ctypes.c_double, ctypes.POINTER(vp)]
+    L.cpm_obj_read.argtypes = [ctypes.c_char_p, vp, ctypes.POINTER(ctypes.c_int64), vp,
+                               ctypes.POINTER(ctypes.c_int64), ctypes.POINTER(ctypes.c_int)]
+    L.cpm_polyline_chain.argtypes = [vp, ctypes.c_int64, vp, ctypes.c_int64, vp, ctypes.POINTER(ctypes.c_int64)]
     L.cpm_tube_destroy.argtypes = [vp]
     L.cpm_tube_get_info.argtypes = [vp, ctypes.POINTER(TubeInfo)]
     L.cpm_tube_export.argtypes = [vp, vp, vp, vp, vp, vp, vp, vp]
@@ -184,6 +190,33 @@ def opts(tol=1e-10, max_iters=100000, precision=CPM_FP64, update_delta=1e-2):
     load().cpm_solve_opts_default(ctypes.byref(o))
     o.tol, o.max_iters, o.precision, o.update_delta, o.reserved = tol, max_iters, precision, update_delta, 0
     return o
+
+
+def read_obj(path):
+    """Wavefront OBJ as the reference reads it (SimplePolygonMesh::readMeshFromObjFile,
+    src/SimplePolygonMesh.cpp:139-213): dict(verts (nv x 3), faces (nf x face_size, 0-based),
+    kind "Triangulation" (face_size 3) or "Polyline" (2-vertex edges))."""
+    L = load()
+    nv, nf, fs = ctypes.c_int64(0), ctypes.c_int64(0), ctypes.c_int(0)
+    bp = os.fsencode(path)
+    _chk(L.cpm_obj_read(bp, None, ctypes.byref(nv), None, ctypes.byref(nf), ctypes.byref(fs)))
+    verts = np.zeros((nv.value, 3))
+    faces = np.zeros((nf.value, fs.value), np.int32)
+    _chk(L.cpm_obj_read(bp, _ptr(verts), ctypes.byref(nv), _ptr(faces), ctypes.byref(nf), ctypes.byref(fs)))
+    return dict(verts=verts, faces=faces, kind="Triangulation" if fs.value == 3 else "Polyline")
+
+
+def polyline_chain(verts, edges):
+    """Consecutive polyline edges -> the vertex chain "Polyline" surfaces and IBCs take; a closed
+    polyline (src/SurfaceSpecifier.cpp:334-341) repeats its first vertex at the end."""
+    L = load()
+    vv = np.ascontiguousarray(verts, np.float64).reshape(-1, 3)
+    ee = np.ascontiguousarray(edges, np.int32).reshape(-1, 2)
+    n = ctypes.c_int64(0)
+    _chk(L.cpm_polyline_chain(_ptr(vv), len(vv), _ptr(ee), len(ee), None, ctypes.byref(n)))
+    out = np.zeros((n.value, 3))
+    _chk(L.cpm_polyline_chain(_ptr(vv), len(vv), _ptr(ee), len(ee), _ptr(out), ctypes.byref(n)))
+    return out
 
 
 class Context:
@@ -366,12 +399,18 @@ class CSR:
 class Tube:
     """GPU-built closest-point tube (Tube(SurfaceSpecifier&, p, dx))."""
 
-    def __init__(self, ctx, surface="Sphere", dx=0.1, params=None, bbox=None, p=3, verts=None, faces=None):
-        """surface "Triangulation" takes verts (nv x 3) and faces (nf x 3); bbox None then
+    def __init__(self, ctx, surface="Sphere", dx=0.1, params=None, bbox=None, p=3, verts=None, faces=None,
+                 obj=None):
+        """surface "Triangulation" takes verts (nv x 3) and faces (nf x 3); obj, a path, takes the
+        surface an OBJ file holds (triangles or a polyline; SurfaceSpecifier(file)); bbox None then
         gives the reference's box [min(0, coordinates), max(0, coordinates)]."""
         self.ctx = ctx
         self._h = vp()
-        if surface == "Triangulation":
+        if obj is not None:
+            bb = bbox if bbox is not None else (0.0, 0.0)
+            _chk(load().cpm_tube_create_obj(ctx._h, os.fsencode(obj), float(bb[0]), float(bb[1]), p, dx,
+                                            ctypes.byref(self._h)))
+        elif surface == "Triangulation":
             vv = np.ascontiguousarray(verts, np.float64).reshape(-1, 3)
             ff = np.ascontiguousarray(faces, np.int32).reshape(-1, 3)
             bb = bbox if bbox is not None else (0.0, 0.0)
@@ -447,6 +486,14 @@ class Tube:
 
 class IBC:
     """An interior boundary curve (SurfaceSpecifier + IBCMetaData)."""
+
+    @classmethod
+    def from_obj(cls, path, **meta):
+        """Polyline curve of an OBJ file ("l" records, SurfaceSpecifier(file))."""
+        d = read_obj(path)
+        if d["kind"] != "Polyline":
+            raise CpmError(-1, f"{path}: an interior boundary curve needs polyline edges, not triangles")
+        return cls("Polyline", verts=polyline_chain(d["verts"], d["faces"]), **meta)
 
     def __init__(self, type, params=None, verts=None, boundary_order=2, boundary_type=(0, 0), on_ibc_tolerance=0.0,
                  dof_radius_scale=1.0):

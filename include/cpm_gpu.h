@@ -179,6 +179,32 @@ int cpm_tube_create(cpm_ctx* ctx, const char* surface_type, const double* params
  * FP64 minimisers (lowest triangle index on ties), from a BVH built here. */
 int cpm_tube_create_mesh(cpm_ctx* ctx, const double* verts, int64_t nverts, const int32_t* faces, int64_t nfaces,
                          double bbox_lo, double bbox_hi, int p, double dx, cpm_tube** out);
+/* ---- mesh and curve files ------------------------------------------------
+ * Wavefront OBJ as SimplePolygonMesh::readMeshFromObjFile reads it
+ * (src/SimplePolygonMesh.cpp:139-213): "v" positions, "f" faces of 1-based
+ * position indices ("p/t/n" forms: texture and normal indices ignored), "l"
+ * polylines split into 2-vertex edges; other records ignored.  Call with
+ * verts = faces = NULL for the counts, then with arrays of at least
+ * *nverts x 3 and *nfaces x *face_size entries (indices returned 0-based).
+ * face_size is 3 (SurfaceSpecifier's "Triangulation") or 2 (polyline edges,
+ * "Polyline", src/SurfaceSpecifier.cpp:298-343).  Rejected with
+ * CPM_ERR_ARG: files mixing triangles and edges, polygons other than
+ * triangles, vertices without exactly 3 coordinates, non-positive or
+ * out-of-range indices. */
+int cpm_obj_read(const char* path, double* verts, int64_t* nverts, int32_t* faces, int64_t* nfaces, int* face_size);
+/* Consecutive polyline edges (edge k starts where edge k-1 ends) as the
+ * vertex chain that "Polyline" surfaces and cpm_ibc_desc take; a closed
+ * polyline (SurfaceSpecifier.cpp:334-341: first and last vertices equal, or
+ * the last edge touching vertex 0) ends with its first vertex again.
+ * chain = NULL returns the count in *nchain. */
+int cpm_polyline_chain(const double* verts, int64_t nverts, const int32_t* edges, int64_t nedges, double* chain,
+                       int64_t* nchain);
+/* Tube of the surface an OBJ file holds: SurfaceSpecifier(file) + Tube
+ * (src/SurfaceSpecifier.cpp:298-363): triangles as cpm_tube_create_mesh,
+ * edges as a "Polyline" surface; bbox_lo >= bbox_hi takes the reference's
+ * box [min(0, coordinates), max(0, coordinates)]. */
+int cpm_tube_create_obj(cpm_ctx* ctx, const char* path, double bbox_lo, double bbox_hi, int p, double dx,
+                        cpm_tube** out);
 int cpm_tube_destroy(cpm_tube* tube);
 int cpm_tube_get_info(const cpm_tube* tube, cpm_tube_info* info);
 /* Node data in tube order (any pointer may be NULL): linear grid index

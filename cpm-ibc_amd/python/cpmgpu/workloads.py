@@ -84,7 +84,7 @@ def real_sph_harm(lmax, pts):
                 cols.append(Pl[(l, 0)])
             else:
                 cols.append(np.sqrt(2) * Pl[(l, m)] * np.cos(m * ph))
-    return np.stack(cols, 1)
+    return np.stack(cols, 0).T      # column-major: stacking rows is contiguous
 
 
 def displaced_icosphere(subdivisions=6, amplitude=0.1, lmax=6, seed=4):
@@ -104,3 +104,93 @@ def box_mesh(lo=-1.0, hi=1.0):
     F = np.array([[0, 1, 3], [0, 3, 2], [4, 6, 7], [4, 7, 5], [0, 4, 5], [0, 5, 1], [2, 3, 7], [2, 7, 6],
                   [0, 2, 6], [0, 6, 4], [1, 5, 7], [1, 7, 3]], np.int32)
     return c, F
+
+
+class MT19937_64:
+    """The 64-bit Mersenne Twister with the parameters of C++'s std::mt19937_64
+    (Matsumoto and Nishimura; w 64, n 312, m 156, r 31), so that seeded streams
+    match a C++ caller's."""
+    _N, _M = 312, 156
+    _MASK = (1 << 64) - 1
+    _UPPER, _LOWER = 0xFFFFFFFF80000000, 0x7FFFFFFF
+
+    def __init__(self, seed=5489):
+        mt = [seed & self._MASK]
+        for i in range(1, self._N):
+            mt.append((6364136223846793005 * (mt[-1] ^ (mt[-1] >> 62)) + i) & self._MASK)
+        self.mt, self.i = mt, self._N
+
+    def _twist(self):
+        mt, N, M = self.mt, self._N, self._M
+        for k in range(N):
+            y = (mt[k] & self._UPPER) | (mt[(k + 1) % N] & self._LOWER)
+            v = mt[(k + M) % N] ^ (y >> 1)
+            if y & 1:
+                v ^= 0xB5026F5AA96619E9
+            mt[k] = v
+        self.i = 0
+
+    def __call__(self):
+        if self.i >= self._N:
+            self._twist()
+        y = self.mt[self.i]
+        self.i += 1
+        y ^= (y >> 29) & 0x5555555555555555
+        y ^= (y << 17) & 0x71D67FFFEDA60000
+        y ^= (y << 37) & 0xFFF7EEE000000000
+        y ^= y >> 43
+        return y & self._MASK
+
+    def uniform(self):
+        """Uniform double in [0, 1): the top 53 bits."""
+        return (self() >> 11) * (1.0 / (1 << 53))
+
+    def normals(self, n):
+        """n standard normals by the Box-Muller transform of pairs (u1, u2), u1 in (0, 1]:
+        sqrt(-2 ln u1) cos(2 pi u2) and sqrt(-2 ln u1) sin(2 pi u2)."""
+        out = []
+        while len(out) < n:
+            u1 = 1.0 - self.uniform()
+            u2 = self.uniform()
+            rad = np.sqrt(-2.0 * np.log(u1))
+            out += [rad * np.cos(2 * np.pi * u2), rad * np.sin(2 * np.pi * u2)]
+        return np.array(out[:n])
+
+
+def multi_surface_set(seed=3):
+    """The 8 surfaces of the multi-tube batch: spheres r = 0.5, 1, 1.5, 2; tori (R, r) = (1, 0.3),
+    (1.5, 0.5); two ellipsoids with semi-axes U[0.5, 1.5] from MT19937_64(seed).  Entries
+    (label, surface type, params, bbox)."""
+    g = MT19937_64(seed)
+    out = [(f"sphere r={r}", "Sphere", [r], (-r, r)) for r in (0.5, 1.0, 1.5, 2.0)]
+    out += [(f"torus R={R} r={r}", "Torus", [R, r], (-(R + r), R + r)) for R, r in ((1.0, 0.3), (1.5, 0.5))]
+    for k in range(2):
+        ax = [0.5 + g.uniform() for _ in range(3)]
+        out.append(("ellipsoid " + "x".join(f"{a:.3f}" for a in ax), "Ellipsoid", ax, (-max(ax), max(ax))))
+    return out
+
+
+def sh_coefficients(instance, lmax=8):
+    """a_lm ~ N(0, 1), l <= lmax, |m| <= l, from MT19937_64(1000 + instance)."""
+    return MT19937_64(1000 + instance).normals((lmax + 1) ** 2)
+
+
+def sh_rhs(cp, coeffs, lmax=8, chunk=1 << 17):
+    """f(cp) = sum_lm a_lm Y_lm(cp / |cp|) for the columns of coeffs ((lmax + 1)^2 x k): N x k
+    (chunks on host threads; numpy releases the interpreter lock inside its array operations)."""
+    import os
+    from concurrent.futures import ThreadPoolExecutor
+    coeffs = np.asarray(coeffs, float).reshape((lmax + 1) ** 2, -1)
+    out = np.empty((len(cp), coeffs.shape[1]))
+
+    def one(s):
+        out[s:s + chunk] = real_sph_harm(lmax, cp[s:s + chunk]) @ coeffs
+
+    starts = range(0, len(cp), chunk)
+    if len(starts) <= 1:
+        for s in starts:
+            one(s)
+    else:
+        with ThreadPoolExecutor(max_workers=min(16, os.cpu_count() or 1, len(starts))) as ex:
+            list(ex.map(one, starts))
+    return out

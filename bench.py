@@ -167,6 +167,12 @@ def bench_surfaces(args, ctx, cpmgpu):
     t_run = time.perf_counter() - t0
     assert all(i["status"] == "converged" for i in infos), infos
     n = len(ops)
+    # the same batch in plain FP64 (the reference's arithmetic), one step
+    t0 = time.perf_counter()
+    infos64 = ctx.solve_batch_device(ops, d_rhs, d_x, cpmgpu.opts(tol=1e-10, precision=cpmgpu.CPM_FP64))
+    ctx.synchronize()
+    t64 = time.perf_counter() - t0
+    print(f"fp64 step done ({t64:.1f} s)", file=sys.stderr, flush=True)
     big = max(range(n), key=lambda q: ops[q].D)
     eg_ms, eg_bytes = ops[big].profile_egather(cpmgpu.CPM_FP32_REFINE, 8, reps=20)
     achieved = eg_bytes / (eg_ms * 1e-3) / 1e9
@@ -182,6 +188,9 @@ def bench_surfaces(args, ctx, cpmgpu):
                    "tube_rows_total": rows, "largest_tube_rows": ops[big].D, "tol": 1e-10, "precond": "diagonal",
                    "setup_s": round(setup_s, 1)},
         "iters_mean": round(float(np.mean(iters)), 1), "iters_max": int(max(iters)),
+        "fp64_fallbacks": sum(1 for i in infos if i["fallback_iters"] > 0),
+        "fp64_solves_per_s": round(n / t64, 3), "fp32refine_vs_fp64": round(n * args.steps / t_run / (n / t64), 3),
+        "iters_fp64_mean": round(float(np.mean([i["iters"] for i in infos64])), 1),
         "roofline": dict(bound="hbm", achieved=round(achieved, 1), peak=HBM_PEAK_GBS, unit="GB/s",
                          frac=round(achieved / HBM_PEAK_GBS, 4), traffic=None, kernel="k_egather<float,8>",
                          avg_launch_ms=round(eg_ms, 5), bytes_per_launch=int(eg_bytes),

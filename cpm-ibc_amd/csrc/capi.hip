@@ -3,6 +3,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 #include <map>
 #include <tuple>
@@ -558,6 +559,39 @@ static void run_stream(Ctx* ctx, Op* op, const std::vector<const double*>& rhs, 
     }
 }
 
+// FP32_REFINE systems whose refinement stopped reducing the true residual
+// (ST_FALLBACK) are solved again in FP64 from zero, the reference algorithm,
+// in one batch; their infos keep the FP32 counts and add fallback_iters.
+static void fp64_fallback(Ctx* ctx, const std::vector<Op*>& ops, const std::vector<const double*>& rhs,
+                          const std::vector<double*>& xs, const cpm_solve_opts& o,
+                          std::vector<cpm_solve_info>& infos, const std::vector<std::pair<double, double>>* mn) {
+    std::vector<int> idx;
+    for (size_t i = 0; i < infos.size(); ++i)
+        if (infos[i].status == ST_FALLBACK) idx.push_back((int)i);
+    if (idx.empty()) return;
+    std::vector<Op*> fo;
+    std::vector<const double*> fr;
+    std::vector<double*> fx;
+    std::vector<std::pair<double, double>> fmn;
+    for (int i : idx) {
+        fo.push_back(ops[(size_t)i]);
+        fr.push_back(rhs[(size_t)i]);
+        fx.push_back(xs[(size_t)i]);
+        fmn.push_back(mn ? (*mn)[(size_t)i] : std::make_pair(ops[(size_t)i]->m, ops[(size_t)i]->n));
+    }
+    cpm_solve_opts o64 = o;
+    o64.precision = CPM_FP64;
+    std::vector<cpm_solve_info> fi;
+    solve_batch(ctx, fo, fr, fx, o64, fi, &fmn);
+    for (size_t j = 0; j < idx.size(); ++j) {
+        cpm_solve_info& inf = infos[(size_t)idx[j]];
+        inf.status = fi[j].status;
+        inf.restarts += fi[j].restarts;
+        inf.fallback_iters = fi[j].iters;
+        inf.rel_residual = fi[j].rel_residual;
+    }
+}
+
 void solve_batch(Ctx* ctx, const std::vector<Op*>& ops, const std::vector<const double*>& rhs,
                  const std::vector<double*>& xs, const cpm_solve_opts& o, std::vector<cpm_solve_info>& infos,
                  const std::vector<std::pair<double, double>>* mn) {
@@ -590,6 +624,7 @@ void solve_batch(Ctx* ctx, const std::vector<Op*>& ops, const std::vector<const 
     else if (nr == 2) run_batch<2>(ctx, groups, rhs, xs, o, infos);
     else if (nr <= 4) run_batch<4>(ctx, groups, rhs, xs, o, infos);
     else run_batch<8>(ctx, groups, rhs, xs, o, infos);
+    if (o.precision == CPM_FP32_REFINE) fp64_fallback(ctx, ops, rhs, xs, o, infos, mn);
 }
 
 static void solve_stream(Op* op, const std::vector<const double*>& rhs, const std::vector<double*>& xs, int lanes,
@@ -604,6 +639,7 @@ static void solve_stream(Op* op, const std::vector<const double*>& rhs, const st
     else if (lanes == 4) run_stream<4>(op->ctx, op, rhs, xs, o, infos);
     else if (lanes == 8) run_stream<8>(op->ctx, op, rhs, xs, o, infos);
     else throw ArgError("lanes must be 1, 2, 4 or 8");
+    fp64_fallback(op->ctx, std::vector<Op*>(rhs.size(), op), rhs, xs, o, infos, nullptr);
 }
 
 void multiply(Op* op, const double* dx, double* dy, double m, double n) {

@@ -23,7 +23,10 @@ enum : int {
     // mixed precision: the FP32 recurrence asks for an FP64 refinement step
     // (UPDATE*), the true residual is rescaled into the FP32 vectors
     // (RESCALE), and the lane rejoins at the next iteration (READY)
-    ST_UPDATE = 6, ST_UPDATE_MAXIT = 7, ST_RESCALE = 8, ST_READY = 9
+    ST_UPDATE = 6, ST_UPDATE_MAXIT = 7, ST_RESCALE = 8, ST_READY = 9,
+    // mixed precision: refinement stopped reducing the true residual; the
+    // host re-solves the system in FP64 from zero
+    ST_FALLBACK = 10
 };
 
 struct KState {
@@ -34,7 +37,10 @@ struct KState {
     double maxrr;        // mixed: largest scaled recurrence residual^2 since the last step
     double delta2;       // mixed: refine when rr < delta2 * maxrr
     int iters, restarts, status, max_it;
-    int updates, pad0, pad1, pad2;
+    int updates;
+    int fails;           // mixed: refinement steps that reduced the true residual by less than 10%
+    int judged;          // mixed: the pending step was asked for by the delta test (its reduction is judged)
+    int pad;
 };
 
 enum : int { OP_ELL = 0, OP_TP = 1 };

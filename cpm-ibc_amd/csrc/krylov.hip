@@ -890,10 +890,21 @@ __device__ __forceinline__ void mx_init(KState& s, double rr, int next) {
 // Refinement step: the true residual ||b - A64 x64||^2 is rt.  A lane that
 // took a new right-hand side (bsq < 0, x64 = 0) learns ||b||^2 = rt here, as
 // FIN_MX_START does for the first ones.
+//
+// The FP32 recurrence can only resolve the correction to about eps32 times
+// the condition number; on fine grids (h = 0.01, condition ~1e5) asking it
+// for delta = 1e-2 per step leaves corrections that do not reduce the true
+// residual.  For steps asked for by the delta test (the recurrence residual
+// fell by delta), a true reduction that falls short of the recurrence's by
+// more than ~3x raises delta 10x (to at most 0.5); a step that does not
+// reduce the true residual, or three that reduce it by less than 10%, hand
+// the lane to an FP64 re-solve from zero (ST_FALLBACK).
 __device__ __forceinline__ void mx_update(KState& s, double rt) {
     ++s.updates;
+    const double prev = s.rr_true;
+    const bool fresh = s.bsq < 0.0;
     s.rr_true = rt;
-    if (s.bsq < 0.0) {
+    if (fresh) {
         s.bsq = rt;
         if (!std::isfinite(rt)) { s.status = ST_BREAKDOWN; return; }
         if (rt == 0.0) { s.status = ST_ZERO_RHS; return; }
@@ -901,9 +912,16 @@ __device__ __forceinline__ void mx_update(KState& s, double rt) {
         if (s.max_it <= 0) { s.status = ST_MAXIT; return; }
         s.tol2 *= rt;
     }
-    if (std::isnan(rt)) { s.status = ST_BREAKDOWN; return; }
+    if (!std::isfinite(rt)) { s.status = ST_FALLBACK; return; }
     if (!(rt > s.tol2)) { s.status = ST_CONVERGED; return; }
     if (s.status == ST_UPDATE_MAXIT) { s.status = ST_MAXIT; return; }
+    if (!fresh && s.judged) {
+        const double a = rt / prev;          // squared true reduction of this step
+        const double c = s.rr / s.r0sq;      // squared reduction the recurrence reached
+        if (!(a < 1.0)) { s.status = ST_FALLBACK; return; }
+        if (a > 10.0 * c) s.delta2 = fmin(0.25, 100.0 * s.delta2);
+        if (a > 0.81 && ++s.fails >= 3) { s.status = ST_FALLBACK; return; }
+    }
     s.scale = 1.0 / sqrt(rt);
     s.status = ST_RESCALE;
 }
@@ -967,9 +985,10 @@ __device__ void fin_scalar(KState& s, const double* tot) {
         s.rr = rr;
         if (std::isnan(rr)) { s.status = ST_BREAKDOWN; return; }
         // scaled tolerance: ||r|| <= tol ||b|| with r scaled by `scale`
+        s.judged = 0;
         if (!(rr > s.tol2 * s.scale * s.scale)) { s.status = ST_UPDATE; return; }
         if (s.iters >= s.max_it) { s.status = ST_UPDATE_MAXIT; return; }
-        if (rr < s.delta2 * s.maxrr) { s.status = ST_UPDATE; return; }
+        if (rr < s.delta2 * s.maxrr) { s.judged = 1; s.status = ST_UPDATE; return; }
         s.rho_old = s.rho;
         s.rho = tot[1];
         if (fabs(s.rho) < kEps * kEps * s.r0sq) { ++s.restarts; s.status = ST_UPDATE; return; }   // BiCGStab.h:256

@@ -37,11 +37,11 @@ class SolveOpts(ctypes.Structure):
 
 class SolveInfo(ctypes.Structure):
     _fields_ = [("status", ctypes.c_int), ("iters", ctypes.c_int), ("restarts", ctypes.c_int),
-                ("updates", ctypes.c_int), ("rel_residual", ctypes.c_double)]
+                ("updates", ctypes.c_int), ("rel_residual", ctypes.c_double), ("fallback_iters", ctypes.c_int)]
 
     def as_dict(self):
         return dict(status=STATUS.get(self.status, self.status), iters=self.iters, restarts=self.restarts,
-                    updates=self.updates, rel_residual=self.rel_residual)
+                    updates=self.updates, rel_residual=self.rel_residual, fallback_iters=self.fallback_iters)
 
 
 class RdParams(ctypes.Structure):

@@ -67,6 +67,8 @@ typedef struct cpm_solve_info {
     int restarts;          /* rho-breakdown restarts (src/BiCGStab.h:62-71)       */
     int updates;           /* FP64 refinement steps (FP32_REFINE)                 */
     double rel_residual;   /* ||b-Ax||/||b||: FP64 recurrence residual (FP64), true FP64 residual (FP32_REFINE) */
+    int fallback_iters;    /* FP32_REFINE: FP64 iterations of the re-solve from zero that a system gets when
+                              its refinement steps stop reducing the true residual (0 = none)             */
 } cpm_solve_info;
 
 /* Defaults of the reference: tol 1e-10, max 100000, FP64. */

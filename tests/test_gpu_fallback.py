@@ -1,0 +1,75 @@
+"""FP32+refine on a fine grid: the screened sphere system at h = 0.01 (1.04e6
+rows, condition ~1e5) is one where FP32 BiCGStab steps stop reducing the
+true FP64 residual.  Such systems are re-solved in FP64 from zero, the
+reference algorithm (src/BiCGStab.h:30-108), so the result equals a plain
+FP64 solve; systems that refine normally are not touched."""
+import numpy as np
+import pytest
+
+import cpmgpu
+from cpmgpu import workloads
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def fine_sphere(gpu_ctx):
+    t = cpmgpu.Tube(gpu_ctx, "Sphere", 0.01, params=[1.0], bbox=(-1.0, 1.0))
+    op = t.operator(1.0, -1.0)
+    cp = t.export()["cp"]
+    A = np.stack([workloads.sh_coefficients(k) for k in range(3)], 1)
+    F = workloads.sh_rhs(cp, A)
+    return t, op, [np.ascontiguousarray(F[:, k]) for k in range(3)]
+
+
+def test_fallback_equals_fp64(gpu_ctx, fine_sphere):
+    print("case fallback sphere h=0.01", flush=True)
+    _, op, rhs = fine_sphere
+    x64, i64 = op.solve(rhs[0], cpmgpu.opts(precision=cpmgpu.CPM_FP64))
+    x32, i32 = op.solve(rhs[0], cpmgpu.opts(precision=cpmgpu.CPM_FP32_REFINE))
+    print(i64, i32, flush=True)
+    assert i64["status"] == i32["status"] == "converged"
+    assert i32["fallback_iters"] == i64["iters"] > 0
+    assert np.array_equal(x32, x64)                  # the same FP64 solve from zero
+    assert i32["rel_residual"] == i64["rel_residual"]
+
+
+def test_fallback_in_batches_and_streams(gpu_ctx, fine_sphere):
+    print("case fallback batch + stream sphere h=0.01", flush=True)
+    _, op, rhs = fine_sphere
+    o32 = cpmgpu.opts(precision=cpmgpu.CPM_FP32_REFINE)
+    ref = [op.solve(b, cpmgpu.opts(precision=cpmgpu.CPM_FP64))[0] for b in rhs]
+    xs, infos = gpu_ctx.solve_batch([op] * 3, rhs, o32)
+    for x, r, inf in zip(xs, ref, infos):
+        assert inf["status"] == "converged"
+        if inf["fallback_iters"]:
+            assert np.linalg.norm(x - r) <= 1e-9 * np.linalg.norm(r)
+        else:
+            assert np.linalg.norm(x - r) <= 1e-8 * np.linalg.norm(r)
+    assert any(i["fallback_iters"] for i in infos)
+    D = op.D
+    d_rhs = [gpu_ctx.alloc(8 * D) for _ in rhs]
+    d_x = [gpu_ctx.alloc(8 * D) for _ in rhs]
+    try:
+        for d, b in zip(d_rhs, rhs):
+            gpu_ctx.h2d(d, b)
+        infos = gpu_ctx.solve_stream_device(op, d_rhs, d_x, 2, o32)
+        for d, r, inf in zip(d_x, ref, infos):
+            x = gpu_ctx.d2h(np.zeros(D), d)
+            assert inf["status"] == "converged"
+            assert np.linalg.norm(x - r) <= 1e-8 * np.linalg.norm(r)
+        assert any(i["fallback_iters"] for i in infos)
+    finally:
+        for d in d_rhs + d_x:
+            gpu_ctx.free(d)
+
+
+def test_no_fallback_where_refinement_works(gpu_ctx):
+    print("case no fallback sphere h=0.05", flush=True)
+    t = cpmgpu.Tube(gpu_ctx, "Sphere", 0.05)
+    op = t.operator(1.0, -1.0)
+    b = workloads.sh_rhs(t.export()["cp"], workloads.sh_coefficients(0)[:, None])[:, 0]
+    x32, i32 = op.solve(b, cpmgpu.opts(precision=cpmgpu.CPM_FP32_REFINE))
+    x64, _ = op.solve(b, cpmgpu.opts(tol=1e-13, precision=cpmgpu.CPM_FP64))
+    assert i32["status"] == "converged" and i32["fallback_iters"] == 0 and i32["updates"] >= 1
+    assert np.linalg.norm(x32 - x64) <= 1e-8 * np.linalg.norm(x64)

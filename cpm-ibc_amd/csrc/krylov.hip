@@ -170,7 +170,7 @@ __device__ __forceinline__ const Seg<T>& seg_of(const Seg<T>* __restrict__ segs,
         const int xcd = b % 8, k = b / 8;
         vb = ((k / kXcdRun) * 8 + xcd) * kXcdRun + k % kXcdRun;
     }
-    const Seg<T>& sg = segs[bseg[vb]];
+    const Seg<T>& sg = segs[bseg ? bseg[vb] : 0];   // one segment: no map (a dependent load less)
     lb = vb - sg.block0;
     row = (int64_t)lb * kBlock + threadIdx.x;
     return sg;
@@ -207,6 +207,31 @@ __device__ __forceinline__ bool seg_any(const Seg<T>& sg, unsigned mask) {
     for (int k = 0; k < sg.nr; ++k) a |= ((mask >> ldg(&sg.st[k].status)) & 1u) != 0;
     return a;
 }
+
+// The statuses of a segment's lanes, read once per kernel with all loads in
+// flight together: a group always owns kMaxNR states, so every load is valid
+// (testing lane by lane up to nr waits for each scalar load in turn, several
+// microseconds before the first vector load of a kernel).  nr = 0: no lane.
+struct LaneStat {
+    int s[kMaxNR];
+    int nr;
+    __device__ __forceinline__ unsigned of(unsigned mask) const {   // bit k: lane k's status is in mask
+        unsigned b = 0;
+#pragma unroll
+        for (int k = 0; k < kMaxNR; ++k) b |= ((mask >> s[k]) & 1u) << k;
+        return b & ((1u << nr) - 1u);
+    }
+};
+template <typename T>
+__device__ __forceinline__ LaneStat lane_stat(const Seg<T>& sg, bool valid = true) {
+    LaneStat L;
+#pragma unroll
+    for (int k = 0; k < kMaxNR; ++k) L.s[k] = valid ? ldg(&sg.st[k].status) : 0;
+    L.nr = valid ? sg.nr : 0;
+    return L;
+}
+__device__ __forceinline__ bool lane_in(const LaneStat& L, int k, unsigned mask) { return (L.of(mask) >> k) & 1u; }
+__device__ __forceinline__ bool seg_any(const LaneStat& L, unsigned mask) { return L.of(mask) != 0; }
 
 // Diagonal_Inverse, GeneralizedBiCGStabSystem.h:303-338 (the branch the
 // reference runs: Compute() is never called).
@@ -500,9 +525,10 @@ __global__ __launch_bounds__(kBlock) void k_pupdate(const Seg<T>* __restrict__ s
     int lb;
     int64_t row;
     const Seg<T>& sg = seg_of(segs, bseg, lb, row);
-    if (row >= sg.op.D || !seg_any(sg, M_STAGE1)) return;
+    const LaneStat L = lane_stat(sg);
+    if (row >= sg.op.D || !seg_any(L, M_STAGE1)) return;
     const size_t i = (size_t)(sg.off + row * NR);
-    if (seg_any(sg, M_RUN)) {
+    if (seg_any(L, M_RUN)) {
         const uint8_t f = ldg(&sg.op.flag[row]);
         auto pv = vload<T, NR>(p + i);
         auto vv = vload<T, NR>(v + i);
@@ -513,18 +539,18 @@ __global__ __launch_bounds__(kBlock) void k_pupdate(const Seg<T>* __restrict__ s
             const KState* st = &sg.st[k < sg.nr ? k : 0];
             const T beta = (T)ldg(&st->beta), w = (T)ldg(&st->omega);
             T pk = (-w * lane(vv, k) + lane(pv, k)) * beta + lane(rv, k);   // p.Copy(-w,v,p); p*=beta; p+=r
-            if (!lane_in(sg, k, M_RUN)) pk = lane(pv, k);
+            if (!lane_in(L, k, M_RUN)) pk = lane(pv, k);
             setlane(pv, k, pk);
             setlane(yv, k, precond_apply(sg, row, f, pk));
         }
         vstore<T, NR>(p + i, pv);
         vstore<T, NR>(y + i, yv);
     }
-    if (std::is_same<T, float>::value && seg_any(sg, M_UPDATE)) {
+    if (std::is_same<T, float>::value && seg_any(L, M_UPDATE)) {
         auto xv = vload<T, NR>(x + i);
 #pragma unroll
         for (int k = 0; k < NR; ++k)
-            if (lane_in(sg, k, M_UPDATE)) {
+            if (lane_in(L, k, M_UPDATE)) {
                 double* xd = mv.x64 + lm_at(sg, k, row);
                 stg(xd, ldg(xd) + (double)lane(xv, k) / ldg(&sg.st[k].scale));
                 setlane(xv, k, (T)0);
@@ -549,8 +575,9 @@ __global__ __launch_bounds__(kBlock) void k_egather(const Seg<T>* __restrict__ s
     int lb;
     int64_t row;
     const Seg<T>& sg = seg_of(segs, bseg, lb, row);
-    if (lanes && !seg_any(sg, lanes)) return;               // uniform per workgroup
-    const bool main = !lanes || seg_any(sg, lanes & ~M_UPDATE);
+    const LaneStat L = lane_stat(sg, lanes != 0);
+    if (lanes && !seg_any(L, lanes)) return;               // uniform per workgroup
+    const bool main = !lanes || seg_any(L, lanes & ~M_UPDATE);
     const size_t i = (size_t)(sg.off + row * NR);
     const T* xb = x + sg.off;
     const int nu = (LR > 0 && main && sg.op.fmt == OP_TP && sg.op.tln) ? ldg(&sg.op.tln[lb]) : -1;
@@ -593,9 +620,9 @@ __global__ __launch_bounds__(kBlock) void k_egather(const Seg<T>* __restrict__ s
         vstore<T, NR>(out + i, erow<T, NR>(sg.op, row, xb, NR));
     }
     if (row >= sg.op.D) return;
-    if (std::is_same<T, float>::value && (lanes & M_UPDATE) && seg_any(sg, M_UPDATE)) {
+    if (std::is_same<T, float>::value && (lanes & M_UPDATE) && seg_any(L, M_UPDATE)) {
         for (int k = 0; k < sg.nr; ++k)
-            if (lane_in(sg, k, M_UPDATE))
+            if (lane_in(L, k, M_UPDATE))
                 stg(&mv.tmp64[lm_at(sg, k, row)], erow<double, 1>(sg.op64, row, mv.x64 + lm_at(sg, k, 0), 1));
     }
 }
@@ -611,7 +638,8 @@ __global__ __launch_bounds__(kBlock) void k_lfinish(const Seg<T>* __restrict__ s
     int lb;
     int64_t row;
     const Seg<T>& sg = seg_of(segs, bseg, lb, row);
-    if (lanes && !seg_any(sg, lanes)) return;   // uniform per segment
+    const LaneStat L = lane_stat(sg, lanes != 0);
+    if (lanes && !seg_any(L, lanes)) return;   // uniform per segment
     constexpr int NV = RED == 2 ? 2 : 1;
     double pv[NV][NR];
 #pragma unroll
@@ -620,7 +648,7 @@ __global__ __launch_bounds__(kBlock) void k_lfinish(const Seg<T>* __restrict__ s
         for (int k = 0; k < NR; ++k) pv[a][k] = 0.0;
     if (row < sg.op.D) {
         const size_t i = (size_t)(sg.off + row * NR);
-        if (!lanes || seg_any(sg, lanes & ~(M_UPDATE | M_RESCALE))) {
+        if (!lanes || seg_any(L, lanes & ~(M_UPDATE | M_RESCALE))) {
             const auto res = arow<T, NR>(sg.op, sg.m, sg.n, row, vload<T, NR>(x + i), tmp + sg.off, NR);
             vstore<T, NR>(out + i, res);
             if (RED > 0) {
@@ -637,10 +665,10 @@ __global__ __launch_bounds__(kBlock) void k_lfinish(const Seg<T>* __restrict__ s
                 }
             }
         }
-        if (std::is_same<T, float>::value && RED == 1 && seg_any(sg, M_UPDATE)) {
+        if (std::is_same<T, float>::value && RED == 1 && seg_any(L, M_UPDATE)) {
 #pragma unroll
             for (int k = 0; k < NR; ++k)   // constant indices keep pv in registers
-                if (k < sg.nr && lane_in(sg, k, M_UPDATE)) {
+                if (k < sg.nr && lane_in(L, k, M_UPDATE)) {
                     const size_t j = lm_at(sg, k, row);
                     const double ax =
                         arow<double, 1>(sg.op64, sg.m, sg.n, row, ldg(&mv.x64[j]), mv.tmp64 + lm_at(sg, k, 0), 1);
@@ -649,11 +677,11 @@ __global__ __launch_bounds__(kBlock) void k_lfinish(const Seg<T>* __restrict__ s
                     pv[0][k] = rk * rk;
                 }
         }
-        if (std::is_same<T, float>::value && RED == 2 && seg_any(sg, M_RESCALE)) {
+        if (std::is_same<T, float>::value && RED == 2 && seg_any(L, M_RESCALE)) {
             auto rr = vload<T, NR>(rv32 + i);
 #pragma unroll
             for (int k = 0; k < NR; ++k)
-                if (lane_in(sg, k, M_RESCALE)) {
+                if (lane_in(L, k, M_RESCALE)) {
                     const double rk = (double)lane(rr, k);
                     pv[0][k] = rk * rk;
                     pv[NV - 1][k] = 0.0;
@@ -671,12 +699,13 @@ __global__ __launch_bounds__(kBlock) void k_supdate(const Seg<T>* __restrict__ s
     int lb;
     int64_t row;
     const Seg<T>& sg = seg_of(segs, bseg, lb, row);
-    if (row >= sg.op.D || !seg_any(sg, M_STAGE2)) return;
+    const LaneStat L = lane_stat(sg);
+    if (row >= sg.op.D || !seg_any(L, M_STAGE2)) return;
     const size_t i = (size_t)(sg.off + row * NR);
     const uint8_t f = ldg(&sg.op.flag[row]);
     auto rv = vload<T, NR>(r + i);
     auto vv = vload<T, NR>(v + i);
-    if (seg_any(sg, M_RUN)) {
+    if (seg_any(L, M_RUN)) {
         typename VT<T, NR>::type sv, zv;
 #pragma unroll
         for (int k = 0; k < NR; ++k) {
@@ -688,12 +717,12 @@ __global__ __launch_bounds__(kBlock) void k_supdate(const Seg<T>* __restrict__ s
         vstore<T, NR>(s + i, sv);
         vstore<T, NR>(z + i, zv);
     }
-    if (std::is_same<T, float>::value && seg_any(sg, M_RESCALE)) {
+    if (std::is_same<T, float>::value && seg_any(L, M_RESCALE)) {
         auto r0v = vload<T, NR>(r0 + i);
         auto pv = vload<T, NR>(p + i);
 #pragma unroll
         for (int k = 0; k < NR; ++k)
-            if (lane_in(sg, k, M_RESCALE)) {
+            if (lane_in(L, k, M_RESCALE)) {
                 const T rk = (T)(ldg(&mv.r64[lm_at(sg, k, row)]) * ldg(&sg.st[k].scale));
                 setlane(rv, k, rk);
                 setlane(r0v, k, rk);
@@ -715,8 +744,9 @@ __global__ __launch_bounds__(kBlock) void k_jacobi(const Seg<T>* __restrict__ se
     int lb;
     int64_t row;
     const Seg<T>& sg = seg_of(segs, bseg, lb, row);
+    const LaneStat L = lane_stat(sg);
     if (sg.precond != 2 || sweep >= sg.precond_iters) return;
-    if (row >= sg.op.D - sg.op.nD || !seg_any(sg, M_RUN)) return;
+    if (row >= sg.op.D - sg.op.nD || !seg_any(L, M_RUN)) return;
     const size_t i = (size_t)(sg.off + row * NR);
     const uint8_t f = ldg(&sg.op.flag[row]);
     auto xv = vload<T, NR>(x + i);
@@ -724,7 +754,7 @@ __global__ __launch_bounds__(kBlock) void k_jacobi(const Seg<T>* __restrict__ se
     auto yv = vload<T, NR>(y + i);
 #pragma unroll
     for (int k = 0; k < NR; ++k)
-        if (lane_in(sg, k, M_RUN))
+        if (lane_in(L, k, M_RUN))
             setlane(yv, k, lane(yv, k) + (T)kJacobiOmega * diag_inverse(sg, row, f, (T)(lane(xv, k) - lane(av, k))));
     vstore<T, NR>(y + i, yv);
 }
@@ -735,7 +765,8 @@ __global__ __launch_bounds__(kBlock) void k_xrupdate(const Seg<T>* __restrict__ 
     int lb;
     int64_t row;
     const Seg<T>& sg = seg_of(segs, bseg, lb, row);
-    if (!seg_any(sg, M_RUN)) return;
+    const LaneStat L = lane_stat(sg);
+    if (!seg_any(L, M_RUN)) return;
     double pv[2][NR];
 #pragma unroll
     for (int k = 0; k < NR; ++k) { pv[0][k] = 0.0; pv[1][k] = 0.0; }
@@ -750,7 +781,7 @@ __global__ __launch_bounds__(kBlock) void k_xrupdate(const Seg<T>* __restrict__ 
         auto rv = vload<T, NR>(r + i);
 #pragma unroll
         for (int k = 0; k < NR; ++k) {
-            if (!lane_in(sg, k, M_RUN)) continue;
+            if (!lane_in(L, k, M_RUN)) continue;
             const T a = (T)ldg(&sg.st[k].alpha), w = (T)ldg(&sg.st[k].omega);
             T xk = a * lane(yv, k) + lane(xv, k);   // x.Copy(alpha, y, x)
             xk = w * lane(zv, k) + xk;               // x.Copy(w, z, x)
@@ -773,7 +804,8 @@ __global__ __launch_bounds__(kBlock) void k_restart(const Seg<T>* __restrict__ s
     int lb;
     int64_t row;
     const Seg<T>& sg = seg_of(segs, bseg, lb, row);
-    if (!seg_any(sg, M_RESTART)) return;
+    const LaneStat L = lane_stat(sg);
+    if (!seg_any(L, M_RESTART)) return;
     double pv[1][NR];
 #pragma unroll
     for (int k = 0; k < NR; ++k) pv[0][k] = 0.0;
@@ -785,7 +817,7 @@ __global__ __launch_bounds__(kBlock) void k_restart(const Seg<T>* __restrict__ s
         auto r0v = vload<T, NR>(r0 + i);
 #pragma unroll
         for (int k = 0; k < NR; ++k) {
-            if (!lane_in(sg, k, M_RESTART)) continue;
+            if (!lane_in(L, k, M_RESTART)) continue;
             T rk = lane(bv, k) - lane(av, k);
             setlane(rv, k, rk);
             setlane(r0v, k, rk);
@@ -829,7 +861,8 @@ __global__ __launch_bounds__(kBlock) void k_mx_down(const Seg<double>* __restric
     int lb;
     int64_t row;
     const Seg<double>& sg = seg_of(segs, bseg, lb, row);
-    if (!seg_any(sg, M_RESCALE)) return;
+    const LaneStat L = lane_stat(sg);
+    if (!seg_any(L, M_RESCALE)) return;
     double pv[1][NR];
 #pragma unroll
     for (int k = 0; k < NR; ++k) pv[0][k] = 0.0;
@@ -842,7 +875,7 @@ __global__ __launch_bounds__(kBlock) void k_mx_down(const Seg<double>* __restric
         auto xo = vload<float, NR>(x + i);
 #pragma unroll
         for (int k = 0; k < NR; ++k) {
-            if (!lane_in(sg, k, M_RESCALE)) continue;
+            if (!lane_in(L, k, M_RESCALE)) continue;
             const float rk = (float)(r64[lm_at(sg, k, row)] * ldg(&sg.st[k].scale));
             setlane(ro, k, rk);
             setlane(r0o, k, rk);
@@ -1063,7 +1096,8 @@ void launch_reset_states(KState* st, int n, double tol2, int max_it, double delt
 }
 
 // ---------------------------------------------------------------- launchers
-#define LAUNCH(kern, ...) hipLaunchKernelGGL(kern, dim3(c.nblocks), dim3(kBlock), 0, c.stream, c.segs, c.bseg, __VA_ARGS__)
+#define LAUNCH(kern, ...) \
+    hipLaunchKernelGGL(kern, dim3(c.nblocks), dim3(kBlock), 0, c.stream, c.segs, c.nseg == 1 ? nullptr : c.bseg, __VA_ARGS__)
 #define EGATHER(...) LAUNCH((k_egather<T, NR>), __VA_ARGS__)
 #define FIN(T, NR, KIND) hipLaunchKernelGGL((k_fin<T, NR, KIND>), dim3(c.nseg * NR), dim3(kBlock), 0, c.stream, c.segs, c.nseg)
 

@@ -212,26 +212,29 @@ __device__ __forceinline__ bool seg_any(const Seg<T>& sg, unsigned mask) {
 // flight together: a group always owns kMaxNR states, so every load is valid
 // (testing lane by lane up to nr waits for each scalar load in turn, several
 // microseconds before the first vector load of a kernel).  nr = 0: no lane.
+template <int NR>
 struct LaneStat {
-    int s[kMaxNR];
+    int s[NR];
     int nr;
     __device__ __forceinline__ unsigned of(unsigned mask) const {   // bit k: lane k's status is in mask
         unsigned b = 0;
 #pragma unroll
-        for (int k = 0; k < kMaxNR; ++k) b |= ((mask >> s[k]) & 1u) << k;
+        for (int k = 0; k < NR; ++k) b |= ((mask >> s[k]) & 1u) << k;
         return b & ((1u << nr) - 1u);
     }
 };
-template <typename T>
-__device__ __forceinline__ LaneStat lane_stat(const Seg<T>& sg, bool valid = true) {
-    LaneStat L;
+template <int NR, typename T>
+__device__ __forceinline__ LaneStat<NR> lane_stat(const Seg<T>& sg, bool valid = true) {
+    LaneStat<NR> L;
 #pragma unroll
-    for (int k = 0; k < kMaxNR; ++k) L.s[k] = valid ? ldg(&sg.st[k].status) : 0;
+    for (int k = 0; k < NR; ++k) L.s[k] = valid ? ldg(&sg.st[k].status) : 0;
     L.nr = valid ? sg.nr : 0;
     return L;
 }
-__device__ __forceinline__ bool lane_in(const LaneStat& L, int k, unsigned mask) { return (L.of(mask) >> k) & 1u; }
-__device__ __forceinline__ bool seg_any(const LaneStat& L, unsigned mask) { return L.of(mask) != 0; }
+template <int NR>
+__device__ __forceinline__ bool lane_in(const LaneStat<NR>& L, int k, unsigned mask) { return (L.of(mask) >> k) & 1u; }
+template <int NR>
+__device__ __forceinline__ bool seg_any(const LaneStat<NR>& L, unsigned mask) { return L.of(mask) != 0; }
 
 // Diagonal_Inverse, GeneralizedBiCGStabSystem.h:303-338 (the branch the
 // reference runs: Compute() is never called).
@@ -525,7 +528,7 @@ __global__ __launch_bounds__(kBlock) void k_pupdate(const Seg<T>* __restrict__ s
     int lb;
     int64_t row;
     const Seg<T>& sg = seg_of(segs, bseg, lb, row);
-    const LaneStat L = lane_stat(sg);
+    const auto L = lane_stat<NR>(sg);
     if (row >= sg.op.D || !seg_any(L, M_STAGE1)) return;
     const size_t i = (size_t)(sg.off + row * NR);
     if (seg_any(L, M_RUN)) {
@@ -575,7 +578,7 @@ __global__ __launch_bounds__(kBlock) void k_egather(const Seg<T>* __restrict__ s
     int lb;
     int64_t row;
     const Seg<T>& sg = seg_of(segs, bseg, lb, row);
-    const LaneStat L = lane_stat(sg, lanes != 0);
+    const auto L = lane_stat<NR>(sg, lanes != 0);
     if (lanes && !seg_any(L, lanes)) return;               // uniform per workgroup
     const bool main = !lanes || seg_any(L, lanes & ~M_UPDATE);
     const size_t i = (size_t)(sg.off + row * NR);
@@ -638,7 +641,7 @@ __global__ __launch_bounds__(kBlock) void k_lfinish(const Seg<T>* __restrict__ s
     int lb;
     int64_t row;
     const Seg<T>& sg = seg_of(segs, bseg, lb, row);
-    const LaneStat L = lane_stat(sg, lanes != 0);
+    const auto L = lane_stat<NR>(sg, lanes != 0);
     if (lanes && !seg_any(L, lanes)) return;   // uniform per segment
     constexpr int NV = RED == 2 ? 2 : 1;
     double pv[NV][NR];
@@ -699,7 +702,7 @@ __global__ __launch_bounds__(kBlock) void k_supdate(const Seg<T>* __restrict__ s
     int lb;
     int64_t row;
     const Seg<T>& sg = seg_of(segs, bseg, lb, row);
-    const LaneStat L = lane_stat(sg);
+    const auto L = lane_stat<NR>(sg);
     if (row >= sg.op.D || !seg_any(L, M_STAGE2)) return;
     const size_t i = (size_t)(sg.off + row * NR);
     const uint8_t f = ldg(&sg.op.flag[row]);
@@ -744,7 +747,7 @@ __global__ __launch_bounds__(kBlock) void k_jacobi(const Seg<T>* __restrict__ se
     int lb;
     int64_t row;
     const Seg<T>& sg = seg_of(segs, bseg, lb, row);
-    const LaneStat L = lane_stat(sg);
+    const auto L = lane_stat<NR>(sg);
     if (sg.precond != 2 || sweep >= sg.precond_iters) return;
     if (row >= sg.op.D - sg.op.nD || !seg_any(L, M_RUN)) return;
     const size_t i = (size_t)(sg.off + row * NR);
@@ -765,7 +768,7 @@ __global__ __launch_bounds__(kBlock) void k_xrupdate(const Seg<T>* __restrict__ 
     int lb;
     int64_t row;
     const Seg<T>& sg = seg_of(segs, bseg, lb, row);
-    const LaneStat L = lane_stat(sg);
+    const auto L = lane_stat<NR>(sg);
     if (!seg_any(L, M_RUN)) return;
     double pv[2][NR];
 #pragma unroll
@@ -804,7 +807,7 @@ __global__ __launch_bounds__(kBlock) void k_restart(const Seg<T>* __restrict__ s
     int lb;
     int64_t row;
     const Seg<T>& sg = seg_of(segs, bseg, lb, row);
-    const LaneStat L = lane_stat(sg);
+    const auto L = lane_stat<NR>(sg);
     if (!seg_any(L, M_RESTART)) return;
     double pv[1][NR];
 #pragma unroll
@@ -861,7 +864,7 @@ __global__ __launch_bounds__(kBlock) void k_mx_down(const Seg<double>* __restric
     int lb;
     int64_t row;
     const Seg<double>& sg = seg_of(segs, bseg, lb, row);
-    const LaneStat L = lane_stat(sg);
+    const auto L = lane_stat<NR>(sg);
     if (!seg_any(L, M_RESCALE)) return;
     double pv[1][NR];
 #pragma unroll

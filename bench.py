@@ -2,7 +2,7 @@
 FP64 refinement vs plain FP64, with p50/p99 latency, the E-gather roofline and
 the CPU oracle timed on the host.
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload torus-knot|sphere|surfaces]
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload torus-knot|sphere|surfaces|serving]
 
 A step is one pass of the hot path over one batch: `--batch` right-hand sides
 on the workload's operator solved to a 1e-10 relative residual with FP32
@@ -201,6 +201,99 @@ def bench_surfaces(args, ctx, cpmgpu):
     ctx.close()
 
 
+def bench_serving(args, ctx, cpmgpu):
+    """Serving loop: displaced icosphere (6 subdivisions, 81,920 triangles, radial displacement of
+    amplitude 0.1 from degree <= 6 harmonics, seed 4), h = 0.01; tube and BVH stay resident.  Requests
+    arrive as a Poisson process at --qps (exponential gaps, numpy seed 44); each carries 1..8 closed
+    curves (plane sections of the mesh, 256 samples) with a random RGB colour as 2nd-order Dirichlet
+    data, i.e. 3 right-hand sides of Laplace's equation.  On arrival the request's IBC problem and
+    operator are built on the GPU; it then joins the running FP32+refine iteration (cpm_server) as soon
+    as pool memory allows and leaves when its systems stop.  Latency = completion - arrival."""
+    from cpmgpu import workloads
+    h = 0.01
+    V, F = workloads.displaced_icosphere(6, seed=4)
+    t0 = time.perf_counter()
+    tube = cpmgpu.Tube(ctx, "Triangulation", h, verts=V, faces=F)
+    print(f"mesh tube: {tube.N} nodes ({time.perf_counter() - t0:.1f} s)", file=sys.stderr, flush=True)
+    n = args.requests
+    gaps = np.random.default_rng(44).exponential(1.0 / args.qps, n)
+    arrive = np.cumsum(gaps) - gaps[0]
+    payload = []
+    for i in range(n):
+        payload.append(workloads.serving_request(V, F, 4400 + i))
+    print(f"{n} requests generated ({sum(len(c) for c, _ in payload)} curves)", file=sys.stderr, flush=True)
+    srv = cpmgpu.Server(ctx, int(args.pool_gb * (1 << 30)), lanes=4,
+                        o=cpmgpu.opts(tol=1e-10, precision=cpmgpu.CPM_FP32_REFINE, update_delta=args.update_delta))
+    live, waiting, rec = {}, [], {}
+    nxt = 0
+    start = time.perf_counter()
+    while len(rec) < n:
+        now = time.perf_counter() - start
+        while nxt < n and arrive[nxt] <= now:             # arrivals: build the problem on the GPU
+            tb = time.perf_counter()
+            curves, cols = payload[nxt]
+            P = cpmgpu.Problem(tube, [cpmgpu.IBC("Polyline", verts=c, boundary_order=2, on_ibc_tolerance=0.1 * h * h)
+                                      for c in curves])
+            op = P.operator(0.0, -1.0)
+            dr = [ctx.alloc(8 * P.D) for _ in range(3)]
+            dx = [ctx.alloc(8 * P.D) for _ in range(3)]
+            for ch in range(3):                            # each curve's Dirichlet rows take its colour
+                b = np.zeros(P.D)
+                for c in range(len(curves)):
+                    _, _, r0, nr = P.curve_rows(c)
+                    b[r0:r0 + nr] = cols[c, ch]
+                ctx.h2d(dr[ch], b)
+            live[nxt] = (P, op, dr, dx)
+            build_ms = 1000 * (time.perf_counter() - tb)
+            waiting.append((nxt, build_ms))
+            nxt += 1
+        while waiting:                                     # admission by pool memory
+            i, bms = waiting[0]
+            if not srv.submit(live[i][1], live[i][2], live[i][3], i):
+                break
+            waiting.pop(0)
+            live[i] = live[i] + (bms, time.perf_counter() - start)
+        done, active = srv.run(16)
+        for tag, infos in done:
+            t_done = time.perf_counter() - start
+            P, op, dr, dx, bms, t_adm = live.pop(tag)
+            rec[tag] = dict(latency_ms=1000 * (t_done - arrive[tag]), build_ms=bms, rows=P.D,
+                            iters=max(j["iters"] for j in infos),
+                            fallback=sum(1 for j in infos if j["fallback_iters"] > 0),
+                            ok=all(j["status"] == "converged" for j in infos))
+            print(f"request {tag}: {len(payload[tag][0])} curves, {P.D} rows, latency {rec[tag]['latency_ms']:.0f} ms",
+                  file=sys.stderr, flush=True)
+            for d in dr + dx:
+                ctx.free(d)
+            op.close()
+            P.close()
+        if active == 0 and not waiting and nxt < n:
+            time.sleep(max(0.0, arrive[nxt] - (time.perf_counter() - start)))
+    makespan = time.perf_counter() - start
+    srv.close()
+    lat = sorted(r["latency_ms"] for r in rec.values())
+    p99 = lat[min(len(lat) - 1, int(math.ceil(0.99 * len(lat))) - 1)]
+    out = {
+        "metric": "serving: solves/s with p50/p99 request latency, FP32+refine at 1e-10",
+        "value": round(3 * n / makespan, 3), "unit": "solves/s", "n_gpus": 1, "steps": n, "warmup": 0,
+        "ms_per_step": round(1000 * makespan / n, 1), "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f32 Krylov + f64 refinement",
+        "data": "synthetic (seeded displaced icosphere mesh, plane-section curves, random RGB)",
+        "config": {"workload": "serving loop: displaced icosphere 81,920 triangles, h=0.01, Poisson arrivals, "
+                               "1-8 curves x 3 RHS per request", "tube_nodes": tube.N, "requests": n,
+                   "qps": args.qps, "pool_gb": args.pool_gb, "lanes": 4},
+        "p50_ms": round(statistics.median(lat), 1), "p99_ms": round(p99, 1),
+        "build_ms_mean": round(float(np.mean([r["build_ms"] for r in rec.values()])), 1),
+        "rows_mean": int(np.mean([r["rows"] for r in rec.values()])),
+        "iters_mean": round(float(np.mean([r["iters"] for r in rec.values()])), 1),
+        "fp64_fallbacks": int(sum(r["fallback"] for r in rec.values())),
+        "all_converged": all(r["ok"] for r in rec.values()),
+        "roofline": None, "cpu_baseline": None,
+    }
+    print(json.dumps(out))
+    ctx.close()
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -208,7 +301,10 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--batch", type=int, default=8, help="interleaved lanes (1, 2, 4 or 8)")
     ap.add_argument("--stream", type=int, default=32, help="right-hand sides streamed through the lanes per step")
-    ap.add_argument("--workload", default="torus-knot", choices=["torus-knot", "sphere", "surfaces"])
+    ap.add_argument("--workload", default="torus-knot", choices=["torus-knot", "sphere", "surfaces", "serving"])
+    ap.add_argument("--requests", type=int, default=16, help="serving: requests")
+    ap.add_argument("--qps", type=float, default=1.0, help="serving: mean arrival rate")
+    ap.add_argument("--pool-gb", type=float, default=8.0, help="serving: workspace pool")
     ap.add_argument("--cpu-iters", type=int, default=100)
     ap.add_argument("--update-delta", type=float, default=1e-2)
     ap.add_argument("--latency-samples", type=int, default=30)
@@ -223,6 +319,8 @@ def main():
     ctx = cpmgpu.Context(0)
     if args.workload == "surfaces":
         return bench_surfaces(args, ctx, cpmgpu)
+    if args.workload == "serving":
+        return bench_serving(args, ctx, cpmgpu)
     t0 = time.perf_counter()
     S = max(args.stream, args.batch)
     if args.workload == "sphere":

@@ -5,6 +5,7 @@
 #include <cmath>
 #include <cstdlib>
 #include <cstring>
+#include <deque>
 #include <map>
 #include <tuple>
 #include <mutex>
@@ -963,6 +964,260 @@ static void profile_copy(Ctx* ctx, int64_t bytes, int reps, double* avg_ms) {
     *avg_ms = ms / reps;
 }
 
+// ---------------------------------------------------------------- serving
+// Continuous batching across operators: every admitted request (an operator
+// and up to NR right-hand sides) is one segment of the FP32+refine iteration;
+// requests join and leave between chunks of iterations.  Workspace vectors
+// live in pools of `cap` elements per vector, handed out as ranges (first
+// fit); a request that does not fit waits (admission by device memory).  A
+// lane's arithmetic and its reductions depend only on its own segment, so a
+// request's results are the same alone and in any mix.
+struct ServerBase {
+    virtual ~ServerBase() = default;
+    virtual bool submit(Op* op, int nrhs, const double* const* rhs, double* const* xs, int64_t tag) = 0;
+    virtual void run(int iters) = 0;
+    virtual int active() const = 0;
+    struct Done {
+        int64_t tag;
+        std::vector<cpm_solve_info> infos;
+    };
+    std::deque<Done> done;
+    int lanes = 0;
+};
+
+template <int NR>
+struct Server : ServerBase {
+    struct Req {
+        int64_t tag;
+        Op* op;
+        int nr;
+        int64_t off, len;
+        int slot;
+        std::vector<const double*> rhs;
+        std::vector<double*> xs;
+    };
+    Ctx* ctx;
+    cpm_solve_opts o;
+    int64_t cap;
+    Buffer p32, p64, states, tables;
+    int slots;
+    std::vector<int> free_slots;
+    std::map<int64_t, int64_t> free_ranges;   // start -> length
+    std::vector<Req> reqs;
+    bool dirty = true;
+    int nblocks = 0;
+    Seg<float>* d_segs = nullptr;
+    int* d_bseg = nullptr;
+    double* d_part = nullptr;
+    PackEntry* d_pack = nullptr;
+    int* d_zmap = nullptr;   // all-zero block map of single-entry pack launches
+    size_t tables_bytes = 0;
+
+    Server(Ctx* c, int64_t pool_bytes, const cpm_solve_opts& opts) : ctx(c), o(opts) {
+        lanes = NR;
+        const int64_t per_elem = 10 * 4 + 5 * 8;   // FP32 x r r0 p v s t y z tmp, FP64 b x r t tmp
+        cap = (pool_bytes / per_elem) & ~int64_t(63);
+        if (cap < 64) throw ArgError("server pool too small");
+        p32 = Buffer(sizeof(float) * (size_t)cap * 10);
+        p64 = Buffer(sizeof(double) * (size_t)cap * 5);
+        slots = Ctx::kHostStates / kMaxNR;
+        states = Buffer(sizeof(KState) * (size_t)slots * kMaxNR);
+        for (int i = slots - 1; i >= 0; --i) free_slots.push_back(i);
+        free_ranges[0] = cap;
+    }
+    float* v32(int k) const { return p32.template as<float>() + (size_t)k * cap; }
+    double* v64(int k) const { return p64.template as<double>() + (size_t)k * cap; }
+    int active() const override { return (int)reqs.size(); }
+
+    bool take_range(int64_t len, int64_t& off) {
+        for (auto it = free_ranges.begin(); it != free_ranges.end(); ++it)
+            if (it->second >= len) {
+                off = it->first;
+                const int64_t rest = it->second - len;
+                free_ranges.erase(it);
+                if (rest > 0) free_ranges[off + len] = rest;
+                return true;
+            }
+        return false;
+    }
+    void give_range(int64_t off, int64_t len) {
+        auto it = free_ranges.emplace(off, len).first;
+        auto nx = std::next(it);
+        if (nx != free_ranges.end() && it->first + it->second == nx->first) {
+            it->second += nx->second;
+            free_ranges.erase(nx);
+        }
+        if (it != free_ranges.begin()) {
+            auto pv = std::prev(it);
+            if (pv->first + pv->second == it->first) {
+                pv->second += it->second;
+                free_ranges.erase(it);
+            }
+        }
+    }
+    // one pack launch per lane: b64 (lane-major) from the caller, or x64 to the caller
+    void lane_pack(const Req& q, int k, double* ws64, int to_ws) {
+        const hipStream_t st = ctx->stream;
+        PackEntry e{};
+        e.src = to_ws ? (k < q.nr ? q.rhs[(size_t)k] : nullptr) : nullptr;
+        e.dst = to_ws ? nullptr : (k < q.nr ? q.xs[(size_t)k] : nullptr);
+        e.perm = q.op->perm.template as<int>();
+        e.D = q.op->D;
+        e.off = q.off;
+        e.lane = k;
+        e.nr = 1;
+        e.ls = q.op->D;
+        e.block0 = 0;
+        ensure_tables();
+        HIPCHK(hipMemcpyAsync(d_pack, &e, sizeof(e), hipMemcpyHostToDevice, st));
+        const int nb = (int)((q.op->D + kBlock - 1) / kBlock);
+        hipLaunchKernelGGL(k_pack, dim3((unsigned)nb), dim3(256), 0, st, d_pack, d_zmap, ws64, (float*)nullptr, to_ws);
+        HIPCHK(hipStreamSynchronize(st));   // the entry is rewritten by the next call
+    }
+    // Device tables sized for the current requests: segment table, block map,
+    // partial sums, one pack entry and an all-zero block map for packing.
+    // Growing them reallocates (and marks the segment table for rebuilding).
+    void ensure_tables() {
+        int blocks = 0;
+        for (auto& q : reqs) blocks += (int)((q.op->D + kBlock - 1) / kBlock);
+        const int zn = (int)((cap + kBlock - 1) / kBlock);   // no request has more blocks
+        auto r256 = [](size_t k) { return (k + 255) & ~size_t(255); };
+        const size_t sseg = r256(sizeof(Seg<float>) * std::max<size_t>(reqs.size(), 1));
+        const size_t smap = r256(sizeof(int) * std::max(blocks, 1));
+        const size_t spart = r256(sizeof(double) * 2 * NR * std::max(blocks, 1));
+        const size_t spack = r256(sizeof(PackEntry));
+        const size_t szero = r256(sizeof(int) * zn);
+        const size_t need = sseg + smap + spart + spack + szero;
+        char* t;
+        if (need > tables_bytes) {
+            HIPCHK(hipStreamSynchronize(ctx->stream));
+            tables = Buffer(need * 2);   // room to grow
+            tables_bytes = need * 2;
+            dirty = true;
+            t = tables.template as<char>();
+            HIPCHK(hipMemsetAsync(t, 0, szero, ctx->stream));
+        }
+        t = tables.template as<char>();   // fixed-size parts first: zero map, pack entry
+        d_zmap = reinterpret_cast<int*>(t);
+        t += szero;
+        d_pack = reinterpret_cast<PackEntry*>(t);
+        t += spack;
+        d_segs = reinterpret_cast<Seg<float>*>(t);
+        t += sseg;
+        d_bseg = reinterpret_cast<int*>(t);
+        t += smap;
+        d_part = reinterpret_cast<double*>(t);
+    }
+
+    bool submit(Op* op, int nrhs, const double* const* rhs, double* const* xs, int64_t tag) override {
+        if (nrhs < 1 || nrhs > NR) throw ArgError("a request carries 1.." + std::to_string(NR) + " right-hand sides");
+        if (op->ctx != ctx) throw ArgError("operator belongs to another context");
+        if (op->nD != 0) throw ArgError("served operators have no trailing nD rows");
+        if (op->precond == CPM_PRECOND_JACOBI) throw ArgError("served operators use the diagonal or no preconditioner");
+        const int64_t len = (op->D * NR + 63) & ~int64_t(63);
+        if (len > cap) throw ArgError("request larger than the server pool");
+        int64_t off;
+        if (free_slots.empty() || !take_range(len, off)) return false;   // wait for memory
+        Req q{tag, op, nrhs, off, len, free_slots.back(), {}, {}};
+        free_slots.pop_back();
+        q.rhs.assign(rhs, rhs + nrhs);
+        q.xs.assign(xs, xs + nrhs);
+        const hipStream_t st = ctx->stream;
+        // x64 = x32 = 0 over the range, b64 lanes from the caller (idle lanes 0)
+        HIPCHK(hipMemsetAsync(v64(1) + off, 0, sizeof(double) * len, st));
+        HIPCHK(hipMemsetAsync(v32(0) + off, 0, sizeof(float) * len, st));
+        reqs.push_back(q);
+        dirty = true;
+        for (int k = 0; k < NR; ++k) lane_pack(reqs.back(), k, v64(0), 1);
+        // fresh lanes enter the refinement step (||b - A 0||^2 = ||b||^2), idle lanes stay stopped
+        const double delta = o.update_delta > 0 ? std::min(o.update_delta, 0.999) : kDefaultUpdateDelta;
+        std::vector<KState> hs(kMaxNR);
+        for (int k = 0; k < kMaxNR; ++k) {
+            KState z{};
+            z.tol2 = o.tol * o.tol;
+            z.max_it = o.max_iters;
+            z.delta2 = delta * delta;
+            if (k < nrhs) {
+                z.status = ST_UPDATE;
+                z.bsq = -1.0;
+                z.updates = -1;
+                z.scale = 1.0;
+            } else {
+                z.status = ST_CONVERGED;
+            }
+            hs[(size_t)k] = z;
+        }
+        HIPCHK(hipMemcpyAsync(states.template as<KState>() + (size_t)q.slot * kMaxNR, hs.data(), sizeof(KState) * kMaxNR,
+                              hipMemcpyHostToDevice, st));
+        HIPCHK(hipStreamSynchronize(st));
+        return true;
+    }
+
+    void rebuild() {
+        ensure_tables();
+        std::vector<Seg<float>> h(reqs.size());
+        std::vector<int> map;
+        int b0 = 0;
+        for (size_t g = 0; g < reqs.size(); ++g) {
+            const Req& q = reqs[g];
+            const int nb = (int)((q.op->D + kBlock - 1) / kBlock);
+            fill_seg(h[g], q.op, q.off, b0, nb, q.nr, states.template as<KState>() + (size_t)q.slot * kMaxNR,
+                     d_part + (size_t)2 * NR * b0);
+            for (int b = 0; b < nb; ++b) map.push_back((int)g);
+            b0 += nb;
+        }
+        nblocks = b0;
+        const hipStream_t st = ctx->stream;
+        HIPCHK(hipMemcpyAsync(d_segs, h.data(), sizeof(Seg<float>) * h.size(), hipMemcpyHostToDevice, st));
+        HIPCHK(hipMemcpyAsync(d_bseg, map.data(), sizeof(int) * map.size(), hipMemcpyHostToDevice, st));
+        HIPCHK(hipStreamSynchronize(st));
+        dirty = false;
+    }
+
+    void run(int iters) override {
+        if (reqs.empty()) return;
+        if (dirty) rebuild();
+        const hipStream_t st = ctx->stream;
+        const Vecs<float> V32{nullptr, v32(0), v32(1), v32(2), v32(3), v32(4), v32(5), v32(6), v32(7), v32(8), v32(9)};
+        const MixVec mv{v64(1), v64(2), v64(0), v64(4)};
+        const LaunchCfg<float> c{d_segs, d_bseg, nblocks, (int)reqs.size(), st, 0};
+        Vecs<float> w = V32;
+        for (int i = 0; i < iters; ++i) Launch<float, NR>::iteration(c, w, true, mv);
+        HIPCHK(hipGetLastError());
+        KState* hs = ctx->host_states;
+        HIPCHK(hipMemcpyAsync(hs, states.template as<KState>(), sizeof(KState) * (size_t)slots * kMaxNR,
+                              hipMemcpyDeviceToHost, st));
+        HIPCHK(hipStreamSynchronize(st));
+        std::vector<Req> keep;
+        for (Req& q : reqs) {
+            const KState* s = hs + (size_t)q.slot * kMaxNR;
+            bool running = false;
+            for (int k = 0; k < q.nr; ++k)
+                running |= s[k].status == ST_RUN || (s[k].status >= ST_UPDATE && s[k].status <= ST_READY);
+            if (running) {
+                keep.push_back(q);
+                continue;
+            }
+            Done d{q.tag, std::vector<cpm_solve_info>((size_t)q.nr)};
+            for (int k = 0; k < q.nr; ++k) {
+                cpm_solve_info& inf = d.infos[(size_t)k];
+                inf.status = s[k].status;
+                inf.iters = s[k].iters;
+                inf.restarts = s[k].restarts;
+                inf.updates = s[k].updates;
+                inf.rel_residual = s[k].bsq > 0 ? std::sqrt(s[k].rr_true / s[k].bsq) : 0.0;
+            }
+            for (int k = 0; k < q.nr; ++k) lane_pack(q, k, v64(1), 0);
+            fp64_fallback(ctx, std::vector<Op*>((size_t)q.nr, q.op), q.rhs, q.xs, o, d.infos, nullptr);
+            done.push_back(std::move(d));
+            give_range(q.off, q.len);
+            free_slots.push_back(q.slot);
+            dirty = true;
+        }
+        reqs.swap(keep);
+    }
+};
+
 }  // namespace cpmgpu
 
 // ====================================================================== C ABI
@@ -1306,6 +1561,60 @@ int cpm_memcpy_d2h(cpm_ctx* ctx, void* dst, const void* src, int64_t bytes) {
         if (!ctx) throw ArgError("null context");
         HIPCHK(hipMemcpyAsync(dst, src, (size_t)bytes, hipMemcpyDeviceToHost, C(ctx)->stream));
         HIPCHK(hipStreamSynchronize(C(ctx)->stream));
+    });
+}
+
+int cpm_server_create(cpm_ctx* ctx, int64_t pool_bytes, int lanes, const cpm_solve_opts* opts, cpm_server** out) {
+    return guard([&] {
+        if (!ctx || !out) throw ArgError("null argument");
+        HIPCHK(hipSetDevice(C(ctx)->device));
+        cpm_solve_opts o = resolve_opts(opts);
+        o.precision = CPM_FP32_REFINE;
+        ServerBase* s = nullptr;
+        if (lanes == 1) s = new Server<1>(C(ctx), pool_bytes, o);
+        else if (lanes == 2) s = new Server<2>(C(ctx), pool_bytes, o);
+        else if (lanes == 4) s = new Server<4>(C(ctx), pool_bytes, o);
+        else if (lanes == 8) s = new Server<8>(C(ctx), pool_bytes, o);
+        else throw ArgError("lanes must be 1, 2, 4 or 8");
+        *out = reinterpret_cast<cpm_server*>(s);
+    });
+}
+
+int cpm_server_destroy(cpm_server* s) {
+    return guard([&] { delete reinterpret_cast<ServerBase*>(s); });
+}
+
+int cpm_server_submit(cpm_server* sv, cpm_op* op, int n_rhs, const double* const* d_rhs, double* const* d_x,
+                      int64_t tag) {
+    int admitted = 0;
+    const int rc = guard([&] {
+        if (!sv || !op || !d_rhs || !d_x) throw ArgError("null argument");
+        for (int i = 0; i < n_rhs; ++i)
+            if (!d_rhs[i] || !d_x[i]) throw ArgError("null vector");
+        HIPCHK(hipSetDevice(O(op)->ctx->device));
+        admitted = reinterpret_cast<ServerBase*>(sv)->submit(O(op), n_rhs, d_rhs, d_x, tag) ? 1 : 0;
+    });
+    return rc != CPM_OK ? rc : (admitted ? CPM_OK : CPM_BUSY);
+}
+
+int cpm_server_run(cpm_server* sv, int iterations, int max_done, int64_t* done_tags, cpm_solve_info* done_infos,
+                   int* n_done, int* n_active) {
+    return guard([&] {
+        if (!sv || !n_done || iterations < 0 || max_done < 0 || (max_done > 0 && (!done_tags || !done_infos)))
+            throw ArgError("null argument");
+        ServerBase* s = reinterpret_cast<ServerBase*>(sv);
+        s->run(iterations);
+        int k = 0;
+        while (k < max_done && !s->done.empty()) {
+            const ServerBase::Done& d = s->done.front();
+            done_tags[k] = d.tag;
+            for (int j = 0; j < s->lanes; ++j)
+                done_infos[(size_t)k * s->lanes + j] = j < (int)d.infos.size() ? d.infos[(size_t)j] : cpm_solve_info{};
+            s->done.pop_front();
+            ++k;
+        }
+        *n_done = k;
+        if (n_active) *n_active = s->active() + (int)s->done.size();
     });
 }
 

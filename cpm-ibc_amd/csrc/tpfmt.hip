@@ -20,6 +20,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <thread>
 #include <cstdlib>
 #include <string>
 #include <vector>
@@ -265,10 +266,21 @@ static void build_lds_tiles(Op& op, int64_t D, int64_t ne, const Buffer& tpc, co
     const int64_t ntiles = (D + kBlock - 1) / kBlock;
     std::vector<int> tln((size_t)ntiles), tlo((size_t)ntiles), tlr;
     std::vector<int16_t> tpl((size_t)kS * kS * D, 0), tlx((size_t)std::max<int64_t>(ne, 1) * kExcPos, -1);
-    std::vector<int> u;
-    for (int64_t tile = 0; tile < ntiles; ++tile) {
+    std::vector<std::vector<int>> lists((size_t)ntiles);
+    // tiles are independent: host threads over contiguous tile ranges
+    const int nth = (int)std::max<int64_t>(1, std::min<int64_t>({16, ntiles, (int64_t)std::max(1u, std::thread::hardware_concurrency())}));
+    auto parallel = [&](auto&& body) {
+        std::vector<std::thread> th;
+        for (int w = 0; w < nth; ++w)
+            th.emplace_back([&, w] {
+                for (int64_t tile = ntiles * w / nth; tile < ntiles * (w + 1) / nth; ++tile) body(tile);
+            });
+        for (auto& x : th) x.join();
+    };
+    // 1. each tile's sorted distinct x rows
+    parallel([&](int64_t tile) {
         const int64_t r0 = tile * kBlock, r1 = std::min<int64_t>(D, r0 + kBlock);
-        u.clear();
+        std::vector<int>& u = lists[(size_t)tile];
         for (int64_t r = r0; r < r1; ++r) {
             const int e = he[(size_t)r];
             if (e >= 0) {
@@ -283,6 +295,11 @@ static void build_lds_tiles(Op& op, int64_t D, int64_t ne, const Buffer& tpc, co
         }
         std::sort(u.begin(), u.end());
         u.erase(std::unique(u.begin(), u.end()), u.end());
+    });
+    // 2. offsets, counts, the concatenated lists
+    for (int64_t tile = 0; tile < ntiles; ++tile) {
+        const int64_t r0 = tile * kBlock, r1 = std::min<int64_t>(D, r0 + kBlock);
+        const std::vector<int>& u = lists[(size_t)tile];
         tlo[(size_t)tile] = (int)tlr.size();
         const bool fits = (int64_t)u.size() <= kLdsRows;
         for (int64_t r = r0; r < r1; ++r)
@@ -294,6 +311,12 @@ static void build_lds_tiles(Op& op, int64_t D, int64_t ne, const Buffer& tpc, co
         tln[(size_t)tile] = (int)u.size();
         op.tile_list += (int64_t)u.size();
         tlr.insert(tlr.end(), u.begin(), u.end());
+    }
+    // 3. positions of every line (or exception column) in its tile's list
+    parallel([&](int64_t tile) {
+        if (tln[(size_t)tile] < 0) return;
+        const int64_t r0 = tile * kBlock, r1 = std::min<int64_t>(D, r0 + kBlock);
+        const std::vector<int>& u = lists[(size_t)tile];
         auto pos = [&](int c) { return (int16_t)(std::lower_bound(u.begin(), u.end(), c) - u.begin()); };
         for (int64_t r = r0; r < r1; ++r) {
             const int e = he[(size_t)r];
@@ -304,7 +327,7 @@ static void build_lds_tiles(Op& op, int64_t D, int64_t ne, const Buffer& tpc, co
             }
             for (int l = 0; l < kS * kS; ++l) tpl[(size_t)l * D + r] = pos(hc[(size_t)l * D + r]);
         }
-    }
+    });
     if (tlr.empty()) tlr.push_back(0);
     auto up = [](Buffer& b, const void* p, size_t bytes) {
         b = Buffer(bytes);

@@ -267,6 +267,27 @@ int cpm_problem_subset(const cpm_problem* pp, int c, int which, int32_t* set_of_
     });
 }
 
+int cpm_problem_curve_rows(const cpm_problem* pp, int c, int64_t* rows) {
+    return guard([&] {
+        if (!pp || !rows) throw ArgError("null argument");
+        const Problem& P = *PR(pp);
+        if (c < 0 || c >= (int)P.ibc.size()) throw ArgError("no such IBC");
+        const IbcData& I = P.ibc[c];
+        rows[0] = I.start;
+        rows[1] = I.nd;
+        if (I.second_dirichlet()) {
+            rows[2] = P.identity_rows_start + I.dir_start;
+            rows[3] = I.ndir;
+        } else if (I.spec.btype[0] == 0) {   // 1st-order Dirichlet: the ghost rows carry the values
+            rows[2] = I.start;
+            rows[3] = I.nd;
+        } else {
+            rows[2] = -1;
+            rows[3] = 0;
+        }
+    });
+}
+
 int cpm_problem_export(const cpm_problem* pp, int32_t* Lp, int32_t* Lc, double* Lv, int32_t* Ep, int32_t* Ec,
                        double* Ev, uint8_t* ident, int64_t* nnzL, int64_t* nnzE) {
     return guard([&] {

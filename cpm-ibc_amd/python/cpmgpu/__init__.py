@@ -26,7 +26,8 @@ EXPORTS = [
     "cpm_tube_export_E", "cpm_tube_export_L", "cpm_tube_interp_matrix", "cpm_op_create_from_tube", "cpm_profile_egather", "cpm_profile_copy", "cpm_heat_step_device", "cpm_rd_params_default", "cpm_rd_step_device", "cpm_problem_create", "cpm_problem_destroy",
     "cpm_problem_get_info", "cpm_problem_rhs_points", "cpm_problem_subset", "cpm_problem_export",
     "cpm_op_create_from_problem", "cpm_geodesic_distance", "cpm_problem_plot_matrix", "cpm_obj_read",
-    "cpm_polyline_chain", "cpm_tube_create_obj",
+    "cpm_polyline_chain", "cpm_tube_create_obj", "cpm_server_create", "cpm_server_destroy", "cpm_server_submit",
+    "cpm_server_run", "cpm_problem_curve_rows",
 ]
 
 
@@ -132,6 +133,11 @@ def load(path=LIB_PATH):
                                ctypes.POINTER(ctypes.c_int64), ctypes.POINTER(ctypes.c_int)]
     L.cpm_polyline_chain.argtypes = [vp, ctypes.c_int64, vp, ctypes.c_int64, vp, ctypes.POINTER(ctypes.c_int64)]
     L.cpm_tube_destroy.argtypes = [vp]
+    L.cpm_server_create.argtypes = [vp, ctypes.c_int64, ctypes.c_int, ctypes.POINTER(SolveOpts), ctypes.POINTER(vp)]
+    L.cpm_server_destroy.argtypes = [vp]
+    L.cpm_server_submit.argtypes = [vp, vp, ctypes.c_int, ctypes.POINTER(vp), ctypes.POINTER(vp), ctypes.c_int64]
+    L.cpm_server_run.argtypes = [vp, ctypes.c_int, ctypes.c_int, ctypes.POINTER(ctypes.c_int64),
+                                 ctypes.POINTER(SolveInfo), ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int)]
     L.cpm_tube_get_info.argtypes = [vp, ctypes.POINTER(TubeInfo)]
     L.cpm_tube_export.argtypes = [vp, vp, vp, vp, vp, vp, vp, vp]
     L.cpm_tube_export_E.argtypes = [vp, vp, vp, vp]
@@ -155,6 +161,7 @@ def load(path=LIB_PATH):
                                       ctypes.POINTER(ctypes.c_double)]
     L.cpm_problem_create.argtypes = [vp, ctypes.c_int, vp, ctypes.POINTER(vp)]
     L.cpm_problem_destroy.argtypes = [vp]
+    L.cpm_problem_curve_rows.argtypes = [vp, ctypes.c_int, vp]
     L.cpm_problem_get_info.argtypes = [vp, ctypes.POINTER(ProblemInfo)]
     L.cpm_problem_rhs_points.argtypes = [vp, vp, vp]
     L.cpm_problem_subset.argtypes = [vp, ctypes.c_int, ctypes.c_int, vp, vp, vp]
@@ -294,6 +301,55 @@ class Context:
         infos = (SolveInfo * n)()
         _chk(load().cpm_solve_batch_device(self._h, n, OA, RA, XA, ctypes.byref(o), infos))
         return [infos[i].as_dict() for i in range(n)]
+
+
+class Server:
+    """Continuous batching across operators (FP32 Krylov + FP64 refinement): requests (an operator
+    and 1..lanes device right-hand sides) join the running iteration as pool memory allows and
+    leave when they stop; results are bit-identical alone and in any mix."""
+
+    def __init__(self, ctx, pool_bytes, lanes=4, o=None):
+        self.ctx, self.lanes = ctx, lanes
+        self._h = vp()
+        self._keep = {}
+        o = o or opts(precision=CPM_FP32_REFINE)
+        _chk(load().cpm_server_create(ctx._h, int(pool_bytes), lanes, ctypes.byref(o), ctypes.byref(self._h)))
+
+    def close(self):
+        if self._h:
+            load().cpm_server_destroy(self._h)
+            self._h = vp()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def submit(self, op, d_rhs, d_x, tag):
+        """True if admitted; False if the pool has no room yet (submit again after run())."""
+        n = len(d_rhs)
+        RA = (vp * n)(*d_rhs)
+        XA = (vp * n)(*d_x)
+        rc = load().cpm_server_submit(self._h, op._h, n, RA, XA, int(tag))
+        if rc == 1:
+            return False
+        _chk(rc)
+        self._keep[int(tag)] = (op, RA, XA)
+        return True
+
+    def run(self, iterations=16, max_done=64):
+        """Runs `iterations` iterations; returns ([(tag, [info dicts of its right-hand sides])], n_active)."""
+        tags = (ctypes.c_int64 * max_done)()
+        infos = (SolveInfo * (max_done * self.lanes))()
+        nd, na = ctypes.c_int(0), ctypes.c_int(0)
+        _chk(load().cpm_server_run(self._h, iterations, max_done, tags, infos, ctypes.byref(nd), ctypes.byref(na)))
+        out = []
+        for k in range(nd.value):
+            t = int(tags[k])
+            op, RA, _ = self._keep.pop(t)
+            out.append((t, [infos[k * self.lanes + j].as_dict() for j in range(len(RA))]))
+        return out, na.value
 
 
 class Operator:
@@ -546,6 +602,12 @@ class Problem:
         kind = np.zeros(self.D, np.int32)
         _chk(load().cpm_problem_rhs_points(self._h, _ptr(pts), _ptr(kind)))
         return pts, kind
+
+    def curve_rows(self, c):
+        """Rows of curve c: (first ghost row, ghost rows, first Dirichlet-value row, Dirichlet-value rows)."""
+        r = np.zeros(4, np.int64)
+        _chk(load().cpm_problem_curve_rows(self._h, c, _ptr(r)))
+        return tuple(int(v) for v in r)
 
     def subset(self, c=0, which=0):
         n = self.info["tagging_size0" if which == 0 else "dof_size0"] if c == 0 else None

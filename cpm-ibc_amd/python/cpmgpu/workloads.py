@@ -194,3 +194,102 @@ def sh_rhs(cp, coeffs, lmax=8, chunk=1 << 17):
         with ThreadPoolExecutor(max_workers=min(16, os.cpu_count() or 1, len(starts))) as ex:
             list(ex.map(one, starts))
     return out
+
+
+def plane_section_loops(V, F, normal, offset):
+    """Closed polylines where the plane normal . x = offset cuts the closed triangle mesh (V, F): the
+    crossing point of every edge whose ends lie on opposite sides (a vertex on the plane counts as
+    above it), joined through the triangles they cut.  Returns a list of (n + 1) x 3 arrays whose last
+    vertex repeats the first; the points lie on the mesh's edges."""
+    s = V @ np.asarray(normal, float) - offset
+    up = s >= 0
+    E = np.concatenate([F[:, [0, 1]], F[:, [1, 2]], F[:, [2, 0]]])
+    uniq, inv = np.unique(np.sort(E, axis=1), axis=0, return_inverse=True)
+    inv = inv.reshape(-1)
+    a, b = uniq[:, 0], uniq[:, 1]
+    cross = up[a] != up[b]
+    t = np.where(cross, s[a] / np.where(cross, s[a] - s[b], 1.0), 0.0)
+    P = V[a] + t[:, None] * (V[b] - V[a])
+    fe = inv.reshape(3, len(F)).T
+    fc = cross[fe]
+    cut = fc.sum(1) == 2
+    pairs = fe[cut][fc[cut]].reshape(-1, 2)
+    nbr = {}
+    for e1, e2 in pairs:
+        nbr.setdefault(int(e1), []).append(int(e2))
+        nbr.setdefault(int(e2), []).append(int(e1))
+    seen, loops = set(), []
+    for start in nbr:
+        if start in seen:
+            continue
+        seq, prev, cur = [start], -1, start
+        seen.add(start)
+        while True:
+            nx = [e for e in nbr[cur] if e != prev]
+            nxt = nx[0] if nx else nbr[cur][0]
+            if nxt == start or nxt in seen:
+                break
+            seq.append(nxt)
+            seen.add(nxt)
+            prev, cur = cur, nxt
+        if len(seq) >= 3:
+            L = P[seq]
+            loops.append(np.vstack([L, L[:1]]))
+    return loops
+
+
+def resample_closed(L, n=256):
+    """n points spaced evenly by arc length along the closed polyline L (last vertex = first), returned
+    as an (n + 1) x 3 closed polyline; points stay on L's segments."""
+    seg = np.linalg.norm(np.diff(L, axis=0), axis=1)
+    c = np.concatenate([[0.0], np.cumsum(seg)])
+    u = c[-1] * np.arange(n) / n
+    i = np.clip(np.searchsorted(c, u, side="right") - 1, 0, len(seg) - 1)
+    w = np.where(seg[i] > 0, (u - c[i]) / np.where(seg[i] > 0, seg[i], 1.0), 0.0)
+    Q = L[i] + w[:, None] * (L[i + 1] - L[i])
+    return np.vstack([Q, Q[:1]])
+
+
+def serving_request(V, F, seed, max_curves=8, samples=256, spacing=0.25):
+    """One request of the serving workload: 1..max_curves closed curves where parallel planes (a random
+    normal, offsets at least `spacing` apart) cut the mesh, each the longest loop of its plane resampled
+    to `samples` points, and a random RGB colour per curve (numpy default_rng(seed))."""
+    rng = np.random.default_rng(seed)
+    k = int(rng.integers(1, max_curves + 1))
+    nrm = rng.standard_normal(3)
+    nrm /= np.linalg.norm(nrm)
+    h = V @ nrm
+    lo, hi = h.min() + 0.15, h.max() - 0.15
+    offs = []
+    tries = 0
+    while len(offs) < k and tries < 1000:
+        o = float(rng.uniform(lo, hi))
+        if all(abs(o - p) >= spacing for p in offs):
+            offs.append(o)
+        tries += 1
+    curves = []
+    for o in sorted(offs):
+        loops = plane_section_loops(V, F, nrm, o)
+        if loops:
+            L = max(loops, key=lambda q: np.linalg.norm(np.diff(q, axis=0), axis=1).sum())
+            curves.append(resample_closed(L, samples))
+    colours = rng.uniform(0.0, 1.0, (len(curves), 3))
+    return curves, colours
+
+
+def curve_of_points(curves, pts, chunk=4096):
+    """Index of the nearest curve (closed polylines) for each point."""
+    best = np.full(len(pts), np.inf)
+    idx = np.zeros(len(pts), int)
+    for c, C in enumerate(curves):
+        A, B = C[:-1], C[1:]
+        AB = B - A
+        L2 = np.maximum((AB * AB).sum(1), 1e-300)
+        for s in range(0, len(pts), chunk):
+            Pp = pts[s:s + chunk]
+            t = np.clip(((Pp[:, None, :] - A[None]) * AB[None]).sum(2) / L2[None], 0.0, 1.0)
+            d = np.linalg.norm(A[None] + t[:, :, None] * AB[None] - Pp[:, None, :], axis=2).min(1)
+            better = d < best[s:s + chunk]
+            best[s:s + chunk] = np.where(better, d, best[s:s + chunk])
+            idx[s:s + chunk] = np.where(better, c, idx[s:s + chunk])
+    return idx

@@ -29,7 +29,8 @@ enum cpm_status {
     CPM_ERR_HIP = -2,       /* HIP runtime failure                           */
     CPM_ERR_NOMEM = -3,     /* device allocation failed or budget exceeded   */
     CPM_ERR_GEOMETRY = -4,  /* tube construction failed (stencil off tube)   */
-    CPM_ERR_STATE = -5      /* handle used in the wrong state                */
+    CPM_ERR_STATE = -5,     /* handle used in the wrong state                */
+    CPM_BUSY = 1            /* cpm_server_submit: not admitted, no room yet  */
 };
 
 /* Preconditioner, src/Defines.h:13-18 */
@@ -50,6 +51,7 @@ enum cpm_solve_status {
 };
 
 typedef struct cpm_ctx cpm_ctx;
+typedef struct cpm_server cpm_server;
 typedef struct cpm_op cpm_op;
 typedef struct cpm_tube cpm_tube;
 
@@ -142,6 +144,30 @@ int cpm_solve_batch_device(cpm_ctx* ctx, int n_inst, cpm_op* const* ops, const d
  * counterpart in the reference, which solves one system per Solver::Solve. */
 int cpm_solve_stream_device(cpm_op* op, int n_rhs, const double* const* d_rhs, double* const* d_x, int lanes,
                             const cpm_solve_opts* opts, cpm_solve_info* infos);
+
+/* ---- serving: continuous batching across operators --------------------
+ * No counterpart in the reference, which runs one Solver::Solve at a time
+ * (src/Solver.cpp:69-114).  A server keeps a pool of FP32+refine workspace
+ * (pool_bytes: 80 B per row and lane) on the context's device and iterates
+ * every admitted request together: a request is an operator with 1..lanes
+ * right-hand sides (device vectors, caller row order, kept valid until the
+ * request completes).  cpm_server_submit admits a request when a slot of
+ * its size is free and returns CPM_BUSY otherwise (submit it again after
+ * cpm_server_run has retired others).  cpm_server_run queues `iterations`
+ * iterations over all admitted requests, then writes the solutions of the
+ * requests that stopped into their d_x and reports them: done_tags[k] and
+ * done_infos[k * lanes + j] for j < that request's n_rhs (at most max_done
+ * per call; the rest wait for the next call), n_active = requests admitted
+ * and not yet reported.  Each lane's arithmetic depends only on its own
+ * request, so results are bit-identical alone and in any mix.  Operators
+ * with trailing nD rows or the Jacobi preconditioner are rejected.  One host
+ * thread per server. */
+int cpm_server_create(cpm_ctx* ctx, int64_t pool_bytes, int lanes, const cpm_solve_opts* opts, cpm_server** out);
+int cpm_server_destroy(cpm_server* server);
+int cpm_server_submit(cpm_server* server, cpm_op* op, int n_rhs, const double* const* d_rhs, double* const* d_x,
+                      int64_t tag);
+int cpm_server_run(cpm_server* server, int iterations, int max_done, int64_t* done_tags, cpm_solve_info* done_infos,
+                   int* n_done, int* n_active);
 
 /* ---- tube (closest-point computational band) -------------------------
  * Replaces Tube(SurfaceSpecifier&, interp_deg, dx) (src/Tube.h:45,
@@ -266,6 +292,11 @@ typedef struct cpm_problem_info {
 
 int cpm_problem_create(cpm_tube* tube, int n_ibc, const cpm_ibc_desc* ibcs, cpm_problem** out);
 int cpm_problem_destroy(cpm_problem* prob);
+/* Rows of curve c in [N | G | K] (IBCSubsets::IBCStartingColumns, src/IBCSubsets.cpp:74-116):
+ * rows[0], rows[1] = first ghost row and ghost-row count (the DOF subset), rows[2], rows[3] = first
+ * row and count of the rows that take the curve's Dirichlet values (GetRHS kind 2; -1, 0 for
+ * Neumann curves). */
+int cpm_problem_curve_rows(const cpm_problem* prob, int c, int64_t* rows);
 int cpm_problem_get_info(const cpm_problem* prob, cpm_problem_info* info);
 /* Evaluation point of every row's right-hand side (PoissonWithIBC::GetRHS,
  * problems/PoissonWithIBC.cpp:50-90): kind 0 = f at cp(x), 1 = f at cp_c,

@@ -2,7 +2,7 @@
 FP64 refinement vs plain FP64, with p50/p99 latency, the E-gather roofline and
 the CPU oracle timed on the host.
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload torus-knot|sphere|surfaces|serving]
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload torus-knot|sphere|surfaces|serving|rd]
 
 A step is one pass of the hot path over one batch: `--batch` right-hand sides
 on the workload's operator solved to a 1e-10 relative residual with FP32
@@ -294,6 +294,80 @@ def bench_serving(args, ctx, cpmgpu):
     ctx.close()
 
 
+def bench_rd(args, ctx, cpmgpu):
+    """Large time-stepped batch: Gray-Scott (F 0.037, k 0.06, D_u 2e-5, D_v 1e-5) on a displaced icosphere
+    (8 subdivisions, 1,310,720 triangles, amplitude 0.1, seed 5) at h = 0.005 with Neumann IBCs on 4 closed
+    curves (parallel plane sections), implicit diffusion with dt = 1: each step solves
+    (1 - dt D Lap) u' = u + dt R(u, v) for u and v of 16 seeded initial conditions, 32 systems in one
+    FP32+refine batch (cpm_rd_step_device).  A step is one time step of all 16 states; --steps steps are
+    timed (the full run is 2000 steps)."""
+    from cpmgpu import workloads
+    h = 0.005
+    t0 = time.perf_counter()
+    V, F = workloads.displaced_icosphere(8, seed=5)
+    tube = cpmgpu.Tube(ctx, "Triangulation", h, verts=V, faces=F)
+    print(f"mesh tube: {tube.N} nodes ({time.perf_counter() - t0:.1f} s)", file=sys.stderr, flush=True)
+    curves = workloads.plane_section_curves(V, F, 4, seed=55)
+    prob = cpmgpu.Problem(tube, [cpmgpu.IBC("Polyline", verts=c, boundary_order=2, boundary_type=(1, 1),
+                                            on_ibc_tolerance=0.1 * h * h) for c in curves])
+    op = prob.operator(1.0, -1.0)
+    D = prob.D
+    print(f"problem: {D} rows, {len(curves)} Neumann curves ({time.perf_counter() - t0:.1f} s)", file=sys.stderr,
+          flush=True)
+    p = cpmgpu.rd_params(h, method=1)
+    p.F, p.k, p.nu_u, p.nu_v, p.dt = 0.037, 0.06, 2e-5, 1e-5, 1.0
+    pts, kind = prob.rhs_points()
+    tubemask = kind == 0
+    n_ic = 16
+    d_u = [ctx.alloc(8 * D) for _ in range(n_ic)]
+    d_v = [ctx.alloc(8 * D) for _ in range(n_ic)]
+    for i in range(n_ic):
+        pert = np.random.default_rng(500 + i).uniform(0.0, 1.0, D)
+        ctx.h2d(d_u[i], np.where(tubemask, 1.0 - 0.5 * pert, 0.0))
+        ctx.h2d(d_v[i], np.where(tubemask, 0.25 * pert, 0.0))
+    d_g = ctx.alloc(8 * D)
+    ctx.h2d(d_g, np.zeros(D))
+    setup_s = time.perf_counter() - t0
+    o = cpmgpu.opts(tol=1e-10, precision=cpmgpu.CPM_FP32_REFINE, update_delta=args.update_delta)
+    for _ in range(args.warmup):
+        op.rd_step_device(p, d_u, d_v, d_g, d_g, o)
+    ctx.synchronize()
+    iters = []
+    t0 = time.perf_counter()
+    for k in range(args.steps):
+        infos = op.rd_step_device(p, d_u, d_v, d_g, d_g, o)
+        iters += [j["iters"] for j in infos]
+        assert all(j["status"] == "converged" for j in infos), infos
+        print(f"step {k} done ({time.perf_counter() - t0:.1f} s)", file=sys.stderr, flush=True)
+    ctx.synchronize()
+    t_run = time.perf_counter() - t0
+    # the same steps in plain FP64 from the current states (one step, for the ratio)
+    o64 = cpmgpu.opts(tol=1e-10, precision=cpmgpu.CPM_FP64)
+    t1 = time.perf_counter()
+    infos64 = op.rd_step_device(p, d_u, d_v, d_g, d_g, o64)
+    ctx.synchronize()
+    t64 = time.perf_counter() - t1
+    u0 = ctx.d2h(np.zeros(D), d_u[0])
+    out = {
+        "metric": "time-stepped batch: Gray-Scott steps/s (32 screened-Poisson solves per step), FP32+refine",
+        "value": round(args.steps / t_run, 4), "unit": "steps/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": round(1000 * t_run / args.steps, 1), "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f32 Krylov + f64 refinement",
+        "data": "synthetic (seeded displaced icosphere mesh, plane-section curves, seeded random initial states)",
+        "config": {"workload": "Gray-Scott on a displaced icosphere (1,310,720 triangles), h=0.005, 4 Neumann curves, "
+                               "dt=1, 16 initial states", "tube_nodes": tube.N, "rows": D, "states": n_ic,
+                   "solves_per_step": 2 * n_ic, "setup_s": round(setup_s, 1)},
+        "solves_per_s": round(2 * n_ic * args.steps / t_run, 2),
+        "iters_mean": round(float(np.mean(iters)), 1), "iters_max": int(max(iters)),
+        "fp64_step_ms": round(1000 * t64, 1), "iters_fp64_mean": round(float(np.mean([j["iters"] for j in infos64])), 1),
+        "full_run_2000_steps_s": round(2000 * t_run / args.steps, 0),
+        "u0_range": [float(u0[tubemask].min()), float(u0[tubemask].max())],
+        "roofline": None, "cpu_baseline": None,
+    }
+    print(json.dumps(out))
+    ctx.close()
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -301,7 +375,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--batch", type=int, default=8, help="interleaved lanes (1, 2, 4 or 8)")
     ap.add_argument("--stream", type=int, default=32, help="right-hand sides streamed through the lanes per step")
-    ap.add_argument("--workload", default="torus-knot", choices=["torus-knot", "sphere", "surfaces", "serving"])
+    ap.add_argument("--workload", default="torus-knot", choices=["torus-knot", "sphere", "surfaces", "serving", "rd"])
     ap.add_argument("--requests", type=int, default=16, help="serving: requests")
     ap.add_argument("--qps", type=float, default=1.0, help="serving: mean arrival rate")
     ap.add_argument("--pool-gb", type=float, default=8.0, help="serving: workspace pool")
@@ -321,6 +395,8 @@ def main():
         return bench_surfaces(args, ctx, cpmgpu)
     if args.workload == "serving":
         return bench_serving(args, ctx, cpmgpu)
+    if args.workload == "rd":
+        return bench_rd(args, ctx, cpmgpu)
     t0 = time.perf_counter()
     S = max(args.stream, args.batch)
     if args.workload == "sphere":

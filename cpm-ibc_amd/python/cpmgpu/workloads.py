@@ -196,16 +196,21 @@ def sh_rhs(cp, coeffs, lmax=8, chunk=1 << 17):
     return out
 
 
-def plane_section_loops(V, F, normal, offset):
+def mesh_edges(F):
+    """Distinct edges of the triangles F and, per face, the indices of its three edges."""
+    E = np.concatenate([F[:, [0, 1]], F[:, [1, 2]], F[:, [2, 0]]])
+    uniq, inv = np.unique(np.sort(E, axis=1), axis=0, return_inverse=True)
+    return uniq, inv.reshape(-1)
+
+
+def plane_section_loops(V, F, normal, offset, edges=None):
     """Closed polylines where the plane normal . x = offset cuts the closed triangle mesh (V, F): the
     crossing point of every edge whose ends lie on opposite sides (a vertex on the plane counts as
     above it), joined through the triangles they cut.  Returns a list of (n + 1) x 3 arrays whose last
-    vertex repeats the first; the points lie on the mesh's edges."""
+    vertex repeats the first; the points lie on the mesh's edges.  edges: mesh_edges(F), if known."""
     s = V @ np.asarray(normal, float) - offset
     up = s >= 0
-    E = np.concatenate([F[:, [0, 1]], F[:, [1, 2]], F[:, [2, 0]]])
-    uniq, inv = np.unique(np.sort(E, axis=1), axis=0, return_inverse=True)
-    inv = inv.reshape(-1)
+    uniq, inv = edges if edges is not None else mesh_edges(F)
     a, b = uniq[:, 0], uniq[:, 1]
     cross = up[a] != up[b]
     t = np.where(cross, s[a] / np.where(cross, s[a] - s[b], 1.0), 0.0)
@@ -268,8 +273,9 @@ def serving_request(V, F, seed, max_curves=8, samples=256, spacing=0.25):
             offs.append(o)
         tries += 1
     curves = []
+    edges = mesh_edges(F)
     for o in sorted(offs):
-        loops = plane_section_loops(V, F, nrm, o)
+        loops = plane_section_loops(V, F, nrm, o, edges)
         if loops:
             L = max(loops, key=lambda q: np.linalg.norm(np.diff(q, axis=0), axis=1).sum())
             curves.append(resample_closed(L, samples))
@@ -293,3 +299,25 @@ def curve_of_points(curves, pts, chunk=4096):
             best[s:s + chunk] = np.where(better, d, best[s:s + chunk])
             idx[s:s + chunk] = np.where(better, c, idx[s:s + chunk])
     return idx
+
+
+def plane_section_curves(V, F, k, seed, samples=256, spacing=0.25):
+    """k closed curves where parallel planes (a random normal, offsets at least `spacing` apart, numpy
+    default_rng(seed)) cut the mesh: the longest loop of each plane resampled to `samples` points."""
+    rng = np.random.default_rng(seed)
+    nrm = rng.standard_normal(3)
+    nrm /= np.linalg.norm(nrm)
+    h = V @ nrm
+    lo, hi = h.min() + 0.15, h.max() - 0.15
+    offs = []
+    while len(offs) < k:
+        o = float(rng.uniform(lo, hi))
+        if all(abs(o - q) >= spacing for q in offs):
+            offs.append(o)
+    out = []
+    edges = mesh_edges(F)
+    for o in sorted(offs):
+        loops = plane_section_loops(V, F, nrm, o, edges)
+        L = max(loops, key=lambda q: np.linalg.norm(np.diff(q, axis=0), axis=1).sum())
+        out.append(resample_closed(L, samples))
+    return out

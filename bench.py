@@ -224,53 +224,86 @@ def bench_serving(args, ctx, cpmgpu):
     print(f"{n} requests generated ({sum(len(c) for c, _ in payload)} curves)", file=sys.stderr, flush=True)
     srv = cpmgpu.Server(ctx, int(args.pool_gb * (1 << 30)), lanes=4,
                         o=cpmgpu.opts(tol=1e-10, precision=cpmgpu.CPM_FP32_REFINE, update_delta=args.update_delta))
-    live, waiting, rec = {}, [], {}
-    nxt = 0
+    # A build thread with its own context builds each request's problem, operator and right-hand sides
+    # when it arrives (its kernels run on its own stream beside the iterations) and releases finished
+    # requests; this thread admits built requests and drives the iterations.
+    import queue
+    import threading
+    bctx = cpmgpu.Context(0)
+    built, finished = queue.Queue(), queue.Queue()
     start = time.perf_counter()
-    while len(rec) < n:
-        now = time.perf_counter() - start
-        while nxt < n and arrive[nxt] <= now:             # arrivals: build the problem on the GPU
+
+    def release():
+        """Frees finished requests; True once the loop has ended."""
+        while True:
+            try:
+                item = finished.get_nowait()
+            except queue.Empty:
+                return False
+            if item is None:
+                return True
+            P, op, dr, dx = item
+            for d in dr + dx:
+                bctx.free(d)
+            op.close()
+            P.close()
+
+    def builder():
+        for i in range(n):
+            while time.perf_counter() - start < arrive[i]:
+                release()
+                time.sleep(0.002)
             tb = time.perf_counter()
-            curves, cols = payload[nxt]
+            curves, cols = payload[i]
             P = cpmgpu.Problem(tube, [cpmgpu.IBC("Polyline", verts=c, boundary_order=2, on_ibc_tolerance=0.1 * h * h)
-                                      for c in curves])
+                                      for c in curves], ctx=bctx)
             op = P.operator(0.0, -1.0)
-            dr = [ctx.alloc(8 * P.D) for _ in range(3)]
-            dx = [ctx.alloc(8 * P.D) for _ in range(3)]
+            dr = [bctx.alloc(8 * P.D) for _ in range(3)]
+            dx = [bctx.alloc(8 * P.D) for _ in range(3)]
             for ch in range(3):                            # each curve's Dirichlet rows take its colour
                 b = np.zeros(P.D)
                 for c in range(len(curves)):
                     _, _, r0, nr = P.curve_rows(c)
                     b[r0:r0 + nr] = cols[c, ch]
-                ctx.h2d(dr[ch], b)
-            live[nxt] = (P, op, dr, dx)
-            build_ms = 1000 * (time.perf_counter() - tb)
-            waiting.append((nxt, build_ms))
-            nxt += 1
+                bctx.h2d(dr[ch], b)
+            built.put((i, (P, op, dr, dx), 1000 * (time.perf_counter() - tb)))
+        while not release():                               # releases until the loop ends
+            time.sleep(0.005)
+
+    th = threading.Thread(target=builder, daemon=True)
+    th.start()
+    live, waiting, rec = {}, [], {}
+    while len(rec) < n:
+        while True:
+            try:
+                i, req, bms = built.get_nowait()
+            except queue.Empty:
+                break
+            live[i] = req + (bms,)
+            waiting.append(i)
         while waiting:                                     # admission by pool memory
-            i, bms = waiting[0]
+            i = waiting[0]
             if not srv.submit(live[i][1], live[i][2], live[i][3], i):
                 break
             waiting.pop(0)
-            live[i] = live[i] + (bms, time.perf_counter() - start)
         done, active = srv.run(16)
         for tag, infos in done:
             t_done = time.perf_counter() - start
-            P, op, dr, dx, bms, t_adm = live.pop(tag)
+            P, op, dr, dx, bms = live.pop(tag)
             rec[tag] = dict(latency_ms=1000 * (t_done - arrive[tag]), build_ms=bms, rows=P.D,
                             iters=max(j["iters"] for j in infos),
                             fallback=sum(1 for j in infos if j["fallback_iters"] > 0),
                             ok=all(j["status"] == "converged" for j in infos))
             print(f"request {tag}: {len(payload[tag][0])} curves, {P.D} rows, latency {rec[tag]['latency_ms']:.0f} ms",
                   file=sys.stderr, flush=True)
-            for d in dr + dx:
-                ctx.free(d)
-            op.close()
-            P.close()
-        if active == 0 and not waiting and nxt < n:
-            time.sleep(max(0.0, arrive[nxt] - (time.perf_counter() - start)))
+            finished.put((P, op, dr, dx))
+        if active == 0 and not waiting:
+            time.sleep(0.002)
     makespan = time.perf_counter() - start
+    finished.put(None)
+    th.join()
     srv.close()
+    bctx.close()
     lat = sorted(r["latency_ms"] for r in rec.values())
     p99 = lat[min(len(lat) - 1, int(math.ceil(0.99 * len(lat))) - 1)]
     out = {

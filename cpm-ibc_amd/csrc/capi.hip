@@ -601,7 +601,9 @@ void solve_batch(Ctx* ctx, const std::vector<Op*>& ops, const std::vector<const 
     if (n == 0) return;
     if (o.precision != CPM_FP64 && o.precision != CPM_FP32_REFINE) throw ArgError("unknown precision");
     for (Op* op : ops) {
-        if (!op || op->ctx != ctx) throw ArgError("operator belongs to another context");
+        // operators of another context on the same device are complete (their builds end with a
+        // stream synchronisation) and are solved on this context's stream
+        if (!op || op->ctx->device != ctx->device) throw ArgError("operator lives on another device");
         if (op->precond == CPM_PRECOND_JACOBI && op->precond_iters > 100000) throw ArgError("too many Jacobi sweeps");
     }
     // group instances that share an operator and its (m, n) (at most kMaxNR per group)
@@ -1111,7 +1113,9 @@ struct Server : ServerBase {
 
     bool submit(Op* op, int nrhs, const double* const* rhs, double* const* xs, int64_t tag) override {
         if (nrhs < 1 || nrhs > NR) throw ArgError("a request carries 1.." + std::to_string(NR) + " right-hand sides");
-        if (op->ctx != ctx) throw ArgError("operator belongs to another context");
+        // operators built on another context of the device (a build thread's) are complete: their
+        // builds end with a stream synchronisation
+        if (op->ctx->device != ctx->device) throw ArgError("operator lives on another device");
         if (op->nD != 0) throw ArgError("served operators have no trailing nD rows");
         if (op->precond == CPM_PRECOND_JACOBI) throw ArgError("served operators use the diagonal or no preconditioner");
         const int64_t len = (op->D * NR + 63) & ~int64_t(63);

@@ -56,7 +56,8 @@ struct Problem {
     double build_ms = 0;
 };
 
-Problem* problem_build(Tube* tube, const std::vector<IbcSpec>& specs);
+// bctx: the context whose stream and workspaces the build uses (the tube's when null; same device).
+Problem* problem_build(Tube* tube, const std::vector<IbcSpec>& specs, Ctx* bctx = nullptr);
 Op* op_from_problem(Problem* P, double m, double n, int precond, int precond_iters);
 // Plotting rows at nq host points: tube interpolation modified by the curves
 // (AddIBCPlotInterpolation), nq x D CSR with (p+1)^3 entries per row.

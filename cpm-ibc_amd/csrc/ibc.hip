@@ -612,9 +612,7 @@ static int64_t select_flags(Ctx* ctx, const uint8_t* flag, int64_t n, int32_t* o
 }
 
 // Tagging subset of one curve (IBCSubset::ComputeSubset, IBCSubset.cpp:12-36).
-static void tagging_subset(Tube* t, IbcData& I, const double* tube_normals, const int* ecol_cpc_scratch) {
-    (void)ecol_cpc_scratch;
-    Ctx* ctx = t->ctx;
+static void tagging_subset(Tube* t, Ctx* ctx, IbcData& I, const double* tube_normals) {
     const hipStream_t st = ctx->stream;
     const int64_t N = t->N;
     const double radius = 2.0 * t->r_t;
@@ -727,7 +725,7 @@ static void tagging_subset(Tube* t, IbcData& I, const double* tube_normals, cons
     const int S = t->p + 1, W = S * S * S;
     int* ec = reinterpret_cast<int*>(ctx->workspace("ibc_ec", (size_t)W * nt * 4));
     double* ev = ctx->workspace("ibc_ev", (size_t)W * nt * 8);
-    build_E_ell(*t, I.t_cpc.as<double>(), nt, ec, ev, nullptr, nt, st);
+    build_E_ell(*t, I.t_cpc.as<double>(), nt, ec, ev, nullptr, nt, st, ctx);
     double* gsub = ctx->workspace("ibc_gsub", (size_t)9 * nt * 8);
     realloc<double>(I.t_dir, nt * 3);
     realloc<double>(I.t_bn, nt * 3);
@@ -741,8 +739,7 @@ static void tagging_subset(Tube* t, IbcData& I, const double* tube_normals, cons
 }
 
 // DOF subset (IBCDOFSubset::ComputeDOFSubset, IBCDOFSubset.cpp:10-93).
-static void dof_subset(Tube* t, IbcData& I) {
-    Ctx* ctx = t->ctx;
+static void dof_subset(Tube* t, Ctx* ctx, IbcData& I) {
     const hipStream_t st = ctx->stream;
     uint8_t* f = reinterpret_cast<uint8_t*>(ctx->workspace("ibc_dflag", I.nt));
     hipLaunchKernelGGL(k_dof_flag, dim3(nblk(I.nt)), dim3(256), 0, st, I.nt, I.t_dist.as<double>(),
@@ -762,10 +759,11 @@ static void dof_subset(Tube* t, IbcData& I) {
     HIPCHK(hipStreamSynchronize(st));
 }
 
-Problem* problem_build(Tube* t, const std::vector<IbcSpec>& specs) {
+Problem* problem_build(Tube* t, const std::vector<IbcSpec>& specs, Ctx* bctx) {
     auto t0 = std::chrono::steady_clock::now();
     if ((int)specs.size() > kMaxIbc) throw ArgError("too many IBC curves");
-    Ctx* ctx = t->ctx;
+    Ctx* ctx = bctx ? bctx : t->ctx;
+    if (ctx->device != t->ctx->device) throw ArgError("the problem's context is on another device than the tube");
     const hipStream_t st = ctx->stream;
     std::unique_ptr<Problem> P(new Problem());
     P->ctx = ctx;
@@ -776,7 +774,7 @@ Problem* problem_build(Tube* t, const std::vector<IbcSpec>& specs) {
     // E of the tube (tube-index columns)
     int* ect = reinterpret_cast<int*>(ctx->workspace("prob_ect", (size_t)W * N * 4));
     double* evt = ctx->workspace("prob_evt", (size_t)W * N * 8);
-    build_E_ell(*t, t->cpx.as<double>(), N, ect, evt, nullptr, N, st);
+    build_E_ell(*t, t->cpx.as<double>(), N, ect, evt, nullptr, N, st, ctx);
     // whole-tube normals from the Jacobian of cp
     realloc<double>(P->normals, N * 3);
     double* G = ctx->workspace("prob_grad", (size_t)9 * N * 8);
@@ -793,9 +791,9 @@ Problem* problem_build(Tube* t, const std::vector<IbcSpec>& specs) {
             HIPCHK(hipMemcpy(I.verts.p, I.spec.surf.verts.data(), I.spec.surf.verts.size() * 8, hipMemcpyHostToDevice));
         }
         I.sd = make_surfdev(I.spec.surf, I.verts.as<double>());
-        tagging_subset(t, I, P->normals.as<double>(), nullptr);
+        tagging_subset(t, ctx, I, P->normals.as<double>());
     }
-    for (auto& I : P->ibc) dof_subset(t, I);
+    for (auto& I : P->ibc) dof_subset(t, ctx, I);
     // IBCSubsets::IBCStartingColumns / TotalDOFs (IBCSubsets.cpp:74-116)
     int64_t G_ = 0, K_ = 0, ident_start = 0;
     bool any2 = false;
@@ -859,7 +857,7 @@ Problem* problem_build(Tube* t, const std::vector<IbcSpec>& specs) {
                            I.d_cpc.as<double>(), t->rec.as<int64_t>(), t->N1d, t->xstart, t->dx, t->sd, q);
         int* ecg = reinterpret_cast<int*>(ctx->workspace("prob_ecg", (size_t)W * I.nd * 4));
         double* evg = ctx->workspace("prob_evg", (size_t)W * I.nd * 8);
-        build_E_ell(*t, q, I.nd, ecg, evg, nullptr, I.nd, st);
+        build_E_ell(*t, q, I.nd, ecg, evg, nullptr, I.nd, st, ctx);
         HIPCHK(hipMemcpyAsync(P->qpts.as<double>() + 3 * I.start, q, (size_t)I.nd * 24, hipMemcpyDeviceToDevice, st));
         const int64_t ident0 = ident_start + I.dir_start;
         hipLaunchKernelGGL(k_asm_ghost, dim3(nblk(I.nd)), dim3(256), 0, st, a, c, I.nd, I.spec.order, I.spec.btype[0],
@@ -1099,7 +1097,7 @@ void problem_plot_matrix(Problem* P, const double* xq, int64_t nq, int32_t* rowp
     HIPCHK(hipMemcpyAsync(dxq, xq, (size_t)nq * 24, hipMemcpyHostToDevice, st));
     int* ec = reinterpret_cast<int*>(ctx->workspace("plot_ec", (size_t)W * nq * 4));
     double* ev = ctx->workspace("plot_ev", (size_t)W * nq * 8);
-    build_E_ell(*t, dxq, nq, ec, ev, nullptr, nq, st);
+    build_E_ell(*t, dxq, nq, ec, ev, nullptr, nq, st, ctx);
     int* oc = reinterpret_cast<int*>(ctx->workspace("plot_oc", (size_t)W * nq * 4));
     double* ov = ctx->workspace("plot_ov", (size_t)W * nq * 8);
     GridDev g{t->N1d, t->xstart, t->dx, t->r_t};

@@ -20,6 +20,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <mutex>
 #include <thread>
 #include <cstdlib>
 #include <string>
@@ -215,9 +216,13 @@ inline unsigned nblk(int64_t n) { return (unsigned)((n + 255) / 256); }
 
 }  // namespace
 
-static void ensure_lex(Tube& t) {
+// The tube's lexicographic order, computed on first use (on the calling
+// context's stream; the lock keeps concurrent first uses apart).
+static void ensure_lex(Tube& t, Ctx* ctx) {
+    static std::mutex lock;
+    std::lock_guard<std::mutex> g(lock);
     if (t.lex.p) return;
-    const hipStream_t st = t.ctx->stream;
+    const hipStream_t st = ctx->stream;
     const int64_t G = t.N1d * t.N1d * t.N1d;
     const int64_t nb = (G + 255) / 256;
     Buffer bcount((size_t)(nb + 1) * 4), flags(64);
@@ -343,8 +348,8 @@ static void build_lds_tiles(Op& op, int64_t D, int64_t ne, const Buffer& tpc, co
 bool op_build_tp(Op& op, Tube& t, const TpSource& src) {
     if (t.p != 3 || t.dim != 3 || src.ew < kS * kS * kS || src.lw > 6 || op.nD != 0) return false;
     if (src.D > (int64_t)INT32_MAX / 2) return false;
-    ensure_lex(t);
-    const hipStream_t st = t.ctx->stream;
+    ensure_lex(t, op.ctx);
+    const hipStream_t st = op.ctx->stream;
     const int64_t D = src.D;
     TpArgs a{};
     a.D = D;

@@ -90,8 +90,9 @@ SurfDev make_surfdev(const SurfaceHost& s, const double* dverts, const MeshDev* 
 // E rows at query points xq (device, 3 doubles each) as slot-major ELL of width
 // (p+1)^dim; columns are tube indices.  Throws GeomError when a stencil leaves
 // the tube (the reference throws in Tube::ComputeBasePoints).
+// wctx: the context whose workspace the launch uses (the tube's when null).
 void build_E_ell(const Tube& t, const double* d_xq, int64_t nq, int* ecol, double* eval, float* eval32,
-                 int64_t ld, hipStream_t st);
+                 int64_t ld, hipStream_t st, Ctx* wctx = nullptr);
 
 // Operator (m I + n (Ld + Loff E)) of the plain closest-point problem on a GPU tube.
 Op* op_from_tube(Tube* t, double m, double n, int precond, int precond_iters);

@@ -763,8 +763,8 @@ Tube* tube_build(Ctx* ctx, const SurfaceHost& s, int p, double dx) {
 }
 
 void build_E_ell(const Tube& t, const double* d_xq, int64_t nq, int* ecol, double* eval, float* eval32, int64_t ld,
-                 hipStream_t st) {
-    int* err = reinterpret_cast<int*>(t.ctx->workspace("tube_int", 64));
+                 hipStream_t st, Ctx* wctx) {
+    int* err = reinterpret_cast<int*>((wctx ? wctx : t.ctx)->workspace("tube_int", 64));
     HIPCHK(hipMemsetAsync(err, 0, 4, st));
     GridDev g{t.N1d, t.xstart, t.dx, t.r_t};
     hipLaunchKernelGGL(k_erows, dim3((unsigned)((nq + 255) / 256)), dim3(256), 0, st, nq, d_xq, t.gidx.as<int32_t>(),

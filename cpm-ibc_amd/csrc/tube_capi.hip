@@ -206,6 +206,10 @@ int cpm_tube_export_L(const cpm_tube* tp, int32_t* rowptr, int32_t* col, double*
 }
 
 int cpm_problem_create(cpm_tube* tp, int n_ibc, const cpm_ibc_desc* ibcs, cpm_problem** out) {
+    return cpm_problem_create_on(nullptr, tp, n_ibc, ibcs, out);
+}
+
+int cpm_problem_create_on(cpm_ctx* bctx, cpm_tube* tp, int n_ibc, const cpm_ibc_desc* ibcs, cpm_problem** out) {
     return guard([&] {
         if (!tp || !out || n_ibc < 0 || (n_ibc > 0 && !ibcs)) throw ArgError("null argument");
         Tube* t = T(tp);
@@ -221,7 +225,7 @@ int cpm_problem_create(cpm_tube* tp, int n_ibc, const cpm_ibc_desc* ibcs, cpm_pr
             specs[c].tol = d.on_ibc_tolerance;
             specs[c].dof_scale = d.dof_radius_scale > 0 ? d.dof_radius_scale : 1.0;
         }
-        *out = reinterpret_cast<cpm_problem*>(problem_build(t, specs));
+        *out = reinterpret_cast<cpm_problem*>(problem_build(t, specs, C(bctx)));
     });
 }
 

@@ -27,7 +27,7 @@ EXPORTS = [
     "cpm_problem_get_info", "cpm_problem_rhs_points", "cpm_problem_subset", "cpm_problem_export",
     "cpm_op_create_from_problem", "cpm_geodesic_distance", "cpm_problem_plot_matrix", "cpm_obj_read",
     "cpm_polyline_chain", "cpm_tube_create_obj", "cpm_server_create", "cpm_server_destroy", "cpm_server_submit",
-    "cpm_server_run", "cpm_problem_curve_rows",
+    "cpm_server_run", "cpm_problem_curve_rows", "cpm_problem_create_on",
 ]
 
 
@@ -160,6 +160,7 @@ def load(path=LIB_PATH):
     L.cpm_profile_egather.argtypes = [vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.POINTER(ctypes.c_double),
                                       ctypes.POINTER(ctypes.c_double)]
     L.cpm_problem_create.argtypes = [vp, ctypes.c_int, vp, ctypes.POINTER(vp)]
+    L.cpm_problem_create_on.argtypes = [vp, vp, ctypes.c_int, vp, ctypes.POINTER(vp)]
     L.cpm_problem_destroy.argtypes = [vp]
     L.cpm_problem_curve_rows.argtypes = [vp, ctypes.c_int, vp]
     L.cpm_problem_get_info.argtypes = [vp, ctypes.POINTER(ProblemInfo)]
@@ -561,7 +562,9 @@ class IBC:
 class Problem:
     """PoissonWithIBC on a GPU tube: subsets, directions and the IBC-modified operator built on the device."""
 
-    def __init__(self, tube, ibcs):
+    def __init__(self, tube, ibcs, ctx=None):
+        """ctx: build on this context (same device as the tube; e.g. a build thread's) instead of the
+        tube's; the problem's operators then belong to it."""
         self.tube = tube
         n = len(ibcs)
         descs = (IbcDesc * n)()
@@ -580,7 +583,9 @@ class Problem:
             d.on_ibc_tolerance = i.on_ibc_tolerance
             d.dof_radius_scale = i.dof_radius_scale
         self._h = vp()
-        _chk(load().cpm_problem_create(tube._h, n, ctypes.cast(descs, vp), ctypes.byref(self._h)))
+        self.ctx = ctx or tube.ctx
+        _chk(load().cpm_problem_create_on(ctx._h if ctx is not None else None, tube._h, n, ctypes.cast(descs, vp),
+                                          ctypes.byref(self._h)))
         inf = ProblemInfo()
         _chk(load().cpm_problem_get_info(self._h, ctypes.byref(inf)))
         self.info = inf.as_dict()
@@ -635,7 +640,7 @@ class Problem:
     def operator(self, m, n, precond=PRECOND_DIAGONAL, precond_iters=0):
         h = vp()
         _chk(load().cpm_op_create_from_problem(self._h, m, n, precond, precond_iters, ctypes.byref(h)))
-        return Operator(self.tube.ctx, None, None, None, m, n, handle=h)
+        return Operator(self.ctx, None, None, None, m, n, handle=h)
 
     def plot_matrix(self, xq):
         """Plotting rows at points xq (n x 3) with the curves' ghost columns (AddIBCPlotInterpolation)."""

@@ -291,6 +291,10 @@ typedef struct cpm_problem_info {
 } cpm_problem_info;
 
 int cpm_problem_create(cpm_tube* tube, int n_ibc, const cpm_ibc_desc* ibcs, cpm_problem** out);
+/* The same, built with ctx's stream and workspaces (a context on the tube's device), so that a
+ * second host thread can build problems on a resident tube while the tube's context keeps
+ * solving; operators made from the problem belong to ctx.  No counterpart in the reference. */
+int cpm_problem_create_on(cpm_ctx* ctx, cpm_tube* tube, int n_ibc, const cpm_ibc_desc* ibcs, cpm_problem** out);
 int cpm_problem_destroy(cpm_problem* prob);
 /* Rows of curve c in [N | G | K] (IBCSubsets::IBCStartingColumns, src/IBCSubsets.cpp:74-116):
  * rows[0], rows[1] = first ghost row and ghost-row count (the DOF subset), rows[2], rows[3] = first

@@ -120,3 +120,33 @@ def test_submit_rejections(gpu_ctx):
         srv.close()
         for p in d:
             gpu_ctx.free(p)
+
+
+def test_problem_built_on_second_context(gpu_ctx):
+    """A problem built on another context of the device (a build thread's) gives the same operator and
+    solutions, solved or served on the first context."""
+    print("case serving build context", flush=True)
+    V, F = workloads.icosphere(3)
+    tube = cpmgpu.Tube(gpu_ctx, "Triangulation", H, verts=V, faces=F)
+    curves, cols = workloads.serving_request(V, F, 901, max_curves=2, spacing=0.85)
+    ibcs = [cpmgpu.IBC("Polyline", verts=c, boundary_order=2, on_ibc_tolerance=0.1 * H * H) for c in curves]
+    bctx = cpmgpu.Context(0)
+    try:
+        P1 = cpmgpu.Problem(tube, ibcs)
+        P2 = cpmgpu.Problem(tube, ibcs, ctx=bctx)
+        (L1, E1, I1), (L2, E2, I2) = P1.export(), P2.export()
+        for u, v in ((L1, L2), (E1, E2)):
+            assert np.array_equal(u.ptr, v.ptr) and np.array_equal(u.col, v.col) and np.array_equal(u.val, v.val)
+        assert np.array_equal(I1, I2)
+        op1, op2 = P1.operator(0.0, -1.0), P2.operator(0.0, -1.0)
+        b = np.zeros(P1.D)
+        _, _, r0, nr = P1.curve_rows(0)
+        b[r0:r0 + nr] = 1.0
+        o = cpmgpu.opts(precision=cpmgpu.CPM_FP32_REFINE)
+        x1, i1 = gpu_ctx.solve_batch([op1], [b], o)
+        x2, i2 = gpu_ctx.solve_batch([op2], [b], o)      # operator of the other context, solved here
+        assert i1[0]["status"] == i2[0]["status"] == "converged"
+        assert np.array_equal(x1[0], x2[0])
+        op1.close(), op2.close(), P1.close(), P2.close()
+    finally:
+        bctx.close()

@@ -1188,10 +1188,12 @@ struct Server : ServerBase {
         Vecs<float> w = V32;
         for (int i = 0; i < iters; ++i) Launch<float, NR>::iteration(c, w, true, mv);
         HIPCHK(hipGetLastError());
-        KState* hs = ctx->host_states;
-        HIPCHK(hipMemcpyAsync(hs, states.template as<KState>(), sizeof(KState) * (size_t)slots * kMaxNR,
+        HIPCHK(hipMemcpyAsync(ctx->host_states, states.template as<KState>(), sizeof(KState) * (size_t)slots * kMaxNR,
                               hipMemcpyDeviceToHost, st));
         HIPCHK(hipStreamSynchronize(st));
+        // a private copy: FP64 re-solves below poll through the context's pinned state buffer
+        const std::vector<KState> snap(ctx->host_states, ctx->host_states + (size_t)slots * kMaxNR);
+        const KState* hs = snap.data();
         std::vector<Req> keep;
         for (Req& q : reqs) {
             const KState* s = hs + (size_t)q.slot * kMaxNR;

@@ -160,8 +160,11 @@ int cpm_solve_stream_device(cpm_op* op, int n_rhs, const double* const* d_rhs, d
  * per call; the rest wait for the next call), n_active = requests admitted
  * and not yet reported.  Each lane's arithmetic depends only on its own
  * request, so results are bit-identical alone and in any mix.  Operators
- * with trailing nD rows or the Jacobi preconditioner are rejected.  One host
- * thread per server. */
+ * with trailing nD rows or the Jacobi preconditioner are rejected; an
+ * operator may come from another context on the same device (e.g. a build
+ * thread's, cpm_problem_create_on) and must stay alive until its request is
+ * reported.  Systems whose refinement stalls are re-solved in FP64 when their
+ * request leaves (cpm_solve_info.fallback_iters).  One host thread per server. */
 int cpm_server_create(cpm_ctx* ctx, int64_t pool_bytes, int lanes, const cpm_solve_opts* opts, cpm_server** out);
 int cpm_server_destroy(cpm_server* server);
 int cpm_server_submit(cpm_server* server, cpm_op* op, int n_rhs, const double* const* d_rhs, double* const* d_x,

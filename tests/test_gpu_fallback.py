@@ -73,3 +73,35 @@ def test_no_fallback_where_refinement_works(gpu_ctx):
     x64, _ = op.solve(b, cpmgpu.opts(tol=1e-13, precision=cpmgpu.CPM_FP64))
     assert i32["status"] == "converged" and i32["fallback_iters"] == 0 and i32["updates"] >= 1
     assert np.linalg.norm(x32 - x64) <= 1e-8 * np.linalg.norm(x64)
+
+
+def test_fallback_in_server(gpu_ctx, fine_sphere):
+    """Two requests that stall in FP32 finish in the same serving round: each is re-solved in FP64."""
+    print("case fallback server sphere h=0.01", flush=True)
+    _, op, rhs = fine_sphere
+    D = op.D
+    ref = [op.solve(b, cpmgpu.opts(precision=cpmgpu.CPM_FP64))[0] for b in rhs[:2]]
+    srv = cpmgpu.Server(gpu_ctx, 80 * 2 * 2 * ((D + 63) // 64 + 1) * 64, lanes=2)
+    d_rhs = [gpu_ctx.alloc(8 * D) for _ in range(2)]
+    d_x = [gpu_ctx.alloc(8 * D) for _ in range(2)]
+    try:
+        for d, b in zip(d_rhs, rhs[:2]):
+            gpu_ctx.h2d(d, b)
+        assert srv.submit(op, d_rhs[:1], d_x[:1], 0) and srv.submit(op, d_rhs[1:], d_x[1:], 1)
+        got = {}
+        for _ in range(2000):
+            done, active = srv.run(16)
+            for tag, inf in done:
+                got[tag] = inf
+            if len(got) == 2:
+                break
+        assert len(got) == 2
+        for tag in (0, 1):
+            assert got[tag][0]["status"] == "converged"
+            x = gpu_ctx.d2h(np.zeros(D), d_x[tag])
+            assert np.linalg.norm(x - ref[tag]) <= 1e-8 * np.linalg.norm(ref[tag])
+        assert any(got[t][0]["fallback_iters"] for t in (0, 1))
+    finally:
+        srv.close()
+        for d in d_rhs + d_x:
+            gpu_ctx.free(d)

@@ -8,6 +8,8 @@
 #include <string>
 #include <vector>
 
+#include "tuned.h"
+
 namespace cpmgpu {
 
 constexpr int kBlock = 256;          // 4 wave64 per workgroup
@@ -87,8 +89,8 @@ struct OpView {
 };
 
 constexpr double kDefaultUpdateDelta = 1e-2;   // FP32+refine: refine when the FP32 residual fell 100x
-constexpr int kLdsRows = 2560;   // longest LDS tile list
-constexpr int kLdsBytes = 81920; // LDS of one E-gather workgroup (gfx950: 160 KB per CU, two workgroups)
+constexpr int kLdsRows = CPMGPU_LDS_ROWS;        // longest LDS tile list
+constexpr int kLdsBytes = 32 * CPMGPU_LDS_ROWS;  // LDS of one E-gather workgroup (gfx950: 160 KB per CU)
 constexpr int kExcPos = 72;      // positions per exception record (64 slots, extra entry, padding)
 
 // One instance group of a batched launch: an operator, NR right-hand sides

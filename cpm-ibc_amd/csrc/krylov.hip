@@ -152,7 +152,7 @@ __device__ void block_partials(const double (&v)[NV][NR], double* part, int lb, 
     }
 }
 
-constexpr int kXcdRun = 64;
+constexpr int kXcdRun = CPMGPU_XCD_RUN;
 
 template <typename T>
 __device__ __forceinline__ const Seg<T>& seg_of(const Seg<T>* __restrict__ segs, const int* __restrict__ bseg, int& lb, int64_t& row) {

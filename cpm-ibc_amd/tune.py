@@ -1,0 +1,131 @@
+"""Autotuner of libcpmgpu's kernel shape parameters on one MI355X.
+
+Parameters (csrc/tuned.h): CPMGPU_XCD_RUN, the run of consecutive row blocks
+each XCD takes, and CPMGPU_LDS_ROWS, the longest LDS tile list of the E
+gather.  Three steps:
+
+    python cpm-ibc_amd/tune.py build            # one library per variant under cpm-ibc_amd/_tune/
+    python cpm-ibc_amd/tune.py measure out.json # on the GPU: time every built variant
+    python cpm-ibc_amd/tune.py write out.json   # record the fastest in csrc/tuned.h
+
+Each variant is timed in a child process on the torus-knot system (Laplace
+on the torus R=1, r=0.4, h=0.02 with the 2000-segment knot IBC): the E gather
+in FP32 x 8 (50 launches, HIP events) and 32 right-hand sides streamed
+through 8 FP32+refine lanes (best of 3).  The stream time decides.
+"""
+import json
+import os
+import subprocess
+import sys
+import time
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+REPO = os.path.dirname(HERE)
+TUNE = os.path.join(HERE, "_tune")
+
+VARIANTS = [dict(xcd_run=x, lds_rows=2560) for x in (16, 32, 64, 128, 256)] + [dict(xcd_run=64, lds_rows=3072)]
+
+
+def name(v):
+    return f"x{v['xcd_run']}_l{v['lds_rows']}"
+
+
+def build():
+    jobs = str(min(16, os.cpu_count() or 4))
+    for v in VARIANTS:
+        lib = os.path.join(TUNE, name(v), "libcpmgpu.so")
+        obj = os.path.join("/tmp", "cpmgpu_tune", name(v))
+        extra = f"-DCPMGPU_XCD_RUN={v['xcd_run']} -DCPMGPU_LDS_ROWS={v['lds_rows']}"
+        print(f"building {name(v)}", flush=True)
+        subprocess.run(["make", "-s", "-j", jobs, "-C", HERE, f"BUILD={obj}", f"LIB={lib}", f"EXTRA={extra}"],
+                       check=True)
+
+
+def measure_one(lib):
+    sys.path.insert(0, os.path.join(HERE, "python"))
+    sys.path.insert(0, REPO)
+    import numpy as np
+    import cpmgpu
+    import bench
+    cpmgpu.load(lib)
+    ctx = cpmgpu.Context(0)
+    work = bench.torus_knot_workload(ctx, cpmgpu, 32)
+    op, rhs = work["op"], work["rhs"]
+    D = op.D
+    d_rhs = [ctx.alloc(8 * D) for _ in rhs]
+    d_x = [ctx.alloc(8 * D) for _ in rhs]
+    for d, r in zip(d_rhs, rhs):
+        ctx.h2d(d, np.ascontiguousarray(r))
+    o32 = cpmgpu.opts(tol=1e-10, precision=cpmgpu.CPM_FP32_REFINE)
+    ctx.solve_stream_device(op, d_rhs, d_x, 8, o32)
+    ctx.synchronize()
+    ts = []
+    for _ in range(3):
+        t0 = time.perf_counter()
+        infos = ctx.solve_stream_device(op, d_rhs, d_x, 8, o32)
+        ctx.synchronize()
+        ts.append(time.perf_counter() - t0)
+    assert all(i["status"] == "converged" for i in infos)
+    eg_ms, _ = op.profile_egather(cpmgpu.CPM_FP32_REFINE, 8, reps=50)
+    print(json.dumps(dict(stream_ms=1000 * min(ts), egather_us=1000 * eg_ms)))
+
+
+def measure(out):
+    res = []
+    for v in VARIANTS:
+        lib = os.path.join(TUNE, name(v), "libcpmgpu.so")
+        if not os.path.exists(lib):
+            raise SystemExit(f"{lib} missing: run `tune.py build` first")
+        print(f"measuring {name(v)}", flush=True)
+        p = subprocess.run([sys.executable, __file__, "one", lib], check=True, capture_output=True, text=True,
+                           timeout=600)
+        r = dict(v, **json.loads(p.stdout.strip().splitlines()[-1]))
+        print(json.dumps(r), flush=True)
+        res.append(r)
+    with open(out, "w") as f:
+        json.dump(res, f, indent=1)
+
+
+def write(src):
+    res = json.load(open(src))
+    best = min(res, key=lambda r: r["stream_ms"])
+    table = "\n".join(f"//   {r['xcd_run']:>8} {r['lds_rows']:>9} {r['egather_us']:>11.1f} {r['stream_ms']:>12.1f}"
+                      + ("   <- chosen" if r is best else "") for r in res)
+    text = f"""// Kernel shape parameters of libcpmgpu.so, chosen by cpm-ibc_amd/tune.py on one
+// MI355X (gfx950); a build can override them with -D.
+//
+//   CPMGPU_XCD_RUN   consecutive row blocks each XCD takes (workgroups are
+//                    dispatched to the 8 XCDs round-robin; runs keep a tile's
+//                    gathered x rows in that XCD's L2)
+//   CPMGPU_LDS_ROWS  longest LDS tile list of the E gather (FP32 x 8 rows of
+//                    32 B: the LDS of one workgroup is 32 B x this)
+//
+// Torus-knot system (405,928 rows): E gather FP32 x 8 per launch and 32
+// right-hand sides streamed through 8 FP32+refine lanes:
+//   xcd_run  lds_rows  gather (us)  stream (ms)
+{table}
+#pragma once
+#ifndef CPMGPU_XCD_RUN
+#define CPMGPU_XCD_RUN {best['xcd_run']}
+#endif
+#ifndef CPMGPU_LDS_ROWS
+#define CPMGPU_LDS_ROWS {best['lds_rows']}
+#endif
+"""
+    with open(os.path.join(HERE, "csrc", "tuned.h"), "w") as f:
+        f.write(text)
+    print(f"chosen: {best}")
+
+
+if __name__ == "__main__":
+    cmd = sys.argv[1] if len(sys.argv) > 1 else ""
+    if cmd == "build":
+        build()
+    elif cmd == "measure":
+        measure(sys.argv[2])
+    elif cmd == "one":
+        measure_one(sys.argv[2])
+    elif cmd == "write":
+        write(sys.argv[2])
+    else:
+        raise SystemExit(__doc__)

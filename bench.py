@@ -550,6 +550,16 @@ def main():
     x_tight = ctx.d2h(np.zeros(D), d_x[0])
     out["err_fp32refine_vs_fp64_tight"] = float(np.linalg.norm(x32[0] - x_tight) / np.linalg.norm(x_tight))
     out["err_fp64_vs_fp64_tight"] = float(np.linalg.norm(x64 - x_tight) / np.linalg.norm(x_tight))
+    if args.workload == "sphere":
+        # max surface error vs the exact u = z + 3xy at 200 x 200 (theta, phi) plot points, through the
+        # interpolation rows of the tube (Helpers::GetPlotInterpMatrix, src/Helpers.cpp:97-103)
+        th = np.repeat(np.arange(200) * (2 * np.pi / 199), 200)
+        ph = np.tile(np.arange(200) * (np.pi / 199), 200)
+        xp = np.stack([np.sin(ph) * np.cos(th), np.sin(ph) * np.sin(th), np.cos(ph)], 1)
+        Ep = work["tube"].interp(xp)
+        uex = xp[:, 2] + 3 * xp[:, 0] * xp[:, 1]
+        out["max_surface_error_fp64"] = float(np.abs(Ep.matvec(x_tight) - uex).max())
+        out["max_surface_error_fp32refine"] = float(np.abs(Ep.matvec(x32[0]) - uex).max())
     if "prob" in work:
         out["config"]["tube_nodes"] = work["prob"].info["tube_nodes"]
         out["config"]["ibc_dofs"] = work["prob"].info["ibc_dofs"]

@@ -418,8 +418,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (independent solves shard "
-              f"across GPUs with no communication; see DESIGN.md)", file=sys.stderr)
+        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (independent solves would shard "
+              f"across GPUs by instance with no communication)", file=sys.stderr)
         sys.exit(2)
 
     import cpmgpu

@@ -116,7 +116,7 @@ def test_first_order_and_neumann(gpu_ctx):
 
 
 def test_torus_knot_system(gpu_ctx):
-    """Config 2: Laplace on the torus R=1, r=0.4, h=0.02 with a 2000-segment (2,3) torus-knot Dirichlet curve."""
+    """The torus-knot system: Laplace on the torus R=1, r=0.4, h=0.02 with a 2000-segment (2,3) torus-knot Dirichlet curve."""
     print("case torus knot h=0.02", flush=True)
     dx = 0.02
     V = torus_knot_polyline()

@@ -86,8 +86,8 @@ def test_interp_matrix_plot_points(gpu_ctx):
 
 @pytest.mark.parametrize("prec", [cpmgpu.CPM_FP64, cpmgpu.CPM_FP32_REFINE])
 def test_solve_on_gpu_tube_matches_oracle(gpu_ctx, prec):
-    """Config-1 system (sphere, h=0.05, (1 - Laplace) u = 3z + 21xy) built entirely on the GPU."""
-    print(f"case cfg1 solve prec={prec}", flush=True)
+    """The unit-sphere system (h=0.05, (1 - Laplace) u = 3z + 21xy) built entirely on the GPU."""
+    print(f"case unit sphere solve prec={prec}", flush=True)
     P = orc.Problem("Sphere", 0.05)
     cp = P.tube()["cp"]
     b = 3 * cp[:, 2] + 21 * cp[:, 0] * cp[:, 1]

@@ -12,13 +12,13 @@
 //   xcd_run  lds_rows  gather (us)  stream (ms)
 //         16      2560        36.5        345.3
 //         32      2560        37.3        346.7
-//         64      2560        36.5        345.3
-//        128      2560        35.8        344.8   <- chosen
+//         64      2560        36.5        345.3   <- chosen
+//        128      2560        35.8        344.8
 //        256      2560        37.6        354.9
 //         64      3072        55.6        426.2
 #pragma once
 #ifndef CPMGPU_XCD_RUN
-#define CPMGPU_XCD_RUN 128
+#define CPMGPU_XCD_RUN 64
 #endif
 #ifndef CPMGPU_LDS_ROWS
 #define CPMGPU_LDS_ROWS 2560

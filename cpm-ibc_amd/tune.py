@@ -11,7 +11,9 @@ gather.  Three steps:
 Each variant is timed in a child process on the torus-knot system (Laplace
 on the torus R=1, r=0.4, h=0.02 with the 2000-segment knot IBC): the E gather
 in FP32 x 8 (50 launches, HIP events) and 32 right-hand sides streamed
-through 8 FP32+refine lanes (best of 3).  The stream time decides.
+through 8 FP32+refine lanes (best of 3).  The stream time decides; the
+default (64-block runs, 2,560 rows) stays unless a variant beats it by more
+than 0.5% (run-to-run noise; at equal time shorter runs fetched less from HBM).
 """
 import json
 import os
@@ -86,9 +88,16 @@ def measure(out):
         json.dump(res, f, indent=1)
 
 
+DEFAULT = dict(xcd_run=64, lds_rows=2560)   # kept unless a variant is more than 0.5% faster
+TIE = 0.005
+
+
 def write(src):
     res = json.load(open(src))
     best = min(res, key=lambda r: r["stream_ms"])
+    base = [r for r in res if r["xcd_run"] == DEFAULT["xcd_run"] and r["lds_rows"] == DEFAULT["lds_rows"]]
+    if base and base[0]["stream_ms"] <= best["stream_ms"] * (1 + TIE):
+        best = base[0]
     table = "\n".join(f"//   {r['xcd_run']:>8} {r['lds_rows']:>9} {r['egather_us']:>11.1f} {r['stream_ms']:>12.1f}"
                       + ("   <- chosen" if r is best else "") for r in res)
     text = f"""// Kernel shape parameters of libcpmgpu.so, chosen by cpm-ibc_amd/tune.py on one

@@ -192,7 +192,7 @@ def bench_surfaces(args, ctx, cpmgpu):
         "fp64_solves_per_s": round(n / t64, 3), "fp32refine_vs_fp64": round(n * args.steps / t_run / (n / t64), 3),
         "iters_fp64_mean": round(float(np.mean([i["iters"] for i in infos64])), 1),
         "roofline": dict(bound="hbm", achieved=round(achieved, 1), peak=HBM_PEAK_GBS, unit="GB/s",
-                         frac=round(achieved / HBM_PEAK_GBS, 4), traffic=None, kernel="k_egather<float,8>",
+                         frac=round(achieved / HBM_PEAK_GBS, 4), traffic=None, kernel=ops[big].gather_kernel(cpmgpu.CPM_FP32_REFINE, 8),
                          avg_launch_ms=round(eg_ms, 5), bytes_per_launch=int(eg_bytes),
                          on=f"largest tube ({ops[big].D} rows)"),
         "cpu_baseline": None,
@@ -502,7 +502,7 @@ def main():
     nr = 8 if B >= 8 else (4 if B >= 4 else (2 if B >= 2 else 1))
     eg_ms, eg_bytes = op.profile_egather(cpmgpu.CPM_FP32_REFINE, nr, reps=50)
     achieved = eg_bytes / (eg_ms * 1e-3) / 1e9
-    kname = f"k_egather<float,{nr}>"
+    kname = op.gather_kernel(cpmgpu.CPM_FP32_REFINE, nr)
     roof = dict(bound="hbm", achieved=round(achieved, 1), peak=HBM_PEAK_GBS, unit="GB/s",
                 frac=round(achieved / HBM_PEAK_GBS, 4), traffic=egather_traffic(kname, args.workload),
                 kernel=kname, avg_launch_ms=round(eg_ms, 5), bytes_per_launch=int(eg_bytes))

@@ -117,7 +117,7 @@ void Op::upload_ell(const std::vector<int>& ec, const std::vector<double>& ev, i
 
 template <typename T>
 static OpView<T> make_view(const Op& o, const Buffer& ev, const Buffer& lv, const Buffer& ld, const Buffer& tw,
-                           const Buffer& xv) {
+                           const Buffer& xv, const Buffer& gw) {
     OpView<T> v{};
     v.D = o.D;
     v.nD = o.nD;
@@ -143,18 +143,31 @@ static OpView<T> make_view(const Op& o, const Buffer& ev, const Buffer& lv, cons
     v.tlr = o.tlr.as<int>();
     v.tpl = o.tpl.as<int16_t>();
     v.tlx = o.tlx.as<int16_t>();
+    v.ngt = o.ngt;
+    v.nitem = o.nitem;
+    v.nslot = o.nslot;
+    v.gto = o.gto.as<int>();
+    v.gofs = o.gofs.as<int>();
+    v.grow = o.grow.as<int>();
+    v.gpl = o.gpl.as<int16_t>();
+    v.gw = gw.as<T>();
+    v.gln = o.gln.as<int>();
+    v.glo = o.glo.as<int>();
+    v.glr = o.glr.as<int>();
+    v.glx = o.glx.as<int16_t>();
     return v;
 }
 template <>
-OpView<double> Op::view<double>() const { return make_view<double>(*this, eval64, lval64, ldiag64, tpw64, xval64); }
+OpView<double> Op::view<double>() const { return make_view<double>(*this, eval64, lval64, ldiag64, tpw64, xval64, gw64); }
 template <>
-OpView<float> Op::view<float>() const { return make_view<float>(*this, eval32, lval32, ldiag32, tpw32, xval32); }
+OpView<float> Op::view<float>() const { return make_view<float>(*this, eval32, lval32, ldiag32, tpw32, xval32, gw32); }
 
 size_t Op::device_bytes() const {
     return ecol.bytes + eval64.bytes + eval32.bytes + lcol.bytes + lval64.bytes + lval32.bytes + ldiag64.bytes +
            ldiag32.bytes + flag.bytes + perm.bytes + tpc.bytes + tpw64.bytes + tpw32.bytes + exc.bytes + ecx.bytes +
            xcol.bytes + xval64.bytes + xval32.bytes + hrow.bytes + hcol.bytes + hval.bytes + tln.bytes + tlo.bytes +
-           tlr.bytes + tpl.bytes + tlx.bytes;
+           tlr.bytes + tpl.bytes + tlx.bytes + gto.bytes + gofs.bytes + grow.bytes + gpl.bytes + gw64.bytes + gw32.bytes + gln.bytes +
+           glo.bytes + glr.bytes + glx.bytes;
 }
 
 Op* op_from_csr(Ctx* ctx, int64_t D, int nD, const int32_t* Lp, const int32_t* Lc, const double* Lv,
@@ -370,8 +383,10 @@ static void run_batch(Ctx* ctx, const std::vector<Group>& groups, const std::vec
     int sweeps = 0;
     for (auto& g : groups)
         if (g.op->precond == CPM_PRECOND_JACOBI) sweeps = std::max(sweeps, g.op->precond_iters);
-    const LaunchCfg<double> c64{segs64, bseg, blocks, G, st, sweeps};
-    const LaunchCfg<float> c32{segs32, bseg, blocks, G, st, sweeps};
+    int grp = 1;
+    for (auto& g : groups) grp &= g.op->ngt > 0;
+    const LaunchCfg<double> c64{segs64, bseg, blocks, G, st, sweeps, grp};
+    const LaunchCfg<float> c32{segs32, bseg, blocks, G, st, sweeps, grp};
     KState* hs = ctx->host_states;
     const double delta = o.update_delta > 0 ? std::min(o.update_delta, 0.999) : kDefaultUpdateDelta;
     launch_reset_states(sts, nstate, o.tol * o.tol, o.max_iters, delta * delta, st);
@@ -521,7 +536,7 @@ static void run_stream(Ctx* ctx, Op* op, const std::vector<const double*>& rhs, 
     }
     HIPCHK(hipMemcpyAsync(sts, hst.data(), sizeof(KState) * kMaxNR, hipMemcpyHostToDevice, st));
     int sweeps = op->precond == CPM_PRECOND_JACOBI ? op->precond_iters : 0;
-    const LaunchCfg<float> c32{seg32, bseg, nb, 1, st, sweeps};
+    const LaunchCfg<float> c32{seg32, bseg, nb, 1, st, sweeps, op->ngt > 0};
     const MixVec mv{V64.x, V64.r, V64.b, V64.tmp};
     KState* hs = ctx->host_states;
     int done = 0;
@@ -671,7 +686,7 @@ void multiply(Op* op, const double* dx, double* dy, double m, double n) {
     HIPCHK(hipMemcpyAsync(dbseg, hb.data(), sizeof(int) * nblocks, hipMemcpyHostToDevice, st));
     double* w = ctx->workspace("mul_tmp", sizeof(double) * op->D * 3);
     double *xi = w, *yi = w + op->D, *tmp = w + 2 * op->D;
-    LaunchCfg<double> c{dseg, dbseg, nblocks, 1, st, 0};
+    LaunchCfg<double> c{dseg, dbseg, nblocks, 1, st, 0, op->ngt > 0};
     if (op->perm.p) {   // caller order -> internal order and back
         hipLaunchKernelGGL(k_pack, dim3(nblocks), dim3(256), 0, st, dpe, dbseg, xi, (float*)nullptr, 1);
         Launch<double, 1>::apply(c, xi, tmp, yi);
@@ -698,7 +713,7 @@ static void profile_egather_t(Op* op, int reps, double* avg_ms, double* bytes) {
     T* x = reinterpret_cast<T*>(ctx->workspace("prof_x", sizeof(T) * op->D * NR));
     T* y = reinterpret_cast<T*>(ctx->workspace("prof_y", sizeof(T) * op->D * NR));
     HIPCHK(hipMemsetAsync(x, 0, sizeof(T) * op->D * NR, st));
-    LaunchCfg<T> c{dseg, dbseg, nblocks, 1, st, 0};
+    LaunchCfg<T> c{dseg, dbseg, nblocks, 1, st, 0, op->ngt > 0};
     launch_egather<T, NR>(c, x, y);   // warm-up
     hipEvent_t e0, e1;
     HIPCHK(hipEventCreate(&e0));
@@ -723,6 +738,20 @@ static void profile_egather_t(Op* op, int reps, double* avg_ms, double* bytes) {
     // extra entry; LDS tiles also read their lists, 4 bytes per row)
     double opb;
     const double sT = sizeof(T);
+    if (op->fmt == OP_TP && op->ngt > 0 && grp_gather(sizeof(T), NR) && GrpTile<T, NR>::kRows > 0) {
+        // shared-stencil tiles: per plain item 16 int16 positions and its
+        // slot offset, per plain row 12 weights and its row index; exception
+        // rows as in the row tiles plus slot offset and row index; identity
+        // items their slot offset, identity rows their row index; 4 bytes per
+        // list row and 12 per tile
+        opb = (double)op->grp_items0 * (16 * 2.0 + 4) + (double)op->grp_rows0 * (12 * sT + 4) +
+              (double)op->grp_exc * (72 * 2.0 + 12 * sT + 4 + sT + 8) +
+              (double)op->grp_ident_items * 4.0 + (double)op->grp_ident * 4.0 + 4.0 * (double)op->grp_list +
+              12.0 * (double)op->ngt;
+        *avg_ms = ms / reps;
+        *bytes = opb + 2.0 * op->D * NR * sizeof(T);
+        return;
+    }
     if (op->fmt == OP_TP && op->tln.p) {
         const int64_t* n = op->tile_rows;
         opb = (double)n[0] * (16 * 2.0 + 12 * sT + 4) + (double)n[1] * (72 * 2.0 + 12 * sT + 4 + sT) +
@@ -1184,7 +1213,9 @@ struct Server : ServerBase {
         const hipStream_t st = ctx->stream;
         const Vecs<float> V32{nullptr, v32(0), v32(1), v32(2), v32(3), v32(4), v32(5), v32(6), v32(7), v32(8), v32(9)};
         const MixVec mv{v64(1), v64(2), v64(0), v64(4)};
-        const LaunchCfg<float> c{d_segs, d_bseg, nblocks, (int)reqs.size(), st, 0};
+        int grp = 1;
+        for (const Req& q : reqs) grp &= q.op->ngt > 0;
+        const LaunchCfg<float> c{d_segs, d_bseg, nblocks, (int)reqs.size(), st, 0, grp};
         Vecs<float> w = V32;
         for (int i = 0; i < iters; ++i) Launch<float, NR>::iteration(c, w, true, mv);
         HIPCHK(hipGetLastError());
@@ -1346,6 +1377,39 @@ int cpm_op_destroy(cpm_op* op) {
 int64_t cpm_op_rows(const cpm_op* op) { return op ? O(op)->D : -1; }
 
 int cpm_op_format(const cpm_op* op) { return op ? O(op)->fmt : -1; }
+
+int cpm_op_gather_kind(const cpm_op* op, int precision, int nrhs) {
+    if (!op || (nrhs != 1 && nrhs != 2 && nrhs != 4 && nrhs != 8)) return -1;
+    const Op* o = O(op);
+    const bool f32 = precision == CPM_FP32_REFINE;
+    const bool fits = f32 ? (nrhs == 1 ? GrpTile<float, 1>::kRows : nrhs == 2 ? GrpTile<float, 2>::kRows
+                             : nrhs == 4 ? GrpTile<float, 4>::kRows : GrpTile<float, 8>::kRows) > 0
+                          : (nrhs == 1 ? GrpTile<double, 1>::kRows : nrhs == 2 ? GrpTile<double, 2>::kRows
+                             : nrhs == 4 ? GrpTile<double, 4>::kRows : GrpTile<double, 8>::kRows) > 0;
+    if (o->fmt == OP_TP && o->ngt > 0 && fits && grp_gather(f32 ? 4 : 8, nrhs)) return 2;
+    const int bytes = (f32 ? 4 : 8) * nrhs;
+    if (o->fmt == OP_TP && o->tln.p && kLdsRows * bytes <= kLdsBytes) return 1;
+    return 0;
+}
+
+int cpm_op_tile_stats(const cpm_op* op, int64_t* out) {
+    return guard([&] {
+        if (!op || !out) throw ArgError("null operator or output");
+        const Op* o = O(op);
+        int64_t row_tiles = 0;
+        if (o->tln.p) {
+            std::vector<int> tln(o->tln.bytes / 4);
+            HIPCHK(hipMemcpy(tln.data(), o->tln.p, o->tln.bytes, hipMemcpyDeviceToHost));
+            for (int v : tln) row_tiles += v >= 0;
+        }
+        out[0] = o->ngt;
+        out[1] = o->nitem;
+        out[2] = o->grp_items0;
+        out[3] = o->grp_list;
+        out[4] = row_tiles;
+        out[5] = o->tile_list;
+    });
+}
 
 int cpm_op_set_mn(cpm_op* op, double m, double n) {
     return guard([&] {

@@ -86,12 +86,48 @@ struct OpView {
     const int* tlr;      // lists of x rows
     const int16_t* tpl;  // [16][D] position of line (i, j)'s first row in the tile's list
     const int16_t* tlx;  // [nexc][kExcPos] positions of an exception row's 64 columns and extra entry
+    // Shared-stencil tiles (built beside the row tiles; gln, glo, glr, glx
+    // are their tln, tlo, tlr, tlx).  Rows whose E stencils are the same
+    // 4x4x4 block of nodes are handled by one thread, which reads each x row
+    // of the block from LDS once for all of them.  Tile t owns the items
+    // gto[t] .. gto[t + 1] - 1, one per thread; an item is up to kGrpRows
+    // rows with one stencil (kind 0), one exception row (kind 1) or up to
+    // kGrpRows identity rows (kind 2), listed in slots gofs[i] .. gofs[i + 1] - 1.
+    int ngt;             // shared-stencil tiles (0: none)
+    int nitem;
+    int64_t nslot;
+    const int* gto;      // [ngt + 1]
+    const int* gofs;     // [nitem + 1] first slot | kind << kGrpKindShift
+    const int* grow;     // [nslot] row of each slot
+    const int16_t* gpl;  // [16][nitem] position of line (i, j)'s first row in the tile's list (kind 0)
+    const T* gw;         // [12][nslot] the slot's row's wx[4], wy[4], wz[4]
+    const int* gln;
+    const int* glo;
+    const int* glr;
+    const int16_t* glx;
 };
 
 constexpr double kDefaultUpdateDelta = 1e-2;   // FP32+refine: refine when the FP32 residual fell 100x
 constexpr int kLdsRows = CPMGPU_LDS_ROWS;        // longest LDS tile list
 constexpr int kLdsBytes = 32 * CPMGPU_LDS_ROWS;  // LDS of one E-gather workgroup (gfx950: 160 KB per CU)
 constexpr int kExcPos = 72;      // positions per exception record (64 slots, extra entry, padding)
+#ifndef CPMGPU_GROUP_ROWS
+#define CPMGPU_GROUP_ROWS 2
+#endif
+#ifndef CPMGPU_GROUP_LDS_ROWS
+#define CPMGPU_GROUP_LDS_ROWS 1280
+#endif
+constexpr int kGrpRows = CPMGPU_GROUP_ROWS;   // rows per shared-stencil item
+constexpr int kGrpLdsRows = CPMGPU_GROUP_LDS_ROWS;        // longest shared-stencil tile list
+constexpr int kGrpLdsBytes = 32 * CPMGPU_GROUP_LDS_ROWS;  // LDS of one shared-stencil workgroup
+// LDS rows of a shared-stencil tile for T x NR values per row (0: these
+// tiles are not used for this value size).
+template <typename T, int NR>
+struct GrpTile {
+    static constexpr int kRows = (kGrpLdsRows * NR * (int)sizeof(T) <= kGrpLdsBytes) ? kGrpLdsRows : 0;
+};
+constexpr int kGrpKindShift = 28;             // item kind above the slot offset in gofs
+constexpr int kGrpOfsMask = (1 << kGrpKindShift) - 1;
 
 // One instance group of a batched launch: an operator, NR right-hand sides
 // and the contiguous block range that covers its D rows.

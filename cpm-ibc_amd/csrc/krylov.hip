@@ -52,6 +52,8 @@
 #include <hip/hip_runtime.h>
 
 #include <cmath>
+#include <cstdlib>
+#include <string>
 #include <type_traits>
 
 #include "engine.h"
@@ -396,10 +398,9 @@ struct ExcRow {
     __device__ __forceinline__ int pos(int s) const { return (int)(int16_t)(pk[s >> 1] >> (16 * (s & 1))); }
 };
 
-template <typename T>
-__device__ __forceinline__ void exc_row_load(const OpView<T>& op, int e, ExcRow& m) {
+__device__ __forceinline__ void exc_row_load(const int16_t* tlx, int e, ExcRow& m) {
     typedef int I4 __attribute__((ext_vector_type(4)));
-    const I4* src = reinterpret_cast<const I4*>(op.tlx + (size_t)e * kExcPos);
+    const I4* src = reinterpret_cast<const I4*>(tlx + (size_t)e * kExcPos);
 #pragma unroll
     for (int q = 0; q < kExcPos / 8; ++q) {
         const I4 v = ldg(src + q);
@@ -432,6 +433,121 @@ __device__ __forceinline__ void egather_tp_lds_exc(const OpView<T>& op, int64_t 
     }
     const int xp = m.pos(64);
     if (xp >= 0) acc += ldg(&op.xval[e]) * lx.load(xp);
+}
+
+// A kind-0 item of a shared-stencil tile: up to kGrpRows rows with the same
+// stencil block (absent rows have weight 0 and are not stored).
+template <typename T>
+struct GrpItem {
+    T w[kGrpRows][12];   // wx[4], wy[4], wz[4] of each row
+    int p[16];           // position of each line's first x row in the tile's list
+    int row[kGrpRows];
+};
+
+template <typename T>
+__device__ __forceinline__ void grp_item_load(const OpView<T>& op, int item, int s0, int n, GrpItem<T>& m) {
+    const int64_t S = op.nslot, NI = op.nitem;
+#pragma unroll
+    for (int l = 0; l < 16; ++l) m.p[l] = ldg(&op.gpl[l * NI + item]);
+#pragma unroll
+    for (int g = 0; g < kGrpRows; ++g) {
+        const bool on = g < n;
+        m.row[g] = on ? ldg(&op.grow[s0 + g]) : 0;
+#pragma unroll
+        for (int q = 0; q < 12; ++q) m.w[g][q] = on ? ldg(&op.gw[q * S + s0 + g]) : (T)0;
+    }
+}
+
+// The rows of a kind-0 item from the tile's LDS copy of x: every x row of the
+// block is read once and used by all rows of the item; each row's products
+// and sums are those of egather_tp_lds, in its order, so all paths agree to
+// the bit.
+template <typename T, typename V, int W>
+__device__ __forceinline__ void egather_grp_lds(const GrpItem<T>& m, int n, const LdsRows<V>& lx, V (&acc)[kGrpRows]) {
+#pragma unroll
+    for (int l = 0; l < 16; ++l) {
+        const int q = m.p[l];
+        const V x0 = lx.load(q), x1 = lx.load(q + 1), x2 = lx.load(q + 2), x3 = lx.load(q + 3);
+#pragma unroll
+        for (int g = 0; g < kGrpRows; ++g) {
+            if (g < n) {
+                const T wx = m.w[g][l >> 2], wy = m.w[g][4 + (l & 3)];
+                acc[g] += ((m.w[g][8] * wy) * wx) * x0;
+                acc[g] += ((m.w[g][9] * wy) * wx) * x1;
+                acc[g] += ((m.w[g][10] * wy) * wx) * x2;
+                acc[g] += ((m.w[g][11] * wy) * wx) * x3;
+            }
+        }
+    }
+}
+
+// The tile's x rows (list lst of nu rows) into LDS, all loads in flight
+// together, then a workgroup barrier.
+template <typename T, int NR, int LR, typename V>
+__device__ __forceinline__ void lds_fill(const int* lst, int nu, const T* xb, const LdsRows<V>& lrows) {
+    constexpr int KF = LR > 0 ? (LR + kBlock - 1) / kBlock : 1;
+    int src[KF];
+#pragma unroll
+    for (int k = 0; k < KF; ++k) {
+        const int u = (int)threadIdx.x + k * kBlock;
+        src[k] = u < nu ? ldg(&lst[u]) : 0;
+    }
+    V buf[KF];
+#pragma unroll
+    for (int k = 0; k < KF; ++k)
+        if ((int)threadIdx.x + k * kBlock < nu) buf[k] = vload<T, NR>(xb + (size_t)src[k] * NR);
+#pragma unroll
+    for (int k = 0; k < KF; ++k) {
+        const int u = (int)threadIdx.x + k * kBlock;
+        if (u < nu) lrows.store(u, buf[k]);
+    }
+    __syncthreads();
+}
+
+// One shared-stencil tile of E x: thread t takes item gto[tile] + t.
+template <typename T, int NR, int LR, typename V>
+__device__ __forceinline__ void egather_grp_tile(const Seg<T>& sg, int tile, const T* xb, T* out,
+                                                 const LdsRows<V>& lrows) {
+    const OpView<T>& op = sg.op;
+    const int i0 = ldg(&op.gto[tile]), i1 = ldg(&op.gto[tile + 1]);
+    const int item = i0 + (int)threadIdx.x;
+    int s0 = 0, n = 0, kind = 3;
+    if (item < i1) {
+        const int a = ldg(&op.gofs[item]), b = ldg(&op.gofs[item + 1]);
+        s0 = a & kGrpOfsMask;
+        kind = a >> kGrpKindShift;
+        n = (b & kGrpOfsMask) - s0;
+    }
+    GrpItem<T> m;
+    ExcRow mx;
+    int r1 = 0, e = -1;
+    if (kind == 0) {
+        grp_item_load(op, item, s0, n, m);
+    } else if (kind == 1) {
+        r1 = ldg(&op.grow[s0]);
+        e = ldg(&op.exc[r1]);
+        exc_row_load(op.glx, e, mx);
+    }
+    lds_fill<T, NR, LR>(op.glr + ldg(&op.glo[tile]), ldg(&op.gln[tile]), xb, lrows);
+    T* ob = out + sg.off;
+    if (kind == 0) {
+        V acc[kGrpRows];
+#pragma unroll
+        for (int g = 0; g < kGrpRows; ++g) acc[g] = (V)(T)0;
+        egather_grp_lds<T, V, NR>(m, n, lrows, acc);
+#pragma unroll
+        for (int g = 0; g < kGrpRows; ++g)
+            if (g < n) vstore<T, NR>(ob + (size_t)m.row[g] * NR, acc[g]);
+    } else if (kind == 1) {
+        V acc = (V)(T)0;
+        egather_tp_lds_exc<T, V, NR>(op, r1, e, mx, lrows, acc);   // rows an interior boundary moved
+        vstore<T, NR>(ob + (size_t)r1 * NR, acc);
+    } else if (kind == 2) {
+        for (int g = 0; g < n; ++g) {
+            const int r = ldg(&op.grow[s0 + g]);
+            vstore<T, NR>(ob + (size_t)r * NR, vload<T, NR>(xb + (size_t)r * NR));
+        }
+    }
 }
 
 // E row of the ELL format (eight slots in flight).
@@ -593,25 +709,8 @@ __global__ __launch_bounds__(kBlock) void k_egather(const Seg<T>* __restrict__ s
         TpRow<T> m;
         ExcRow mx;
         if (kind == 0) tp_row_load(sg.op, row, m);
-        else if (kind == 1) exc_row_load(sg.op, e, mx);
-        const int* lst = sg.op.tlr + ldg(&sg.op.tlo[lb]);
-        constexpr int KF = LR > 0 ? (LR + kBlock - 1) / kBlock : 1;
-        int src[KF];
-#pragma unroll
-        for (int k = 0; k < KF; ++k) {
-            const int u = (int)threadIdx.x + k * kBlock;
-            src[k] = u < nu ? ldg(&lst[u]) : 0;
-        }
-        V buf[KF];
-#pragma unroll
-        for (int k = 0; k < KF; ++k)
-            if ((int)threadIdx.x + k * kBlock < nu) buf[k] = vload<T, NR>(xb + (size_t)src[k] * NR);
-#pragma unroll
-        for (int k = 0; k < KF; ++k) {
-            const int u = (int)threadIdx.x + k * kBlock;
-            if (u < nu) lrows.store(u, buf[k]);
-        }
-        __syncthreads();
+        else if (kind == 1) exc_row_load(sg.op.tlx, e, mx);
+        lds_fill<T, NR, LR>(sg.op.tlr + ldg(&sg.op.tlo[lb]), nu, xb, lrows);
         if (act) {
             V acc = (V)(T)0;
             if (kind == 2) acc = vload<T, NR>(xb + (size_t)row * NR);
@@ -622,6 +721,33 @@ __global__ __launch_bounds__(kBlock) void k_egather(const Seg<T>* __restrict__ s
     } else if (main && row < sg.op.D) {
         vstore<T, NR>(out + i, erow<T, NR>(sg.op, row, xb, NR));
     }
+    if (row >= sg.op.D) return;
+    if (std::is_same<T, float>::value && (lanes & M_UPDATE) && seg_any(L, M_UPDATE)) {
+        for (int k = 0; k < sg.nr; ++k)
+            if (lane_in(L, k, M_UPDATE))
+                stg(&mv.tmp64[lm_at(sg, k, row)], erow<double, 1>(sg.op64, row, mv.x64 + lm_at(sg, k, 0), 1));
+    }
+}
+
+// E x for launches whose every segment has shared-stencil tiles: block lb of
+// a segment takes its tile lb (blocks past the last tile only do the FP64
+// gathers of refinement lanes).  40 KB of LDS and at most 128 VGPRs, so four
+// workgroups share a CU.
+template <typename T, int NR>
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))) void k_egather_grp(
+    const Seg<T>* __restrict__ segs, const int* __restrict__ bseg, const T* x, T* out, unsigned lanes, MixVec mv) {
+    typedef typename VT<T, NR>::type V;
+    constexpr int LR = GrpTile<T, NR>::kRows;
+    static_assert(LR > 0, "shared-stencil tiles do not fit this value size");
+    __shared__ V lx[LR];
+    const LdsRows<V> lrows{lx, LR};
+    int lb;
+    int64_t row;
+    const Seg<T>& sg = seg_of(segs, bseg, lb, row);
+    const auto L = lane_stat<NR>(sg, lanes != 0);
+    if (lanes && !seg_any(L, lanes)) return;               // uniform per workgroup
+    const bool main = !lanes || seg_any(L, lanes & ~M_UPDATE);
+    if (main && lb < sg.op.ngt) egather_grp_tile<T, NR, LR>(sg, lb, x + sg.off, out, lrows);
     if (row >= sg.op.D) return;
     if (std::is_same<T, float>::value && (lanes & M_UPDATE) && seg_any(L, M_UPDATE)) {
         for (int k = 0; k < sg.nr; ++k)
@@ -1101,8 +1227,29 @@ void launch_reset_states(KState* st, int n, double tol2, int max_it, double delt
 // ---------------------------------------------------------------- launchers
 #define LAUNCH(kern, ...) \
     hipLaunchKernelGGL(kern, dim3(c.nblocks), dim3(kBlock), 0, c.stream, c.segs, c.nseg == 1 ? nullptr : c.bseg, __VA_ARGS__)
-#define EGATHER(...) LAUNCH((k_egather<T, NR>), __VA_ARGS__)
+#define EGATHER(...)                                                         \
+    do {                                                                     \
+        if constexpr (GrpTile<T, NR>::kRows > 0) {                           \
+            if (c.grp && grp_gather(sizeof(T), NR)) {                        \
+                LAUNCH((k_egather_grp<T, NR>), __VA_ARGS__);                 \
+                break;                                                       \
+            }                                                                \
+        }                                                                    \
+        LAUNCH((k_egather<T, NR>), __VA_ARGS__);                             \
+    } while (0)
 #define FIN(T, NR, KIND) hipLaunchKernelGGL((k_fin<T, NR, KIND>), dim3(c.nseg * NR), dim3(kBlock), 0, c.stream, c.segs, c.nseg)
+
+// Value shapes whose E gathers use shared-stencil tiles when every operator
+// of the launch has them: FP32 x 8, x 2 and x 1 by default, the shapes where
+// they were measured faster than row tiles on the torus-knot system (FP32 x 4
+// and the FP64 shapes need more than 128 VGPRs with them).
+// CPMGPU_GROUP_GATHER=all uses them for every shape whose tile fits the LDS,
+// =none for none.
+bool grp_gather(int tsize, int nr) {
+    const char* e = std::getenv("CPMGPU_GROUP_GATHER");
+    const std::string v = e ? e : "";
+    return v == "all" || (v != "none" && tsize == 4 && nr != 4);
+}
 
 template <typename T, int NR>
 void Launch<T, NR>::init(const LaunchCfg<T>& c, const T* b, T* x, T* r, T* r0, T* p, T* v) {

@@ -47,10 +47,16 @@ struct Op {
     int64_t nexc = 0, nmoved = 0;
     double loff = 0, ldu = 0;
     Buffer tln, tlo, tlr, tpl, tlx;   // LDS tiles (OP_TP)
+    Buffer gto, gofs, grow, gpl, gw64, gw32, gln, glo, glr, glx;   // shared-stencil tiles (OP_TP)
+    int ngt = 0, nitem = 0;
+    int64_t nslot = 0;
     // gathered (non-identity) rows: [0] plain and [1] exception rows of LDS
-    // tiles, [2] plain and [3] exception rows read directly; rows in all lists
+    // tiles, [2] plain and [3] exception rows read directly; rows in all lists;
+    // kind-0 items of shared-stencil tiles
     int64_t tile_rows[4] = {0, 0, 0, 0};
     int64_t tile_list = 0;
+    int64_t grp_items0 = 0, grp_list = 0;
+    int64_t grp_rows0 = 0, grp_exc = 0, grp_ident = 0, grp_ident_items = 0;
     // Lfull rows of the rows that solves treat as identity rows (internal
     // order, hw slots each, padded with weight 0): explicit time stepping
     // applies Lfull to every row (problems/HeatWithIBC.cpp:84-88).  Entries

@@ -20,6 +20,8 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
+#include <climits>
 #include <mutex>
 #include <thread>
 #include <cstdlib>
@@ -243,10 +245,243 @@ static void ensure_lex(Tube& t, Ctx* ctx) {
     t.rlex = std::move(rlex);
 }
 
-// CPMGPU_LDS_TILES=0 builds tensor-product operators without LDS tiles.
-static bool lds_tiles_disabled() {
+// CPMGPU_LDS_TILES selects the LDS tiles built with tensor-product
+// operators: unset or "1", row tiles and shared-stencil tiles (these unless
+// they would need more workgroups than the operator has row blocks); "rows",
+// row tiles only; "0", none (every gather reads x directly).
+static int lds_tile_mode() {
     const char* e = std::getenv("CPMGPU_LDS_TILES");
-    return e && std::string(e) == "0";
+    if (!e) return 2;
+    const std::string v(e);
+    return v == "0" ? 0 : v == "rows" ? 1 : 2;
+}
+
+// The slots' weights in slot order (grouped copy of tpw).
+template <typename T>
+__global__ void k_slot_weights(const T* tpw, int64_t D, const int* grow, int64_t S, T* gw) {
+    const int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= S) return;
+    const int64_t r = grow[s];
+#pragma unroll
+    for (int q = 0; q < 12; ++q) gw[q * S + s] = tpw[q * D + r];
+}
+
+// Tensor-product rows on the host, for the tile builders.
+struct HostTp {
+    int64_t D = 0, ne = 0;
+    std::vector<uint8_t> flag;
+    std::vector<int> tpc, exc, ecx, xcol;   // [16][D], [D], [ne][64], [ne]
+    HostTp(int64_t D_, int64_t ne_, const Buffer& tpc_, const Buffer& exc_, const Buffer& ecx_, const Buffer& xcol_,
+           const Buffer& flag_)
+        : D(D_), ne(ne_), flag((size_t)D_), tpc((size_t)16 * D_), exc((size_t)D_),
+          ecx((size_t)std::max<int64_t>(ne_, 1) * 64), xcol((size_t)std::max<int64_t>(ne_, 1)) {
+        HIPCHK(hipMemcpy(flag.data(), flag_.p, flag.size(), hipMemcpyDeviceToHost));
+        HIPCHK(hipMemcpy(tpc.data(), tpc_.p, tpc.size() * 4, hipMemcpyDeviceToHost));
+        HIPCHK(hipMemcpy(exc.data(), exc_.p, exc.size() * 4, hipMemcpyDeviceToHost));
+        if (ne > 0) {
+            HIPCHK(hipMemcpy(ecx.data(), ecx_.p, (size_t)ne * 64 * 4, hipMemcpyDeviceToHost));
+            HIPCHK(hipMemcpy(xcol.data(), xcol_.p, (size_t)ne * 4, hipMemcpyDeviceToHost));
+        }
+    }
+    // the x rows row r's E row reads (kind 0: 16 lines of 4; exception rows:
+    // the 64 slot columns and the extra column)
+    template <typename F>
+    void columns(int64_t r, F&& f) const {
+        const int e = exc[(size_t)r];
+        if (e >= 0) {
+            for (int s = 0; s < 64; ++s) f(ecx[(size_t)e * 64 + s]);
+            if (xcol[(size_t)e] >= 0) f(xcol[(size_t)e]);
+            return;
+        }
+        for (int l = 0; l < 16; ++l)
+            for (int k = 0; k < 4; ++k) f(tpc[(size_t)l * D + r] + k);
+    }
+};
+
+static void upload(Buffer& b, const void* p, size_t bytes) {
+    b = Buffer(bytes);
+    HIPCHK(hipMemcpy(b.p, p, bytes, hipMemcpyHostToDevice));
+}
+
+template <typename Body>
+static void parallel_for(int64_t n, Body&& body) {
+    const int nth = (int)std::max<int64_t>(
+        1, std::min<int64_t>({16, n, (int64_t)std::max(1u, std::thread::hardware_concurrency())}));
+    std::vector<std::thread> th;
+    for (int w = 0; w < nth; ++w)
+        th.emplace_back([&, w] {
+            for (int64_t i = n * w / nth; i < n * (w + 1) / nth; ++i) body(i);
+        });
+    for (auto& x : th) x.join();
+}
+
+// Shared-stencil tiles.  Plain rows are sorted by their stencil block (the
+// first column of line (0, 0)) and rows with the same block are cut into
+// items of up to kGrpRows rows; exception rows are single items placed by
+// their smallest column, identity rows are items of up to kGrpRows rows at
+// the end.  Tiles take items in that order while the tile's list of distinct
+// x rows stays within kGrpLdsRows and its items within kBlock; inside a tile,
+// items are ordered by row count (larger first) so the waves' loops over the
+// rows of an item end together.  Returns false (nothing built) if the tiles
+// would outnumber the operator's row blocks, which size every launch.
+static bool build_group_tiles(Op& op, const HostTp& h, const Buffer& tpw64, const Buffer& tpw32) {
+    const int64_t D = h.D;
+    int max_items = kBlock;   // CPMGPU_GROUP_ITEMS: fewer items per tile (more, smaller tiles)
+    if (const char* e = std::getenv("CPMGPU_GROUP_ITEMS")) max_items = std::max(1, std::min(kBlock, std::atoi(e)));
+    const int64_t nblocks = (D + kBlock - 1) / kBlock;
+    if (D >= (int64_t)kGrpOfsMask) return false;
+    struct Item {
+        int64_t key;
+        int r0, n, kind;   // rows trow[r0 .. r0 + n)
+    };
+    std::vector<std::pair<int64_t, int>> plain, special;
+    std::vector<int> ident;
+    for (int64_t r = 0; r < D; ++r) {
+        if (h.flag[(size_t)r] & ROW_IDENTITY) {
+            ident.push_back((int)r);
+        } else if (h.exc[(size_t)r] >= 0) {
+            int64_t k = INT64_MAX;
+            h.columns(r, [&](int c) { k = std::min<int64_t>(k, c); });
+            special.push_back({k, (int)r});
+        } else {
+            plain.push_back({h.tpc[(size_t)r], (int)r});
+        }
+    }
+    std::sort(plain.begin(), plain.end());
+    std::vector<int> trow;
+    trow.reserve((size_t)D);
+    std::vector<Item> items;
+    for (size_t a = 0; a < plain.size();) {
+        size_t b = a;
+        while (b < plain.size() && plain[b].first == plain[a].first) ++b;
+        for (size_t c = a; c < b; c += kGrpRows) {
+            const int n = (int)std::min<size_t>(kGrpRows, b - c);
+            items.push_back({plain[a].first, (int)trow.size(), n, 0});
+            for (int g = 0; g < n; ++g) trow.push_back(plain[c + g].second);
+        }
+        a = b;
+    }
+    const int64_t items0 = (int64_t)items.size();
+    for (const auto& pr : special) {
+        items.push_back({pr.first, (int)trow.size(), 1, 1});
+        trow.push_back(pr.second);
+    }
+    std::stable_sort(items.begin(), items.end(), [](const Item& u, const Item& v) { return u.key < v.key; });
+    for (size_t c = 0; c < ident.size(); c += kGrpRows) {
+        const int n = (int)std::min<size_t>(kGrpRows, ident.size() - c);
+        items.push_back({INT64_MAX, (int)trow.size(), n, 2});
+        for (int g = 0; g < n; ++g) trow.push_back(ident[c + g]);
+    }
+    const int64_t nitem = (int64_t)items.size();
+    // greedy tiles: stamp[c] = tile whose list holds column c
+    std::vector<int> stamp((size_t)D, -1), fresh;
+    std::vector<int> first{0};
+    int tile = 0, list = 0, nin = 0;
+    for (int64_t i = 0; i < nitem;) {
+        const Item& it = items[(size_t)i];
+        fresh.clear();
+        if (it.kind != 2)
+            h.columns(trow[(size_t)it.r0], [&](int c) {
+                if (stamp[(size_t)c] != tile) {
+                    stamp[(size_t)c] = tile;
+                    fresh.push_back(c);
+                }
+            });
+        if (nin > 0 && (list + (int)fresh.size() > kGrpLdsRows || nin + 1 > max_items)) {
+            for (int c : fresh) stamp[(size_t)c] = -1;
+            first.push_back((int)i);
+            ++tile;
+            list = 0;
+            nin = 0;
+            continue;   // the item again, in a new tile
+        }
+        list += (int)fresh.size();
+        ++nin;
+        ++i;
+    }
+    first.push_back((int)nitem);
+    const int64_t ngt = (int64_t)first.size() - 1;
+    if (ngt > nblocks) return false;
+    // per tile: items by row count, the sorted list, positions
+    std::vector<std::vector<int>> lists((size_t)ngt);
+    std::vector<int> order((size_t)nitem);
+    std::vector<int16_t> gpl((size_t)16 * std::max<int64_t>(nitem, 1), 0),
+        tlx((size_t)std::max<int64_t>(h.ne, 1) * kExcPos, -1);
+    std::atomic<int> bad{0};
+    parallel_for(ngt, [&](int64_t t) {
+        const int i0 = first[(size_t)t], i1 = first[(size_t)t + 1];
+        for (int i = i0; i < i1; ++i) order[(size_t)i] = i;
+        std::stable_sort(order.begin() + i0, order.begin() + i1,
+                         [&](int u, int v) { return items[(size_t)u].n > items[(size_t)v].n; });
+        std::vector<int>& u = lists[(size_t)t];
+        for (int i = i0; i < i1; ++i)
+            if (items[(size_t)i].kind != 2) h.columns(trow[(size_t)items[(size_t)i].r0], [&](int c) { u.push_back(c); });
+        std::sort(u.begin(), u.end());
+        u.erase(std::unique(u.begin(), u.end()), u.end());
+        if ((int64_t)u.size() > kGrpLdsRows || i1 - i0 > kBlock) bad = 1;
+        auto pos = [&](int c) { return (int16_t)(std::lower_bound(u.begin(), u.end(), c) - u.begin()); };
+        for (int q = i0; q < i1; ++q) {
+            const Item& it = items[(size_t)order[(size_t)q]];
+            const int r = trow[(size_t)it.r0];
+            if (it.kind == 0) {
+                for (int l = 0; l < 16; ++l) {
+                    const int16_t p = pos(h.tpc[(size_t)l * D + r]);
+                    if (p < 0 || p + 3 >= (int)u.size()) bad = 1;
+                    gpl[(size_t)l * nitem + q] = p;
+                }
+            } else if (it.kind == 1) {
+                const int e = h.exc[(size_t)r];
+                for (int s = 0; s < 64; ++s) tlx[(size_t)e * kExcPos + s] = pos(h.ecx[(size_t)e * 64 + s]);
+                if (h.xcol[(size_t)e] >= 0) tlx[(size_t)e * kExcPos + 64] = pos(h.xcol[(size_t)e]);
+            }
+        }
+    });
+    if (bad) throw std::logic_error("shared-stencil tiles: a list or position is out of range");
+    // slots in final item order
+    std::vector<int> gofs((size_t)nitem + 1), grow;
+    grow.reserve((size_t)D);
+    for (int64_t q = 0; q < nitem; ++q) {
+        const Item& it = items[(size_t)order[(size_t)q]];
+        gofs[(size_t)q] = (int)grow.size() | (it.kind << kGrpKindShift);
+        for (int g = 0; g < it.n; ++g) grow.push_back(trow[(size_t)it.r0 + g]);
+    }
+    gofs[(size_t)nitem] = (int)grow.size();
+    if ((int64_t)grow.size() != D) throw std::logic_error("shared-stencil tiles: not every row has a slot");
+    std::vector<int> tln((size_t)ngt), tlo((size_t)ngt), tlr;
+    op.grp_list = 0;
+    for (int64_t t = 0; t < ngt; ++t) {
+        tln[(size_t)t] = (int)lists[(size_t)t].size();
+        tlo[(size_t)t] = (int)tlr.size();
+        tlr.insert(tlr.end(), lists[(size_t)t].begin(), lists[(size_t)t].end());
+        op.grp_list += tln[(size_t)t];
+    }
+    op.grp_items0 = items0;
+    op.grp_rows0 = (int64_t)plain.size();
+    op.grp_exc = (int64_t)special.size();
+    op.grp_ident = (int64_t)ident.size();
+    op.grp_ident_items = ((int64_t)ident.size() + kGrpRows - 1) / kGrpRows;
+    if (tlr.empty()) tlr.push_back(0);
+    upload(op.gln, tln.data(), tln.size() * 4);
+    upload(op.glo, tlo.data(), tlo.size() * 4);
+    upload(op.glr, tlr.data(), tlr.size() * 4);
+    upload(op.glx, tlx.data(), tlx.size() * 2);
+    upload(op.gto, first.data(), first.size() * 4);
+    upload(op.gofs, gofs.data(), gofs.size() * 4);
+    upload(op.grow, grow.data(), grow.size() * 4);
+    upload(op.gpl, gpl.data(), gpl.size() * 2);
+    op.gw64 = Buffer((size_t)12 * D * 8);
+    op.gw32 = Buffer((size_t)12 * D * 4);
+    const hipStream_t st = op.ctx->stream;
+    hipLaunchKernelGGL(k_slot_weights<double>, dim3(nblk(D)), dim3(256), 0, st, tpw64.as<double>(), D,
+                       op.grow.as<int>(), D, op.gw64.as<double>());
+    hipLaunchKernelGGL(k_slot_weights<float>, dim3(nblk(D)), dim3(256), 0, st, tpw32.as<float>(), D,
+                       op.grow.as<int>(), D, op.gw32.as<float>());
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipStreamSynchronize(st));
+    op.ngt = (int)ngt;
+    op.nitem = (int)nitem;
+    op.nslot = D;
+    return true;
 }
 
 // LDS tiles: for every block of kBlock internal rows, the sorted distinct x
@@ -256,18 +491,10 @@ static bool lds_tiles_disabled() {
 // of their 64 columns and of the extra entry instead (tlx, kExcPos per
 // record, -1 = no extra entry).  Tiles whose list would exceed kLdsRows keep
 // reading x directly (tln = -1).
-static void build_lds_tiles(Op& op, int64_t D, int64_t ne, const Buffer& tpc, const Buffer& exc, const Buffer& ecx,
-                            const Buffer& xcol, const Buffer& flag) {
-    std::vector<uint8_t> hf((size_t)D);
-    HIPCHK(hipMemcpy(hf.data(), flag.p, hf.size(), hipMemcpyDeviceToHost));
-    std::vector<int> hc((size_t)kS * kS * D), he((size_t)D), hx((size_t)std::max<int64_t>(ne, 1) * 64),
-        hxc((size_t)std::max<int64_t>(ne, 1));
-    HIPCHK(hipMemcpy(hc.data(), tpc.p, hc.size() * 4, hipMemcpyDeviceToHost));
-    HIPCHK(hipMemcpy(he.data(), exc.p, he.size() * 4, hipMemcpyDeviceToHost));
-    if (ne > 0) {
-        HIPCHK(hipMemcpy(hx.data(), ecx.p, (size_t)ne * 64 * 4, hipMemcpyDeviceToHost));
-        HIPCHK(hipMemcpy(hxc.data(), xcol.p, (size_t)ne * 4, hipMemcpyDeviceToHost));
-    }
+static void build_lds_tiles(Op& op, const HostTp& h) {
+    const int64_t D = h.D, ne = h.ne;
+    const std::vector<uint8_t>& hf = h.flag;
+    const std::vector<int>&hc = h.tpc, &he = h.exc, &hx = h.ecx, &hxc = h.xcol;
     const int64_t ntiles = (D + kBlock - 1) / kBlock;
     std::vector<int> tln((size_t)ntiles), tlo((size_t)ntiles), tlr;
     std::vector<int16_t> tpl((size_t)kS * kS * D, 0), tlx((size_t)std::max<int64_t>(ne, 1) * kExcPos, -1);
@@ -394,7 +621,11 @@ bool op_build_tp(Op& op, Tube& t, const TpSource& src) {
     HIPCHK(hipGetLastError());
     HIPCHK(hipStreamSynchronize(st));
 
-    if (!lds_tiles_disabled()) build_lds_tiles(op, D, ne, tpc, exc, ecx, xcol, flag);
+    if (const int mode = lds_tile_mode()) {
+        const HostTp h(D, ne, tpc, exc, ecx, xcol, flag);
+        build_lds_tiles(op, h);
+        if (mode == 2) build_group_tiles(op, h, tpw64, tpw32);
+    }
     op.fmt = OP_TP;
     op.ew = kS * kS * kS;
     op.lw = 6;

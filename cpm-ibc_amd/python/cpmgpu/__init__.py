@@ -19,7 +19,7 @@ STATUS = {1: "converged", 2: "maxit", 4: "zero_rhs", 5: "breakdown"}
 
 EXPORTS = [
     "cpm_solve_opts_default", "cpm_last_error", "cpm_ctx_create", "cpm_ctx_destroy", "cpm_ctx_synchronize",
-    "cpm_ctx_device_bytes", "cpm_op_create_csr", "cpm_op_destroy", "cpm_op_rows", "cpm_op_format", "cpm_op_set_mn",
+    "cpm_ctx_device_bytes", "cpm_op_create_csr", "cpm_op_destroy", "cpm_op_rows", "cpm_op_format", "cpm_op_tile_stats", "cpm_op_gather_kind", "cpm_op_set_mn",
     "cpm_op_set_precond", "cpm_multiply", "cpm_multiply_device", "cpm_solve", "cpm_solve_device",
     "cpm_solve_batch", "cpm_solve_batch_device", "cpm_solve_stream_device", "cpm_device_alloc", "cpm_device_free", "cpm_memcpy_h2d",
     "cpm_memcpy_d2h", "cpm_tube_create", "cpm_tube_create_mesh", "cpm_tube_destroy", "cpm_tube_get_info", "cpm_tube_export",
@@ -109,6 +109,9 @@ def load(path=LIB_PATH):
     L.cpm_op_rows.restype = ctypes.c_int64
     L.cpm_op_format.argtypes = [vp]
     L.cpm_op_format.restype = ctypes.c_int
+    L.cpm_op_tile_stats.argtypes = [vp, vp]
+    L.cpm_op_gather_kind.argtypes = [vp, ctypes.c_int, ctypes.c_int]
+    L.cpm_op_gather_kind.restype = ctypes.c_int
     L.cpm_op_set_mn.argtypes = [vp, ctypes.c_double, ctypes.c_double]
     L.cpm_op_set_precond.argtypes = [vp, ctypes.c_int, ctypes.c_int]
     L.cpm_multiply.argtypes = [vp, vp, vp, ctypes.c_double, ctypes.c_double]
@@ -389,6 +392,21 @@ class Operator:
     def format(self):
         """'ell' (general sparse rows) or 'tp' (tensor-product stencils of a tube-built operator)."""
         return {0: "ell", 1: "tp"}[load().cpm_op_format(self._h)]
+
+    def tile_info(self):
+        """LDS tiles of the E gather (tensor-product operators)."""
+        v = np.zeros(6, np.int64)
+        _chk(load().cpm_op_tile_stats(self._h, _ptr(v)))
+        keys = ("shared_stencil_tiles", "items", "shared_stencil_items", "list_rows", "row_tiles", "row_list_rows")
+        return {k: int(x) for k, x in zip(keys, v)}
+
+    def gather_kernel(self, precision=CPM_FP64, nrhs=1):
+        """Name of the E-gather kernel of single-operator launches of this shape."""
+        k = load().cpm_op_gather_kind(self._h, precision, nrhs)
+        if k < 0:
+            raise CpmError(f"no E gather for precision {precision}, {nrhs} right-hand sides")
+        t = "float" if precision == CPM_FP32_REFINE else "double"
+        return f"{'k_egather_grp' if k == 2 else 'k_egather'}<{t},{nrhs}>"
 
     def set_mn(self, m, n):
         _chk(load().cpm_op_set_mn(self._h, m, n))

@@ -107,6 +107,17 @@ int64_t cpm_op_rows(const cpm_op* op);
  * counterpart in the reference, which stores E as a general sparse matrix
  * (src/Interpolation.cpp:204-225). */
 int cpm_op_format(const cpm_op* op);
+/* LDS tiles of a tensor-product operator's E gather, into out[6]:
+ * shared-stencil tiles (0: none), their items (one per thread), their items
+ * of plain rows sharing a stencil block, rows in their lists; row tiles with
+ * a list, rows in the row tiles' lists.  No counterpart in the reference. */
+int cpm_op_tile_stats(const cpm_op* op, int64_t* out);
+/* Which E gather single-operator launches of `precision` (CPM_FP64 or
+ * CPM_FP32_REFINE) with nrhs (1, 2, 4, 8) interleaved right-hand sides use:
+ * 0 direct reads of x (kernel k_egather), 1 row tiles (k_egather),
+ * 2 shared-stencil tiles (k_egather_grp); -1 bad arguments.  No counterpart
+ * in the reference. */
+int cpm_op_gather_kind(const cpm_op* op, int precision, int nrhs);
 /* Time stepping re-uses the operator with new (m, n) (problems/HeatWithIBC.cpp:166-167). */
 int cpm_op_set_mn(cpm_op* op, double m, double n);
 /* Replaces cpm::Solver::SetPrecond(precond_iters) (src/Solver.h:25). */

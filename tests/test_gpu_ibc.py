@@ -207,25 +207,39 @@ def test_plot_matrix_matches_oracle(gpu_ctx, kw):
     _operator_parity(Ag, Ao, x, what="plot rows")
 
 
-def test_lds_tiles_bitwise(gpu_ctx, monkeypatch):
-    """The LDS-tiled E gather (tile lists built with the operator) forms the same
-    products in the same order as the direct gather: FP64 apply and an FP32+refine
-    batch solve agree to the bit with CPMGPU_LDS_TILES=0, on the torus-knot system
-    (tube rows, ghost rows with moved columns, identity rows)."""
+@pytest.mark.parametrize("shapes", ["default", "all"])
+def test_lds_tiles_bitwise(gpu_ctx, monkeypatch, shapes):
+    """The LDS-tiled E gathers (row tiles, CPMGPU_LDS_TILES=rows, and
+    shared-stencil tiles, built beside them by default and used for FP32 x 8
+    or, with CPMGPU_GROUP_GATHER=all, for every value shape) form the same
+    products in the same order as the direct gather (CPMGPU_LDS_TILES=0): an
+    FP64 apply and FP32+refine batch solves of 8 and 3 right-hand sides agree
+    to the bit, on the torus-knot system (tube rows, ghost rows with moved
+    columns, identity rows)."""
+    if shapes == "all":
+        monkeypatch.setenv("CPMGPU_GROUP_GATHER", "all")
     print("case lds tiles bitwise", flush=True)
     dx = 0.02
     tube = cpmgpu.Tube(gpu_ctx, "Torus", dx, params=[1.0, 0.4], bbox=(-1.4, 1.4))
     prob = cpmgpu.Problem(tube, [cpmgpu.IBC("Polyline", verts=torus_knot_polyline(), boundary_order=2,
                                             on_ibc_tolerance=0.1 * dx * dx)])
     ops = {}
-    for mode in ("1", "0"):
+    for mode in ("1", "rows", "0"):
         monkeypatch.setenv("CPMGPU_LDS_TILES", mode)
         ops[mode] = prob.operator(0.0, -1.0)
+    info = {m: ops[m].tile_info() for m in ops}
+    print(f"tiles: {info}", flush=True)
+    assert info["1"]["shared_stencil_tiles"] > 0 and info["rows"]["shared_stencil_tiles"] == 0
+    assert info["1"]["row_tiles"] == info["rows"]["row_tiles"] > 0
     x = np.random.default_rng(3).standard_normal(prob.D)
-    assert np.array_equal(ops["1"].multiply(x, 0.0, -1.0), ops["0"].multiply(x, 0.0, -1.0))
-    b = [np.random.default_rng(k).standard_normal(prob.D) for k in range(3)]
+    ref = ops["0"].multiply(x, 0.0, -1.0)
+    for mode in ("1", "rows"):
+        assert np.array_equal(ops[mode].multiply(x, 0.0, -1.0), ref), mode
+    # 8 right-hand sides: FP32 x 8 gathers (32-byte rows in two LDS planes)
+    b = [np.random.default_rng(k).standard_normal(prob.D) for k in range(8)]
     o = cpmgpu.opts(tol=1e-8, precision=cpmgpu.CPM_FP32_REFINE)
-    xa, ia = gpu_ctx.solve_batch([ops["1"]] * 3, b, o)
-    xb, ib = gpu_ctx.solve_batch([ops["0"]] * 3, b, o)
-    assert [i["iters"] for i in ia] == [i["iters"] for i in ib]
-    assert all(np.array_equal(p, q) for p, q in zip(xa, xb))
+    for nb in (8, 3):
+        xs = {m: gpu_ctx.solve_batch([ops[m]] * nb, b[:nb], o) for m in ops}
+        for mode in ("1", "rows"):
+            assert [i["iters"] for i in xs[mode][1]] == [i["iters"] for i in xs["0"][1]], (mode, nb)
+            assert all(np.array_equal(p, q) for p, q in zip(xs[mode][0], xs["0"][0])), (mode, nb)

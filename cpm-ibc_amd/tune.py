@@ -1,8 +1,9 @@
 """Autotuner of libcpmgpu's kernel shape parameters on one MI355X.
 
 Parameters (csrc/tuned.h): CPMGPU_XCD_RUN, the run of consecutive row blocks
-each XCD takes, and CPMGPU_LDS_ROWS, the longest LDS tile list of the E
-gather.  Three steps:
+each XCD takes; CPMGPU_LDS_ROWS, the longest list of a row tile of the E
+gather; CPMGPU_GROUP_ROWS, the rows of a shared-stencil item; and
+CPMGPU_GROUP_LDS_ROWS, the longest list of a shared-stencil tile.  Three steps:
 
     python cpm-ibc_amd/tune.py build            # one library per variant under cpm-ibc_amd/_tune/
     python cpm-ibc_amd/tune.py measure out.json # on the GPU: time every built variant
@@ -11,9 +12,10 @@ gather.  Three steps:
 Each variant is timed in a child process on the torus-knot system (Laplace
 on the torus R=1, r=0.4, h=0.02 with the 2000-segment knot IBC): the E gather
 in FP32 x 8 (50 launches, HIP events) and 32 right-hand sides streamed
-through 8 FP32+refine lanes (best of 3).  The stream time decides; the
-default (64-block runs, 2,560 rows) stays unless a variant beats it by more
-than 0.5% (run-to-run noise; at equal time shorter runs fetched less from HBM).
+through 8 FP32+refine lanes (best of 3).  Variants change one parameter of
+the default at a time.  The stream time decides; the default stays unless a
+variant beats it by more than 0.5% (run-to-run noise; at equal time shorter
+XCD runs fetched less from HBM).
 """
 import json
 import os
@@ -25,11 +27,16 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 REPO = os.path.dirname(HERE)
 TUNE = os.path.join(HERE, "_tune")
 
-VARIANTS = [dict(xcd_run=x, lds_rows=2560) for x in (16, 32, 64, 128, 256)] + [dict(xcd_run=64, lds_rows=3072)]
+DEFAULT = dict(xcd_run=64, lds_rows=2560, grp_rows=2, grp_lds=1280)
+TIE = 0.005
+VARIANTS = [DEFAULT] + [dict(DEFAULT, **{k: v}) for k, vs in (
+    ("xcd_run", (32, 128)), ("lds_rows", (3072,)), ("grp_rows", (1, 3)), ("grp_lds", (1024, 1536))) for v in vs]
+DEFINES = dict(xcd_run="CPMGPU_XCD_RUN", lds_rows="CPMGPU_LDS_ROWS", grp_rows="CPMGPU_GROUP_ROWS",
+               grp_lds="CPMGPU_GROUP_LDS_ROWS")
 
 
 def name(v):
-    return f"x{v['xcd_run']}_l{v['lds_rows']}"
+    return f"x{v['xcd_run']}_l{v['lds_rows']}_g{v['grp_rows']}_{v['grp_lds']}"
 
 
 def build():
@@ -37,7 +44,7 @@ def build():
     for v in VARIANTS:
         lib = os.path.join(TUNE, name(v), "libcpmgpu.so")
         obj = os.path.join("/tmp", "cpmgpu_tune", name(v))
-        extra = f"-DCPMGPU_XCD_RUN={v['xcd_run']} -DCPMGPU_LDS_ROWS={v['lds_rows']}"
+        extra = " ".join(f"-D{DEFINES[k]}={v[k]}" for k in DEFINES)
         print(f"building {name(v)}", flush=True)
         subprocess.run(["make", "-s", "-j", jobs, "-C", HERE, f"BUILD={obj}", f"LIB={lib}", f"EXTRA={extra}"],
                        check=True)
@@ -88,38 +95,33 @@ def measure(out):
         json.dump(res, f, indent=1)
 
 
-DEFAULT = dict(xcd_run=64, lds_rows=2560)   # kept unless a variant is more than 0.5% faster
-TIE = 0.005
-
-
 def write(src):
     res = json.load(open(src))
     best = min(res, key=lambda r: r["stream_ms"])
-    base = [r for r in res if r["xcd_run"] == DEFAULT["xcd_run"] and r["lds_rows"] == DEFAULT["lds_rows"]]
+    base = [r for r in res if all(r[k] == DEFAULT[k] for k in DEFINES)]
     if base and base[0]["stream_ms"] <= best["stream_ms"] * (1 + TIE):
         best = base[0]
-    table = "\n".join(f"//   {r['xcd_run']:>8} {r['lds_rows']:>9} {r['egather_us']:>11.1f} {r['stream_ms']:>12.1f}"
-                      + ("   <- chosen" if r is best else "") for r in res)
+    table = "\n".join(f"//   {r['xcd_run']:>8} {r['lds_rows']:>9} {r['grp_rows']:>9} {r['grp_lds']:>8} "
+                      f"{r['egather_us']:>11.1f} {r['stream_ms']:>12.1f}" + ("   <- chosen" if r is best else "")
+                      for r in res)
+    defs = "\n".join(f"#ifndef {DEFINES[k]}\n#define {DEFINES[k]} {best[k]}\n#endif" for k in DEFINES)
     text = f"""// Kernel shape parameters of libcpmgpu.so, chosen by cpm-ibc_amd/tune.py on one
 // MI355X (gfx950); a build can override them with -D.
 //
 //   CPMGPU_XCD_RUN   consecutive row blocks each XCD takes (workgroups are
 //                    dispatched to the 8 XCDs round-robin; runs keep a tile's
 //                    gathered x rows in that XCD's L2)
-//   CPMGPU_LDS_ROWS  longest LDS tile list of the E gather (FP32 x 8 rows of
-//                    32 B: the LDS of one workgroup is 32 B x this)
+//   CPMGPU_LDS_ROWS  longest list of a row tile of the E gather (FP32 x 8
+//                    rows of 32 B: the LDS of one workgroup is 32 B x this)
+//   CPMGPU_GROUP_ROWS      rows of one shared-stencil item (one thread)
+//   CPMGPU_GROUP_LDS_ROWS  longest list of a shared-stencil tile
 //
 // Torus-knot system (405,928 rows): E gather FP32 x 8 per launch and 32
 // right-hand sides streamed through 8 FP32+refine lanes:
-//   xcd_run  lds_rows  gather (us)  stream (ms)
+//   xcd_run  lds_rows  grp_rows  grp_lds  gather (us)  stream (ms)
 {table}
 #pragma once
-#ifndef CPMGPU_XCD_RUN
-#define CPMGPU_XCD_RUN {best['xcd_run']}
-#endif
-#ifndef CPMGPU_LDS_ROWS
-#define CPMGPU_LDS_ROWS {best['lds_rows']}
-#endif
+{defs}
 """
     with open(os.path.join(HERE, "csrc", "tuned.h"), "w") as f:
         f.write(text)

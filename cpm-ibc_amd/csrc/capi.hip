@@ -289,6 +289,11 @@ static void run_batch(Ctx* ctx, const std::vector<Group>& groups, const std::vec
     const hipStream_t st = ctx->stream;
     const int G = (int)groups.size();
     const bool refine = o.precision == CPM_FP32_REFINE;
+    {
+        std::vector<Op*> ops;
+        for (auto& g : groups) ops.push_back(g.op);
+        ensure_group_tiles_for(ops, refine ? 4 : 8, NR);   // before the views are taken
+    }
     // layout
     std::vector<int64_t> off(G);
     std::vector<int> b0(G), nb(G);
@@ -461,6 +466,7 @@ static void run_stream(Ctx* ctx, Op* op, const std::vector<const double*>& rhs, 
     const int n = (int)rhs.size();
     const int64_t D = op->D;
     const int nb = (int)((D + kBlock - 1) / kBlock);
+    ensure_group_tiles_for({op}, o.precision == CPM_FP32_REFINE ? 4 : 8, NR);
     const int64_t total = (D * NR + 63) & ~int64_t(63);
     auto r256 = [](size_t k) { return (k + 255) & ~size_t(255); };
     const size_t part_stride = 2 * NR;
@@ -699,6 +705,7 @@ void multiply(Op* op, const double* dx, double* dy, double m, double n) {
 
 template <typename T, int NR>
 static void profile_egather_t(Op* op, int reps, double* avg_ms, double* bytes) {
+    ensure_group_tiles_for({op}, sizeof(T), NR);
     Ctx* ctx = op->ctx;
     const hipStream_t st = ctx->stream;
     const int nblocks = (int)((op->D + kBlock - 1) / kBlock);
@@ -1188,6 +1195,11 @@ struct Server : ServerBase {
 
     void rebuild() {
         ensure_tables();
+        {
+            std::vector<Op*> ops;
+            for (const Req& q : reqs) ops.push_back(q.op);
+            ensure_group_tiles_for(ops, 4, NR);
+        }
         std::vector<Seg<float>> h(reqs.size());
         std::vector<int> map;
         int b0 = 0;
@@ -1380,8 +1392,13 @@ int cpm_op_format(const cpm_op* op) { return op ? O(op)->fmt : -1; }
 
 int cpm_op_gather_kind(const cpm_op* op, int precision, int nrhs) {
     if (!op || (nrhs != 1 && nrhs != 2 && nrhs != 4 && nrhs != 8)) return -1;
-    const Op* o = O(op);
+    Op* o = const_cast<Op*>(O(op));   // the lazy tile build does not change what the operator computes
     const bool f32 = precision == CPM_FP32_REFINE;
+    try {
+        ensure_group_tiles_for({o}, f32 ? 4 : 8, nrhs);
+    } catch (...) {
+        return -1;
+    }
     const bool fits = f32 ? (nrhs == 1 ? GrpTile<float, 1>::kRows : nrhs == 2 ? GrpTile<float, 2>::kRows
                              : nrhs == 4 ? GrpTile<float, 4>::kRows : GrpTile<float, 8>::kRows) > 0
                           : (nrhs == 1 ? GrpTile<double, 1>::kRows : nrhs == 2 ? GrpTile<double, 2>::kRows
@@ -1395,7 +1412,8 @@ int cpm_op_gather_kind(const cpm_op* op, int precision, int nrhs) {
 int cpm_op_tile_stats(const cpm_op* op, int64_t* out) {
     return guard([&] {
         if (!op || !out) throw ArgError("null operator or output");
-        const Op* o = O(op);
+        Op* o = const_cast<Op*>(O(op));   // the lazy tile build does not change what the operator computes
+        op_ensure_group_tiles(*o);
         int64_t row_tiles = 0;
         if (o->tln.p) {
             std::vector<int> tln(o->tln.bytes / 4);

@@ -127,6 +127,7 @@ struct GrpTile {
     static constexpr int kRows = (kGrpLdsRows * NR * (int)sizeof(T) <= kGrpLdsBytes) ? kGrpLdsRows : 0;
 };
 constexpr int kGrpKindShift = 28;             // item kind above the slot offset in gofs
+constexpr int kGrpMinTiles = 512;             // fewer shared-stencil tiles: row tiles only
 constexpr int kGrpOfsMask = (1 << kGrpKindShift) - 1;
 
 // One instance group of a batched launch: an operator, NR right-hand sides

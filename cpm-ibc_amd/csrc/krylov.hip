@@ -732,7 +732,8 @@ __global__ __launch_bounds__(kBlock) void k_egather(const Seg<T>* __restrict__ s
 // E x for launches whose every segment has shared-stencil tiles: block lb of
 // a segment takes its tile lb (blocks past the last tile only do the FP64
 // gathers of refinement lanes).  40 KB of LDS and at most 128 VGPRs, so four
-// workgroups share a CU.
+// workgroups share a CU (a direct-read path for other segments, even out of
+// line, made the FP32 x 8 kernel spill).
 template <typename T, int NR>
 __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))) void k_egather_grp(
     const Seg<T>* __restrict__ segs, const int* __restrict__ bseg, const T* x, T* out, unsigned lanes, MixVec mv) {

@@ -50,6 +50,7 @@ struct Op {
     Buffer gto, gofs, grow, gpl, gw64, gw32, gln, glo, glr, glx;   // shared-stencil tiles (OP_TP)
     int ngt = 0, nitem = 0;
     int64_t nslot = 0;
+    bool grp_mode = false, grp_tried = false;   // shared-stencil tiles wanted / build attempted
     // gathered (non-identity) rows: [0] plain and [1] exception rows of LDS
     // tiles, [2] plain and [3] exception rows read directly; rows in all lists;
     // kind-0 items of shared-stencil tiles
@@ -72,6 +73,13 @@ struct Op {
     template <typename T> OpView<T> view() const;
     size_t device_bytes() const;
 };
+
+// Builds a tensor-product operator's shared-stencil tiles on the first call
+// (when its build asked for them); later calls return at once.
+void op_ensure_group_tiles(Op& op);
+// The same for every operator of a launch of T x nr values, if that shape
+// uses shared-stencil tiles.
+void ensure_group_tiles_for(const std::vector<Op*>& ops, int tsize, int nr);
 
 // Batched BiCGStab over instances (ops[i], rhs[i] -> xs[i], caller-order
 // device vectors); mn, when given, overrides each instance's (m, n).

@@ -114,6 +114,9 @@ constexpr int kExcPos = 72;      // positions per exception record (64 slots, ex
 #ifndef CPMGPU_GROUP_ROWS
 #define CPMGPU_GROUP_ROWS 2
 #endif
+#ifndef CPMGPU_GROUP_WAVES_WIDE
+#define CPMGPU_GROUP_WAVES_WIDE 2
+#endif
 #ifndef CPMGPU_GROUP_LDS_ROWS
 #define CPMGPU_GROUP_LDS_ROWS 1280
 #endif

@@ -734,8 +734,8 @@ __global__ __launch_bounds__(kBlock) void k_egather(const Seg<T>* __restrict__ s
 // gathers of refinement lanes).  40 KB of LDS and at most 128 VGPRs, so four
 // workgroups share a CU (a direct-read path for other segments, even out of
 // line, made the FP32 x 8 kernel spill).
-template <typename T, int NR>
-__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))) void k_egather_grp(
+template <typename T, int NR, int WPE>
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(WPE))) void k_egather_grp(
     const Seg<T>* __restrict__ segs, const int* __restrict__ bseg, const T* x, T* out, unsigned lanes, MixVec mv) {
     typedef typename VT<T, NR>::type V;
     constexpr int LR = GrpTile<T, NR>::kRows;
@@ -1232,7 +1232,7 @@ void launch_reset_states(KState* st, int n, double tol2, int max_it, double delt
     do {                                                                     \
         if constexpr (GrpTile<T, NR>::kRows > 0) {                           \
             if (c.grp && grp_gather(sizeof(T), NR)) {                        \
-                LAUNCH((k_egather_grp<T, NR>), __VA_ARGS__);                 \
+                LAUNCH((k_egather_grp<T, NR, GrpWaves<T, NR>::kWaves>), __VA_ARGS__); \
                 break;                                                       \
             }                                                                \
         }                                                                    \
@@ -1240,16 +1240,22 @@ void launch_reset_states(KState* st, int n, double tol2, int max_it, double delt
     } while (0)
 #define FIN(T, NR, KIND) hipLaunchKernelGGL((k_fin<T, NR, KIND>), dim3(c.nseg * NR), dim3(kBlock), 0, c.stream, c.segs, c.nseg)
 
+// Waves per SIMD the shared-stencil gather is compiled for: four (128 VGPRs)
+// where that does not spill, two (256 VGPRs) for FP32 x 4 and the FP64 shapes.
+template <typename T, int NR>
+struct GrpWaves {
+    static constexpr int kWaves = (sizeof(T) == 4 && NR != 4) ? 4 : CPMGPU_GROUP_WAVES_WIDE;
+};
+
 // Value shapes whose E gathers use shared-stencil tiles when every operator
-// of the launch has them: FP32 x 8, x 2 and x 1 by default, the shapes where
-// they were measured faster than row tiles on the torus-knot system (FP32 x 4
-// and the FP64 shapes need more than 128 VGPRs with them).
-// CPMGPU_GROUP_GATHER=all uses them for every shape whose tile fits the LDS,
-// =none for none.
+// of the launch has them: by default FP32 x 8, x 2, x 1 and FP64 x 4, the
+// shapes where they were measured faster than row tiles on the torus-knot
+// system.  CPMGPU_GROUP_GATHER=all uses them for every shape whose tile fits
+// the LDS, =none for none.
 bool grp_gather(int tsize, int nr) {
     const char* e = std::getenv("CPMGPU_GROUP_GATHER");
     const std::string v = e ? e : "";
-    return v == "all" || (v != "none" && tsize == 4 && nr != 4);
+    return v == "all" || (v != "none" && ((tsize == 4 && nr != 4) || (tsize == 8 && nr == 4)));
 }
 
 template <typename T, int NR>

@@ -213,9 +213,9 @@ def test_lds_tiles_bitwise(gpu_ctx, monkeypatch, shapes):
     shared-stencil tiles, built beside them by default and used for FP32 x 8
     or, with CPMGPU_GROUP_GATHER=all, for every value shape) form the same
     products in the same order as the direct gather (CPMGPU_LDS_TILES=0): an
-    FP64 apply and FP32+refine batch solves of 8 and 3 right-hand sides agree
-    to the bit, on the torus-knot system (tube rows, ghost rows with moved
-    columns, identity rows)."""
+    FP64 apply, FP32+refine batch solves of 8 and 3 right-hand sides and an
+    FP64 batch solve of 3 agree to the bit, on the torus-knot system (tube
+    rows, ghost rows with moved columns, identity rows)."""
     if shapes == "all":
         monkeypatch.setenv("CPMGPU_GROUP_GATHER", "all")
     print("case lds tiles bitwise", flush=True)
@@ -238,8 +238,12 @@ def test_lds_tiles_bitwise(gpu_ctx, monkeypatch, shapes):
     # 8 right-hand sides: FP32 x 8 gathers (32-byte rows in two LDS planes)
     b = [np.random.default_rng(k).standard_normal(prob.D) for k in range(8)]
     o = cpmgpu.opts(tol=1e-8, precision=cpmgpu.CPM_FP32_REFINE)
-    for nb in (8, 3):
-        xs = {m: gpu_ctx.solve_batch([ops[m]] * nb, b[:nb], o) for m in ops}
+    o64 = cpmgpu.opts(tol=1e-8, precision=cpmgpu.CPM_FP64)
+    for nb, oo in ((8, o), (3, o), (3, o64)):   # FP32 x 8, FP32 x 4, FP64 x 4
+        xs = {m: gpu_ctx.solve_batch([ops[m]] * nb, b[:nb], oo) for m in ops}
         for mode in ("1", "rows"):
             assert [i["iters"] for i in xs[mode][1]] == [i["iters"] for i in xs["0"][1]], (mode, nb)
             assert all(np.array_equal(p, q) for p, q in zip(xs[mode][0], xs["0"][0])), (mode, nb)
+    assert ops["1"].gather_kernel(cpmgpu.CPM_FP64, 4) == "k_egather_grp<double,4>"
+    assert ops["1"].gather_kernel(cpmgpu.CPM_FP32_REFINE, 8) == "k_egather_grp<float,8>"
+    assert ops["rows"].gather_kernel(cpmgpu.CPM_FP32_REFINE, 8) == "k_egather<float,8>"

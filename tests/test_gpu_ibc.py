@@ -247,3 +247,28 @@ def test_lds_tiles_bitwise(gpu_ctx, monkeypatch, shapes):
     assert ops["1"].gather_kernel(cpmgpu.CPM_FP64, 4) == "k_egather_grp<double,4>"
     assert ops["1"].gather_kernel(cpmgpu.CPM_FP32_REFINE, 8) == "k_egather_grp<float,8>"
     assert ops["rows"].gather_kernel(cpmgpu.CPM_FP32_REFINE, 8) == "k_egather<float,8>"
+
+
+def test_shared_stencil_tiles_mesh_bitwise(gpu_ctx, monkeypatch):
+    """Shared-stencil tiles on a second geometry, a displaced icosphere mesh tube
+    (h = 0.02, plain CPM, (1 - Laplace_S) u = f): 30 FP32+refine iterations of 8
+    right-hand sides, and 30 FP64 iterations of 3, end bit for bit where the
+    row tiles and the direct gather end."""
+    from cpmgpu.workloads import displaced_icosphere
+    print("case shared-stencil tiles mesh bitwise", flush=True)
+    V, F = displaced_icosphere(5, seed=4)
+    tube = cpmgpu.Tube(gpu_ctx, "Triangulation", 0.02, verts=V, faces=F)
+    ops = {}
+    for mode in ("1", "rows", "0"):
+        monkeypatch.setenv("CPMGPU_LDS_TILES", mode)
+        ops[mode] = tube.operator(1.0, -1.0)
+    info = ops["1"].tile_info()
+    print(f"rows {ops['1'].D}: {info}", flush=True)
+    assert info["shared_stencil_tiles"] >= 512
+    b = [np.random.default_rng(10 + k).standard_normal(ops["1"].D) for k in range(8)]
+    for nb, prec in ((8, cpmgpu.CPM_FP32_REFINE), (3, cpmgpu.CPM_FP64)):
+        o = cpmgpu.opts(tol=1e-14, max_iters=30, precision=prec)
+        xs = {m: gpu_ctx.solve_batch([ops[m]] * nb, b[:nb], o) for m in ops}
+        for mode in ("1", "rows"):
+            assert [i["iters"] for i in xs[mode][1]] == [i["iters"] for i in xs["0"][1]], (mode, nb)
+            assert all(np.array_equal(p, q) for p, q in zip(xs[mode][0], xs["0"][0])), (mode, nb)

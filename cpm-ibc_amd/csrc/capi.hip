@@ -2,6 +2,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <climits>
 #include <cmath>
 #include <cstdlib>
 #include <cstring>
@@ -388,8 +389,8 @@ static void run_batch(Ctx* ctx, const std::vector<Group>& groups, const std::vec
     int sweeps = 0;
     for (auto& g : groups)
         if (g.op->precond == CPM_PRECOND_JACOBI) sweeps = std::max(sweeps, g.op->precond_iters);
-    int grp = 1;
-    for (auto& g : groups) grp &= g.op->ngt > 0;
+    int grp = groups.empty() ? 0 : INT_MAX;
+    for (auto& g : groups) grp = std::min(grp, g.op->ngt);
     const LaunchCfg<double> c64{segs64, bseg, blocks, G, st, sweeps, grp};
     const LaunchCfg<float> c32{segs32, bseg, blocks, G, st, sweeps, grp};
     KState* hs = ctx->host_states;
@@ -542,7 +543,7 @@ static void run_stream(Ctx* ctx, Op* op, const std::vector<const double*>& rhs, 
     }
     HIPCHK(hipMemcpyAsync(sts, hst.data(), sizeof(KState) * kMaxNR, hipMemcpyHostToDevice, st));
     int sweeps = op->precond == CPM_PRECOND_JACOBI ? op->precond_iters : 0;
-    const LaunchCfg<float> c32{seg32, bseg, nb, 1, st, sweeps, op->ngt > 0};
+    const LaunchCfg<float> c32{seg32, bseg, nb, 1, st, sweeps, op->ngt};
     const MixVec mv{V64.x, V64.r, V64.b, V64.tmp};
     KState* hs = ctx->host_states;
     int done = 0;
@@ -692,7 +693,7 @@ void multiply(Op* op, const double* dx, double* dy, double m, double n) {
     HIPCHK(hipMemcpyAsync(dbseg, hb.data(), sizeof(int) * nblocks, hipMemcpyHostToDevice, st));
     double* w = ctx->workspace("mul_tmp", sizeof(double) * op->D * 3);
     double *xi = w, *yi = w + op->D, *tmp = w + 2 * op->D;
-    LaunchCfg<double> c{dseg, dbseg, nblocks, 1, st, 0, op->ngt > 0};
+    LaunchCfg<double> c{dseg, dbseg, nblocks, 1, st, 0, op->ngt};
     if (op->perm.p) {   // caller order -> internal order and back
         hipLaunchKernelGGL(k_pack, dim3(nblocks), dim3(256), 0, st, dpe, dbseg, xi, (float*)nullptr, 1);
         Launch<double, 1>::apply(c, xi, tmp, yi);
@@ -720,7 +721,7 @@ static void profile_egather_t(Op* op, int reps, double* avg_ms, double* bytes) {
     T* x = reinterpret_cast<T*>(ctx->workspace("prof_x", sizeof(T) * op->D * NR));
     T* y = reinterpret_cast<T*>(ctx->workspace("prof_y", sizeof(T) * op->D * NR));
     HIPCHK(hipMemsetAsync(x, 0, sizeof(T) * op->D * NR, st));
-    LaunchCfg<T> c{dseg, dbseg, nblocks, 1, st, 0, op->ngt > 0};
+    LaunchCfg<T> c{dseg, dbseg, nblocks, 1, st, 0, op->ngt};
     launch_egather<T, NR>(c, x, y);   // warm-up
     hipEvent_t e0, e1;
     HIPCHK(hipEventCreate(&e0));
@@ -745,7 +746,7 @@ static void profile_egather_t(Op* op, int reps, double* avg_ms, double* bytes) {
     // extra entry; LDS tiles also read their lists, 4 bytes per row)
     double opb;
     const double sT = sizeof(T);
-    if (op->fmt == OP_TP && op->ngt > 0 && grp_gather(sizeof(T), NR) && GrpTile<T, NR>::kRows > 0) {
+    if (op->fmt == OP_TP && op->ngt > 0 && grp_gather(sizeof(T), NR, op->ngt) && GrpTile<T, NR>::kRows > 0) {
         // shared-stencil tiles: per plain item 16 int16 positions and its
         // slot offset, per plain row 12 weights and its row index; exception
         // rows as in the row tiles plus slot offset and row index; identity
@@ -1225,8 +1226,8 @@ struct Server : ServerBase {
         const hipStream_t st = ctx->stream;
         const Vecs<float> V32{nullptr, v32(0), v32(1), v32(2), v32(3), v32(4), v32(5), v32(6), v32(7), v32(8), v32(9)};
         const MixVec mv{v64(1), v64(2), v64(0), v64(4)};
-        int grp = 1;
-        for (const Req& q : reqs) grp &= q.op->ngt > 0;
+        int grp = INT_MAX;
+        for (const Req& q : reqs) grp = std::min(grp, q.op->ngt);
         const LaunchCfg<float> c{d_segs, d_bseg, nblocks, (int)reqs.size(), st, 0, grp};
         Vecs<float> w = V32;
         for (int i = 0; i < iters; ++i) Launch<float, NR>::iteration(c, w, true, mv);
@@ -1403,7 +1404,7 @@ int cpm_op_gather_kind(const cpm_op* op, int precision, int nrhs) {
                              : nrhs == 4 ? GrpTile<float, 4>::kRows : GrpTile<float, 8>::kRows) > 0
                           : (nrhs == 1 ? GrpTile<double, 1>::kRows : nrhs == 2 ? GrpTile<double, 2>::kRows
                              : nrhs == 4 ? GrpTile<double, 4>::kRows : GrpTile<double, 8>::kRows) > 0;
-    if (o->fmt == OP_TP && o->ngt > 0 && fits && grp_gather(f32 ? 4 : 8, nrhs)) return 2;
+    if (o->fmt == OP_TP && o->ngt > 0 && fits && grp_gather(f32 ? 4 : 8, nrhs, o->ngt)) return 2;
     const int bytes = (f32 ? 4 : 8) * nrhs;
     if (o->fmt == OP_TP && o->tln.p && kLdsRows * bytes <= kLdsBytes) return 1;
     return 0;

@@ -130,7 +130,10 @@ struct GrpTile {
     static constexpr int kRows = (kGrpLdsRows * NR * (int)sizeof(T) <= kGrpLdsBytes) ? kGrpLdsRows : 0;
 };
 constexpr int kGrpKindShift = 28;             // item kind above the slot offset in gofs
-constexpr int kGrpMinTiles = 512;             // fewer shared-stencil tiles: row tiles only
+#ifndef CPMGPU_GROUP_MIN_TILES
+#define CPMGPU_GROUP_MIN_TILES 256
+#endif
+constexpr int kGrpMinTiles = CPMGPU_GROUP_MIN_TILES;   // fewer shared-stencil tiles: row tiles only
 constexpr int kGrpOfsMask = (1 << kGrpKindShift) - 1;
 
 // One instance group of a batched launch: an operator, NR right-hand sides

@@ -12,11 +12,12 @@ struct LaunchCfg {
     int nseg;
     hipStream_t stream;
     int jacobi_sweeps;     // largest precond_iters over Jacobi-preconditioned segments (0: none)
-    int grp = 0;           // every segment's operator has shared-stencil tiles (k_egather_grp)
+    int grp = 0;           // fewest shared-stencil tiles of the segments' operators (0: some have none)
 };
 
-// Whether E gathers of T x nr values use shared-stencil tiles where built.
-bool grp_gather(int tsize, int nr);
+// Whether E gathers of T x nr values use shared-stencil tiles when every
+// operator of the launch has at least ntiles of them.
+bool grp_gather(int tsize, int nr, int ntiles);
 
 // Workspace vectors of one precision; every vector spans all segments
 // (segment g occupies [off_g, off_g + D_g * NR)).

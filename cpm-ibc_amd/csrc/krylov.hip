@@ -1231,7 +1231,7 @@ void launch_reset_states(KState* st, int n, double tol2, int max_it, double delt
 #define EGATHER(...)                                                         \
     do {                                                                     \
         if constexpr (GrpTile<T, NR>::kRows > 0) {                           \
-            if (c.grp && grp_gather(sizeof(T), NR)) {                        \
+            if (c.grp > 0 && grp_gather(sizeof(T), NR, c.grp)) {             \
                 LAUNCH((k_egather_grp<T, NR, GrpWaves<T, NR>::kWaves>), __VA_ARGS__); \
                 break;                                                       \
             }                                                                \
@@ -1248,14 +1248,18 @@ struct GrpWaves {
 };
 
 // Value shapes whose E gathers use shared-stencil tiles when every operator
-// of the launch has them: by default FP32 x 8, x 2, x 1 and FP64 x 4, the
-// shapes where they were measured faster than row tiles on the torus-knot
-// system.  CPMGPU_GROUP_GATHER=all uses them for every shape whose tile fits
-// the LDS, =none for none.
-bool grp_gather(int tsize, int nr) {
+// of the launch has at least ntiles of them: by default FP32 x 8, x 2 and
+// FP64 x 4 from kGrpMinTiles tiles, FP32 x 1 from 768, where they were
+// measured faster than row tiles (tubes of 42 k to 406 k rows).
+// CPMGPU_GROUP_GATHER=all uses them for every shape whose tile fits the LDS,
+// =none for none.
+bool grp_gather(int tsize, int nr, int ntiles) {
     const char* e = std::getenv("CPMGPU_GROUP_GATHER");
     const std::string v = e ? e : "";
-    return v == "all" || (v != "none" && ((tsize == 4 && nr != 4) || (tsize == 8 && nr == 4)));
+    if (v == "all" || v == "none") return v == "all";
+    if (!((tsize == 4 && nr != 4) || (tsize == 8 && nr == 4))) return false;
+    // one right-hand side: row tiles were faster up to ~600 tiles
+    return ntiles >= (nr == 1 ? 768 : kGrpMinTiles);
 }
 
 template <typename T, int NR>

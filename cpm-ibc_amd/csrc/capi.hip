@@ -17,6 +17,7 @@
 #include "engine.h"
 #include "krylov.h"
 #include "ops.h"
+#include "pool.h"
 
 namespace cpmgpu {
 
@@ -284,13 +285,43 @@ static void fill_seg(Seg<T>& s, const Op* op, int64_t off, int block0, int nbloc
     s.part = part;
 }
 
+// Workspace bytes run_batch<NR> asks the context for, over G groups of
+// `blocks` row blocks and `total` vector elements in all (each vector).
+struct BatchBytes {
+    size_t tables, vec64, vec32, pack;
+};
+static BatchBytes batch_bytes(int G, int blocks, int64_t total, int NR, bool refine) {
+    auto r256 = [](size_t n) { return (n + 255) & ~size_t(255); };
+    BatchBytes b;
+    b.tables = r256(sizeof(Seg<double>) * G) + r256(sizeof(Seg<float>) * G) + r256(sizeof(int) * blocks) +
+               r256(sizeof(KState) * (size_t)G * kMaxNR) + r256(sizeof(double) * 2 * NR * (size_t)blocks) + 256;
+    b.vec64 = sizeof(double) * (size_t)total * (refine ? 5 : 11) + 1024;
+    b.vec32 = refine ? sizeof(float) * (size_t)total * 10 + 1024 : 0;
+    b.pack = sizeof(PackEntry) * (size_t)G * NR + sizeof(int) * (size_t)blocks * NR + 512;
+    return b;
+}
+// One operator of D rows with NR right-hand sides.
+static BatchBytes batch_bytes1(int64_t D, int NR, bool refine) {
+    return batch_bytes(1, (int)((D + kBlock - 1) / kBlock), (D * NR + 63) & ~int64_t(63), NR, refine);
+}
+
+// Grow the context's workspaces to what a batch of one operator of D rows
+// with NR right-hand sides needs, so that such a batch allocates nothing.
+static void reserve_batch_workspaces(Ctx* ctx, int64_t D, int NR, bool refine) {
+    const BatchBytes b = batch_bytes1(D, NR, refine);
+    ctx->workspace("tables", b.tables);
+    ctx->workspace("vec64", b.vec64);
+    if (refine) ctx->workspace("vec32", b.vec32);
+    ctx->workspace("pack", b.pack);
+}
+
 template <int NR>
 static void run_batch(Ctx* ctx, const std::vector<Group>& groups, const std::vector<const double*>& rhs,
                       const std::vector<double*>& xs, const cpm_solve_opts& o, std::vector<cpm_solve_info>& infos) {
     const hipStream_t st = ctx->stream;
     const int G = (int)groups.size();
     const bool refine = o.precision == CPM_FP32_REFINE;
-    {
+    if (!ctx->no_builds) {
         std::vector<Op*> ops;
         for (auto& g : groups) ops.push_back(g.op);
         ensure_group_tiles_for(ops, refine ? 4 : 8, NR);   // before the views are taken
@@ -314,8 +345,8 @@ static void run_batch(Ctx* ctx, const std::vector<Group>& groups, const std::vec
     if (nstate > Ctx::kHostStates) throw ArgError("batch too large");
     const size_t part_stride = 2 * NR;
     auto r256 = [](size_t n) { return (n + 255) & ~size_t(255); };
-    const size_t bytes_tables = r256(sizeof(Seg<double>) * G) + r256(sizeof(Seg<float>) * G) + r256(sizeof(int) * blocks) +
-                                r256(sizeof(KState) * nstate) + r256(sizeof(double) * part_stride * blocks) + 256;
+    const BatchBytes bb = batch_bytes(G, blocks, total, NR, refine);
+    const size_t bytes_tables = bb.tables;
     char* tb = reinterpret_cast<char*>(ctx->workspace("tables", bytes_tables));
     char* const tb_end = tb + bytes_tables;
     auto carve = [&](size_t n) {
@@ -353,11 +384,11 @@ static void run_batch(Ctx* ctx, const std::vector<Group>& groups, const std::vec
         std::vector<double**> f64;
         if (refine) f64 = {&V64.b, &V64.x, &V64.r, &V64.t, &V64.tmp};
         else f64 = {&V64.b, &V64.x, &V64.r, &V64.r0, &V64.p, &V64.v, &V64.s, &V64.t, &V64.y, &V64.z, &V64.tmp};
-        double* q = ctx->workspace("vec64", sizeof(double) * total * f64.size() + 1024);
+        double* q = ctx->workspace("vec64", bb.vec64);
         for (auto* pp : f64) { *pp = q; q += total; }
         if (refine) {
             float** f32[] = {&V32.x, &V32.r, &V32.r0, &V32.p, &V32.v, &V32.s, &V32.t, &V32.y, &V32.z, &V32.tmp};
-            float* q32 = reinterpret_cast<float*>(ctx->workspace("vec32", sizeof(float) * total * 10 + 1024));
+            float* q32 = reinterpret_cast<float*>(ctx->workspace("vec32", bb.vec32));
             for (auto* pp : f32) { *pp = q32; q32 += total; }
         }
     }
@@ -380,7 +411,8 @@ static void run_batch(Ctx* ctx, const std::vector<Group>& groups, const std::vec
             for (int b = 0; b < nb[g]; ++b) pb.push_back((int)pe.size());
             pe.push_back(e);
         }
-    PackEntry* dpe = reinterpret_cast<PackEntry*>(ctx->workspace("pack", sizeof(PackEntry) * pe.size() + sizeof(int) * pb.size() + 512));
+    if (pe.size() != (size_t)G * NR || pb.size() != (size_t)blocks * NR) throw std::logic_error("pack table size");
+    PackEntry* dpe = reinterpret_cast<PackEntry*>(ctx->workspace("pack", bb.pack));
     int* dpb = reinterpret_cast<int*>(reinterpret_cast<char*>(dpe) + ((sizeof(PackEntry) * pe.size() + 255) & ~size_t(255)));
     HIPCHK(hipMemcpyAsync(dpe, pe.data(), sizeof(PackEntry) * pe.size(), hipMemcpyHostToDevice, st));
     HIPCHK(hipMemcpyAsync(dpb, pb.data(), sizeof(int) * pb.size(), hipMemcpyHostToDevice, st));
@@ -1041,7 +1073,7 @@ struct Server : ServerBase {
     Buffer p32, p64, states, tables;
     int slots;
     std::vector<int> free_slots;
-    std::map<int64_t, int64_t> free_ranges;   // start -> length
+    RangePool ranges;   // vector ranges of the pools
     std::vector<Req> reqs;
     bool dirty = true;
     int nblocks = 0;
@@ -1062,38 +1094,12 @@ struct Server : ServerBase {
         slots = Ctx::kHostStates / kMaxNR;
         states = Buffer(sizeof(KState) * (size_t)slots * kMaxNR);
         for (int i = slots - 1; i >= 0; --i) free_slots.push_back(i);
-        free_ranges[0] = cap;
+        ranges.reset(cap);
     }
     float* v32(int k) const { return p32.template as<float>() + (size_t)k * cap; }
     double* v64(int k) const { return p64.template as<double>() + (size_t)k * cap; }
     int active() const override { return (int)reqs.size(); }
 
-    bool take_range(int64_t len, int64_t& off) {
-        for (auto it = free_ranges.begin(); it != free_ranges.end(); ++it)
-            if (it->second >= len) {
-                off = it->first;
-                const int64_t rest = it->second - len;
-                free_ranges.erase(it);
-                if (rest > 0) free_ranges[off + len] = rest;
-                return true;
-            }
-        return false;
-    }
-    void give_range(int64_t off, int64_t len) {
-        auto it = free_ranges.emplace(off, len).first;
-        auto nx = std::next(it);
-        if (nx != free_ranges.end() && it->first + it->second == nx->first) {
-            it->second += nx->second;
-            free_ranges.erase(nx);
-        }
-        if (it != free_ranges.begin()) {
-            auto pv = std::prev(it);
-            if (pv->first + pv->second == it->first) {
-                pv->second += it->second;
-                free_ranges.erase(it);
-            }
-        }
-    }
     // one pack launch per lane: b64 (lane-major) from the caller, or x64 to the caller
     void lane_pack(const Req& q, int k, double* ws64, int to_ws) {
         const hipStream_t st = ctx->stream;
@@ -1157,8 +1163,11 @@ struct Server : ServerBase {
         if (op->precond == CPM_PRECOND_JACOBI) throw ArgError("served operators use the diagonal or no preconditioner");
         const int64_t len = (op->D * NR + 63) & ~int64_t(63);
         if (len > cap) throw ArgError("request larger than the server pool");
+        // the FP64 re-solve of this request's lanes (when their refinement stalls) runs in the
+        // context's workspaces: grow them now, so that retiring the request allocates nothing
+        reserve_batch_workspaces(ctx, op->D, NR, false);
         int64_t off;
-        if (free_slots.empty() || !take_range(len, off)) return false;   // wait for memory
+        if (free_slots.empty() || !ranges.take(len, off)) return false;   // wait for memory
         Req q{tag, op, nrhs, off, len, free_slots.back(), {}, {}};
         free_slots.pop_back();
         q.rhs.assign(rhs, rhs + nrhs);
@@ -1238,16 +1247,19 @@ struct Server : ServerBase {
         // a private copy: FP64 re-solves below poll through the context's pinned state buffer
         const std::vector<KState> snap(ctx->host_states, ctx->host_states + (size_t)slots * kMaxNR);
         const KState* hs = snap.data();
-        std::vector<Req> keep;
+        std::vector<Req> keep, leaving;
         for (Req& q : reqs) {
             const KState* s = hs + (size_t)q.slot * kMaxNR;
             bool running = false;
             for (int k = 0; k < q.nr; ++k)
                 running |= s[k].status == ST_RUN || (s[k].status >= ST_UPDATE && s[k].status <= ST_READY);
-            if (running) {
-                keep.push_back(q);
-                continue;
-            }
+            (running ? keep : leaving).push_back(q);
+        }
+        reqs.swap(keep);   // the leaving requests are out of the iteration before anything can throw
+        if (leaving.empty()) return;
+        dirty = true;
+        auto finish = [&](const Req& q) {
+            const KState* s = hs + (size_t)q.slot * kMaxNR;
             Done d{q.tag, std::vector<cpm_solve_info>((size_t)q.nr)};
             for (int k = 0; k < q.nr; ++k) {
                 cpm_solve_info& inf = d.infos[(size_t)k];
@@ -1257,14 +1269,22 @@ struct Server : ServerBase {
                 inf.updates = s[k].updates;
                 inf.rel_residual = s[k].bsq > 0 ? std::sqrt(s[k].rr_true / s[k].bsq) : 0.0;
             }
-            for (int k = 0; k < q.nr; ++k) lane_pack(q, k, v64(1), 0);
-            fp64_fallback(ctx, std::vector<Op*>((size_t)q.nr, q.op), q.rhs, q.xs, o, d.infos, nullptr);
+            try {
+                for (int k = 0; k < q.nr; ++k) lane_pack(q, k, v64(1), 0);
+                NoBuilds nb(ctx);   // the re-solve uses the workspaces reserved at submission
+                fp64_fallback(ctx, std::vector<Op*>((size_t)q.nr, q.op), q.rhs, q.xs, o, d.infos, nullptr);
+            } catch (...) {
+                for (auto& inf : d.infos) inf.status = CPM_SOLVE_NOT_WRITTEN;
+                done.push_back(std::move(d));
+                throw;
+            }
             done.push_back(std::move(d));
-            give_range(q.off, q.len);
+        };
+        auto release = [&](const Req& q) {
+            if (!ranges.give(q.off, q.len)) throw std::logic_error("server range given back twice");
             free_slots.push_back(q.slot);
-            dirty = true;
-        }
-        reqs.swap(keep);
+        };
+        retire_requests(leaving, finish, release);
     }
 };
 

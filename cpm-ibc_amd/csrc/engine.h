@@ -8,6 +8,7 @@
 #include <string>
 #include <vector>
 
+#include "refine.h"
 #include "tuned.h"
 
 namespace cpmgpu {
@@ -17,33 +18,6 @@ constexpr int kMaxNR = 8;            // right-hand sides per multi-RHS sweep
 
 // Row flags (one byte per row).
 enum : uint8_t { ROW_IDENTITY = 1, ROW_NODIAG = 2 };
-
-// Per (instance, rhs) Krylov state.  Written only by the one-block finalise
-// kernels (krylov.hip) and read uniformly by every block of the next kernels.
-enum : int {
-    ST_RUN = 0, ST_CONVERGED = 1, ST_MAXIT = 2, ST_RESTART = 3, ST_ZERO_RHS = 4, ST_BREAKDOWN = 5,
-    // mixed precision: the FP32 recurrence asks for an FP64 refinement step
-    // (UPDATE*), the true residual is rescaled into the FP32 vectors
-    // (RESCALE), and the lane rejoins at the next iteration (READY)
-    ST_UPDATE = 6, ST_UPDATE_MAXIT = 7, ST_RESCALE = 8, ST_READY = 9,
-    // mixed precision: refinement stopped reducing the true residual; the
-    // host re-solves the system in FP64 from zero
-    ST_FALLBACK = 10
-};
-
-struct KState {
-    double rho, rho_old, alpha, omega, beta;
-    double bsq, r0sq, tol2, rr;
-    double rr_true;      // mixed: ||b - A x64||^2 at the last refinement step
-    double scale;        // mixed: 1 / ||b - A x64|| (FP32 vectors are scaled by it)
-    double maxrr;        // mixed: largest scaled recurrence residual^2 since the last step
-    double delta2;       // mixed: refine when rr < delta2 * maxrr
-    int iters, restarts, status, max_it;
-    int updates;
-    int fails;           // mixed: refinement steps that reduced the true residual by less than 10%
-    int judged;          // mixed: the pending step was asked for by the delta test (its reduction is judged)
-    int pad;
-};
 
 enum : int { OP_ELL = 0, OP_TP = 1 };
 

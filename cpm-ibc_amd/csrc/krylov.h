@@ -18,6 +18,8 @@ struct LaunchCfg {
 // Whether E gathers of T x nr values use shared-stencil tiles when every
 // operator of the launch has at least ntiles of them.
 bool grp_gather(int tsize, int nr, int ntiles);
+// Re-read the CPMGPU_GROUP_GATHER override (once per launch sequence).
+void grp_refresh_override();
 
 // Workspace vectors of one precision; every vector spans all segments
 // (segment g occupies [off_g, off_g + D_g * NR)).

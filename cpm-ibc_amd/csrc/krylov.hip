@@ -51,6 +51,8 @@
 // rounding: the FP32 and FP64 operators differ in the last bits.)
 #include <hip/hip_runtime.h>
 
+#include <atomic>
+
 #include <cmath>
 #include <cstdlib>
 #include <string>
@@ -1043,52 +1045,6 @@ template <> struct FinTraits<FIN_ITER_MX> { static constexpr int NV = 2; static 
 constexpr double kEps = 2.220446049250313e-16;   // NumTraits<double>::epsilon (BiCGStab.h:250)
 
 // FP32 recurrence (re)started from the scaled true residual.
-__device__ __forceinline__ void mx_init(KState& s, double rr, int next) {
-    s.rr = s.rho = s.r0sq = s.maxrr = rr;
-    s.rho_old = s.alpha = s.omega = 1.0;
-    s.beta = s.rho;
-    s.status = (rr > 0.0 && std::isfinite(rr)) ? next : ST_BREAKDOWN;
-}
-
-// Refinement step: the true residual ||b - A64 x64||^2 is rt.  A lane that
-// took a new right-hand side (bsq < 0, x64 = 0) learns ||b||^2 = rt here, as
-// FIN_MX_START does for the first ones.
-//
-// The FP32 recurrence can only resolve the correction to about eps32 times
-// the condition number; on fine grids (h = 0.01, condition ~1e5) asking it
-// for delta = 1e-2 per step leaves corrections that do not reduce the true
-// residual.  For steps asked for by the delta test (the recurrence residual
-// fell by delta), a true reduction that falls short of the recurrence's by
-// more than ~3x raises delta 10x (to at most 0.5); a step that does not
-// reduce the true residual, or three that reduce it by less than 10%, hand
-// the lane to an FP64 re-solve from zero (ST_FALLBACK).
-__device__ __forceinline__ void mx_update(KState& s, double rt) {
-    ++s.updates;
-    const double prev = s.rr_true;
-    const bool fresh = s.bsq < 0.0;
-    s.rr_true = rt;
-    if (fresh) {
-        s.bsq = rt;
-        if (!std::isfinite(rt)) { s.status = ST_BREAKDOWN; return; }
-        if (rt == 0.0) { s.status = ST_ZERO_RHS; return; }
-        if (!(rt > s.tol2 * rt)) { s.status = ST_CONVERGED; return; }
-        if (s.max_it <= 0) { s.status = ST_MAXIT; return; }
-        s.tol2 *= rt;
-    }
-    if (!std::isfinite(rt)) { s.status = ST_FALLBACK; return; }
-    if (!(rt > s.tol2)) { s.status = ST_CONVERGED; return; }
-    if (s.status == ST_UPDATE_MAXIT) { s.status = ST_MAXIT; return; }
-    if (!fresh && s.judged) {
-        const double a = rt / prev;          // squared true reduction of this step
-        const double c = s.rr / s.r0sq;      // squared reduction the recurrence reached
-        if (!(a < 1.0)) { s.status = ST_FALLBACK; return; }
-        if (a > 10.0 * c) s.delta2 = fmin(0.25, 100.0 * s.delta2);
-        if (a > 0.81 && ++s.fails >= 3) { s.status = ST_FALLBACK; return; }
-    }
-    s.scale = 1.0 / sqrt(rt);
-    s.status = ST_RESCALE;
-}
-
 // The scalar part of the recurrence for one (instance, rhs).
 template <int KIND>
 __device__ void fin_scalar(KState& s, const double* tot) {
@@ -1149,7 +1105,7 @@ __device__ void fin_scalar(KState& s, const double* tot) {
         if (std::isnan(rr)) { s.status = ST_BREAKDOWN; return; }
         // scaled tolerance: ||r|| <= tol ||b|| with r scaled by `scale`
         s.judged = 0;
-        if (!(rr > s.tol2 * s.scale * s.scale)) { s.status = ST_UPDATE; return; }
+        if (!(rr > s.tol2 * s.scale * s.scale)) { s.judged = 2; s.status = ST_UPDATE; return; }
         if (s.iters >= s.max_it) { s.status = ST_UPDATE_MAXIT; return; }
         if (rr < s.delta2 * s.maxrr) { s.judged = 1; s.status = ST_UPDATE; return; }
         s.rho_old = s.rho;
@@ -1253,10 +1209,21 @@ struct GrpWaves {
 // measured faster than row tiles (tubes of 42 k to 406 k rows).
 // CPMGPU_GROUP_GATHER=all uses them for every shape whose tile fits the LDS,
 // =none for none.
-bool grp_gather(int tsize, int nr, int ntiles) {
+// The override is read from the environment once per solve, tile build or
+// query (grp_refresh_override, called where a launch sequence starts), not
+// per launch: a solve makes thousands of launches.
+static std::atomic<int> g_grp_override{-2};   // -2 unread, -1 none, 0 "none", 1 "all"
+
+void grp_refresh_override() {
     const char* e = std::getenv("CPMGPU_GROUP_GATHER");
     const std::string v = e ? e : "";
-    if (v == "all" || v == "none") return v == "all";
+    g_grp_override.store(v == "all" ? 1 : v == "none" ? 0 : -1, std::memory_order_relaxed);
+}
+
+bool grp_gather(int tsize, int nr, int ntiles) {
+    if (g_grp_override.load(std::memory_order_relaxed) == -2) grp_refresh_override();
+    const int ov = g_grp_override.load(std::memory_order_relaxed);
+    if (ov >= 0) return ov == 1;
     if (!((tsize == 4 && nr != 4) || (tsize == 8 && nr == 4))) return false;
     // one right-hand side: row tiles were faster up to ~600 tiles
     return ntiles >= (nr == 1 ? 768 : kGrpMinTiles);

@@ -24,9 +24,20 @@ struct Ctx {
     std::map<std::string, std::unique_ptr<Buffer>> ws;        // grow-only named workspaces
     std::map<void*, int64_t> user_bytes;
     int64_t op_bytes = 0;
+    bool no_builds = false;   // set: solves build no shared-stencil tiles (they would allocate)
     explicit Ctx(int dev);
     ~Ctx();
     double* workspace(const char* name, size_t bytes);
+};
+
+// Solves on the context build no tiles while this object lives.
+struct NoBuilds {
+    Ctx* c;
+    bool old;
+    explicit NoBuilds(Ctx* c_) : c(c_), old(c_->no_builds) { c->no_builds = true; }
+    ~NoBuilds() { c->no_builds = old; }
+    NoBuilds(const NoBuilds&) = delete;
+    NoBuilds& operator=(const NoBuilds&) = delete;
 };
 
 // Operator in device memory, in one of two execution formats (see OpView):

@@ -328,8 +328,11 @@ static void parallel_for(int64_t n, Body&& body) {
 // number fewer than kGrpMinTiles.
 static bool build_group_tiles(Op& op, const HostTp& h, const Buffer& tpw64, const Buffer& tpw32) {
     const int64_t D = h.D;
-    int max_items = kBlock;   // CPMGPU_GROUP_ITEMS: fewer items per tile (more, smaller tiles)
-    if (const char* e = std::getenv("CPMGPU_GROUP_ITEMS")) max_items = std::max(1, std::min(kBlock, std::atoi(e)));
+    // CPMGPU_GROUP_ITEMS: fewer items per tile (more, smaller tiles); read once per process
+    static const int max_items = [] {
+        const char* e = std::getenv("CPMGPU_GROUP_ITEMS");
+        return e ? std::max(1, std::min(kBlock, std::atoi(e))) : kBlock;
+    }();
     const int64_t nblocks = (D + kBlock - 1) / kBlock;
     if (D >= (int64_t)kGrpOfsMask || nblocks < kGrpMinTiles) return false;
     struct Item {
@@ -686,6 +689,7 @@ void op_ensure_group_tiles(Op& op) {
 }
 
 void ensure_group_tiles_for(const std::vector<Op*>& ops, int tsize, int nr) {
+    grp_refresh_override();
     if (!grp_gather(tsize, nr, INT_MAX)) return;
     for (Op* o : ops) op_ensure_group_tiles(*o);
 }

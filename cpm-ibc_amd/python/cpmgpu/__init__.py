@@ -15,7 +15,7 @@ LIB_PATH = os.path.normpath(os.path.join(_HERE, "..", "..", "libcpmgpu.so"))
 CPM_FP64 = 0
 CPM_FP32_REFINE = 1
 PRECOND_IDENTITY, PRECOND_DIAGONAL, PRECOND_JACOBI = 0, 1, 2
-STATUS = {1: "converged", 2: "maxit", 4: "zero_rhs", 5: "breakdown"}
+STATUS = {1: "converged", 2: "maxit", 4: "zero_rhs", 5: "breakdown", 6: "not_written"}
 
 EXPORTS = [
     "cpm_solve_opts_default", "cpm_last_error", "cpm_ctx_create", "cpm_ctx_destroy", "cpm_ctx_synchronize",
@@ -456,6 +456,7 @@ class Operator:
         info = SolveInfo()
         _chk(load().cpm_solve_device(self._h, d_rhs, d_x, ctypes.byref(o or opts()), ctypes.byref(info)))
         return info.as_dict()
+
 
 
 class CSR:

@@ -47,7 +47,9 @@ enum cpm_solve_status {
     CPM_SOLVE_CONVERGED = 1,
     CPM_SOLVE_MAXIT = 2,
     CPM_SOLVE_ZERO_RHS = 4,
-    CPM_SOLVE_BREAKDOWN = 5   /* NaN/Inf in a recurrence scalar */
+    CPM_SOLVE_BREAKDOWN = 5,  /* NaN/Inf in a recurrence scalar */
+    CPM_SOLVE_NOT_WRITTEN = 6 /* serving: writing out the request's results failed (the cpm_server_run
+                                 call that retired it returned an error); its d_x are not valid */
 };
 
 typedef struct cpm_ctx cpm_ctx;
@@ -175,7 +177,12 @@ int cpm_solve_stream_device(cpm_op* op, int n_rhs, const double* const* d_rhs, d
  * operator may come from another context on the same device (e.g. a build
  * thread's, cpm_problem_create_on) and must stay alive until its request is
  * reported.  Systems whose refinement stalls are re-solved in FP64 when their
- * request leaves (cpm_solve_info.fallback_iters).  One host thread per server. */
+ * request leaves (cpm_solve_info.fallback_iters); cpm_server_submit grows the
+ * context's workspaces for that re-solve (about 90 B per row and lane) before
+ * it admits a request, so retiring requests allocates nothing, and returns
+ * CPM_ERR_NOMEM if they cannot grow.  If writing out a request's results fails,
+ * cpm_server_run still retires every stopped request, reports that one with
+ * status CPM_SOLVE_NOT_WRITTEN and returns the error.  One host thread per server. */
 int cpm_server_create(cpm_ctx* ctx, int64_t pool_bytes, int lanes, const cpm_solve_opts* opts, cpm_server** out);
 int cpm_server_destroy(cpm_server* server);
 int cpm_server_submit(cpm_server* server, cpm_op* op, int n_rhs, const double* const* d_rhs, double* const* d_x,

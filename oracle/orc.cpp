@@ -105,11 +105,13 @@ SurfaceSpec SurfaceSpec::defaults(const std::string& type) {
         s.params = {2.0, 1.0}; s.bbox = {-3.0, 3.0}; s.dim = 3;
     } else if (type == "Ellipsoid") {
         s.params = {1.5, 1.0, 0.75}; s.bbox = {-1.5, 1.5}; s.dim = 3;
+    } else if (type == "Arc") {                    // :104-111: radius, angle1, angle2
+        s.params = {1.0, -3 * M_PI_4, M_PI_4}; s.bbox = {-1.0, 1.0}; s.dim = 2;
     } else {
         throw std::runtime_error("No default parameters for surface '" + type + "'");
     }
-    // SetIsOpen: of these, only Hemisphere/Arc/Line... are open; not needed for cp.
-    s.is_open = (type == "Hemisphere");
+    // SetIsOpen (:370-381): of these, Hemisphere and Arc are open.
+    s.is_open = (type == "Hemisphere" || type == "Arc");
     return s;
 }
 
@@ -197,6 +199,16 @@ void Surface::parameterise() {
             xp[i][0] = a * std::sin(thetap[i]) * std::cos(phip[i]);
             xp[i][1] = b * std::sin(thetap[i]) * std::sin(phip[i]);
             xp[i][2] = c * std::cos(thetap[i]);
+        }
+    } else if (ty == "Arc") {                     // paramArc :1907-1924, Np = 100 (z = 0 in 3-D)
+        size_t Np = 100;
+        double R = P[0], a1 = P[1], a2 = P[2], dth = (a2 - a1) / (Np - 1);
+        xp.assign(Np, Vec((size_t)s_.dim, 0.0));
+        thetap.assign(Np, 0);
+        for (size_t i = 0; i < Np; ++i) {
+            thetap[i] = a1 + i * dth;
+            xp[i][0] = R * std::cos(thetap[i]);
+            xp[i][1] = R * std::sin(thetap[i]);
         }
     } else if (ty == "Point") {
         xp.assign(1, Vec(P.begin(), P.begin() + s_.dim));
@@ -408,6 +420,8 @@ void Surface::cp(const double* x, double* cpx, double& dist, i64& bdy) const {
         dist = 0.0;
         for (int d = 0; d < 2; ++d) dist += sq(x[d] - cpx[d]);
         dist = std::sqrt(dist);
+    } else if (ty == "Arc") {                      // cpArc :507-579
+        cp_arc(x, cpx, dist, bdy);
     } else if (ty == "Point") {                    // :840-848
         for (int d = 0; d < s_.dim; ++d) cpx[d] = P[d];
         dist = distance(x, cpx, s_.dim);
@@ -450,6 +464,41 @@ void Surface::cp(const double* x, double* cpx, double& dist, i64& bdy) const {
     } else {
         throw std::runtime_error("Invalid surface '" + ty + "' in Surface::cp");
     }
+}
+
+// Arc of the circle of radius R about the origin in the xy-plane between the
+// angles a1 < a2 (both in (-pi, pi]; z = 0 in 3-D), Surface::cpArc
+// (Surface.cpp:507-579): points whose polar angle lies strictly inside the
+// arc project radially (bdy 0); the others go to the nearer end along the
+// circle, split at the angle opposite the arc's midpoint: end 1 at a1
+// (bdy 1), end 2 at a2 (bdy 2).
+void Surface::cp_arc(const double* x, double* cpx, double& dist, i64& bdy) const {
+    const auto& P = s_.params;
+    const double R = P[0], a1 = P[1], a2 = P[2];
+    if (a1 < -M_PI) throw std::runtime_error("Arc: angle1 must lie in (-pi, pi]");
+    if (a2 > M_PI) throw std::runtime_error("Arc: angle2 must lie in (-pi, pi]");
+    if (a1 >= a2) throw std::runtime_error("Arc: angle1 must be less than angle2");
+    const double half = 0.5 * (2 * M_PI - (a2 - a1));
+    const double th = std::atan2(x[1], x[0]);
+    double ang;
+    if (th > a1 && th < a2) {
+        ang = th;
+        bdy = 0;
+    } else if (th <= a1) {
+        const bool first = th >= a1 - half;
+        bdy = first ? 1 : 2;
+        ang = first ? a1 : a2;
+    } else {   // th >= a2
+        const bool second = th < a2 + half;
+        bdy = second ? 2 : 1;
+        ang = second ? a2 : a1;
+    }
+    cpx[0] = R * std::cos(ang);
+    cpx[1] = R * std::sin(ang);
+    if (s_.dim == 3) cpx[2] = 0.0;
+    double d2 = 0.0;
+    for (int d = 0; d < s_.dim; ++d) d2 += sq(x[d] - cpx[d]);
+    dist = std::sqrt(d2);
 }
 
 // Polyline closest point.  The reference delegates to fcpw (FP32 SAH BVH,

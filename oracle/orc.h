@@ -91,6 +91,7 @@ private:
     SurfaceSpec s_;
     void parameterise();        // :193-366
     void cp_polyline(const double* x, double* cpx, double& dist, i64& bdy) const;
+    void cp_arc(const double* x, double* cpx, double& dist, i64& bdy) const;
     void cp_triangles(const double* x, double* cpx, double& dist) const;
     std::vector<i64> nearest_two_xp(const double* x, double& min_dist) const;  // :2661-2728
 };

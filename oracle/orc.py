@@ -71,11 +71,11 @@ def lib():
         L.orc_rd_run.argtypes = [ctypes.c_int, ctypes.c_double, ctypes.c_int, ctypes.c_int, f64p, ctypes.c_int,
                                  ctypes.c_void_p, ctypes.c_void_p]
         L.orc_heat_study_order.restype = ctypes.c_double
-        L.orc_heat_study_order.argtypes = [ctypes.c_int]
+        L.orc_heat_study_order.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int]
         L.orc_heat_problem_error.restype = ctypes.c_double
         L.orc_heat_problem_error.argtypes = [ctypes.c_int, ctypes.c_double, ctypes.c_int, ctypes.c_int]
         L.orc_ibc_study_order.restype = ctypes.c_double
-        L.orc_ibc_study_order.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.POINTER(ctypes.c_int)]
+        L.orc_ibc_study_order.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.POINTER(ctypes.c_int)]
         _LIB = L
     return _LIB
 
@@ -317,9 +317,10 @@ class IBCProblem(Problem):
         return pts.reshape(-1, 3), kind
 
 
-def heat_study_order(method):
-    """HeatProblemsWithIBC problems 3-5 (method 0 explicit, 1 backward Euler, 2 Crank-Nicolson)."""
-    o = lib().orc_heat_study_order(method)
+def heat_study_order(method, boundary_order=2, boundary_type=0):
+    """HeatProblemsWithIBC problems 3-5 (method 0 explicit, 1 backward Euler, 2 Crank-Nicolson) with a
+    Dirichlet (type 0) or Neumann (type 1) curve of boundary order 1 or 2."""
+    o = lib().orc_heat_study_order(method, boundary_order, boundary_type)
     if o < 0:
         raise RuntimeError(_err())
     return o
@@ -345,9 +346,11 @@ def heat_problem_error(method, dx, boundary_order=2, boundary_type=0):
     return e
 
 
-def ibc_study_order(problem, g):
+def ibc_study_order(problem, g, boundary_order=2):
+    """PoissonProblemsWithIBC problem 2 (sphere + circle), 5 (sphere + point) or 8 (sphere + arc) on
+    the seeded geometry g, Dirichlet curves of boundary order 1 or 2."""
     it = ctypes.c_int(0)
-    o = lib().orc_ibc_study_order(problem, g, ctypes.byref(it))
+    o = lib().orc_ibc_study_order(problem, g, boundary_order, ctypes.byref(it))
     if o < 0:
         raise RuntimeError(_err())
     return o

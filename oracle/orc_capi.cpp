@@ -12,8 +12,8 @@
 
 namespace orc {
 double poisson_problem_error(int problem, double dx, int* iters);
-double ibc_study_order(int problem, int g, int* iters_last);
-double heat_study_order(int method);
+double ibc_study_order(int problem, int g, int* iters_last, int order);
+double heat_study_order(int method, int order, int btype);
 double heat_problem_error(int method, double dx, int order, int btype);
 int64_t heat_run_study(int method, double dx, int steps, int order, int btype, double* u_out, double* dt_out);
 std::array<int, 4> geodesic_run(const Tube& t, const IBCData& I, const CSR& E, const CSR& Lfull, const CSR& Efull,
@@ -252,18 +252,18 @@ void orc_rhs_points(void* hp, double* pts, int32_t* kind) {
     std::memcpy(kind, h->rhs_kind.data(), h->rhs_kind.size() * 4);
 }
 
-double orc_ibc_study_order(int problem, int g, int* iters) {
+double orc_ibc_study_order(int problem, int g, int order, int* iters) {
     try {
-        return ibc_study_order(problem, g, iters);
+        return ibc_study_order(problem, g, iters, order);
     } catch (const std::exception& e) {
         g_err = e.what();
         return -1.0;
     }
 }
 
-double orc_heat_study_order(int method) {
+double orc_heat_study_order(int method, int order, int btype) {
     try {
-        return heat_study_order(method);
+        return heat_study_order(method, order, btype);
     } catch (const std::exception& e) {
         g_err = e.what();
         return -1.0;

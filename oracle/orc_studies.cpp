@@ -112,15 +112,23 @@ static double sphere_harmonic(const double* x) {
     return std::cos(3 * th) * std::pow(std::sin(ph), 3) * (9 * std::pow(std::cos(ph), 2) - 1);
 }
 
+// The study's curve of problem 2 (PoissonSphereCircleIBC, :221-233: circle
+// with the seeded normal), 5 (PoissonSpherePointIBC, :274-279: point at
+// angles (endpoint1, endpoint2)) or 8 (PoissonSphereArcIBC, :318-326: arc of
+// the equator between endpoint1 and endpoint2; endpoints :398-405).
 SurfaceSpec ibc_study_spec(int problem, int g, Vec& normal_state) {
     SurfaceSpec s;
+    const double e1 = 2.0 * M_PI * kRand1[g] - M_PI, e2 = 2.0 * M_PI * kRand2[g] - M_PI;
     if (problem == 2) {
         s.type = "Circle3D";
         s.params = circle3d_params(normal_state);
     } else if (problem == 5) {
-        const double th = 2.0 * M_PI * kRand1[g] - M_PI, ph = 2.0 * M_PI * kRand2[g] - M_PI;
         s.type = "Point";
-        s.params = {std::cos(ph) * std::cos(th), std::cos(ph) * std::sin(th), std::sin(ph)};
+        s.params = {std::cos(e2) * std::cos(e1), std::cos(e2) * std::sin(e1), std::sin(e2)};
+    } else if (problem == 8) {
+        s.type = "Arc";
+        s.params = {1.0, e1, e2};
+        s.is_open = true;   // SetIsOpen (SurfaceSpecifier.cpp:377)
     } else {
         throw std::runtime_error("unsupported IBC study problem");
     }
@@ -129,11 +137,12 @@ SurfaceSpec ibc_study_spec(int problem, int g, Vec& normal_state) {
     return s;
 }
 
-// One level of the study: error of the 2nd-order Dirichlet IBC solve at dx.
-double ibc_problem_error(int problem, int g, double dx, Vec& normal_state, int* iters) {
+// One level of the study: error of the Dirichlet IBC solve of boundary order
+// `order` (g_boundary_order, :14) at dx.
+double ibc_problem_error(int problem, int g, double dx, Vec& normal_state, int* iters, int order) {
     SurfaceSpec ibc = ibc_study_spec(problem, g, normal_state);
     ibc.meta.on_ibc_tolerance = 0.1 * dx * dx;
-    ibc.meta.boundary_order = 2;
+    ibc.meta.boundary_order = order;
     PoissonIBCProblem P = make_poisson_ibc(SurfaceSpec::defaults("Sphere"), {ibc}, dx);
     auto f = [](const double* x) { return 30 * sphere_harmonic(x); };
     Vec rhs = poisson_ibc_rhs(P, f, sphere_harmonic);
@@ -155,12 +164,12 @@ double ibc_problem_error(int problem, int g, double dx, Vec& normal_state, int* 
     return e;
 }
 
-double ibc_study_order(int problem, int g, int* iters_last) {
+double ibc_study_order(int problem, int g, int* iters_last, int order) {
     Vec normal = {kRand1[g], kRand2[g], kRand3[g]};
     Vec err, dxs;
     for (int lv = 0; lv < 3; ++lv) {
         const double dx = std::pow(0.5, lv) * 0.2;
-        err.push_back(ibc_problem_error(problem, g, dx, normal, iters_last));
+        err.push_back(ibc_problem_error(problem, g, dx, normal, iters_last, order));
         dxs.push_back(dx);
     }
     return average_order(err, dxs);
@@ -183,12 +192,23 @@ static Vec circle3d_params_pp(Vec normal, const double* pp) {
     return {std::sqrt(1 - rho * rho), rho * normal[0], rho * normal[1], rho * normal[2], normal[0], normal[1], normal[2]};
 }
 
+// The study's curve (:94-111): for Dirichlet curves (g_boundary_type 0) the
+// circle cut by the plane of normal (1.1, 1.234, 1.321) through a point at
+// r = 0.27, theta 0.132, phi 0.221; for Neumann curves the great circle y = 0
+// (normal (0, 1, 0) through the origin), on which u = e^{-2t} cos(phi) has a
+// zero normal derivative.
 SurfaceSpec heat_study_ibc(double dx, int order = 2, int btype = 0) {
-    const double r = 0.27, theta = 0.132, phi = 0.221;    // :96-100
-    const double pp[3] = {r * std::cos(phi) * std::sin(theta), r * std::sin(phi) * std::sin(theta), r * std::cos(theta)};
     SurfaceSpec s;
     s.type = "Circle3D";
-    s.params = circle3d_params_pp({1.1, 1.234, 1.321}, pp);
+    if (btype == 0) {
+        const double r = 0.27, theta = 0.132, phi = 0.221;
+        const double pp[3] = {r * std::cos(phi) * std::sin(theta), r * std::sin(phi) * std::sin(theta),
+                              r * std::cos(theta)};
+        s.params = circle3d_params_pp({1.1, 1.234, 1.321}, pp);
+    } else {
+        const double pp[3] = {0.0, 0.0, 0.0};
+        s.params = circle3d_params_pp({0.0, 1.0, 0.0}, pp);
+    }
     s.bbox = {-1.0, 1.0};
     s.dim = 3;
     s.meta.on_ibc_tolerance = 0.1 * dx * dx;
@@ -298,11 +318,13 @@ double heat_problem_error(int method, double dx, int order, int btype) {
     return e;
 }
 
-double heat_study_order(int method) {   // ConvergenceStudy(0.1, 3, Heat) (:363-366)
+// ConvergenceStudy(0.1, 3, Heat) (:363-366) of problems 3-5 for boundary
+// order `order` and type `btype` (g_boundary_order, g_boundary_type, :20-21).
+double heat_study_order(int method, int order, int btype) {
     Vec err, dxs;
     for (int lv = 0; lv < 3; ++lv) {
         const double dx = std::pow(0.5, lv) * 0.1;
-        err.push_back(heat_problem_error(method, dx, 2, 0));
+        err.push_back(heat_problem_error(method, dx, order, btype));
         dxs.push_back(dx);
     }
     return average_order(err, dxs);

@@ -1,6 +1,6 @@
 """HeatWithIBC time stepping on the GPU (cpm_heat_step_device) against the
 oracle's restatement of HeatWithIBC::TimeStep on the heat study's system
-(sphere, Circle3D Dirichlet IBC, 2nd order; HeatProblemsWithIBC.cpp:90-193)."""
+(sphere with a Circle3D curve; HeatProblemsWithIBC.cpp:90-193)."""
 import numpy as np
 import pytest
 
@@ -10,7 +10,11 @@ import orc
 pytestmark = pytest.mark.gpu
 
 
-def _heat_ibc_params():
+def _heat_ibc_params(btype=0):
+    """The study's curve (HeatProblemsWithIBC.cpp:94-111): a tilted circle for Dirichlet curves,
+    the great circle y = 0 for Neumann curves."""
+    if btype == 1:
+        return [1.0, 0.0, 0.0, 0.0, 0.0, 1.0, 0.0]
     r, th, ph = 0.27, 0.132, 0.221
     pp = np.array([r * np.cos(ph) * np.sin(th), r * np.sin(ph) * np.sin(th), r * np.cos(th)])
     n = np.array([1.1, 1.234, 1.321])
@@ -32,7 +36,7 @@ def test_heat_steps_match_oracle(gpu_ctx, method, steps, order, btype):
     dx = 0.1
     u_ref, dt = orc.heat_run(method, dx, steps, order, btype)
     tube = cpmgpu.Tube(gpu_ctx, "Sphere", dx)
-    prob = cpmgpu.Problem(tube, [cpmgpu.IBC("Circle3D", params=_heat_ibc_params(), boundary_order=order,
+    prob = cpmgpu.Problem(tube, [cpmgpu.IBC("Circle3D", params=_heat_ibc_params(btype), boundary_order=order,
                                             boundary_type=(btype, btype), on_ibc_tolerance=0.1 * dx * dx)])
     assert prob.D == len(u_ref)
     r = 1.0 if method == 1 else 0.5

@@ -12,8 +12,11 @@ import orc
 pytestmark = pytest.mark.gpu
 
 
-def _circle():
-    """The heat study's Circle3D curve (HeatProblemsWithIBC.cpp:96-110), which the oracle builds."""
+def _circle(btype=0):
+    """The heat study's Circle3D curve (HeatProblemsWithIBC.cpp:94-111), which the oracle builds:
+    tilted for Dirichlet curves, the great circle y = 0 for Neumann curves."""
+    if btype == 1:
+        return [1.0, 0.0, 0.0, 0.0, 0.0, 1.0, 0.0]
     r, th, ph = 0.27, 0.132, 0.221
     pp = np.array([r * np.cos(ph) * np.sin(th), r * np.sin(ph) * np.sin(th), r * np.cos(th)])
     n = np.array([1.1, 1.234, 1.321])
@@ -37,7 +40,7 @@ def test_rd_steps_match_oracle(gpu_ctx, method, steps, order, btype):
     print(f"case rd method={method} steps={steps} order={order} type={btype}", flush=True)
     dx = 0.1
     tube = cpmgpu.Tube(gpu_ctx, "Sphere", dx)
-    prob = cpmgpu.Problem(tube, [cpmgpu.IBC("Circle3D", params=_circle(), boundary_order=order,
+    prob = cpmgpu.Problem(tube, [cpmgpu.IBC("Circle3D", params=_circle(btype), boundary_order=order,
                                             boundary_type=(btype, btype), on_ibc_tolerance=0.1 * dx * dx)])
     p = cpmgpu.rd_params(dx, method=method)
     if method == 1:

@@ -219,15 +219,6 @@ def test_reference_heat_ibc_convergence_orders(parallel_oracle, method, expected
     assert abs(order - expected) < 1e-2, order      # the reference's criterion (:372)
 
 
-def _heat_study_circle():
-    r, th, ph = 0.27, 0.132, 0.221
-    pp = np.array([r * np.cos(ph) * np.sin(th), r * np.sin(ph) * np.sin(th), r * np.cos(th)])
-    n = np.array([1.1, 1.234, 1.321])
-    n /= np.linalg.norm(n)
-    rho = -pp @ n
-    return [np.sqrt(1 - rho * rho), *(rho * n), *n]
-
-
 @pytest.mark.parametrize("method", [0, 1])
 def test_rd_uniform_state_follows_reaction_ode(method):
     """ReactionDiffusionWithIBC::TimeStep (ReactionDiffusionWithIBC.cpp:79-104) with a
@@ -238,8 +229,10 @@ def test_rd_uniform_state_follows_reaction_ode(method):
     whose stencil is cut by the tube edge keep other values, unread."""
     dx, F, k, nu = 0.2, 0.054, 0.063, (0.2 / 3) ** 2
     dt = 0.9 if method == 0 else 5.0
-    P = orc.IBCProblem("Sphere", dx, ibcs=[orc.IBC("Circle3D", params=_heat_study_circle(), boundary_order=2,
-                                                   boundary_type=(1, 1), on_ibc_tolerance=0.1 * dx * dx)])
+    # the heat study's Neumann curve, the great circle y = 0 (HeatProblemsWithIBC.cpp:102-106)
+    P = orc.IBCProblem("Sphere", dx, ibcs=[orc.IBC("Circle3D", params=[1.0, 0.0, 0.0, 0.0, 0.0, 1.0, 0.0],
+                                                   boundary_order=2, boundary_type=(1, 1),
+                                                   on_ibc_tolerance=0.1 * dx * dx)])
     L, E = P.L(), P.E()
     rowsum = np.bincount(np.repeat(np.arange(P.D), np.diff(L.ptr)), weights=L.val, minlength=P.D)
     complete = np.abs(rowsum) < 1e-8

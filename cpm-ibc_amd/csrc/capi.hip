@@ -71,10 +71,62 @@ __global__ void k_pack(const PackEntry* ent, const int* bent, double* ws64, floa
     }
 }
 
+// ---------------------------------------------------------------- no-alloc scratch
+// Device memory of cpm_solve_device_noalloc: a file-scope array that every
+// GPU running this library holds until the process ends (its own copy at its
+// own address).  Host code never uses the array's name as an address: the
+// address on each device comes from hipGetSymbolAddress, taken once per
+// device when the first context on it is created.
+static __device__ __align__(256) char scratch[CPM_NOALLOC_SCRATCH_BYTES];
+
+namespace {
+struct ScratchAddr {
+    void* ptr = nullptr;
+    std::string err;   // why the address could not be taken (ptr stays null)
+};
+std::mutex g_scratch_mu;
+std::map<int, ScratchAddr> g_scratch;   // device -> address of `scratch` there
+}  // namespace
+
+// With `dev` current: record the device address of `scratch` on it.
+static void scratch_setup(int dev) {
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (g_scratch.count(dev)) return;
+    ScratchAddr a;
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p) {
+        (void)hipGetLastError();
+        a.err = std::string("hipGetSymbolAddress(scratch) failed: ") + hipGetErrorString(e);
+    } else {
+        a.ptr = p;
+    }
+    g_scratch[dev] = a;
+}
+
+static void* scratch_ptr(int dev) {
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    auto it = g_scratch.find(dev);
+    if (it == g_scratch.end()) throw std::logic_error("no-alloc scratch not set up on this device");
+    if (!it->second.ptr) throw HipError(it->second.err);
+    return it->second.ptr;
+}
+
+void* Arena::take(size_t bytes, const char* what) {
+    const size_t n = (bytes + 255) & ~size_t(255);
+    if (n > cap - used)
+        throw ArgError(std::string("no-alloc scratch of ") + std::to_string(cap) + " bytes too small: " + what +
+                       " needs " + std::to_string(n) + " more bytes after " + std::to_string(used));
+    void* p = base + used;
+    used += n;
+    return p;
+}
+
 Ctx::Ctx(int dev) : device(dev) {
     HIPCHK(hipSetDevice(dev));
     HIPCHK(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
     HIPCHK(hipHostMalloc(&host_states, sizeof(KState) * kHostStates));
+    scratch_setup(dev);
 }
 Ctx::~Ctx() {
     (void)hipStreamSynchronize(stream);
@@ -84,6 +136,7 @@ Ctx::~Ctx() {
 }
 
 double* Ctx::workspace(const char* name, size_t bytes) {
+    if (arena) return static_cast<double*>(arena->take(std::max(bytes, (size_t)256), name));
     auto& b = ws[name];
     if (!b || b->bytes < bytes) {
         b.reset();
@@ -286,9 +339,14 @@ static void fill_seg(Seg<T>& s, const Op* op, int64_t off, int block0, int nbloc
 }
 
 // Workspace bytes run_batch<NR> asks the context for, over G groups of
-// `blocks` row blocks and `total` vector elements in all (each vector).
+// `blocks` row blocks and `total` vector elements in all (each vector), and
+// their sum as Arena::take rounds them.
 struct BatchBytes {
     size_t tables, vec64, vec32, pack;
+    size_t arena() const {
+        auto r256 = [](size_t n) { return (n + 255) & ~size_t(255); };
+        return r256(tables) + r256(vec64) + (vec32 ? r256(vec32) : 0) + r256(pack);
+    }
 };
 static BatchBytes batch_bytes(int G, int blocks, int64_t total, int NR, bool refine) {
     auto r256 = [](size_t n) { return (n + 255) & ~size_t(255); };
@@ -321,7 +379,13 @@ static void run_batch(Ctx* ctx, const std::vector<Group>& groups, const std::vec
     const hipStream_t st = ctx->stream;
     const int G = (int)groups.size();
     const bool refine = o.precision == CPM_FP32_REFINE;
-    if (!ctx->no_builds) {
+    if (ctx->arena) {
+        // no-alloc solve: every workspace of this batch is carved from the
+        // scratch (an earlier batch of the same call, e.g. the FP64 re-solve's
+        // first attempt, has already written its results out); shared-stencil
+        // tiles are used only if an earlier call built them
+        ctx->arena->used = 0;
+    } else if (!ctx->no_builds) {
         std::vector<Op*> ops;
         for (auto& g : groups) ops.push_back(g.op);
         ensure_group_tiles_for(ops, refine ? 4 : 8, NR);   // before the views are taken
@@ -1560,6 +1624,35 @@ int cpm_solve_device(cpm_op* op, const double* d_rhs, double* d_x, const cpm_sol
     const double* r[1] = {d_rhs};
     double* x[1] = {d_x};
     return cpm_solve_batch_device(reinterpret_cast<cpm_ctx*>(O(op)->ctx), 1, ops, r, x, opts, info);
+}
+
+int cpm_solve_device_noalloc(cpm_op* op, const double* d_rhs, double* d_x, const cpm_solve_opts* opts,
+                             cpm_solve_info* info) {
+    return guard([&] {
+        if (!op || !d_rhs || !d_x) throw ArgError("null argument");
+        Op* o = O(op);
+        Ctx* cx = o->ctx;
+        const cpm_solve_opts so = resolve_opts(opts);
+        if (so.precision != CPM_FP64 && so.precision != CPM_FP32_REFINE) throw ArgError("unknown precision");
+        const size_t need = std::max(batch_bytes1(o->D, 1, false).arena(), batch_bytes1(o->D, 1, true).arena());
+        if (need > (size_t)CPM_NOALLOC_SCRATCH_BYTES)
+            throw ArgError("operator of " + std::to_string(o->D) + " rows needs " + std::to_string(need) +
+                           " bytes of workspace; the no-alloc scratch holds " +
+                           std::to_string((long long)CPM_NOALLOC_SCRATCH_BYTES));
+        HIPCHK(hipSetDevice(cx->device));
+        Arena a;
+        a.base = static_cast<char*>(scratch_ptr(cx->device));
+        a.cap = (size_t)CPM_NOALLOC_SCRATCH_BYTES;
+        struct Use {   // the context carves from the scratch only inside this call
+            Ctx* c;
+            Use(Ctx* c_, Arena* ar) : c(c_) { c->arena = ar; }
+            ~Use() { c->arena = nullptr; }
+        } use(cx, &a);
+        grp_refresh_override();
+        std::vector<cpm_solve_info> inf;
+        solve_batch(cx, std::vector<Op*>{o}, std::vector<const double*>{d_rhs}, std::vector<double*>{d_x}, so, inf);
+        if (info) *info = inf[0];
+    });
 }
 
 int cpm_solve(cpm_op* op, const double* rhs, double* x, const cpm_solve_opts* opts, cpm_solve_info* info) {

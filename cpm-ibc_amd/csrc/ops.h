@@ -16,6 +16,15 @@ struct NoMem : std::runtime_error { using std::runtime_error::runtime_error; };
 struct GeomError : std::runtime_error { using std::runtime_error::runtime_error; };
 struct HipError : std::runtime_error { using std::runtime_error::runtime_error; };
 
+// Fixed device memory carved front to back: the workspaces of a solve that
+// may not allocate (cpm_solve_device_noalloc) come from here instead of the
+// context's grow-only buffers.
+struct Arena {
+    char* base = nullptr;
+    size_t cap = 0, used = 0;
+    void* take(size_t bytes, const char* what);
+};
+
 struct Ctx {
     static constexpr int kHostStates = 8192;
     int device = 0;
@@ -24,6 +33,7 @@ struct Ctx {
     std::map<std::string, std::unique_ptr<Buffer>> ws;        // grow-only named workspaces
     std::map<void*, int64_t> user_bytes;
     int64_t op_bytes = 0;
+    Arena* arena = nullptr;   // set: workspace() carves from it and never allocates
     bool no_builds = false;   // set: solves build no shared-stencil tiles (they would allocate)
     explicit Ctx(int dev);
     ~Ctx();

@@ -27,7 +27,7 @@ EXPORTS = [
     "cpm_problem_get_info", "cpm_problem_rhs_points", "cpm_problem_subset", "cpm_problem_export",
     "cpm_op_create_from_problem", "cpm_geodesic_distance", "cpm_problem_plot_matrix", "cpm_obj_read",
     "cpm_polyline_chain", "cpm_tube_create_obj", "cpm_server_create", "cpm_server_destroy", "cpm_server_submit",
-    "cpm_server_run", "cpm_problem_curve_rows", "cpm_problem_create_on",
+    "cpm_server_run", "cpm_problem_curve_rows", "cpm_problem_create_on", "cpm_solve_device_noalloc",
 ]
 
 
@@ -118,6 +118,7 @@ def load(path=LIB_PATH):
     L.cpm_multiply_device.argtypes = [vp, vp, vp, ctypes.c_double, ctypes.c_double]
     L.cpm_solve.argtypes = [vp, vp, vp, ctypes.POINTER(SolveOpts), ctypes.POINTER(SolveInfo)]
     L.cpm_solve_device.argtypes = [vp, vp, vp, ctypes.POINTER(SolveOpts), ctypes.POINTER(SolveInfo)]
+    L.cpm_solve_device_noalloc.argtypes = [vp, vp, vp, ctypes.POINTER(SolveOpts), ctypes.POINTER(SolveInfo)]
     L.cpm_solve_batch.argtypes = [vp, ctypes.c_int, vp, vp, vp, ctypes.POINTER(SolveOpts), ctypes.POINTER(SolveInfo)]
     L.cpm_solve_batch_device.argtypes = [vp, ctypes.c_int, vp, vp, vp, ctypes.POINTER(SolveOpts),
                                          ctypes.POINTER(SolveInfo)]
@@ -457,6 +458,11 @@ class Operator:
         _chk(load().cpm_solve_device(self._h, d_rhs, d_x, ctypes.byref(o or opts()), ctypes.byref(info)))
         return info.as_dict()
 
+    def solve_device_noalloc(self, d_rhs, d_x, o=None):
+        """solve_device with its workspaces in the library's fixed device scratch (no allocation)."""
+        info = SolveInfo()
+        _chk(load().cpm_solve_device_noalloc(self._h, d_rhs, d_x, ctypes.byref(o or opts()), ctypes.byref(info)))
+        return info.as_dict()
 
 
 class CSR:

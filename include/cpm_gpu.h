@@ -138,6 +138,24 @@ int cpm_solve(cpm_op* op, const double* rhs, double* x, const cpm_solve_opts* op
 int cpm_solve_device(cpm_op* op, const double* d_rhs, double* d_x, const cpm_solve_opts* opts,
                      cpm_solve_info* info);
 
+/* Solve without device allocation.  No counterpart in the reference; it is
+ * for callers that cannot allocate device memory (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc).  Same as
+ * cpm_solve_device (one right-hand side, device pointers, FP64 or
+ * FP32_REFINE with its FP64 re-solve) with every workspace -- Krylov vectors,
+ * segment and pack tables, states' partial sums -- carved from a fixed device
+ * array of CPM_NOALLOC_SCRATCH_BYTES that the library holds on each GPU it
+ * runs on until the process ends; nothing is allocated or freed.  An operator
+ * needs about 88 B per row (up to ~760,000 rows); larger ones are rejected
+ * with CPM_ERR_ARG before anything is launched.  Shared-stencil tiles are
+ * used only if an earlier solve of the operator built them (results are the
+ * same either way).  Every call on one GPU shares that GPU's array and leaves
+ * its contents behind: calls that overlap on one GPU (other threads, other
+ * contexts) race, and the caller must serialise them. */
+#define CPM_NOALLOC_SCRATCH_BYTES (64ll << 20)
+int cpm_solve_device_noalloc(cpm_op* op, const double* d_rhs, double* d_x, const cpm_solve_opts* opts,
+                             cpm_solve_info* info);
+
 /* Batched solve: n_inst independent systems, instance i solving
  * ops[i] x_i = rhs_i, all advanced together (one launch per kernel covers every
  * instance; converged instances are masked).  rhs/x are host pointer arrays,

@@ -621,6 +621,12 @@ static void tagging_subset(Tube* t, Ctx* ctx, IbcData& I, const double* tube_nor
     double x0[3];
     if (p0) { x0[0] = p0[0]; x0[1] = p0[1]; x0[2] = p0[2]; }
     else if (I.spec.surf.type == "Point") { for (int k = 0; k < 3; ++k) x0[k] = I.spec.surf.params[k]; }
+    else if (I.spec.surf.type == "Arc") {   // paramArc, first point at angle1 (Surface.cpp:1907-1924)
+        const auto& P = I.spec.surf.params;
+        x0[0] = P[0] * std::cos(P[1]);
+        x0[1] = P[0] * std::sin(P[1]);
+        x0[2] = 0.0;
+    }
     else if (I.spec.surf.type == "Circle3D") {   // paramCircle3D, theta = 0 (Surface.cpp:1883-1904)
         const auto& P = I.spec.surf.params;
         double nr[3] = {P[4], P[5], P[6]}, t1[3], t2[3];

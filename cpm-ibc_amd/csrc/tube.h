@@ -12,7 +12,7 @@ namespace cpmgpu {
 
 enum SurfType : int {
     SURF_SPHERE = 1, SURF_HEMISPHERE = 2, SURF_TORUS = 3, SURF_CIRCLE3D = 4, SURF_POINT = 5,
-    SURF_POLYLINE = 6, SURF_ELLIPSOID = 7, SURF_TRIANGULATION = 8
+    SURF_POLYLINE = 6, SURF_ELLIPSOID = 7, SURF_TRIANGULATION = 8, SURF_ARC = 9
 };
 
 // Device description of a surface (src/Surface.cpp:61-190 dispatch, resolved

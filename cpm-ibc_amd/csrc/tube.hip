@@ -252,6 +252,28 @@ __device__ void cp_surface(const SurfDev& s, const double* x, double* c, double&
         }
         break;
     }
+    case SURF_ARC: {   // Surface.cpp:507-579: the arc of radius P[0] about the origin in z = 0
+        // between the angles P[1] < P[2]; points outside the arc's angles go to the nearer end
+        // along the circle (split opposite the arc's midpoint), tagged 1 (first end) or 2
+        const double R = P[0], a1 = P[1], a2 = P[2];
+        const double half = 0.5 * (2 * M_PI - (a2 - a1));
+        const double th = atan2(x[1], x[0]);
+        double ang;
+        if (th > a1 && th < a2) {
+            ang = th;
+        } else if (th <= a1) {
+            bdy = th >= a1 - half ? 1 : 2;
+            ang = bdy == 1 ? a1 : a2;
+        } else {
+            bdy = th < a2 + half ? 2 : 1;
+            ang = bdy == 1 ? a1 : a2;
+        }
+        c[0] = R * cos(ang);
+        c[1] = R * sin(ang);
+        c[2] = 0.0;
+        dist = sqrt(dsq(x[0] - c[0]) + dsq(x[1] - c[1]) + dsq(x[2] - c[2]));
+        break;
+    }
     case SURF_ELLIPSOID: {
         cp_ellipsoid(s.P, x, c);
         dist = sqrt((x[0] - c[0]) * (x[0] - c[0]) + (x[1] - c[1]) * (x[1] - c[1]) + (x[2] - c[2]) * (x[2] - c[2]));
@@ -556,10 +578,12 @@ SurfaceHost surface_from_desc(const char* type, const double* params, int nparam
         else if (s.type == "Torus") { s.params = {2.0, 1.0}; bb0 = -3; bb1 = 3; }
         else if (s.type == "Circle3D") { s.params = {1, 0, 0, 0, 0, 0, 1}; bb0 = -1; bb1 = 1; }
         else if (s.type == "Ellipsoid") { s.params = {1.5, 1.0, 0.75}; bb0 = -1.5; bb1 = 1.5; }
+        else if (s.type == "Arc") { s.params = {1.0, -3 * M_PI_4, M_PI_4}; bb0 = -1; bb1 = 1; }
         else throw ArgError("no default parameters for surface '" + s.type + "'");
     } else {
         s.params.assign(params, params + nparams);
     }
+    s.is_open = s.type == "Hemisphere" || s.type == "Arc";   // SetIsOpen (SurfaceSpecifier.cpp:370-381)
     s.bbox[0] = bb0;
     s.bbox[1] = bb1;
     return s;
@@ -598,6 +622,13 @@ SurfDev make_surfdev(const SurfaceHost& s, const double* dverts, const MeshDev* 
     else if (s.type == "Torus") d.type = SURF_TORUS;
     else if (s.type == "Circle3D") d.type = SURF_CIRCLE3D;
     else if (s.type == "Point") d.type = SURF_POINT;
+    else if (s.type == "Arc") {   // Surface::cpArc's argument checks (Surface.cpp:513-527)
+        if (s.params.size() < 3) throw ArgError("Arc needs radius, angle1, angle2");
+        if (s.params[1] < -M_PI || s.params[2] > M_PI) throw ArgError("Arc angles must lie in (-pi, pi]");
+        if (!(s.params[1] < s.params[2])) throw ArgError("Arc: angle1 must be less than angle2");
+        d.type = SURF_ARC;
+        d.is_open = 1;
+    }
     else if (s.type == "Ellipsoid") {
         if (s.params.size() < 3 || !(s.params[0] > 0 && s.params[1] > 0 && s.params[2] > 0))
             throw ArgError("Ellipsoid needs three positive semi-axes");

@@ -165,6 +165,7 @@ void* orc_problem_poisson_ibc(const char* type, int use_defaults, const double* 
                 is.params.assign(iparams + po, iparams + po + niparams[c]);
                 is.bbox = {bb0, bb1};
                 is.dim = 3;
+                is.is_open = ty == "Arc";   // SetIsOpen (SurfaceSpecifier.cpp:370-381)
             }
             po += (size_t)niparams[c];
             is.meta.boundary_order = border[c];

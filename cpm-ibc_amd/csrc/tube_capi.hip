@@ -271,6 +271,15 @@ int cpm_problem_subset(const cpm_problem* pp, int c, int which, int32_t* set_of_
     });
 }
 
+int cpm_problem_subset_size(const cpm_problem* pp, int c, int which, int64_t* n) {
+    return guard([&] {
+        if (!pp || !n) throw ArgError("null argument");
+        const Problem& P = *PR(pp);
+        if (c < 0 || c >= (int)P.ibc.size()) throw ArgError("no such IBC");
+        *n = which == 0 ? P.ibc[c].nt : P.ibc[c].nd;
+    });
+}
+
 int cpm_problem_curve_rows(const cpm_problem* pp, int c, int64_t* rows) {
     return guard([&] {
         if (!pp || !rows) throw ArgError("null argument");

@@ -27,7 +27,7 @@ EXPORTS = [
     "cpm_problem_get_info", "cpm_problem_rhs_points", "cpm_problem_subset", "cpm_problem_export",
     "cpm_op_create_from_problem", "cpm_geodesic_distance", "cpm_problem_plot_matrix", "cpm_obj_read",
     "cpm_polyline_chain", "cpm_tube_create_obj", "cpm_server_create", "cpm_server_destroy", "cpm_server_submit",
-    "cpm_server_run", "cpm_problem_curve_rows", "cpm_problem_create_on", "cpm_solve_device_noalloc",
+    "cpm_server_run", "cpm_problem_curve_rows", "cpm_problem_create_on", "cpm_problem_subset_size", "cpm_solve_device_noalloc",
 ]
 
 
@@ -170,6 +170,7 @@ def load(path=LIB_PATH):
     L.cpm_problem_get_info.argtypes = [vp, ctypes.POINTER(ProblemInfo)]
     L.cpm_problem_rhs_points.argtypes = [vp, vp, vp]
     L.cpm_problem_subset.argtypes = [vp, ctypes.c_int, ctypes.c_int, vp, vp, vp]
+    L.cpm_problem_subset_size.argtypes = [vp, ctypes.c_int, ctypes.c_int, ctypes.POINTER(ctypes.c_int64)]
     L.cpm_problem_export.argtypes = [vp, vp, vp, vp, vp, vp, vp, vp, ctypes.POINTER(ctypes.c_int64),
                                      ctypes.POINTER(ctypes.c_int64)]
     L.cpm_op_create_from_problem.argtypes = [vp, ctypes.c_double, ctypes.c_double, ctypes.c_int, ctypes.c_int,
@@ -640,9 +641,10 @@ class Problem:
         return tuple(int(v) for v in r)
 
     def subset(self, c=0, which=0):
-        n = self.info["tagging_size0" if which == 0 else "dof_size0"] if c == 0 else None
-        if n is None:
-            raise ValueError("subset sizes are reported for curve 0")
+        """Tagging (which 0) or DOF (1) subset of curve c: tube indices, closest points on the curve, directions."""
+        nn = ctypes.c_int64()
+        _chk(load().cpm_problem_subset_size(self._h, c, which, ctypes.byref(nn)))
+        n = nn.value
         s = np.zeros(n, np.int32)
         cp = np.zeros((n, 3))
         d = np.zeros((n, 3))

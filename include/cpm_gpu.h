@@ -348,6 +348,8 @@ int cpm_problem_rhs_points(const cpm_problem* prob, double* pts, int32_t* kind);
 /* Tagging (which = 0) or DOF (which = 1) subset of curve c in subset order:
  * tube index, closest point on the curve, stencil direction. */
 int cpm_problem_subset(const cpm_problem* prob, int c, int which, int32_t* set_of_sub, double* cpc, double* dirs);
+/* Number of nodes in that subset (IBCSubset::size / IBCDOFSubset::size, src/IBCSubset.h, src/IBCDOFSubset.h). */
+int cpm_problem_subset_size(const cpm_problem* prob, int c, int which, int64_t* n);
 /* Lfull / Efull as CSR (ascending columns, explicit zeros dropped) and the
  * identity rows.  Call with NULL arrays to get nnz_L / nnz_E. */
 int cpm_problem_export(const cpm_problem* prob, int32_t* L_rowptr, int32_t* L_col, double* L_val, int32_t* E_rowptr,

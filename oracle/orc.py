@@ -63,6 +63,8 @@ def lib():
                                               ctypes.c_int]
         L.orc_ibc_info.argtypes = [ctypes.c_void_p, i64p]
         L.orc_ibc_subset.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, i64p, f64p, f64p]
+        L.orc_ibc_subset_size.restype = ctypes.c_int64
+        L.orc_ibc_subset_size.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int]
         L.orc_rhs_points.argtypes = [ctypes.c_void_p, f64p, np.ctypeslib.ndpointer(np.int32, flags="C_CONTIGUOUS")]
         L.orc_heat_run.restype = ctypes.c_int64
         L.orc_heat_run.argtypes = [ctypes.c_int, ctypes.c_double, ctypes.c_int, ctypes.c_int, ctypes.c_int,
@@ -70,6 +72,8 @@ def lib():
         L.orc_rd_run.restype = ctypes.c_int64
         L.orc_rd_run.argtypes = [ctypes.c_int, ctypes.c_double, ctypes.c_int, ctypes.c_int, f64p, ctypes.c_int,
                                  ctypes.c_void_p, ctypes.c_void_p]
+        L.orc_rd_run_problem.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p, ctypes.c_int,
+                                         ctypes.c_void_p, ctypes.c_void_p]
         L.orc_heat_study_order.restype = ctypes.c_double
         L.orc_heat_study_order.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int]
         L.orc_heat_problem_error.restype = ctypes.c_double
@@ -285,7 +289,9 @@ class IBCProblem(Problem):
         self.G, self.K, self.identity_rows_start, self.n_ibc, self.n_tag0, self.n_dof0 = map(int, ii)
 
     def subset(self, c=0, which=0):
-        n = self.n_tag0 if which == 0 else self.n_dof0
+        n = lib().orc_ibc_subset_size(self._h, c, which)
+        if n < 0:
+            raise ValueError(f"no curve {c}")
         s = np.zeros(n, np.int64)
         cp = np.zeros(n * 3)
         d = np.zeros(n * 3)
@@ -309,6 +315,16 @@ class IBCProblem(Problem):
         if lib().orc_geodesic(self._h, dt, sweeps, phi, it) < 0:
             raise RuntimeError(_err())
         return phi, it
+
+    def rd_run(self, method, u0, v0, F, k, nu_u, nu_v, dt, steps):
+        """Gray-Scott steps on this problem from (u0, v0), Dirichlet values 0 (ReactionDiffusionWithIBC::
+        TimeStep for method 0, implicit diffusion for method 1)."""
+        prm = np.array([F, k, nu_u, nu_v, dt], np.float64)
+        u, v = np.array(u0, np.float64), np.array(v0, np.float64)
+        assert len(u) == self.D and len(v) == self.D
+        if lib().orc_rd_run_problem(self._h, method, prm.ctypes.data, steps, u.ctypes.data, v.ctypes.data) < 0:
+            raise RuntimeError(_err())
+        return u, v
 
     def rhs_points(self):
         pts = np.zeros(self.D * 3)

@@ -20,6 +20,8 @@ std::array<int, 4> geodesic_run(const Tube& t, const IBCData& I, const CSR& E, c
                                 const std::vector<uint8_t>& ident, double dt, int sweeps, Vec& phi);
 int64_t rd_run_study(int method, double dx, int order, int btype, const double* prm, int steps, double* u_io,
                      double* v_io);
+void rd_run_problem(const CSR& Lfull, const CSR& Efull, const std::vector<uint8_t>& ident, const IBCData& I,
+                    int method, const double* prm, int steps, double* u_io, double* v_io);
 }
 
 using namespace orc;
@@ -227,6 +229,13 @@ void orc_ibc_info(void* hp, int64_t* ints) {
     ints[5] = I.dof.empty() ? 0 : (int64_t)I.dof[0].set_of_sub.size();
 }
 
+int64_t orc_ibc_subset_size(void* hp, int c, int which) {
+    Handle* h = static_cast<Handle*>(hp);
+    const IBCData& I = h->ibc;
+    if (c < 0 || c >= (int)I.dof.size()) return -1;
+    return which == 0 ? (int64_t)I.tag[(size_t)c].set_of_sub.size() : (int64_t)I.dof[(size_t)c].set_of_sub.size();
+}
+
 // Tagging / DOF subsets of IBC c: tube index of each member (in order), cp_c,
 // directions (3 each).
 void orc_ibc_subset(void* hp, int c, int which, int64_t* set_of_sub, double* cpc, double* dirs) {
@@ -293,6 +302,21 @@ int64_t orc_rd_run(int method, double dx, int order, int btype, const double* pr
                    double* v_io) {
     try {
         return rd_run_study(method, dx, order, btype, prm, steps, u_io, v_io);
+    } catch (const std::exception& e) {
+        g_err = e.what();
+        return -1;
+    }
+}
+
+// Gray-Scott steps (method 0 the reference's explicit step, 1 implicit
+// diffusion) on an IBC problem built by orc_problem_poisson_ibc, Dirichlet
+// values 0; u_io / v_io hold D values each and are updated in place.
+int orc_rd_run_problem(void* hp, int method, const double* prm, int steps, double* u_io, double* v_io) {
+    try {
+        auto* h = static_cast<Handle*>(hp);
+        if (!h->has_ibc) throw std::runtime_error("not an IBC problem");
+        rd_run_problem(h->P.Lfull, h->P.Efull, h->P.ident, h->ibc, method, prm, steps, u_io, v_io);
+        return 0;
     } catch (const std::exception& e) {
         g_err = e.what();
         return -1;

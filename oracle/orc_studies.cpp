@@ -361,8 +361,7 @@ int64_t heat_run_study(int method, double dx, int steps, int order, int btype, d
 //             in the reference; the same Solver(1, -dt nu) as HeatWithIBC.cpp:160-166)
 namespace orc {
 
-static void rd_set_dirichlet(const PoissonIBCProblem& P, Vec& u, double value) {
-    const IBCData& I = P.ibc;
+static void rd_set_dirichlet(const IBCData& I, Vec& u, double value) {
     for (size_t c = 0; c < I.dof.size(); ++c)
         for (size_t i = 0; i < I.dof[c].set_of_sub.size(); ++i) {
             const i64 r = heat_dirichlet_row(I, c, i);
@@ -370,50 +369,57 @@ static void rd_set_dirichlet(const PoissonIBCProblem& P, Vec& u, double value) {
         }
 }
 
-int64_t rd_run_study(int method, double dx, int order, int btype, const double* prm, int steps, double* u_io,
-                     double* v_io) {
-    PoissonIBCProblem P = make_poisson_ibc(SurfaceSpec::defaults("Sphere"), {heat_study_ibc(dx, order, btype)}, dx);
-    const size_t D = (size_t)P.Lfull.rows;
-    if (!u_io) return (int64_t)D;
+// `steps` steps on any IBC problem (Lfull, Efull, identity rows, curves I).
+void rd_run_problem(const CSR& Lfull, const CSR& Efull, const std::vector<uint8_t>& ident, const IBCData& I,
+                    int method, const double* prm, int steps, double* u_io, double* v_io) {
+    const size_t D = (size_t)Lfull.rows;
     const double F = prm[0], k = prm[1], nu_u = prm[2], nu_v = prm[3], dt = prm[4];
     Vec u(u_io, u_io + D), v(v_io, v_io + D), un(D), vn(D), tu(D), tv(D);
     System Su, Sv;
     for (System* S : {&Su, &Sv}) {
         S->m = 1.0;
-        S->L = &P.Lfull;
-        S->E = &P.Efull;
-        S->ident = P.ident;
+        S->L = &Lfull;
+        S->E = &Efull;
+        S->ident = ident;
         S->precond = PRECOND_DIAGONAL;
     }
     Su.n = -dt * nu_u;
     Sv.n = -dt * nu_v;
     for (int s = 0; s < steps; ++s) {
         if (method == 0) {
-            P.Lfull.matvec(u.data(), tu.data());
-            P.Lfull.matvec(v.data(), tv.data());
+            Lfull.matvec(u.data(), tu.data());
+            Lfull.matvec(v.data(), tv.data());
             for (size_t i = 0; i < D; ++i) {
                 tu[i] = u[i] + nu_u * dt * tu[i];          // :86-87
                 tv[i] = v[i] + nu_v * dt * tv[i];
                 tu[i] += dt * (-u[i] * v[i] * v[i] + F * (1.0 - u[i]));   // :89-93
                 tv[i] += dt * (u[i] * v[i] * v[i] - (F + k) * v[i]);
             }
-            P.Efull.matvec(tu.data(), u.data());            // :96-97
-            P.Efull.matvec(tv.data(), v.data());
+            Efull.matvec(tu.data(), u.data());            // :96-97
+            Efull.matvec(tv.data(), v.data());
         } else {
             for (size_t i = 0; i < D; ++i) {
                 tu[i] = u[i] + dt * (-u[i] * v[i] * v[i] + F * (1.0 - u[i]));
                 tv[i] = v[i] + dt * (u[i] * v[i] * v[i] - (F + k) * v[i]);
             }
-            rd_set_dirichlet(P, tu, 0.0);
-            rd_set_dirichlet(P, tv, 0.0);
+            rd_set_dirichlet(I, tu, 0.0);
+            rd_set_dirichlet(I, tv, 0.0);
             bicgstab(Su, tu.data(), u.data(), 1e-10, 100000);
             bicgstab(Sv, tv.data(), v.data(), 1e-10, 100000);
         }
-        rd_set_dirichlet(P, u, 0.0);                         // :99-100
-        rd_set_dirichlet(P, v, 0.0);
+        rd_set_dirichlet(I, u, 0.0);                         // :99-100
+        rd_set_dirichlet(I, v, 0.0);
     }
     std::copy(u.begin(), u.end(), u_io);
     std::copy(v.begin(), v.end(), v_io);
+}
+
+int64_t rd_run_study(int method, double dx, int order, int btype, const double* prm, int steps, double* u_io,
+                     double* v_io) {
+    PoissonIBCProblem P = make_poisson_ibc(SurfaceSpec::defaults("Sphere"), {heat_study_ibc(dx, order, btype)}, dx);
+    const size_t D = (size_t)P.Lfull.rows;
+    if (!u_io) return (int64_t)D;
+    rd_run_problem(P.Lfull, P.Efull, P.ident, P.ibc, method, prm, steps, u_io, v_io);
     return (int64_t)D;
 }
 }  // namespace orc

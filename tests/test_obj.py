@@ -5,11 +5,13 @@ src/SurfaceSpecifier.cpp:298-363).
 CPU tests read files through the library's host-only entry points and check
 them against a reader written here from the OBJ format; GPU tests check that
 a tube or an interior boundary curve built from a file equals the one built
-from the same arrays."""
+from the same arrays, and the mesh tube the oracle's tube of the mesh that
+reader takes from the file."""
 import numpy as np
 import pytest
 
 import cpmgpu
+import orc
 from cpmgpu import workloads
 
 
@@ -115,6 +117,18 @@ def test_mesh_tube_from_obj_equals_arrays(gpu_ctx, tmp_path):
     A, B = a.export(), b.export()
     for k in A:
         assert np.array_equal(A[k], B[k]), k
+    # and against the oracle's tube of the mesh this test's own reader takes from the file
+    Vr, Fr = _independent_read(open(path).read())
+    P = orc.mesh_problem(Vr, Fr, 0.05)
+    To = P.tube()
+    assert a.N == P.N and a.info["xstart"] == P.xstart and a.info["N1d"] == P.N1d
+    for k in ("rec", "nbr", "bdy", "bpt"):
+        assert np.array_equal(A[k], To[k]), k
+    assert np.abs(A["cp"] - To["cp"]).max() <= 1e-15 and np.abs(A["dist"] - To["dist"]).max() <= 1e-15
+    Lg, Lo = a.L(), P.L()
+    assert np.array_equal(Lg.col, Lo.col) and np.array_equal(Lg.val, Lo.val)
+    Eg, Eo = a.E(), P.E()
+    assert np.array_equal(Eg.col, Eo.col) and np.abs(Eg.val - Eo.val).max() <= 1e-15
 
 
 @pytest.mark.gpu

@@ -132,7 +132,7 @@ __global__ void k_grad_tube(int64_t N, const int32_t* nbr, const double* cpx, do
 }
 
 // J = E G (extension of the Jacobian), symmetrised; eigenvector of the
-// smallest eigenvalue = unoriented surface normal (IBCSubset.cpp:325-339).
+// smallest eigenvalue = unoriented surface normal (IBCSubset.cpp:192-226).
 __global__ void k_tube_normals(int64_t N, const int* ecol, const double* eval, int ew, int64_t ld, const double* G,
                                double* normals) {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -289,7 +289,7 @@ __global__ void k_onibc_fix(const int32_t* list, int64_t nl, const int32_t* t_se
 // subset-restricted Jacobian of cp_C (Geometry.cpp:27-69 with the subset
 // maps), binormal b = t x n and direction (b . cp_diff) b; a point curve
 // (largest eigenvalue ~ 0) uses cp_diff minus its normal component
-// (IBCSubset.cpp:359-383).  ecol/eval: E rows at cp_c (tube columns).
+// (IBCSubset.cpp:242-265).  ecol/eval: E rows at cp_c (tube columns).
 __global__ void k_frame(int64_t nt, const int* ecol, const double* eval, int ew, int64_t ld, const double* tube_normals,
                         const int32_t* t_set, const int32_t* t_sub, const int32_t* nbr, const double* t_cpc,
                         const double* cpdiff, double dx, double* gsub, int pass, double* dir, double* bn) {
@@ -616,7 +616,7 @@ static void tagging_subset(Tube* t, Ctx* ctx, IbcData& I, const double* tube_nor
     const hipStream_t st = ctx->stream;
     const int64_t N = t->N;
     const double radius = 2.0 * t->r_t;
-    // FindStartingPoint (:228-232): nearest tube node of the curve's first parameterisation point
+    // FindStartingPoint (IBCSubset.cpp:112-116): nearest tube node of the curve's first parameterisation point
     const double* p0 = I.spec.surf.type == "Polyline" ? I.spec.surf.verts.data() : nullptr;
     double x0[3];
     if (p0) { x0[0] = p0[0]; x0[1] = p0[1]; x0[2] = p0[2]; }

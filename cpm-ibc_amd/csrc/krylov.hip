@@ -2,18 +2,19 @@
 //
 // Restates the reference's right-preconditioned BiCGStab (src/BiCGStab.h:30-108)
 // and its matrix-free operator (src/GeneralizedBiCGStabSystem.h:191-261) as
-// bandwidth-bound kernels; one iteration is
+// bandwidth-bound kernels; one iteration is (B = BiCGStab.h,
+// G = GeneralizedBiCGStabSystem.h)
 //
-//   pupdate   p = (p - w v) beta + r ; y = P^-1 p                 (:268-271)
-//   egather   tmp = E y          (identity rows copy)              (:235-261)
-//   lfinish   v = m y + n (Ld y + Loff tmp) ; partial (r0, v)       (:93-115,:166-189)
-//   fin       alpha = rho / (r0, v)                                (:273)
-//   supdate   s = r - alpha v ; z = P^-1 s                         (:275-277)
+//   pupdate   p = (p - w v) beta + r ; y = P^-1 p                 (B:73-77)
+//   egather   tmp = E y          (identity rows copy)              (G:235-261)
+//   lfinish   v = m y + n (Ld y + Loff tmp) ; partial (r0, v)       (G:93-115,166-189; B:78)
+//   fin       alpha = rho / (r0, v)                                (B:79)
+//   supdate   s = r - alpha v ; z = P^-1 s                         (B:81-83)
 //   egather   tmp = E z
-//   lfinish   t = m z + n (Ld z + Loff tmp) ; partials (t,t),(t,s)  (:278-288)
-//   fin       omega = (t, s) / (t, t)
-//   xrupdate  x += alpha y ; x += w z ; r = s - w t ; (r,r),(r0,r)  (:290-292)
-//   fin       loop test (:253), rho (:255), restart test (:256), beta (:267)
+//   lfinish   t = m z + n (Ld z + Loff tmp) ; partials (t,t),(t,s)  (B:84-86)
+//   fin       omega = (t, s) / (t, t)                              (B:87-94)
+//   xrupdate  x += alpha y ; x += w z ; r = s - w t ; (r,r),(r0,r)  (B:96-98)
+//   fin       loop test (B:59), rho (B:61), restart test (B:62), beta (B:73)
 //
 // Reductions: every workgroup writes its partial sums; a one-workgroup-per-
 // instance "fin" kernel adds them in a fixed order and runs the scalar part
@@ -929,7 +930,7 @@ __global__ __launch_bounds__(kBlock) void k_xrupdate(const Seg<T>* __restrict__ 
     block_partials<2, NR>(pv, sg.part, lb, sg.nblocks);
 }
 
-// FP64 restart, BiCGStab.h:257-264: r = b - A x (A x already in ax), r0 = r; partial (r, r).
+// FP64 restart, BiCGStab.h:62-71: r = b - A x (A x already in ax), r0 = r; partial (r, r).
 template <typename T, int NR>
 __global__ __launch_bounds__(kBlock) void k_restart(const Seg<T>* __restrict__ segs, const int* __restrict__ bseg, const T* b,
                                                     const T* ax, T* r, T* r0) {
@@ -1042,13 +1043,13 @@ template <> struct FinTraits<FIN_MX_START> { static constexpr int NV = 1; static
 template <> struct FinTraits<FIN_MX_INIT> { static constexpr int NV = 1; static constexpr unsigned M = M_RESCALE; };
 template <> struct FinTraits<FIN_ITER_MX> { static constexpr int NV = 2; static constexpr unsigned M = M_STAGE3; };
 
-constexpr double kEps = 2.220446049250313e-16;   // NumTraits<double>::epsilon (BiCGStab.h:250)
+constexpr double kEps = 2.220446049250313e-16;   // NumTraits<double>::epsilon (BiCGStab.h:56)
 
 // FP32 recurrence (re)started from the scaled true residual.
 // The scalar part of the recurrence for one (instance, rhs).
 template <int KIND>
 __device__ void fin_scalar(KState& s, const double* tot) {
-    if (KIND == FIN_INIT) {   // BiCGStab.h:235-252 with x0 = 0: r = r0 = b
+    if (KIND == FIN_INIT) {   // BiCGStab.h:41-58 with x0 = 0: r = r0 = b
         const double bsq = tot[0];
         s.bsq = s.r0sq = s.rr = bsq;
         s.rho_old = s.alpha = s.omega = 1.0;
@@ -1063,23 +1064,23 @@ __device__ void fin_scalar(KState& s, const double* tot) {
         s.status = ST_RUN;
     } else if (KIND == FIN_ALPHA) {
         if (s.status != ST_RUN) { mx_update(s, tot[0]); return; }
-        s.alpha = s.rho / tot[0];                                      // :273
+        s.alpha = s.rho / tot[0];                                      // BiCGStab.h:79
         if (!std::isfinite(s.alpha)) s.status = ST_BREAKDOWN;
     } else if (KIND == FIN_OMEGA) {
         if (s.status != ST_RUN) { mx_init(s, tot[0], ST_READY); return; }
-        s.omega = tot[0] > 0.0 ? tot[1] / tot[0] : 0.0;                // :280-288
+        s.omega = tot[0] > 0.0 ? tot[1] / tot[0] : 0.0;                // BiCGStab.h:86-94
         if (!std::isfinite(s.omega)) s.status = ST_BREAKDOWN;
     } else if (KIND == FIN_ITER64) {
         ++s.iters;
         s.rr = tot[0];
-        if (!(s.rr > s.tol2)) { s.status = std::isnan(s.rr) ? ST_BREAKDOWN : ST_CONVERGED; return; }   // :253
+        if (!(s.rr > s.tol2)) { s.status = std::isnan(s.rr) ? ST_BREAKDOWN : ST_CONVERGED; return; }   // BiCGStab.h:59
         if (s.iters >= s.max_it) { s.status = ST_MAXIT; return; }
         s.rho_old = s.rho;
-        s.rho = tot[1];                                                // :255
-        if (fabs(s.rho) < kEps * kEps * s.r0sq) { s.status = ST_RESTART; return; }   // :256
-        s.beta = (s.rho / s.rho_old) * (s.alpha / s.omega);           // :267
+        s.rho = tot[1];                                                // BiCGStab.h:61
+        if (fabs(s.rho) < kEps * kEps * s.r0sq) { s.status = ST_RESTART; return; }   // BiCGStab.h:62
+        s.beta = (s.rho / s.rho_old) * (s.alpha / s.omega);           // BiCGStab.h:73
         if (!std::isfinite(s.beta)) s.status = ST_BREAKDOWN;
-    } else if (KIND == FIN_RESTART64) {                                // :257-264
+    } else if (KIND == FIN_RESTART64) {                                // BiCGStab.h:62-71
         s.rho = s.r0sq = tot[0];
         if (s.restarts++ == 0) s.iters = 0;
         s.beta = (s.rho / s.rho_old) * (s.alpha / s.omega);
@@ -1110,7 +1111,7 @@ __device__ void fin_scalar(KState& s, const double* tot) {
         if (rr < s.delta2 * s.maxrr) { s.judged = 1; s.status = ST_UPDATE; return; }
         s.rho_old = s.rho;
         s.rho = tot[1];
-        if (fabs(s.rho) < kEps * kEps * s.r0sq) { ++s.restarts; s.status = ST_UPDATE; return; }   // BiCGStab.h:256
+        if (fabs(s.rho) < kEps * kEps * s.r0sq) { ++s.restarts; s.status = ST_UPDATE; return; }   // BiCGStab.h:62
         s.beta = (s.rho / s.rho_old) * (s.alpha / s.omega);
         if (!std::isfinite(s.beta)) { s.status = ST_BREAKDOWN; return; }
         if (rr > s.maxrr) s.maxrr = rr;

@@ -146,7 +146,7 @@ int cpm_tube_export(const cpm_tube* tp, int64_t* rec, double* x, double* cpx, do
         std::vector<int64_t> r(N);
         HIPCHK(hipMemcpy(r.data(), t.rec.p, N * 8, hipMemcpyDeviceToHost));
         if (rec) std::copy(r.begin(), r.end(), rec);
-        if (x)   // Grid::CoordinateFromTriplet (Grid.cpp:292-303)
+        if (x)   // Grid::CoordinateFromTriplet (Grid.cpp:28-38)
             for (size_t i = 0; i < N; ++i) {
                 const int64_t l = r[i], k = l % t.N1d, j = (l / t.N1d) % t.N1d, ii = l / (t.N1d * t.N1d);
                 x[3 * i + 0] = t.xstart + ii * t.dx;

@@ -58,8 +58,8 @@ typedef struct cpm_op cpm_op;
 typedef struct cpm_tube cpm_tube;
 
 typedef struct cpm_solve_opts {
-    double tol;            /* relative residual ||b-Ax|| <= tol ||b||; reference 1e-10 (src/Solver.cpp:164) */
-    int max_iters;         /* reference 100000 (src/Solver.cpp:164)                                         */
+    double tol;            /* relative residual ||b-Ax|| <= tol ||b||; reference 1e-10 (src/Solver.cpp:110) */
+    int max_iters;         /* reference 100000 (src/Solver.cpp:110)                                         */
     int precision;         /* enum cpm_precision                                                            */
     double update_delta;   /* FP32_REFINE: FP32 residual drop that triggers an FP64 refinement step (<=0: 1e-2) */
     int reserved;          /* must be 0                                                                     */
@@ -126,7 +126,7 @@ int cpm_op_set_mn(cpm_op* op, double m, double n);
 int cpm_op_set_precond(cpm_op* op, int precond, int precond_iters);
 
 /* y = A(m, n) x with host buffers of D doubles.
- * Replaces cpm::Solver::Multiply (src/Solver.cpp:176-192). */
+ * Replaces cpm::Solver::Multiply (src/Solver.cpp:122-138). */
 int cpm_multiply(cpm_op* op, const double* x, double* y, double m, double n);
 /* Same with device pointers (D doubles each), on the context stream. */
 int cpm_multiply_device(cpm_op* op, const double* d_x, double* d_y, double m, double n);
@@ -275,7 +275,7 @@ int cpm_tube_create_obj(cpm_ctx* ctx, const char* path, double bbox_lo, double b
 int cpm_tube_destroy(cpm_tube* tube);
 int cpm_tube_get_info(const cpm_tube* tube, cpm_tube_info* info);
 /* Node data in tube order (any pointer may be NULL): linear grid index
- * (Grid.cpp:387-408), coordinates x (3 per node), closest points, distance,
+ * (Grid.cpp:123-144), coordinates x (3 per node), closest points, distance,
  * boundary tag, 6 neighbours (+x,-x,+y,-y,+z,-z; -1 = none, src/Node.h:23),
  * base point (tube index). */
 int cpm_tube_export(const cpm_tube* tube, int64_t* rec, double* x, double* cpx, double* dist, int32_t* bdy,

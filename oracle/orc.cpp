@@ -674,7 +674,7 @@ std::vector<i64> Tube::base_points(const std::vector<Vec>& xq) const {   // Tube
     return b;
 }
 
-i64 Tube::nearest_tube_index(const double* xq) const {   // Tube.h:105-110, Grid.cpp:411-420
+i64 Tube::nearest_tube_index(const double* xq) const {   // Tube.h:105-110, Grid.cpp:147-156
     i64 t[3];
     for (int d = 0; d < dim; ++d) t[d] = (i64)std::round((xq[d] - xstart) / dx);
     return tube_index(lin(t));
@@ -957,7 +957,7 @@ static double vdot(const Vec& a, const Vec& b) {
 }
 
 SolveInfo bicgstab(const System& sys, const double* bp, double* xp, double tol, int max_it) {
-    // BiCGStab.h:30-108.  x starts at zero (Solver.cpp:152).
+    // BiCGStab.h:30-108.  x starts at zero (Solver.cpp:71).
     const size_t D = (size_t)sys.L->rows;
     Vec x(D, 0.0), b(bp, bp + D), r(D), r0(D, 0.0), v(D, 0.0), p(D, 0.0), s(D), t(D), y(D), z(D);
     SolveInfo info;

@@ -111,8 +111,8 @@ struct Tube {
     std::vector<i64> nbr;       // N*6, order +x,-x,+y,-y,+z,-z; -1 = none (Node.h:23)
     std::vector<i64> bpt;       // interpolation base point (tube index)
     i64 N() const { return (i64)rec.size(); }
-    i64 lin(const i64* t) const;                        // Grid.cpp:387-408
-    void trip(i64 l, i64* t) const;                     // Grid.cpp:360-384
+    i64 lin(const i64* t) const;                        // Grid.cpp:123-144
+    void trip(i64 l, i64* t) const;                     // Grid.cpp:96-120
     i64 tube_index(i64 l) const { return grid[(size_t)l]; }
     void base_triplet(const double* xq, i64* t) const;  // Tube.cpp:502-531
     std::vector<i64> base_points(const std::vector<Vec>& xq) const;  // :464-483
@@ -160,11 +160,11 @@ struct SolveInfo { int iters = 0; int restarts = 0; double rel_res = 0; };
 //          enables in GeneralizedBiCGStabSystem.h:71-72, off as shipped)
 struct Parallel { bool vec = false, apply = false; };
 extern Parallel g_parallel;
-// BiCGStab.h:30-108 with Solver.cpp:150-164 (x0 = 0, min 1, max_it).
+// BiCGStab.h:30-108 with Solver.cpp:69-114 (x0 = 0, min 1, max_it).
 SolveInfo bicgstab(const System& sys, const double* b, double* x, double tol, int max_it);
 
 // Explicit operator m I + n (diag L + (L - diag L) E) with identity rows,
-// Helpers.cpp:107-115 + Solver.cpp:176-191.  Independent of System::multiply.
+// Helpers.cpp:107-115 + Solver.cpp:122-138.  Independent of System::multiply.
 CSR explicit_operator(const CSR& L, const CSR& E, double m, double n,
                       const std::vector<uint8_t>& ident, int nD);
 

@@ -150,7 +150,7 @@ static void interp_from_neighbours(const Tube& t, std::vector<V3>& sv, const std
 
 // ---------------------------------------------------------------- subsets
 
-// Whole-tube unoriented surface normals from Jcp (IBCSubset.cpp:325-339).
+// Whole-tube unoriented surface normals from Jcp (IBCSubset.cpp:192-226).
 std::vector<V3> tube_jcp_normals(const Tube& t) {
     std::vector<i64> id((size_t)t.N());
     std::iota(id.begin(), id.end(), 0);

@@ -25,7 +25,7 @@ struct DofSubset {
     CSR E;   // extension rows at cp_c (first order) or cp_c_bar (second order)
 };
 
-// IBCSubsets (src/IBCSubsets.h:43-131).
+// IBCSubsets (src/IBCSubsets.h:18-106).
 struct IBCData {
     std::vector<Surface> surf;
     std::vector<IBCMeta> meta;

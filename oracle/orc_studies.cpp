@@ -51,7 +51,7 @@ static double solve_and_error(const PoissonProblem& P, double m, double n, const
     S.ident = P.ident;
     S.precond = PRECOND_DIAGONAL;
     Vec u(rhs.size());
-    SolveInfo info = bicgstab(S, rhs.data(), u.data(), 1e-10, 100000);   // Solver.cpp:164
+    SolveInfo info = bicgstab(S, rhs.data(), u.data(), 1e-10, 100000);   // Solver.cpp:110
     if (iters) *iters = info.iters;
     return plot_error(P.tube, u, exact);
 }

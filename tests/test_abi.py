@@ -32,7 +32,7 @@ def test_library_exports_declared_symbols():
 
 def test_default_opts_match_reference():
     o = cpmgpu.opts()
-    # src/Solver.cpp:164: tolerance 1e-10, at most 100000 iterations
+    # src/Solver.cpp:110: tolerance 1e-10, at most 100000 iterations
     assert o.tol == 1e-10 and o.max_iters == 100000 and o.precision == cpmgpu.CPM_FP64
 
 

@@ -43,8 +43,10 @@ def sphere_workload(ctx, cpmgpu, batch, seed=0):
 
 def torus_knot_workload(ctx, cpmgpu, batch, seed=0):
     """Laplace on the torus R=1, r=0.4 (h = 0.02, p = 3) with a 2000-segment (2,3) torus-knot polyline
-    carrying 2nd-order Dirichlet data u = sin(3t), t the knot parameter (+ sin(k t + phase) for extra RHS)."""
-    from cpmgpu.workloads import polyline_parameter, torus_knot_polyline
+    carrying 2nd-order Dirichlet data u = sin(3t) (+ sin(k t + phase) for extra RHS), t the knot
+    parameter of the nearest sample t_i = 2 pi i / 2000: Surface::IBCParameter's rule for the
+    reference's parameterised "Torus Knot" (src/Surface.cpp:2737-2751), computed on the GPU."""
+    from cpmgpu.workloads import torus_knot_polyline
     dx = 0.02
     V = torus_knot_polyline(1.0, 0.4, 2000)
     tube = cpmgpu.Tube(ctx, "Torus", dx, params=[1.0, 0.4], bbox=(-1.4, 1.4))
@@ -52,7 +54,7 @@ def torus_knot_workload(ctx, cpmgpu, batch, seed=0):
     op = prob.operator(0.0, -1.0)
     pts, kind = prob.rhs_points()
     idr = np.flatnonzero(kind == 2)
-    t = polyline_parameter(V, pts[idr])
+    t = prob.ibc_parameter(0, pts[idr], vertex_param=2 * np.pi * np.arange(2000) / 2000)
     rng = np.random.default_rng(seed)
     rhs = []
     for k in range(batch):

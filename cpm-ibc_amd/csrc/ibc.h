@@ -62,6 +62,10 @@ Op* op_from_problem(Problem* P, double m, double n, int precond, int precond_ite
 // Plotting rows at nq host points: tube interpolation modified by the curves
 // (AddIBCPlotInterpolation), nq x D CSR with (p+1)^3 entries per row.
 void problem_plot_matrix(Problem* P, const double* xq, int64_t nq, int32_t* rowptr, int32_t* col, double* val);
+// Surface::IBCParameter of n host points (3 each) on curve c: normalised arc
+// length along the curve (vparam null), or the parameter vparam[i] of the
+// nearest of the curve's parameterisation points.
+void problem_ibc_parameter(Problem* P, int c, const double* pts, int64_t n, const double* vparam, double* out);
 // GeodesicDistance::ComputeDistance up to its plotting interpolation: phi on
 // the D rows into d_phi; infos[0] the heat solve, infos[1..3] the Poisson solves.
 void geodesic_distance(Problem* P, double dt, int sweeps, const cpm_solve_opts& o, double* d_phi,

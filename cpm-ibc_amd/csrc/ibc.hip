@@ -16,6 +16,8 @@
 // Everything is computed on the device; the host only sizes buffers.
 #include <hip/hip_runtime.h>
 
+#include <cfloat>
+
 #include <chrono>
 #include <cmath>
 
@@ -1117,6 +1119,173 @@ void problem_plot_matrix(Problem* P, const double* xq, int64_t nq, int32_t* rowp
     HIPCHK(hipStreamSynchronize(st));
     if (h & 1) throw GeomError("a plotting stencil entry crosses more than one IBC (curves too close together)");
     for (int64_t r = 0; r <= nq; ++r) rowptr[r] = (int32_t)(r * W);
+}
+
+// ---------------------------------------------------------------- curve parameter
+// Surface::IBCParameter (src/Surface.cpp:2731-2788) of points on a curve.
+// Parameterisation points of the curve (Surface::GetSurfaceParameterization,
+// :193-366): a polyline's vertices, 100 points of a Circle3D (paramCircle3D,
+// :1883-1904) or of an Arc (paramArc, :1907-1924), computed on the host in the
+// reference's order.
+static std::vector<double> curve_param_points(const SurfaceHost& s) {
+    if (s.type == "Polyline") {
+        // a closed polyline's vertex list holds each vertex once in the reference (its last edge
+        // returns to vertex 0, SurfaceSpecifier.cpp:334-341); the chain here repeats vertex 0
+        std::vector<double> v = s.verts;
+        if (!s.is_open && v.size() >= 6) v.resize(v.size() - 3);
+        return v;
+    }
+    const auto& P = s.params;
+    const int Np = 100;
+    std::vector<double> xp(3 * Np, 0.0);
+    if (s.type == "Circle3D") {
+        double nr[3] = {P[4], P[5], P[6]}, t1[3], t2[3];
+        if (std::fabs(nr[0]) > std::fabs(nr[1])) { t1[0] = -nr[2]; t1[1] = 0; t1[2] = nr[0]; }
+        else { t1[0] = 0; t1[1] = nr[2]; t1[2] = -nr[1]; }
+        t2[0] = nr[1] * t1[2] - nr[2] * t1[1];
+        t2[1] = nr[2] * t1[0] - nr[0] * t1[2];
+        t2[2] = nr[0] * t1[1] - nr[1] * t1[0];
+        const double n1 = std::sqrt(t1[0] * t1[0] + t1[1] * t1[1] + t1[2] * t1[2]);
+        const double n2 = std::sqrt(t2[0] * t2[0] + t2[1] * t2[1] + t2[2] * t2[2]);
+        for (int k = 0; k < 3; ++k) { t1[k] /= n1; t2[k] /= n2; }
+        const double dth = 2 * M_PI / (Np - 1);
+        for (int i = 0; i < Np; ++i) {
+            const double th = i * dth, a = std::cos(th), b = std::sin(th);
+            for (int k = 0; k < 3; ++k) xp[3 * i + k] = P[1 + k] + P[0] * (a * t1[k] + b * t2[k]);
+        }
+    } else if (s.type == "Arc") {
+        const double dth = (P[2] - P[1]) / (Np - 1);
+        for (int i = 0; i < Np; ++i) {
+            const double th = P[1] + i * dth;
+            xp[3 * i] = P[0] * std::cos(th);
+            xp[3 * i + 1] = P[0] * std::sin(th);
+        }
+    } else {
+        throw ArgError("no parameterisation points for IBC curve type '" + s.type + "'");
+    }
+    return xp;
+}
+
+__device__ __forceinline__ double dist3(const double* a, const double* b) {
+    return sqrt((a[0] - b[0]) * (a[0] - b[0]) + (a[1] - b[1]) * (a[1] - b[1]) + (a[2] - b[2]) * (a[2] - b[2]));
+}
+
+// Normalised arc length of the point's closest point on the curve along the
+// parameterisation points (the reference's rule for polylines and curves
+// with parameterisation points): IBCNearestTwoXpIndices (:2661-2728) finds
+// the nearest point xp[mi] to the closest point and the neighbour xp[nn]
+// bounding the edge it lies on; the parameter is the length up to the edge
+// plus the distance along it, over the total length (prefix[k]: length of the
+// first k edges, summed in order).  err: no edge found (the reference throws).
+__global__ void k_ibc_param_arclength(SurfDev sd, const double* xp, int64_t nxp, const double* prefix,
+                                      const double* pts, int64_t n, double* out, int* err) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const double x[3] = {pts[3 * i], pts[3 * i + 1], pts[3 * i + 2]};
+    double c[3], dist;
+    int b;
+    cp_surface(sd, x, c, dist, b);
+    int64_t mi = 0;
+    double md = DBL_MAX;
+    for (int64_t k = 0; k < nxp; ++k) {
+        const double d = dist3(xp + 3 * k, c);
+        if (d < md) { md = d; mi = k; }
+    }
+    int64_t nn;
+    if (mi == 0) {
+        nn = 1;
+    } else if (mi == nxp - 1) {
+        nn = mi - 1;
+    } else if (dist3(c, xp + 3 * mi) < 1e-6) {   // on the point: the shorter edge
+        nn = dist3(xp + 3 * (mi + 1), c) > dist3(xp + 3 * (mi - 1), c) ? mi - 1 : mi + 1;
+    } else {
+        const double* q = xp + 3 * mi;
+        const double a[3] = {c[0] - q[0], c[1] - q[1], c[2] - q[2]};
+        double e[3], t[3];
+        for (int k = 0; k < 3; ++k) e[k] = xp[3 * (mi - 1) + k] - q[k];
+        t[0] = a[1] * e[2] - a[2] * e[1];
+        t[1] = a[2] * e[0] - a[0] * e[2];
+        t[2] = a[0] * e[1] - a[1] * e[0];
+        if (sqrt(t[0] * t[0] + t[1] * t[1] + t[2] * t[2]) < 1e-4) {
+            nn = mi - 1;
+        } else {
+            for (int k = 0; k < 3; ++k) e[k] = xp[3 * (mi + 1) + k] - q[k];
+            t[0] = a[1] * e[2] - a[2] * e[1];
+            t[1] = a[2] * e[0] - a[0] * e[2];
+            t[2] = a[0] * e[1] - a[1] * e[0];
+            if (sqrt(t[0] * t[0] + t[1] * t[1] + t[2] * t[2]) < 1e-4) {
+                nn = mi + 1;
+            } else {
+                atomicOr(err, 1);
+                out[i] = NAN;
+                return;
+            }
+        }
+    }
+    const double to_seg = prefix[mi < nn ? mi : nn];
+    const double par = mi < nn ? to_seg + md : to_seg + dist3(xp + 3 * nn, xp + 3 * mi) - md;
+    out[i] = par / prefix[nxp - 1];
+}
+
+// The nearest parameterisation point's parameter: the reference's rule for
+// curves given by parameterised samples (Dumbbell, Planar Curve, Torus Knot,
+// :2737-2751): the first point at the smallest distance from x itself.
+__global__ void k_ibc_param_vertex(const double* xp, int64_t nxp, const double* vparam, const double* pts, int64_t n,
+                                   double* out) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const double x[3] = {pts[3 * i], pts[3 * i + 1], pts[3 * i + 2]};
+    int64_t mi = 0;
+    double md = DBL_MAX;
+    for (int64_t k = 0; k < nxp; ++k) {
+        const double d = dist3(x, xp + 3 * k);
+        if (d < md) { md = d; mi = k; }
+    }
+    out[i] = vparam[mi];
+}
+
+void problem_ibc_parameter(Problem* P, int c, const double* pts, int64_t n, const double* vparam, double* out) {
+    if (c < 0 || c >= (int)P->ibc.size()) throw ArgError("no such IBC");
+    if (n < 0 || (n > 0 && (!pts || !out))) throw ArgError("bad points");
+    if (n == 0) return;
+    const IbcData& I = P->ibc[(size_t)c];
+    if (I.spec.surf.type == "Point") {   // :2733-2736
+        std::fill(out, out + n, 0.0);
+        return;
+    }
+    const std::vector<double> xp = curve_param_points(I.spec.surf);
+    const int64_t nxp = (int64_t)xp.size() / 3;
+    if (nxp < 2) throw ArgError("a curve parameter needs two parameterisation points");
+    std::vector<double> prefix((size_t)nxp, 0.0);
+    for (int64_t k = 0; k + 1 < nxp; ++k) {   // total length summed edge by edge (:2763-2771)
+        double d2 = 0.0;
+        for (int q = 0; q < 3; ++q) d2 += (xp[3 * k + q] - xp[3 * (k + 1) + q]) * (xp[3 * k + q] - xp[3 * (k + 1) + q]);
+        prefix[(size_t)k + 1] = prefix[(size_t)k] + std::sqrt(d2);
+    }
+    Ctx* ctx = P->ctx;
+    const hipStream_t st = ctx->stream;
+    Buffer dxp(xp.size() * 8), dpre(prefix.size() * 8), dpts((size_t)n * 24), dout((size_t)n * 8), derr(64),
+        dvp(vparam ? (size_t)nxp * 8 : 0);
+    HIPCHK(hipMemcpyAsync(dxp.p, xp.data(), xp.size() * 8, hipMemcpyHostToDevice, st));
+    HIPCHK(hipMemcpyAsync(dpre.p, prefix.data(), prefix.size() * 8, hipMemcpyHostToDevice, st));
+    HIPCHK(hipMemcpyAsync(dpts.p, pts, (size_t)n * 24, hipMemcpyHostToDevice, st));
+    HIPCHK(hipMemsetAsync(derr.p, 0, 4, st));
+    const unsigned nb = (unsigned)((n + 255) / 256);
+    if (vparam) {
+        HIPCHK(hipMemcpyAsync(dvp.p, vparam, (size_t)nxp * 8, hipMemcpyHostToDevice, st));
+        hipLaunchKernelGGL(k_ibc_param_vertex, dim3(nb), dim3(256), 0, st, dxp.as<double>(), nxp, dvp.as<double>(),
+                           dpts.as<double>(), n, dout.as<double>());
+    } else {
+        hipLaunchKernelGGL(k_ibc_param_arclength, dim3(nb), dim3(256), 0, st, I.sd, dxp.as<double>(), nxp,
+                           dpre.as<double>(), dpts.as<double>(), n, dout.as<double>(), derr.as<int>());
+    }
+    HIPCHK(hipGetLastError());
+    int e = 0;
+    HIPCHK(hipMemcpyAsync(out, dout.p, (size_t)n * 8, hipMemcpyDeviceToHost, st));
+    HIPCHK(hipMemcpyAsync(&e, derr.p, 4, hipMemcpyDeviceToHost, st));
+    HIPCHK(hipStreamSynchronize(st));
+    if (e) throw GeomError("IBC parameter: a point's closest point lies on no edge of the curve "
+                           "(Surface::IBCNearestTwoXpIndices)");
 }
 
 }  // namespace cpmgpu

@@ -271,6 +271,15 @@ int cpm_problem_subset(const cpm_problem* pp, int c, int which, int32_t* set_of_
     });
 }
 
+int cpm_problem_ibc_parameter(cpm_problem* pp, int c, const double* pts, int64_t n, const double* vertex_param,
+                              double* out) {
+    return guard([&] {
+        if (!pp) throw ArgError("null problem");
+        HIPCHK(hipSetDevice(PR(pp)->ctx->device));
+        problem_ibc_parameter(PR(pp), c, pts, n, vertex_param, out);
+    });
+}
+
 int cpm_problem_subset_size(const cpm_problem* pp, int c, int which, int64_t* n) {
     return guard([&] {
         if (!pp || !n) throw ArgError("null argument");

@@ -27,7 +27,7 @@ EXPORTS = [
     "cpm_problem_get_info", "cpm_problem_rhs_points", "cpm_problem_subset", "cpm_problem_export",
     "cpm_op_create_from_problem", "cpm_geodesic_distance", "cpm_problem_plot_matrix", "cpm_obj_read",
     "cpm_polyline_chain", "cpm_tube_create_obj", "cpm_server_create", "cpm_server_destroy", "cpm_server_submit",
-    "cpm_server_run", "cpm_problem_curve_rows", "cpm_problem_create_on", "cpm_problem_subset_size", "cpm_solve_device_noalloc",
+    "cpm_server_run", "cpm_problem_curve_rows", "cpm_problem_create_on", "cpm_problem_subset_size", "cpm_problem_ibc_parameter", "cpm_solve_device_noalloc",
 ]
 
 
@@ -171,6 +171,7 @@ def load(path=LIB_PATH):
     L.cpm_problem_rhs_points.argtypes = [vp, vp, vp]
     L.cpm_problem_subset.argtypes = [vp, ctypes.c_int, ctypes.c_int, vp, vp, vp]
     L.cpm_problem_subset_size.argtypes = [vp, ctypes.c_int, ctypes.c_int, ctypes.POINTER(ctypes.c_int64)]
+    L.cpm_problem_ibc_parameter.argtypes = [vp, ctypes.c_int, vp, ctypes.c_int64, vp, vp]
     L.cpm_problem_export.argtypes = [vp, vp, vp, vp, vp, vp, vp, vp, ctypes.POINTER(ctypes.c_int64),
                                      ctypes.POINTER(ctypes.c_int64)]
     L.cpm_op_create_from_problem.argtypes = [vp, ctypes.c_double, ctypes.c_double, ctypes.c_int, ctypes.c_int,
@@ -639,6 +640,18 @@ class Problem:
         r = np.zeros(4, np.int64)
         _chk(load().cpm_problem_curve_rows(self._h, c, _ptr(r)))
         return tuple(int(v) for v in r)
+
+    def ibc_parameter(self, c, pts, vertex_param=None):
+        """Surface::IBCParameter of points on curve c (GPU): normalised arc length, or with vertex_param
+        (one value per polyline vertex) the nearest vertex's value."""
+        P = np.ascontiguousarray(pts, np.float64).reshape(-1, 3)
+        out = np.zeros(len(P))
+        vpp = None
+        if vertex_param is not None:
+            vpa = np.ascontiguousarray(vertex_param, np.float64)
+            vpp = _ptr(vpa)
+        _chk(load().cpm_problem_ibc_parameter(self._h, c, _ptr(P), len(P), vpp, _ptr(out)))
+        return out
 
     def subset(self, c=0, which=0):
         """Tagging (which 0) or DOF (1) subset of curve c: tube indices, closest points on the curve, directions."""

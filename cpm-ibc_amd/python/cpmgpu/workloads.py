@@ -11,25 +11,6 @@ def torus_knot_polyline(R=1.0, r=0.4, segments=2000):
     return V
 
 
-def polyline_parameter(V, pts, t0=0.0, t1=2 * np.pi, chunk=512):
-    """Curve parameter of the closest polyline point: vertex i carries t0 + (t1 - t0) i / (len(V) - 1),
-    linear along each segment (first minimum wins, exact FP64 segment projection)."""
-    A, B = V[:-1], V[1:]
-    AB = B - A
-    L2 = (AB * AB).sum(1)
-    nseg = len(A)
-    out = np.empty(len(pts))
-    for s in range(0, len(pts), chunk):
-        P = pts[s:s + chunk]
-        tt = np.clip(np.einsum("pk,sk->ps", P, AB) - (A * AB).sum(1)[None, :], 0, None) / L2[None, :]
-        tt = np.minimum(tt, 1.0)
-        Q = A[None, :, :] + tt[:, :, None] * AB[None, :, :]
-        d = np.linalg.norm(Q - P[:, None, :], axis=2)
-        i = np.argmin(d, axis=1)
-        out[s:s + chunk] = t0 + (t1 - t0) * (i + tt[np.arange(len(P)), i]) / nseg
-    return out
-
-
 def icosphere(subdivisions=0, radius=1.0):
     """Unit icosahedron refined `subdivisions` times by edge midpoints projected to the sphere:
     20 * 4**subdivisions triangles, outward (counter-clockwise) orientation."""

@@ -348,6 +348,20 @@ int cpm_problem_rhs_points(const cpm_problem* prob, double* pts, int32_t* kind);
 /* Tagging (which = 0) or DOF (which = 1) subset of curve c in subset order:
  * tube index, closest point on the curve, stencil direction. */
 int cpm_problem_subset(const cpm_problem* prob, int c, int which, int32_t* set_of_sub, double* cpc, double* dirs);
+/* Curve parameter of n points (3 doubles each, host memory) on curve c:
+ * Surface::IBCParameter (src/Surface.cpp:2731-2788), reached through
+ * PoissonWithIBC::IBCParameter / IBCSubsets::IBCParameter (src/IBCSubsets.h:43-46)
+ * for boundary data such as u = sin(t) (examples/LaplaceTorusTorusKnot/
+ * LaplaceTorusTorusKnot.cpp:35).  vertex_param NULL: the normalised arc length
+ * in [0, 1] of the point's closest point along the curve's parameterisation
+ * points (a polyline's vertices; 100 points of a Circle3D or an Arc), the
+ * reference's rule for polylines; 0 for a Point.  vertex_param non-NULL (one
+ * value per polyline vertex): the value of the vertex nearest to the point,
+ * the reference's rule for curves given by parameterised samples (its
+ * "Torus Knot", Surface.cpp:2737-2751).  Computed on the GPU.
+ * CPM_ERR_GEOMETRY if a closest point lies on no edge (the reference throws). */
+int cpm_problem_ibc_parameter(cpm_problem* prob, int c, const double* pts, int64_t n, const double* vertex_param,
+                              double* out);
 /* Number of nodes in that subset (IBCSubset::size / IBCDOFSubset::size, src/IBCSubset.h, src/IBCDOFSubset.h). */
 int cpm_problem_subset_size(const cpm_problem* prob, int c, int which, int64_t* n);
 /* Lfull / Efull as CSR (ascending columns, explicit zeros dropped) and the

@@ -588,20 +588,20 @@ void Surface::cpbar(const double* x, double* cpx, double& dist) const {   // :39
     cp(proj, cpx, dist, bdy);
 }
 
-std::vector<i64> Surface::nearest_two_xp(const double* x, double& min_dist) const {   // :2661-2728
+std::vector<i64> Surface::nearest_two_xp(const double* x, double& min_dist, size_t np) const {   // :2661-2728
     const int D = s_.dim;
     double cpx[3], dist;
     i64 bdy;
     cp(x, cpx, dist, bdy);
     size_t mi = 0;
     min_dist = std::numeric_limits<double>::max();
-    for (size_t i = 0; i < xp.size(); ++i) {
+    for (size_t i = 0; i < np; ++i) {
         double cd = distance(xp[i].data(), cpx, D);
         if (cd < min_dist) { min_dist = cd; mi = i; }
     }
     size_t nn;
     if (mi == 0) nn = 1;
-    else if (mi == xp.size() - 1) nn = mi - 1;
+    else if (mi == np - 1) nn = mi - 1;
     else if (distance(cpx, xp[mi].data(), D) < 1e-6) {
         nn = distance(xp[mi + 1].data(), cpx, D) > distance(xp[mi - 1].data(), cpx, D) ? mi - 1 : mi + 1;
     } else {
@@ -622,16 +622,29 @@ std::vector<i64> Surface::nearest_two_xp(const double* x, double& min_dist) cons
     return {(i64)mi, (i64)nn};
 }
 
-double Surface::ibc_parameter(const double* x) const {   // :2731-2788
+double Surface::ibc_parameter(const double* x, const Vec* vparam) const {   // :2731-2788
     const std::string& ty = s_.type;
     if (ty == "Point") return 0.0;
+    // a closed polyline's vertex list holds each vertex once in the reference (its last edge
+    // returns to vertex 0); the vertex chain here repeats vertex 0 at the end
+    const size_t np = (ty == "Polyline" && !s_.is_open && xp.size() > 2) ? xp.size() - 1 : xp.size();
+    if (vparam) {                                  // :2737-2751: nearest parameterisation point to x
+        if (vparam->size() != np) throw std::runtime_error("one parameter per parameterisation point");
+        double md = std::numeric_limits<double>::max();
+        size_t mi = 0;
+        for (size_t i = 0; i < np; ++i) {
+            const double d = distance(x, xp[i].data(), s_.dim);
+            if (d < md) { md = d; mi = i; }
+        }
+        return (*vparam)[mi];
+    }
     if (ty == "Circle") return std::atan2(x[1], x[0]);
     const int D = s_.dim;
     double min_dist;
-    auto ni = nearest_two_xp(x, min_dist);
+    auto ni = nearest_two_xp(x, min_dist, np);
     double total = 0.0, to_seg = 0.0;
     i64 lo = std::min(ni[0], ni[1]);
-    for (size_t i = 0; i + 1 < xp.size(); ++i) {
+    for (size_t i = 0; i + 1 < np; ++i) {
         if ((i64)i == lo) to_seg = total;
         total += distance(xp[i].data(), xp[i + 1].data(), D);
     }

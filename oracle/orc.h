@@ -82,7 +82,9 @@ public:
     explicit Surface(const SurfaceSpec& s);
     void cp(const double* x, double* cpx, double& dist, i64& bdy) const;   // :61-190
     void cpbar(const double* x, double* cpx, double& dist) const;          // :399-409
-    double ibc_parameter(const double* x) const;                            // :2731-2788
+    // :2731-2788; with vparam (one value per parameterisation point), the
+    // nearest point's value (the rule for parameterised samples, :2737-2751)
+    double ibc_parameter(const double* x, const Vec* vparam = nullptr) const;
     const SurfaceSpec& spec() const { return s_; }
     int dim() const { return s_.dim; }
     std::vector<Vec> xp;        // parameterisation points (m_mesh.vertices)
@@ -93,7 +95,7 @@ private:
     void cp_polyline(const double* x, double* cpx, double& dist, i64& bdy) const;
     void cp_arc(const double* x, double* cpx, double& dist, i64& bdy) const;
     void cp_triangles(const double* x, double* cpx, double& dist) const;
-    std::vector<i64> nearest_two_xp(const double* x, double& min_dist) const;  // :2661-2728
+    std::vector<i64> nearest_two_xp(const double* x, double& min_dist, size_t np) const;  // :2661-2728
 };
 
 // Tube + Grid + Node, src/Tube.cpp, src/Grid.cpp, src/Node.h.

@@ -63,6 +63,8 @@ def lib():
                                               ctypes.c_int]
         L.orc_ibc_info.argtypes = [ctypes.c_void_p, i64p]
         L.orc_ibc_subset.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, i64p, f64p, f64p]
+        L.orc_ibc_parameter.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p, ctypes.c_int64,
+                                        ctypes.c_void_p, ctypes.c_int64, ctypes.c_void_p]
         L.orc_ibc_subset_size.restype = ctypes.c_int64
         L.orc_ibc_subset_size.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int]
         L.orc_rhs_points.argtypes = [ctypes.c_void_p, f64p, np.ctypeslib.ndpointer(np.int32, flags="C_CONTIGUOUS")]
@@ -315,6 +317,16 @@ class IBCProblem(Problem):
         if lib().orc_geodesic(self._h, dt, sweeps, phi, it) < 0:
             raise RuntimeError(_err())
         return phi, it
+
+    def ibc_parameter(self, c, pts, vertex_param=None):
+        """Surface::IBCParameter (src/Surface.cpp:2731-2788) of points on curve c."""
+        P = np.ascontiguousarray(pts, np.float64).reshape(-1, 3)
+        out = np.zeros(len(P))
+        vp = None if vertex_param is None else np.ascontiguousarray(vertex_param, np.float64)
+        if lib().orc_ibc_parameter(self._h, c, P.ctypes.data, len(P), None if vp is None else vp.ctypes.data,
+                                   0 if vp is None else len(vp), out.ctypes.data) < 0:
+            raise RuntimeError(_err())
+        return out
 
     def rd_run(self, method, u0, v0, F, k, nu_u, nu_v, dt, steps):
         """Gray-Scott steps on this problem from (u0, v0), Dirichlet values 0 (ReactionDiffusionWithIBC::

@@ -229,6 +229,23 @@ void orc_ibc_info(void* hp, int64_t* ints) {
     ints[5] = I.dof.empty() ? 0 : (int64_t)I.dof[0].set_of_sub.size();
 }
 
+// Surface::IBCParameter of n points on IBC c (vparam: one value per
+// parameterisation point, or null for the arc-length rule).
+int orc_ibc_parameter(void* hp, int c, const double* pts, int64_t n, const double* vparam, int64_t nv, double* out) {
+    try {
+        Handle* h = static_cast<Handle*>(hp);
+        const IBCData& I = h->ibc;
+        if (c < 0 || c >= (int)I.surf.size()) throw std::runtime_error("no such IBC");
+        Vec vp;
+        if (vparam) vp.assign(vparam, vparam + nv);
+        for (int64_t i = 0; i < n; ++i) out[i] = I.surf[(size_t)c].ibc_parameter(pts + 3 * i, vparam ? &vp : nullptr);
+        return 0;
+    } catch (const std::exception& e) {
+        g_err = e.what();
+        return -1;
+    }
+}
+
 int64_t orc_ibc_subset_size(void* hp, int c, int which) {
     Handle* h = static_cast<Handle*>(hp);
     const IBCData& I = h->ibc;

@@ -6,7 +6,7 @@ import pytest
 
 import cpmgpu
 import orc
-from cpmgpu.workloads import polyline_parameter, torus_knot_polyline
+from cpmgpu.workloads import torus_knot_polyline
 
 pytestmark = pytest.mark.gpu
 
@@ -169,7 +169,7 @@ def test_torus_knot_system(gpu_ctx):
     pts, kind = Q.rhs_points()
     b = np.zeros(Q.D)
     idr = np.flatnonzero(kind == 2)
-    b[idr] = np.sin(3 * polyline_parameter(V, pts[idr]))
+    b[idr] = np.sin(3 * Q.ibc_parameter(0, pts[idr], vertex_param=2 * np.pi * np.arange(2000) / 2000))
     op = Q.operator(0.0, -1.0)
     x, inf = op.solve(b, cpmgpu.opts(tol=1e-12, max_iters=20000))
     assert inf["status"] == "converged"
@@ -367,3 +367,38 @@ def test_open_curves(gpu_ctx, name, kw):
     at_end = (np.linalg.norm(cp[:, None, :] - ends[None], axis=2) < 1e-12).any(1)
     print(f"tagging nodes at a curve end: {at_end.sum()} of {len(cp)}", flush=True)
     assert at_end.sum() > 0
+
+
+def test_ibc_parameter_matches_oracle(gpu_ctx):
+    """Surface::IBCParameter (src/Surface.cpp:2731-2788) on the GPU against the oracle: normalised arc
+    length along a closed square polyline, a torus-knot polyline, a Circle3D and an Arc, at the
+    closest points of their DOF nodes (and, for the knot, at the Dirichlet rows' points), and the
+    nearest-vertex rule with the knot parameter t_i = 2 pi i / 2000 as vertex values (the reference's
+    rule for its parameterised "Torus Knot")."""
+    print("case ibc parameter", flush=True)
+    a, dx = 0.7, 0.1
+    sq = np.array([[a, a, 0], [-a, a, 0], [-a, -a, 0], [a, -a, 0], [a, a, 0]], float)
+    V = torus_knot_polyline(1.0, 0.4, 2000)
+    t = 2 * np.pi * np.arange(2000) / 2000
+    cases = [("Sphere", 0.2, None, None, dict(type="Polyline", verts=sq), None),   # a chord square: coarse tube
+             ("Sphere", 0.2, None, None, dict(type="Polyline", verts=sq), [0.0, 1.0, 2.0, 3.0]),
+             ("Sphere", dx, None, None, dict(type="Circle3D", params=circle3d_params(2)), None),
+             ("Sphere", dx, None, None, dict(type="Arc", params=_sphere_arc(1)), None),
+             ("Torus", 0.05, [1.0, 0.4], (-1.4, 1.4), dict(type="Polyline", verts=V), None),
+             ("Torus", 0.05, [1.0, 0.4], (-1.4, 1.4), dict(type="Polyline", verts=V), t)]
+    for surface, h, params, bbox, kw, vparam in cases:
+        kw = dict(kw, boundary_order=2, on_ibc_tolerance=0.1 * h * h)
+        P = orc.IBCProblem(surface, h, params=params, bbox=bbox, ibcs=[orc.IBC(**kw)])
+        tg = cpmgpu.Tube(gpu_ctx, surface, h, params=params, bbox=bbox)
+        Q = cpmgpu.Problem(tg, [cpmgpu.IBC(**kw)])
+        _, cpc, _ = P.subset(0, 1)
+        pts, kind = Q.rhs_points()
+        x = np.concatenate([cpc, pts[kind == 2]])
+        po = P.ibc_parameter(0, x, vparam)
+        pg = Q.ibc_parameter(0, x, vparam)
+        err = np.abs(pg - po).max()
+        print(f"{surface} {kw['type']} {'vertex values' if vparam is not None else 'arc length'}: "
+              f"{len(x)} points, range [{po.min():.4f}, {po.max():.4f}], max |gpu - oracle| {err:.2e}", flush=True)
+        assert err <= 1e-12
+        if vparam is None:
+            assert po.min() >= -1e-12 and po.max() <= 1 + 1e-12

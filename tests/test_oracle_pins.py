@@ -383,3 +383,33 @@ def test_geodesic_distance_sphere_point(parallel_oracle):
     assert it1[0] >= 1 and all(k > 0 for k in it1[1:])
     assert emax1 < 0.1 and emean1 < 0.035, (emax1, emean1)
     assert emean1 < 0.85 * emean2 and emax1 < 0.8 * emax2, (emean1, emean2, emax1, emax2)
+
+
+def test_ibc_parameter_closed_forms():
+    """Surface::IBCParameter (src/Surface.cpp:2731-2788) on a square polyline, against closed forms:
+    normalised arc length along the vertex list, which for a closed polyline holds each vertex once,
+    so the closing edge is not counted (the total is 3 edges) and a point on it takes the parameter
+    of its nearer end's edge; and the nearest vertex's value for parameterised samples."""
+    a = 0.7
+    sq = np.array([[a, a, 0], [-a, a, 0], [-a, -a, 0], [a, -a, 0], [a, a, 0]], float)
+    P = orc.IBCProblem("Sphere", 0.2, ibcs=[orc.IBC("Polyline", verts=sq, boundary_order=2,
+                                                    on_ibc_tolerance=0.004)])
+    e = 2 * a
+    s = np.array([0.1, 0.3, 0.55, 0.9, 1.25])
+    pts = np.concatenate([np.stack([a - s, np.full_like(s, a), 0 * s], 1),        # edge 0 -> 1
+                          np.stack([np.full_like(s, -a), a - s, 0 * s], 1),       # edge 1 -> 2
+                          np.stack([s - a, np.full_like(s, -a), 0 * s], 1)])      # edge 2 -> 3
+    want = np.concatenate([s, e + s, 2 * e + s]) / (3 * e)
+    got = P.ibc_parameter(0, pts)
+    assert np.abs(got - want).max() < 1e-14, (got, want)
+    # on the vertices: the shorter-edge rule picks the following edge (equal edges)
+    got = P.ibc_parameter(0, sq[1:3])
+    assert np.abs(got - np.array([1, 2]) / 3).max() < 1e-14
+    # the closing edge (vertex 3 -> vertex 0): nearer vertex 3, the last, counts back along edge 2 -> 3
+    # from the end; nearer vertex 0, the first, counts its distance along edge 0 -> 1
+    c = np.array([[a, -a + 0.2, 0], [a, a - 0.3, 0]])
+    got = P.ibc_parameter(0, c)
+    assert np.abs(got - np.array([3 * e - 0.2, 0.3]) / (3 * e)).max() < 1e-14, got
+    # parameterised samples: the nearest vertex's value
+    got = P.ibc_parameter(0, [[a - 0.2, a, 0], [-a, -a + 0.65, 0], [a, -a + 0.2, 0]], vertex_param=[0.0, 1.5, 2.5, 3.5])
+    assert np.array_equal(got, [0.0, 2.5, 3.5])

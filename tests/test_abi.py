@@ -37,13 +37,15 @@ def test_default_opts_match_reference():
 
 
 def test_cpp_facade_compiles(tmp_path):
-    """The C++ cpm::Solver facade and the drop-in test program compile against the C ABI header."""
+    """The C++ facade (cpm::Solver and the problem classes) and the drop-in test programs compile
+    against the C ABI header."""
     import shutil
     import subprocess
     if shutil.which("g++") is None:
         pytest.skip("g++ not available")
-    src = os.path.join(REPO, "tests", "cpp", "solver_dropin.cpp")
-    out = subprocess.run(["g++", "-std=c++17", "-fsyntax-only", "-Wall", "-Werror",
-                          "-I", os.path.join(REPO, "include"), "-I", os.path.join(REPO, "cpm-ibc_amd", "host"), src],
-                         capture_output=True, text=True, timeout=120)
-    assert out.returncode == 0, out.stderr
+    for prog in ("solver_dropin", "problems_dropin"):
+        src = os.path.join(REPO, "tests", "cpp", prog + ".cpp")
+        out = subprocess.run(["g++", "-std=c++17", "-fsyntax-only", "-Wall", "-Werror", "-I",
+                              os.path.join(REPO, "include"), "-I", os.path.join(REPO, "cpm-ibc_amd", "host"), src],
+                             capture_output=True, text=True, timeout=120)
+        assert out.returncode == 0, (prog, out.stderr)

@@ -681,6 +681,15 @@ static void run_stream(Ctx* ctx, Op* op, const std::vector<const double*>& rhs, 
 // FP32_REFINE systems whose refinement stopped reducing the true residual
 // (ST_FALLBACK) are solved again in FP64 from zero, the reference algorithm,
 // in one batch; their infos keep the FP32 counts and add fallback_iters.
+// FP32+refine solves of an operator that once fell back to FP64 (its FP32
+// refinement stalled: the system is past what the FP32 recurrence resolves)
+// start in FP64, instead of spending FP32 iterations first
+// (CPMGPU_FP64_ROUTING=0: always try FP32 first).
+static bool fp64_routing() {   // read per solve call (not per launch)
+    const char* e = std::getenv("CPMGPU_FP64_ROUTING");
+    return !(e && std::string(e) == "0");
+}
+
 static void fp64_fallback(Ctx* ctx, const std::vector<Op*>& ops, const std::vector<const double*>& rhs,
                           const std::vector<double*>& xs, const cpm_solve_opts& o,
                           std::vector<cpm_solve_info>& infos, const std::vector<std::pair<double, double>>* mn) {
@@ -698,6 +707,7 @@ static void fp64_fallback(Ctx* ctx, const std::vector<Op*>& ops, const std::vect
         fx.push_back(xs[(size_t)i]);
         fmn.push_back(mn ? (*mn)[(size_t)i] : std::make_pair(ops[(size_t)i]->m, ops[(size_t)i]->n));
     }
+    for (Op* op : fo) op->fp32_failed = true;   // later FP32+refine solves of op run in FP64 (fp64_routing)
     cpm_solve_opts o64 = o;
     o64.precision = CPM_FP64;
     std::vector<cpm_solve_info> fi;
@@ -724,7 +734,42 @@ void solve_batch(Ctx* ctx, const std::vector<Op*>& ops, const std::vector<const 
         if (!op || op->ctx->device != ctx->device) throw ArgError("operator lives on another device");
         if (op->precond == CPM_PRECOND_JACOBI && op->precond_iters > 100000) throw ArgError("too many Jacobi sweeps");
     }
-    // group instances that share an operator and its (m, n) (at most kMaxNR per group)
+    if (o.precision == CPM_FP32_REFINE && fp64_routing()) {
+        // operators whose FP32+refine solves fell back to FP64 before: FP64 from the start
+        std::vector<int> r64, r32;
+        for (int i = 0; i < n; ++i) (ops[i]->fp32_failed ? r64 : r32).push_back(i);
+        if (!r64.empty()) {
+            cpm_solve_opts o64 = o;
+            o64.precision = CPM_FP64;
+            for (int pass = 0; pass < 2; ++pass) {
+                const std::vector<int>& sel = pass == 0 ? r64 : r32;
+                if (sel.empty()) continue;
+                std::vector<Op*> so;
+                std::vector<const double*> sr;
+                std::vector<double*> sx;
+                std::vector<std::pair<double, double>> smn;
+                for (int i : sel) {
+                    so.push_back(ops[i]);
+                    sr.push_back(rhs[(size_t)i]);
+                    sx.push_back(xs[(size_t)i]);
+                    smn.push_back(mn ? (*mn)[(size_t)i] : std::make_pair(ops[i]->m, ops[i]->n));
+                }
+                std::vector<cpm_solve_info> si;
+                solve_batch(ctx, so, sr, sx, pass == 0 ? o64 : o, si, &smn);
+                for (size_t j = 0; j < sel.size(); ++j) {
+                    cpm_solve_info& inf = infos[(size_t)sel[j]];
+                    inf = si[j];
+                    if (pass == 0) {   // reported as an FP64 re-solve without FP32 iterations
+                        inf.fallback_iters = si[j].iters;
+                        inf.iters = 0;
+                    }
+                }
+            }
+            return;
+        }
+    }
+    // group instances that share an operator and its (m, n): at most kMaxNR per group, at most 4
+    // in FP64 where the operator has LDS tiles for FP64 x 4 (FP64 x 8 rows do not fit them)
     std::vector<Group> groups;
     std::map<std::tuple<Op*, double, double>, int> open;
     for (int i = 0; i < n; ++i) {
@@ -732,7 +777,9 @@ void solve_batch(Ctx* ctx, const std::vector<Op*>& ops, const std::vector<const 
         const double nn = mn ? (*mn)[(size_t)i].second : ops[i]->n;
         const auto key = std::make_tuple(ops[i], m, nn);
         auto it = open.find(key);
-        if (it == open.end() || groups[it->second].inst.size() == (size_t)kMaxNR) {
+        const size_t cap = (o.precision == CPM_FP64 && ops[i]->fmt == OP_TP && (ops[i]->ngt > 0 || ops[i]->tln.p))
+                               ? 4 : (size_t)kMaxNR;
+        if (it == open.end() || groups[it->second].inst.size() == cap) {
             groups.push_back(Group{ops[i], m, nn, {}});
             open[key] = (int)groups.size() - 1;
             it = open.find(key);
@@ -755,6 +802,10 @@ static void solve_stream(Op* op, const std::vector<const double*>& rhs, const st
     if (o.precision != CPM_FP32_REFINE) throw ArgError("streamed solves run in FP32_REFINE precision");
     if (op->nD != 0) throw ArgError("streamed solves need an operator without trailing nD rows");
     if (op->precond == CPM_PRECOND_JACOBI && op->precond_iters > 100000) throw ArgError("too many Jacobi sweeps");
+    if (op->fp32_failed && fp64_routing()) {   // see solve_batch
+        solve_batch(op->ctx, std::vector<Op*>(rhs.size(), op), rhs, xs, o, infos, nullptr);
+        return;
+    }
     if (lanes == 1) run_stream<1>(op->ctx, op, rhs, xs, o, infos);
     else if (lanes == 2) run_stream<2>(op->ctx, op, rhs, xs, o, infos);
     else if (lanes == 4) run_stream<4>(op->ctx, op, rhs, xs, o, infos);

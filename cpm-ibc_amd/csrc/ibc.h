@@ -20,6 +20,7 @@ struct IbcSpec {
 struct IbcData {
     IbcSpec spec;
     Buffer verts;
+    MeshBvh bvh;                // polylines: hierarchy over the segments (and the vertices)
     SurfDev sd{};
     int64_t nt = 0, nd = 0;     // tagging / DOF subset sizes
     Buffer t_sub;               // int32 [N]: tagging index of a tube node or -1

@@ -794,11 +794,17 @@ Problem* problem_build(Tube* t, const std::vector<IbcSpec>& specs, Ctx* bctx) {
         IbcData& I = P->ibc[c];
         I.spec = specs[c];
         if (I.spec.order != 1 && I.spec.order != 2) throw ArgError("boundary_order must be 1 or 2");
-        if (!I.spec.surf.verts.empty()) {
-            realloc<double>(I.verts, I.spec.surf.verts.size());
-            HIPCHK(hipMemcpy(I.verts.p, I.spec.surf.verts.data(), I.spec.surf.verts.size() * 8, hipMemcpyHostToDevice));
+        if (I.spec.surf.type == "Polyline") {   // segment hierarchy (its vertex copy serves the parameter)
+            segment_build(I.bvh, I.spec.surf.verts);
+            const MeshDev md = I.bvh.dev();
+            I.sd = make_surfdev(I.spec.surf, I.bvh.v.as<double>(), &md);
+        } else {
+            if (!I.spec.surf.verts.empty()) {
+                realloc<double>(I.verts, I.spec.surf.verts.size());
+                HIPCHK(hipMemcpy(I.verts.p, I.spec.surf.verts.data(), I.spec.surf.verts.size() * 8, hipMemcpyHostToDevice));
+            }
+            I.sd = make_surfdev(I.spec.surf, I.verts.as<double>());
         }
-        I.sd = make_surfdev(I.spec.surf, I.verts.as<double>());
         tagging_subset(t, ctx, I, P->normals.as<double>());
     }
     for (auto& I : P->ibc) dof_subset(t, ctx, I);

@@ -1,5 +1,6 @@
-// Triangle-mesh closest points on the GPU: a bounding-volume hierarchy built
-// on the host (binned surface-area heuristic) and a per-query FP64 traversal.
+// Triangle-mesh and polyline closest points on the GPU: a bounding-volume
+// hierarchy built on the host (binned surface-area heuristic) and a per-query
+// FP64 traversal.
 // The reference queries fcpw's SAH BVH (src/Surface.cpp:1751-1779,2537-2592,
 // FP32 inside fcpw); here every distance is FP64 and the result is the exact
 // minimiser of the computed distance, lowest triangle index on ties, i.e. the
@@ -40,8 +41,13 @@ struct MeshBvh {
 
 // Validates the mesh, builds the hierarchy and uploads everything.
 void mesh_build(MeshBvh& m, const std::vector<double>& verts, const std::vector<int>& faces);
+// The same over the segments of a polyline (vertices k, k + 1; f stays empty).
+void segment_build(MeshBvh& m, const std::vector<double>& verts);
 
 // Closest point of x on the mesh and its distance.
 __device__ void cp_mesh(const MeshDev& m, const double* x, double* c, double& dist);
+// Closest point of x on the polyline (segment projections, the oracle's
+// formula; lowest segment index among equal distances) and its distance.
+__device__ void cp_polyline(const MeshDev& m, const double* x, double* c, double& dist);
 
 }  // namespace cpmgpu

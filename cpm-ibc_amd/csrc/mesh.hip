@@ -44,33 +44,16 @@ struct Box {
     }
 };
 
-}  // namespace
-
-void mesh_build(MeshBvh& m, const std::vector<double>& verts, const std::vector<int>& faces) {
-    if (verts.size() % 3 || faces.size() % 3) throw ArgError("mesh arrays must hold 3 values per vertex / face");
-    const int64_t nv = (int64_t)verts.size() / 3, nf = (int64_t)faces.size() / 3;
-    if (nv < 3 || nf < 1) throw ArgError("mesh needs at least one triangle");
-    if (nf > (1 << 28)) throw ArgError("too many triangles");
-    for (int x : faces)
-        if (x < 0 || x >= nv) throw ArgError("face index out of range");
-    double scale = 0;
-    for (double x : verts) {
-        if (!std::isfinite(x)) throw ArgError("non-finite vertex coordinate");
-        scale = std::max(scale, std::fabs(x));
-    }
-    const int F = (int)nf;
-    std::vector<Box> tb((size_t)F);
+// The hierarchy over primitives with boxes tb (triangles or segments);
+// prim gets the primitive order of the leaves.  Returns the depth.
+int build_nodes(const std::vector<Box>& tb, std::vector<int>& prim, std::vector<BvhNode>& nodes) {
+    const int F = (int)tb.size();
     std::vector<double> cen((size_t)F * 3);
-    for (int t = 0; t < F; ++t) {
-        for (int k = 0; k < 3; ++k) {
-            const double* p = &verts[(size_t)faces[(size_t)3 * t + k] * 3];
-            tb[(size_t)t].grow(p, p);
-        }
+    for (int t = 0; t < F; ++t)
         for (int d = 0; d < 3; ++d) cen[(size_t)3 * t + d] = 0.5 * (tb[(size_t)t].lo[d] + tb[(size_t)t].hi[d]);
-    }
-    std::vector<int> prim((size_t)F);
+    prim.resize((size_t)F);
     std::iota(prim.begin(), prim.end(), 0);
-    std::vector<BvhNode> nodes;
+    nodes.clear();
     nodes.reserve((size_t)2 * F);
     nodes.push_back(BvhNode{});
     struct Item { int node, start, end, depth; };
@@ -157,20 +140,77 @@ void mesh_build(MeshBvh& m, const std::vector<double>& verts, const std::vector<
         work.push_back({left + 1, mid, it.end, it.depth + 1});
         work.push_back({left, it.start, mid, it.depth + 1});
     }
-    if (depth >= kBvhStack - 2) throw GeomError("mesh hierarchy deeper than the traversal stack");
-    auto up = [](Buffer& b, const void* src, size_t bytes) {
-        b = Buffer(bytes);
-        HIPCHK(hipMemcpy(b.p, src, bytes, hipMemcpyHostToDevice));
-    };
-    up(m.v, verts.data(), verts.size() * sizeof(double));
-    up(m.f, faces.data(), faces.size() * sizeof(int));
-    up(m.nodes, nodes.data(), nodes.size() * sizeof(BvhNode));
-    up(m.prim, prim.data(), prim.size() * sizeof(int));
+    if (depth >= kBvhStack - 2) throw GeomError("closest-point hierarchy deeper than the traversal stack");
+    return depth;
+}
+
+void upload(Buffer& b, const void* src, size_t bytes) {
+    b = Buffer(bytes);
+    HIPCHK(hipMemcpy(b.p, src, bytes, hipMemcpyHostToDevice));
+}
+
+double coordinate_scale(const std::vector<double>& verts) {
+    double scale = 0;
+    for (double x : verts) {
+        if (!std::isfinite(x)) throw ArgError("non-finite vertex coordinate");
+        scale = std::max(scale, std::fabs(x));
+    }
+    return scale;
+}
+
+}  // namespace
+
+void mesh_build(MeshBvh& m, const std::vector<double>& verts, const std::vector<int>& faces) {
+    if (verts.size() % 3 || faces.size() % 3) throw ArgError("mesh arrays must hold 3 values per vertex / face");
+    const int64_t nv = (int64_t)verts.size() / 3, nf = (int64_t)faces.size() / 3;
+    if (nv < 3 || nf < 1) throw ArgError("mesh needs at least one triangle");
+    if (nf > (1 << 28)) throw ArgError("too many triangles");
+    for (int x : faces)
+        if (x < 0 || x >= nv) throw ArgError("face index out of range");
+    const double scale = coordinate_scale(verts);
+    const int F = (int)nf;
+    std::vector<Box> tb((size_t)F);
+    for (int t = 0; t < F; ++t)
+        for (int k = 0; k < 3; ++k) {
+            const double* p = &verts[(size_t)faces[(size_t)3 * t + k] * 3];
+            tb[(size_t)t].grow(p, p);
+        }
+    std::vector<int> prim;
+    std::vector<BvhNode> nodes;
+    m.depth = build_nodes(tb, prim, nodes);
+    upload(m.v, verts.data(), verts.size() * sizeof(double));
+    upload(m.f, faces.data(), faces.size() * sizeof(int));
+    upload(m.nodes, nodes.data(), nodes.size() * sizeof(BvhNode));
+    upload(m.prim, prim.data(), prim.size() * sizeof(int));
     m.ntri = F;
     m.nnodes = (int)nodes.size();
-    m.depth = depth;
     // A triangle's computed distance is within a few ulps of coordinate size
     // of the true one, which is at least the box distance.
+    m.slack = 1e-12 * (1.0 + scale);
+}
+
+void segment_build(MeshBvh& m, const std::vector<double>& verts) {
+    if (verts.size() % 3) throw ArgError("polyline vertices must hold 3 values each");
+    const int64_t nv = (int64_t)verts.size() / 3;
+    if (nv < 2) throw ArgError("polyline needs >= 2 vertices");
+    if (nv > (1 << 28)) throw ArgError("too many polyline vertices");
+    const double scale = coordinate_scale(verts);
+    const int F = (int)(nv - 1);   // segment k joins vertices k and k + 1
+    std::vector<Box> tb((size_t)F);
+    for (int t = 0; t < F; ++t) {
+        const double* a = &verts[(size_t)3 * t];
+        tb[(size_t)t].grow(a, a);
+        tb[(size_t)t].grow(a + 3, a + 3);
+    }
+    std::vector<int> prim;
+    std::vector<BvhNode> nodes;
+    m.depth = build_nodes(tb, prim, nodes);
+    upload(m.v, verts.data(), verts.size() * sizeof(double));
+    m.f = Buffer();
+    upload(m.nodes, nodes.data(), nodes.size() * sizeof(BvhNode));
+    upload(m.prim, prim.data(), prim.size() * sizeof(int));
+    m.ntri = F;
+    m.nnodes = (int)nodes.size();
     m.slack = 1e-12 * (1.0 + scale);
 }
 
@@ -229,6 +269,18 @@ __device__ __forceinline__ double dist3(const double* a, const double* b) {
     return sqrt(s);
 }
 
+// Closest point on segment ab to x (the projection clamped to the segment)
+// and its distance, each sum in the order of the oracle's restatement.
+__device__ double cp_segment(const double* x, const double* a, const double* b, double* q) {
+    double ab[3], ax[3];
+    for (int d = 0; d < 3; ++d) { ab[d] = b[d] - a[d]; ax[d] = x[d] - a[d]; }
+    const double L2 = dot3(ab, ab);
+    double t = L2 > 0 ? dot3(ax, ab) / L2 : 0.0;
+    t = fmin(1.0, fmax(0.0, t));
+    for (int d = 0; d < 3; ++d) q[d] = a[d] + t * ab[d];
+    return dist3(x, q);
+}
+
 #pragma clang fp contract(on)
 
 // Squared distance from x to the node's box.
@@ -242,10 +294,12 @@ __device__ __forceinline__ double box_d2(const BvhNode* nd, const double* x) {
 }
 
 // Depth-first traversal, nearer child first.  A box is skipped only when it is
-// farther than the best distance plus the slack, so every triangle whose
+// farther than the best distance plus the slack, so every primitive whose
 // computed distance could tie or beat the best is visited; ties go to the
-// lower triangle index.
-__device__ void cp_mesh(const MeshDev& m, const double* x, double* c, double& dist) {
+// lower primitive index.  SEG: polyline segments (vertices t, t + 1), else
+// triangles.
+template <bool SEG>
+__device__ void cp_bvh(const MeshDev& m, const double* x, double* c, double& dist) {
     int stack[kBvhStack];
     int sp = 0;
     double best = INFINITY;
@@ -257,10 +311,14 @@ __device__ void cp_mesh(const MeshDev& m, const double* x, double* c, double& di
         if (nd->count > 0) {
             for (int k = 0; k < nd->count; ++k) {
                 const int t = m.prim[nd->first + k];
-                const int* fv = m.f + 3 * (int64_t)t;
-                double q[3];
-                cp_triangle(x, m.v + 3 * (int64_t)fv[0], m.v + 3 * (int64_t)fv[1], m.v + 3 * (int64_t)fv[2], q);
-                const double dd = dist3(x, q);
+                double q[3], dd;
+                if (SEG) {
+                    dd = cp_segment(x, m.v + 3 * (int64_t)t, m.v + 3 * (int64_t)t + 3, q);
+                } else {
+                    const int* fv = m.f + 3 * (int64_t)t;
+                    cp_triangle(x, m.v + 3 * (int64_t)fv[0], m.v + 3 * (int64_t)fv[1], m.v + 3 * (int64_t)fv[2], q);
+                    dd = dist3(x, q);
+                }
                 if (dd < best || (dd == best && t < bt)) {
                     best = dd;
                     bt = t;
@@ -294,5 +352,8 @@ __device__ void cp_mesh(const MeshDev& m, const double* x, double* c, double& di
     }
     dist = best;
 }
+
+__device__ void cp_mesh(const MeshDev& m, const double* x, double* c, double& dist) { cp_bvh<false>(m, x, c, dist); }
+__device__ void cp_polyline(const MeshDev& m, const double* x, double* c, double& dist) { cp_bvh<true>(m, x, c, dist); }
 
 }  // namespace cpmgpu

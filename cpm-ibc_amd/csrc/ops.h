@@ -55,6 +55,7 @@ struct NoBuilds {
 // or the tensor-product format of tube-built operators.
 struct Op {
     Ctx* ctx = nullptr;
+    bool fp32_failed = false;   // an FP32+refine solve of this operator fell back to FP64
     int64_t D = 0;
     int nD = 0;
     int ew = 0, lw = 0;

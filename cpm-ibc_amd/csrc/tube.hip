@@ -282,21 +282,8 @@ __device__ void cp_surface(const SurfDev& s, const double* x, double* c, double&
     case SURF_TRIANGULATION:   // Surface.cpp:1751-1779 (|signed distance|, boundary tag 0)
         cp_mesh(s.mesh, x, c, dist);
         break;
-    case SURF_POLYLINE: {   // exact FP64 segment projection, first minimum wins
-        double best = INFINITY;
-        for (int k = 0; k < s.nseg; ++k) {
-            const double* a = s.verts + 3 * k;
-            const double* b = a + 3;
-            const double ab[3] = {b[0] - a[0], b[1] - a[1], b[2] - a[2]};
-            const double ax[3] = {x[0] - a[0], x[1] - a[1], x[2] - a[2]};
-            const double L2 = ab[0] * ab[0] + ab[1] * ab[1] + ab[2] * ab[2];
-            double t = L2 > 0 ? (ax[0] * ab[0] + ax[1] * ab[1] + ax[2] * ab[2]) / L2 : 0.0;
-            t = fmin(1.0, fmax(0.0, t));
-            const double pp[3] = {a[0] + t * ab[0], a[1] + t * ab[1], a[2] + t * ab[2]};
-            const double dd = sqrt(dsq(x[0] - pp[0]) + dsq(x[1] - pp[1]) + dsq(x[2] - pp[2]));
-            if (dd < best) { best = dd; c[0] = pp[0]; c[1] = pp[1]; c[2] = pp[2]; }
-        }
-        dist = best;
+    case SURF_POLYLINE: {   // segment projections through the hierarchy: the exact minimiser, lowest index
+        cp_polyline(s.mesh, x, c, dist);
         if (s.is_open) {   // Surface.cpp:2614-2636
             const double e0 = sqrt(dsq(s.ep0[0] - c[0]) + dsq(s.ep0[1] - c[1]) + dsq(s.ep0[2] - c[2]));
             const double e1 = sqrt(dsq(s.ep1[0] - c[0]) + dsq(s.ep1[1] - c[1]) + dsq(s.ep1[2] - c[2]));
@@ -639,7 +626,9 @@ SurfDev make_surfdev(const SurfaceHost& s, const double* dverts, const MeshDev* 
         d.type = SURF_TRIANGULATION;
         d.mesh = *mesh;
     } else if (s.type == "Polyline") {
+        if (!mesh) throw ArgError("polyline without its hierarchy");
         d.type = SURF_POLYLINE;
+        d.mesh = *mesh;
         d.verts = dverts;
         d.nseg = (int)(s.verts.size() / 3) - 1;
         d.is_open = s.is_open;
@@ -662,10 +651,11 @@ Tube* tube_build(Ctx* ctx, const SurfaceHost& s, int p, double dx) {
     T->dim = 3;
     T->p = p;
     T->dx = dx;
-    if (s.type == "Triangulation") {
-        mesh_build(T->mesh, s.verts, s.faces);
+    if (s.type == "Triangulation" || s.type == "Polyline") {
+        if (s.type == "Triangulation") mesh_build(T->mesh, s.verts, s.faces);
+        else segment_build(T->mesh, s.verts);
         const MeshDev md = T->mesh.dev();
-        T->sd = make_surfdev(s, nullptr, &md);
+        T->sd = make_surfdev(s, T->mesh.v.as<double>(), &md);
     } else {
         if (!s.verts.empty()) {
             T->surf_verts.~Buffer();

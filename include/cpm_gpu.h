@@ -72,7 +72,10 @@ typedef struct cpm_solve_info {
     int updates;           /* FP64 refinement steps (FP32_REFINE)                 */
     double rel_residual;   /* ||b-Ax||/||b||: FP64 recurrence residual (FP64), true FP64 residual (FP32_REFINE) */
     int fallback_iters;    /* FP32_REFINE: FP64 iterations of the re-solve from zero that a system gets when
-                              its refinement steps stop reducing the true residual (0 = none)             */
+                              its refinement steps stop reducing the true residual (0 = none).  Once an
+                              operator has had such a re-solve, later FP32_REFINE solves of it run in FP64
+                              from the start: iters = 0 and fallback_iters = the FP64 iterations
+                              (environment CPMGPU_FP64_ROUTING=0: FP32 is tried every time)               */
 } cpm_solve_info;
 
 /* Defaults of the reference: tol 1e-10, max 100000, FP64. */

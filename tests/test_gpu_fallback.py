@@ -105,3 +105,26 @@ def test_fallback_in_server(gpu_ctx, fine_sphere):
         srv.close()
         for d in d_rhs + d_x:
             gpu_ctx.free(d)
+
+
+def test_fp64_routing_after_fallback(gpu_ctx, fine_sphere, monkeypatch):
+    """Once an operator's FP32+refine solve has fallen back to FP64, later FP32+refine solves of it
+    start in FP64 (no FP32 iterations; the result is the plain FP64 solve); CPMGPU_FP64_ROUTING=0
+    tries FP32 first again."""
+    print("case fp64 routing sphere h=0.01", flush=True)
+    t, _, rhs = fine_sphere
+    op = t.operator(1.0, -1.0)                       # a fresh operator: no fallback recorded
+    o32 = cpmgpu.opts(precision=cpmgpu.CPM_FP32_REFINE)
+    x64, i64 = op.solve(rhs[1], cpmgpu.opts(precision=cpmgpu.CPM_FP64))
+    _, first = op.solve(rhs[1], o32)
+    print("first", first, flush=True)
+    assert first["iters"] > 0 and first["fallback_iters"] == i64["iters"]
+    x2, second = op.solve(rhs[1], o32)
+    print("second", second, flush=True)
+    assert second["status"] == "converged" and second["iters"] == 0 and second["fallback_iters"] == i64["iters"]
+    assert np.array_equal(x2, x64)
+    xs, infos = gpu_ctx.solve_batch([op] * 2, rhs[:2], o32)
+    assert all(i["iters"] == 0 and i["fallback_iters"] > 0 and i["status"] == "converged" for i in infos)
+    monkeypatch.setenv("CPMGPU_FP64_ROUTING", "0")
+    _, third = op.solve(rhs[1], o32)
+    assert third["iters"] > 0 and third["fallback_iters"] == i64["iters"]

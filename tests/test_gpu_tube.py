@@ -4,6 +4,7 @@ import numpy as np
 import pytest
 
 import cpmgpu
+from cpmgpu import workloads
 import orc
 
 pytestmark = pytest.mark.gpu
@@ -121,3 +122,26 @@ def test_tensor_product_format(gpu_ctx, monkeypatch, surface, dx, params, bbox):
     assert all(i["status"] == "converged" for i in infos)
     assert np.linalg.norm(xs[0] - xs[1]) <= 1e-9 * np.linalg.norm(xs[1])
     assert np.linalg.norm(xs[2] - 2 * xs[1]) <= 1e-9 * np.linalg.norm(xs[2])
+
+
+@pytest.mark.parametrize("closed", [True, False])
+def test_polyline_tube_matches_oracle(gpu_ctx, closed):
+    """A 2000-segment torus-knot polyline as the surface (codimension two), closed and open: the GPU's
+    segment hierarchy finds the same closest points, bit for bit, as the oracle's scan over every
+    segment (same formula, lowest segment index among equal distances), so the tubes are identical."""
+    print(f"case polyline tube closed={closed}", flush=True)
+    K = workloads.torus_knot_polyline(1.0, 0.4, 2000)
+    if not closed:
+        K = K[:1500]
+    dx = 0.05
+    bb = (min(0.0, K.min()), max(0.0, K.max()))
+    P = orc.Problem("Polyline", dx, verts=K, bbox=bb)
+    To = P.tube()
+    tg = cpmgpu.Tube(gpu_ctx, "Polyline", dx, verts=K, bbox=bb)
+    Tg = tg.export()
+    print(f"nodes={P.N}", flush=True)
+    assert tg.N == P.N
+    for k in ("rec", "nbr", "bdy", "x", "cp", "dist", "bpt"):
+        assert np.array_equal(Tg[k], To[k]), k
+    if not closed:
+        assert (Tg["bdy"] == 1).any() and (Tg["bdy"] == 2).any()

@@ -134,9 +134,10 @@ def cpu_baseline(work, args):
                        f"{'; '.join(notes)}; extrapolated linearly to the full solve's {full_iters} iterations")
 
 
-def egather_traffic(kernel, workload):
-    """HBM bytes per launch of `kernel` on `workload` measured with rocprofv3 FETCH_SIZE / WRITE_SIZE
-    counters (scaled by the same counters on a plain copy of known size), from the committed summary."""
+def egather_traffic(kernel, workload, groups=1):
+    """HBM bytes per launch of `kernel` on `workload` (`groups` operator copies per launch) measured with
+    rocprofv3 FETCH_SIZE / WRITE_SIZE counters (scaled by the same counters on a plain copy of known
+    size), from the committed summary."""
     path = os.path.join(REPO, "profiles", "egather_traffic.txt")
     if not os.path.exists(path):
         return None
@@ -146,7 +147,7 @@ def egather_traffic(kernel, workload):
             if ":" in line and not line.startswith("#"):
                 k, v = line.split(":", 1)
                 rec[k.strip()] = v.strip()
-    if rec.get("kernel") != kernel or rec.get("workload") != workload:
+    if rec.get("kernel") != kernel or rec.get("workload") != workload or rec.get("groups", "1") != str(groups):
         return None
     try:
         return int(float(rec["traffic_bytes_per_launch"]))
@@ -484,8 +485,9 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--batch", type=int, default=8, help="interleaved lanes (1, 2, 4 or 8)")
-    ap.add_argument("--stream", type=int, default=32, help="right-hand sides streamed through the lanes per step")
+    ap.add_argument("--batch", type=int, default=32,
+                    help="streamed lanes (1, 2, 4, 8, or 16 and 32 as groups of 8 in every launch)")
+    ap.add_argument("--stream", type=int, default=64, help="right-hand sides streamed through the lanes per step")
     ap.add_argument("--workload", default="torus-knot", choices=["torus-knot", "sphere", "surfaces", "serving", "rd"])
     ap.add_argument("--requests", type=int, default=16, help="serving: requests")
     ap.add_argument("--qps", type=float, default=1.0, help="serving: mean arrival rate")
@@ -569,10 +571,11 @@ def main():
     inf64 = ctx.solve_batch_device([op], d_rhs[:1], d_x[:1], o64)[0]
     x64 = ctx.d2h(np.zeros(D), d_x[0])
 
-    # FP64 with the same batching: the step's S right-hand sides as consecutive batches of 4 and
-    # of 8 interleaved lanes (FP64 has no streamed mode: each batch waits for its slowest system)
+    # FP64 with the same batching: the step's S right-hand sides as consecutive batches of 4, 8 and
+    # 32 (FP64 groups 4 interleaved lanes per operator copy; it has no streamed mode: each batch waits
+    # for its slowest system)
     fp64_batched = {}
-    for lanes in (4, 8):
+    for lanes in (4, 8, 32):
         ctx.solve_batch_device([op] * lanes, d_rhs[:lanes], d_x[:lanes], o64)
         ctx.synchronize()
         t0 = time.perf_counter()
@@ -593,14 +596,17 @@ def main():
     p50 = statistics.median(lat)
     p99 = lat[min(len(lat) - 1, int(math.ceil(0.99 * len(lat))) - 1)]
 
-    # roofline of the dominant kernel: E-gather, FP32, 8 interleaved RHS (what each batched iteration runs)
+    # roofline of the dominant kernel: E-gather, FP32, 8 interleaved RHS in B / 8 groups per launch (what
+    # each streamed iteration runs)
     nr = 8 if B >= 8 else (4 if B >= 4 else (2 if B >= 2 else 1))
-    eg_ms, eg_bytes = op.profile_egather(cpmgpu.CPM_FP32_REFINE, nr, reps=50)
+    groups = max(1, B // 8)
+    eg_ms, eg_bytes = op.profile_egather(cpmgpu.CPM_FP32_REFINE, nr, reps=50, groups=groups)
     achieved = eg_bytes / (eg_ms * 1e-3) / 1e9
     kname = op.gather_kernel(cpmgpu.CPM_FP32_REFINE, nr)
     roof = dict(bound="hbm", achieved=round(achieved, 1), peak=HBM_PEAK_GBS, unit="GB/s",
-                frac=round(achieved / HBM_PEAK_GBS, 4), traffic=egather_traffic(kname, args.workload),
-                kernel=kname, avg_launch_ms=round(eg_ms, 5), bytes_per_launch=int(eg_bytes))
+                frac=round(achieved / HBM_PEAK_GBS, 4), traffic=egather_traffic(kname, args.workload, groups),
+                kernel=kname, groups_per_launch=groups, avg_launch_ms=round(eg_ms, 5),
+                bytes_per_launch=int(eg_bytes))
     copy_bytes = 1 << 29
     copy_ms = ctx.profile_copy(copy_bytes, 20)
 
@@ -626,7 +632,7 @@ def main():
         "config": {"workload": work["name"], "rows": D, "rhs_per_step": S, "lanes": B, "tol": tol,
                    "precond": "diagonal", "update_delta": args.update_delta,
                    "tube_build_ms": round(work["tube"].info["build_ms"], 2), "setup_s": round(setup_s, 3)},
-        # FP32+refine against FP64 with the same batching (the faster of 4 and 8 FP64 lanes), and
+        # FP32+refine against FP64 with the same batching (the fastest FP64 batch), and
         # against FP64 one right-hand side per solve (batching and precision together)
         "fp32refine_vs_fp64": round(value / fp64_best, 3),
         "fp64_batched_solves_per_s": {str(k): round(v, 3) for k, v in fp64_batched.items()},

@@ -551,42 +551,54 @@ static void run_batch(Ctx* ctx, const std::vector<Group>& groups, const std::vec
 }
 
 // Continuous batching (FP32 + refinement): n systems of one operator through
-// NR interleaved lanes.  At every poll a lane whose system stopped hands its
-// solution out and takes the next right-hand side: b64 <- new b, x64 = x32 = 0
-// and a fresh state in the refinement status, whose step then computes
-// ||b - A 0||^2 = ||b||^2 and restarts the lane's Krylov space
-// (mx_update).  Lanes never wait for the slowest system of a batch.
+// G x NR interleaved lanes (G segments of NR lanes of the same operator in
+// every launch).  At every poll a lane whose system stopped hands its solution
+// out and takes the next right-hand side: b64 <- new b, x64 = x32 = 0 and a
+// fresh state in the refinement status, whose step then computes
+// ||b - A 0||^2 = ||b||^2 and restarts the lane's Krylov space (mx_update).
+// Lanes never wait for the slowest system of a batch.
 template <int NR>
 static void run_stream(Ctx* ctx, Op* op, const std::vector<const double*>& rhs, const std::vector<double*>& xs,
-                       const cpm_solve_opts& o, std::vector<cpm_solve_info>& infos) {
+                       int G, const cpm_solve_opts& o, std::vector<cpm_solve_info>& infos) {
     const hipStream_t st = ctx->stream;
     const int n = (int)rhs.size();
+    G = std::max(1, std::min(G, (n + NR - 1) / NR));   // no segment without a first system
     const int64_t D = op->D;
     const int nb = (int)((D + kBlock - 1) / kBlock);
+    const int blocks = G * nb;
+    const int nstate = G * kMaxNR;
+    if (nstate > Ctx::kHostStates) throw ArgError("too many lanes");
     ensure_group_tiles_for({op}, o.precision == CPM_FP32_REFINE ? 4 : 8, NR);
-    const int64_t total = (D * NR + 63) & ~int64_t(63);
+    const int64_t seg_total = (D * NR + 63) & ~int64_t(63);   // vector entries of one segment
+    const int64_t total = seg_total * G;
     auto r256 = [](size_t k) { return (k + 255) & ~size_t(255); };
     const size_t part_stride = 2 * NR;
-    const size_t bytes = r256(sizeof(Seg<double>)) + r256(sizeof(Seg<float>)) + r256(sizeof(int) * nb) +
-                         r256(sizeof(KState) * kMaxNR) + r256(sizeof(double) * part_stride * nb) +
+    const size_t bytes = r256(sizeof(Seg<double>) * G) + r256(sizeof(Seg<float>) * G) + 2 * r256(sizeof(int) * blocks) +
+                         r256(sizeof(KState) * nstate) + r256(sizeof(double) * part_stride * blocks) +
                          r256(sizeof(PackEntry) * 4) + 256;
     char* tb = reinterpret_cast<char*>(ctx->workspace("stream_tables", bytes));
     auto carve = [&](size_t k) { char* p = tb; tb += r256(k); return p; };
-    auto* seg64 = reinterpret_cast<Seg<double>*>(carve(sizeof(Seg<double>)));
-    auto* seg32 = reinterpret_cast<Seg<float>*>(carve(sizeof(Seg<float>)));
-    int* bseg = reinterpret_cast<int*>(carve(sizeof(int) * nb));   // all zero: one segment
-    KState* sts = reinterpret_cast<KState*>(carve(sizeof(KState) * kMaxNR));
-    double* part = reinterpret_cast<double*>(carve(sizeof(double) * part_stride * nb));
+    auto* seg64 = reinterpret_cast<Seg<double>*>(carve(sizeof(Seg<double>) * G));
+    auto* seg32 = reinterpret_cast<Seg<float>*>(carve(sizeof(Seg<float>) * G));
+    int* bseg = reinterpret_cast<int*>(carve(sizeof(int) * blocks));    // block -> segment
+    int* bzero = reinterpret_cast<int*>(carve(sizeof(int) * blocks));   // all zero: one pack entry
+    KState* sts = reinterpret_cast<KState*>(carve(sizeof(KState) * nstate));
+    double* part = reinterpret_cast<double*>(carve(sizeof(double) * part_stride * blocks));
     PackEntry* dpe = reinterpret_cast<PackEntry*>(carve(sizeof(PackEntry) * 4));
-    const int nr0 = std::min(n, NR);
-    Seg<double> h64{};
-    Seg<float> h32{};
-    fill_seg(h64, op, 0, 0, nb, nr0, sts, part);
-    fill_seg(h32, op, 0, 0, nb, nr0, sts, part);
-    std::vector<int> hb(nb, 0);
-    HIPCHK(hipMemcpyAsync(seg64, &h64, sizeof(h64), hipMemcpyHostToDevice, st));
-    HIPCHK(hipMemcpyAsync(seg32, &h32, sizeof(h32), hipMemcpyHostToDevice, st));
-    HIPCHK(hipMemcpyAsync(bseg, hb.data(), sizeof(int) * nb, hipMemcpyHostToDevice, st));
+    std::vector<int> nr0(G);
+    std::vector<Seg<double>> h64(G);
+    std::vector<Seg<float>> h32(G);
+    std::vector<int> hb(blocks), hz(blocks, 0);
+    for (int g = 0; g < G; ++g) {
+        nr0[g] = std::min(NR, n - g * NR);
+        fill_seg(h64[g], op, g * seg_total, g * nb, nb, nr0[g], sts + g * kMaxNR, part + part_stride * g * nb);
+        fill_seg(h32[g], op, g * seg_total, g * nb, nb, nr0[g], sts + g * kMaxNR, part + part_stride * g * nb);
+        for (int b = 0; b < nb; ++b) hb[g * nb + b] = g;
+    }
+    HIPCHK(hipMemcpyAsync(seg64, h64.data(), sizeof(Seg<double>) * G, hipMemcpyHostToDevice, st));
+    HIPCHK(hipMemcpyAsync(seg32, h32.data(), sizeof(Seg<float>) * G, hipMemcpyHostToDevice, st));
+    HIPCHK(hipMemcpyAsync(bseg, hb.data(), sizeof(int) * blocks, hipMemcpyHostToDevice, st));
+    HIPCHK(hipMemcpyAsync(bzero, hz.data(), sizeof(int) * blocks, hipMemcpyHostToDevice, st));
     Vecs<double> V64{};
     Vecs<float> V32{};
     {
@@ -599,20 +611,20 @@ static void run_stream(Ctx* ctx, Op* op, const std::vector<const double*>& rhs, 
         }
     }
     // one lane of one vector in or out: a single pack entry with block0 = 0;
-    // FP64 vectors are lane-major, FP32 ones interleaved
-    auto lane_pack = [&](int k, const double* src, double* dst, double* ws64, float* ws32, int to_ws) {
+    // FP64 vectors are lane-major, FP32 ones interleaved, within the lane's segment
+    auto lane_pack = [&](int g, int k, const double* src, double* dst, double* ws64, float* ws32, int to_ws) {
         PackEntry e{};
         e.src = src;
         e.dst = dst;
         e.perm = op->perm.as<int>();
         e.D = D;
-        e.off = 0;
+        e.off = g * seg_total;
         e.lane = k;
         e.nr = ws64 ? 1 : NR;
         e.ls = ws64 ? D : 1;
         e.block0 = 0;
         HIPCHK(hipMemcpyAsync(dpe, &e, sizeof(e), hipMemcpyHostToDevice, st));
-        hipLaunchKernelGGL(k_pack, dim3((unsigned)nb), dim3(256), 0, st, dpe, bseg, ws64, ws32, to_ws);
+        hipLaunchKernelGGL(k_pack, dim3((unsigned)nb), dim3(256), 0, st, dpe, bzero, ws64, ws32, to_ws);
         HIPCHK(hipStreamSynchronize(st));   // the entry is rewritten by the next call
     };
     // all lanes start from zero vectors (padding lanes stay zero and idle)
@@ -629,51 +641,54 @@ static void run_stream(Ctx* ctx, Op* op, const std::vector<const double*>& rhs, 
     fresh.scale = 1.0;
     KState idle{};
     idle.status = ST_CONVERGED;
-    std::vector<KState> hst(kMaxNR, idle);
-    std::vector<int> cur(NR, -1);
+    std::vector<KState> hst(nstate, idle);
+    std::vector<int> cur(nstate, -1);   // system of lane (g, k) at g * kMaxNR + k
     int next = 0;
-    for (int k = 0; k < nr0; ++k) {
-        lane_pack(k, rhs[next], nullptr, V64.b, nullptr, 1);
-        hst[k] = fresh;
-        cur[k] = next++;
-    }
-    HIPCHK(hipMemcpyAsync(sts, hst.data(), sizeof(KState) * kMaxNR, hipMemcpyHostToDevice, st));
+    for (int g = 0; g < G; ++g)
+        for (int k = 0; k < nr0[g]; ++k) {
+            lane_pack(g, k, rhs[next], nullptr, V64.b, nullptr, 1);
+            hst[g * kMaxNR + k] = fresh;
+            cur[g * kMaxNR + k] = next++;
+        }
+    HIPCHK(hipMemcpyAsync(sts, hst.data(), sizeof(KState) * nstate, hipMemcpyHostToDevice, st));
     int sweeps = op->precond == CPM_PRECOND_JACOBI ? op->precond_iters : 0;
-    const LaunchCfg<float> c32{seg32, bseg, nb, 1, st, sweeps, op->ngt};
+    const LaunchCfg<float> c32{seg32, bseg, blocks, G, st, sweeps, op->ngt};
     const MixVec mv{V64.x, V64.r, V64.b, V64.tmp};
     KState* hs = ctx->host_states;
     int done = 0;
     while (done < n) {
         for (int c = 0; c < 16; ++c) Launch<float, NR>::iteration(c32, V32, true, mv);
         HIPCHK(hipGetLastError());
-        HIPCHK(hipMemcpyAsync(hs, sts, sizeof(KState) * kMaxNR, hipMemcpyDeviceToHost, st));
+        HIPCHK(hipMemcpyAsync(hs, sts, sizeof(KState) * nstate, hipMemcpyDeviceToHost, st));
         HIPCHK(hipStreamSynchronize(st));
         bool changed = false;
-        for (int k = 0; k < nr0; ++k) {
-            if (cur[k] < 0) continue;
-            const KState& s = hs[k];
-            const bool running = s.status == ST_RUN || (s.status >= ST_UPDATE && s.status <= ST_READY);
-            if (running) continue;
-            cpm_solve_info& inf = infos[(size_t)cur[k]];
-            inf.status = s.status;
-            inf.iters = s.iters;
-            inf.restarts = s.restarts;
-            inf.updates = s.updates;
-            inf.rel_residual = s.bsq > 0 ? std::sqrt(s.rr_true / s.bsq) : 0.0;
-            lane_pack(k, nullptr, xs[(size_t)cur[k]], V64.x, nullptr, 0);
-            ++done;
-            cur[k] = -1;
-            KState ns = idle;
-            if (next < n) {
-                lane_pack(k, rhs[(size_t)next], nullptr, V64.b, nullptr, 1);
-                lane_pack(k, nullptr, nullptr, V64.x, nullptr, 1);   // x64 = 0
-                lane_pack(k, nullptr, nullptr, nullptr, V32.x, 1);   // x32 = 0
-                ns = fresh;
-                cur[k] = next++;
+        for (int g = 0; g < G; ++g)
+            for (int k = 0; k < nr0[g]; ++k) {
+                const int L = g * kMaxNR + k;
+                if (cur[L] < 0) continue;
+                const KState& s = hs[L];
+                const bool running = s.status == ST_RUN || (s.status >= ST_UPDATE && s.status <= ST_READY);
+                if (running) continue;
+                cpm_solve_info& inf = infos[(size_t)cur[L]];
+                inf.status = s.status;
+                inf.iters = s.iters;
+                inf.restarts = s.restarts;
+                inf.updates = s.updates;
+                inf.rel_residual = s.bsq > 0 ? std::sqrt(s.rr_true / s.bsq) : 0.0;
+                lane_pack(g, k, nullptr, xs[(size_t)cur[L]], V64.x, nullptr, 0);
+                ++done;
+                cur[L] = -1;
+                KState ns = idle;
+                if (next < n) {
+                    lane_pack(g, k, rhs[(size_t)next], nullptr, V64.b, nullptr, 1);
+                    lane_pack(g, k, nullptr, nullptr, V64.x, nullptr, 1);   // x64 = 0
+                    lane_pack(g, k, nullptr, nullptr, nullptr, V32.x, 1);   // x32 = 0
+                    ns = fresh;
+                    cur[L] = next++;
+                }
+                HIPCHK(hipMemcpyAsync(sts + L, &ns, sizeof(KState), hipMemcpyHostToDevice, st));
+                changed = true;
             }
-            HIPCHK(hipMemcpyAsync(sts + k, &ns, sizeof(KState), hipMemcpyHostToDevice, st));
-            changed = true;
-        }
         if (changed) HIPCHK(hipStreamSynchronize(st));
     }
 }
@@ -681,13 +696,17 @@ static void run_stream(Ctx* ctx, Op* op, const std::vector<const double*>& rhs, 
 // FP32_REFINE systems whose refinement stopped reducing the true residual
 // (ST_FALLBACK) are solved again in FP64 from zero, the reference algorithm,
 // in one batch; their infos keep the FP32 counts and add fallback_iters.
-// FP32+refine solves of an operator that once fell back to FP64 (its FP32
-// refinement stalled: the system is past what the FP32 recurrence resolves)
-// start in FP64, instead of spending FP32 iterations first
+// FP64 routing (Fp32Record::prefer_fp64, route.h): FP32+refine solves of an
+// operator start in FP64 when its record says FP64 is cheaper
 // (CPMGPU_FP64_ROUTING=0: always try FP32 first).
 static bool fp64_routing() {   // read per solve call (not per launch)
     const char* e = std::getenv("CPMGPU_FP64_ROUTING");
     return !(e && std::string(e) == "0");
+}
+
+// Adds the outcome of FP32+refine systems (infos after the FP64 re-solves) to their operators' records.
+static void record_fp32(const std::vector<Op*>& ops, const std::vector<cpm_solve_info>& infos) {
+    for (size_t i = 0; i < infos.size(); ++i) ops[i]->fp32.add(infos[i].iters, infos[i].fallback_iters);
 }
 
 static void fp64_fallback(Ctx* ctx, const std::vector<Op*>& ops, const std::vector<const double*>& rhs,
@@ -707,7 +726,6 @@ static void fp64_fallback(Ctx* ctx, const std::vector<Op*>& ops, const std::vect
         fx.push_back(xs[(size_t)i]);
         fmn.push_back(mn ? (*mn)[(size_t)i] : std::make_pair(ops[(size_t)i]->m, ops[(size_t)i]->n));
     }
-    for (Op* op : fo) op->fp32_failed = true;   // later FP32+refine solves of op run in FP64 (fp64_routing)
     cpm_solve_opts o64 = o;
     o64.precision = CPM_FP64;
     std::vector<cpm_solve_info> fi;
@@ -737,7 +755,7 @@ void solve_batch(Ctx* ctx, const std::vector<Op*>& ops, const std::vector<const 
     if (o.precision == CPM_FP32_REFINE && fp64_routing()) {
         // operators whose FP32+refine solves fell back to FP64 before: FP64 from the start
         std::vector<int> r64, r32;
-        for (int i = 0; i < n; ++i) (ops[i]->fp32_failed ? r64 : r32).push_back(i);
+        for (int i = 0; i < n; ++i) (ops[i]->fp32.prefer_fp64() ? r64 : r32).push_back(i);
         if (!r64.empty()) {
             cpm_solve_opts o64 = o;
             o64.precision = CPM_FP64;
@@ -792,7 +810,10 @@ void solve_batch(Ctx* ctx, const std::vector<Op*>& ops, const std::vector<const 
     else if (nr == 2) run_batch<2>(ctx, groups, rhs, xs, o, infos);
     else if (nr <= 4) run_batch<4>(ctx, groups, rhs, xs, o, infos);
     else run_batch<8>(ctx, groups, rhs, xs, o, infos);
-    if (o.precision == CPM_FP32_REFINE) fp64_fallback(ctx, ops, rhs, xs, o, infos, mn);
+    if (o.precision == CPM_FP32_REFINE) {
+        fp64_fallback(ctx, ops, rhs, xs, o, infos, mn);
+        record_fp32(ops, infos);
+    }
 }
 
 static void solve_stream(Op* op, const std::vector<const double*>& rhs, const std::vector<double*>& xs, int lanes,
@@ -802,16 +823,18 @@ static void solve_stream(Op* op, const std::vector<const double*>& rhs, const st
     if (o.precision != CPM_FP32_REFINE) throw ArgError("streamed solves run in FP32_REFINE precision");
     if (op->nD != 0) throw ArgError("streamed solves need an operator without trailing nD rows");
     if (op->precond == CPM_PRECOND_JACOBI && op->precond_iters > 100000) throw ArgError("too many Jacobi sweeps");
-    if (op->fp32_failed && fp64_routing()) {   // see solve_batch
+    if (op->fp32.prefer_fp64() && fp64_routing()) {   // see solve_batch
         solve_batch(op->ctx, std::vector<Op*>(rhs.size(), op), rhs, xs, o, infos, nullptr);
         return;
     }
-    if (lanes == 1) run_stream<1>(op->ctx, op, rhs, xs, o, infos);
-    else if (lanes == 2) run_stream<2>(op->ctx, op, rhs, xs, o, infos);
-    else if (lanes == 4) run_stream<4>(op->ctx, op, rhs, xs, o, infos);
-    else if (lanes == 8) run_stream<8>(op->ctx, op, rhs, xs, o, infos);
-    else throw ArgError("lanes must be 1, 2, 4 or 8");
-    fp64_fallback(op->ctx, std::vector<Op*>(rhs.size(), op), rhs, xs, o, infos, nullptr);
+    if (lanes == 1) run_stream<1>(op->ctx, op, rhs, xs, 1, o, infos);
+    else if (lanes == 2) run_stream<2>(op->ctx, op, rhs, xs, 1, o, infos);
+    else if (lanes == 4) run_stream<4>(op->ctx, op, rhs, xs, 1, o, infos);
+    else if (lanes == 8 || lanes == 16 || lanes == 32) run_stream<8>(op->ctx, op, rhs, xs, lanes / 8, o, infos);
+    else throw ArgError("lanes must be 1, 2, 4, 8, 16 or 32");
+    const std::vector<Op*> ops(rhs.size(), op);
+    fp64_fallback(op->ctx, ops, rhs, xs, o, infos, nullptr);
+    record_fp32(ops, infos);
 }
 
 void multiply(Op* op, const double* dx, double* dy, double m, double n) {
@@ -852,23 +875,28 @@ void multiply(Op* op, const double* dx, double* dy, double m, double n) {
 }
 
 template <typename T, int NR>
-static void profile_egather_t(Op* op, int reps, double* avg_ms, double* bytes) {
+static void profile_egather_t(Op* op, int G, int reps, double* avg_ms, double* bytes) {
     ensure_group_tiles_for({op}, sizeof(T), NR);
     Ctx* ctx = op->ctx;
     const hipStream_t st = ctx->stream;
     const int nblocks = (int)((op->D + kBlock - 1) / kBlock);
-    Seg<T> h{};
-    fill_seg(h, op, 0, 0, nblocks, NR, nullptr, nullptr);
-    char* tb = reinterpret_cast<char*>(ctx->workspace("prof_tables", 4096 + sizeof(int) * nblocks));
+    const int64_t seg_total = (op->D * NR + 63) & ~int64_t(63);
+    std::vector<Seg<T>> h(G);
+    std::vector<int> hb((size_t)G * nblocks);
+    for (int g = 0; g < G; ++g) {   // G copies of the operator in one launch, as streamed solves of 16, 32 lanes
+        fill_seg(h[g], op, g * seg_total, g * nblocks, nblocks, NR, nullptr, nullptr);
+        for (int b = 0; b < nblocks; ++b) hb[(size_t)g * nblocks + b] = g;
+    }
+    const size_t seg_bytes = (sizeof(Seg<T>) * G + 255) & ~size_t(255);
+    char* tb = reinterpret_cast<char*>(ctx->workspace("prof_tables", seg_bytes + sizeof(int) * hb.size()));
     auto* dseg = reinterpret_cast<Seg<T>*>(tb);
-    int* dbseg = reinterpret_cast<int*>(tb + 4096);
-    std::vector<int> hb(nblocks, 0);
-    HIPCHK(hipMemcpyAsync(dseg, &h, sizeof(h), hipMemcpyHostToDevice, st));
-    HIPCHK(hipMemcpyAsync(dbseg, hb.data(), sizeof(int) * nblocks, hipMemcpyHostToDevice, st));
-    T* x = reinterpret_cast<T*>(ctx->workspace("prof_x", sizeof(T) * op->D * NR));
-    T* y = reinterpret_cast<T*>(ctx->workspace("prof_y", sizeof(T) * op->D * NR));
-    HIPCHK(hipMemsetAsync(x, 0, sizeof(T) * op->D * NR, st));
-    LaunchCfg<T> c{dseg, dbseg, nblocks, 1, st, 0, op->ngt};
+    int* dbseg = reinterpret_cast<int*>(tb + seg_bytes);
+    HIPCHK(hipMemcpyAsync(dseg, h.data(), sizeof(Seg<T>) * G, hipMemcpyHostToDevice, st));
+    HIPCHK(hipMemcpyAsync(dbseg, hb.data(), sizeof(int) * hb.size(), hipMemcpyHostToDevice, st));
+    T* x = reinterpret_cast<T*>(ctx->workspace("prof_x", sizeof(T) * seg_total * G));
+    T* y = reinterpret_cast<T*>(ctx->workspace("prof_y", sizeof(T) * seg_total * G));
+    HIPCHK(hipMemsetAsync(x, 0, sizeof(T) * seg_total * G, st));
+    LaunchCfg<T> c{dseg, dbseg, nblocks * G, G, st, 0, op->ngt};
     launch_egather<T, NR>(c, x, y);   // warm-up
     hipEvent_t e0, e1;
     HIPCHK(hipEventCreate(&e0));
@@ -904,7 +932,7 @@ static void profile_egather_t(Op* op, int reps, double* avg_ms, double* bytes) {
               (double)op->grp_ident_items * 4.0 + (double)op->grp_ident * 4.0 + 4.0 * (double)op->grp_list +
               12.0 * (double)op->ngt;
         *avg_ms = ms / reps;
-        *bytes = opb + 2.0 * op->D * NR * sizeof(T);
+        *bytes = G * (opb + 2.0 * op->D * NR * sizeof(T));
         return;
     }
     if (op->fmt == OP_TP && op->tln.p) {
@@ -917,7 +945,7 @@ static void profile_egather_t(Op* op, int reps, double* avg_ms, double* bytes) {
     } else {
         opb = (double)gathered * op->ew * (4.0 + sizeof(T));
     }
-    *bytes = opb + 2.0 * op->D * NR * sizeof(T) + (double)op->D;
+    *bytes = G * (opb + 2.0 * op->D * NR * sizeof(T) + (double)op->D);
 }
 
 // Identity rows of a caller-order vector y take the values of g.
@@ -1717,20 +1745,27 @@ int cpm_solve(cpm_op* op, const double* rhs, double* x, const cpm_solve_opts* op
     return cpm_solve_batch(reinterpret_cast<cpm_ctx*>(O(op)->ctx), 1, ops, r, xx, opts, info);
 }
 
-int cpm_profile_egather(cpm_op* op, int precision, int nrhs, int reps, double* avg_ms, double* bytes) {
+int cpm_profile_egather_groups(cpm_op* op, int precision, int nrhs, int groups, int reps, double* avg_ms,
+                               double* bytes) {
     return guard([&] {
         if (!op || !avg_ms || !bytes || reps <= 0) throw ArgError("bad argument");
+        if (groups < 1 || groups > 64) throw ArgError("groups must be in 1..64");
         Op* o = O(op);
         HIPCHK(hipSetDevice(o->ctx->device));
         const bool f32 = precision == CPM_FP32_REFINE;
+        const int G = groups;
         switch (nrhs) {
-        case 1: f32 ? profile_egather_t<float, 1>(o, reps, avg_ms, bytes) : profile_egather_t<double, 1>(o, reps, avg_ms, bytes); break;
-        case 2: f32 ? profile_egather_t<float, 2>(o, reps, avg_ms, bytes) : profile_egather_t<double, 2>(o, reps, avg_ms, bytes); break;
-        case 4: f32 ? profile_egather_t<float, 4>(o, reps, avg_ms, bytes) : profile_egather_t<double, 4>(o, reps, avg_ms, bytes); break;
-        case 8: f32 ? profile_egather_t<float, 8>(o, reps, avg_ms, bytes) : profile_egather_t<double, 8>(o, reps, avg_ms, bytes); break;
+        case 1: f32 ? profile_egather_t<float, 1>(o, G, reps, avg_ms, bytes) : profile_egather_t<double, 1>(o, G, reps, avg_ms, bytes); break;
+        case 2: f32 ? profile_egather_t<float, 2>(o, G, reps, avg_ms, bytes) : profile_egather_t<double, 2>(o, G, reps, avg_ms, bytes); break;
+        case 4: f32 ? profile_egather_t<float, 4>(o, G, reps, avg_ms, bytes) : profile_egather_t<double, 4>(o, G, reps, avg_ms, bytes); break;
+        case 8: f32 ? profile_egather_t<float, 8>(o, G, reps, avg_ms, bytes) : profile_egather_t<double, 8>(o, G, reps, avg_ms, bytes); break;
         default: throw ArgError("nrhs must be 1, 2, 4 or 8");
         }
     });
+}
+
+int cpm_profile_egather(cpm_op* op, int precision, int nrhs, int reps, double* avg_ms, double* bytes) {
+    return cpm_profile_egather_groups(op, precision, nrhs, 1, reps, avg_ms, bytes);
 }
 
 int cpm_heat_step_device(cpm_op* op, int method, double dt, const double* d_g, double* d_u,

@@ -8,6 +8,7 @@
 
 #include "../../include/cpm_gpu.h"
 #include "engine.h"
+#include "route.h"
 
 namespace cpmgpu {
 
@@ -55,7 +56,7 @@ struct NoBuilds {
 // or the tensor-product format of tube-built operators.
 struct Op {
     Ctx* ctx = nullptr;
-    bool fp32_failed = false;   // an FP32+refine solve of this operator fell back to FP64
+    Fp32Record fp32;            // FP32+refine history of this operator's systems (FP64 routing)
     int64_t D = 0;
     int nD = 0;
     int ew = 0, lw = 0;

@@ -23,7 +23,7 @@ EXPORTS = [
     "cpm_op_set_precond", "cpm_multiply", "cpm_multiply_device", "cpm_solve", "cpm_solve_device",
     "cpm_solve_batch", "cpm_solve_batch_device", "cpm_solve_stream_device", "cpm_device_alloc", "cpm_device_free", "cpm_memcpy_h2d",
     "cpm_memcpy_d2h", "cpm_tube_create", "cpm_tube_create_mesh", "cpm_tube_destroy", "cpm_tube_get_info", "cpm_tube_export",
-    "cpm_tube_export_E", "cpm_tube_export_L", "cpm_tube_interp_matrix", "cpm_op_create_from_tube", "cpm_profile_egather", "cpm_profile_copy", "cpm_heat_step_device", "cpm_rd_params_default", "cpm_rd_step_device", "cpm_problem_create", "cpm_problem_destroy",
+    "cpm_tube_export_E", "cpm_tube_export_L", "cpm_tube_interp_matrix", "cpm_op_create_from_tube", "cpm_profile_egather", "cpm_profile_egather_groups", "cpm_profile_copy", "cpm_heat_step_device", "cpm_rd_params_default", "cpm_rd_step_device", "cpm_problem_create", "cpm_problem_destroy",
     "cpm_problem_get_info", "cpm_problem_rhs_points", "cpm_problem_subset", "cpm_problem_export",
     "cpm_op_create_from_problem", "cpm_geodesic_distance", "cpm_problem_plot_matrix", "cpm_obj_read",
     "cpm_polyline_chain", "cpm_tube_create_obj", "cpm_server_create", "cpm_server_destroy", "cpm_server_submit",
@@ -163,6 +163,8 @@ def load(path=LIB_PATH):
     L.cpm_profile_copy.argtypes = [vp, ctypes.c_int64, ctypes.c_int, ctypes.POINTER(ctypes.c_double)]
     L.cpm_profile_egather.argtypes = [vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.POINTER(ctypes.c_double),
                                       ctypes.POINTER(ctypes.c_double)]
+    L.cpm_profile_egather_groups.argtypes = [vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                             ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double)]
     L.cpm_problem_create.argtypes = [vp, ctypes.c_int, vp, ctypes.POINTER(vp)]
     L.cpm_problem_create_on.argtypes = [vp, vp, ctypes.c_int, vp, ctypes.POINTER(vp)]
     L.cpm_problem_destroy.argtypes = [vp]
@@ -433,10 +435,13 @@ class Operator:
         _chk(load().cpm_solve(self._h, _ptr(rhs), _ptr(x), ctypes.byref(o or opts()), ctypes.byref(info)))
         return x, info.as_dict()
 
-    def profile_egather(self, precision=CPM_FP64, nrhs=1, reps=50):
+    def profile_egather(self, precision=CPM_FP64, nrhs=1, reps=50, groups=1):
+        """Average E-gather launch (ms) and its algorithmic bytes, `groups` copies of the operator per
+        launch (the streamed solves of 16 and 32 lanes run 2 and 4)."""
         ms = ctypes.c_double()
         by = ctypes.c_double()
-        _chk(load().cpm_profile_egather(self._h, precision, nrhs, reps, ctypes.byref(ms), ctypes.byref(by)))
+        _chk(load().cpm_profile_egather_groups(self._h, precision, nrhs, groups, reps, ctypes.byref(ms),
+                                               ctypes.byref(by)))
         return ms.value, by.value
 
     def heat_step_device(self, method, dt, d_g, d_u, o=None):

@@ -73,8 +73,9 @@ typedef struct cpm_solve_info {
     double rel_residual;   /* ||b-Ax||/||b||: FP64 recurrence residual (FP64), true FP64 residual (FP32_REFINE) */
     int fallback_iters;    /* FP32_REFINE: FP64 iterations of the re-solve from zero that a system gets when
                               its refinement steps stop reducing the true residual (0 = none).  Once an
-                              operator has had such a re-solve, later FP32_REFINE solves of it run in FP64
-                              from the start: iters = 0 and fallback_iters = the FP64 iterations
+                              operator's re-solves make FP64 the cheaper choice for it (its fallback
+                              fraction and iteration counts so far), later FP32_REFINE solves of it run
+                              in FP64 from the start: iters = 0 and fallback_iters = the FP64 iterations
                               (environment CPMGPU_FP64_ROUTING=0: FP32 is tried every time)               */
 } cpm_solve_info;
 
@@ -171,8 +172,9 @@ int cpm_solve_batch(cpm_ctx* ctx, int n_inst, cpm_op* const* ops, const double* 
 int cpm_solve_batch_device(cpm_ctx* ctx, int n_inst, cpm_op* const* ops, const double* const* d_rhs,
                            double* const* d_x, const cpm_solve_opts* opts, cpm_solve_info* infos);
 /* Streamed solves of one operator (continuous batching, FP32_REFINE only): the
- * n_rhs systems op x_i = rhs_i flow through `lanes` (1, 2, 4 or 8)
- * interleaved lanes; a lane whose system has stopped takes the next
+ * n_rhs systems op x_i = rhs_i flow through `lanes` (1, 2, 4, 8, or 16 and 32
+ * as 2 and 4 groups of 8 in every launch) interleaved lanes; a lane whose
+ * system has stopped takes the next
  * right-hand side at the next check (every 16 iterations), so a slow system
  * does not hold the others' lanes.  Device pointers, D doubles each.  No
  * counterpart in the reference, which solves one system per Solver::Solve. */
@@ -446,6 +448,12 @@ int cpm_rd_step_device(cpm_op* op, const cpm_rd_params* p, int n_fields, double*
  * duration and the launch's algorithmic HBM bytes: E columns + values once,
  * x read once, tmp written once, one flag byte per row. */
 int cpm_profile_egather(cpm_op* op, int precision, int nrhs, int reps, double* avg_ms, double* bytes_per_launch);
+/* The same with `groups` (1..64) copies of the operator in every launch, as
+ * the launches of streamed solves with 16 or 32 lanes (2 or 4 groups of 8)
+ * and of batches over several operators carry them; bytes cover all groups.
+ * No counterpart in the reference. */
+int cpm_profile_egather_groups(cpm_op* op, int precision, int nrhs, int groups, int reps, double* avg_ms,
+                               double* bytes_per_launch);
 
 /* Plain device-to-device copy of `bytes` (multiple of 16), `reps` launches:
  * the reference point for HBM bandwidth and for scaling hardware traffic

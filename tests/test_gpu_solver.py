@@ -127,11 +127,12 @@ def test_jacobi_preconditioner_matches_oracle(gpu_ctx, sweeps):
     assert np.linalg.norm(x32 - x_ref) / np.linalg.norm(x_ref) <= 1e-8
 
 
-@pytest.mark.parametrize("n,lanes", [(11, 8), (3, 8), (5, 1)])
+@pytest.mark.parametrize("n,lanes", [(11, 8), (3, 8), (5, 1), (37, 16), (9, 16), (45, 32)])
 def test_stream_solves(gpu_ctx, n, lanes):
     """Continuous batching (cpm_solve_stream_device): every system of the stream,
-    whichever lane and refill it ran in, matches its own FP64 solve (both at
-    tol 1e-10) to 1e-8; a zero right-hand side reports zero_rhs and x = 0."""
+    whichever lane, group of 8 lanes (16 and 32 lanes) and refill it ran in,
+    matches its own FP64 solve (both at tol 1e-10) to 1e-8; a zero right-hand
+    side reports zero_rhs and x = 0."""
     print(f"case stream n={n} lanes={lanes}", flush=True)
     tube = cpmgpu.Tube(gpu_ctx, "Sphere", 0.05)
     op = tube.operator(1.0, -1.0)

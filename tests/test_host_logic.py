@@ -38,3 +38,21 @@ def test_refinement_step_rules(tmp_path):
     out = subprocess.run([exe], capture_output=True, text=True, timeout=60)
     assert out.returncode == 0, out.stderr
     assert "refine ok" in out.stdout
+
+
+def test_fp64_routing_rule(tmp_path):
+    """The FP64 routing decision (cpm-ibc_amd/csrc/route.h): an operator whose FP32+refine systems never
+    fell back keeps FP32; one whose expected FP32+refine cost (FP32 iterations at kFp32IterCost plus the
+    re-solves of the fraction that falls back) exceeds the FP64 iterations of its re-solves goes to FP64;
+    the boundary case either side."""
+    if shutil.which("g++") is None:
+        pytest.skip("g++ not available")
+    exe = str(tmp_path / "route_test")
+    src = os.path.join(REPO, "tests", "cpp", "route_test.cpp")
+    inc = os.path.join(REPO, "cpm-ibc_amd", "csrc")
+    build = subprocess.run(["g++", "-std=c++17", "-O1", "-Wall", "-Werror", "-I", inc, src, "-o", exe],
+                           capture_output=True, text=True, timeout=180)
+    assert build.returncode == 0, build.stderr
+    out = subprocess.run([exe], capture_output=True, text=True, timeout=60)
+    assert out.returncode == 0, out.stderr
+    assert "route ok" in out.stdout

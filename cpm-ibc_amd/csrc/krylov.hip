@@ -37,7 +37,7 @@
 // iteration's kernels (no extra launches):
 //
 //   pupdate   x64 += x32 / scale ; x32 = 0
-//   egather   tmp64 = E64 x64
+//   egather_upd  tmp64 = E64 x64   (a launch of its own after the FP32 gather)
 //   lfinish   r64 = b - A64 x64 ; partial (r64, r64)
 //   fin       converged on the true residual?  else scale = 1 / ||r64||
 //   supdate   r32 = r64 scale ; r0 = r32 ; p = v = 0
@@ -686,7 +686,6 @@ __global__ __launch_bounds__(kBlock) void k_pupdate(const Seg<T>* __restrict__ s
 // E x for the instance segments.  Tiles with an LDS list copy their x rows
 // into LDS once and gather every stencil line from there; the other tiles
 // (lists longer than the LDS, or operators without tiles) read x directly.
-// The FP64 gathers of refinement lanes (FP32 segments) always read directly.
 template <typename T, int NR>
 __global__ __launch_bounds__(kBlock) void k_egather(const Seg<T>* __restrict__ segs, const int* __restrict__ bseg, const T* x, T* out,
                                                     unsigned lanes, MixVec mv) {
@@ -724,17 +723,27 @@ __global__ __launch_bounds__(kBlock) void k_egather(const Seg<T>* __restrict__ s
     } else if (main && row < sg.op.D) {
         vstore<T, NR>(out + i, erow<T, NR>(sg.op, row, xb, NR));
     }
-    if (row >= sg.op.D) return;
-    if (std::is_same<T, float>::value && (lanes & M_UPDATE) && seg_any(L, M_UPDATE)) {
-        for (int k = 0; k < sg.nr; ++k)
-            if (lane_in(L, k, M_UPDATE))
-                stg(&mv.tmp64[lm_at(sg, k, row)], erow<double, 1>(sg.op64, row, mv.x64 + lm_at(sg, k, 0), 1));
-    }
+}
+
+// FP64 gathers of the refinement lanes of FP32 segments: tmp64 = E64 x64 for
+// every lane in an UPDATE status (a launch of its own beside the FP32
+// gather, which then carries only FP32 work).
+template <int NR>
+__global__ __launch_bounds__(kBlock) void k_egather_upd(const Seg<float>* __restrict__ segs, const int* __restrict__ bseg,
+                                                        MixVec mv) {
+    int lb;
+    int64_t row;
+    const Seg<float>& sg = seg_of(segs, bseg, lb, row);
+    const auto L = lane_stat<NR>(sg);
+    if (!seg_any(L, M_UPDATE) || row >= sg.op.D) return;
+    for (int k = 0; k < sg.nr; ++k)
+        if (lane_in(L, k, M_UPDATE))
+            stg(&mv.tmp64[lm_at(sg, k, row)], erow<double, 1>(sg.op64, row, mv.x64 + lm_at(sg, k, 0), 1));
 }
 
 // E x for launches whose every segment has shared-stencil tiles: block lb of
-// a segment takes its tile lb (blocks past the last tile only do the FP64
-// gathers of refinement lanes).  40 KB of LDS and at most 128 VGPRs, so four
+// a segment takes its tile lb (blocks past the last tile have nothing to
+// do).  40 KB of LDS and at most 128 VGPRs, so four
 // workgroups share a CU (a direct-read path for other segments, even out of
 // line, made the FP32 x 8 kernel spill).
 template <typename T, int NR, int WPE>
@@ -752,12 +761,6 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(WPE))) v
     if (lanes && !seg_any(L, lanes)) return;               // uniform per workgroup
     const bool main = !lanes || seg_any(L, lanes & ~M_UPDATE);
     if (main && lb < sg.op.ngt) egather_grp_tile<T, NR, LR>(sg, lb, x + sg.off, out, lrows);
-    if (row >= sg.op.D) return;
-    if (std::is_same<T, float>::value && (lanes & M_UPDATE) && seg_any(L, M_UPDATE)) {
-        for (int k = 0; k < sg.nr; ++k)
-            if (lane_in(L, k, M_UPDATE))
-                stg(&mv.tmp64[lm_at(sg, k, row)], erow<double, 1>(sg.op64, row, mv.x64 + lm_at(sg, k, 0), 1));
-    }
 }
 
 // out = A x from tmp = E x.  RED = 0: no reduction; 1: partial (other, out);
@@ -1242,7 +1245,16 @@ void Launch<T, NR>::iteration(const LaunchCfg<T>& c, Vecs<T>& w, bool mixed, con
         apply(c, w.y, w.tmp, w.t, M_RUN);
         LAUNCH((k_jacobi<T, NR>), w.p, w.t, w.y, sw);
     }
-    EGATHER(w.y, w.tmp, M_STAGE1, mv);
+    if constexpr (std::is_same<T, float>::value) {
+        if (mixed) {   // FP32 lanes in the gather, refinement lanes' FP64 gathers in a launch of their own
+            EGATHER(w.y, w.tmp, M_RUN, mv);
+            LAUNCH((k_egather_upd<NR>), mv);
+        } else {
+            EGATHER(w.y, w.tmp, M_STAGE1, mv);
+        }
+    } else {
+        EGATHER(w.y, w.tmp, M_STAGE1, mv);
+    }
     LAUNCH((k_lfinish<T, NR, 1>), w.y, w.tmp, w.v, w.r0, (const T*)nullptr, M_STAGE1, mv);
     FIN(T, NR, FIN_ALPHA);
     LAUNCH((k_supdate<T, NR>), w.r, w.v, w.s, w.z, w.r0, w.p, mv);

@@ -183,3 +183,36 @@ def test_sphere_closed_form_surface_error(gpu_ctx):
         errs.append(np.abs(Ep.matvec(x64) - uex).max())
     print(f"max surface error h=0.1 {errs[0]:.2e}, h=0.05 {errs[1]:.2e}", flush=True)
     assert errs[1] < 2e-3 and errs[0] / errs[1] > 3.2
+
+
+@pytest.mark.parametrize("lanes", [8, 16])
+def test_stream_graph_replays_equal_eager(gpu_ctx, monkeypatch, lanes):
+    """Streamed solves with their iterations captured in a HIP graph and replayed between the host's
+    refills (CPMGPU_GRAPH=1) end bit for bit where the eager launches (CPMGPU_GRAPH=0) end, with the
+    same iteration counts: every replay runs the same kernels on the same tables."""
+    print(f"case stream graph lanes={lanes}", flush=True)
+    tube = cpmgpu.Tube(gpu_ctx, "Sphere", 0.05)
+    op = tube.operator(1.0, -1.0)
+    D = op.D
+    rng = np.random.default_rng(11)
+    B = [rng.standard_normal(D) * (1 + k) for k in range(2 * lanes + 3)]
+    d_b = [gpu_ctx.alloc(8 * D) for _ in B]
+    d_x = [gpu_ctx.alloc(8 * D) for _ in B]
+    for d, b in zip(d_b, B):
+        gpu_ctx.h2d(d, b)
+    o = cpmgpu.opts(tol=1e-10, precision=cpmgpu.CPM_FP32_REFINE)
+    monkeypatch.setenv("CPMGPU_FP64_ROUTING", "0")   # every call runs FP32 (random data: some systems fall back)
+    res = {}
+    for mode in ("0", "1", "1"):
+        monkeypatch.setenv("CPMGPU_GRAPH", mode)
+        infos = gpu_ctx.solve_stream_device(op, d_b, d_x, lanes, o)
+        xs = [gpu_ctx.d2h(np.zeros(D), d) for d in d_x]
+        assert all(i["status"] == "converged" for i in infos)
+        if mode in res:
+            assert [i["iters"] for i in infos] == [i["iters"] for i in res[mode][1]]
+            assert all(np.array_equal(a, b) for a, b in zip(xs, res[mode][0]))
+        res[mode] = (xs, infos)
+    assert [i["iters"] for i in res["1"][1]] == [i["iters"] for i in res["0"][1]]
+    assert all(np.array_equal(a, b) for a, b in zip(res["1"][0], res["0"][0]))
+    for d in d_b + d_x:
+        gpu_ctx.free(d)

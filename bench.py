@@ -86,38 +86,28 @@ def host_cpu():
                 omp_num_threads=os.environ.get("OMP_NUM_THREADS"))
 
 
-def cpu_baseline(work, args):
-    """The oracle (the C++ restatement of the reference solver: same tube, E, L, BiCGStab and
-    preconditioner) timed on the host, FP64 to the reference's 1e-10 stop, in the two variants of
-    the reference's threading: (i) as shipped, OpenMP vector updates with a serial operator apply
-    (OPENMP is never defined for GeneralizedBiCGStabSystem.h:71-72); (ii) OpenMP everywhere.  The
-    faster one is the baseline.  One warm-up, then the median of `--cpu-samples` samples of the
-    first `--cpu-iters` iterations of the same system (fewer for the slower as-shipped variant),
-    extrapolated linearly to the iteration count of the full solve (the GPU FP64 count of the same
-    algorithm), on every host thread this GPU's processes may use."""
+def cpu_rates(L, E, I, b, m, n, full_iters, args, what):
+    """The oracle's BiCGStab (the C++ restatement of the reference solver: same E, L and diagonal
+    preconditioner, FP64 to the reference's 1e-10 stop) timed on the host on the system (L, E, I),
+    in the two variants of the reference's threading: (i) as shipped, OpenMP vector updates with a
+    serial operator apply (OPENMP is never defined for GeneralizedBiCGStabSystem.h:71-72); (ii)
+    OpenMP everywhere.  The faster one is the baseline.  One warm-up, then the median of
+    `--cpu-samples` samples of the first `--cpu-iters` iterations (fewer for the slower as-shipped
+    variant), extrapolated linearly to `full_iters` (the GPU FP64 count of the same algorithm), on
+    every host thread this GPU's processes may use."""
     sys.path.insert(0, os.path.join(REPO, "oracle"))
     import orc
     host = host_cpu()
     threads = host["threads"]
-    surf, dx, params, bbox = work["surface"]
-    if "ibc_verts" in work:
-        P = orc.IBCProblem(surf, dx, params=params, bbox=bbox,
-                           ibcs=[orc.IBC("Polyline", verts=work["ibc_verts"], boundary_order=2,
-                                         on_ibc_tolerance=0.1 * dx * dx)])
-    else:
-        P = orc.Problem(surf, dx, params=params, bbox=bbox)
-    L, E, I = P.L(), P.E(), P.ident()
-    b = work["rhs"][0]
-    full_iters = work["fp64_iters"]
     rates, notes = {}, []
     for name, (vec, apply), iters in (("as_shipped", (True, False), max(10, args.cpu_iters // 3)),
                                       ("openmp_everywhere", (True, True), args.cpu_iters)):
         orc.set_parallel(vec, apply, threads)
-        orc.solve(L, E, I, b, work["m"], work["n"], tol=1e-10, max_it=3)   # warm-up
+        orc.solve(L, E, I, b, m, n, tol=1e-10, max_it=3)   # warm-up
         per_iter = []
         for _ in range(args.cpu_samples):
             t0 = time.perf_counter()
-            x, it, rr = orc.solve(L, E, I, b, work["m"], work["n"], tol=1e-10, max_it=min(full_iters, iters))
+            x, it, rr = orc.solve(L, E, I, b, m, n, tol=1e-10, max_it=min(full_iters, iters))
             per_iter.append((time.perf_counter() - t0) / max(it, 1))
         t_full = statistics.median(per_iter) * full_iters
         rates[name] = 1.0 / t_full
@@ -130,8 +120,23 @@ def cpu_baseline(work, args):
                 as_shipped_solves_per_s=rates["as_shipped"],
                 openmp_everywhere_solves_per_s=rates["openmp_everywhere"],
                 host_nproc=host["nproc"], host_cpu_model=host["model"], omp_num_threads=host["omp_num_threads"],
-                sample=f"FP64 BiCGStab on the same {P.D}-row system, {threads} host threads; "
+                sample=f"FP64 BiCGStab on {what}, {threads} host threads; "
                        f"{'; '.join(notes)}; extrapolated linearly to the full solve's {full_iters} iterations")
+
+
+def cpu_baseline(work, args):
+    """The oracle builds the same system as the GPU (tube, E, L, IBC rows) and cpu_rates times it."""
+    sys.path.insert(0, os.path.join(REPO, "oracle"))
+    import orc
+    surf, dx, params, bbox = work["surface"]
+    if "ibc_verts" in work:
+        P = orc.IBCProblem(surf, dx, params=params, bbox=bbox,
+                           ibcs=[orc.IBC("Polyline", verts=work["ibc_verts"], boundary_order=2,
+                                         on_ibc_tolerance=0.1 * dx * dx)])
+    else:
+        P = orc.Problem(surf, dx, params=params, bbox=bbox)
+    return cpu_rates(P.L(), P.E(), P.ident(), work["rhs"][0], work["m"], work["n"], work["fp64_iters"], args,
+                     f"the same {P.D}-row system")
 
 
 def egather_traffic(kernel, workload, groups=1):
@@ -383,6 +388,31 @@ def bench_serving(args, ctx, cpmgpu):
     th.join()
     srv.close()
     bctx.close()
+    # request 0's system again, on this context: the E gather the server's iterations run (FP32, 4
+    # interleaved lanes) against the HBM roofline, its FP64 iteration count, and the oracle's solve of
+    # the same matrices on the host
+    curves, cols = payload[0]
+    P0 = cpmgpu.Problem(tube, [cpmgpu.IBC("Polyline", verts=c, boundary_order=2, on_ibc_tolerance=0.1 * h * h)
+                               for c in curves])
+    op0 = P0.operator(0.0, -1.0)
+    b0 = np.zeros(P0.D)
+    for c in range(len(curves)):
+        _, _, r0, nr = P0.curve_rows(c)
+        b0[r0:r0 + nr] = cols[c, 0]
+    _, inf0 = op0.solve(b0, cpmgpu.opts(tol=1e-10, precision=cpmgpu.CPM_FP64))
+    eg_ms, eg_bytes = op0.profile_egather(cpmgpu.CPM_FP32_REFINE, 4, reps=30)
+    achieved = eg_bytes / (eg_ms * 1e-3) / 1e9
+    roof = dict(bound="hbm", achieved=round(achieved, 1), peak=HBM_PEAK_GBS, unit="GB/s",
+                frac=round(achieved / HBM_PEAK_GBS, 4), traffic=None,
+                kernel=op0.gather_kernel(cpmgpu.CPM_FP32_REFINE, 4), avg_launch_ms=round(eg_ms, 5),
+                bytes_per_launch=int(eg_bytes), on=f"request 0 ({P0.D} rows)")
+    cb = None
+    if not args.no_cpu:
+        L0, E0, I0 = P0.export()
+        cb = cpu_rates(L0, E0, I0, b0, 0.0, -1.0, inf0["iters"], args,
+                       f"request 0's {P0.D}-row system (matrices exported from the GPU build)")
+    op0.close()
+    P0.close()
     lat = sorted(r["latency_ms"] for r in rec.values())
     p99 = lat[min(len(lat) - 1, int(math.ceil(0.99 * len(lat))) - 1)]
     out = {
@@ -400,7 +430,8 @@ def bench_serving(args, ctx, cpmgpu):
         "iters_mean": round(float(np.mean([r["iters"] for r in rec.values()])), 1),
         "fp64_fallbacks": int(sum(r["fallback"] for r in rec.values())),
         "all_converged": all(r["ok"] for r in rec.values()),
-        "roofline": None, "cpu_baseline": None,
+        "fp64_iters_request0": inf0["iters"],
+        "roofline": roof, "cpu_baseline": cb,
     }
     print(json.dumps(out))
     ctx.close()
@@ -485,9 +516,9 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--batch", type=int, default=32,
-                    help="streamed lanes (1, 2, 4, 8, or 16 and 32 as groups of 8 in every launch)")
-    ap.add_argument("--stream", type=int, default=64, help="right-hand sides streamed through the lanes per step")
+    ap.add_argument("--batch", type=int, default=64,
+                    help="streamed lanes (1, 2, 4, 8, or 16, 32 and 64 as groups of 8 in every launch)")
+    ap.add_argument("--stream", type=int, default=128, help="right-hand sides streamed through the lanes per step")
     ap.add_argument("--workload", default="torus-knot", choices=["torus-knot", "sphere", "surfaces", "serving", "rd"])
     ap.add_argument("--requests", type=int, default=16, help="serving: requests")
     ap.add_argument("--qps", type=float, default=1.0, help="serving: mean arrival rate")
@@ -571,11 +602,11 @@ def main():
     inf64 = ctx.solve_batch_device([op], d_rhs[:1], d_x[:1], o64)[0]
     x64 = ctx.d2h(np.zeros(D), d_x[0])
 
-    # FP64 with the same batching: the step's S right-hand sides as consecutive batches of 4, 8 and
-    # 32 (FP64 groups 4 interleaved lanes per operator copy; it has no streamed mode: each batch waits
-    # for its slowest system)
+    # FP64 with the same batching: the step's S right-hand sides as consecutive batches of 4, 8, 32
+    # and 64 (FP64 groups 4 interleaved lanes per operator copy; it has no streamed mode: each batch
+    # waits for its slowest system)
     fp64_batched = {}
-    for lanes in (4, 8, 32):
+    for lanes in (4, 8, 32, 64):
         ctx.solve_batch_device([op] * lanes, d_rhs[:lanes], d_x[:lanes], o64)
         ctx.synchronize()
         t0 = time.perf_counter()

@@ -872,8 +872,8 @@ static void solve_stream(Op* op, const std::vector<const double*>& rhs, const st
     if (lanes == 1) run_stream<1>(op->ctx, op, rhs, xs, 1, o, infos);
     else if (lanes == 2) run_stream<2>(op->ctx, op, rhs, xs, 1, o, infos);
     else if (lanes == 4) run_stream<4>(op->ctx, op, rhs, xs, 1, o, infos);
-    else if (lanes == 8 || lanes == 16 || lanes == 32) run_stream<8>(op->ctx, op, rhs, xs, lanes / 8, o, infos);
-    else throw ArgError("lanes must be 1, 2, 4, 8, 16 or 32");
+    else if (lanes == 8 || lanes == 16 || lanes == 32 || lanes == 64) run_stream<8>(op->ctx, op, rhs, xs, lanes / 8, o, infos);
+    else throw ArgError("lanes must be 1, 2, 4, 8, 16, 32 or 64");
     const std::vector<Op*> ops(rhs.size(), op);
     fp64_fallback(op->ctx, ops, rhs, xs, o, infos, nullptr);
     record_fp32(ops, infos);

@@ -172,8 +172,8 @@ int cpm_solve_batch(cpm_ctx* ctx, int n_inst, cpm_op* const* ops, const double* 
 int cpm_solve_batch_device(cpm_ctx* ctx, int n_inst, cpm_op* const* ops, const double* const* d_rhs,
                            double* const* d_x, const cpm_solve_opts* opts, cpm_solve_info* infos);
 /* Streamed solves of one operator (continuous batching, FP32_REFINE only): the
- * n_rhs systems op x_i = rhs_i flow through `lanes` (1, 2, 4, 8, or 16 and 32
- * as 2 and 4 groups of 8 in every launch) interleaved lanes; a lane whose
+ * n_rhs systems op x_i = rhs_i flow through `lanes` (1, 2, 4, 8, or 16, 32 and
+ * 64 as 2, 4 and 8 groups of 8 in every launch) interleaved lanes; a lane whose
  * system has stopped takes the next
  * right-hand side at the next check (every 16 iterations), so a slow system
  * does not hold the others' lanes.  Device pointers, D doubles each.  No

@@ -306,8 +306,9 @@ def bench_serving(args, ctx, cpmgpu):
     for i in range(n):
         payload.append(workloads.serving_request(V, F, 4400 + i))
     print(f"{n} requests generated ({sum(len(c) for c, _ in payload)} curves)", file=sys.stderr, flush=True)
+    prec = cpmgpu.CPM_FP64 if args.serve_precision == "fp64" else cpmgpu.CPM_FP32_REFINE
     srv = cpmgpu.Server(ctx, int(args.pool_gb * (1 << 30)), lanes=4,
-                        o=cpmgpu.opts(tol=1e-10, precision=cpmgpu.CPM_FP32_REFINE, update_delta=args.update_delta))
+                        o=cpmgpu.opts(tol=1e-10, precision=prec, update_delta=args.update_delta))
     # A build thread with its own context builds each request's problem, operator and right-hand sides
     # when it arrives (its kernels run on its own stream beside the iterations) and releases finished
     # requests; this thread admits built requests and drives the iterations.
@@ -400,11 +401,11 @@ def bench_serving(args, ctx, cpmgpu):
         _, _, r0, nr = P0.curve_rows(c)
         b0[r0:r0 + nr] = cols[c, 0]
     _, inf0 = op0.solve(b0, cpmgpu.opts(tol=1e-10, precision=cpmgpu.CPM_FP64))
-    eg_ms, eg_bytes = op0.profile_egather(cpmgpu.CPM_FP32_REFINE, 4, reps=30)
+    eg_ms, eg_bytes = op0.profile_egather(prec, 4, reps=30)
     achieved = eg_bytes / (eg_ms * 1e-3) / 1e9
     roof = dict(bound="hbm", achieved=round(achieved, 1), peak=HBM_PEAK_GBS, unit="GB/s",
                 frac=round(achieved / HBM_PEAK_GBS, 4), traffic=None,
-                kernel=op0.gather_kernel(cpmgpu.CPM_FP32_REFINE, 4), avg_launch_ms=round(eg_ms, 5),
+                kernel=op0.gather_kernel(prec, 4), avg_launch_ms=round(eg_ms, 5),
                 bytes_per_launch=int(eg_bytes), on=f"request 0 ({P0.D} rows)")
     cb = None
     if not args.no_cpu:
@@ -416,10 +417,11 @@ def bench_serving(args, ctx, cpmgpu):
     lat = sorted(r["latency_ms"] for r in rec.values())
     p99 = lat[min(len(lat) - 1, int(math.ceil(0.99 * len(lat))) - 1)]
     out = {
-        "metric": "serving: solves/s with p50/p99 request latency, FP32+refine at 1e-10",
+        "metric": f"serving: solves/s with p50/p99 request latency, "
+                  f"{'FP64' if args.serve_precision == 'fp64' else 'FP32+refine'} at 1e-10",
         "value": round(3 * n / makespan, 3), "unit": "solves/s", "n_gpus": 1, "steps": n, "warmup": 0,
         "ms_per_step": round(1000 * makespan / n, 1), "higher_is_better": True, "scaling": "weak",
-        "vs_baseline": None, "dtype": "f32 Krylov + f64 refinement",
+        "vs_baseline": None, "dtype": "f64" if args.serve_precision == "fp64" else "f32 Krylov + f64 refinement",
         "data": "synthetic (seeded displaced icosphere mesh, plane-section curves, random RGB)",
         "config": {"workload": "serving loop: displaced icosphere 81,920 triangles, h=0.01, Poisson arrivals, "
                                "1-8 curves x 3 RHS per request", "tube_nodes": tube.N, "requests": n,
@@ -523,6 +525,8 @@ def main():
     ap.add_argument("--requests", type=int, default=16, help="serving: requests")
     ap.add_argument("--qps", type=float, default=1.0, help="serving: mean arrival rate")
     ap.add_argument("--pool-gb", type=float, default=8.0, help="serving: workspace pool")
+    ap.add_argument("--serve-precision", default="fp32refine", choices=["fp32refine", "fp64"],
+                    help="serving: precision of the request loop")
     ap.add_argument("--cpu-iters", type=int, default=100, help="iterations per CPU-baseline sample")
     ap.add_argument("--cpu-samples", type=int, default=5, help="CPU-baseline samples (the median is reported)")
     ap.add_argument("--surfaces-steps", type=int, default=2,

@@ -1473,6 +1473,221 @@ struct Server : ServerBase {
     }
 };
 
+// The same request loop in FP64 (the reference's arithmetic, no refinement):
+// each admitted request is one segment of the FP64 iteration.  A request's
+// lanes start with the FP64 BiCGStab initialisation of its own segment
+// (r = r0 = b, x = p = v = 0, rho = ||b||^2), launched for that segment alone
+// when it is admitted; rho-restarts (BiCGStab.h:62-71) are taken between
+// chunks of iterations.  Workspace: 11 FP64 vectors, 88 B per row and lane.
+template <int NR>
+struct Server64 : ServerBase {
+    struct Req {
+        int64_t tag;
+        Op* op;
+        int nr;
+        int64_t off, len;
+        int slot;
+        std::vector<double*> xs;
+    };
+    Ctx* ctx;
+    cpm_solve_opts o;
+    int64_t cap;
+    Buffer p64, states, tables, init_tables;
+    int slots;
+    std::vector<int> free_slots;
+    RangePool ranges;
+    std::vector<Req> reqs;
+    bool dirty = true;
+    int nblocks = 0;
+    Seg<double>* d_segs = nullptr;
+    int* d_bseg = nullptr;
+    double* d_part = nullptr;
+    PackEntry* d_pack = nullptr;
+    int* d_zmap = nullptr;
+    size_t tables_bytes = 0;
+
+    Server64(Ctx* c, int64_t pool_bytes, const cpm_solve_opts& opts) : ctx(c), o(opts) {
+        lanes = NR;
+        cap = (pool_bytes / (11 * 8)) & ~int64_t(63);
+        if (cap < 64) throw ArgError("server pool too small");
+        p64 = Buffer(sizeof(double) * (size_t)cap * 11);
+        slots = Ctx::kHostStates / kMaxNR;
+        states = Buffer(sizeof(KState) * (size_t)slots * kMaxNR);
+        for (int i = slots - 1; i >= 0; --i) free_slots.push_back(i);
+        ranges.reset(cap);
+        const int zn = (int)((cap + kBlock - 1) / kBlock);
+        // single-segment tables of the admission launches: segment, partials, pack entry, zero block map
+        init_tables = Buffer(4096 + sizeof(double) * 2 * NR * (size_t)zn + 4096 + sizeof(int) * (size_t)zn);
+        HIPCHK(hipMemset(init_tables.p, 0, init_tables.bytes));
+    }
+    double* v64(int k) const { return p64.template as<double>() + (size_t)k * cap; }   // b x r r0 p v s t y z tmp
+    int active() const override { return (int)reqs.size(); }
+    Seg<double>* i_seg() const { return init_tables.template as<Seg<double>>(); }
+    double* i_part() const { return reinterpret_cast<double*>(init_tables.template as<char>() + 4096); }
+    PackEntry* i_pack() const {
+        const int zn = (int)((cap + kBlock - 1) / kBlock);
+        return reinterpret_cast<PackEntry*>(init_tables.template as<char>() + 4096 + sizeof(double) * 2 * NR * (size_t)zn);
+    }
+    int* i_zmap() const { return reinterpret_cast<int*>(reinterpret_cast<char*>(i_pack()) + 4096); }
+
+    // one pack launch per lane of the interleaved FP64 vectors: b from the caller, or x to the caller
+    void lane_pack(const Req& q, int k, const double* src, double* ws, int to_ws) {
+        const hipStream_t st = ctx->stream;
+        PackEntry e{};
+        e.src = to_ws ? src : nullptr;
+        e.dst = to_ws ? nullptr : (k < q.nr ? q.xs[(size_t)k] : nullptr);
+        e.perm = q.op->perm.template as<int>();
+        e.D = q.op->D;
+        e.off = q.off;
+        e.lane = k;
+        e.nr = NR;
+        e.ls = 1;
+        e.block0 = 0;
+        HIPCHK(hipMemcpyAsync(i_pack(), &e, sizeof(e), hipMemcpyHostToDevice, st));
+        const int nb = (int)((q.op->D + kBlock - 1) / kBlock);
+        hipLaunchKernelGGL(k_pack, dim3((unsigned)nb), dim3(256), 0, st, i_pack(), i_zmap(), ws, (float*)nullptr, to_ws);
+        HIPCHK(hipStreamSynchronize(st));   // the entry is rewritten by the next call
+    }
+    void ensure_tables() {
+        int blocks = 0;
+        for (auto& q : reqs) blocks += (int)((q.op->D + kBlock - 1) / kBlock);
+        auto r256 = [](size_t k) { return (k + 255) & ~size_t(255); };
+        const size_t sseg = r256(sizeof(Seg<double>) * std::max<size_t>(reqs.size(), 1));
+        const size_t smap = r256(sizeof(int) * std::max(blocks, 1));
+        const size_t spart = r256(sizeof(double) * 2 * NR * std::max(blocks, 1));
+        const size_t need = sseg + smap + spart;
+        if (need > tables_bytes) {
+            HIPCHK(hipStreamSynchronize(ctx->stream));
+            tables = Buffer(need * 2);
+            tables_bytes = need * 2;
+        }
+        char* t = tables.template as<char>();
+        d_segs = reinterpret_cast<Seg<double>*>(t);
+        d_bseg = reinterpret_cast<int*>(t + sseg);
+        d_part = reinterpret_cast<double*>(t + sseg + smap);
+    }
+
+    bool submit(Op* op, int nrhs, const double* const* rhs, double* const* xs, int64_t tag) override {
+        if (nrhs < 1 || nrhs > NR) throw ArgError("a request carries 1.." + std::to_string(NR) + " right-hand sides");
+        if (op->ctx->device != ctx->device) throw ArgError("operator lives on another device");
+        if (op->nD != 0) throw ArgError("served operators have no trailing nD rows");
+        if (op->precond == CPM_PRECOND_JACOBI) throw ArgError("served operators use the diagonal or no preconditioner");
+        const int64_t len = (op->D * NR + 63) & ~int64_t(63);
+        if (len > cap) throw ArgError("request larger than the server pool");
+        ensure_group_tiles_for({op}, 8, NR);   // before its view is taken
+        int64_t off;
+        if (free_slots.empty() || !ranges.take(len, off)) return false;   // wait for memory
+        Req q{tag, op, nrhs, off, len, free_slots.back(), {}};
+        free_slots.pop_back();
+        q.xs.assign(xs, xs + nrhs);
+        const hipStream_t st = ctx->stream;
+        for (int k = 0; k < NR; ++k) lane_pack(q, k, k < nrhs ? rhs[k] : nullptr, v64(0), 1);   // b, idle lanes 0
+        // states of the request's lanes: fresh (FIN_INIT completes them) or stopped (idle lanes)
+        std::vector<KState> hs(kMaxNR);
+        for (int k = 0; k < kMaxNR; ++k) {
+            KState z{};
+            z.tol2 = o.tol * o.tol;
+            z.max_it = o.max_iters;
+            z.status = k < nrhs ? ST_RUN : ST_CONVERGED;
+            hs[(size_t)k] = z;
+        }
+        KState* sts = states.template as<KState>() + (size_t)q.slot * kMaxNR;
+        HIPCHK(hipMemcpyAsync(sts, hs.data(), sizeof(KState) * kMaxNR, hipMemcpyHostToDevice, st));
+        // BiCGStab initialisation of this request's segment alone
+        const int nb = (int)((op->D + kBlock - 1) / kBlock);
+        Seg<double> h{};
+        fill_seg(h, op, off, 0, nb, nrhs, sts, i_part());
+        HIPCHK(hipMemcpyAsync(i_seg(), &h, sizeof(h), hipMemcpyHostToDevice, st));
+        const LaunchCfg<double> c1{i_seg(), nullptr, nb, 1, st, 0, op->ngt};
+        Launch<double, NR>::init(c1, v64(0), v64(1), v64(2), v64(3), v64(4), v64(5));
+        HIPCHK(hipGetLastError());
+        HIPCHK(hipStreamSynchronize(st));
+        reqs.push_back(q);
+        dirty = true;
+        return true;
+    }
+
+    void rebuild() {
+        ensure_tables();
+        std::vector<Seg<double>> h(reqs.size());
+        std::vector<int> map;
+        int b0 = 0;
+        for (size_t g = 0; g < reqs.size(); ++g) {
+            const Req& q = reqs[g];
+            const int nb = (int)((q.op->D + kBlock - 1) / kBlock);
+            fill_seg(h[g], q.op, q.off, b0, nb, q.nr, states.template as<KState>() + (size_t)q.slot * kMaxNR,
+                     d_part + (size_t)2 * NR * b0);
+            for (int b = 0; b < nb; ++b) map.push_back((int)g);
+            b0 += nb;
+        }
+        nblocks = b0;
+        const hipStream_t st = ctx->stream;
+        HIPCHK(hipMemcpyAsync(d_segs, h.data(), sizeof(Seg<double>) * h.size(), hipMemcpyHostToDevice, st));
+        HIPCHK(hipMemcpyAsync(d_bseg, map.data(), sizeof(int) * map.size(), hipMemcpyHostToDevice, st));
+        HIPCHK(hipStreamSynchronize(st));
+        dirty = false;
+    }
+
+    void run(int iters) override {
+        if (reqs.empty()) return;
+        if (dirty) rebuild();
+        const hipStream_t st = ctx->stream;
+        Vecs<double> V{v64(0), v64(1), v64(2), v64(3), v64(4), v64(5), v64(6), v64(7), v64(8), v64(9), v64(10)};
+        int grp = INT_MAX;
+        for (const Req& q : reqs) grp = std::min(grp, q.op->ngt);
+        const LaunchCfg<double> c{d_segs, d_bseg, nblocks, (int)reqs.size(), st, 0, grp};
+        for (int i = 0; i < iters; ++i) Launch<double, NR>::iteration(c, V, false, MixVec{});
+        HIPCHK(hipGetLastError());
+        HIPCHK(hipMemcpyAsync(ctx->host_states, states.template as<KState>(), sizeof(KState) * (size_t)slots * kMaxNR,
+                              hipMemcpyDeviceToHost, st));
+        HIPCHK(hipStreamSynchronize(st));
+        const std::vector<KState> snap(ctx->host_states, ctx->host_states + (size_t)slots * kMaxNR);
+        const KState* hs = snap.data();
+        bool restart = false;
+        std::vector<Req> keep, leaving;
+        for (Req& q : reqs) {
+            const KState* s = hs + (size_t)q.slot * kMaxNR;
+            bool running = false;
+            for (int k = 0; k < q.nr; ++k) {
+                running |= s[k].status == ST_RUN || s[k].status == ST_RESTART;
+                restart |= s[k].status == ST_RESTART;
+            }
+            (running ? keep : leaving).push_back(q);
+        }
+        if (restart) {   // rare: the restart kernels act on the lanes in ST_RESTART only
+            Launch<double, NR>::restart(c, V);
+            HIPCHK(hipGetLastError());
+        }
+        reqs.swap(keep);
+        if (leaving.empty()) return;
+        dirty = true;
+        auto finish = [&](const Req& q) {
+            const KState* s = hs + (size_t)q.slot * kMaxNR;
+            Done d{q.tag, std::vector<cpm_solve_info>((size_t)q.nr)};
+            for (int k = 0; k < q.nr; ++k) {
+                cpm_solve_info& inf = d.infos[(size_t)k];
+                inf.status = s[k].status;
+                inf.iters = s[k].iters;
+                inf.restarts = s[k].restarts;
+                inf.rel_residual = s[k].bsq > 0 ? std::sqrt(s[k].rr / s[k].bsq) : 0.0;
+            }
+            try {
+                for (int k = 0; k < q.nr; ++k) lane_pack(q, k, nullptr, v64(1), 0);
+            } catch (...) {
+                for (auto& inf : d.infos) inf.status = CPM_SOLVE_NOT_WRITTEN;
+                done.push_back(std::move(d));
+                throw;
+            }
+            done.push_back(std::move(d));
+        };
+        auto release = [&](const Req& q) {
+            if (!ranges.give(q.off, q.len)) throw std::logic_error("server range given back twice");
+            free_slots.push_back(q.slot);
+        };
+        retire_requests(leaving, finish, release);
+    }
+};
+
 }  // namespace cpmgpu
 
 // ====================================================================== C ABI
@@ -1899,13 +2114,22 @@ int cpm_server_create(cpm_ctx* ctx, int64_t pool_bytes, int lanes, const cpm_sol
         if (!ctx || !out) throw ArgError("null argument");
         HIPCHK(hipSetDevice(C(ctx)->device));
         cpm_solve_opts o = resolve_opts(opts);
-        o.precision = CPM_FP32_REFINE;
+        if (!opts) o.precision = CPM_FP32_REFINE;   // the server's default precision
         ServerBase* s = nullptr;
-        if (lanes == 1) s = new Server<1>(C(ctx), pool_bytes, o);
-        else if (lanes == 2) s = new Server<2>(C(ctx), pool_bytes, o);
-        else if (lanes == 4) s = new Server<4>(C(ctx), pool_bytes, o);
-        else if (lanes == 8) s = new Server<8>(C(ctx), pool_bytes, o);
-        else throw ArgError("lanes must be 1, 2, 4 or 8");
+        if (o.precision == CPM_FP64) {
+            if (lanes == 1) s = new Server64<1>(C(ctx), pool_bytes, o);
+            else if (lanes == 2) s = new Server64<2>(C(ctx), pool_bytes, o);
+            else if (lanes == 4) s = new Server64<4>(C(ctx), pool_bytes, o);
+            else if (lanes == 8) s = new Server64<8>(C(ctx), pool_bytes, o);
+            else throw ArgError("lanes must be 1, 2, 4 or 8");
+        } else {
+            o.precision = CPM_FP32_REFINE;
+            if (lanes == 1) s = new Server<1>(C(ctx), pool_bytes, o);
+            else if (lanes == 2) s = new Server<2>(C(ctx), pool_bytes, o);
+            else if (lanes == 4) s = new Server<4>(C(ctx), pool_bytes, o);
+            else if (lanes == 8) s = new Server<8>(C(ctx), pool_bytes, o);
+            else throw ArgError("lanes must be 1, 2, 4 or 8");
+        }
         *out = reinterpret_cast<cpm_server*>(s);
     });
 }

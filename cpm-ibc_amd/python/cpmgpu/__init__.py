@@ -313,9 +313,10 @@ class Context:
 
 
 class Server:
-    """Continuous batching across operators (FP32 Krylov + FP64 refinement): requests (an operator
-    and 1..lanes device right-hand sides) join the running iteration as pool memory allows and
-    leave when they stop; results are bit-identical alone and in any mix."""
+    """Continuous batching across operators (FP32 Krylov + FP64 refinement, or FP64 with
+    o.precision = CPM_FP64): requests (an operator and 1..lanes device right-hand sides) join the
+    running iteration as pool memory allows and leave when they stop; results are bit-identical
+    alone and in any mix."""
 
     def __init__(self, ctx, pool_bytes, lanes=4, o=None):
         self.ctx, self.lanes = ctx, lanes

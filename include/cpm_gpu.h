@@ -184,7 +184,9 @@ int cpm_solve_stream_device(cpm_op* op, int n_rhs, const double* const* d_rhs, d
 /* ---- serving: continuous batching across operators --------------------
  * No counterpart in the reference, which runs one Solver::Solve at a time
  * (src/Solver.cpp:69-114).  A server keeps a pool of FP32+refine workspace
- * (pool_bytes: 80 B per row and lane) on the context's device and iterates
+ * (pool_bytes: 80 B per row and lane; opts->precision = CPM_FP64 selects an
+ * FP64 server instead: the reference's arithmetic, 88 B per row and lane, no
+ * refinement and no re-solves; opts = NULL: FP32+refine) on the context's device and iterates
  * every admitted request together: a request is an operator with 1..lanes
  * right-hand sides (device vectors, caller row order, kept valid until the
  * request completes).  cpm_server_submit admits a request when a slot of

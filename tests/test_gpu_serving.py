@@ -38,10 +38,10 @@ def _requests(ctx, tube, n):
     return reqs
 
 
-def _serve(ctx, reqs, pool_bytes, order):
+def _serve(ctx, reqs, pool_bytes, order, o=None):
     """Submits requests in `order` (None = run 8 iterations), retrying the ones not admitted; returns
     {index: [x per rhs]}, their infos and how often a submit was refused."""
-    srv = cpmgpu.Server(ctx, pool_bytes, lanes=4)
+    srv = cpmgpu.Server(ctx, pool_bytes, lanes=4, o=o)
     dev = {}
     for i, (P, op, rhs) in enumerate(reqs):
         dr = [ctx.alloc(8 * P.D) for _ in rhs]
@@ -98,6 +98,32 @@ def test_results_same_alone_and_in_the_mix(gpu_ctx):
         xs, _ = gpu_ctx.solve_batch([op] * 3, rhs, cpmgpu.opts(tol=1e-12, precision=cpmgpu.CPM_FP64))
         for a, r in zip(mix[i], xs):
             assert np.linalg.norm(a - r) <= 1e-8 * np.linalg.norm(r)
+
+
+def test_fp64_server_equals_batch_solves(gpu_ctx):
+    """The FP64 server (opts precision CPM_FP64): each request's results are bit-identical alone, in a
+    mix through a pool that holds two of four requests, and to a plain FP64 batch solve of the same
+    operator with the same right-hand sides (the same kernels on the same segment), with the same
+    iteration counts."""
+    print("case fp64 serving", flush=True)
+    V, F = workloads.icosphere(3)
+    tube = cpmgpu.Tube(gpu_ctx, "Triangulation", H, verts=V, faces=F)
+    reqs = _requests(gpu_ctx, tube, 4)
+    o64 = cpmgpu.opts(tol=1e-10, precision=cpmgpu.CPM_FP64)
+    per_req = max(((P.D * 4 + 63) // 64) * 64 for P, _, _ in reqs) * 88
+    alone = {}
+    for i in range(len(reqs)):
+        x, inf, _ = _serve(gpu_ctx, reqs, per_req + 1024, [i], o64)
+        assert all(j["status"] == "converged" for j in inf[i]), inf[i]
+        alone[i] = (x[i], inf[i])
+    mix, infos, busy = _serve(gpu_ctx, reqs, int(2.3 * per_req), [0, 1, None, 2, 3, None, None], o64)
+    print(f"refused submits while the pool was full: {busy}", flush=True)
+    assert busy >= 1
+    for i, (P, op, rhs) in enumerate(reqs):
+        xs, bi = gpu_ctx.solve_batch([op] * 3, rhs, o64)
+        assert [j["iters"] for j in infos[i]] == [j["iters"] for j in bi] == [j["iters"] for j in alone[i][1]]
+        for a, m, r in zip(alone[i][0], mix[i], xs):
+            assert np.array_equal(a, m) and np.array_equal(a, r)
 
 
 def test_submit_rejections(gpu_ctx):

@@ -13,12 +13,13 @@
 namespace cpmgpu {
 
 // Binary BVH node, 64 bytes.  count > 0: leaf holding prim[first .. first+count);
-// count == 0: internal node whose children are nodes first and first + 1.
+// count == 0: internal node whose children are nodes first and second.
 struct BvhNode {
     double lo[3], hi[3];
     int first;
     int count;
-    int pad[2];
+    int second;
+    int pad;
 };
 
 struct MeshDev {
@@ -39,8 +40,18 @@ struct MeshBvh {
     MeshDev dev() const;
 };
 
-// Validates the mesh, builds the hierarchy and uploads everything.
-void mesh_build(MeshBvh& m, const std::vector<double>& verts, const std::vector<int>& faces);
+// Validates the mesh, builds the hierarchy and uploads everything: on the GPU
+// (linear BVH from Morton codes, mesh_build_gpu) for meshes of at least
+// kGpuBvhTriangles triangles, on the host (binned SAH) for smaller ones or
+// when the GPU tree is deeper than the traversal stack.  CPMGPU_BVH=host /
+// gpu forces one builder.
+void mesh_build(MeshBvh& m, const std::vector<double>& verts, const std::vector<int>& faces,
+                hipStream_t stream = nullptr);
+constexpr int kGpuBvhTriangles = 16384;
+// The host builder alone.
+void mesh_build_host(MeshBvh& m, const std::vector<double>& verts, const std::vector<int>& faces);
+// The GPU builder alone; false (m untouched) if its tree is too deep.
+bool mesh_build_gpu(MeshBvh& m, const std::vector<double>& verts, const std::vector<int>& faces, hipStream_t stream);
 // The same over the segments of a polyline (vertices k, k + 1; f stays empty).
 void segment_build(MeshBvh& m, const std::vector<double>& verts);
 

@@ -1,8 +1,11 @@
 // Triangle-mesh closest points: host BVH builder and device traversal.
 #include <hip/hip_runtime.h>
 
+#include <hipcub/hipcub.hpp>
+
 #include <algorithm>
 #include <cmath>
+#include <cstdlib>
 #include <numeric>
 #include <string>
 
@@ -136,6 +139,7 @@ int build_nodes(const std::vector<Box>& tb, std::vector<int>& prim, std::vector<
         nodes.push_back(BvhNode{});
         nodes.push_back(BvhNode{});
         nodes[(size_t)it.node].first = left;
+        nodes[(size_t)it.node].second = left + 1;
         nodes[(size_t)it.node].count = 0;
         work.push_back({left + 1, mid, it.end, it.depth + 1});
         work.push_back({left, it.start, mid, it.depth + 1});
@@ -160,13 +164,27 @@ double coordinate_scale(const std::vector<double>& verts) {
 
 }  // namespace
 
-void mesh_build(MeshBvh& m, const std::vector<double>& verts, const std::vector<int>& faces) {
+static void check_mesh(const std::vector<double>& verts, const std::vector<int>& faces) {
     if (verts.size() % 3 || faces.size() % 3) throw ArgError("mesh arrays must hold 3 values per vertex / face");
     const int64_t nv = (int64_t)verts.size() / 3, nf = (int64_t)faces.size() / 3;
     if (nv < 3 || nf < 1) throw ArgError("mesh needs at least one triangle");
     if (nf > (1 << 28)) throw ArgError("too many triangles");
     for (int x : faces)
         if (x < 0 || x >= nv) throw ArgError("face index out of range");
+}
+
+void mesh_build(MeshBvh& m, const std::vector<double>& verts, const std::vector<int>& faces, hipStream_t stream) {
+    check_mesh(verts, faces);
+    const char* e = std::getenv("CPMGPU_BVH");
+    const std::string mode = e ? e : "";
+    const bool gpu = mode == "gpu" || (mode != "host" && (int64_t)faces.size() / 3 >= kGpuBvhTriangles);
+    if (gpu && mesh_build_gpu(m, verts, faces, stream)) return;
+    mesh_build_host(m, verts, faces);
+}
+
+void mesh_build_host(MeshBvh& m, const std::vector<double>& verts, const std::vector<int>& faces) {
+    check_mesh(verts, faces);
+    const int64_t nf = (int64_t)faces.size() / 3;
     const double scale = coordinate_scale(verts);
     const int F = (int)nf;
     std::vector<Box> tb((size_t)F);
@@ -187,6 +205,185 @@ void mesh_build(MeshBvh& m, const std::vector<double>& verts, const std::vector<
     // A triangle's computed distance is within a few ulps of coordinate size
     // of the true one, which is at least the box distance.
     m.slack = 1e-12 * (1.0 + scale);
+}
+
+// ---------------------------------------------------------------- GPU builder
+// A linear BVH (Karras, "Maximizing parallelism in the construction of BVHs,
+// octrees, and k-d trees", HPG 2012): 30-bit Morton codes of the triangles'
+// box centres in the vertices' bounding box, made unique by the triangle
+// index in the low 32 bits and radix-sorted; internal node i (0 = root) covers
+// the key range its common-prefix search finds and splits it where the prefix
+// changes; leaves n - 1 + k hold one triangle each; boxes are filled bottom-up
+// (the second child to arrive at a node forms its box).  Queries stay exact:
+// they depend on the boxes being conservative, not on the tree's shape.
+namespace {
+
+__device__ __forceinline__ unsigned expand10(unsigned x) {   // bits 0..9 to every third bit
+    x &= 0x3ffu;
+    x = (x | (x << 16)) & 0x030000ffu;
+    x = (x | (x << 8)) & 0x0300f00fu;
+    x = (x | (x << 4)) & 0x030c30c3u;
+    x = (x | (x << 2)) & 0x09249249u;
+    return x;
+}
+
+__device__ __forceinline__ void tri_box(const double* v, const int* f, int t, double* lo, double* hi) {
+    for (int d = 0; d < 3; ++d) {
+        lo[d] = INFINITY;
+        hi[d] = -INFINITY;
+    }
+    for (int k = 0; k < 3; ++k) {
+        const double* p = v + 3 * (int64_t)f[3 * (int64_t)t + k];
+        for (int d = 0; d < 3; ++d) {
+            lo[d] = fmin(lo[d], p[d]);
+            hi[d] = fmax(hi[d], p[d]);
+        }
+    }
+}
+
+__global__ void k_tri_keys(const double* v, const int* f, int F, double3 lo, double3 scale, unsigned long long* keys) {
+    const int t = (int)(blockIdx.x * blockDim.x + threadIdx.x);
+    if (t >= F) return;
+    double a[3], b[3];
+    tri_box(v, f, t, a, b);
+    const double c[3] = {0.5 * (a[0] + b[0]), 0.5 * (a[1] + b[1]), 0.5 * (a[2] + b[2])};
+    const double l[3] = {lo.x, lo.y, lo.z}, s[3] = {scale.x, scale.y, scale.z};
+    unsigned q[3];
+    for (int d = 0; d < 3; ++d) q[d] = (unsigned)fmin(1023.0, fmax(0.0, (c[d] - l[d]) * s[d]));
+    const unsigned long long code = ((unsigned long long)expand10(q[0]) << 2) | ((unsigned long long)expand10(q[1]) << 1) |
+                                    (unsigned long long)expand10(q[2]);
+    keys[t] = (code << 32) | (unsigned)t;
+}
+
+// Length of the common prefix of keys i and j (-1 outside [0, n)); keys are unique.
+__device__ __forceinline__ int prefix(const unsigned long long* k, int n, int i, int j) {
+    if (j < 0 || j >= n) return -1;
+    return __clzll(k[i] ^ k[j]);
+}
+
+__global__ void k_lbvh_nodes(const unsigned long long* k, int n, BvhNode* nodes, int* parent, int* prim) {
+    const int i = (int)(blockIdx.x * blockDim.x + threadIdx.x);
+    if (i < n) prim[i] = (int)(k[i] & 0xffffffffull);
+    if (i >= n - 1) return;
+    const int d = prefix(k, n, i, i + 1) - prefix(k, n, i, i - 1) >= 0 ? 1 : -1;
+    const int pmin = prefix(k, n, i, i - d);
+    int lmax = 2;
+    while (prefix(k, n, i, i + lmax * d) > pmin) lmax *= 2;
+    int l = 0;
+    for (int t = lmax / 2; t >= 1; t /= 2)
+        if (prefix(k, n, i, i + (l + t) * d) > pmin) l += t;
+    const int j = i + l * d;
+    const int pnode = prefix(k, n, i, j);
+    int s = 0;
+    for (int div = 2;; div *= 2) {
+        const int t = (l + div - 1) / div;
+        if (prefix(k, n, i, i + (s + t) * d) > pnode) s += t;
+        if (t <= 1) break;
+    }
+    const int gamma = i + s * d + (d < 0 ? -1 : 0);
+    const int left = min(i, j) == gamma ? n - 1 + gamma : gamma;
+    const int right = max(i, j) == gamma + 1 ? n - 1 + gamma + 1 : gamma + 1;
+    nodes[i].first = left;
+    nodes[i].second = right;
+    nodes[i].count = 0;
+    parent[left] = i;
+    parent[right] = i;
+}
+
+// Leaf boxes, then each node's box from its children's (the second child to arrive computes it).
+__global__ void k_lbvh_boxes(const double* v, const int* f, const int* prim, int n, const int* parent, int* arrived,
+                             BvhNode* nodes) {
+    const int k = (int)(blockIdx.x * blockDim.x + threadIdx.x);
+    if (k >= n) return;
+    int cur = n - 1 + k;
+    BvhNode& leaf = nodes[cur];
+    tri_box(v, f, prim[k], leaf.lo, leaf.hi);
+    leaf.first = k;
+    leaf.count = 1;
+    leaf.second = -1;
+    while (cur != 0) {
+        const int p = parent[cur];
+        if (p < 0) return;   // not reached: every node but the root has a parent
+        __threadfence();   // this subtree's boxes before the arrival count (other XCDs read them)
+        if (atomicAdd(&arrived[p], 1) == 0) return;
+        __threadfence();   // the sibling's boxes after it
+        BvhNode& np = nodes[p];
+        const BvhNode& a = nodes[np.first];
+        const BvhNode& b = nodes[np.second];
+        for (int d = 0; d < 3; ++d) {
+            np.lo[d] = fmin(a.lo[d], b.lo[d]);
+            np.hi[d] = fmax(a.hi[d], b.hi[d]);
+        }
+        cur = p;
+    }
+}
+
+__global__ void k_lbvh_depth(const int* parent, int n, int* depth) {
+    const int k = (int)(blockIdx.x * blockDim.x + threadIdx.x);
+    if (k >= n) return;
+    int cur = n - 1 + k, d = 0;
+    while (cur > 0 && d <= kBvhStack) {
+        cur = parent[cur];
+        ++d;
+    }
+    if (cur < 0) d = kBvhStack + 1;   // not reached: a node without a parent rejects the tree
+    atomicMax(depth, d);
+}
+
+}  // namespace
+
+bool mesh_build_gpu(MeshBvh& m, const std::vector<double>& verts, const std::vector<int>& faces, hipStream_t st) {
+    check_mesh(verts, faces);
+    const double scale = coordinate_scale(verts);
+    const int F = (int)(faces.size() / 3);
+    if (F < 2) return false;
+    double lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
+    for (size_t i = 0; i < verts.size(); i += 3)
+        for (int d = 0; d < 3; ++d) {
+            lo[d] = std::min(lo[d], verts[i + d]);
+            hi[d] = std::max(hi[d], verts[i + d]);
+        }
+    double sc[3];
+    for (int d = 0; d < 3; ++d) sc[d] = hi[d] > lo[d] ? 1024.0 / (hi[d] - lo[d]) : 0.0;
+    Buffer v, f, nodes, prim;
+    upload(v, verts.data(), verts.size() * sizeof(double));
+    upload(f, faces.data(), faces.size() * sizeof(int));
+    const int NN = 2 * F - 1;
+    nodes = Buffer(sizeof(BvhNode) * (size_t)NN);
+    prim = Buffer(sizeof(int) * (size_t)F);
+    Buffer keys(sizeof(unsigned long long) * (size_t)F * 2), parent(sizeof(int) * (size_t)NN),
+        arrived(sizeof(int) * (size_t)F + 64);
+    auto* k_in = keys.as<unsigned long long>();
+    auto* k_out = k_in + F;
+    int* depth = arrived.as<int>() + F;
+    const unsigned nb = (unsigned)((F + 255) / 256);
+    hipLaunchKernelGGL(k_tri_keys, dim3(nb), dim3(256), 0, st, v.as<double>(), f.as<int>(), F,
+                       make_double3(lo[0], lo[1], lo[2]), make_double3(sc[0], sc[1], sc[2]), k_in);
+    size_t tmp_bytes = 0;
+    HIPCHK(hipcub::DeviceRadixSort::SortKeys(nullptr, tmp_bytes, k_in, k_out, F, 0, 62, st));
+    Buffer tmp(std::max<size_t>(tmp_bytes, 16));
+    HIPCHK(hipcub::DeviceRadixSort::SortKeys(tmp.p, tmp_bytes, k_in, k_out, F, 0, 62, st));
+    HIPCHK(hipMemsetAsync(arrived.p, 0, arrived.bytes, st));
+    HIPCHK(hipMemsetAsync(parent.p, 0xff, parent.bytes, st));
+    hipLaunchKernelGGL(k_lbvh_nodes, dim3(nb), dim3(256), 0, st, k_out, F, nodes.as<BvhNode>(), parent.as<int>(),
+                       prim.as<int>());
+    hipLaunchKernelGGL(k_lbvh_boxes, dim3(nb), dim3(256), 0, st, v.as<double>(), f.as<int>(), prim.as<int>(), F,
+                       parent.as<int>(), arrived.as<int>(), nodes.as<BvhNode>());
+    hipLaunchKernelGGL(k_lbvh_depth, dim3(nb), dim3(256), 0, st, parent.as<int>(), F, depth);
+    HIPCHK(hipGetLastError());
+    int hd = 0;
+    HIPCHK(hipMemcpyAsync(&hd, depth, sizeof(int), hipMemcpyDeviceToHost, st));
+    HIPCHK(hipStreamSynchronize(st));
+    if (hd >= kBvhStack - 2) return false;
+    m.v = std::move(v);
+    m.f = std::move(f);
+    m.nodes = std::move(nodes);
+    m.prim = std::move(prim);
+    m.ntri = F;
+    m.nnodes = NN;
+    m.depth = hd;
+    m.slack = 1e-12 * (1.0 + scale);
+    return true;
 }
 
 void segment_build(MeshBvh& m, const std::vector<double>& verts) {
@@ -327,18 +524,18 @@ __device__ void cp_bvh(const MeshDev& m, const double* x, double* c, double& dis
             }
         } else {
             const double lim = (best + m.slack) * (best + m.slack);
-            const int l = nd->first;
-            const double dl = box_d2(m.nodes + l, x), dr = box_d2(m.nodes + l + 1, x);
+            const int l = nd->first, r = nd->second;
+            const double dl = box_d2(m.nodes + l, x), dr = box_d2(m.nodes + r, x);
             const bool vl = dl <= lim, vr = dr <= lim;
             if (vl && vr) {
-                if (sp == kBvhStack) { dist = NAN; return; }   // unreachable: the builder bounds the depth
+                if (sp == kBvhStack) { dist = NAN; return; }   // unreachable: the builders bound the depth
                 const bool lfirst = dl <= dr;
-                stack[sp++] = lfirst ? l + 1 : l;
-                node = lfirst ? l : l + 1;
+                stack[sp++] = lfirst ? r : l;
+                node = lfirst ? l : r;
                 continue;
             }
             if (vl || vr) {
-                node = vl ? l : l + 1;
+                node = vl ? l : r;
                 continue;
             }
         }

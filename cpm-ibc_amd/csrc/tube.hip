@@ -652,7 +652,7 @@ Tube* tube_build(Ctx* ctx, const SurfaceHost& s, int p, double dx) {
     T->p = p;
     T->dx = dx;
     if (s.type == "Triangulation" || s.type == "Polyline") {
-        if (s.type == "Triangulation") mesh_build(T->mesh, s.verts, s.faces);
+        if (s.type == "Triangulation") mesh_build(T->mesh, s.verts, s.faces, ctx->stream);
         else segment_build(T->mesh, s.verts);
         const MeshDev md = T->mesh.dev();
         T->sd = make_surfdev(s, T->mesh.v.as<double>(), &md);

@@ -21,9 +21,12 @@ def _mesh(name):
     raise ValueError(name)
 
 
+@pytest.mark.parametrize("builder", ["host", "gpu"])
 @pytest.mark.parametrize("name,dx", [("icosphere2", 0.1), ("displaced3", 0.05), ("box", 0.1)])
-def test_mesh_tube_matches_oracle(gpu_ctx, name, dx):
-    print(f"case mesh tube {name} dx={dx}", flush=True)
+def test_mesh_tube_matches_oracle(gpu_ctx, monkeypatch, name, dx, builder):
+    """Both hierarchy builders (host SAH, GPU linear BVH): the traversal is exact either way."""
+    print(f"case mesh tube {name} dx={dx} builder={builder}", flush=True)
+    monkeypatch.setenv("CPMGPU_BVH", builder)
     V, F = _mesh(name)
     P = orc.mesh_problem(V, F, dx)
     To = P.tube()
@@ -60,13 +63,22 @@ def test_box_mesh_closed_form(gpu_ctx):
     assert (d <= tg.info["tube_radius"]).all()
 
 
-def test_mesh_serving_size(gpu_ctx):
+def test_mesh_serving_size(gpu_ctx, monkeypatch):
     """The serving workload's surface: displaced icosphere with 81,920 triangles at h = 0.01
-    (about 1.1e6 tube nodes).  A sample of nodes is checked against the brute force."""
+    (about 1.1e6 tube nodes).  A sample of nodes is checked against the brute force, and the tubes
+    of the GPU-built (the default at this size) and host-built hierarchies are identical."""
     print("case mesh serving size", flush=True)
     V, F = workloads.displaced_icosphere(6, seed=4)
     tg = cpmgpu.Tube(gpu_ctx, "Triangulation", 0.01, verts=V, faces=F)
     T = tg.export()
+    monkeypatch.setenv("CPMGPU_BVH", "host")
+    th = cpmgpu.Tube(gpu_ctx, "Triangulation", 0.01, verts=V, faces=F)
+    Th = th.export()
+    monkeypatch.delenv("CPMGPU_BVH")
+    print(f"build_ms gpu bvh {tg.info['build_ms']:.1f} host bvh {th.info['build_ms']:.1f}", flush=True)
+    for k in T:
+        assert np.array_equal(T[k], Th[k]), k
+    del th, Th
     print(f"nodes={tg.N} build_ms={tg.info['build_ms']:.1f}", flush=True)
     assert 0.8e6 < tg.N < 1.6e6
     assert (T["dist"] <= tg.info["tube_radius"]).all()

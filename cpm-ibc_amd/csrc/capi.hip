@@ -935,16 +935,21 @@ static void profile_egather_t(Op* op, int G, int reps, double* avg_ms, double* b
     int* dbseg = reinterpret_cast<int*>(tb + seg_bytes);
     HIPCHK(hipMemcpyAsync(dseg, h.data(), sizeof(Seg<T>) * G, hipMemcpyHostToDevice, st));
     HIPCHK(hipMemcpyAsync(dbseg, hb.data(), sizeof(int) * hb.size(), hipMemcpyHostToDevice, st));
-    T* x = reinterpret_cast<T*>(ctx->workspace("prof_x", sizeof(T) * seg_total * G));
-    T* y = reinterpret_cast<T*>(ctx->workspace("prof_y", sizeof(T) * seg_total * G));
-    HIPCHK(hipMemsetAsync(x, 0, sizeof(T) * seg_total * G, st));
+    // x and out rotate over enough copies that a launch finds none of its vectors in the 256 MB
+    // last-level cache (as inside a solve, where every iteration streams ~10 such vectors)
+    const size_t vec_bytes = sizeof(T) * (size_t)seg_total * G;
+    const int nbuf = (int)std::max<size_t>(1, std::min<size_t>(8, (size_t(640) << 20) / (2 * vec_bytes) + 1));
+    T* xs = reinterpret_cast<T*>(ctx->workspace("prof_x", vec_bytes * nbuf));
+    T* ys = reinterpret_cast<T*>(ctx->workspace("prof_y", vec_bytes * nbuf));
+    HIPCHK(hipMemsetAsync(xs, 0, vec_bytes * nbuf, st));
     LaunchCfg<T> c{dseg, dbseg, nblocks * G, G, st, 0, op->ngt};
-    launch_egather<T, NR>(c, x, y);   // warm-up
+    auto buf = [&](T* b, int r) { return b + (size_t)(r % nbuf) * seg_total * G; };
+    launch_egather<T, NR>(c, buf(xs, 0), buf(ys, 0));   // warm-up
     hipEvent_t e0, e1;
     HIPCHK(hipEventCreate(&e0));
     HIPCHK(hipEventCreate(&e1));
     HIPCHK(hipEventRecord(e0, st));
-    for (int r = 0; r < reps; ++r) launch_egather<T, NR>(c, x, y);
+    for (int r = 0; r < reps; ++r) launch_egather<T, NR>(c, buf(xs, r + 1), buf(ys, r + 1));
     HIPCHK(hipEventRecord(e1, st));
     HIPCHK(hipEventSynchronize(e1));
     float ms = 0;

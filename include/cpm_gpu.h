@@ -446,7 +446,8 @@ int cpm_rd_step_device(cpm_op* op, const cpm_rd_params* p, int n_fields, double*
  * No counterpart in the reference.  Times the operator's E-gather kernel (the
  * dominant kernel of every BiCGStab iteration) on the context stream with HIP
  * events: `reps` back-to-back launches over `nrhs` interleaved right-hand
- * sides (1, 2, 4 or 8) in the given precision.  Returns the average launch
+ * sides (1, 2, 4 or 8) in the given precision, rotating over copies of the
+ * input and output vectors that together exceed the last-level cache.  Returns the average launch
  * duration and the launch's algorithmic HBM bytes: E columns + values once,
  * x read once, tmp written once, one flag byte per row. */
 int cpm_profile_egather(cpm_op* op, int precision, int nrhs, int reps, double* avg_ms, double* bytes_per_launch);

@@ -204,7 +204,7 @@ static OpView<T> make_view(const Op& o, const Buffer& ev, const Buffer& lv, cons
     v.gto = o.gto.as<int>();
     v.gofs = o.gofs.as<int>();
     v.grow = o.grow.as<int>();
-    v.gpl = o.gpl.as<int16_t>();
+    v.gpl = o.gpl.as<int>();
     v.gw = gw.as<T>();
     v.gln = o.gln.as<int>();
     v.glo = o.glo.as<int>();

@@ -73,7 +73,9 @@ struct OpView {
     const int* gto;      // [ngt + 1]
     const int* gofs;     // [nitem + 1] first slot | kind << kGrpKindShift
     const int* grow;     // [nslot] row of each slot
-    const int16_t* gpl;  // [16][nitem] position of line (i, j)'s first row in the tile's list (kind 0)
+    const int* gpl;      // [8][nitem] kind 0: word h holds the positions of lines 2h and 2h + 1 (line
+                         // (i, j) = 4 i + j) in the tile's list, low and high 16 bits; kind 1: word 0
+                         // is the exception record; kind 2: words 0 .. kGrpRows - 1 are the rows (-1: none)
     const T* gw;         // [12][nslot] the slot's row's wx[4], wy[4], wz[4]
     const int* gln;
     const int* glo;
@@ -95,6 +97,7 @@ constexpr int kExcPos = 72;      // positions per exception record (64 slots, ex
 #define CPMGPU_GROUP_LDS_ROWS 1280
 #endif
 constexpr int kGrpRows = CPMGPU_GROUP_ROWS;   // rows per shared-stencil item
+static_assert(kGrpRows >= 1 && kGrpRows <= 8, "an identity item's rows fit the 8 words of its line positions");
 constexpr int kGrpLdsRows = CPMGPU_GROUP_LDS_ROWS;        // longest shared-stencil tile list
 constexpr int kGrpLdsBytes = 32 * CPMGPU_GROUP_LDS_ROWS;  // LDS of one shared-stencil workgroup
 // LDS rows of a shared-stencil tile for T x NR values per row (0: these

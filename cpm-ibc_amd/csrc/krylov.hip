@@ -438,26 +438,47 @@ __device__ __forceinline__ void egather_tp_lds_exc(const OpView<T>& op, int64_t 
     if (xp >= 0) acc += ldg(&op.xval[e]) * lx.load(xp);
 }
 
+// An exception row of a shared-stencil tile from LDS with its weights
+// (wx[4], wy[4], wz[4]) and extra value loaded before the tile's fill: the
+// products and sums of egather_tp_lds_exc, in its order.
+template <typename T, typename V>
+__device__ __forceinline__ void egather_grp_exc(const T (&w)[12], T xv, const ExcRow& m, const LdsRows<V>& lx, V& acc) {
+#pragma unroll
+    for (int l = 0; l < 16; ++l) {
+        const T wx = w[l >> 2], wy = w[4 + (l & 3)];
+        const V x0 = lx.load(m.pos(4 * l)), x1 = lx.load(m.pos(4 * l + 1)), x2 = lx.load(m.pos(4 * l + 2)),
+                x3 = lx.load(m.pos(4 * l + 3));
+        acc += ((w[8] * wy) * wx) * x0;
+        acc += ((w[9] * wy) * wx) * x1;
+        acc += ((w[10] * wy) * wx) * x2;
+        acc += ((w[11] * wy) * wx) * x3;
+    }
+    const int xp = m.pos(64);
+    if (xp >= 0) acc += xv * lx.load(xp);
+}
+
 // A kind-0 item of a shared-stencil tile: up to kGrpRows rows with the same
 // stencil block (absent rows have weight 0 and are not stored).
 template <typename T>
 struct GrpItem {
     T w[kGrpRows][12];   // wx[4], wy[4], wz[4] of each row
-    int p[16];           // position of each line's first x row in the tile's list
+    int pp[8];           // positions of lines 2h and 2h + 1's first x rows in the tile's list (low, high 16 bits)
     int row[kGrpRows];
+    __device__ __forceinline__ int p(int l) const { return (int)(((unsigned)pp[l >> 1] >> (16 * (l & 1))) & 0xffffu); }
 };
 
+// The rows and weights of the kGrpRows slots from s0 (slots past the last
+// are read as the last: rows g >= n of an item are never used).  Loads with
+// no condition, so that none waits on another.
 template <typename T>
-__device__ __forceinline__ void grp_item_load(const OpView<T>& op, int item, int s0, int n, GrpItem<T>& m) {
-    const int64_t S = op.nslot, NI = op.nitem;
-#pragma unroll
-    for (int l = 0; l < 16; ++l) m.p[l] = ldg(&op.gpl[l * NI + item]);
+__device__ __forceinline__ void grp_item_load(const OpView<T>& op, int s0, GrpItem<T>& m) {
+    const int64_t S = op.nslot;
 #pragma unroll
     for (int g = 0; g < kGrpRows; ++g) {
-        const bool on = g < n;
-        m.row[g] = on ? ldg(&op.grow[s0 + g]) : 0;
+        const int64_t sl = s0 + g < S ? s0 + g : S - 1;
+        m.row[g] = ldg(&op.grow[sl]);
 #pragma unroll
-        for (int q = 0; q < 12; ++q) m.w[g][q] = on ? ldg(&op.gw[q * S + s0 + g]) : (T)0;
+        for (int q = 0; q < 12; ++q) m.w[g][q] = ldg(&op.gw[q * S + sl]);
     }
 }
 
@@ -469,7 +490,7 @@ template <typename T, typename V, int W>
 __device__ __forceinline__ void egather_grp_lds(const GrpItem<T>& m, int n, const LdsRows<V>& lx, V (&acc)[kGrpRows]) {
 #pragma unroll
     for (int l = 0; l < 16; ++l) {
-        const int q = m.p[l];
+        const int q = m.p(l);
         const V x0 = lx.load(q), x1 = lx.load(q + 1), x2 = lx.load(q + 2), x3 = lx.load(q + 3);
 #pragma unroll
         for (int g = 0; g < kGrpRows; ++g) {
@@ -484,27 +505,51 @@ __device__ __forceinline__ void egather_grp_lds(const GrpItem<T>& m, int n, cons
     }
 }
 
-// The tile's x rows (list lst of nu rows) into LDS, all loads in flight
-// together, then a workgroup barrier.
-template <typename T, int NR, int LR, typename V>
-__device__ __forceinline__ void lds_fill(const int* lst, int nu, const T* xb, const LdsRows<V>& lrows) {
-    constexpr int KF = LR > 0 ? (LR + kBlock - 1) / kBlock : 1;
+// A thread's entries of a tile's list of x rows (entry threadIdx.x + k kBlock).
+template <int LR>
+struct FillIdx {
+    static constexpr int KF = LR > 0 ? (LR + kBlock - 1) / kBlock : 1;
     int src[KF];
+    int nu;
+};
+
+// The list entries (lst of nu rows; lst[0] readable even when nu = 0), all
+// loads in flight together: entries past the list read its last entry
+// instead of branching (a load in a branch is waited for inside it).
+template <int LR>
+__device__ __forceinline__ void lds_fill_index(const int* lst, int nu, FillIdx<LR>& f) {
+    f.nu = nu;
+    const int last = nu > 0 ? nu - 1 : 0;
 #pragma unroll
-    for (int k = 0; k < KF; ++k) {
+    for (int k = 0; k < FillIdx<LR>::KF; ++k) {
         const int u = (int)threadIdx.x + k * kBlock;
-        src[k] = u < nu ? ldg(&lst[u]) : 0;
+        f.src[k] = ldg(&lst[u < last ? u : last]);
     }
+}
+
+// The listed x rows into LDS, all loads in flight together, then a
+// workgroup barrier.
+template <typename T, int NR, int LR, typename V>
+__device__ __forceinline__ void lds_fill_rows(const FillIdx<LR>& f, const T* xb, const LdsRows<V>& lrows) {
+    constexpr int KF = FillIdx<LR>::KF;
     V buf[KF];
 #pragma unroll
     for (int k = 0; k < KF; ++k)
-        if ((int)threadIdx.x + k * kBlock < nu) buf[k] = vload<T, NR>(xb + (size_t)src[k] * NR);
+        if ((int)threadIdx.x + k * kBlock < f.nu) buf[k] = vload<T, NR>(xb + (size_t)f.src[k] * NR);
 #pragma unroll
     for (int k = 0; k < KF; ++k) {
         const int u = (int)threadIdx.x + k * kBlock;
-        if (u < nu) lrows.store(u, buf[k]);
+        if (u < f.nu) lrows.store(u, buf[k]);
     }
     __syncthreads();
+}
+
+// The tile's x rows (list lst of nu rows) into LDS, then a workgroup barrier.
+template <typename T, int NR, int LR, typename V>
+__device__ __forceinline__ void lds_fill(const int* lst, int nu, const T* xb, const LdsRows<V>& lrows) {
+    FillIdx<LR> f;
+    lds_fill_index<LR>(lst, nu, f);
+    lds_fill_rows<T, NR, LR>(f, xb, lrows);
 }
 
 // One shared-stencil tile of E x: thread t takes item gto[tile] + t.
@@ -512,26 +557,45 @@ template <typename T, int NR, int LR, typename V>
 __device__ __forceinline__ void egather_grp_tile(const Seg<T>& sg, int tile, const T* xb, T* out,
                                                  const LdsRows<V>& lrows) {
     const OpView<T>& op = sg.op;
+    // Loads in three rounds, each issued as soon as the previous one's
+    // values are in: the tile's list and item ranges; the list entries
+    // (first: the x rows wait on them), the items' offsets and line words;
+    // the x rows and the items' data.
+    const int lo = ldg(&op.glo[tile]), nu = ldg(&op.gln[tile]);
     const int i0 = ldg(&op.gto[tile]), i1 = ldg(&op.gto[tile + 1]);
+    FillIdx<LR> f;
+    lds_fill_index<LR>(op.glr + lo, nu, f);
     const int item = i0 + (int)threadIdx.x;
-    int s0 = 0, n = 0, kind = 3;
-    if (item < i1) {
-        const int a = ldg(&op.gofs[item]), b = ldg(&op.gofs[item + 1]);
-        s0 = a & kGrpOfsMask;
-        kind = a >> kGrpKindShift;
-        n = (b & kGrpOfsMask) - s0;
-    }
+    const int64_t NI = op.nitem;
+    const int itc = item < NI ? item : (int)NI - 1;   // threads past the tile's items read a valid item
+    const int a = ldg(&op.gofs[itc]), b = ldg(&op.gofs[itc + 1]);
     GrpItem<T> m;
+#pragma unroll
+    for (int h = 0; h < 8; ++h) m.pp[h] = ldg(&op.gpl[h * NI + itc]);
+    const bool has = item < i1;
+    const int s0 = a & kGrpOfsMask;
+    const int kind = has ? a >> kGrpKindShift : 3;
+    const int n = has ? (b & kGrpOfsMask) - s0 : 0;
+    // Every global load of the item is issued before the barrier: a load
+    // after it would hold its wave (and the workgroup's next tile) for a
+    // whole memory latency.  kind 0: the rows' weights, line positions
+    // and rows; kind 1 (a row an interior boundary moved): its weights in
+    // m.w[0], its row, positions and extra value; kind 2 (identity rows):
+    // their rows and x values.
     ExcRow mx;
-    int r1 = 0, e = -1;
-    if (kind == 0) {
-        grp_item_load(op, item, s0, n, m);
-    } else if (kind == 1) {
-        r1 = ldg(&op.grow[s0]);
-        e = ldg(&op.exc[r1]);
+    T xval = (T)0;
+    V idv[kGrpRows];
+    if (kind == 1) {   // issued before the weights: they need only the line words
+        const int e = m.pp[0];
         exc_row_load(op.glx, e, mx);
+        xval = ldg(&op.xval[e]);
+    } else if (kind == 2) {
+#pragma unroll
+        for (int g = 0; g < kGrpRows; ++g)
+            if (g < n) idv[g] = vload<T, NR>(xb + (size_t)m.pp[g] * NR);
     }
-    lds_fill<T, NR, LR>(op.glr + ldg(&op.glo[tile]), ldg(&op.gln[tile]), xb, lrows);
+    grp_item_load(op, s0, m);   // kinds 0 and 1 (kind 1: slot s0's row and weights)
+    lds_fill_rows<T, NR, LR>(f, xb, lrows);
     T* ob = out + sg.off;
     if (kind == 0) {
         V acc[kGrpRows];
@@ -543,13 +607,12 @@ __device__ __forceinline__ void egather_grp_tile(const Seg<T>& sg, int tile, con
             if (g < n) vstore<T, NR>(ob + (size_t)m.row[g] * NR, acc[g]);
     } else if (kind == 1) {
         V acc = (V)(T)0;
-        egather_tp_lds_exc<T, V, NR>(op, r1, e, mx, lrows, acc);   // rows an interior boundary moved
-        vstore<T, NR>(ob + (size_t)r1 * NR, acc);
+        egather_grp_exc<T, V>(m.w[0], xval, mx, lrows, acc);
+        vstore<T, NR>(ob + (size_t)m.row[0] * NR, acc);
     } else if (kind == 2) {
-        for (int g = 0; g < n; ++g) {
-            const int r = ldg(&op.grow[s0 + g]);
-            vstore<T, NR>(ob + (size_t)r * NR, vload<T, NR>(xb + (size_t)r * NR));
-        }
+#pragma unroll
+        for (int g = 0; g < kGrpRows; ++g)
+            if (g < n) vstore<T, NR>(ob + (size_t)m.pp[g] * NR, idv[g]);
     }
 }
 

@@ -322,8 +322,10 @@ static void parallel_for(int64_t n, Body&& body) {
 // their smallest column, identity rows are items of up to kGrpRows rows at
 // the end.  Tiles take items in that order while the tile's list of distinct
 // x rows stays within kGrpLdsRows and its items within kBlock; inside a tile,
-// items are ordered by row count (larger first) so the waves' loops over the
-// rows of an item end together.  Returns false (nothing built) if the tiles
+// plain items come first, by row count (larger first), then exception rows,
+// then identity rows, so that a wave's threads take one code path and their
+// loops over the rows of an item end together (which item a thread takes
+// changes no row's arithmetic).  Returns false (nothing built) if the tiles
 // would outnumber the operator's row blocks, which size every launch, or
 // number fewer than kGrpMinTiles.
 static bool build_group_tiles(Op& op, const HostTp& h, const Buffer& tpw64, const Buffer& tpw32) {
@@ -432,14 +434,17 @@ static bool build_group_tiles(Op& op, const HostTp& h, const Buffer& tpw64, cons
     // per tile: items by row count, the sorted list, positions
     std::vector<std::vector<int>> lists((size_t)ngt);
     std::vector<int> order((size_t)nitem);
-    std::vector<int16_t> gpl((size_t)16 * std::max<int64_t>(nitem, 1), 0),
-        tlx((size_t)std::max<int64_t>(h.ne, 1) * kExcPos, -1);
+    std::vector<int> gpl((size_t)8 * std::max<int64_t>(nitem, 1), 0);
+    std::vector<int16_t> tlx((size_t)std::max<int64_t>(h.ne, 1) * kExcPos, -1);
     std::atomic<int> bad{0};
     parallel_for(ngt, [&](int64_t t) {
         const int i0 = first[(size_t)t], i1 = first[(size_t)t + 1];
         for (int i = i0; i < i1; ++i) order[(size_t)i] = i;
-        std::stable_sort(order.begin() + i0, order.begin() + i1,
-                         [&](int u, int v) { return items[(size_t)u].n > items[(size_t)v].n; });
+        auto rank = [&](int u) {   // plain items by row count, then exception rows, then identity rows
+            const Item& it = items[(size_t)u];
+            return it.kind == 0 ? kGrpRows - it.n : kGrpRows + it.kind;
+        };
+        std::stable_sort(order.begin() + i0, order.begin() + i1, [&](int u, int v) { return rank(u) < rank(v); });
         std::vector<int>& u = lists[(size_t)t];
         for (int i = i0; i < i1; ++i)
             if (items[(size_t)i].kind != 2) h.columns(trow[(size_t)items[(size_t)i].r0], [&](int c) { u.push_back(c); });
@@ -454,12 +459,16 @@ static bool build_group_tiles(Op& op, const HostTp& h, const Buffer& tpw64, cons
                 for (int l = 0; l < 16; ++l) {
                     const int16_t p = pos(h.tpc[(size_t)l * D + r]);
                     if (p < 0 || p + 3 >= (int)u.size()) bad = 1;
-                    gpl[(size_t)l * nitem + q] = p;
+                    gpl[(size_t)(l >> 1) * nitem + q] |= (int)((unsigned)(uint16_t)p << (16 * (l & 1)));
                 }
             } else if (it.kind == 1) {
                 const int e = h.exc[(size_t)r];
                 for (int s = 0; s < 64; ++s) tlx[(size_t)e * kExcPos + s] = pos(h.ecx[(size_t)e * 64 + s]);
                 if (h.xcol[(size_t)e] >= 0) tlx[(size_t)e * kExcPos + 64] = pos(h.xcol[(size_t)e]);
+                gpl[(size_t)q] = e;   // read with the line positions, a step before the record
+            } else {
+                for (int g = 0; g < kGrpRows; ++g)   // the rows, read a step before their x values
+                    gpl[(size_t)g * nitem + q] = g < it.n ? trow[(size_t)it.r0 + g] : -1;
             }
         }
     });
@@ -487,7 +496,7 @@ static bool build_group_tiles(Op& op, const HostTp& h, const Buffer& tpw64, cons
     op.grp_exc = nexcr;
     op.grp_ident = (int64_t)ident.size();
     op.grp_ident_items = ((int64_t)ident.size() + kGrpRows - 1) / kGrpRows;
-    if (tlr.empty()) tlr.push_back(0);
+    tlr.push_back(0);   // an entry at every tile's list offset: the gather clamps its list reads into the list
     upload(op.gln, tln.data(), tln.size() * 4);
     upload(op.glo, tlo.data(), tlo.size() * 4);
     upload(op.glr, tlr.data(), tlr.size() * 4);
@@ -495,7 +504,7 @@ static bool build_group_tiles(Op& op, const HostTp& h, const Buffer& tpw64, cons
     upload(op.gto, first.data(), first.size() * 4);
     upload(op.gofs, gofs.data(), gofs.size() * 4);
     upload(op.grow, grow.data(), grow.size() * 4);
-    upload(op.gpl, gpl.data(), gpl.size() * 2);
+    upload(op.gpl, gpl.data(), gpl.size() * 4);
     op.gw64 = Buffer((size_t)12 * D * 8);
     op.gw32 = Buffer((size_t)12 * D * 4);
     const hipStream_t st = op.ctx->stream;

@@ -453,7 +453,12 @@ __device__ __forceinline__ void egather_grp_exc(const T (&w)[12], T xv, const Ex
         acc += ((w[10] * wy) * wx) * x2;
         acc += ((w[11] * wy) * wx) * x3;
     }
-    const int xp = m.pos(64);
+    // The extra entry's position, taken from its word only here: the empty
+    // asm (no instruction) keeps the compiler from decoding it before the
+    // tile's barrier, where it would wait for the record's load.
+    int w64 = m.pk[32];
+    asm volatile("" : "+v"(w64));
+    const int xp = (int)(int16_t)w64;
     if (xp >= 0) acc += xv * lx.load(xp);
 }
 
@@ -576,23 +581,20 @@ __device__ __forceinline__ void egather_grp_tile(const Seg<T>& sg, int tile, con
     const int s0 = a & kGrpOfsMask;
     const int kind = has ? a >> kGrpKindShift : 3;
     const int n = has ? (b & kGrpOfsMask) - s0 : 0;
-    // Every global load of the item is issued before the barrier: a load
-    // after it would hold its wave (and the workgroup's next tile) for a
-    // whole memory latency.  kind 0: the rows' weights, line positions
-    // and rows; kind 1 (a row an interior boundary moved): its weights in
-    // m.w[0], its row, positions and extra value; kind 2 (identity rows):
-    // their rows and x values.
+    // The items' global loads are issued before the barrier: a load after it
+    // holds its wave (and the workgroup's next tile) for a whole memory
+    // latency.  kind 0: the rows' weights, line positions and rows; kind 1
+    // (a row an interior boundary moved): its weights in m.w[0], its row,
+    // positions and extra value.  kind 2 (identity rows, their rows in the
+    // line words) reads its x values after the barrier: such items sit in
+    // the last tiles only, and loads of them here reused registers that the
+    // kind 1 loads then had to wait for.
     ExcRow mx;
     T xval = (T)0;
-    V idv[kGrpRows];
     if (kind == 1) {   // issued before the weights: they need only the line words
         const int e = m.pp[0];
         exc_row_load(op.glx, e, mx);
         xval = ldg(&op.xval[e]);
-    } else if (kind == 2) {
-#pragma unroll
-        for (int g = 0; g < kGrpRows; ++g)
-            if (g < n) idv[g] = vload<T, NR>(xb + (size_t)m.pp[g] * NR);
     }
     grp_item_load(op, s0, m);   // kinds 0 and 1 (kind 1: slot s0's row and weights)
     lds_fill_rows<T, NR, LR>(f, xb, lrows);
@@ -612,7 +614,7 @@ __device__ __forceinline__ void egather_grp_tile(const Seg<T>& sg, int tile, con
     } else if (kind == 2) {
 #pragma unroll
         for (int g = 0; g < kGrpRows; ++g)
-            if (g < n) vstore<T, NR>(ob + (size_t)m.pp[g] * NR, idv[g]);
+            if (g < n) vstore<T, NR>(ob + (size_t)m.pp[g] * NR, vload<T, NR>(xb + (size_t)m.pp[g] * NR));
     }
 }
 

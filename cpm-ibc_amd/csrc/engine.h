@@ -76,7 +76,7 @@ struct OpView {
     const int* gpl;      // [8][nitem] kind 0: word h holds the positions of lines 2h and 2h + 1 (line
                          // (i, j) = 4 i + j) in the tile's list, low and high 16 bits; kind 1: word 0
                          // is the exception record; kind 2: words 0 .. kGrpRows - 1 are the rows (-1: none)
-    const T* gw;         // [12][nslot] the slot's row's wx[4], wy[4], wz[4]
+    const T* gw;         // [nslot][12] the slot's row's wx[4], wy[4], wz[4] (48 or 96 contiguous bytes)
     const int* gln;
     const int* glo;
     const int* glr;

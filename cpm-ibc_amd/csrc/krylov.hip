@@ -474,16 +474,24 @@ struct GrpItem {
 
 // The rows and weights of the kGrpRows slots from s0 (slots past the last
 // are read as the last: rows g >= n of an item are never used).  Loads with
-// no condition, so that none waits on another.
+// no condition, so that none waits on another; a slot's 12 weights are
+// contiguous, read 16 bytes at a time.
 template <typename T>
 __device__ __forceinline__ void grp_item_load(const OpView<T>& op, int s0, GrpItem<T>& m) {
+    constexpr int E = 16 / (int)sizeof(T);   // weights per 16-byte load
+    typedef T V16 __attribute__((ext_vector_type(E)));
     const int64_t S = op.nslot;
 #pragma unroll
     for (int g = 0; g < kGrpRows; ++g) {
         const int64_t sl = s0 + g < S ? s0 + g : S - 1;
         m.row[g] = ldg(&op.grow[sl]);
+        const V16* src = reinterpret_cast<const V16*>(op.gw + sl * 12);
 #pragma unroll
-        for (int q = 0; q < 12; ++q) m.w[g][q] = ldg(&op.gw[q * S + sl]);
+        for (int q = 0; q < 12 / E; ++q) {
+            const V16 v = ldg(src + q);
+#pragma unroll
+            for (int e = 0; e < E; ++e) m.w[g][q * E + e] = v[e];
+        }
     }
 }
 

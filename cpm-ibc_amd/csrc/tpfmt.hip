@@ -264,7 +264,7 @@ __global__ void k_slot_weights(const T* tpw, int64_t D, const int* grow, int64_t
     if (s >= S) return;
     const int64_t r = grow[s];
 #pragma unroll
-    for (int q = 0; q < 12; ++q) gw[q * S + s] = tpw[q * D + r];
+    for (int q = 0; q < 12; ++q) gw[s * 12 + q] = tpw[q * D + r];
 }
 
 // Tensor-product rows on the host, for the tile builders.

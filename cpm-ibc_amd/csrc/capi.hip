@@ -201,13 +201,11 @@ static OpView<T> make_view(const Op& o, const Buffer& ev, const Buffer& lv, cons
     v.ngt = o.ngt;
     v.nitem = o.nitem;
     v.nslot = o.nslot;
-    v.gto = o.gto.as<int>();
     v.gofs = o.gofs.as<int>();
     v.grow = o.grow.as<int>();
     v.gpl = o.gpl.as<int>();
     v.gw = gw.as<T>();
     v.gln = o.gln.as<int>();
-    v.glo = o.glo.as<int>();
     v.glr = o.glr.as<int>();
     v.glx = o.glx.as<int16_t>();
     return v;
@@ -221,8 +219,8 @@ size_t Op::device_bytes() const {
     return ecol.bytes + eval64.bytes + eval32.bytes + lcol.bytes + lval64.bytes + lval32.bytes + ldiag64.bytes +
            ldiag32.bytes + flag.bytes + perm.bytes + tpc.bytes + tpw64.bytes + tpw32.bytes + exc.bytes + ecx.bytes +
            xcol.bytes + xval64.bytes + xval32.bytes + hrow.bytes + hcol.bytes + hval.bytes + tln.bytes + tlo.bytes +
-           tlr.bytes + tpl.bytes + tlx.bytes + gto.bytes + gofs.bytes + grow.bytes + gpl.bytes + gw64.bytes + gw32.bytes + gln.bytes +
-           glo.bytes + glr.bytes + glx.bytes;
+           tlr.bytes + tpl.bytes + tlx.bytes + gofs.bytes + grow.bytes + gpl.bytes + gw64.bytes + gw32.bytes + gln.bytes +
+           glr.bytes + glx.bytes;
 }
 
 Op* op_from_csr(Ctx* ctx, int64_t D, int nD, const int32_t* Lp, const int32_t* Lc, const double* Lv,
@@ -1847,7 +1845,7 @@ int cpm_op_tile_stats(const cpm_op* op, int64_t* out) {
             for (int v : tln) row_tiles += v >= 0;
         }
         out[0] = o->ngt;
-        out[1] = o->nitem;
+        out[1] = o->grp_items;
         out[2] = o->grp_items0;
         out[3] = o->grp_list;
         out[4] = row_tiles;

@@ -60,26 +60,27 @@ struct OpView {
     const int* tlr;      // lists of x rows
     const int16_t* tpl;  // [16][D] position of line (i, j)'s first row in the tile's list
     const int16_t* tlx;  // [nexc][kExcPos] positions of an exception row's 64 columns and extra entry
-    // Shared-stencil tiles (built beside the row tiles; gln, glo, glr, glx
-    // are their tln, tlo, tlr, tlx).  Rows whose E stencils are the same
-    // 4x4x4 block of nodes are handled by one thread, which reads each x row
-    // of the block from LDS once for all of them.  Tile t owns the items
-    // gto[t] .. gto[t + 1] - 1, one per thread; an item is up to kGrpRows
-    // rows with one stencil (kind 0), one exception row (kind 1) or up to
-    // kGrpRows identity rows (kind 2), listed in slots gofs[i] .. gofs[i + 1] - 1.
+    // Shared-stencil tiles (built beside the row tiles; gln, glr, glx are
+    // their tln, tlr, tlx).  Rows whose E stencils are the same 4x4x4 block
+    // of nodes are handled by one thread, which reads each x row of the block
+    // from LDS once for all of them.  Tile t owns the item places t kBlock ..
+    // (t + 1) kBlock - 1, one per thread (places past its items hold empty
+    // items, kind 3), and the list places t kGrpLdsRows .. (t + 1)
+    // kGrpLdsRows - 1 (gln[t] in use, the rest repeat the last): a tile's
+    // data needs no lookup.  An item is up to kGrpRows rows with one stencil
+    // (kind 0), one exception row (kind 1) or up to kGrpRows identity rows
+    // (kind 2), listed in slots gofs[i] .. gofs[i + 1] - 1.
     int ngt;             // shared-stencil tiles (0: none)
-    int nitem;
+    int nitem;           // item places, ngt kBlock
     int64_t nslot;
-    const int* gto;      // [ngt + 1]
     const int* gofs;     // [nitem + 1] first slot | kind << kGrpKindShift
     const int* grow;     // [nslot] row of each slot
     const int* gpl;      // [8][nitem] kind 0: word h holds the positions of lines 2h and 2h + 1 (line
                          // (i, j) = 4 i + j) in the tile's list, low and high 16 bits; kind 1: word 0
                          // is the exception record; kind 2: words 0 .. kGrpRows - 1 are the rows (-1: none)
     const T* gw;         // [nslot][12] the slot's row's wx[4], wy[4], wz[4] (48 or 96 contiguous bytes)
-    const int* gln;
-    const int* glo;
-    const int* glr;
+    const int* gln;      // [ngt] rows in use in the tile's list
+    const int* glr;      // [ngt][kGrpLdsRows] the tiles' lists of x rows
     const int16_t* glx;
 };
 

@@ -565,30 +565,30 @@ __device__ __forceinline__ void lds_fill(const int* lst, int nu, const T* xb, co
     lds_fill_rows<T, NR, LR>(f, xb, lrows);
 }
 
-// One shared-stencil tile of E x: thread t takes item gto[tile] + t.
+// One shared-stencil tile of E x: thread t takes the item at place
+// tile kBlock + t.
 template <typename T, int NR, int LR, typename V>
 __device__ __forceinline__ void egather_grp_tile(const Seg<T>& sg, int tile, const T* xb, T* out,
                                                  const LdsRows<V>& lrows) {
     const OpView<T>& op = sg.op;
-    // Loads in three rounds, each issued as soon as the previous one's
-    // values are in: the tile's list and item ranges; the list entries
-    // (first: the x rows wait on them), the items' offsets and line words;
-    // the x rows and the items' data.
-    const int lo = ldg(&op.glo[tile]), nu = ldg(&op.gln[tile]);
-    const int i0 = ldg(&op.gto[tile]), i1 = ldg(&op.gto[tile + 1]);
+    // Loads in two rounds, each issued as soon as the previous one's values
+    // are in: the list entries (first: the x rows wait on them), the item's
+    // offsets and line words, all at fixed places of the tile; then the x
+    // rows and the item's data.
+    static_assert(FillIdx<LR>::KF * kBlock == LR && LR == kGrpLdsRows, "a tile's list places are read whole");
     FillIdx<LR> f;
-    lds_fill_index<LR>(op.glr + lo, nu, f);
-    const int item = i0 + (int)threadIdx.x;
+    f.nu = ldg(&op.gln[tile]);
+#pragma unroll
+    for (int k = 0; k < FillIdx<LR>::KF; ++k) f.src[k] = ldg(&op.glr[(size_t)tile * LR + threadIdx.x + k * kBlock]);
     const int64_t NI = op.nitem;
-    const int itc = item < NI ? item : (int)NI - 1;   // threads past the tile's items read a valid item
-    const int a = ldg(&op.gofs[itc]), b = ldg(&op.gofs[itc + 1]);
+    const int item = tile * kBlock + (int)threadIdx.x;
+    const int a = ldg(&op.gofs[item]), b = ldg(&op.gofs[item + 1]);
     GrpItem<T> m;
 #pragma unroll
-    for (int h = 0; h < 8; ++h) m.pp[h] = ldg(&op.gpl[h * NI + itc]);
-    const bool has = item < i1;
+    for (int h = 0; h < 8; ++h) m.pp[h] = ldg(&op.gpl[h * NI + item]);
     const int s0 = a & kGrpOfsMask;
-    const int kind = has ? a >> kGrpKindShift : 3;
-    const int n = has ? (b & kGrpOfsMask) - s0 : 0;
+    const int kind = a >> kGrpKindShift;   // 3: an empty place
+    const int n = (b & kGrpOfsMask) - s0;
     // The items' global loads are issued before the barrier: a load after it
     // holds its wave (and the workgroup's next tile) for a whole memory
     // latency.  kind 0: the rows' weights, line positions and rows; kind 1

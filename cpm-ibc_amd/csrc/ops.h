@@ -70,7 +70,7 @@ struct Op {
     int64_t nexc = 0, nmoved = 0;
     double loff = 0, ldu = 0;
     Buffer tln, tlo, tlr, tpl, tlx;   // LDS tiles (OP_TP)
-    Buffer gto, gofs, grow, gpl, gw64, gw32, gln, glo, glr, glx;   // shared-stencil tiles (OP_TP)
+    Buffer gofs, grow, gpl, gw64, gw32, gln, glr, glx;   // shared-stencil tiles (OP_TP)
     int ngt = 0, nitem = 0;
     int64_t nslot = 0;
     bool grp_mode = false, grp_tried = false;   // shared-stencil tiles wanted / build attempted
@@ -79,7 +79,7 @@ struct Op {
     // kind-0 items of shared-stencil tiles
     int64_t tile_rows[4] = {0, 0, 0, 0};
     int64_t tile_list = 0;
-    int64_t grp_items0 = 0, grp_list = 0;
+    int64_t grp_items = 0, grp_items0 = 0, grp_list = 0;
     int64_t grp_rows0 = 0, grp_exc = 0, grp_ident = 0, grp_ident_items = 0;
     // Lfull rows of the rows that solves treat as identity rows (internal
     // order, hw slots each, padded with weight 0): explicit time stepping

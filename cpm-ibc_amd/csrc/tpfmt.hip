@@ -434,7 +434,8 @@ static bool build_group_tiles(Op& op, const HostTp& h, const Buffer& tpw64, cons
     // per tile: items by row count, the sorted list, positions
     std::vector<std::vector<int>> lists((size_t)ngt);
     std::vector<int> order((size_t)nitem);
-    std::vector<int> gpl((size_t)8 * std::max<int64_t>(nitem, 1), 0);
+    const int64_t nip = ngt * kBlock;   // item places: tile t owns t kBlock .. (t + 1) kBlock - 1
+    std::vector<int> gpl((size_t)8 * nip, 0);
     std::vector<int16_t> tlx((size_t)std::max<int64_t>(h.ne, 1) * kExcPos, -1);
     std::atomic<int> bad{0};
     parallel_for(ngt, [&](int64_t t) {
@@ -452,56 +453,68 @@ static bool build_group_tiles(Op& op, const HostTp& h, const Buffer& tpw64, cons
         u.erase(std::unique(u.begin(), u.end()), u.end());
         if ((int64_t)u.size() > kGrpLdsRows || i1 - i0 > kBlock) bad = 1;
         auto pos = [&](int c) { return (int16_t)(std::lower_bound(u.begin(), u.end(), c) - u.begin()); };
-        for (int q = i0; q < i1; ++q) {
+        for (int q = i0; q < i1 && q - i0 < kBlock; ++q) {
             const Item& it = items[(size_t)order[(size_t)q]];
             const int r = trow[(size_t)it.r0];
+            const size_t qp = (size_t)t * kBlock + (size_t)(q - i0);   // the item's place
             if (it.kind == 0) {
                 for (int l = 0; l < 16; ++l) {
                     const int16_t p = pos(h.tpc[(size_t)l * D + r]);
                     if (p < 0 || p + 3 >= (int)u.size()) bad = 1;
-                    gpl[(size_t)(l >> 1) * nitem + q] |= (int)((unsigned)(uint16_t)p << (16 * (l & 1)));
+                    gpl[(size_t)(l >> 1) * nip + qp] |= (int)((unsigned)(uint16_t)p << (16 * (l & 1)));
                 }
             } else if (it.kind == 1) {
                 const int e = h.exc[(size_t)r];
                 for (int s = 0; s < 64; ++s) tlx[(size_t)e * kExcPos + s] = pos(h.ecx[(size_t)e * 64 + s]);
                 if (h.xcol[(size_t)e] >= 0) tlx[(size_t)e * kExcPos + 64] = pos(h.xcol[(size_t)e]);
-                gpl[(size_t)q] = e;   // read with the line positions, a step before the record
+                gpl[qp] = e;   // read with the line positions, a step before the record
             } else {
-                for (int g = 0; g < kGrpRows; ++g)   // the rows, read a step before their x values
-                    gpl[(size_t)g * nitem + q] = g < it.n ? trow[(size_t)it.r0 + g] : -1;
+                for (int g = 0; g < kGrpRows; ++g)   // the rows (kind 2 reads their x values after the barrier)
+                    gpl[(size_t)g * nip + qp] = g < it.n ? trow[(size_t)it.r0 + g] : -1;
             }
         }
     });
     if (bad) throw std::logic_error("shared-stencil tiles: a list or position is out of range");
-    // slots in final item order
-    std::vector<int> gofs((size_t)nitem + 1), grow;
+    // slots in final item order, item places padded to kBlock per tile with
+    // empty items (kind 3, no slot)
+    std::vector<int> gofs((size_t)nip + 1), grow;
     grow.reserve((size_t)D);
-    for (int64_t q = 0; q < nitem; ++q) {
-        const Item& it = items[(size_t)order[(size_t)q]];
-        gofs[(size_t)q] = (int)grow.size() | (it.kind << kGrpKindShift);
-        for (int g = 0; g < it.n; ++g) grow.push_back(trow[(size_t)it.r0 + g]);
+    for (int64_t t = 0; t < ngt; ++t) {
+        const int i0 = first[(size_t)t], i1 = first[(size_t)t + 1];
+        for (int j = 0; j < kBlock; ++j) {
+            const size_t qp = (size_t)t * kBlock + (size_t)j;
+            if (j >= i1 - i0) {
+                gofs[qp] = (int)grow.size() | (3 << kGrpKindShift);
+                continue;
+            }
+            const Item& it = items[(size_t)order[(size_t)(i0 + j)]];
+            gofs[qp] = (int)grow.size() | (it.kind << kGrpKindShift);
+            for (int g = 0; g < it.n; ++g) grow.push_back(trow[(size_t)it.r0 + g]);
+        }
     }
-    gofs[(size_t)nitem] = (int)grow.size();
+    gofs[(size_t)nip] = (int)grow.size();
     if ((int64_t)grow.size() != D) throw std::logic_error("shared-stencil tiles: not every row has a slot");
-    std::vector<int> tln((size_t)ngt), tlo((size_t)ngt), tlr;
+    // lists at kGrpLdsRows places per tile, the unused places repeating the
+    // last entry (the gather reads every place without a branch)
+    std::vector<int> tln((size_t)ngt), tlr((size_t)ngt * kGrpLdsRows, 0);
     op.grp_list = 0;
     for (int64_t t = 0; t < ngt; ++t) {
-        tln[(size_t)t] = (int)lists[(size_t)t].size();
-        tlo[(size_t)t] = (int)tlr.size();
-        tlr.insert(tlr.end(), lists[(size_t)t].begin(), lists[(size_t)t].end());
+        const std::vector<int>& u = lists[(size_t)t];
+        tln[(size_t)t] = (int)u.size();
+        int* dst = tlr.data() + (size_t)t * kGrpLdsRows;
+        std::copy(u.begin(), u.end(), dst);
+        std::fill(dst + u.size(), dst + kGrpLdsRows, u.empty() ? 0 : u.back());
         op.grp_list += tln[(size_t)t];
     }
+    op.grp_items = nitem;
     op.grp_items0 = items0;
     op.grp_rows0 = nplain;
     op.grp_exc = nexcr;
     op.grp_ident = (int64_t)ident.size();
     op.grp_ident_items = ((int64_t)ident.size() + kGrpRows - 1) / kGrpRows;
-    tlr.push_back(0);   // an entry at every tile's list offset: the gather clamps its list reads into the list
     upload(op.gln, tln.data(), tln.size() * 4);
-    upload(op.glo, tlo.data(), tlo.size() * 4);
     upload(op.glr, tlr.data(), tlr.size() * 4);
     upload(op.glx, tlx.data(), tlx.size() * 2);
-    upload(op.gto, first.data(), first.size() * 4);
     upload(op.gofs, gofs.data(), gofs.size() * 4);
     upload(op.grow, grow.data(), grow.size() * 4);
     upload(op.gpl, gpl.data(), gpl.size() * 4);
@@ -515,7 +528,7 @@ static bool build_group_tiles(Op& op, const HostTp& h, const Buffer& tpw64, cons
     HIPCHK(hipGetLastError());
     HIPCHK(hipStreamSynchronize(st));
     op.ngt = (int)ngt;
-    op.nitem = (int)nitem;
+    op.nitem = (int)nip;
     op.nslot = D;
     return true;
 }
@@ -596,7 +609,7 @@ static void build_lds_tiles(Op& op, const HostTp& h) {
             for (int l = 0; l < kS * kS; ++l) tpl[(size_t)l * D + r] = pos(hc[(size_t)l * D + r]);
         }
     });
-    if (tlr.empty()) tlr.push_back(0);
+    tlr.push_back(0);   // an entry at every tile's list offset: the gather clamps its list reads into the list
     auto up = [](Buffer& b, const void* p, size_t bytes) {
         b = Buffer(bytes);
         HIPCHK(hipMemcpy(b.p, p, bytes, hipMemcpyHostToDevice));

@@ -987,7 +987,9 @@ __global__ __launch_bounds__(kBlock) void k_xrupdate(const Seg<T>* __restrict__ 
         auto sv = vload<T, NR>(s + i);
         auto tv = vload<T, NR>(t + i);
         auto r0v = vload<T, NR>(r0 + i);
-        auto rv = vload<T, NR>(r + i);
+        // r is rewritten on every RUN lane: read only to keep the others
+        const bool all_run = L.of(M_RUN) == (1u << L.nr) - 1u && L.nr == NR;
+        typename VT<T, NR>::type rv = all_run ? (typename VT<T, NR>::type)(T)0 : vload<T, NR>(r + i);
 #pragma unroll
         for (int k = 0; k < NR; ++k) {
             if (!lane_in(L, k, M_RUN)) continue;

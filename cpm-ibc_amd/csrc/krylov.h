@@ -61,9 +61,10 @@ struct MixedLaunch {
 template <typename T>
 void launch_lfinish_plain(const LaunchCfg<T>& c, const T* x, const T* tmp, T* out);
 
-// E-gather alone (identity rows copy), for measurement.
+// E-gather alone (identity rows copy), for measurement; fly: of P^-1 x, as
+// the iterations gather (where the launch uses shared-stencil tiles).
 template <typename T, int NR>
-void launch_egather(const LaunchCfg<T>& c, const T* x, T* out);
+void launch_egather(const LaunchCfg<T>& c, const T* x, T* out, bool fly = false);
 
 // State reset before a (re)started solve: status, tolerance^2, iteration cap.
 void launch_reset_states(KState* st, int n, double tol2, int max_it, double delta2, hipStream_t s);

@@ -269,6 +269,24 @@ __device__ __forceinline__ T precond_apply(const Seg<T>& sg, int64_t row, uint8_
     return val;
 }
 
+// P^-1 of a row's NR values, lane by lane with precond_apply: the iteration
+// forms y = P^-1 p and z = P^-1 s where it reads them (the gathers' LDS
+// fills, lfinish, xrupdate) instead of storing them, with the same
+// operations, so the results are those of stored vectors to the bit.
+template <typename T, int NR>
+__device__ __forceinline__ typename VT<T, NR>::type precond_row(const Seg<T>& sg, int64_t row, uint8_t f,
+                                                                const typename VT<T, NR>::type& v) {
+    typename VT<T, NR>::type y;
+#pragma unroll
+    for (int k = 0; k < NR; ++k) setlane(y, k, precond_apply(sg, row, f, lane(v, k)));
+    return y;
+}
+
+// A shared-stencil list entry: the x row, and in bit 30 whether the row is an
+// identity or no-diagonal row (its P^-1 leaves the value as it is).
+constexpr int kListRowMask = (1 << 30) - 1;
+constexpr int kListPlainBit = 30;
+
 // E row of the tensor-product format: slot (i, j, k) in the reference's
 // order, value (wz[k] wy[j]) wx[i] formed exactly as when E was built.  One
 // stencil line (4 consecutive x rows) per step, two steps in flight (fully
@@ -540,19 +558,19 @@ __device__ __forceinline__ void lds_fill_index(const int* lst, int nu, FillIdx<L
     }
 }
 
-// The listed x rows into LDS, all loads in flight together, then a
-// workgroup barrier.
-template <typename T, int NR, int LR, typename V>
-__device__ __forceinline__ void lds_fill_rows(const FillIdx<LR>& f, const T* xb, const LdsRows<V>& lrows) {
+// The listed x rows into LDS, all loads in flight together, each row put
+// through xform(row value, list entry) on its way, then a workgroup barrier.
+template <typename T, int NR, int LR, typename V, typename F>
+__device__ __forceinline__ void lds_fill_rows(const FillIdx<LR>& f, const T* xb, const LdsRows<V>& lrows, F&& xform) {
     constexpr int KF = FillIdx<LR>::KF;
     V buf[KF];
 #pragma unroll
     for (int k = 0; k < KF; ++k)
-        if ((int)threadIdx.x + k * kBlock < f.nu) buf[k] = vload<T, NR>(xb + (size_t)f.src[k] * NR);
+        if ((int)threadIdx.x + k * kBlock < f.nu) buf[k] = vload<T, NR>(xb + (size_t)(f.src[k] & kListRowMask) * NR);
 #pragma unroll
     for (int k = 0; k < KF; ++k) {
         const int u = (int)threadIdx.x + k * kBlock;
-        if (u < f.nu) lrows.store(u, buf[k]);
+        if (u < f.nu) lrows.store(u, xform(buf[k], f.src[k]));
     }
     __syncthreads();
 }
@@ -562,12 +580,14 @@ template <typename T, int NR, int LR, typename V>
 __device__ __forceinline__ void lds_fill(const int* lst, int nu, const T* xb, const LdsRows<V>& lrows) {
     FillIdx<LR> f;
     lds_fill_index<LR>(lst, nu, f);
-    lds_fill_rows<T, NR, LR>(f, xb, lrows);
+    lds_fill_rows<T, NR, LR>(f, xb, lrows, [](const V& v, int) { return v; });
 }
 
 // One shared-stencil tile of E x: thread t takes the item at place
 // tile kBlock + t.
-template <typename T, int NR, int LR, typename V>
+// fly: x is p (or s) and the tile gathers y = P^-1 p (z = P^-1 s), formed
+// as the rows go into LDS.
+template <typename T, int NR, int LR, bool FLY, typename V>
 __device__ __forceinline__ void egather_grp_tile(const Seg<T>& sg, int tile, const T* xb, T* out,
                                                  const LdsRows<V>& lrows) {
     const OpView<T>& op = sg.op;
@@ -605,7 +625,10 @@ __device__ __forceinline__ void egather_grp_tile(const Seg<T>& sg, int tile, con
         xval = ldg(&op.xval[e]);
     }
     grp_item_load(op, s0, m);   // kinds 0 and 1 (kind 1: slot s0's row and weights)
-    lds_fill_rows<T, NR, LR>(f, xb, lrows);
+    lds_fill_rows<T, NR, LR>(f, xb, lrows, [&](const V& v, int e) {
+        if constexpr (FLY) return precond_row<T, NR>(sg, e & kListRowMask, (e >> kListPlainBit) & 1 ? ROW_IDENTITY : 0, v);
+        else return v;
+    });
     T* ob = out + sg.off;
     if (kind == 0) {
         V acc[kGrpRows];
@@ -622,7 +645,12 @@ __device__ __forceinline__ void egather_grp_tile(const Seg<T>& sg, int tile, con
     } else if (kind == 2) {
 #pragma unroll
         for (int g = 0; g < kGrpRows; ++g)
-            if (g < n) vstore<T, NR>(ob + (size_t)m.pp[g] * NR, vload<T, NR>(xb + (size_t)m.pp[g] * NR));
+            if (g < n) {
+                const int64_t r = m.pp[g];
+                V v = vload<T, NR>(xb + (size_t)r * NR);
+                if constexpr (FLY) v = precond_row<T, NR>(sg, r, ldg(&op.flag[r]), v);
+                vstore<T, NR>(ob + (size_t)r * NR, v);
+            }
     }
 }
 
@@ -712,11 +740,11 @@ __global__ __launch_bounds__(kBlock) void k_init(const Seg<T>* __restrict__ segs
     block_partials<1, NR>(pv, sg.part, lb, sg.nblocks);
 }
 
-// RUN lanes: p = (p - w v) beta + r ; y = P^-1 p.
+// RUN lanes: p = (p - w v) beta + r ; y = P^-1 p (fly: y is formed where it is read).
 // UPDATE lanes (mixed): x64 += x32 / scale ; x32 = 0.
 template <typename T, int NR>
 __global__ __launch_bounds__(kBlock) void k_pupdate(const Seg<T>* __restrict__ segs, const int* __restrict__ bseg, T* p, const T* v,
-                                                    const T* r, T* y, T* x, MixVec mv) {
+                                                    const T* r, T* y, T* x, MixVec mv, int fly) {
     int lb;
     int64_t row;
     const Seg<T>& sg = seg_of(segs, bseg, lb, row);
@@ -739,7 +767,7 @@ __global__ __launch_bounds__(kBlock) void k_pupdate(const Seg<T>* __restrict__ s
             setlane(yv, k, precond_apply(sg, row, f, pk));
         }
         vstore<T, NR>(p + i, pv);
-        vstore<T, NR>(y + i, yv);
+        if (!fly) vstore<T, NR>(y + i, yv);
     }
     if (std::is_same<T, float>::value && seg_any(L, M_UPDATE)) {
         auto xv = vload<T, NR>(x + i);
@@ -819,7 +847,8 @@ __global__ __launch_bounds__(kBlock) void k_egather_upd(const Seg<float>* __rest
 // do).  40 KB of LDS and at most 128 VGPRs, so four
 // workgroups share a CU (a direct-read path for other segments, even out of
 // line, made the FP32 x 8 kernel spill).
-template <typename T, int NR, int WPE>
+// FLY: x is p (or s) and the launch gathers y = P^-1 p (z = P^-1 s).
+template <typename T, int NR, int WPE, bool FLY>
 __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(WPE))) void k_egather_grp(
     const Seg<T>* __restrict__ segs, const int* __restrict__ bseg, const T* x, T* out, unsigned lanes, MixVec mv) {
     typedef typename VT<T, NR>::type V;
@@ -833,7 +862,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(WPE))) v
     const auto L = lane_stat<NR>(sg, lanes != 0);
     if (lanes && !seg_any(L, lanes)) return;               // uniform per workgroup
     const bool main = !lanes || seg_any(L, lanes & ~M_UPDATE);
-    if (main && lb < sg.op.ngt) egather_grp_tile<T, NR, LR>(sg, lb, x + sg.off, out, lrows);
+    if (main && lb < sg.op.ngt) egather_grp_tile<T, NR, LR, FLY>(sg, lb, x + sg.off, out, lrows);
 }
 
 // out = A x from tmp = E x.  RED = 0: no reduction; 1: partial (other, out);
@@ -843,7 +872,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(WPE))) v
 template <typename T, int NR, int RED>
 __global__ __launch_bounds__(kBlock) void k_lfinish(const Seg<T>* __restrict__ segs, const int* __restrict__ bseg, const T* x,
                                                     const T* tmp, T* out, const T* other, const T* rv32,
-                                                    unsigned lanes, MixVec mv) {
+                                                    unsigned lanes, MixVec mv, int fly) {
     int lb;
     int64_t row;
     const Seg<T>& sg = seg_of(segs, bseg, lb, row);
@@ -858,7 +887,9 @@ __global__ __launch_bounds__(kBlock) void k_lfinish(const Seg<T>* __restrict__ s
     if (row < sg.op.D) {
         const size_t i = (size_t)(sg.off + row * NR);
         if (!lanes || seg_any(L, lanes & ~(M_UPDATE | M_RESCALE))) {
-            const auto res = arow<T, NR>(sg.op, sg.m, sg.n, row, vload<T, NR>(x + i), tmp + sg.off, NR);
+            auto xv = vload<T, NR>(x + i);
+            if (fly) xv = precond_row<T, NR>(sg, row, ldg(&sg.op.flag[row]), xv);   // x is p or s
+            const auto res = arow<T, NR>(sg.op, sg.m, sg.n, row, xv, tmp + sg.off, NR);
             vstore<T, NR>(out + i, res);
             if (RED > 0) {
                 auto ov = vload<T, NR>(other + i);
@@ -900,11 +931,11 @@ __global__ __launch_bounds__(kBlock) void k_lfinish(const Seg<T>* __restrict__ s
     if (RED > 0) block_partials<NV, NR>(pv, sg.part, lb, sg.nblocks);
 }
 
-// RUN lanes: s = r - alpha v ; z = P^-1 s.
+// RUN lanes: s = r - alpha v ; z = P^-1 s (fly: z is formed where it is read).
 // RESCALE lanes (mixed): r32 = r64 scale ; r0 = r32 ; p = v = 0.
 template <typename T, int NR>
 __global__ __launch_bounds__(kBlock) void k_supdate(const Seg<T>* __restrict__ segs, const int* __restrict__ bseg, T* r, T* v, T* s, T* z,
-                                                    T* r0, T* p, MixVec mv) {
+                                                    T* r0, T* p, MixVec mv, int fly) {
     int lb;
     int64_t row;
     const Seg<T>& sg = seg_of(segs, bseg, lb, row);
@@ -924,7 +955,7 @@ __global__ __launch_bounds__(kBlock) void k_supdate(const Seg<T>* __restrict__ s
             setlane(zv, k, precond_apply(sg, row, f, sk));
         }
         vstore<T, NR>(s + i, sv);
-        vstore<T, NR>(z + i, zv);
+        if (!fly) vstore<T, NR>(z + i, zv);
     }
     if (std::is_same<T, float>::value && seg_any(L, M_RESCALE)) {
         auto r0v = vload<T, NR>(r0 + i);
@@ -970,7 +1001,7 @@ __global__ __launch_bounds__(kBlock) void k_jacobi(const Seg<T>* __restrict__ se
 
 template <typename T, int NR>
 __global__ __launch_bounds__(kBlock) void k_xrupdate(const Seg<T>* __restrict__ segs, const int* __restrict__ bseg, T* x, const T* y,
-                                                     const T* z, T* r, const T* s, const T* t, const T* r0) {
+                                                     const T* z, T* r, const T* s, const T* t, const T* r0, int fly) {
     int lb;
     int64_t row;
     const Seg<T>& sg = seg_of(segs, bseg, lb, row);
@@ -982,11 +1013,18 @@ __global__ __launch_bounds__(kBlock) void k_xrupdate(const Seg<T>* __restrict__ 
     if (row < sg.op.D) {
         const size_t i = (size_t)(sg.off + row * NR);
         auto xv = vload<T, NR>(x + i);
-        auto yv = vload<T, NR>(y + i);
-        auto zv = vload<T, NR>(z + i);
+        auto yv = vload<T, NR>(y + i);   // fly: p
         auto sv = vload<T, NR>(s + i);
         auto tv = vload<T, NR>(t + i);
         auto r0v = vload<T, NR>(r0 + i);
+        typename VT<T, NR>::type zv;
+        if (fly) {   // y = P^-1 p, z = P^-1 s
+            const uint8_t f = ldg(&sg.op.flag[row]);
+            yv = precond_row<T, NR>(sg, row, f, yv);
+            zv = precond_row<T, NR>(sg, row, f, sv);
+        } else {
+            zv = vload<T, NR>(z + i);
+        }
         // r is rewritten on every RUN lane: read only to keep the others
         const bool all_run = L.of(M_RUN) == (1u << L.nr) - 1u && L.nr == NR;
         typename VT<T, NR>::type rv = all_run ? (typename VT<T, NR>::type)(T)0 : vload<T, NR>(r + i);
@@ -1263,11 +1301,14 @@ void launch_reset_states(KState* st, int n, double tol2, int max_it, double delt
 // ---------------------------------------------------------------- launchers
 #define LAUNCH(kern, ...) \
     hipLaunchKernelGGL(kern, dim3(c.nblocks), dim3(kBlock), 0, c.stream, c.segs, c.nseg == 1 ? nullptr : c.bseg, __VA_ARGS__)
-#define EGATHER(...)                                                         \
+// E gather of launch c; FL: form P^-1 of the gathered vector (only with
+// shared-stencil tiles, grp_launch).
+#define EGATHER(FL, ...)                                                     \
     do {                                                                     \
         if constexpr (GrpTile<T, NR>::kRows > 0) {                           \
             if (c.grp > 0 && grp_gather(sizeof(T), NR, c.grp)) {             \
-                LAUNCH((k_egather_grp<T, NR, GrpWaves<T, NR>::kWaves>), __VA_ARGS__); \
+                if (FL) LAUNCH((k_egather_grp<T, NR, GrpWaves<T, NR>::kWaves, true>), __VA_ARGS__); \
+                else LAUNCH((k_egather_grp<T, NR, GrpWaves<T, NR>::kWaves, false>), __VA_ARGS__); \
                 break;                                                       \
             }                                                                \
         }                                                                    \
@@ -1292,11 +1333,17 @@ struct GrpWaves {
 // query (grp_refresh_override, called where a launch sequence starts), not
 // per launch: a solve makes thousands of launches.
 static std::atomic<int> g_grp_override{-2};   // -2 unread, -1 none, 0 "none", 1 "all"
+// y = P^-1 p and z = P^-1 s formed where they are read (launches that allow
+// it, Launch::iteration); CPMGPU_PRECOND_FLY=0 stores them instead.
+static std::atomic<int> g_fly_override{-1};   // -1 none, 0 off
+static bool precond_fly() { return g_fly_override.load(std::memory_order_relaxed) != 0; }
 
 void grp_refresh_override() {
     const char* e = std::getenv("CPMGPU_GROUP_GATHER");
     const std::string v = e ? e : "";
     g_grp_override.store(v == "all" ? 1 : v == "none" ? 0 : -1, std::memory_order_relaxed);
+    const char* fe = std::getenv("CPMGPU_PRECOND_FLY");
+    g_fly_override.store(fe && std::string(fe) == "0" ? 0 : -1, std::memory_order_relaxed);
 }
 
 bool grp_gather(int tsize, int nr, int ntiles) {
@@ -1308,6 +1355,13 @@ bool grp_gather(int tsize, int nr, int ntiles) {
     return ntiles >= (nr == 1 ? 768 : kGrpMinTiles);
 }
 
+// Whether the E gathers of launch c use shared-stencil tiles (EGATHER's choice).
+template <typename T, int NR>
+static bool grp_launch(const LaunchCfg<T>& c) {
+    if constexpr (GrpTile<T, NR>::kRows > 0) return c.grp > 0 && grp_gather(sizeof(T), NR, c.grp);
+    return false;
+}
+
 template <typename T, int NR>
 void Launch<T, NR>::init(const LaunchCfg<T>& c, const T* b, T* x, T* r, T* r0, T* p, T* v) {
     LAUNCH((k_init<T, NR>), b, x, r, r0, p, v);
@@ -1315,61 +1369,70 @@ void Launch<T, NR>::init(const LaunchCfg<T>& c, const T* b, T* x, T* r, T* r0, T
 }
 template <typename T, int NR>
 void Launch<T, NR>::iteration(const LaunchCfg<T>& c, Vecs<T>& w, bool mixed, const MixVec& mv) {
-    LAUNCH((k_pupdate<T, NR>), w.p, w.v, w.r, w.y, w.x, mv);
+    // fly: y = P^-1 p and z = P^-1 s are formed where they are read (the
+    // gathers' LDS fills, lfinish, xrupdate) rather than stored: launches
+    // whose gathers fill LDS from shared-stencil tiles and whose
+    // preconditioner is pointwise (no Jacobi sweeps after the first)
+    const bool fly = grp_launch<T, NR>(c) && c.jacobi_sweeps <= 1 && precond_fly();
+    const int fl = fly ? 1 : 0;
+    const T* yp = fly ? w.p : w.y;   // the vector the first gather and lfinish read
+    const T* zs = fly ? w.s : w.z;   // ... the second
+    LAUNCH((k_pupdate<T, NR>), w.p, w.v, w.r, w.y, w.x, mv, fl);
     for (int sw = 1; sw < c.jacobi_sweeps; ++sw) {   // y = P^-1 p, Jacobi sweeps after the first
         apply(c, w.y, w.tmp, w.t, M_RUN);
         LAUNCH((k_jacobi<T, NR>), w.p, w.t, w.y, sw);
     }
     if constexpr (std::is_same<T, float>::value) {
         if (mixed) {   // FP32 lanes in the gather, refinement lanes' FP64 gathers in a launch of their own
-            EGATHER(w.y, w.tmp, M_RUN, mv);
+            EGATHER(fly, yp, w.tmp, M_RUN, mv);
             LAUNCH((k_egather_upd<NR>), mv);
         } else {
-            EGATHER(w.y, w.tmp, M_STAGE1, mv);
+            EGATHER(fly, yp, w.tmp, M_STAGE1, mv);
         }
     } else {
-        EGATHER(w.y, w.tmp, M_STAGE1, mv);
+        EGATHER(fly, yp, w.tmp, M_STAGE1, mv);
     }
-    LAUNCH((k_lfinish<T, NR, 1>), w.y, w.tmp, w.v, w.r0, (const T*)nullptr, M_STAGE1, mv);
+    LAUNCH((k_lfinish<T, NR, 1>), yp, w.tmp, w.v, w.r0, (const T*)nullptr, M_STAGE1, mv, fl);
     FIN(T, NR, FIN_ALPHA);
-    LAUNCH((k_supdate<T, NR>), w.r, w.v, w.s, w.z, w.r0, w.p, mv);
+    LAUNCH((k_supdate<T, NR>), w.r, w.v, w.s, w.z, w.r0, w.p, mv, fl);
     for (int sw = 1; sw < c.jacobi_sweeps; ++sw) {   // z = P^-1 s
         apply(c, w.z, w.tmp, w.t, M_RUN);
         LAUNCH((k_jacobi<T, NR>), w.s, w.t, w.z, sw);
     }
-    EGATHER(w.z, w.tmp, M_RUN, mv);
-    LAUNCH((k_lfinish<T, NR, 2>), w.z, w.tmp, w.t, w.s, (const T*)w.r, M_STAGE2, mv);
+    EGATHER(fly, zs, w.tmp, M_RUN, mv);
+    LAUNCH((k_lfinish<T, NR, 2>), zs, w.tmp, w.t, w.s, (const T*)w.r, M_STAGE2, mv, fl);
     FIN(T, NR, FIN_OMEGA);
-    LAUNCH((k_xrupdate<T, NR>), w.x, w.y, w.z, w.r, w.s, w.t, w.r0);
+    LAUNCH((k_xrupdate<T, NR>), w.x, yp, w.z, w.r, w.s, w.t, w.r0, fl);
     if (mixed) FIN(T, NR, FIN_ITER_MX);
     else FIN(T, NR, FIN_ITER64);
 }
 template <typename T, int NR>
 void Launch<T, NR>::restart(const LaunchCfg<T>& c, Vecs<T>& w) {
     const MixVec none{};
-    EGATHER(w.x, w.tmp, M_RESTART, none);
-    LAUNCH((k_lfinish<T, NR, 0>), w.x, w.tmp, w.t, (const T*)nullptr, (const T*)nullptr, M_RESTART, none);
+    EGATHER(false, w.x, w.tmp, M_RESTART, none);
+    LAUNCH((k_lfinish<T, NR, 0>), w.x, w.tmp, w.t, (const T*)nullptr, (const T*)nullptr, M_RESTART, none, 0);
     LAUNCH((k_restart<T, NR>), w.b, w.t, w.r, w.r0);
     FIN(T, NR, FIN_RESTART64);
 }
 template <typename T, int NR>
 void Launch<T, NR>::apply(const LaunchCfg<T>& c, const T* x, T* tmp, T* y, unsigned lanes) {
     const MixVec none{};
-    EGATHER(x, tmp, lanes, none);
-    LAUNCH((k_lfinish<T, NR, 0>), x, tmp, y, (const T*)nullptr, (const T*)nullptr, lanes, none);
+    EGATHER(false, x, tmp, lanes, none);
+    LAUNCH((k_lfinish<T, NR, 0>), x, tmp, y, (const T*)nullptr, (const T*)nullptr, lanes, none, 0);
 }
 
 template <typename T>
 void launch_lfinish_plain(const LaunchCfg<T>& c, const T* x, const T* tmp, T* out) {
     const MixVec none{};
-    LAUNCH((k_lfinish<T, 1, 0>), x, tmp, out, (const T*)nullptr, (const T*)nullptr, 0u, none);
+    LAUNCH((k_lfinish<T, 1, 0>), x, tmp, out, (const T*)nullptr, (const T*)nullptr, 0u, none, 0);
 }
 template void launch_lfinish_plain<double>(const LaunchCfg<double>&, const double*, const double*, double*);
 
 template <typename T, int NR>
-void launch_egather(const LaunchCfg<T>& c, const T* x, T* out) {
+void launch_egather(const LaunchCfg<T>& c, const T* x, T* out, bool fly) {
     const MixVec none{};
-    EGATHER(x, out, 0u, none);
+    const bool fl = fly && grp_launch<T, NR>(c) && precond_fly();
+    EGATHER(fl, x, out, 0u, none);
 }
 
 template <int NR>
@@ -1383,7 +1446,7 @@ void MixedLaunch<NR>::down(const LaunchCfg<double>& c, const double* r64, Vecs<f
     FIN(double, NR, FIN_MX_INIT);
 }
 
-#define CPM_INST(T, NR) template struct Launch<T, NR>; template void launch_egather<T, NR>(const LaunchCfg<T>&, const T*, T*);
+#define CPM_INST(T, NR) template struct Launch<T, NR>; template void launch_egather<T, NR>(const LaunchCfg<T>&, const T*, T*, bool);
 CPM_INST(double, 1) CPM_INST(double, 2) CPM_INST(double, 4) CPM_INST(double, 8)
 CPM_INST(float, 1) CPM_INST(float, 2) CPM_INST(float, 4) CPM_INST(float, 8)
 template struct MixedLaunch<1>;

@@ -495,15 +495,18 @@ static bool build_group_tiles(Op& op, const HostTp& h, const Buffer& tpw64, cons
     gofs[(size_t)nip] = (int)grow.size();
     if ((int64_t)grow.size() != D) throw std::logic_error("shared-stencil tiles: not every row has a slot");
     // lists at kGrpLdsRows places per tile, the unused places repeating the
-    // last entry (the gather reads every place without a branch)
+    // last entry (the gather reads every place without a branch); an entry is
+    // the x row with the row's class in bit 30 (the gather's kListPlainBit)
     std::vector<int> tln((size_t)ngt), tlr((size_t)ngt * kGrpLdsRows, 0);
     op.grp_list = 0;
     for (int64_t t = 0; t < ngt; ++t) {
         const std::vector<int>& u = lists[(size_t)t];
         tln[(size_t)t] = (int)u.size();
         int* dst = tlr.data() + (size_t)t * kGrpLdsRows;
-        std::copy(u.begin(), u.end(), dst);
-        std::fill(dst + u.size(), dst + kGrpLdsRows, u.empty() ? 0 : u.back());
+        // bit 30: the row's P^-1 leaves it as it is (identity or no-diagonal row)
+        for (size_t k = 0; k < u.size(); ++k)
+            dst[k] = u[k] | ((h.flag[(size_t)u[k]] & (ROW_IDENTITY | ROW_NODIAG)) ? 1 << 30 : 0);
+        std::fill(dst + u.size(), dst + kGrpLdsRows, u.empty() ? 0 : dst[u.size() - 1]);
         op.grp_list += tln[(size_t)t];
     }
     op.grp_items = nitem;

@@ -332,6 +332,11 @@ static void fill_seg(Seg<T>& s, const Op* op, int64_t off, int block0, int nbloc
     s.nr = nr;
     s.precond = op->precond;
     s.precond_iters = op->precond_iters;
+    // FP32 P^-1 multiplies by these (set again if m or n change: fill_seg
+    // callers that override m, n use FP64 segments, which divide)
+    const float m32 = (float)op->m, n32 = (float)op->n;
+    s.dinv = (T)(1.0f / (m32 + (float)op->ldu * n32));
+    s.tinv = (T)(1.0f / (float)(op->m - op->n));
     s.st = st;
     s.part = part;
 }

@@ -126,6 +126,7 @@ struct Seg {
     int nr;              // rhs in this group (<= template NR; extra lanes idle)
     int precond;         // CPM_PRECOND_*
     int precond_iters;   // Jacobi sweeps
+    T dinv, tinv;        // FP32: 1 / (m + ldu n) (tensor-product format) and 1 / (m - n), for P^-1
     KState* st;          // nr states
     double* part;        // [2 * NR][nblocks] partial sums of the last reducing kernel
 };

@@ -242,15 +242,28 @@ template <int NR>
 __device__ __forceinline__ bool seg_any(const LaneStat<NR>& L, unsigned mask) { return L.of(mask) != 0; }
 
 // Diagonal_Inverse, GeneralizedBiCGStabSystem.h:303-338 (the branch the
-// reference runs: Compute() is never called).
-template <typename T>
+// reference runs: Compute() is never called).  FP64 divides, as the
+// reference does.  FP32 (a precision of this library's own, refined against
+// FP64) multiplies by the segment's reciprocals of its trailing-row and (in
+// the tensor-product format) its one plain diagonal value, formed once when
+// the segment is set up: a division per value costs ~10 instructions where
+// the gathers form P^-1 of every x row they load.
+// TP: the operator is known to be in the tensor-product format.
+template <typename T, bool TP = false>
 __device__ __forceinline__ T diag_inverse(const Seg<T>& sg, int64_t row, uint8_t f, T val) {
+    constexpr bool kRecip = std::is_same<T, float>::value;
     const int64_t dof = sg.op.D - sg.op.nD;
-    if (row >= dof) return val / (T)(sg.m - sg.n);   // :335-338 overwrite the trailing rows last
+    if (row >= dof) {   // :335-338 overwrite the trailing rows last
+        if constexpr (kRecip) return val * sg.tinv;
+        return val / (T)(sg.m - sg.n);
+    }
     if (f & ROW_IDENTITY) return val;
     if (f & ROW_NODIAG) return val;
-    const T ld = sg.op.fmt == OP_TP ? sg.op.ldu : ldg(&sg.op.ldiag[row]);
-    return val / ((T)sg.m + ld * (T)sg.n);
+    if (TP || sg.op.fmt == OP_TP) {
+        if constexpr (kRecip) return val * sg.dinv;
+        return val / ((T)sg.m + sg.op.ldu * (T)sg.n);
+    }
+    return val / ((T)sg.m + ldg(&sg.op.ldiag[row]) * (T)sg.n);
 }
 
 constexpr double kJacobiOmega = 2.0 / 3.0;   // GeneralizedBiCGStabSystem.h:364
@@ -259,12 +272,12 @@ constexpr double kJacobiOmega = 2.0 / 3.0;   // GeneralizedBiCGStabSystem.h:364
 // precond_iters damped sweeps from result = 0 (A 0 = 0), result = omega
 // D^-1 x on the rows below dofs and 0 on the trailing nD rows; the further
 // sweeps (k_jacobi) need an operator apply each.
-template <typename T>
+template <typename T, bool TP = false>
 __device__ __forceinline__ T precond_apply(const Seg<T>& sg, int64_t row, uint8_t f, T val) {
-    if (sg.precond == 1 /*diagonal*/) return diag_inverse(sg, row, f, val);
+    if (sg.precond == 1 /*diagonal*/) return diag_inverse<T, TP>(sg, row, f, val);
     if (sg.precond == 2 /*jacobi*/) {
         if (sg.precond_iters <= 0 || row >= sg.op.D - sg.op.nD) return (T)0;
-        return (T)kJacobiOmega * diag_inverse(sg, row, f, val);
+        return (T)kJacobiOmega * diag_inverse<T, TP>(sg, row, f, val);
     }
     return val;
 }
@@ -273,12 +286,12 @@ __device__ __forceinline__ T precond_apply(const Seg<T>& sg, int64_t row, uint8_
 // forms y = P^-1 p and z = P^-1 s where it reads them (the gathers' LDS
 // fills, lfinish, xrupdate) instead of storing them, with the same
 // operations, so the results are those of stored vectors to the bit.
-template <typename T, int NR>
+template <typename T, int NR, bool TP = false>
 __device__ __forceinline__ typename VT<T, NR>::type precond_row(const Seg<T>& sg, int64_t row, uint8_t f,
                                                                 const typename VT<T, NR>::type& v) {
     typename VT<T, NR>::type y;
 #pragma unroll
-    for (int k = 0; k < NR; ++k) setlane(y, k, precond_apply(sg, row, f, lane(v, k)));
+    for (int k = 0; k < NR; ++k) setlane(y, k, precond_apply<T, TP>(sg, row, f, lane(v, k)));
     return y;
 }
 
@@ -585,12 +598,12 @@ __device__ __forceinline__ void lds_fill(const int* lst, int nu, const T* xb, co
 
 // One shared-stencil tile of E x: thread t takes the item at place
 // tile kBlock + t.
-// fly: x is p (or s) and the tile gathers y = P^-1 p (z = P^-1 s), formed
-// as the rows go into LDS.
+// Tile `tile` of op: ob[row] = (E x)_row from xb (both with NR values per
+// row).  FLY: xb holds p (or s) and the tile gathers y = P^-1 p (z = P^-1 s)
+// with the preconditioner of segment *pre, formed as the rows go into LDS.
 template <typename T, int NR, int LR, bool FLY, typename V>
-__device__ __forceinline__ void egather_grp_tile(const Seg<T>& sg, int tile, const T* xb, T* out,
+__device__ __forceinline__ void egather_grp_tile(const OpView<T>& op, const Seg<T>* pre, int tile, const T* xb, T* ob,
                                                  const LdsRows<V>& lrows) {
-    const OpView<T>& op = sg.op;
     // Loads in two rounds, each issued as soon as the previous one's values
     // are in: the list entries (first: the x rows wait on them), the item's
     // offsets and line words, all at fixed places of the tile; then the x
@@ -626,10 +639,9 @@ __device__ __forceinline__ void egather_grp_tile(const Seg<T>& sg, int tile, con
     }
     grp_item_load(op, s0, m);   // kinds 0 and 1 (kind 1: slot s0's row and weights)
     lds_fill_rows<T, NR, LR>(f, xb, lrows, [&](const V& v, int e) {
-        if constexpr (FLY) return precond_row<T, NR>(sg, e & kListRowMask, (e >> kListPlainBit) & 1 ? ROW_IDENTITY : 0, v);
+        if constexpr (FLY) return precond_row<T, NR, true>(*pre, e & kListRowMask, (e >> kListPlainBit) & 1 ? ROW_IDENTITY : 0, v);
         else return v;
     });
-    T* ob = out + sg.off;
     if (kind == 0) {
         V acc[kGrpRows];
 #pragma unroll
@@ -648,7 +660,7 @@ __device__ __forceinline__ void egather_grp_tile(const Seg<T>& sg, int tile, con
             if (g < n) {
                 const int64_t r = m.pp[g];
                 V v = vload<T, NR>(xb + (size_t)r * NR);
-                if constexpr (FLY) v = precond_row<T, NR>(sg, r, ldg(&op.flag[r]), v);
+                if constexpr (FLY) v = precond_row<T, NR, true>(*pre, r, ldg(&op.flag[r]), v);
                 vstore<T, NR>(ob + (size_t)r * NR, v);
             }
     }
@@ -828,15 +840,35 @@ __global__ __launch_bounds__(kBlock) void k_egather(const Seg<T>* __restrict__ s
 
 // FP64 gathers of the refinement lanes of FP32 segments: tmp64 = E64 x64 for
 // every lane in an UPDATE status (a launch of its own beside the FP32
-// gather, which then carries only FP32 work).
+// gather, which then carries only FP32 work).  A lane's FP64 vectors are a
+// run of D values, the one-right-hand-side layout, so block lb gathers tile
+// lb of the operator's shared-stencil tiles for each such lane in turn;
+// operators without tiles read x64 directly, a row per thread.
 template <int NR>
 __global__ __launch_bounds__(kBlock) void k_egather_upd(const Seg<float>* __restrict__ segs, const int* __restrict__ bseg,
                                                         MixVec mv) {
+    constexpr int LR = GrpTile<double, 1>::kRows;
+    static_assert(LR > 0, "FP64 shared-stencil tiles fit the LDS");
+    __shared__ double lx[LR];
+    const LdsRows<double> lrows{lx, LR};
     int lb;
     int64_t row;
     const Seg<float>& sg = seg_of(segs, bseg, lb, row);
     const auto L = lane_stat<NR>(sg);
-    if (!seg_any(L, M_UPDATE) || row >= sg.op.D) return;
+    if (!seg_any(L, M_UPDATE)) return;   // uniform per workgroup
+    if (sg.op64.ngt > 0) {
+        if (lb >= sg.op64.ngt) return;
+        bool first = true;
+        for (int k = 0; k < sg.nr; ++k)
+            if (lane_in(L, k, M_UPDATE)) {
+                if (!first) __syncthreads();   // the previous lane's LDS reads are done
+                first = false;
+                egather_grp_tile<double, 1, LR, false>(sg.op64, (const Seg<double>*)nullptr, lb, mv.x64 + lm_at(sg, k, 0),
+                                                       mv.tmp64 + lm_at(sg, k, 0), lrows);
+            }
+        return;
+    }
+    if (row >= sg.op.D) return;
     for (int k = 0; k < sg.nr; ++k)
         if (lane_in(L, k, M_UPDATE))
             stg(&mv.tmp64[lm_at(sg, k, row)], erow<double, 1>(sg.op64, row, mv.x64 + lm_at(sg, k, 0), 1));
@@ -862,7 +894,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(WPE))) v
     const auto L = lane_stat<NR>(sg, lanes != 0);
     if (lanes && !seg_any(L, lanes)) return;               // uniform per workgroup
     const bool main = !lanes || seg_any(L, lanes & ~M_UPDATE);
-    if (main && lb < sg.op.ngt) egather_grp_tile<T, NR, LR, FLY>(sg, lb, x + sg.off, out, lrows);
+    if (main && lb < sg.op.ngt) egather_grp_tile<T, NR, LR, FLY>(sg.op, &sg, lb, x + sg.off, out + sg.off, lrows);
 }
 
 // out = A x from tmp = E x.  RED = 0: no reduction; 1: partial (other, out);

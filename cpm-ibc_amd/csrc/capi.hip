@@ -341,6 +341,48 @@ static void fill_seg(Seg<T>& s, const Op* op, int64_t off, int block0, int nbloc
     s.part = part;
 }
 
+// The launches of a run of iterations between two host checks (16 of a
+// streamed solve, 4 at a time of a batch solve's chunks) captured once into a
+// HIP graph and replayed (CPMGPU_GRAPH=1): the iteration kernels take the same
+// arguments every time (segment tables, vectors and states stay in place for
+// the whole solve) and record no memset, so every replay ends where the eager
+// launches end, bit for bit.
+// Replays measured faster than eager launches only for FP32+refine batch
+// solves of one group of lanes (one right-hand side on the torus-knot system:
+// 28.8 against 30.0 ms per solve); FP64 one-RHS solves were 2% slower and
+// streamed solves of 8 and 32 lanes 0.2-0.9% slower (their launches are
+// queued 16 iterations ahead), so those replay only with CPMGPU_GRAPH=1.
+// CPMGPU_GRAPH=0 turns replays off everywhere.
+static bool use_graph(bool by_default = false) {   // read per solve call (not per launch)
+    const char* e = std::getenv("CPMGPU_GRAPH");
+    if (e && std::string(e) == "1") return true;
+    if (e && std::string(e) == "0") return false;
+    return by_default;
+}
+
+struct IterGraph {
+    hipGraphExec_t exec = nullptr;
+    IterGraph() = default;
+    IterGraph(const IterGraph&) = delete;
+    IterGraph& operator=(const IterGraph&) = delete;
+    ~IterGraph() {
+        if (exec) (void)hipGraphExecDestroy(exec);
+    }
+    template <class F>
+    void run(hipStream_t st, F&& launches) {
+        if (!exec) {
+            hipGraph_t g = nullptr;
+            HIPCHK(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+            launches();   // kernel launches only: nothing here can throw mid-capture
+            HIPCHK(hipStreamEndCapture(st, &g));
+            const hipError_t ie = hipGraphInstantiate(&exec, g, nullptr, nullptr, 0);
+            (void)hipGraphDestroy(g);
+            HIPCHK(ie);
+        }
+        HIPCHK(hipGraphLaunch(exec, st));
+    }
+};
+
 // Workspace bytes run_batch<NR> asks the context for, over G groups of
 // `blocks` row blocks and `total` vector elements in all (each vector), and
 // their sum as Arena::take rounds them.
@@ -512,10 +554,22 @@ static void run_batch(Ctx* ctx, const std::vector<Group>& groups, const std::vec
     };
     int chunk = 4;
     bool run = false, restart = false;
+    // graph replays (use_graph): the launches of 4 iterations captured once
+    // into a HIP graph and replayed chunk / 4 times per chunk (chunks are
+    // multiples of 4)
+    // (never inside the allocation-free entry point: instantiating a graph allocates)
+    const bool graph = !ctx->arena && use_graph(refine && G == 1);
+    IterGraph ig;
     if (!refine) {
         Launch<double, NR>::init(c64, V64.b, V64.x, V64.r, V64.r0, V64.p, V64.v);
+        auto four = [&] {
+            for (int c = 0; c < 4; ++c) Launch<double, NR>::iteration(c64, V64, false, MixVec{});
+        };
         for (;;) {
-            for (int c = 0; c < chunk; ++c) Launch<double, NR>::iteration(c64, V64, false, MixVec{});
+            for (int c = 0; c < chunk; c += 4) {
+                if (graph) ig.run(st, four);
+                else four();
+            }
             poll(run, restart);
             if (restart) {   // rare: handled between chunks, the other instances wait
                 Launch<double, NR>::restart(c64, V64);
@@ -530,8 +584,14 @@ static void run_batch(Ctx* ctx, const std::vector<Group>& groups, const std::vec
         MixedLaunch<NR>::start(c64, V64.b, x64, r64);
         MixedLaunch<NR>::down(c64, r64, V32);
         const MixVec mv{x64, r64, V64.b, V64.tmp};
+        auto four = [&] {
+            for (int c = 0; c < 4; ++c) Launch<float, NR>::iteration(c32, V32, true, mv);
+        };
         for (;;) {
-            for (int c = 0; c < chunk; ++c) Launch<float, NR>::iteration(c32, V32, true, mv);
+            for (int c = 0; c < chunk; c += 4) {
+                if (graph) ig.run(st, four);
+                else four();
+            }
             poll(run, restart);
             if (!run) break;
             chunk = std::min(chunk * 2, 64);
@@ -552,42 +612,6 @@ static void run_batch(Ctx* ctx, const std::vector<Group>& groups, const std::vec
             inf.rel_residual = s.bsq > 0 ? std::sqrt(rr / s.bsq) : 0.0;
         }
 }
-
-// The launches of the 16 iterations between two host checks of a streamed
-// solve captured once into a HIP graph and replayed (CPMGPU_GRAPH=1): the
-// iteration kernels take the same arguments every time (segment tables,
-// vectors and states stay in place for the whole solve) and record no memset,
-// so every replay ends where the eager launches end, bit for bit.  Off by
-// default: on the 41,870-row sphere and the 405,928-row torus-knot systems,
-// with 8 and 32 lanes, replays were 0.2-0.9% slower than eager launches (the
-// GPU is already kept busy: launches are queued 16 iterations ahead).
-static bool use_graph() {   // read per solve call (not per launch)
-    const char* e = std::getenv("CPMGPU_GRAPH");
-    return e && std::string(e) == "1";
-}
-
-struct IterGraph {
-    hipGraphExec_t exec = nullptr;
-    IterGraph() = default;
-    IterGraph(const IterGraph&) = delete;
-    IterGraph& operator=(const IterGraph&) = delete;
-    ~IterGraph() {
-        if (exec) (void)hipGraphExecDestroy(exec);
-    }
-    template <class F>
-    void run(hipStream_t st, F&& launches) {
-        if (!exec) {
-            hipGraph_t g = nullptr;
-            HIPCHK(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
-            launches();   // kernel launches only: nothing here can throw mid-capture
-            HIPCHK(hipStreamEndCapture(st, &g));
-            const hipError_t ie = hipGraphInstantiate(&exec, g, nullptr, nullptr, 0);
-            (void)hipGraphDestroy(g);
-            HIPCHK(ie);
-        }
-        HIPCHK(hipGraphLaunch(exec, st));
-    }
-};
 
 // Continuous batching (FP32 + refinement): n systems of one operator through
 // G x NR interleaved lanes (G segments of NR lanes of the same operator in

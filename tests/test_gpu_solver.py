@@ -127,6 +127,33 @@ def test_jacobi_preconditioner_matches_oracle(gpu_ctx, sweeps):
     assert np.linalg.norm(x32 - x_ref) / np.linalg.norm(x_ref) <= 1e-8
 
 
+@pytest.mark.parametrize("precision,nb", [("fp32refine", 1), ("fp32refine", 8), ("fp64", 1), ("fp64", 4)])
+def test_batch_graph_replays_equal_eager(gpu_ctx, monkeypatch, precision, nb):
+    """Batch solves whose iterations are captured in a HIP graph (4 at a time) and replayed for every chunk
+    between the host's polls (CPMGPU_GRAPH=1) end bit for bit where the eager launches end, with the same
+    iteration counts, two calls in a row each, FP32+refine and FP64, one and several right-hand sides."""
+    print(f"case batch graph {precision} nb={nb}", flush=True)
+    tube = cpmgpu.Tube(gpu_ctx, "Sphere", 0.05)
+    op = tube.operator(1.0, -1.0)
+    D = op.D
+    rng = np.random.default_rng(13)
+    B = [rng.standard_normal(D) * (1 + k) for k in range(nb)]
+    prec = cpmgpu.CPM_FP32_REFINE if precision == "fp32refine" else cpmgpu.CPM_FP64
+    o = cpmgpu.opts(tol=1e-10, precision=prec)
+    monkeypatch.setenv("CPMGPU_FP64_ROUTING", "0")
+    res = {}
+    for mode in ("0", "1", "1"):
+        monkeypatch.setenv("CPMGPU_GRAPH", mode)
+        xs, infos = gpu_ctx.solve_batch([op] * nb, B, o)
+        assert all(i["status"] == "converged" for i in infos)
+        if mode in res:
+            assert [i["iters"] for i in infos] == [i["iters"] for i in res[mode][1]]
+            assert all(np.array_equal(a, b) for a, b in zip(xs, res[mode][0]))
+        res[mode] = (xs, infos)
+    assert [i["iters"] for i in res["1"][1]] == [i["iters"] for i in res["0"][1]]
+    assert all(np.array_equal(a, b) for a, b in zip(res["1"][0], res["0"][0]))
+
+
 @pytest.mark.parametrize("n,lanes", [(11, 8), (3, 8), (5, 1), (37, 16), (9, 16), (45, 32)])
 def test_stream_solves(gpu_ctx, n, lanes):
     """Continuous batching (cpm_solve_stream_device): every system of the stream,

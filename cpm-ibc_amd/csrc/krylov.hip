@@ -249,6 +249,10 @@ __device__ __forceinline__ bool seg_any(const LaneStat<NR>& L, unsigned mask) { 
 // the segment is set up: a division per value costs ~10 instructions where
 // the gathers form P^-1 of every x row they load.
 // TP: the operator is known to be in the tensor-product format.
+// The diagonal of A = m I + n L at a row whose L_ii is ld.
+template <typename T>
+__device__ __forceinline__ T diag_denom(const Seg<T>& sg, T ld) { return (T)sg.m + ld * (T)sg.n; }
+
 template <typename T, bool TP = false>
 __device__ __forceinline__ T diag_inverse(const Seg<T>& sg, int64_t row, uint8_t f, T val) {
     constexpr bool kRecip = std::is_same<T, float>::value;
@@ -261,9 +265,9 @@ __device__ __forceinline__ T diag_inverse(const Seg<T>& sg, int64_t row, uint8_t
     if (f & ROW_NODIAG) return val;
     if (TP || sg.op.fmt == OP_TP) {
         if constexpr (kRecip) return val * sg.dinv;
-        return val / ((T)sg.m + sg.op.ldu * (T)sg.n);
+        return val / diag_denom(sg, sg.op.ldu);
     }
-    return val / ((T)sg.m + ldg(&sg.op.ldiag[row]) * (T)sg.n);
+    return val / diag_denom(sg, ldg(&sg.op.ldiag[row]));
 }
 
 constexpr double kJacobiOmega = 2.0 / 3.0;   // GeneralizedBiCGStabSystem.h:364
@@ -292,6 +296,60 @@ __device__ __forceinline__ typename VT<T, NR>::type precond_row(const Seg<T>& sg
     typename VT<T, NR>::type y;
 #pragma unroll
     for (int k = 0; k < NR; ++k) setlane(y, k, precond_apply<T, TP>(sg, row, f, lane(v, k)));
+    return y;
+}
+
+// P^-1 of one row as factors its lanes share, found once per row: y = c ((v
+// / dd) mul), or 0 where `zero`.  A division by 1 and products with 1 change
+// no value, so for every preconditioner, row class and format this performs
+// precond_apply's operations in its order (the vector kernels' form of it:
+// the per-lane branches and scalar loads of precond_apply, taken for each of
+// a row's lanes, cost them a quarter of their time).  div: whether the
+// segment's rows divide (FP64, or FP32 in the ELL format), uniform per
+// segment.
+template <typename T>
+struct PRow {
+    T dd, mul, c;
+    bool zero;
+    __device__ __forceinline__ T apply(T v, bool div) const {
+        T q = div ? v / dd : v;
+        q = q * mul;
+        q = c * q;
+        return zero ? (T)0 : q;
+    }
+};
+template <typename T>
+__device__ __forceinline__ bool prow_div(const Seg<T>& sg) {
+    return !std::is_same<T, float>::value || sg.op.fmt != OP_TP;
+}
+template <typename T>
+__device__ __forceinline__ PRow<T> prow_of(const Seg<T>& sg, int64_t row, uint8_t f) {
+    constexpr bool kRecip = std::is_same<T, float>::value;
+    PRow<T> p{(T)1, (T)1, (T)1, false};
+    if (sg.precond != 1 && sg.precond != 2) return p;   // none: y = v
+    const bool trail = row >= sg.op.D - sg.op.nD;
+    if (sg.precond == 2) {   // Jacobi's first sweep: omega D^-1 v, 0 on the trailing rows
+        p.c = (T)kJacobiOmega;
+        p.zero = sg.precond_iters <= 0 || trail;
+    }
+    if (trail) {
+        if constexpr (kRecip) p.mul = sg.tinv;
+        else p.dd = (T)(sg.m - sg.n);
+    } else if (!(f & (ROW_IDENTITY | ROW_NODIAG))) {
+        if (sg.op.fmt == OP_TP) {
+            if constexpr (kRecip) p.mul = sg.dinv;
+            else p.dd = diag_denom(sg, sg.op.ldu);
+        } else {
+            p.dd = diag_denom(sg, ldg(&sg.op.ldiag[row]));
+        }
+    }
+    return p;
+}
+template <typename T, int NR>
+__device__ __forceinline__ typename VT<T, NR>::type prow_apply(const PRow<T>& p, bool div, const typename VT<T, NR>::type& v) {
+    typename VT<T, NR>::type y;
+#pragma unroll
+    for (int k = 0; k < NR; ++k) setlane(y, k, p.apply(lane(v, k), div));
     return y;
 }
 
@@ -727,6 +785,50 @@ __device__ __forceinline__ typename VT<T, W>::type arow(const OpView<T>& op, dou
     return (T)m * xv + (T)n * (ld * xv + lo);
 }
 
+// arow of the tensor-product format with every load issued up front: the
+// row's flag and L columns with x (and the caller's other loads), then the
+// six neighbours' E x rows, then the products and sums of arow in its order,
+// the trailing and identity rows taking their values by selection (a
+// missing neighbour reads the row's own entry with weight 0).  pre: x goes
+// through P^-1 first (x is p or s of an iteration that forms y and z where
+// it reads them).
+template <typename T, int W>
+struct TpLRow {
+    typename VT<T, W>::type xv;
+    int c[6];
+    uint8_t f;
+};
+template <typename T, int W>
+__device__ __forceinline__ void arow_tp_load(const OpView<T>& op, int64_t row, const T* xr, TpLRow<T, W>& a) {
+    const int64_t D = op.D;
+    a.f = ldg(&op.flag[row]);
+#pragma unroll
+    for (int s = 0; s < 6; ++s) a.c[s] = ldg(&op.lcol[(int64_t)s * D + row]);
+    a.xv = vload<T, W>(xr);
+}
+template <typename T, int W>
+__device__ __forceinline__ typename VT<T, W>::type arow_tp(const Seg<T>& sg, int64_t row, TpLRow<T, W>& a, const T* tb,
+                                                           int stride, bool pre) {
+    typedef typename VT<T, W>::type V;
+    const OpView<T>& op = sg.op;
+    V nb[6];
+#pragma unroll
+    for (int s = 0; s < 6; ++s) nb[s] = vload<T, W>(tb + (size_t)(a.c[s] >= 0 ? a.c[s] : row) * stride);
+    V xv = a.xv;
+    if (pre) xv = prow_apply<T, W>(prow_of(sg, row, a.f), prow_div(sg), xv);
+    V lo = (V)(T)0;
+#pragma unroll
+    for (int s = 0; s < 6; ++s) {
+        const T wl = a.c[s] >= 0 ? op.loff : (T)0;
+        lo += wl * nb[s];
+    }
+    const T ld = op.ldu;
+    V res = (T)sg.m * xv + (T)sg.n * (ld * xv + lo);
+    if (a.f & ROW_IDENTITY) res = xv;
+    if (row >= op.D - op.nD) res = (T)(sg.m - sg.n) * xv;
+    return res;
+}
+
 // ---------------------------------------------------------------- vector kernels
 template <typename T, int NR>
 __global__ __launch_bounds__(kBlock) void k_init(const Seg<T>* __restrict__ segs, const int* __restrict__ bseg, const T* b, T* x, T* r,
@@ -768,18 +870,15 @@ __global__ __launch_bounds__(kBlock) void k_pupdate(const Seg<T>* __restrict__ s
         auto pv = vload<T, NR>(p + i);
         auto vv = vload<T, NR>(v + i);
         auto rv = vload<T, NR>(r + i);
-        typename VT<T, NR>::type yv;
+        const unsigned run = L.of(M_RUN);
 #pragma unroll
-        for (int k = 0; k < NR; ++k) {
-            const KState* st = &sg.st[k < sg.nr ? k : 0];
-            const T beta = (T)ldg(&st->beta), w = (T)ldg(&st->omega);
-            T pk = (-w * lane(vv, k) + lane(pv, k)) * beta + lane(rv, k);   // p.Copy(-w,v,p); p*=beta; p+=r
-            if (!lane_in(L, k, M_RUN)) pk = lane(pv, k);
-            setlane(pv, k, pk);
-            setlane(yv, k, precond_apply(sg, row, f, pk));
+        for (int k = 0; k < NR; ++k) {   // a group owns kMaxNR states: every lane's load is valid
+            const T beta = (T)ldg(&sg.st[k].beta), w = (T)ldg(&sg.st[k].omega);
+            const T pk = (-w * lane(vv, k) + lane(pv, k)) * beta + lane(rv, k);   // p.Copy(-w,v,p); p*=beta; p+=r
+            if ((run >> k) & 1u) setlane(pv, k, pk);
         }
         vstore<T, NR>(p + i, pv);
-        if (!fly) vstore<T, NR>(y + i, yv);
+        if (!fly) vstore<T, NR>(y + i, prow_apply<T, NR>(prow_of(sg, row, f), prow_div(sg), pv));
     }
     if (std::is_same<T, float>::value && seg_any(L, M_UPDATE)) {
         auto xv = vload<T, NR>(x + i);
@@ -919,12 +1018,20 @@ __global__ __launch_bounds__(kBlock) void k_lfinish(const Seg<T>* __restrict__ s
     if (row < sg.op.D) {
         const size_t i = (size_t)(sg.off + row * NR);
         if (!lanes || seg_any(L, lanes & ~(M_UPDATE | M_RESCALE))) {
-            auto xv = vload<T, NR>(x + i);
-            if (fly) xv = precond_row<T, NR>(sg, row, ldg(&sg.op.flag[row]), xv);   // x is p or s
-            const auto res = arow<T, NR>(sg.op, sg.m, sg.n, row, xv, tmp + sg.off, NR);
+            typename VT<T, NR>::type res, ov;
+            if (sg.op.fmt == OP_TP) {   // every load that waits on no other in one round
+                TpLRow<T, NR> a;
+                arow_tp_load<T, NR>(sg.op, row, x + i, a);
+                if (RED > 0) ov = vload<T, NR>(other + i);
+                res = arow_tp<T, NR>(sg, row, a, tmp + sg.off, NR, fly);   // fly: x is p or s
+            } else {
+                auto xv = vload<T, NR>(x + i);
+                if (RED > 0) ov = vload<T, NR>(other + i);
+                if (fly) xv = precond_row<T, NR>(sg, row, ldg(&sg.op.flag[row]), xv);
+                res = arow<T, NR>(sg.op, sg.m, sg.n, row, xv, tmp + sg.off, NR);
+            }
             vstore<T, NR>(out + i, res);
             if (RED > 0) {
-                auto ov = vload<T, NR>(other + i);
 #pragma unroll
                 for (int k = 0; k < NR; ++k) {
                     const double rk = (double)lane(res, k);
@@ -978,16 +1085,14 @@ __global__ __launch_bounds__(kBlock) void k_supdate(const Seg<T>* __restrict__ s
     auto rv = vload<T, NR>(r + i);
     auto vv = vload<T, NR>(v + i);
     if (seg_any(L, M_RUN)) {
-        typename VT<T, NR>::type sv, zv;
+        typename VT<T, NR>::type sv;
 #pragma unroll
-        for (int k = 0; k < NR; ++k) {
-            const T a = (T)ldg(&sg.st[k < sg.nr ? k : 0].alpha);
-            T sk = -a * lane(vv, k) + lane(rv, k);   // s.Copy(-alpha, v, r)
-            setlane(sv, k, sk);
-            setlane(zv, k, precond_apply(sg, row, f, sk));
+        for (int k = 0; k < NR; ++k) {   // a group owns kMaxNR states: every lane's load is valid
+            const T a = (T)ldg(&sg.st[k].alpha);
+            setlane(sv, k, -a * lane(vv, k) + lane(rv, k));   // s.Copy(-alpha, v, r)
         }
         vstore<T, NR>(s + i, sv);
-        if (!fly) vstore<T, NR>(z + i, zv);
+        if (!fly) vstore<T, NR>(z + i, prow_apply<T, NR>(prow_of(sg, row, f), prow_div(sg), sv));
     }
     if (std::is_same<T, float>::value && seg_any(L, M_RESCALE)) {
         auto r0v = vload<T, NR>(r0 + i);
@@ -1049,20 +1154,23 @@ __global__ __launch_bounds__(kBlock) void k_xrupdate(const Seg<T>* __restrict__ 
         auto sv = vload<T, NR>(s + i);
         auto tv = vload<T, NR>(t + i);
         auto r0v = vload<T, NR>(r0 + i);
+        const uint8_t f = ldg(&sg.op.flag[row]);
+        // r is rewritten on every RUN lane: read only to keep the others
+        const unsigned run = L.of(M_RUN);
+        const bool all_run = run == (1u << NR) - 1u;
+        typename VT<T, NR>::type rv = all_run ? (typename VT<T, NR>::type)(T)0 : vload<T, NR>(r + i);
         typename VT<T, NR>::type zv;
         if (fly) {   // y = P^-1 p, z = P^-1 s
-            const uint8_t f = ldg(&sg.op.flag[row]);
-            yv = precond_row<T, NR>(sg, row, f, yv);
-            zv = precond_row<T, NR>(sg, row, f, sv);
+            const PRow<T> pr = prow_of(sg, row, f);
+            const bool div = prow_div(sg);
+            yv = prow_apply<T, NR>(pr, div, yv);
+            zv = prow_apply<T, NR>(pr, div, sv);
         } else {
             zv = vload<T, NR>(z + i);
         }
-        // r is rewritten on every RUN lane: read only to keep the others
-        const bool all_run = L.of(M_RUN) == (1u << L.nr) - 1u && L.nr == NR;
-        typename VT<T, NR>::type rv = all_run ? (typename VT<T, NR>::type)(T)0 : vload<T, NR>(r + i);
 #pragma unroll
-        for (int k = 0; k < NR; ++k) {
-            if (!lane_in(L, k, M_RUN)) continue;
+        for (int k = 0; k < NR; ++k) {   // a group owns kMaxNR states: every lane's load is valid
+            if (!((run >> k) & 1u)) continue;
             const T a = (T)ldg(&sg.st[k].alpha), w = (T)ldg(&sg.st[k].omega);
             T xk = a * lane(yv, k) + lane(xv, k);   // x.Copy(alpha, y, x)
             xk = w * lane(zv, k) + xk;               // x.Copy(w, z, x)

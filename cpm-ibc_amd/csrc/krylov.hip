@@ -303,8 +303,8 @@ __device__ __forceinline__ typename VT<T, NR>::type precond_row(const Seg<T>& sg
 // / dd) mul), or 0 where `zero`.  A division by 1 and products with 1 change
 // no value, so for every preconditioner, row class and format this performs
 // precond_apply's operations in its order (the vector kernels' form of it:
-// the per-lane branches and scalar loads of precond_apply, taken for each of
-// a row's lanes, cost them a quarter of their time).  div: whether the
+// precond_apply's branches and scalar loads, taken for each of a row's
+// lanes, made up two thirds of their instructions).  div: whether the
 // segment's rows divide (FP64, or FP32 in the ELL format), uniform per
 // segment.
 template <typename T>

@@ -474,7 +474,12 @@ template <typename T, typename V, int W>
 __device__ __forceinline__ void egather_tp_lds(const TpRow<T>& m, const LdsRows<V>& lx, V& acc) {
 #pragma unroll
     for (int l = 0; l < 16; ++l) {
-        const T wx = m.w[l >> 2], wy = m.w[4 + (l & 3)];
+        T wx = m.w[l >> 2], wy = m.w[4 + (l & 3)];
+        // the empty asm (no instruction) holds each line's products to their
+        // line: formed all together ahead of the LDS reads, the 64 products
+        // take 64 (FP64: 128) registers, which halved the FP64 kernel's
+        // occupancy once the row's weights were loaded with the tile's list
+        asm volatile("" : "+v"(wx), "+v"(wy));
         const int q = m.p[l];
         const V x0 = lx.load(q), x1 = lx.load(q + 1), x2 = lx.load(q + 2), x3 = lx.load(q + 3);
         acc += ((m.w[8] * wy) * wx) * x0;
@@ -503,33 +508,10 @@ __device__ __forceinline__ void exc_row_load(const int16_t* tlx, int e, ExcRow& 
     }
 }
 
-// Exception row from LDS: the slots and the extra entry of egather_tp's
-// exception path, in its order.
-template <typename T, typename V, int W>
-__device__ __forceinline__ void egather_tp_lds_exc(const OpView<T>& op, int64_t row, int e, const ExcRow& m,
-                                                   const LdsRows<V>& lx, V& acc) {
-    const int64_t D = op.D;
-    T wz[4];
-#pragma unroll
-    for (int q = 0; q < 4; ++q) wz[q] = ldg(&op.tpw[(8 + q) * D + row]);
-#pragma unroll
-    for (int l = 0; l < 16; ++l) {
-        const T wx = ldg(&op.tpw[(l >> 2) * D + row]);
-        const T wy = ldg(&op.tpw[(4 + (l & 3)) * D + row]);
-        const V x0 = lx.load(m.pos(4 * l)), x1 = lx.load(m.pos(4 * l + 1)), x2 = lx.load(m.pos(4 * l + 2)),
-                x3 = lx.load(m.pos(4 * l + 3));
-        acc += ((wz[0] * wy) * wx) * x0;
-        acc += ((wz[1] * wy) * wx) * x1;
-        acc += ((wz[2] * wy) * wx) * x2;
-        acc += ((wz[3] * wy) * wx) * x3;
-    }
-    const int xp = m.pos(64);
-    if (xp >= 0) acc += ldg(&op.xval[e]) * lx.load(xp);
-}
-
-// An exception row of a shared-stencil tile from LDS with its weights
-// (wx[4], wy[4], wz[4]) and extra value loaded before the tile's fill: the
-// products and sums of egather_tp_lds_exc, in its order.
+// An exception row (a row an interior boundary moved) from a tile's LDS copy
+// of x, with its weights (wx[4], wy[4], wz[4]) and extra value loaded before
+// the tile's fill: the slots and the extra entry of egather_tp's exception
+// path, in its order.
 template <typename T, typename V>
 __device__ __forceinline__ void egather_grp_exc(const T (&w)[12], T xv, const ExcRow& m, const LdsRows<V>& lx, V& acc) {
 #pragma unroll
@@ -644,14 +626,6 @@ __device__ __forceinline__ void lds_fill_rows(const FillIdx<LR>& f, const T* xb,
         if (u < f.nu) lrows.store(u, xform(buf[k], f.src[k]));
     }
     __syncthreads();
-}
-
-// The tile's x rows (list lst of nu rows) into LDS, then a workgroup barrier.
-template <typename T, int NR, int LR, typename V>
-__device__ __forceinline__ void lds_fill(const int* lst, int nu, const T* xb, const LdsRows<V>& lrows) {
-    FillIdx<LR> f;
-    lds_fill_index<LR>(lst, nu, f);
-    lds_fill_rows<T, NR, LR>(f, xb, lrows, [](const V& v, int) { return v; });
 }
 
 // One shared-stencil tile of E x: thread t takes the item at place
@@ -915,21 +889,45 @@ __global__ __launch_bounds__(kBlock) void k_egather(const Seg<T>* __restrict__ s
     const T* xb = x + sg.off;
     const int nu = (LR > 0 && main && sg.op.fmt == OP_TP && sg.op.tln) ? ldg(&sg.op.tln[lb]) : -1;
     if (nu >= 0 && nu <= LR) {
-        // the row's metadata, then the tile's x rows into LDS (all loads of
-        // the fill in flight together), then every stencil line from LDS
+        // Loads in two rounds, each issued as soon as the previous one's
+        // values are in: the tile's list entries with the row's flag,
+        // exception record and weights (rows past the end read the last
+        // row's, unused), all unconditional; then the listed x rows with the
+        // row's LDS positions (a plain row's 16 line positions, or a moved
+        // row's exception record and extra value, in one register array) or
+        // an identity row's own x.  Then the LDS copy, the barrier and every
+        // stencil line from LDS.
+        FillIdx<LR> f;
+        lds_fill_index<LR>(sg.op.tlr + ldg(&sg.op.tlo[lb]), nu, f);
         const bool act = row < sg.op.D;
-        const int e = act ? ldg(&sg.op.exc[row]) : -1;
-        const int kind = act ? ((ldg(&sg.op.flag[row]) & ROW_IDENTITY) ? 2 : (e < 0 ? 0 : 1)) : 3;
+        const int64_t rr = act ? row : sg.op.D - 1;
+        const int64_t D = sg.op.D;
+        const int e = ldg(&sg.op.exc[rr]);
+        const uint8_t fl = ldg(&sg.op.flag[rr]);
         TpRow<T> m;
-        ExcRow mx;
-        if (kind == 0) tp_row_load(sg.op, row, m);
-        else if (kind == 1) exc_row_load(sg.op.tlx, e, mx);
-        lds_fill<T, NR, LR>(sg.op.tlr + ldg(&sg.op.tlo[lb]), nu, xb, lrows);
+#pragma unroll
+        for (int q = 0; q < 12; ++q) m.w[q] = ldg(&sg.op.tpw[q * D + rr]);
+        const int kind = act ? ((fl & ROW_IDENTITY) ? 2 : (e < 0 ? 0 : 1)) : 3;
+        ExcRow mx;   // kind 0: words 0..15 are the line positions; kind 1: the exception record
+        T xval = (T)0;
+        V xi = (V)(T)0;
+        if (kind == 0) {
+#pragma unroll
+            for (int l = 0; l < 16; ++l) mx.pk[l] = ldg(&sg.op.tpl[l * D + row]);
+        } else if (kind == 1) {   // rows an interior boundary moved
+            exc_row_load(sg.op.tlx, e, mx);
+            xval = ldg(&sg.op.xval[e]);
+        } else if (kind == 2) {
+            xi = vload<T, NR>(xb + (size_t)row * NR);
+        }
+        lds_fill_rows<T, NR, LR>(f, xb, lrows, [](const V& v, int) { return v; });
         if (act) {
             V acc = (V)(T)0;
-            if (kind == 2) acc = vload<T, NR>(xb + (size_t)row * NR);
+#pragma unroll
+            for (int l = 0; l < 16; ++l) m.p[l] = mx.pk[l];
+            if (kind == 2) acc = xi;
             else if (kind == 0) egather_tp_lds<T, V, NR>(m, lrows, acc);
-            else egather_tp_lds_exc<T, V, NR>(sg.op, row, e, mx, lrows, acc);   // rows an interior boundary moved
+            else egather_grp_exc<T, V>(m.w, xval, mx, lrows, acc);
             vstore<T, NR>(out + i, acc);
         }
     } else if (main && row < sg.op.D) {

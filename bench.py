@@ -378,6 +378,7 @@ def bench_serving(args, ctx, cpmgpu):
             rec[tag] = dict(latency_ms=1000 * (t_done - arrive[tag]), build_ms=bms, rows=P.D,
                             iters=max(j["iters"] for j in infos),
                             fallback=sum(1 for j in infos if j["fallback_iters"] > 0),
+                            routed=sum(1 for j in infos if j["fallback_iters"] > 0 and j["iters"] == 0),
                             ok=all(j["status"] == "converged" for j in infos))
             print(f"request {tag}: {len(payload[tag][0])} curves, {P.D} rows, latency {rec[tag]['latency_ms']:.0f} ms",
                   file=sys.stderr, flush=True)
@@ -431,6 +432,7 @@ def bench_serving(args, ctx, cpmgpu):
         "rows_mean": int(np.mean([r["rows"] for r in rec.values()])),
         "iters_mean": round(float(np.mean([r["iters"] for r in rec.values()])), 1),
         "fp64_fallbacks": int(sum(r["fallback"] for r in rec.values())),
+        "routed_to_fp64": int(sum(r["routed"] for r in rec.values())),
         "all_converged": all(r["ok"] for r in rec.values()),
         "fp64_iters_request0": inf0["iters"],
         "roofline": roof, "cpu_baseline": cb,

@@ -766,16 +766,31 @@ static void run_stream(Ctx* ctx, Op* op, const std::vector<const double*>& rhs, 
 // (ST_FALLBACK) are solved again in FP64 from zero, the reference algorithm,
 // in one batch; their infos keep the FP32 counts and add fallback_iters.
 // FP64 routing (Fp32Record::prefer_fp64, route.h): FP32+refine solves of an
-// operator start in FP64 when its record says FP64 is cheaper
-// (CPMGPU_FP64_ROUTING=0: always try FP32 first).
+// operator start in FP64 when its record (or, for a tensor-product operator
+// without one, its family's) says FP64 is cheaper (CPMGPU_FP64_ROUTING=0:
+// always try FP32 first).
 static bool fp64_routing() {   // read per solve call (not per launch)
     const char* e = std::getenv("CPMGPU_FP64_ROUTING");
     return !(e && std::string(e) == "0");
 }
 
-// Adds the outcome of FP32+refine systems (infos after the FP64 re-solves) to their operators' records.
-static void record_fp32(const std::vector<Op*>& ops, const std::vector<cpm_solve_info>& infos) {
-    for (size_t i = 0; i < infos.size(); ++i) ops[i]->fp32.add(infos[i].iters, infos[i].fallback_iters);
+static std::array<double, 4> fp32_family_key(const Op* op) { return {op->m, op->n, op->loff, op->ldu}; }
+
+// Whether FP32+refine solves of op on ctx start in FP64: by the operator's own
+// record once it has one, else (the tensor-product format) by its family's.
+static bool prefer_fp64(Ctx* ctx, const Op* op) {
+    if (op->fp32.solved > 0 || op->fmt != OP_TP) return op->fp32.prefer_fp64();
+    const auto it = ctx->fp32_family.find(fp32_family_key(op));
+    return it != ctx->fp32_family.end() && it->second.prefer_fp64();
+}
+
+// Adds the outcome of FP32+refine systems solved on ctx (infos after the FP64
+// re-solves) to their operators' records and their families'.
+static void record_fp32(Ctx* ctx, const std::vector<Op*>& ops, const std::vector<cpm_solve_info>& infos) {
+    for (size_t i = 0; i < infos.size(); ++i) {
+        ops[i]->fp32.add(infos[i].iters, infos[i].fallback_iters);
+        if (ops[i]->fmt == OP_TP) ctx->fp32_family[fp32_family_key(ops[i])].add(infos[i].iters, infos[i].fallback_iters);
+    }
 }
 
 static void fp64_fallback(Ctx* ctx, const std::vector<Op*>& ops, const std::vector<const double*>& rhs,
@@ -824,7 +839,7 @@ void solve_batch(Ctx* ctx, const std::vector<Op*>& ops, const std::vector<const 
     if (o.precision == CPM_FP32_REFINE && fp64_routing()) {
         // operators whose FP32+refine solves fell back to FP64 before: FP64 from the start
         std::vector<int> r64, r32;
-        for (int i = 0; i < n; ++i) (ops[i]->fp32.prefer_fp64() ? r64 : r32).push_back(i);
+        for (int i = 0; i < n; ++i) (prefer_fp64(ctx, ops[i]) ? r64 : r32).push_back(i);
         if (!r64.empty()) {
             cpm_solve_opts o64 = o;
             o64.precision = CPM_FP64;
@@ -881,7 +896,7 @@ void solve_batch(Ctx* ctx, const std::vector<Op*>& ops, const std::vector<const 
     else run_batch<8>(ctx, groups, rhs, xs, o, infos);
     if (o.precision == CPM_FP32_REFINE) {
         fp64_fallback(ctx, ops, rhs, xs, o, infos, mn);
-        record_fp32(ops, infos);
+        record_fp32(ctx, ops, infos);
     }
 }
 
@@ -892,7 +907,7 @@ static void solve_stream(Op* op, const std::vector<const double*>& rhs, const st
     if (o.precision != CPM_FP32_REFINE) throw ArgError("streamed solves run in FP32_REFINE precision");
     if (op->nD != 0) throw ArgError("streamed solves need an operator without trailing nD rows");
     if (op->precond == CPM_PRECOND_JACOBI && op->precond_iters > 100000) throw ArgError("too many Jacobi sweeps");
-    if (op->fp32.prefer_fp64() && fp64_routing()) {   // see solve_batch
+    if (prefer_fp64(op->ctx, op) && fp64_routing()) {   // see solve_batch
         solve_batch(op->ctx, std::vector<Op*>(rhs.size(), op), rhs, xs, o, infos, nullptr);
         return;
     }
@@ -903,7 +918,7 @@ static void solve_stream(Op* op, const std::vector<const double*>& rhs, const st
     else throw ArgError("lanes must be 1, 2, 4, 8, 16, 32 or 64");
     const std::vector<Op*> ops(rhs.size(), op);
     fp64_fallback(op->ctx, ops, rhs, xs, o, infos, nullptr);
-    record_fp32(ops, infos);
+    record_fp32(op->ctx, ops, infos);
 }
 
 void multiply(Op* op, const double* dx, double* dy, double m, double n) {
@@ -1388,6 +1403,17 @@ struct Server : ServerBase {
         // the FP64 re-solve of this request's lanes (when their refinement stalls) runs in the
         // context's workspaces: grow them now, so that retiring the request allocates nothing
         reserve_batch_workspaces(ctx, op->D, NR, false);
+        if (fp64_routing() && prefer_fp64(ctx, op)) {
+            // the operator (or its family) stalled in FP32 before: the request is solved in FP64
+            // from zero now, as the re-solve of a stalled request is, without FP32 iterations
+            Done d{tag, std::vector<cpm_solve_info>((size_t)nrhs)};
+            for (auto& inf : d.infos) inf.status = ST_FALLBACK;
+            NoBuilds nb(ctx);
+            fp64_fallback(ctx, std::vector<Op*>((size_t)nrhs, op), std::vector<const double*>(rhs, rhs + nrhs),
+                          std::vector<double*>(xs, xs + nrhs), o, d.infos, nullptr);
+            done.push_back(std::move(d));
+            return true;
+        }
         int64_t off;
         if (free_slots.empty() || !ranges.take(len, off)) return false;   // wait for memory
         Req q{tag, op, nrhs, off, len, free_slots.back(), {}, {}};
@@ -1494,7 +1520,9 @@ struct Server : ServerBase {
             try {
                 for (int k = 0; k < q.nr; ++k) lane_pack(q, k, v64(1), 0);
                 NoBuilds nb(ctx);   // the re-solve uses the workspaces reserved at submission
-                fp64_fallback(ctx, std::vector<Op*>((size_t)q.nr, q.op), q.rhs, q.xs, o, d.infos, nullptr);
+                const std::vector<Op*> ops((size_t)q.nr, q.op);
+                fp64_fallback(ctx, ops, q.rhs, q.xs, o, d.infos, nullptr);
+                record_fp32(ctx, ops, d.infos);
             } catch (...) {
                 for (auto& inf : d.infos) inf.status = CPM_SOLVE_NOT_WRITTEN;
                 done.push_back(std::move(d));

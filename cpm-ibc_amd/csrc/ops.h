@@ -1,5 +1,6 @@
 // Host-side handles of libcpmgpu: context and operator.
 #pragma once
+#include <array>
 #include <map>
 #include <memory>
 #include <stdexcept>
@@ -36,6 +37,10 @@ struct Ctx {
     int64_t op_bytes = 0;
     Arena* arena = nullptr;   // set: workspace() carves from it and never allocates
     bool no_builds = false;   // set: solves build no shared-stencil tiles (they would allocate)
+    // FP32+refine history of tensor-product operators by family (the same m, n, L_off and L_ii:
+    // one equation on one grid spacing), which FP64 routing consults for an operator with no
+    // history of its own (a served request's operator is built for it and solved once)
+    std::map<std::array<double, 4>, Fp32Record> fp32_family;
     explicit Ctx(int dev);
     ~Ctx();
     double* workspace(const char* name, size_t bytes);

@@ -75,9 +75,11 @@ def test_no_fallback_where_refinement_works(gpu_ctx):
     assert np.linalg.norm(x32 - x64) <= 1e-8 * np.linalg.norm(x64)
 
 
-def test_fallback_in_server(gpu_ctx, fine_sphere):
-    """Two requests that stall in FP32 finish in the same serving round: each is re-solved in FP64."""
+def test_fallback_in_server(gpu_ctx, fine_sphere, monkeypatch):
+    """Two requests that stall in FP32 finish in the same serving round: each is re-solved in FP64
+    (FP32 first: the operator's record would route them to FP64 at submission)."""
     print("case fallback server sphere h=0.01", flush=True)
+    monkeypatch.setenv("CPMGPU_FP64_ROUTING", "0")
     _, op, rhs = fine_sphere
     D = op.D
     ref = [op.solve(b, cpmgpu.opts(precision=cpmgpu.CPM_FP64))[0] for b in rhs[:2]]
@@ -116,9 +118,11 @@ def test_fp64_routing_after_fallback(gpu_ctx, fine_sphere, monkeypatch):
     op = t.operator(1.0, -1.0)                       # a fresh operator: no fallback recorded
     o32 = cpmgpu.opts(precision=cpmgpu.CPM_FP32_REFINE)
     x64, i64 = op.solve(rhs[1], cpmgpu.opts(precision=cpmgpu.CPM_FP64))
+    monkeypatch.setenv("CPMGPU_FP64_ROUTING", "0")   # FP32 first (its family's record would route it)
     _, first = op.solve(rhs[1], o32)
     print("first", first, flush=True)
     assert first["iters"] > 0 and first["fallback_iters"] == i64["iters"]
+    monkeypatch.delenv("CPMGPU_FP64_ROUTING")
     x2, second = op.solve(rhs[1], o32)
     print("second", second, flush=True)
     assert second["status"] == "converged" and second["iters"] == 0 and second["fallback_iters"] == i64["iters"]
@@ -128,3 +132,43 @@ def test_fp64_routing_after_fallback(gpu_ctx, fine_sphere, monkeypatch):
     monkeypatch.setenv("CPMGPU_FP64_ROUTING", "0")
     _, third = op.solve(rhs[1], o32)
     assert third["iters"] > 0 and third["fallback_iters"] == i64["iters"]
+
+
+def test_fp64_routing_by_family(gpu_ctx, fine_sphere):
+    """A fresh tensor-product operator has no record of its own: its FP32+refine solves follow its
+    family's (the same m, n and L on the same grid spacing: the fine sphere's systems, which fell
+    back above) and start in FP64, in solves and in a server (solved at submission, no FP32
+    iterations, the plain FP64 result); an operator of another equation (m = 2) tries FP32 first."""
+    print("case fp64 routing by family sphere h=0.01", flush=True)
+    t, op, rhs = fine_sphere
+    o32 = cpmgpu.opts(precision=cpmgpu.CPM_FP32_REFINE)
+    o64 = cpmgpu.opts(precision=cpmgpu.CPM_FP64)
+    _, first = op.solve(rhs[0], o32)   # the family's record: this operator's fallbacks (or its routing)
+    assert first["fallback_iters"] > 0
+    x64, i64 = op.solve(rhs[2], o64)
+    fresh = t.operator(1.0, -1.0)
+    x, inf = fresh.solve(rhs[2], o32)
+    print("fresh", inf, flush=True)
+    assert inf["status"] == "converged" and inf["iters"] == 0 and inf["fallback_iters"] == i64["iters"]
+    assert np.array_equal(x, x64)
+    D = op.D
+    served = t.operator(1.0, -1.0)
+    srv = cpmgpu.Server(gpu_ctx, 80 * 2 * 2 * ((D + 63) // 64 + 1) * 64, lanes=2)
+    d_b, d_x = gpu_ctx.alloc(8 * D), gpu_ctx.alloc(8 * D)
+    try:
+        gpu_ctx.h2d(d_b, rhs[2])
+        assert srv.submit(served, [d_b], [d_x], 7)
+        done, active = srv.run(1)
+        print("served", done, active, flush=True)
+        assert active == 0 and len(done) == 1 and done[0][0] == 7
+        s = done[0][1][0]
+        assert s["status"] == "converged" and s["iters"] == 0 and s["fallback_iters"] == i64["iters"]
+        assert np.array_equal(gpu_ctx.d2h(np.zeros(D), d_x), x64)
+    finally:
+        srv.close()
+        gpu_ctx.free(d_b)
+        gpu_ctx.free(d_x)
+    other = t.operator(2.0, -1.0)
+    _, oinf = other.solve(rhs[2], o32)
+    print("other family", oinf, flush=True)
+    assert oinf["status"] == "converged" and oinf["iters"] > 0

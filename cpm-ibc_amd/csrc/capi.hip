@@ -321,13 +321,12 @@ struct Group {
 };
 
 template <typename T>
-static void fill_seg(Seg<T>& s, const Op* op, int64_t off, int block0, int nblocks, int nr, KState* st, double* part) {
+static void fill_seg(Seg<T>& s, const Op* op, int64_t off, int nblocks, int nr, KState* st, double* part) {
     s.op = op->view<T>();
     s.op64 = op->view<double>();
     s.m = op->m;
     s.n = op->n;
     s.off = off;
-    s.block0 = block0;
     s.nblocks = nblocks;
     s.nr = nr;
     s.precond = op->precond;
@@ -339,6 +338,32 @@ static void fill_seg(Seg<T>& s, const Op* op, int64_t off, int block0, int nbloc
     s.tinv = (T)(1.0f / (float)(op->m - op->n));
     s.st = st;
     s.part = part;
+}
+
+// Block maps of a launch over segments of nb[g] blocks each (entries: segment
+// | block within it << kSegBits), one after the other: segment by segment,
+// which the vector kernels run (a workgroup's rows stream from one region of
+// each vector), then round-robin (block b of every segment, then block b +
+// 1), which the E gathers run: segments that share an operator (the groups of
+// a streamed solve) gather through each tile together, its metadata read
+// into an L2 once for all of them rather than once per segment.
+static std::vector<int> block_map(const std::vector<int>& nb) {
+    if (nb.size() > (size_t)kMaxSegs) throw ArgError("too many segments in one launch");
+    int most = 0;
+    size_t total = 0;
+    for (int v : nb) {
+        most = std::max(most, v);
+        total += (size_t)v;
+    }
+    if (most > (1 << (31 - kSegBits))) throw ArgError("operator too large for one launch's block map");
+    std::vector<int> m;
+    m.reserve(2 * total);
+    for (size_t g = 0; g < nb.size(); ++g)   // segment by segment: the vector kernels
+        for (int b = 0; b < nb[g]; ++b) m.push_back((int)g | (b << kSegBits));
+    for (int b = 0; b < most; ++b)           // round-robin: the gathers
+        for (size_t g = 0; g < nb.size(); ++g)
+            if (b < nb[g]) m.push_back((int)g | (b << kSegBits));
+    return m;
 }
 
 // The launches of a run of iterations between two host checks (16 of a
@@ -396,7 +421,7 @@ struct BatchBytes {
 static BatchBytes batch_bytes(int G, int blocks, int64_t total, int NR, bool refine) {
     auto r256 = [](size_t n) { return (n + 255) & ~size_t(255); };
     BatchBytes b;
-    b.tables = r256(sizeof(Seg<double>) * G) + r256(sizeof(Seg<float>) * G) + r256(sizeof(int) * blocks) +
+    b.tables = r256(sizeof(Seg<double>) * G) + r256(sizeof(Seg<float>) * G) + r256(sizeof(int) * 2 * (size_t)blocks) +
                r256(sizeof(KState) * (size_t)G * kMaxNR) + r256(sizeof(double) * 2 * NR * (size_t)blocks) + 256;
     b.vec64 = sizeof(double) * (size_t)total * (refine ? 5 : 11) + 1024;
     b.vec32 = refine ? sizeof(float) * (size_t)total * 10 + 1024 : 0;
@@ -466,24 +491,23 @@ static void run_batch(Ctx* ctx, const std::vector<Group>& groups, const std::vec
     };
     auto* segs64 = reinterpret_cast<Seg<double>*>(carve(sizeof(Seg<double>) * G));
     auto* segs32 = reinterpret_cast<Seg<float>*>(carve(sizeof(Seg<float>) * G));
-    int* bseg = reinterpret_cast<int*>(carve(sizeof(int) * blocks));
+    int* bseg = reinterpret_cast<int*>(carve(sizeof(int) * 2 * (size_t)blocks));   // two block maps
     KState* sts = reinterpret_cast<KState*>(carve(sizeof(KState) * nstate));
     double* part = reinterpret_cast<double*>(carve(sizeof(double) * part_stride * blocks));
 
     std::vector<Seg<double>> h64(G);
     std::vector<Seg<float>> h32(G);
-    std::vector<int> hb(blocks);
     for (int g = 0; g < G; ++g) {
         const int nr = (int)groups[g].inst.size();
-        fill_seg(h64[g], groups[g].op, off[g], b0[g], nb[g], nr, sts + g * kMaxNR, part + part_stride * b0[g]);
-        fill_seg(h32[g], groups[g].op, off[g], b0[g], nb[g], nr, sts + g * kMaxNR, part + part_stride * b0[g]);
+        fill_seg(h64[g], groups[g].op, off[g], nb[g], nr, sts + g * kMaxNR, part + part_stride * b0[g]);
+        fill_seg(h32[g], groups[g].op, off[g], nb[g], nr, sts + g * kMaxNR, part + part_stride * b0[g]);
         h64[g].m = h32[g].m = groups[g].m;
         h64[g].n = h32[g].n = groups[g].n;
-        for (int b = 0; b < nb[g]; ++b) hb[b0[g] + b] = g;
     }
+    const std::vector<int> hb = block_map(nb);
     HIPCHK(hipMemcpyAsync(segs64, h64.data(), sizeof(Seg<double>) * G, hipMemcpyHostToDevice, st));
     if (refine) HIPCHK(hipMemcpyAsync(segs32, h32.data(), sizeof(Seg<float>) * G, hipMemcpyHostToDevice, st));
-    HIPCHK(hipMemcpyAsync(bseg, hb.data(), sizeof(int) * blocks, hipMemcpyHostToDevice, st));
+    HIPCHK(hipMemcpyAsync(bseg, hb.data(), sizeof(int) * hb.size(), hipMemcpyHostToDevice, st));
 
     // workspace vectors, each `total` entries: FP64 b x r r0 p v s t y z tmp;
     // mixed: FP64 b x r t tmp and FP32 x r r0 p v s t y z tmp.
@@ -636,14 +660,14 @@ static void run_stream(Ctx* ctx, Op* op, const std::vector<const double*>& rhs, 
     const int64_t total = seg_total * G;
     auto r256 = [](size_t k) { return (k + 255) & ~size_t(255); };
     const size_t part_stride = 2 * NR;
-    const size_t bytes = r256(sizeof(Seg<double>) * G) + r256(sizeof(Seg<float>) * G) + 2 * r256(sizeof(int) * blocks) +
+    const size_t bytes = r256(sizeof(Seg<double>) * G) + r256(sizeof(Seg<float>) * G) + 3 * r256(sizeof(int) * blocks) +
                          r256(sizeof(KState) * nstate) + r256(sizeof(double) * part_stride * blocks) +
                          r256(sizeof(PackEntry) * 4) + 256;
     char* tb = reinterpret_cast<char*>(ctx->workspace("stream_tables", bytes));
     auto carve = [&](size_t k) { char* p = tb; tb += r256(k); return p; };
     auto* seg64 = reinterpret_cast<Seg<double>*>(carve(sizeof(Seg<double>) * G));
     auto* seg32 = reinterpret_cast<Seg<float>*>(carve(sizeof(Seg<float>) * G));
-    int* bseg = reinterpret_cast<int*>(carve(sizeof(int) * blocks));    // block -> segment
+    int* bseg = reinterpret_cast<int*>(carve(sizeof(int) * 2 * (size_t)blocks));   // two block maps
     int* bzero = reinterpret_cast<int*>(carve(sizeof(int) * blocks));   // all zero: one pack entry
     KState* sts = reinterpret_cast<KState*>(carve(sizeof(KState) * nstate));
     double* part = reinterpret_cast<double*>(carve(sizeof(double) * part_stride * blocks));
@@ -651,16 +675,16 @@ static void run_stream(Ctx* ctx, Op* op, const std::vector<const double*>& rhs, 
     std::vector<int> nr0(G);
     std::vector<Seg<double>> h64(G);
     std::vector<Seg<float>> h32(G);
-    std::vector<int> hb(blocks), hz(blocks, 0);
+    std::vector<int> hz(blocks, 0);
     for (int g = 0; g < G; ++g) {
         nr0[g] = std::min(NR, n - g * NR);
-        fill_seg(h64[g], op, g * seg_total, g * nb, nb, nr0[g], sts + g * kMaxNR, part + part_stride * g * nb);
-        fill_seg(h32[g], op, g * seg_total, g * nb, nb, nr0[g], sts + g * kMaxNR, part + part_stride * g * nb);
-        for (int b = 0; b < nb; ++b) hb[g * nb + b] = g;
+        fill_seg(h64[g], op, g * seg_total, nb, nr0[g], sts + g * kMaxNR, part + part_stride * g * nb);
+        fill_seg(h32[g], op, g * seg_total, nb, nr0[g], sts + g * kMaxNR, part + part_stride * g * nb);
     }
+    const std::vector<int> hb = block_map(std::vector<int>((size_t)G, nb));
     HIPCHK(hipMemcpyAsync(seg64, h64.data(), sizeof(Seg<double>) * G, hipMemcpyHostToDevice, st));
     HIPCHK(hipMemcpyAsync(seg32, h32.data(), sizeof(Seg<float>) * G, hipMemcpyHostToDevice, st));
-    HIPCHK(hipMemcpyAsync(bseg, hb.data(), sizeof(int) * blocks, hipMemcpyHostToDevice, st));
+    HIPCHK(hipMemcpyAsync(bseg, hb.data(), sizeof(int) * hb.size(), hipMemcpyHostToDevice, st));
     HIPCHK(hipMemcpyAsync(bzero, hz.data(), sizeof(int) * blocks, hipMemcpyHostToDevice, st));
     Vecs<double> V64{};
     Vecs<float> V32{};
@@ -927,7 +951,7 @@ void multiply(Op* op, const double* dx, double* dy, double m, double n) {
     const int nblocks = (int)((op->D + kBlock - 1) / kBlock);
     char* tb = reinterpret_cast<char*>(ctx->workspace("mul_tables", 8192 + sizeof(int) * nblocks));
     Seg<double> h{};
-    fill_seg(h, op, 0, 0, nblocks, 1, nullptr, nullptr);
+    fill_seg(h, op, 0, nblocks, 1, nullptr, nullptr);
     h.m = m;
     h.n = n;
     std::vector<int> hb(nblocks, 0);
@@ -968,9 +992,9 @@ static void profile_egather_t(Op* op, int G, int reps, double* avg_ms, double* b
     std::vector<Seg<T>> h(G);
     std::vector<int> hb((size_t)G * nblocks);
     for (int g = 0; g < G; ++g) {   // G copies of the operator in one launch, as streamed solves of 16, 32 lanes
-        fill_seg(h[g], op, g * seg_total, g * nblocks, nblocks, NR, nullptr, nullptr);
-        for (int b = 0; b < nblocks; ++b) hb[(size_t)g * nblocks + b] = g;
+        fill_seg(h[g], op, g * seg_total, nblocks, NR, nullptr, nullptr);
     }
+    hb = block_map(std::vector<int>((size_t)G, nblocks));   // both maps (the gather runs the second)
     const size_t seg_bytes = (sizeof(Seg<T>) * G + 255) & ~size_t(255);
     char* tb = reinterpret_cast<char*>(ctx->workspace("prof_tables", seg_bytes + sizeof(int) * hb.size()));
     auto* dseg = reinterpret_cast<Seg<T>*>(tb);
@@ -1080,7 +1104,7 @@ static StepTables step_tables(Op* op, const char* name, const std::vector<double
     char* tb = reinterpret_cast<char*>(ctx->workspace(name, seg_bytes + pe_bytes + sizeof(int) * t.nblocks));
     std::vector<Seg<double>> h(coef.size());
     for (size_t k = 0; k < coef.size(); ++k) {
-        fill_seg(h[k], op, 0, 0, t.nblocks, 1, nullptr, nullptr);
+        fill_seg(h[k], op, 0, t.nblocks, 1, nullptr, nullptr);
         h[k].m = 1.0;
         h[k].n = coef[k];
     }
@@ -1365,7 +1389,7 @@ struct Server : ServerBase {
         const int zn = (int)((cap + kBlock - 1) / kBlock);   // no request has more blocks
         auto r256 = [](size_t k) { return (k + 255) & ~size_t(255); };
         const size_t sseg = r256(sizeof(Seg<float>) * std::max<size_t>(reqs.size(), 1));
-        const size_t smap = r256(sizeof(int) * std::max(blocks, 1));
+        const size_t smap = r256(sizeof(int) * 2 * std::max(blocks, 1));   // two block maps
         const size_t spart = r256(sizeof(double) * 2 * NR * std::max(blocks, 1));
         const size_t spack = r256(sizeof(PackEntry));
         const size_t szero = r256(sizeof(int) * zn);
@@ -1459,16 +1483,17 @@ struct Server : ServerBase {
             ensure_group_tiles_for(ops, 4, NR);
         }
         std::vector<Seg<float>> h(reqs.size());
-        std::vector<int> map;
+        std::vector<int> nbs;
         int b0 = 0;
         for (size_t g = 0; g < reqs.size(); ++g) {
             const Req& q = reqs[g];
             const int nb = (int)((q.op->D + kBlock - 1) / kBlock);
-            fill_seg(h[g], q.op, q.off, b0, nb, q.nr, states.template as<KState>() + (size_t)q.slot * kMaxNR,
+            fill_seg(h[g], q.op, q.off, nb, q.nr, states.template as<KState>() + (size_t)q.slot * kMaxNR,
                      d_part + (size_t)2 * NR * b0);
-            for (int b = 0; b < nb; ++b) map.push_back((int)g);
+            nbs.push_back(nb);
             b0 += nb;
         }
+        const std::vector<int> map = block_map(nbs);
         nblocks = b0;
         const hipStream_t st = ctx->stream;
         HIPCHK(hipMemcpyAsync(d_segs, h.data(), sizeof(Seg<float>) * h.size(), hipMemcpyHostToDevice, st));
@@ -1618,7 +1643,7 @@ struct Server64 : ServerBase {
         for (auto& q : reqs) blocks += (int)((q.op->D + kBlock - 1) / kBlock);
         auto r256 = [](size_t k) { return (k + 255) & ~size_t(255); };
         const size_t sseg = r256(sizeof(Seg<double>) * std::max<size_t>(reqs.size(), 1));
-        const size_t smap = r256(sizeof(int) * std::max(blocks, 1));
+        const size_t smap = r256(sizeof(int) * 2 * std::max(blocks, 1));   // two block maps
         const size_t spart = r256(sizeof(double) * 2 * NR * std::max(blocks, 1));
         const size_t need = sseg + smap + spart;
         if (need > tables_bytes) {
@@ -1661,7 +1686,7 @@ struct Server64 : ServerBase {
         // BiCGStab initialisation of this request's segment alone
         const int nb = (int)((op->D + kBlock - 1) / kBlock);
         Seg<double> h{};
-        fill_seg(h, op, off, 0, nb, nrhs, sts, i_part());
+        fill_seg(h, op, off, nb, nrhs, sts, i_part());
         HIPCHK(hipMemcpyAsync(i_seg(), &h, sizeof(h), hipMemcpyHostToDevice, st));
         const LaunchCfg<double> c1{i_seg(), nullptr, nb, 1, st, 0, op->ngt};
         Launch<double, NR>::init(c1, v64(0), v64(1), v64(2), v64(3), v64(4), v64(5));
@@ -1675,16 +1700,17 @@ struct Server64 : ServerBase {
     void rebuild() {
         ensure_tables();
         std::vector<Seg<double>> h(reqs.size());
-        std::vector<int> map;
+        std::vector<int> nbs;
         int b0 = 0;
         for (size_t g = 0; g < reqs.size(); ++g) {
             const Req& q = reqs[g];
             const int nb = (int)((q.op->D + kBlock - 1) / kBlock);
-            fill_seg(h[g], q.op, q.off, b0, nb, q.nr, states.template as<KState>() + (size_t)q.slot * kMaxNR,
+            fill_seg(h[g], q.op, q.off, nb, q.nr, states.template as<KState>() + (size_t)q.slot * kMaxNR,
                      d_part + (size_t)2 * NR * b0);
-            for (int b = 0; b < nb; ++b) map.push_back((int)g);
+            nbs.push_back(nb);
             b0 += nb;
         }
+        const std::vector<int> map = block_map(nbs);
         nblocks = b0;
         const hipStream_t st = ctx->stream;
         HIPCHK(hipMemcpyAsync(d_segs, h.data(), sizeof(Seg<double>) * h.size(), hipMemcpyHostToDevice, st));

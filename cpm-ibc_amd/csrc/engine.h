@@ -14,6 +14,10 @@
 namespace cpmgpu {
 
 constexpr int kBlock = 256;          // 4 wave64 per workgroup
+// Block map entries of a launch over several segments: the segment's index in
+// the low kSegBits bits, the block's index within its segment above them.
+constexpr int kSegBits = 11;
+constexpr int kMaxSegs = 1 << kSegBits;
 constexpr int kMaxNR = 8;            // right-hand sides per multi-RHS sweep
 
 // Row flags (one byte per row).
@@ -122,7 +126,7 @@ struct Seg {
     OpView<double> op64; // the same operator in FP64 (refinement steps of FP32 segments)
     double m, n;
     int64_t off;         // element offset of row 0 in every workspace vector (rows interleave NR rhs)
-    int block0, nblocks;
+    int nblocks;
     int nr;              // rhs in this group (<= template NR; extra lanes idle)
     int precond;         // CPM_PRECOND_*
     int precond_iters;   // Jacobi sweeps

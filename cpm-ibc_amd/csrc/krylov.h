@@ -7,7 +7,8 @@ namespace cpmgpu {
 template <typename T>
 struct LaunchCfg {
     const Seg<T>* segs;    // device segment table
-    const int* bseg;       // device block -> segment map
+    const int* bseg;       // device block maps (segment | block << kSegBits), nblocks entries segment by
+                           // segment, then nblocks round-robin over the segments (the E gathers')
     int nblocks;
     int nseg;
     hipStream_t stream;

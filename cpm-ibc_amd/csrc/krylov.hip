@@ -175,10 +175,15 @@ __device__ __forceinline__ const Seg<T>& seg_of(const Seg<T>* __restrict__ segs,
         const int xcd = b % 8, k = b / 8;
         vb = ((k / kXcdRun) * 8 + xcd) * kXcdRun + k % kXcdRun;
     }
-    const Seg<T>& sg = segs[bseg ? bseg[vb] : 0];   // one segment: no map (a dependent load less)
-    lb = vb - sg.block0;
+    if (!bseg) {   // one segment: no map (a dependent load less)
+        lb = vb;
+        row = (int64_t)lb * kBlock + threadIdx.x;
+        return segs[0];
+    }
+    const int e = bseg[vb];
+    lb = e >> kSegBits;
     row = (int64_t)lb * kBlock + threadIdx.x;
-    return sg;
+    return segs[e & (kMaxSegs - 1)];
 }
 
 // Lane masks: bit s set = act on lanes whose status is s.
@@ -1439,18 +1444,22 @@ void launch_reset_states(KState* st, int n, double tol2, int max_it, double delt
 // ---------------------------------------------------------------- launchers
 #define LAUNCH(kern, ...) \
     hipLaunchKernelGGL(kern, dim3(c.nblocks), dim3(kBlock), 0, c.stream, c.segs, c.nseg == 1 ? nullptr : c.bseg, __VA_ARGS__)
+// The E gathers run the round-robin block map.
+#define LAUNCH_GATHER(kern, ...)                                                                         \
+    hipLaunchKernelGGL(kern, dim3(c.nblocks), dim3(kBlock), 0, c.stream, c.segs,                        \
+                       c.nseg == 1 ? nullptr : c.bseg + c.nblocks, __VA_ARGS__)
 // E gather of launch c; FL: form P^-1 of the gathered vector (only with
 // shared-stencil tiles, grp_launch).
 #define EGATHER(FL, ...)                                                     \
     do {                                                                     \
         if constexpr (GrpTile<T, NR>::kRows > 0) {                           \
             if (c.grp > 0 && grp_gather(sizeof(T), NR, c.grp)) {             \
-                if (FL) LAUNCH((k_egather_grp<T, NR, GrpWaves<T, NR>::kWaves, true>), __VA_ARGS__); \
-                else LAUNCH((k_egather_grp<T, NR, GrpWaves<T, NR>::kWaves, false>), __VA_ARGS__); \
+                if (FL) LAUNCH_GATHER((k_egather_grp<T, NR, GrpWaves<T, NR>::kWaves, true>), __VA_ARGS__); \
+                else LAUNCH_GATHER((k_egather_grp<T, NR, GrpWaves<T, NR>::kWaves, false>), __VA_ARGS__); \
                 break;                                                       \
             }                                                                \
         }                                                                    \
-        LAUNCH((k_egather<T, NR>), __VA_ARGS__);                             \
+        LAUNCH_GATHER((k_egather<T, NR>), __VA_ARGS__);                      \
     } while (0)
 #define FIN(T, NR, KIND) hipLaunchKernelGGL((k_fin<T, NR, KIND>), dim3(c.nseg * NR), dim3(kBlock), 0, c.stream, c.segs, c.nseg)
 
@@ -1523,7 +1532,7 @@ void Launch<T, NR>::iteration(const LaunchCfg<T>& c, Vecs<T>& w, bool mixed, con
     if constexpr (std::is_same<T, float>::value) {
         if (mixed) {   // FP32 lanes in the gather, refinement lanes' FP64 gathers in a launch of their own
             EGATHER(fly, yp, w.tmp, M_RUN, mv);
-            LAUNCH((k_egather_upd<NR>), mv);
+            LAUNCH_GATHER((k_egather_upd<NR>), mv);
         } else {
             EGATHER(fly, yp, w.tmp, M_STAGE1, mv);
         }

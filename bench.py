@@ -640,10 +640,16 @@ def main():
     eg_ms, eg_bytes = op.profile_egather(cpmgpu.CPM_FP32_REFINE, nr, reps=50, groups=groups)
     achieved = eg_bytes / (eg_ms * 1e-3) / 1e9
     kname = op.gather_kernel(cpmgpu.CPM_FP32_REFINE, nr)
+    # bytes_per_launch counts the operator's data once per group (each group is one operator copy with
+    # its nr right-hand sides); the groups of a streamed solve share one operator, so the launch's
+    # distinct bytes are the operator's data once plus every group's x read and tmp written
+    per_group_vec = 2.0 * op.D * nr * 4
+    unique = eg_bytes / groups + (groups - 1) * per_group_vec
     roof = dict(bound="hbm", achieved=round(achieved, 1), peak=HBM_PEAK_GBS, unit="GB/s",
                 frac=round(achieved / HBM_PEAK_GBS, 4), traffic=egather_traffic(kname, args.workload, groups),
                 kernel=kname, groups_per_launch=groups, avg_launch_ms=round(eg_ms, 5),
-                bytes_per_launch=int(eg_bytes))
+                bytes_per_launch=int(eg_bytes), distinct_bytes_per_launch=int(unique),
+                distinct_achieved=round(unique / (eg_ms * 1e-3) / 1e9, 1))
     copy_bytes = 1 << 29
     copy_ms = ctx.profile_copy(copy_bytes, 20)
 

@@ -1532,7 +1532,7 @@ void Launch<T, NR>::iteration(const LaunchCfg<T>& c, Vecs<T>& w, bool mixed, con
     if constexpr (std::is_same<T, float>::value) {
         if (mixed) {   // FP32 lanes in the gather, refinement lanes' FP64 gathers in a launch of their own
             EGATHER(fly, yp, w.tmp, M_RUN, mv);
-            LAUNCH_GATHER((k_egather_upd<NR>), mv);
+            LAUNCH((k_egather_upd<NR>), mv);   // few groups refine at once: their blocks together
         } else {
             EGATHER(fly, yp, w.tmp, M_STAGE1, mv);
         }

@@ -20,9 +20,17 @@
 //         64      2560         3     1280        32.9        324.5
 //         64      2560         2     1024        37.7        355.0
 //         64      2560         2     1536        30.8        318.5
+//
+// Measured again once the E gathers took a launch's segments round-robin
+// (bench.py's default run: 128 right-hand sides streamed through 64 lanes,
+// FP64 batches and one-right-hand-side solves), xcd_run 64 -> 16: streamed
+// 161.3 -> 162.5 solves/s, FP64 batches of 64 96.1 -> 96.5, p50 of one
+// right-hand side 28.7 -> 28.6 ms, one-RHS FP64 30.5 -> 30.3 solves/s;
+// xcd_run 16 chosen (runs of 1/8/32 gave 791.0/795.3/788.7 ms per streamed
+// step against 786.8 for 16).
 #pragma once
 #ifndef CPMGPU_XCD_RUN
-#define CPMGPU_XCD_RUN 64
+#define CPMGPU_XCD_RUN 16
 #endif
 #ifndef CPMGPU_LDS_ROWS
 #define CPMGPU_LDS_ROWS 2560

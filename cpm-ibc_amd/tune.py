@@ -27,10 +27,10 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 REPO = os.path.dirname(HERE)
 TUNE = os.path.join(HERE, "_tune")
 
-DEFAULT = dict(xcd_run=64, lds_rows=2560, grp_rows=2, grp_lds=1280)
+DEFAULT = dict(xcd_run=16, lds_rows=2560, grp_rows=2, grp_lds=1280)
 TIE = 0.005
 VARIANTS = [DEFAULT] + [dict(DEFAULT, **{k: v}) for k, vs in (
-    ("xcd_run", (32, 128)), ("lds_rows", (3072,)), ("grp_rows", (1, 3)), ("grp_lds", (1024, 1536))) for v in vs]
+    ("xcd_run", (8, 32, 64)), ("lds_rows", (3072,)), ("grp_rows", (1, 3)), ("grp_lds", (1024, 1536))) for v in vs]
 DEFINES = dict(xcd_run="CPMGPU_XCD_RUN", lds_rows="CPMGPU_LDS_ROWS", grp_rows="CPMGPU_GROUP_ROWS",
                grp_lds="CPMGPU_GROUP_LDS_ROWS")
 

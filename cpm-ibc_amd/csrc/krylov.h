@@ -19,6 +19,9 @@ struct LaunchCfg {
 // Whether E gathers of T x nr values use shared-stencil tiles when every
 // operator of the launch has at least ntiles of them.
 bool grp_gather(int tsize, int nr, int ntiles);
+// Whether a launch of T x nr values builds its operators' shared-stencil tiles
+// (other shapes use them only where an earlier launch built them).
+bool grp_build(int tsize, int nr);
 // Re-read the CPMGPU_GROUP_GATHER override (once per launch sequence).
 void grp_refresh_override();
 

@@ -1494,12 +1494,26 @@ void grp_refresh_override() {
 }
 
 bool grp_gather(int tsize, int nr, int ntiles) {
+    (void)tsize;
     if (g_grp_override.load(std::memory_order_relaxed) == -2) grp_refresh_override();
     const int ov = g_grp_override.load(std::memory_order_relaxed);
     if (ov >= 0) return ov == 1;
-    if (!((tsize == 4 && nr != 4) || (tsize == 8 && nr == 4))) return false;
-    // one right-hand side: row tiles were faster up to ~600 tiles
+    // every shape whose rows fit the tiles' LDS is faster through them where the
+    // operator has them (torus-knot system, one / eight groups per launch: FP64 x
+    // 1 20.0 / 87.0 -> 16.9 / 69.8 us, FP64 x 2 23.6 / 99.8 -> 19.6 / 90.9 us,
+    // FP32 x 4 21.3 / 92.1 -> 17.7 / 72.2 us); one right-hand side: row tiles
+    // were faster up to ~600 tiles
     return ntiles >= (nr == 1 ? 768 : kGrpMinTiles);
+}
+
+bool grp_build(int tsize, int nr) {
+    if (g_grp_override.load(std::memory_order_relaxed) == -2) grp_refresh_override();
+    const int ov = g_grp_override.load(std::memory_order_relaxed);
+    if (ov >= 0) return ov == 1;
+    // the shapes of the solves that repeat on one operator: FP32 x 1, 2, 8 (FP32+refine
+    // batches and streams) and FP64 x 4 (FP64 batches); an operator served at FP32 x 4
+    // is solved once and would not win back the build (0.14 s at 1.27 M rows)
+    return (tsize == 4 && nr != 4) || (tsize == 8 && nr == 4);
 }
 
 // Whether the E gathers of launch c use shared-stencil tiles (EGATHER's choice).

@@ -715,7 +715,7 @@ void op_ensure_group_tiles(Op& op) {
 
 void ensure_group_tiles_for(const std::vector<Op*>& ops, int tsize, int nr) {
     grp_refresh_override();
-    if (!grp_gather(tsize, nr, INT_MAX)) return;
+    if (!grp_build(tsize, nr)) return;
     for (Op* o : ops) op_ensure_group_tiles(*o);
 }
 
